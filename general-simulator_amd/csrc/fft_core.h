@@ -1,0 +1,189 @@
+// LDS-staged Stockham FFT engine for CDNA4 (gfx950).
+//
+// A tile of independent lines lives in LDS. Line l starts at
+//   base(l) = (l / lw) * ls1 + (l % lw) * ls0
+// and its element p sits at base(l) + p * ps (complex units). One radix-R
+// pass maps butterfly b (line-fastest: l = b % nl, j = b / nl) to
+//   read  in[j + r*N/R]            r = 0..R-1
+//   twiddle by w^(r*k*N/(Ns*R)),   k = j mod Ns, w = exp(-+2 pi i / N)
+//   R-point DFT in registers
+//   write out[(j/Ns)*Ns*R + k + r*Ns]
+// which is the self-sorting Stockham formulation: after the last pass the
+// line is in natural order. Each pass reads all operands into registers,
+// synchronises, then writes back into the same LDS buffer.
+// The block must provide at least nl*N/Rmin threads.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gsk {
+
+template <typename R> struct V2;
+template <> struct V2<float> { using T = float2; };
+template <> struct V2<double> { using T = double2; };
+
+template <typename T2> __device__ __forceinline__ T2 mk(decltype(T2::x) a, decltype(T2::x) b) {
+  T2 r; r.x = a; r.y = b; return r;
+}
+template <typename T2> __device__ __forceinline__ T2 cadd(T2 a, T2 b) { return mk<T2>(a.x + b.x, a.y + b.y); }
+template <typename T2> __device__ __forceinline__ T2 csub(T2 a, T2 b) { return mk<T2>(a.x - b.x, a.y - b.y); }
+template <typename T2> __device__ __forceinline__ T2 cmul(T2 a, T2 b) {
+  return mk<T2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T2> __device__ __forceinline__ T2 cconj(T2 a) { return mk<T2>(a.x, -a.y); }
+template <typename T2, typename S> __device__ __forceinline__ T2 cscale(T2 a, S s) { return mk<T2>(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <bool FWD, typename T2> __device__ __forceinline__ T2 rot(T2 a) {
+  return FWD ? mk<T2>(a.y, -a.x) : mk<T2>(-a.y, a.x);
+}
+
+// ----------------------------------------------------------------- small DFTs
+template <bool FWD, typename T2> __device__ __forceinline__ void dft2(T2* v) {
+  T2 a = v[0], b = v[1];
+  v[0] = cadd(a, b); v[1] = csub(a, b);
+}
+
+template <bool FWD, typename T2> __device__ __forceinline__ void dft3(T2* v) {
+  using S = decltype(T2::x);
+  const S h = S(0.86602540378443864676);  // sqrt(3)/2
+  T2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+  T2 m = mk<T2>(v[0].x - S(0.5) * s.x, v[0].y - S(0.5) * s.y);
+  T2 t = rot<FWD>(d);  // -+i (a1 - a2)
+  v[0] = cadd(v[0], s);
+  v[1] = mk<T2>(m.x + h * t.x, m.y + h * t.y);
+  v[2] = mk<T2>(m.x - h * t.x, m.y - h * t.y);
+}
+
+template <bool FWD, typename T2> __device__ __forceinline__ void dft4v(T2& a0, T2& a1, T2& a2, T2& a3) {
+  T2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = rot<FWD>(csub(a1, a3));
+  a0 = cadd(t0, t2); a2 = csub(t0, t2); a1 = cadd(t1, t3); a3 = csub(t1, t3);
+}
+template <bool FWD, typename T2> __device__ __forceinline__ void dft4(T2* v) { dft4v<FWD>(v[0], v[1], v[2], v[3]); }
+
+template <bool FWD, typename T2> __device__ __forceinline__ void dft8(T2* v) {
+  using S = decltype(T2::x);
+  const S c = S(0.70710678118654752440);
+  T2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+  T2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+  dft4v<FWD>(e0, e1, e2, e3);
+  dft4v<FWD>(o0, o1, o2, o3);
+  // w^1 = (c, -+c), w^2 = -+i, w^3 = (-c, -+c)
+  T2 w1o1 = FWD ? mk<T2>(c * (o1.x + o1.y), c * (o1.y - o1.x)) : mk<T2>(c * (o1.x - o1.y), c * (o1.y + o1.x));
+  T2 w2o2 = rot<FWD>(o2);
+  T2 w3o3 = FWD ? mk<T2>(c * (-o3.x + o3.y), c * (-o3.y - o3.x)) : mk<T2>(c * (-o3.x - o3.y), c * (-o3.y + o3.x));
+  v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+  v[1] = cadd(e1, w1o1); v[5] = csub(e1, w1o1);
+  v[2] = cadd(e2, w2o2); v[6] = csub(e2, w2o2);
+  v[3] = cadd(e3, w3o3); v[7] = csub(e3, w3o3);
+}
+
+template <bool FWD, typename T2> __device__ __forceinline__ void dft16(T2* v) {
+  using S = decltype(T2::x);
+  T2 e[8], o[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+  dft8<FWD>(e);
+  dft8<FWD>(o);
+  // cos(pi k/8), sin(pi k/8) for k = 0..7
+  const S cs[8] = {S(1.0), S(0.92387953251128675613), S(0.70710678118654752440), S(0.38268343236508977173),
+                   S(0.0), S(-0.38268343236508977173), S(-0.70710678118654752440), S(-0.92387953251128675613)};
+  const S sn[8] = {S(0.0), S(0.38268343236508977173), S(0.70710678118654752440), S(0.92387953251128675613),
+                   S(1.0), S(0.92387953251128675613), S(0.70710678118654752440), S(0.38268343236508977173)};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    T2 w = mk<T2>(cs[k], FWD ? -sn[k] : sn[k]);
+    T2 t = (k == 0) ? o[k] : cmul(o[k], w);
+    v[k] = cadd(e[k], t);
+    v[k + 8] = csub(e[k], t);
+  }
+}
+
+template <int R, bool FWD, typename T2> __device__ __forceinline__ void dftR(T2* v) {
+  if constexpr (R == 2) dft2<FWD>(v);
+  else if constexpr (R == 3) dft3<FWD>(v);
+  else if constexpr (R == 4) dft4<FWD>(v);
+  else if constexpr (R == 8) dft8<FWD>(v);
+  else if constexpr (R == 16) dft16<FWD>(v);
+  else static_assert(R == 2, "unsupported radix");
+}
+
+// ----------------------------------------------------------------- plans
+template <int... Rs> struct Radices {};
+template <int N> struct PlanOf;
+template <> struct PlanOf<2> { using type = Radices<2>; static constexpr int rmin = 2; };
+template <> struct PlanOf<4> { using type = Radices<4>; static constexpr int rmin = 4; };
+template <> struct PlanOf<8> { using type = Radices<8>; static constexpr int rmin = 8; };
+template <> struct PlanOf<16> { using type = Radices<16>; static constexpr int rmin = 16; };
+template <> struct PlanOf<32> { using type = Radices<8, 4>; static constexpr int rmin = 4; };
+template <> struct PlanOf<48> { using type = Radices<3, 16>; static constexpr int rmin = 3; };
+template <> struct PlanOf<64> { using type = Radices<8, 8>; static constexpr int rmin = 8; };
+template <> struct PlanOf<96> { using type = Radices<3, 4, 8>; static constexpr int rmin = 3; };
+template <> struct PlanOf<128> { using type = Radices<16, 8>; static constexpr int rmin = 8; };
+template <> struct PlanOf<192> { using type = Radices<3, 8, 8>; static constexpr int rmin = 3; };
+template <> struct PlanOf<256> { using type = Radices<16, 16>; static constexpr int rmin = 16; };
+template <> struct PlanOf<512> { using type = Radices<8, 8, 8>; static constexpr int rmin = 8; };
+
+struct Lines {
+  int nl;   // number of lines
+  int lw;   // lines per group
+  int ls0;  // stride between lines inside a group
+  int ls1;  // stride between groups
+  int ps;   // stride between elements of a line
+  __device__ __forceinline__ int base(int l) const { return (l / lw) * ls1 + (l % lw) * ls0; }
+};
+
+// tw[m] = exp(-2 pi i m / N) (forward convention); the inverse conjugates.
+template <typename T2, int N, int R, bool FWD>
+__device__ __forceinline__ void fft_pass(T2* lds, const Lines& L, int Ns, const T2* tw) {
+  const int nb = L.nl * (N / R);
+  const int b = threadIdx.x;
+  const bool act = b < nb;
+  T2 v[R];
+  int base = 0, j = 0;
+  if (act) {
+    const int l = b % L.nl;
+    j = b / L.nl;
+    base = L.base(l);
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = lds[base + (j + r * (N / R)) * L.ps];
+  }
+  __syncthreads();
+  if (act) {
+    const int k = j % Ns;
+    if (Ns > 1) {
+      const int st = k * (N / (Ns * R));
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        T2 w = tw[r * st];
+        if (!FWD) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+      }
+    }
+    dftR<R, FWD>(v);
+    const int d = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[base + (d + r * Ns) * L.ps] = v[r];
+  }
+  __syncthreads();
+}
+
+template <typename T2, int N, bool FWD, int R0, int... Rs>
+__device__ __forceinline__ void fft_chain(T2* lds, const Lines& L, const T2* tw, int Ns) {
+  fft_pass<T2, N, R0, FWD>(lds, L, Ns, tw);
+  if constexpr (sizeof...(Rs) > 0) fft_chain<T2, N, FWD, Rs...>(lds, L, tw, Ns * R0);
+}
+
+template <typename T2, int N, bool FWD, int... Rs>
+__device__ __forceinline__ void fft_dispatch(T2* lds, const Lines& L, const T2* tw, Radices<Rs...>) {
+  fft_chain<T2, N, FWD, Rs...>(lds, L, tw, 1);
+}
+
+// Transform all lines described by L; every thread of the block must call it.
+template <typename T2, int N, bool FWD>
+__device__ __forceinline__ void fft_lines(T2* lds, const Lines& L, const T2* tw) {
+  fft_dispatch<T2, N, FWD>(lds, L, tw, typename PlanOf<N>::type{});
+}
+
+// Threads needed to transform nl lines of length N in one call.
+template <int N> constexpr int fft_threads(int nl) { return nl * (N / PlanOf<N>::rmin); }
+
+}  // namespace gsk

@@ -1,0 +1,1085 @@
+// C ABI of the MI355X phase-field step library. See include/gsk.h.
+#include "gsk.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels_aux.h"
+
+using namespace gsk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                                  \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess)                                                                             \
+      return fail(GSK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + \
+                                   ":" + std::to_string(__LINE__) + ")");                             \
+  } while (0)
+
+#define CHK(expr)                  \
+  do {                             \
+    int rc_ = (expr);              \
+    if (rc_ != GSK_OK) return rc_; \
+  } while (0)
+
+constexpr double kPi = 3.14159265358979323846;
+
+bool supported_n(int n) {
+  switch (n) {
+    case 16: case 32: case 64: case 96: case 128: case 192: case 256: case 512: return true;
+    default: return false;
+  }
+}
+
+int rmin_of(int N) {
+  switch (N) {
+    case 8: return PlanOf<8>::rmin;
+    case 16: return PlanOf<16>::rmin;
+    case 32: return PlanOf<32>::rmin;
+    case 48: return PlanOf<48>::rmin;
+    case 64: return PlanOf<64>::rmin;
+    case 96: return PlanOf<96>::rmin;
+    case 128: return PlanOf<128>::rmin;
+    case 192: return PlanOf<192>::rmin;
+    case 256: return PlanOf<256>::rmin;
+    case 512: return PlanOf<512>::rmin;
+  }
+  return 1;
+}
+
+int round64(int t) { return ((t + 63) / 64) * 64; }
+
+template <typename T2>
+std::vector<T2> twiddles(int N, int denom) {
+  std::vector<T2> t(N);
+  for (int m = 0; m < N; ++m) {
+    double a = -2.0 * kPi * m / denom;
+    t[m].x = (decltype(t[m].x))std::cos(a);
+    t[m].y = (decltype(t[m].x))std::sin(a);
+  }
+  return t;
+}
+
+struct Slot {
+  const char* name;
+  double bytes;  // algorithmic HBM bytes per step
+};
+
+}  // namespace
+
+struct gsk_sim {
+  gsk_config cfg{};
+  int n = 0, V = 0, F = 0, dev = 0;
+  Geo g{};
+  int zthreads = 0, xthreads = 0;
+  size_t lds_z = 0, lds_y = 0, lds_xc = 0;
+  hipStream_t stream = nullptr;
+  // twiddles
+  float2 *twN = nullptr, *twM = nullptr, *rtn = nullptr;
+  double2* twN64 = nullptr;
+  // fused FP32 buffers
+  float* fld = nullptr;
+  float2* kbuf = nullptr;
+  float2* state = nullptr;
+  float* Bt = nullptr;
+  // plain FP64 buffers
+  double* eta64 = nullptr;
+  double2 *S64 = nullptr, *R64 = nullptr, *E64 = nullptr, *F64 = nullptr;
+  double* B64 = nullptr;
+  int pL = 0, pW = 0, pzthreads = 0, pxthreads = 0;
+  size_t lds_pz = 0, lds_px = 0;
+  // reductions
+  double* part = nullptr;
+  double* rout = nullptr;
+  int rblocks = 512;
+  // state
+  double cbar = 0.0;
+  double cee0 = 0.0;  // B(k = 0) of the c-sourced tensor
+  bool haveB = false, dirty = true;
+  long long step = 0;
+  // profiling
+  bool prof = false;
+  std::vector<hipEvent_t> ev;
+  int evi = 0;
+  std::vector<Slot> slots;
+};
+
+namespace {
+
+size_t nvox(const gsk_sim* s) { return (size_t)s->n * s->n * s->n; }
+size_t hs_count(const gsk_sim* s) { return (size_t)s->n * s->n * (s->n / 2 + 1); }
+
+int mark(gsk_sim* s) {
+  if (s->prof) {
+    if (s->evi >= (int)s->ev.size()) return fail(GSK_ERR, "profiling event overflow");
+    HIPCHK(hipEventRecord(s->ev[s->evi++], s->stream));
+  }
+  return GSK_OK;
+}
+
+#define LAUNCH(kern, grid, threads, lds, s, ...)                                              \
+  do {                                                                                      \
+    CHK(mark(s));                                                                           \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, (s)->stream, __VA_ARGS__);     \
+    HIPCHK(hipGetLastError());                                                              \
+  } while (0)
+
+MartK mart_consts(const gsk_sim* s) {
+  const gsk_config& c = s->cfg;
+  MartK k{};
+  k.A = (float)c.A; k.B = (float)c.B; k.C = (float)c.C;
+  k.two_wel = (float)(2.0 * c.w_el);
+  k.noise = (float)c.noise;
+  k.dtL = (float)(c.dt * c.L);
+  k.dtLk = (float)(c.dt * c.L * c.kappa);
+  k.kfac = (float)(c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx));
+  k.inv_n3 = (float)(1.0 / ((double)s->n * s->n * s->n));
+  k.F = s->V;
+  k.npq = s->V * (s->V + 1) / 2;
+  k.step = (uint32_t)s->step;
+  k.seed = c.seed;
+  return k;
+}
+
+PrecK prec_consts(const gsk_sim* s) {
+  const gsk_config& c = s->cfg;
+  PrecK k{};
+  k.A1 = (float)c.a[1]; k.A2 = (float)c.a[2]; k.A3 = (float)c.a[3]; k.A4 = (float)c.a[4];
+  k.A5 = (float)c.a[5]; k.A6 = (float)c.a[6]; k.A7 = (float)c.a[7];
+  k.c1 = (float)c.c1; k.c2 = (float)c.c2;
+  k.cbar = (float)s->cbar;
+  k.noise = (float)c.noise;
+  k.dtM = (float)(c.dt * c.M);
+  k.dtMkc = (float)(c.dt * c.M * c.kappa_c);
+  k.dtL = (float)(c.dt * c.L);
+  k.dtLke = (float)(c.dt * c.L * c.kappa);
+  k.xi = (float)c.xi;
+  k.S = (float)c.S;
+  k.kfac = (float)(c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx));
+  k.inv_n3 = (float)(1.0 / ((double)s->n * s->n * s->n));
+  k.V = s->V;
+  k.step = (uint32_t)s->step;
+  k.seed = c.seed;
+  return k;
+}
+
+PlainK plain_consts(const gsk_sim* s) {
+  const gsk_config& c = s->cfg;
+  PlainK k{};
+  k.A = c.A; k.B = c.B; k.C = c.C;
+  k.two_wel = 2.0 * c.w_el;
+  k.noise = c.noise;
+  k.dtL = c.dt * c.L;
+  k.dtLk = c.dt * c.L * c.kappa;
+  k.kfac = (c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx));
+  k.inv_n3 = 1.0 / ((double)s->n * s->n * s->n);
+  k.V = s->V;
+  k.step = (uint32_t)s->step;
+  k.seed = c.seed;
+  return k;
+}
+
+int grid_1d(long work, int threads = 256) {
+  long b = (work + threads - 1) / threads;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+// ------------------------------------------------------------------ per-size implementation
+template <int N>
+struct Impl {
+  static constexpr int M = N / 2;
+
+  static int set_attrs() {
+    const int lim = 160 * 1024;
+    HIPCHK(hipFuncSetAttribute((const void*)k_zfwd<M, Z_COPY>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_zfwd<M, Z_SQUARE>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_zfwd<M, Z_PREC_LOCAL>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_zmid_mart<M>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_zinv<M>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_ypass<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_ypass<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_xpass<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_xpass<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_xcontract<N>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_xupdate_mart<N, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_xupdate_mart<N, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_xupdate_prec<N, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)k_xupdate_prec<N, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)r_xenergy_mart<N>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)p_zrows<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)p_zrows<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)p_strided<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    HIPCHK(hipFuncSetAttribute((const void*)p_strided<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    return GSK_OK;
+  }
+
+  // ---- fused building blocks
+  static int zfwd(gsk_sim* s, int op, const float* in, int F) {
+    const int grid = s->n * s->n / s->g.L;
+    MartK mk = mart_consts(s);
+    PrecK pk = prec_consts(s);
+    const size_t lds = (2 * M + (size_t)F * s->g.L * (M + 1)) * sizeof(float2);
+    if (op == Z_COPY)
+      LAUNCH((k_zfwd<M, Z_COPY>), grid, s->zthreads, lds, s, in, s->kbuf, s->g, F, mk, pk, s->twM, s->rtn);
+    else if (op == Z_SQUARE)
+      LAUNCH((k_zfwd<M, Z_SQUARE>), grid, s->zthreads, lds, s, in, s->kbuf, s->g, F, mk, pk, s->twM, s->rtn);
+    else
+      LAUNCH((k_zfwd<M, Z_PREC_LOCAL>), grid, s->zthreads, lds, s, in, s->kbuf, s->g, F, mk, pk, s->twM, s->rtn);
+    return GSK_OK;
+  }
+  static int ypass(gsk_sim* s, bool fwd, int F) {
+    const int grid = s->n * s->g.nch;
+    if (fwd) LAUNCH((k_ypass<N, true>), grid, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, s->twN);
+    else LAUNCH((k_ypass<N, false>), grid, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, s->twN);
+    return GSK_OK;
+  }
+  static int zinv(gsk_sim* s, float* out, int F) {
+    const int grid = s->n * s->n / s->g.L;
+    const size_t lds = (2 * M + (size_t)F * s->g.L * (M + 1)) * sizeof(float2);
+    LAUNCH((k_zinv<M>), grid, s->zthreads, lds, s, s->kbuf, out, s->g, F, s->twM, s->rtn);
+    return GSK_OK;
+  }
+
+  static int init_state(gsk_sim* s) {
+    const int gx = s->n * s->g.nch;
+    if (s->cfg.physics == GSK_PHYS_MART) {
+      CHK(zfwd(s, Z_COPY, s->fld, s->F));
+      CHK(ypass(s, true, s->F));
+      MartK mk = mart_consts(s);
+      LAUNCH((k_xupdate_mart<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN);
+    } else {
+      CHK(zfwd(s, Z_COPY, s->fld, s->F));
+      CHK(ypass(s, true, s->F));
+      PrecK pk = prec_consts(s);
+      LAUNCH((k_xupdate_prec<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->Bt, s->g, pk, s->twN);
+    }
+    return GSK_OK;
+  }
+
+  static int step_mart_fused(gsk_sim* s) {
+    const int gx = s->n * s->g.nch;
+    const int F = s->F;
+    MartK mk = mart_consts(s);
+    CHK(zfwd(s, Z_SQUARE, s->fld, F));
+    CHK(ypass(s, true, F));
+    LAUNCH((k_xcontract<N>), gx, s->xthreads, s->lds_xc, s, s->kbuf, s->Bt, s->g, mk, s->twN);
+    CHK(ypass(s, false, F));
+    {
+      const int grid = s->n * s->n / s->g.L;
+      const size_t lds = (2 * M + (size_t)F * s->g.L * (M + 1)) * sizeof(float2);
+      LAUNCH((k_zmid_mart<M>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn);
+    }
+    CHK(ypass(s, true, F));
+    LAUNCH((k_xupdate_mart<N, 0>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN);
+    CHK(ypass(s, false, F));
+    CHK(zinv(s, s->fld, F));
+    return GSK_OK;
+  }
+
+  static int step_prec_fused(gsk_sim* s) {
+    const int gx = s->n * s->g.nch;
+    const int F = s->F;
+    PrecK pk = prec_consts(s);
+    CHK(zfwd(s, Z_PREC_LOCAL, s->fld, F));
+    CHK(ypass(s, true, F));
+    LAUNCH((k_xupdate_prec<N, 0>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->Bt, s->g, pk, s->twN);
+    CHK(ypass(s, false, F));
+    CHK(zinv(s, s->fld, F));
+    return GSK_OK;
+  }
+
+  // ---- plain FP64 3D C2C on V fields of buf
+  static int c2c(gsk_sim* s, double2* buf, bool fwd) {
+    const long n = s->n;
+    const long N3 = n * n * n;
+    for (int f = 0; f < s->V; ++f) {
+      double2* b = buf + f * N3;
+      const int gz = (int)(n * n / s->pL);
+      if (fwd) LAUNCH((p_zrows<N, true>), gz, s->pzthreads, s->lds_pz, s, b, s->pL, s->twN64);
+      else LAUNCH((p_zrows<N, false>), gz, s->pzthreads, s->lds_pz, s, b, s->pL, s->twN64);
+      const int gs = (int)(n * (n / s->pW));
+      // y: outer x (stride n^2), positions y (stride n)
+      if (fwd) LAUNCH((p_strided<N, true>), gs, s->pxthreads, s->lds_px, s, b, s->pW, n * n, n, s->twN64);
+      else LAUNCH((p_strided<N, false>), gs, s->pxthreads, s->lds_px, s, b, s->pW, n * n, n, s->twN64);
+      // x: outer y (stride n), positions x (stride n^2)
+      if (fwd) LAUNCH((p_strided<N, true>), gs, s->pxthreads, s->lds_px, s, b, s->pW, n, n * n, s->twN64);
+      else LAUNCH((p_strided<N, false>), gs, s->pxthreads, s->lds_px, s, b, s->pW, n, n * n, s->twN64);
+    }
+    return GSK_OK;
+  }
+
+  static int step_mart_plain(gsk_sim* s) {
+    const long N3 = (long)nvox(s), VN = s->V * N3;
+    PlainK k = plain_consts(s);
+    LAUNCH(p_square, grid_1d(VN), 256, 0, s, s->eta64, s->S64, VN);
+    CHK(c2c(s, s->S64, true));
+    LAUNCH(p_contract, grid_1d(N3), 256, 0, s, s->S64, s->R64, s->B64, N3, k);
+    CHK(c2c(s, s->R64, false));
+    LAUNCH(p_force, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->R64, s->F64, N3, k);
+    LAUNCH(p_load, grid_1d(VN), 256, 0, s, s->eta64, s->E64, VN);
+    CHK(c2c(s, s->F64, true));
+    CHK(c2c(s, s->E64, true));
+    LAUNCH(p_update, grid_1d(VN), 256, 0, s, s->E64, s->F64, s->n, k);
+    CHK(c2c(s, s->E64, false));
+    LAUNCH(p_extract, grid_1d(VN), 256, 0, s, s->E64, s->eta64, VN);
+    return GSK_OK;
+  }
+
+  static int energy_mart_el_fused(gsk_sim* s, double* el) {
+    // S_hat = FFT(eta^2) through z and y; the energy kernel applies x and contracts.
+    CHK(zfwd(s, Z_SQUARE, s->fld, s->F));
+    CHK(ypass(s, true, s->F));
+    const int gx = s->n * s->g.nch;
+    if (gx > s->rblocks) return fail(GSK_ERR, "reduction scratch too small");
+    LAUNCH((r_xenergy_mart<N>), gx, s->xthreads, s->lds_xc, s, s->kbuf, s->Bt, s->g, s->F, s->twN, s->part);
+    LAUNCH(r_finish, 1, 1024, 0, s, s->part, gx, 1, s->rout);
+    HIPCHK(hipMemcpyAsync(el, s->rout, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return GSK_OK;
+  }
+
+  static int energy_mart_el_plain(gsk_sim* s, double* gr, double* el) {
+    const long N3 = (long)nvox(s), VN = s->V * N3;
+    PlainK k = plain_consts(s);
+    LAUNCH(p_square, grid_1d(VN), 256, 0, s, s->eta64, s->S64, VN);
+    CHK(c2c(s, s->S64, true));
+    LAUNCH(p_load, grid_1d(VN), 256, 0, s, s->eta64, s->E64, VN);
+    CHK(c2c(s, s->E64, true));
+    const int nb = 256;
+    LAUNCH(r_plain_kspace, nb, 256, 0, s, s->E64, s->S64, s->B64, s->n, s->V, k.kfac, s->part);
+    LAUNCH(r_finish, 1, 1024, 0, s, s->part, nb, 2, s->rout);
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, s->rout, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    *gr = h[0];
+    *el = h[1];
+    return GSK_OK;
+  }
+};
+
+template <template <int> class Fn, typename... Args>
+int dispatch_n(int n, Args&&... args) {
+  switch (n) {
+    case 16: return Fn<16>::run(args...);
+    case 32: return Fn<32>::run(args...);
+    case 64: return Fn<64>::run(args...);
+    case 96: return Fn<96>::run(args...);
+    case 128: return Fn<128>::run(args...);
+    case 192: return Fn<192>::run(args...);
+    case 256: return Fn<256>::run(args...);
+    case 512: return Fn<512>::run(args...);
+  }
+  return fail(GSK_ERR_UNSUPPORTED, "unsupported grid size " + std::to_string(n));
+}
+
+template <int N> struct DAttrs { static int run() { return Impl<N>::set_attrs(); } };
+template <int N> struct DInit { static int run(gsk_sim* s) { return Impl<N>::init_state(s); } };
+template <int N> struct DStep {
+  static int run(gsk_sim* s) {
+    if (s->cfg.path == GSK_PATH_PLAIN64) return Impl<N>::step_mart_plain(s);
+    if (s->cfg.physics == GSK_PHYS_MART) return Impl<N>::step_mart_fused(s);
+    return Impl<N>::step_prec_fused(s);
+  }
+};
+template <int N> struct DElF { static int run(gsk_sim* s, double* el) { return Impl<N>::energy_mart_el_fused(s, el); } };
+template <int N> struct DElP {
+  static int run(gsk_sim* s, double* gr, double* el) { return Impl<N>::energy_mart_el_plain(s, gr, el); }
+};
+template <int N> struct DZfwd { static int run(gsk_sim* s, const float* in, int F) { return Impl<N>::zfwd(s, Z_COPY, in, F); } };
+template <int N> struct DY { static int run(gsk_sim* s, bool fwd, int F) { return Impl<N>::ypass(s, fwd, F); } };
+template <int N> struct DX {
+  static int run(gsk_sim* s, bool fwd, int F, float scale) {
+    const int gx = s->n * s->g.nch;
+    if (fwd) LAUNCH((k_xpass<N, true>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, scale, s->twN);
+    else LAUNCH((k_xpass<N, false>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, scale, s->twN);
+    return GSK_OK;
+  }
+};
+template <int N> struct DZinv { static int run(gsk_sim* s, float* out, int F) { return Impl<N>::zinv(s, out, F); } };
+
+int ensure_device(int dev) {
+  int cnt = 0;
+  HIPCHK(hipGetDeviceCount(&cnt));
+  if (cnt <= 0) return fail(GSK_ERR_HIP, "no HIP device available");
+  if (dev < 0 || dev >= cnt) return fail(GSK_ERR_ARG, "device ordinal out of range");
+  HIPCHK(hipSetDevice(dev));
+  return GSK_OK;
+}
+
+// Choose tile shapes for a given field count F.
+void choose_geometry(gsk_sim* s, int Fz, int Fx_resident) {
+  const int n = s->n, M = n / 2;
+  Geo& g = s->g;
+  g.n = n; g.M = M; g.Pz = M + 16;
+  g.nvox = (long long)n * n * n;
+  g.kfs = (long long)n * n * g.Pz;
+  // z tiles: rows L, threads L*M/rmin <= 256, LDS <= 96 KB
+  const int rz = rmin_of(M);
+  int L = 1;
+  while (L * 2 <= n * n && (L * 2) * M / rz <= 256 &&
+         (2.0 * M + (double)Fz * (L * 2) * (M + 1)) * 8.0 <= 96 * 1024)
+    L *= 2;
+  g.L = L;
+  s->zthreads = round64(std::max(L * M / rz, 64));
+  // x/y tiles: width W <= 16, threads W*n/rmin <= 1024, resident tiles <= 120 KB
+  const int rx = rmin_of(n);
+  int W = 16;
+  while (W > 1 && (W * n / rx > 1024 || (double)(n + (double)Fx_resident * n * W) * 8.0 > 120 * 1024)) W /= 2;
+  g.W = W;
+  g.nch = (M + 1 + W - 1) / W;
+  s->xthreads = round64(std::max(W * n / rx, 64));
+  s->lds_y = (size_t)(n + n * W) * sizeof(float2);
+  s->lds_xc = (size_t)(n + (size_t)Fx_resident * n * W) * sizeof(float2);
+  s->lds_z = (2 * M + (size_t)Fz * L * (M + 1)) * sizeof(float2);
+}
+
+void choose_plain(gsk_sim* s) {
+  const int n = s->n, r = rmin_of(n);
+  int L = 1;
+  while (L * 2 <= n * n && (L * 2) * n / r <= 256 && (double)(n + (L * 2) * (n + 1)) * 16.0 <= 96 * 1024) L *= 2;
+  s->pL = L;
+  s->pzthreads = round64(std::max(L * n / r, 64));
+  int W = 16;
+  while (W > 1 && (W * n / r > 1024 || (double)(n + n * W) * 16.0 > 96 * 1024 || n % W)) W /= 2;
+  s->pW = W;
+  s->pxthreads = round64(std::max(W * n / r, 64));
+  s->lds_pz = (size_t)(n + L * (n + 1)) * sizeof(double2);
+  s->lds_px = (size_t)(n + n * W) * sizeof(double2);
+}
+
+void build_slots(gsk_sim* s) {
+  s->slots.clear();
+  const double hs = (double)hs_count(s) * 8.0;  // one half-spectrum complex field
+  const double rv = (double)nvox(s) * 4.0;      // one real fp32 field
+  const int F = s->F;
+  if (s->cfg.path == GSK_PATH_PLAIN64) {
+    const double cf = (double)nvox(s) * 16.0, r8 = (double)nvox(s) * 8.0;
+    const int V = s->V;
+    auto add_c2c = [&](const char* a, const char* b, const char* c) {
+      for (int f = 0; f < V; ++f) {
+        s->slots.push_back({a, 2 * cf});
+        s->slots.push_back({b, 2 * cf});
+        s->slots.push_back({c, 2 * cf});
+      }
+    };
+    s->slots.push_back({"p_square", V * (r8 + cf)});
+    add_c2c("p_z_fwdS", "p_y_fwdS", "p_x_fwdS");
+    s->slots.push_back({"p_contract", V * 2 * cf + V * V * r8});
+    add_c2c("p_z_invR", "p_y_invR", "p_x_invR");
+    s->slots.push_back({"p_force", V * (r8 + cf + cf)});
+    s->slots.push_back({"p_load", V * (r8 + cf)});
+    add_c2c("p_z_fwdF", "p_y_fwdF", "p_x_fwdF");
+    add_c2c("p_z_fwdE", "p_y_fwdE", "p_x_fwdE");
+    s->slots.push_back({"p_update", V * 3 * cf});
+    add_c2c("p_z_invE", "p_y_invE", "p_x_invE");
+    s->slots.push_back({"p_extract", V * (cf + r8)});
+    return;
+  }
+  if (s->cfg.physics == GSK_PHYS_MART) {
+    const int npq = F * (F + 1) / 2;
+    s->slots.push_back({"zfwd_square", F * (rv + hs)});
+    s->slots.push_back({"y_fwd_S", F * 2 * hs});
+    s->slots.push_back({"x_contract", F * 2 * hs + npq * hs / 2});
+    s->slots.push_back({"y_inv_R", F * 2 * hs});
+    s->slots.push_back({"zmid_force", F * (2 * hs + rv)});
+    s->slots.push_back({"y_fwd_F", F * 2 * hs});
+    s->slots.push_back({"x_update", F * 4 * hs});
+    s->slots.push_back({"y_inv_eta", F * 2 * hs});
+    s->slots.push_back({"zinv_eta", F * (hs + rv)});
+  } else {
+    s->slots.push_back({"zfwd_local", F * (rv + hs)});
+    s->slots.push_back({"y_fwd", F * 2 * hs});
+    s->slots.push_back({"x_update", F * 4 * hs + hs / 2});
+    s->slots.push_back({"y_inv", F * 2 * hs});
+    s->slots.push_back({"zinv", F * (hs + rv)});
+  }
+}
+
+int sync(gsk_sim* s) {
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return GSK_OK;
+}
+
+int prepare(gsk_sim* s) {
+  if (!s->haveB) return fail(GSK_ERR_STATE, "B tensor not built (call gsk_build_btensor first)");
+  if (s->dirty && s->cfg.path == GSK_PATH_FUSED32) {
+    CHK(dispatch_n<DInit>(s->n, s));
+  }
+  s->dirty = false;
+  return GSK_OK;
+}
+
+template <typename T>
+void free_dev(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace
+
+// ======================================================================== C ABI
+extern "C" {
+
+const char* gsk_last_error(void) { return g_err.c_str(); }
+const char* gsk_version(void) { return "gsk 0.1 (gfx950)"; }
+
+int gsk_config_default(gsk_config* c, int physics) {
+  if (!c) return fail(GSK_ERR_ARG, "null config");
+  std::memset(c, 0, sizeof(*c));
+  c->n = 64;
+  c->physics = physics;
+  c->path = GSK_PATH_FUSED32;
+  c->kconv = GSK_K_EXACT;
+  c->dx = 1.0;
+  if (physics == GSK_PHYS_MART) {
+    c->nvar = 3;
+    c->A = 0.2; c->B = 3.0; c->C = 3.0;
+    c->kappa = 1.0; c->L = 1.0; c->dt = 0.01; c->w_el = 1.0;
+  } else if (physics == GSK_PHYS_PREC) {
+    // src/dynamics_precipitate.cu:21-29 defaults
+    c->nvar = 4;
+    const double A[8] = {0.0, 54.0, -17.0, 7.0, 2.5, 0.2, 0.2, 0.2};
+    for (int i = 0; i < 8; ++i) c->a[i] = A[i];
+    c->kappa = 300.0; c->kappa_c = 300.0; c->M = 0.4; c->L = 0.4; c->xi = 400.0;
+    c->c1 = 0.1; c->c2 = 0.44; c->dt = 0.1;
+  } else {
+    return fail(GSK_ERR_ARG, "unknown physics");
+  }
+  return GSK_OK;
+}
+
+int gsk_create(const gsk_config* cfg, gsk_sim** out) {
+  if (!cfg || !out) return fail(GSK_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (!supported_n(cfg->n)) return fail(GSK_ERR_UNSUPPORTED, "unsupported grid size " + std::to_string(cfg->n));
+  if (cfg->physics != GSK_PHYS_MART && cfg->physics != GSK_PHYS_PREC) return fail(GSK_ERR_ARG, "unknown physics");
+  if (cfg->path != GSK_PATH_FUSED32 && cfg->path != GSK_PATH_PLAIN64) return fail(GSK_ERR_ARG, "unknown path");
+  if (cfg->path == GSK_PATH_PLAIN64 && cfg->physics != GSK_PHYS_MART)
+    return fail(GSK_ERR_UNSUPPORTED, "the FP64 plain path implements the martensite rule only");
+  const int maxv = cfg->physics == GSK_PHYS_MART ? 8 : 7;
+  if (cfg->nvar < 1 || cfg->nvar > maxv) return fail(GSK_ERR_ARG, "nvar out of range");
+  if (!(cfg->dx > 0)) return fail(GSK_ERR_ARG, "dx must be positive");
+  CHK(ensure_device(cfg->device));
+  gsk_sim* s = new gsk_sim();
+  s->cfg = *cfg;
+  s->n = cfg->n;
+  s->V = cfg->nvar;
+  s->F = cfg->physics == GSK_PHYS_MART ? s->V : s->V + 1;
+  s->dev = cfg->device;
+  int rc = dispatch_n<DAttrs>(s->n);
+  if (rc != GSK_OK) { delete s; return rc; }
+  auto bail = [&](int code) { gsk_destroy(s); return code; };
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(GSK_ERR_HIP, "hipStreamCreate failed"));
+  const int n = s->n, M = n / 2;
+  const size_t N3 = nvox(s);
+  choose_geometry(s, s->F, s->cfg.physics == GSK_PHYS_MART ? s->F : 1);
+  choose_plain(s);
+  auto up = [&](auto*& dst, const auto& host) -> int {
+    using T = std::remove_reference_t<decltype(*dst)>;
+    if (hipMalloc((void**)&dst, host.size() * sizeof(T)) != hipSuccess) return fail(GSK_ERR_HIP, "hipMalloc failed");
+    if (hipMemcpy(dst, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(GSK_ERR_HIP, "hipMemcpy failed");
+    return GSK_OK;
+  };
+  if ((rc = up(s->twN, twiddles<float2>(n, n))) != GSK_OK) return bail(rc);
+  if ((rc = up(s->twM, twiddles<float2>(M, M))) != GSK_OK) return bail(rc);
+  if ((rc = up(s->rtn, twiddles<float2>(M, n))) != GSK_OK) return bail(rc);
+  if ((rc = up(s->twN64, twiddles<double2>(n, n))) != GSK_OK) return bail(rc);
+  auto alloc = [&](auto*& p, size_t count) -> int {
+    using T = std::remove_reference_t<decltype(*p)>;
+    if (hipMalloc((void**)&p, count * sizeof(T)) != hipSuccess)
+      return fail(GSK_ERR_HIP, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    if (hipMemset(p, 0, count * sizeof(T)) != hipSuccess) return fail(GSK_ERR_HIP, "hipMemset failed");
+    return GSK_OK;
+  };
+  const size_t T = (size_t)n * s->g.W, ntile = (size_t)n * s->g.nch;
+  if (cfg->path == GSK_PATH_FUSED32) {
+    if ((rc = alloc(s->fld, (size_t)s->F * N3)) != GSK_OK) return bail(rc);
+    if ((rc = alloc(s->kbuf, (size_t)s->F * s->g.kfs)) != GSK_OK) return bail(rc);
+    if ((rc = alloc(s->state, ntile * s->F * T)) != GSK_OK) return bail(rc);
+    const int npq = cfg->physics == GSK_PHYS_MART ? s->V * (s->V + 1) / 2 : 1;
+    if ((rc = alloc(s->Bt, ntile * npq * T)) != GSK_OK) return bail(rc);
+  } else {
+    const size_t VN = (size_t)s->V * N3;
+    if ((rc = alloc(s->eta64, VN)) != GSK_OK) return bail(rc);
+    if ((rc = alloc(s->S64, VN)) != GSK_OK) return bail(rc);
+    if ((rc = alloc(s->R64, VN)) != GSK_OK) return bail(rc);
+    if ((rc = alloc(s->E64, VN)) != GSK_OK) return bail(rc);
+    if ((rc = alloc(s->F64, VN)) != GSK_OK) return bail(rc);
+    if ((rc = alloc(s->B64, (size_t)s->V * s->V * N3)) != GSK_OK) return bail(rc);
+  }
+  s->rblocks = std::max<int>(1024, (int)ntile);
+  if ((rc = alloc(s->part, (size_t)s->rblocks * 16)) != GSK_OK) return bail(rc);
+  if ((rc = alloc(s->rout, 16)) != GSK_OK) return bail(rc);
+  build_slots(s);
+  *out = s;
+  return GSK_OK;
+}
+
+int gsk_destroy(gsk_sim* s) {
+  if (!s) return GSK_OK;
+  (void)hipSetDevice(s->dev);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  free_dev(s->twN); free_dev(s->twM); free_dev(s->rtn); free_dev(s->twN64);
+  free_dev(s->fld); free_dev(s->kbuf); free_dev(s->state); free_dev(s->Bt);
+  free_dev(s->eta64); free_dev(s->S64); free_dev(s->R64); free_dev(s->E64); free_dev(s->F64); free_dev(s->B64);
+  free_dev(s->part); free_dev(s->rout);
+  for (auto& e : s->ev) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return GSK_OK;
+}
+
+static int fill_bargs(BArgs& a, const double* C81, const double* e1, int V1, const double* e2, int V2, double kfac) {
+  if (V1 < 1 || V1 > 8 || V2 < 1 || V2 > 8) return fail(GSK_ERR_ARG, "eigenstrain count out of range (1..8)");
+  std::memset(&a, 0, sizeof(a));
+  for (int i = 0; i < 81; ++i) a.C[i] = C81[i];
+  auto sig = [&](const double* e, int V, double* s) {
+    for (int p = 0; p < V; ++p)
+      for (int ij = 0; ij < 9; ++ij) {
+        double acc = 0;
+        for (int kl = 0; kl < 9; ++kl) acc += C81[ij * 9 + kl] * e[p * 9 + kl];
+        s[p * 9 + ij] = acc;
+      }
+  };
+  sig(e1, V1, a.s1);
+  sig(e2, V2, a.s2);
+  for (int p = 0; p < V1; ++p)
+    for (int q = 0; q < V2; ++q) {
+      double acc = 0;
+      for (int ij = 0; ij < 9; ++ij)
+        for (int kl = 0; kl < 9; ++kl) acc += C81[ij * 9 + kl] * e1[p * 9 + ij] * e2[q * 9 + kl];
+      a.cee[p * V2 + q] = acc;
+    }
+  a.V1 = V1;
+  a.V2 = V2;
+  a.kfac = kfac;
+  return GSK_OK;
+}
+
+int gsk_build_btensor(gsk_sim* s, const double* C81, const double* eps, int neps) {
+  if (!s || !C81 || !eps) return fail(GSK_ERR_ARG, "null argument");
+  const int need = s->cfg.physics == GSK_PHYS_MART ? s->V : 1;
+  if (neps != need) return fail(GSK_ERR_ARG, "eigenstrain count must be " + std::to_string(need));
+  CHK(ensure_device(s->dev));
+  const double kfac =
+      s->cfg.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * s->cfg.dx / s->n : 2.0 * kPi / (s->n * s->cfg.dx);
+  BArgs a;
+  CHK(fill_bargs(a, C81, eps, neps, eps, neps, kfac));
+  s->cee0 = a.cee[0];
+  if (s->cfg.path == GSK_PATH_FUSED32) {
+    const long total = (long)s->n * s->g.nch * s->n * s->g.W;
+    hipLaunchKernelGGL(k_btensor_tiles, dim3(grid_1d(total)), dim3(256), 0, s->stream, s->Bt, s->g, a);
+  } else {
+    hipLaunchKernelGGL(k_btensor_full, dim3(grid_1d((long)nvox(s))), dim3(256), 0, s->stream, s->B64, s->n, a);
+  }
+  HIPCHK(hipGetLastError());
+  CHK(sync(s));
+  s->haveB = true;
+  s->dirty = true;
+  return GSK_OK;
+}
+
+static int field_info(gsk_sim* s, const char* name, int* offset_fields, int* count) {
+  if (!name) return fail(GSK_ERR_ARG, "null field name");
+  if (std::strcmp(name, "eta") == 0) {
+    *offset_fields = s->cfg.physics == GSK_PHYS_MART ? 0 : 1;
+    *count = s->V;
+    return GSK_OK;
+  }
+  if (std::strcmp(name, "concentration") == 0 && s->cfg.physics == GSK_PHYS_PREC) {
+    *offset_fields = 0;
+    *count = 1;
+    return GSK_OK;
+  }
+  return fail(GSK_ERR_ARG, std::string("unknown field \"") + name + "\"");
+}
+
+int gsk_set_field(gsk_sim* s, const char* name, const void* host, int dtype) {
+  if (!s || !host) return fail(GSK_ERR_ARG, "null argument");
+  if (dtype != GSK_F32 && dtype != GSK_F64) return fail(GSK_ERR_ARG, "bad dtype");
+  int off, cnt;
+  CHK(field_info(s, name, &off, &cnt));
+  CHK(ensure_device(s->dev));
+  const size_t N3 = nvox(s), count = (size_t)cnt * N3;
+  if (s->cfg.path == GSK_PATH_PLAIN64) {
+    std::vector<double> h(count);
+    for (size_t i = 0; i < count; ++i)
+      h[i] = dtype == GSK_F32 ? (double)((const float*)host)[i] : ((const double*)host)[i];
+    HIPCHK(hipMemcpy(s->eta64, h.data(), count * sizeof(double), hipMemcpyHostToDevice));
+    return GSK_OK;
+  }
+  std::vector<float> h(count);
+  const bool conc = std::strcmp(name, "concentration") == 0;
+  if (conc) {
+    // keep the mean in FP64 and store the fluctuation in FP32
+    double sum = 0.0;
+    for (size_t i = 0; i < count; ++i) sum += dtype == GSK_F32 ? (double)((const float*)host)[i] : ((const double*)host)[i];
+    double mean = sum / (double)count;
+    double resid = 0.0;
+    for (size_t i = 0; i < count; ++i) {
+      double v = dtype == GSK_F32 ? (double)((const float*)host)[i] : ((const double*)host)[i];
+      h[i] = (float)(v - mean);
+      resid += (double)h[i];
+    }
+    s->cbar = mean + resid / (double)count;
+  } else {
+    for (size_t i = 0; i < count; ++i) h[i] = dtype == GSK_F32 ? ((const float*)host)[i] : (float)((const double*)host)[i];
+  }
+  HIPCHK(hipMemcpy(s->fld + (size_t)off * N3, h.data(), count * sizeof(float), hipMemcpyHostToDevice));
+  s->dirty = true;
+  return GSK_OK;
+}
+
+int gsk_get_field(gsk_sim* s, const char* name, void* host, int dtype) {
+  if (!s || !host) return fail(GSK_ERR_ARG, "null argument");
+  if (dtype != GSK_F32 && dtype != GSK_F64) return fail(GSK_ERR_ARG, "bad dtype");
+  int off, cnt;
+  CHK(field_info(s, name, &off, &cnt));
+  CHK(ensure_device(s->dev));
+  CHK(sync(s));
+  const size_t N3 = nvox(s), count = (size_t)cnt * N3;
+  if (s->cfg.path == GSK_PATH_PLAIN64) {
+    std::vector<double> h(count);
+    HIPCHK(hipMemcpy(h.data(), s->eta64, count * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < count; ++i) {
+      if (dtype == GSK_F32) ((float*)host)[i] = (float)h[i];
+      else ((double*)host)[i] = h[i];
+    }
+    return GSK_OK;
+  }
+  std::vector<float> h(count);
+  HIPCHK(hipMemcpy(h.data(), s->fld + (size_t)off * N3, count * sizeof(float), hipMemcpyDeviceToHost));
+  const double add = std::strcmp(name, "concentration") == 0 ? s->cbar : 0.0;
+  for (size_t i = 0; i < count; ++i) {
+    if (dtype == GSK_F32) ((float*)host)[i] = (float)(add + (double)h[i]);
+    else ((double*)host)[i] = add + (double)h[i];
+  }
+  return GSK_OK;
+}
+
+int gsk_init_uniform(gsk_sim* s, const char* name, double lo, double hi, unsigned long long seed) {
+  if (!s) return fail(GSK_ERR_ARG, "null argument");
+  int off, cnt;
+  CHK(field_info(s, name, &off, &cnt));
+  CHK(ensure_device(s->dev));
+  const size_t N3 = nvox(s);
+  // generated on the device in FP32, then passed through set_field so the
+  // concentration mean handling and the FP64 path see the same values
+  float* tmp = nullptr;
+  HIPCHK(hipMalloc((void**)&tmp, (size_t)cnt * N3 * sizeof(float)));
+  for (int f = 0; f < cnt; ++f)
+    hipLaunchKernelGGL(k_uniform, dim3(grid_1d((long)N3 / 4)), dim3(256), 0, s->stream, tmp + f * N3, (long)N3, seed,
+                       (uint32_t)f, lo, hi);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  std::vector<float> h((size_t)cnt * N3);
+  if (e == hipSuccess) e = hipMemcpy(h.data(), tmp, h.size() * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(GSK_ERR_HIP, std::string("init_uniform: ") + hipGetErrorString(e));
+  return gsk_set_field(s, name, h.data(), GSK_F32);
+}
+
+int gsk_step(gsk_sim* s, int nsteps) {
+  if (!s) return fail(GSK_ERR_ARG, "null handle");
+  if (nsteps < 0) return fail(GSK_ERR_ARG, "negative step count");
+  CHK(ensure_device(s->dev));
+  CHK(prepare(s));
+  for (int i = 0; i < nsteps; ++i) {
+    CHK(dispatch_n<DStep>(s->n, s));
+    ++s->step;
+  }
+  return sync(s);
+}
+
+int gsk_set_step(gsk_sim* s, long long step) {
+  if (!s || step < 0) return fail(GSK_ERR_ARG, "bad argument");
+  s->step = step;
+  return GSK_OK;
+}
+long long gsk_get_step(const gsk_sim* s) { return s ? s->step : -1; }
+
+int gsk_set_param(gsk_sim* s, const char* key, double v) {
+  if (!s || !key) return fail(GSK_ERR_ARG, "null argument");
+  gsk_config& c = s->cfg;
+  std::string k(key);
+  if (k == "A") c.A = v; else if (k == "B") c.B = v; else if (k == "C") c.C = v;
+  else if (k == "L") c.L = v; else if (k == "kappa") c.kappa = v; else if (k == "dt") c.dt = v;
+  else if (k == "noise") c.noise = v; else if (k == "w_el") c.w_el = v; else if (k == "M") c.M = v;
+  else if (k == "kappa_c") c.kappa_c = v; else if (k == "xi") c.xi = v; else if (k == "c1") c.c1 = v;
+  else if (k == "c2") c.c2 = v; else if (k == "S") c.S = v;
+  else if (k.size() == 2 && k[0] == 'a' && k[1] >= '1' && k[1] <= '7') c.a[k[1] - '0'] = v;
+  else return fail(GSK_ERR_ARG, "unknown parameter " + k);
+  return GSK_OK;
+}
+
+int gsk_reduce(gsk_sim* s, int what, double* out) {
+  if (!s || !out) return fail(GSK_ERR_ARG, "null argument");
+  CHK(ensure_device(s->dev));
+  CHK(prepare(s));
+  const long N3 = (long)nvox(s);
+  const int nb = 512;
+  const gsk_config& c = s->cfg;
+  double h[16];
+  auto real_pass = [&]() -> int {
+    if (s->cfg.path == GSK_PATH_PLAIN64) {
+      LAUNCH(r_mart_real<double>, nb, 256, 0, s, s->eta64, N3, s->V, c.A, c.B, c.C, s->part);
+    } else if (c.physics == GSK_PHYS_MART) {
+      LAUNCH(r_mart_real<float>, nb, 256, 0, s, s->fld, N3, s->V, c.A, c.B, c.C, s->part);
+    } else {
+      PrecE pe{};
+      for (int i = 0; i < 8; ++i) pe.A[i] = c.a[i];
+      pe.c1 = c.c1; pe.c2 = c.c2; pe.cbar = s->cbar; pe.V = s->V;
+      LAUNCH(r_prec_real, nb, 256, 0, s, s->fld, N3, pe, s->part);
+    }
+    LAUNCH(r_finish, 1, 1024, 0, s, s->part, nb, 2 + s->V, s->rout);
+    HIPCHK(hipMemcpyAsync(h, s->rout, (2 + s->V) * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    return sync(s);
+  };
+  if (what == GSK_RED_MEAN_ETA || what == GSK_RED_SUM_ETA2) {
+    CHK(real_pass());
+    if (what == GSK_RED_SUM_ETA2) {
+      if (c.physics != GSK_PHYS_MART) return fail(GSK_ERR_UNSUPPORTED, "sum eta^2 is a martensite reduction");
+      out[0] = h[1];
+    } else {
+      for (int v = 0; v < s->V; ++v) out[v] = h[2 + v] / (double)N3;
+    }
+    return GSK_OK;
+  }
+  if (what == GSK_RED_MASS) {
+    if (c.physics != GSK_PHYS_PREC) return fail(GSK_ERR_UNSUPPORTED, "mass is a precipitate reduction");
+    CHK(real_pass());
+    out[0] = s->cbar * (double)N3 + h[1];
+    return GSK_OK;
+  }
+  if (what == GSK_RED_ENERGY) {
+    CHK(real_pass());
+    const double chem = h[0];
+    double gr = 0.0, el = 0.0;
+    if (s->cfg.path == GSK_PATH_PLAIN64) {
+      double g2, e2;
+      CHK(dispatch_n<DElP>(s->n, s, &g2, &e2));
+      gr = 0.5 * c.kappa * g2 / (double)N3;
+      el = 0.5 * c.w_el * e2 / (double)N3;
+    } else {
+      const int ntile = s->n * s->g.nch;
+      const double kfac = c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx);
+      LAUNCH(r_state_grad, nb, 256, 0, s, s->state, s->g, s->F, (float)kfac, s->part);
+      LAUNCH(r_finish, 1, 1024, 0, s, s->part, nb, s->F, s->rout);
+      double gh[16];
+      HIPCHK(hipMemcpyAsync(gh, s->rout, s->F * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+      CHK(sync(s));
+      if (c.physics == GSK_PHYS_MART) {
+        double acc = 0;
+        for (int f = 0; f < s->F; ++f) acc += gh[f];
+        gr = 0.5 * c.kappa * acc / (double)N3;
+        double e2;
+        CHK(dispatch_n<DElF>(s->n, s, &e2));
+        el = 0.5 * c.w_el * e2 / (double)N3;
+      } else {
+        double acc = 0;
+        for (int f = 1; f < s->F; ++f) acc += gh[f];
+        gr = 0.5 * c.kappa_c * gh[0] / (double)N3 + 0.5 * c.kappa * acc / (double)N3;
+        LAUNCH(r_state_el_prec, nb, 256, 0, s, s->state, s->Bt, s->g, s->F, s->part);
+        LAUNCH(r_finish, 1, 1024, 0, s, s->part, nb, 1, s->rout);
+        double eh;
+        HIPCHK(hipMemcpyAsync(&eh, s->rout, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        CHK(sync(s));
+        // the k = 0 mode of c is N * cbar (the FP32 state holds the fluctuation only)
+        const double c0 = (double)N3 * s->cbar;
+        el = 0.5 * c.xi * (eh + s->cee0 * c0 * c0) / (double)N3;
+        (void)ntile;
+      }
+    }
+    out[0] = chem; out[1] = gr; out[2] = el; out[3] = chem + gr + el;
+    return GSK_OK;
+  }
+  return fail(GSK_ERR_ARG, "unknown reduction");
+}
+
+int gsk_time_steps(gsk_sim* s, int nsteps, double* ms) {
+  if (!s || !ms || nsteps <= 0) return fail(GSK_ERR_ARG, "bad argument");
+  CHK(ensure_device(s->dev));
+  CHK(prepare(s));
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  HIPCHK(hipEventRecord(a, s->stream));
+  for (int i = 0; i < nsteps; ++i) {
+    int rc = dispatch_n<DStep>(s->n, s);
+    if (rc != GSK_OK) return rc;
+    ++s->step;
+  }
+  HIPCHK(hipEventRecord(b, s->stream));
+  HIPCHK(hipEventSynchronize(b));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = (double)t;
+  return GSK_OK;
+}
+
+int gsk_kernel_times(gsk_sim* s, int nsteps, double* ms, int maxk, int* nk, char* names, int names_len) {
+  if (!s || !ms || !nk || nsteps <= 0) return fail(GSK_ERR_ARG, "bad argument");
+  CHK(ensure_device(s->dev));
+  CHK(prepare(s));
+  const int ns = (int)s->slots.size();
+  const int need = ns * nsteps + 1;
+  while ((int)s->ev.size() < need) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    s->ev.push_back(e);
+  }
+  s->prof = true;
+  s->evi = 0;
+  for (int i = 0; i < nsteps; ++i) {
+    int rc = dispatch_n<DStep>(s->n, s);
+    if (rc != GSK_OK) { s->prof = false; return rc; }
+    ++s->step;
+  }
+  CHK(mark(s));
+  s->prof = false;
+  HIPCHK(hipEventSynchronize(s->ev[s->evi - 1]));
+  if (s->evi != need) return fail(GSK_ERR, "kernel slot count mismatch");
+  const int out = std::min(ns, maxk);
+  for (int k = 0; k < out; ++k) ms[k] = 0.0;
+  for (int i = 0; i < nsteps; ++i)
+    for (int k = 0; k < ns; ++k) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, s->ev[i * ns + k], s->ev[i * ns + k + 1]));
+      if (k < out) ms[k] += t;
+    }
+  *nk = ns;
+  if (names && names_len > 0) {
+    std::string all;
+    for (int k = 0; k < ns; ++k) { if (k) all += ","; all += s->slots[k].name; }
+    std::snprintf(names, names_len, "%s", all.c_str());
+  }
+  return GSK_OK;
+}
+
+int gsk_kernel_bytes(gsk_sim* s, double* bytes, int maxk, int* nk) {
+  if (!s || !bytes || !nk) return fail(GSK_ERR_ARG, "null argument");
+  const int ns = (int)s->slots.size();
+  for (int k = 0; k < std::min(ns, maxk); ++k) bytes[k] = s->slots[k].bytes;
+  *nk = ns;
+  return GSK_OK;
+}
+
+int gsk_noise_field(unsigned long long seed, unsigned int step, unsigned int field, long long nv, float* host_out,
+                    int device) {
+  if (!host_out || nv <= 0 || nv % 4) return fail(GSK_ERR_ARG, "nvox must be a positive multiple of 4");
+  CHK(ensure_device(device));
+  float* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, nv * sizeof(float)));
+  hipLaunchKernelGGL(k_noise, dim3(grid_1d(nv / 4)), dim3(256), 0, 0, d, (long)nv, seed, step, field);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(host_out, d, nv * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(GSK_ERR_HIP, std::string("noise_field: ") + hipGetErrorString(e));
+  return GSK_OK;
+}
+
+int gsk_btensor_full(int n, double dx, int kconv, const double* C81, const double* eps1, int V1, const double* eps2,
+                     int V2, double* host_out, int device) {
+  if (!C81 || !eps1 || !eps2 || !host_out || n < 2) return fail(GSK_ERR_ARG, "bad argument");
+  CHK(ensure_device(device));
+  const double kfac = kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * dx / n : 2.0 * kPi / (n * dx);
+  BArgs a;
+  CHK(fill_bargs(a, C81, eps1, V1, eps2, V2, kfac));
+  const size_t N3 = (size_t)n * n * n, cnt = N3 * V1 * V2;
+  double* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, cnt * sizeof(double)));
+  hipLaunchKernelGGL(k_btensor_full, dim3(grid_1d((long)N3)), dim3(256), 0, 0, d, n, a);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(host_out, d, cnt * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(GSK_ERR_HIP, std::string("btensor_full: ") + hipGetErrorString(e));
+  return GSK_OK;
+}
+
+static int fft_sim(int n, int count, int device, gsk_sim** out) {
+  if (count < 1 || count > 8) return fail(GSK_ERR_ARG, "count out of range (1..8)");
+  gsk_config c;
+  CHK(gsk_config_default(&c, GSK_PHYS_MART));
+  c.n = n;
+  c.nvar = count;
+  c.device = device;
+  return gsk_create(&c, out);
+}
+
+int gsk_fft_r2c(int n, int count, const float* in, float* out, int device) {
+  if (!in || !out) return fail(GSK_ERR_ARG, "null argument");
+  gsk_sim* s = nullptr;
+  CHK(fft_sim(n, count, device, &s));
+  const size_t N3 = nvox(s), M = n / 2;
+  int rc = GSK_OK;
+  if (hipMemcpy(s->fld, in, count * N3 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(GSK_ERR_HIP, "copy in failed");
+  if (rc == GSK_OK) rc = dispatch_n<DZfwd>(n, s, (const float*)s->fld, count);
+  if (rc == GSK_OK) rc = dispatch_n<DY>(n, s, true, count);
+  if (rc == GSK_OK) rc = dispatch_n<DX>(n, s, true, count, 1.0f);
+  if (rc == GSK_OK) rc = sync(s);
+  if (rc == GSK_OK) {
+    std::vector<float2> h((size_t)count * s->g.kfs);
+    if (hipMemcpy(h.data(), s->kbuf, h.size() * sizeof(float2), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(GSK_ERR_HIP, "copy out failed");
+    for (size_t f = 0; rc == GSK_OK && f < (size_t)count; ++f)
+      for (size_t r = 0; r < (size_t)n * n; ++r)
+        for (size_t k = 0; k <= M; ++k) {
+          float2 v = h[f * s->g.kfs + r * s->g.Pz + k];
+          size_t o = ((f * n * n + r) * (M + 1) + k) * 2;
+          out[o] = v.x;
+          out[o + 1] = v.y;
+        }
+  }
+  gsk_destroy(s);
+  return rc;
+}
+
+int gsk_fft_c2r(int n, int count, const float* in, float* out, int device) {
+  if (!in || !out) return fail(GSK_ERR_ARG, "null argument");
+  gsk_sim* s = nullptr;
+  CHK(fft_sim(n, count, device, &s));
+  const size_t N3 = nvox(s), M = n / 2;
+  std::vector<float2> h((size_t)count * s->g.kfs, make_float2(0.f, 0.f));
+  for (size_t f = 0; f < (size_t)count; ++f)
+    for (size_t r = 0; r < (size_t)n * n; ++r)
+      for (size_t k = 0; k <= M; ++k) {
+        size_t o = ((f * n * n + r) * (M + 1) + k) * 2;
+        h[f * s->g.kfs + r * s->g.Pz + k] = make_float2(in[o], in[o + 1]);
+      }
+  int rc = GSK_OK;
+  if (hipMemcpy(s->kbuf, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(GSK_ERR_HIP, "copy in failed");
+  if (rc == GSK_OK) rc = dispatch_n<DX>(n, s, false, count, 1.0f);
+  if (rc == GSK_OK) rc = dispatch_n<DY>(n, s, false, count);
+  if (rc == GSK_OK) rc = dispatch_n<DZinv>(n, s, s->fld, count);
+  if (rc == GSK_OK) rc = sync(s);
+  if (rc == GSK_OK && hipMemcpy(out, s->fld, count * N3 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(GSK_ERR_HIP, "copy out failed");
+  gsk_destroy(s);
+  return rc;
+}
+
+}  // extern "C"

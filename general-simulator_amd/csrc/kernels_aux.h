@@ -1,0 +1,451 @@
+// B-tensor builder (FP64), FP64 plain-path kernels and FP64 reductions.
+#pragma once
+#include "kernels_fused.h"
+
+namespace gsk {
+
+// ====================================================================== Khachaturyan B tensor
+// B_pq(n) = C_ijkl e^p_ij e^q_kl - n_i s^p_ij Omega_jk s^q_kl n_l, s = C:e,
+// Omega = (C_ikjl n_k n_l)^-1, or 0 when |det| < 1e-8; n = k/|k|, or 0 when
+// |k| <= 5e-7 (src/gtensorb.cu:14-34, 110-182). Evaluated in FP64.
+struct BArgs {
+  double C[81];
+  double s1[8 * 9];
+  double s2[8 * 9];
+  double cee[64];
+  int V1, V2;
+  double kfac;
+};
+
+__device__ __forceinline__ void btensor_at(int kx, int ky, int kz, int n, const BArgs& a, double* out) {
+  const int mx = (kx < n / 2) ? kx : kx - n, my = (ky < n / 2) ? ky : ky - n, mz = (kz < n / 2) ? kz : kz - n;
+  const double g0 = a.kfac * mx, g1 = a.kfac * my, g2 = a.kfac * mz;
+  const double mo = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+  double u[3] = {0.0, 0.0, 0.0};
+  if (mo > 5e-7) { u[0] = g0 / mo; u[1] = g1 / mo; u[2] = g2 / mo; }
+  double m[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) acc += a.C[((i * 3 + k) * 3 + l) * 3 + j] * u[k] * u[l];
+      m[i * 3 + j] = acc;
+    }
+  const double det = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                     m[2] * (m[3] * m[7] - m[4] * m[6]);
+  double om[9];
+  if (fabs(det) < 1e-8) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) om[i] = 0.0;
+  } else {
+    const double r = 1.0 / det;
+    om[0] = (m[4] * m[8] - m[5] * m[7]) * r;
+    om[1] = (m[2] * m[7] - m[1] * m[8]) * r;
+    om[2] = (m[1] * m[5] - m[2] * m[4]) * r;
+    om[3] = (m[5] * m[6] - m[3] * m[8]) * r;
+    om[4] = (m[0] * m[8] - m[2] * m[6]) * r;
+    om[5] = (m[2] * m[3] - m[0] * m[5]) * r;
+    om[6] = (m[3] * m[7] - m[4] * m[6]) * r;
+    om[7] = (m[1] * m[6] - m[0] * m[7]) * r;
+    om[8] = (m[0] * m[4] - m[1] * m[3]) * r;
+  }
+  for (int p = 0; p < a.V1; ++p) {
+    double av[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) av[j] = u[0] * a.s1[p * 9 + j] + u[1] * a.s1[p * 9 + 3 + j] + u[2] * a.s1[p * 9 + 6 + j];
+    double aw[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) aw[k] = av[0] * om[k] + av[1] * om[3 + k] + av[2] * om[6 + k];
+    for (int q = 0; q < a.V2; ++q) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        t += aw[k] * (a.s2[q * 9 + k * 3] * u[0] + a.s2[q * 9 + k * 3 + 1] * u[1] + a.s2[q * 9 + k * 3 + 2] * u[2]);
+      out[p * a.V2 + q] = a.cee[p * a.V2 + q] - t;
+    }
+  }
+}
+
+// Full layout B[p][q][x][y][z], FP64.
+__global__ void k_btensor_full(double* __restrict__ B, int n, BArgs a) {
+  const long N3 = (long)n * n * n;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(i / ((long)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
+    double out[64];
+    btensor_at(x, y, z, n, a, out);
+    for (int pq = 0; pq < a.V1 * a.V2; ++pq) B[pq * N3 + i] = out[pq];
+  }
+}
+
+// Tile-ordered FP32 layout for the fused x pass, packed p <= q (V1 == V2).
+__global__ void k_btensor_tiles(float* __restrict__ Bt, Geo g, BArgs a) {
+  const int n = g.n, W = g.W, T = n * W;
+  const int V = a.V1, npq = V * (V + 1) / 2;
+  const long total = (long)n * g.nch * T;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int e = (int)(i % T);
+    const long t = i / T;
+    const int ky = (int)(t / g.nch), ch = (int)(t % g.nch);
+    const int kx = e / W, kz = ch * W + e % W;
+    double out[64];
+    if (kz <= g.M) btensor_at(kx, ky, kz, n, a, out);
+    for (int p = 0; p < V; ++p)
+      for (int q = p; q < V; ++q) {
+        const int pq = p * V - (p * (p - 1)) / 2 + (q - p);
+        Bt[(t * npq + pq) * T + e] = (kz <= g.M) ? (float)out[p * V + q] : 0.f;
+      }
+  }
+}
+
+// ====================================================================== FP64 plain path
+// One kernel per reference operation; 3D C2C transforms as three axis passes.
+struct PlainK {
+  double A, B, C, two_wel, noise, dtL, dtLk, kfac, inv_n3;
+  int V;
+  uint32_t step;
+  unsigned long long seed;
+};
+
+// contiguous rows (z axis), L rows of N per tile
+template <int N, bool FWD>
+__global__ void __launch_bounds__(1024) p_zrows(double2* __restrict__ buf, int L, const double2* __restrict__ twN) {
+  extern __shared__ double2 psm[];
+  double2* tw = psm;
+  double2* tile = psm + N;
+  const long r0 = (long)blockIdx.x * L;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twN[i];
+  for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
+    const int l = e / N, m = e % N;
+    tile[l * (N + 1) + m] = buf[(r0 + l) * N + m];
+  }
+  __syncthreads();
+  Lines Ls{L, L, N + 1, 0, 1};
+  fft_lines<double2, N, FWD>(tile, Ls, tw);
+  for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
+    const int l = e / N, m = e % N;
+    buf[(r0 + l) * N + m] = tile[l * (N + 1) + m];
+  }
+}
+
+// strided lines: block -> (outer, chunk); element (pos, c) at outer*os + pos*ps + chunk*W + c
+template <int N, bool FWD>
+__global__ void __launch_bounds__(1024) p_strided(double2* __restrict__ buf, int W, long os, long ps,
+                                                 const double2* __restrict__ twN) {
+  extern __shared__ double2 psm[];
+  double2* tw = psm;
+  double2* tile = psm + N;
+  const int nch = N / W;
+  const long o = blockIdx.x / nch;
+  const int ch = blockIdx.x % nch;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twN[i];
+  double2* base = buf + o * os + (long)ch * W;
+  for (int e = threadIdx.x; e < N * W; e += blockDim.x) tile[e] = base[(e / W) * ps + e % W];
+  __syncthreads();
+  Lines Ls{W, W, 1, 0, W};
+  fft_lines<double2, N, FWD>(tile, Ls, tw);
+  for (int e = threadIdx.x; e < N * W; e += blockDim.x) base[(e / W) * ps + e % W] = tile[e];
+}
+
+__global__ void p_square(const double* __restrict__ eta, double2* __restrict__ S, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double e = eta[i];
+    S[i] = make_double2(e * e, 0.0);
+  }
+}
+
+__global__ void p_load(const double* __restrict__ eta, double2* __restrict__ E, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    E[i] = make_double2(eta[i], 0.0);
+}
+
+__global__ void p_contract(const double2* __restrict__ S, double2* __restrict__ R, const double* __restrict__ B,
+                           long N3, PlainK k) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
+    for (int p = 0; p < k.V; ++p) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int q = 0; q < k.V; ++q) {
+        const double b = B[(long)(p * k.V + q) * N3 + i];
+        const double2 s = S[q * N3 + i];
+        acc.x += b * s.x;
+        acc.y += b * s.y;
+      }
+      R[p * N3 + i] = make_double2(acc.x * k.inv_n3, acc.y * k.inv_n3);
+    }
+  }
+}
+
+__global__ void p_force(const double* __restrict__ eta, const double2* __restrict__ R, double2* __restrict__ Fh,
+                        long N3, PlainK k) {
+  for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
+    for (int i = 0; i < 4; ++i) {
+      const long v = g4 * 4 + i;
+      double s2 = 0.0;
+      for (int p = 0; p < k.V; ++p) s2 += eta[p * N3 + v] * eta[p * N3 + v];
+      for (int p = 0; p < k.V; ++p) {
+        const double e = eta[p * N3 + v];
+        double f = k.A * e - k.B * e * e + k.C * e * s2 + k.two_wel * e * R[p * N3 + v].x;
+        Fh[p * N3 + v] = make_double2(f, 0.0);
+      }
+    }
+    if (k.noise != 0.0) {
+      for (int p = 0; p < k.V; ++p) {
+        float xi[4];
+        normal4(k.seed, (uint32_t)g4, (uint32_t)p, k.step, xi);
+        for (int i = 0; i < 4; ++i) Fh[p * N3 + g4 * 4 + i].x += k.noise * (double)xi[i];
+      }
+    }
+  }
+}
+
+__global__ void p_update(double2* __restrict__ E, const double2* __restrict__ Fh, int n, PlainK k) {
+  const long N3 = (long)n * n * n;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < (long)k.V * N3; i += (long)gridDim.x * blockDim.x) {
+    const long v = i % N3;
+    const int x = (int)(v / ((long)n * n)), y = (int)((v / n) % n), z = (int)(v % n);
+    const double kx = k.kfac * ((x < n / 2) ? x : x - n), ky = k.kfac * ((y < n / 2) ? y : y - n),
+                 kz = k.kfac * ((z < n / 2) ? z : z - n);
+    const double den = 1.0 / (1.0 + k.dtLk * (kx * kx + ky * ky + kz * kz)) * k.inv_n3;
+    const double2 e = E[i], f = Fh[i];
+    E[i] = make_double2((e.x - k.dtL * f.x) * den, (e.y - k.dtL * f.y) * den);
+  }
+}
+
+__global__ void p_extract(const double2* __restrict__ E, double* __restrict__ eta, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) eta[i] = E[i].x;
+}
+
+// ====================================================================== FP64 reductions
+// Two-stage deterministic sums: each block writes a partial, a single block adds them.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)((blockDim.x + 63) / 64); ++i) r += sh[i];
+  }
+  __syncthreads();
+  return r;
+}
+
+__global__ void r_finish(const double* __restrict__ part, int np, int nout, double* __restrict__ out) {
+  // part[b*nout + j]
+  __shared__ double sh[16];
+  for (int j = 0; j < nout; ++j) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < np; b += blockDim.x) v += part[(long)b * nout + j];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) out[j] = v;
+  }
+}
+
+// Real-space terms: martensite chemical energy, sum eta^2, per-variant sums.
+// out slots: 0 chem, 1 sum eta^2, 2.. 2+V per-variant sums
+template <typename T>
+__global__ void r_mart_real(const T* __restrict__ eta, long N3, int V, double A, double B, double C,
+                            double* __restrict__ part) {
+  __shared__ double sh[16];
+  double chem = 0.0, s2t = 0.0, sv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
+    double s2 = 0.0, s3 = 0.0;
+    for (int v = 0; v < V; ++v) {
+      const double e = (double)eta[v * N3 + i];
+      s2 += e * e; s3 += e * e * e; sv[v] += e;
+    }
+    chem += 0.5 * A * s2 - B / 3.0 * s3 + 0.25 * C * s2 * s2;
+    s2t += s2;
+  }
+  const int nout = 2 + V;
+  double r = block_sum(chem, sh);
+  if (threadIdx.x == 0) part[blockIdx.x * nout + 0] = r;
+  r = block_sum(s2t, sh);
+  if (threadIdx.x == 0) part[blockIdx.x * nout + 1] = r;
+  for (int v = 0; v < V; ++v) {
+    r = block_sum(sv[v], sh);
+    if (threadIdx.x == 0) part[blockIdx.x * nout + 2 + v] = r;
+  }
+}
+
+// Precipitate real-space: chemical energy (exact antiderivative of the local forces),
+// sum of dc, per-variant sums. c = cbar + dc.
+__device__ __forceinline__ double prec_f(double c, const double* e, int V, const double* A, double c1, double c2) {
+  double s2 = 0.0;
+  for (int v = 0; v < V; ++v) s2 += e[v] * e[v];
+  double f = 0.5 * A[1] * (c - c1) * (c - c1) + 0.5 * A[2] * (c - c2) * s2;
+  for (int v = 0; v < V; ++v) {
+    const double q = e[v] * e[v];
+    f += -0.25 * A[3] * q * q + A[4] / 6.0 * q * q * q;
+  }
+  for (int i = 0; i < V; ++i)
+    for (int j = i + 1; j < V; ++j) {
+      const double a = e[i] * e[i], b = e[j] * e[j];
+      f += A[5] * a * b + A[6] * (a * a * b + a * b * b);
+      for (int k = j + 1; k < V; ++k) f += 2.0 * A[7] * a * b * e[k] * e[k];
+    }
+  return f;
+}
+
+struct PrecE { double A[8]; double c1, c2, cbar; int V; };
+
+__global__ void r_prec_real(const float* __restrict__ fld, long N3, PrecE pe, double* __restrict__ part) {
+  __shared__ double sh[16];
+  double chem = 0.0, sdc = 0.0, sv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int V = pe.V;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
+    double e[8];
+    const double dc = (double)fld[i];
+    for (int v = 0; v < V; ++v) { e[v] = (double)fld[(v + 1) * N3 + i]; sv[v] += e[v]; }
+    chem += prec_f(pe.cbar + dc, e, V, pe.A, pe.c1, pe.c2);
+    sdc += dc;
+  }
+  const int nout = 2 + V;
+  double r = block_sum(chem, sh);
+  if (threadIdx.x == 0) part[blockIdx.x * nout + 0] = r;
+  r = block_sum(sdc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x * nout + 1] = r;
+  for (int v = 0; v < V; ++v) {
+    r = block_sum(sv[v], sh);
+    if (threadIdx.x == 0) part[blockIdx.x * nout + 2 + v] = r;
+  }
+}
+
+// Gradient energy from the tile-ordered fused state: sum_k w(kz) k^2 |X(k)|^2 per field
+// (w = 1 for kz in {0, n/2}, 2 otherwise); optional k^2 exponent for Cahn-Hilliard is not needed.
+__global__ void r_state_grad(const float2* __restrict__ state, Geo g, int F, float kfac, double* __restrict__ part) {
+  __shared__ double sh[16];
+  const int n = g.n, W = g.W, T = n * W;
+  const long total = (long)n * g.nch * F * T;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int e = (int)(i % T);
+    const long tf = i / T;
+    const int f = (int)(tf % F);
+    const long t = tf / F;
+    const int ky = (int)(t / g.nch), ch = (int)(t % g.nch);
+    const int kx = e / W, kz = ch * W + e % W;
+    if (kz > g.M) continue;
+    const double kxv = (double)kfac * ((kx < n / 2) ? kx : kx - n), kyv = (double)kfac * ((ky < n / 2) ? ky : ky - n),
+                 kzv = (double)kfac * kz;
+    const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
+    const float2 s = state[i];
+    acc[f] += w * (kxv * kxv + kyv * kyv + kzv * kzv) * ((double)s.x * s.x + (double)s.y * s.y);
+  }
+  for (int f = 0; f < F; ++f) {
+    double r = block_sum(acc[f], sh);
+    if (threadIdx.x == 0) part[blockIdx.x * F + f] = r;
+  }
+}
+
+// Precipitate elastic energy sum_k w B(k) |dc_hat(k)|^2 from the state (field 0).
+__global__ void r_state_el_prec(const float2* __restrict__ state, const float* __restrict__ Bt, Geo g, int F,
+                                double* __restrict__ part) {
+  __shared__ double sh[16];
+  const int W = g.W, T = g.n * W;
+  const long total = (long)g.n * g.nch * T;
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int e = (int)(i % T);
+    const long t = i / T;
+    const int kz = (int)(t % g.nch) * W + e % W;
+    if (kz > g.M) continue;
+    const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
+    const float2 s = state[(t * F + 0) * T + e];
+    acc += w * (double)Bt[t * T + e] * ((double)s.x * s.x + (double)s.y * s.y);
+  }
+  double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// Martensite elastic energy sum_k w sum_pq B_pq Re(conj(S_p) S_q) after the forward x transform of S.
+template <int N>
+__global__ void __launch_bounds__(1024) r_xenergy_mart(const float2* __restrict__ kbuf, const float* __restrict__ Bt,
+                                                      Geo g, int F, const float2* __restrict__ twN,
+                                                      double* __restrict__ part) {
+  extern __shared__ float2 smem[];
+  __shared__ double sh[16];
+  float2* tw = smem;
+  float2* tiles = smem + N;
+  const int W = g.W, T = N * W;
+  const int t = blockIdx.x, ky = t / g.nch, ch = t % g.nch;
+  load_tw<N>(tw, twN);
+  for (int f = 0; f < F; ++f) x_load(tiles + f * T, kbuf + f * g.kfs, g, N, ky, ch);
+  __syncthreads();
+  Lines Ls{W, W, 1, 0, W};
+  for (int f = 0; f < F; ++f) fft_lines<float2, N, true>(tiles + f * T, Ls, tw);
+  const int npq = F * (F + 1) / 2;
+  const float* Bb = Bt + (long)t * npq * T;
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < T; e += blockDim.x) {
+    const int kz = ch * W + e % W;
+    if (kz > g.M) continue;
+    const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
+    double s = 0.0;
+    for (int p = 0; p < F; ++p)
+      for (int q = 0; q < F; ++q) {
+        const int a = p < q ? p : q, b = p < q ? q : p;
+        const int pq = a * F - (a * (a - 1)) / 2 + (b - a);
+        const float2 sp = tiles[p * T + e], sq = tiles[q * T + e];
+        s += (double)Bb[(long)pq * T + e] * ((double)sp.x * sq.x + (double)sp.y * sq.y);
+      }
+    acc += w * s;
+  }
+  double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// Plain-path energy pieces on full complex spectra (FP64): gradient sum k^2 |E|^2, elastic B_pq Re(S_p* S_q).
+__global__ void r_plain_kspace(const double2* __restrict__ E, const double2* __restrict__ S,
+                               const double* __restrict__ B, int n, int V, double kfac, double* __restrict__ part) {
+  __shared__ double sh[16];
+  const long N3 = (long)n * n * n;
+  double gr = 0.0, el = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(i / ((long)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
+    const double kx = kfac * ((x < n / 2) ? x : x - n), ky = kfac * ((y < n / 2) ? y : y - n),
+                 kz = kfac * ((z < n / 2) ? z : z - n);
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    for (int p = 0; p < V; ++p) {
+      const double2 e = E[p * N3 + i];
+      gr += k2 * (e.x * e.x + e.y * e.y);
+      for (int q = 0; q < V; ++q) {
+        const double2 sp = S[p * N3 + i], sq = S[q * N3 + i];
+        el += B[(long)(p * V + q) * N3 + i] * (sp.x * sq.x + sp.y * sq.y);
+      }
+    }
+  }
+  double r = block_sum(gr, sh);
+  if (threadIdx.x == 0) part[blockIdx.x * 2] = r;
+  r = block_sum(el, sh);
+  if (threadIdx.x == 0) part[blockIdx.x * 2 + 1] = r;
+}
+
+// Device Philox normal field (for bit-exact host comparison).
+__global__ void k_noise(float* __restrict__ out, long nvox, unsigned long long seed, uint32_t step, uint32_t field) {
+  for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < nvox / 4; g4 += (long)gridDim.x * blockDim.x) {
+    float xi[4];
+    normal4(seed, (uint32_t)g4, field, step, xi);
+    *reinterpret_cast<float4*>(out + 4 * g4) = make_float4(xi[0], xi[1], xi[2], xi[3]);
+  }
+}
+
+// Philox uniform initial field lo + (hi - lo) * u, rounded to fp32 from fp64.
+__global__ void k_uniform(float* __restrict__ out, long nvox, unsigned long long seed, uint32_t field, double lo,
+                          double hi) {
+  for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < nvox / 4; g4 += (long)gridDim.x * blockDim.x) {
+    uint32_t u[4];
+    uniform4_bits(seed, (uint32_t)g4, field, u);
+    float4 r;
+    r.x = (float)(lo + (hi - lo) * ((double)u[0] * (1.0 / 16777216.0)));
+    r.y = (float)(lo + (hi - lo) * ((double)u[1] * (1.0 / 16777216.0)));
+    r.z = (float)(lo + (hi - lo) * ((double)u[2] * (1.0 / 16777216.0)));
+    r.w = (float)(lo + (hi - lo) * ((double)u[3] * (1.0 / 16777216.0)));
+    *reinterpret_cast<float4*>(out + 4 * g4) = r;
+  }
+}
+
+}  // namespace gsk

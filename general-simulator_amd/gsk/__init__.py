@@ -1,0 +1,271 @@
+"""Python binding of libgsk.so, the MI355X phase-field step library (C ABI in include/gsk.h).
+
+The library is the only compute path: there is no CPU fallback. Importing this
+module loads ``libgsk.so`` from the package directory and raises if it is absent.
+
+Mirrors the reference plugin surface for the phase-field step
+(General-Simulator ``src/dynamics.h:9-25``): a simulation handle is created with the
+parameters ``Dynamics_mart::Initialize`` / ``Dynamics_precipitate::Initialize`` read
+(``src/dynamics_mart.cu:14-173``, ``src/dynamics_precipitate.cu:17-110``), the
+Khachaturyan tensor is built once (``src/gtensorb.cu:55-108``), and ``step`` advances
+the fields like ``Calculate`` inside ``GS::Run`` (``src/gs.cu:201-209``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libgsk.so")
+
+GSK_OK = 0
+PHYS_MART, PHYS_PREC = 0, 1
+PATH_FUSED32, PATH_PLAIN64 = 0, 1
+K_EXACT, K_REFERENCE = 0, 1
+F32, F64 = 0, 1
+RED_ENERGY, RED_MASS, RED_MEAN_ETA, RED_SUM_ETA2 = 1, 2, 3, 4
+
+
+class GskError(RuntimeError):
+    """Raised when a gsk_* call returns a negative status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gsk error {code}: {msg}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("nvar", ctypes.c_int), ("physics", ctypes.c_int), ("path", ctypes.c_int),
+        ("kconv", ctypes.c_int), ("device", ctypes.c_int), ("dx", ctypes.c_double), ("seed", ctypes.c_ulonglong),
+        ("L", ctypes.c_double), ("kappa", ctypes.c_double), ("dt", ctypes.c_double), ("noise", ctypes.c_double),
+        ("A", ctypes.c_double), ("B", ctypes.c_double), ("C", ctypes.c_double), ("w_el", ctypes.c_double),
+        ("a", ctypes.c_double * 8), ("c1", ctypes.c_double), ("c2", ctypes.c_double), ("M", ctypes.c_double),
+        ("kappa_c", ctypes.c_double), ("xi", ctypes.c_double), ("S", ctypes.c_double),
+    ]
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libgsk.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libgsk.so not found at {LIB_PATH}; build it with __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I, D, U = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong
+    L.gsk_last_error.restype = ctypes.c_char_p
+    L.gsk_version.restype = ctypes.c_char_p
+    L.gsk_config_default.argtypes = [ctypes.POINTER(Config), I]
+    L.gsk_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(P)]
+    L.gsk_destroy.argtypes = [P]
+    L.gsk_build_btensor.argtypes = [P, P, P, I]
+    L.gsk_set_field.argtypes = [P, ctypes.c_char_p, P, I]
+    L.gsk_get_field.argtypes = [P, ctypes.c_char_p, P, I]
+    L.gsk_init_uniform.argtypes = [P, ctypes.c_char_p, D, D, U]
+    L.gsk_step.argtypes = [P, I]
+    L.gsk_set_step.argtypes = [P, ctypes.c_longlong]
+    L.gsk_get_step.argtypes = [P]
+    L.gsk_get_step.restype = ctypes.c_longlong
+    L.gsk_set_param.argtypes = [P, ctypes.c_char_p, D]
+    L.gsk_reduce.argtypes = [P, I, P]
+    L.gsk_time_steps.argtypes = [P, I, P]
+    L.gsk_kernel_times.argtypes = [P, I, P, I, P, ctypes.c_char_p, I]
+    L.gsk_kernel_bytes.argtypes = [P, P, I, P]
+    L.gsk_noise_field.argtypes = [U, ctypes.c_uint, ctypes.c_uint, ctypes.c_longlong, P, I]
+    L.gsk_btensor_full.argtypes = [I, D, I, P, P, I, P, I, P, I]
+    L.gsk_fft_r2c.argtypes = [I, I, P, P, I]
+    L.gsk_fft_c2r.argtypes = [I, I, P, P, I]
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != GSK_OK:
+        raise GskError(rc, lib().gsk_last_error().decode())
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+# ----------------------------------------------------------------------------- helpers
+def cubic_modulus(c11: float, c12: float, c44: float) -> np.ndarray:
+    """C[i][j][k][l] of a cubic crystal (the reference's cijkl, src/dynamics_mart.cu:80-89)."""
+    C = np.zeros((3, 3, 3, 3))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                for l in range(3):
+                    if i == j and k == l:
+                        C[i, j, k, l] = c11 if i == k else c12
+                    elif (i == k and j == l) or (i == l and j == k):
+                        C[i, j, k, l] = c44
+    return C
+
+
+def mart_eigenstrains(nvar: int = 3, base: float = -0.04, axis: float = 0.12) -> np.ndarray:
+    """Variant p: diag(base, base, base) + axis * e_p e_p (cubic -> tetragonal)."""
+    e = np.zeros((nvar, 3, 3))
+    for p in range(nvar):
+        e[p] = np.eye(3) * base
+        e[p, p % 3, p % 3] += axis
+    return e
+
+
+def default_config(physics: int = PHYS_MART, **kw) -> Config:
+    c = Config()
+    _check(lib().gsk_config_default(ctypes.byref(c), physics))
+    for k, v in kw.items():
+        if k == "a":
+            for i, x in enumerate(v):
+                c.a[i] = x
+        else:
+            setattr(c, k, v)
+    return c
+
+
+class Sim:
+    """One phase-field simulation on one GPU (a gsk_sim handle)."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        _check(lib().gsk_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self.n, self.nvar, self.physics = cfg.n, cfg.nvar, cfg.physics
+
+    def close(self) -> None:
+        if self._h:
+            lib().gsk_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- setup
+    def build_btensor(self, C: np.ndarray, eps: np.ndarray) -> None:
+        C = np.ascontiguousarray(C, dtype=np.float64).reshape(81)
+        eps = np.ascontiguousarray(eps, dtype=np.float64).reshape(-1, 9)
+        _check(lib().gsk_build_btensor(self._h, _ptr(C), _ptr(eps), eps.shape[0]))
+
+    def _shape(self, name: str):
+        n = self.n
+        return (self.nvar, n, n, n) if name == "eta" else (n, n, n)
+
+    def set_field(self, name: str, a: np.ndarray) -> None:
+        a = np.asarray(a)
+        dt = F64 if a.dtype == np.float64 else F32
+        a = np.ascontiguousarray(a, dtype=np.float64 if dt == F64 else np.float32)
+        if a.size != int(np.prod(self._shape(name))):
+            raise ValueError(f"{name}: expected {self._shape(name)}, got {a.shape}")
+        _check(lib().gsk_set_field(self._h, name.encode(), _ptr(a), dt))
+
+    def get_field(self, name: str, dtype=np.float32) -> np.ndarray:
+        out = np.empty(self._shape(name), dtype=dtype)
+        _check(lib().gsk_get_field(self._h, name.encode(), _ptr(out), F64 if dtype == np.float64 else F32))
+        return out
+
+    def init_uniform(self, name: str, lo: float, hi: float, seed: int) -> None:
+        _check(lib().gsk_init_uniform(self._h, name.encode(), lo, hi, seed))
+
+    def set_param(self, key: str, value: float) -> None:
+        _check(lib().gsk_set_param(self._h, key.encode(), value))
+
+    # -- run
+    def step(self, nsteps: int = 1) -> None:
+        _check(lib().gsk_step(self._h, nsteps))
+
+    @property
+    def step_index(self) -> int:
+        return int(lib().gsk_get_step(self._h))
+
+    def set_step(self, s: int) -> None:
+        _check(lib().gsk_set_step(self._h, s))
+
+    def reduce(self, what: int) -> np.ndarray:
+        out = np.zeros(16)
+        _check(lib().gsk_reduce(self._h, what, _ptr(out)))
+        return out
+
+    def energy(self) -> np.ndarray:
+        """[chemical, gradient, elastic, total] in FP64."""
+        return self.reduce(RED_ENERGY)[:4].copy()
+
+    def mass(self) -> float:
+        return float(self.reduce(RED_MASS)[0])
+
+    def mean_eta(self) -> np.ndarray:
+        return self.reduce(RED_MEAN_ETA)[: self.nvar].copy()
+
+    def time_steps(self, nsteps: int) -> float:
+        ms = ctypes.c_double()
+        _check(lib().gsk_time_steps(self._h, nsteps, ctypes.byref(ms)))
+        return ms.value
+
+    def kernel_times(self, nsteps: int):
+        ms = np.zeros(128)
+        nk = ctypes.c_int()
+        names = ctypes.create_string_buffer(8192)
+        _check(lib().gsk_kernel_times(self._h, nsteps, _ptr(ms), 128, ctypes.byref(nk), names, 8192))
+        return names.value.decode().split(","), ms[: nk.value].copy()
+
+    def kernel_bytes(self) -> np.ndarray:
+        b = np.zeros(128)
+        nk = ctypes.c_int()
+        _check(lib().gsk_kernel_bytes(self._h, _ptr(b), 128, ctypes.byref(nk)))
+        return b[: nk.value].copy()
+
+
+# ----------------------------------------------------------------------------- free functions
+def noise_field(seed: int, step: int, field: int, nvox: int, device: int = 0) -> np.ndarray:
+    out = np.empty(nvox, dtype=np.float32)
+    _check(lib().gsk_noise_field(seed, step, field, nvox, _ptr(out), device))
+    return out
+
+
+def btensor_full(n: int, dx: float, kconv: int, C: np.ndarray, eps1: np.ndarray, eps2: Optional[np.ndarray] = None,
+                 device: int = 0) -> np.ndarray:
+    C = np.ascontiguousarray(C, dtype=np.float64).reshape(81)
+    e1 = np.ascontiguousarray(eps1, dtype=np.float64).reshape(-1, 9)
+    e2 = e1 if eps2 is None else np.ascontiguousarray(eps2, dtype=np.float64).reshape(-1, 9)
+    out = np.empty((e1.shape[0], e2.shape[0], n, n, n))
+    _check(lib().gsk_btensor_full(n, dx, kconv, _ptr(C), _ptr(e1), e1.shape[0], _ptr(e2), e2.shape[0], _ptr(out), device))
+    return out
+
+
+def fft_r2c(x: np.ndarray, device: int = 0) -> np.ndarray:
+    """Unnormalised 3D R2C of real fields x[count, n, n, n] -> complex64 [count, n, n, n//2+1]."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    count, n = x.shape[0], x.shape[1]
+    out = np.empty((count, n, n, n // 2 + 1), dtype=np.complex64)
+    _check(lib().gsk_fft_r2c(n, count, _ptr(x), _ptr(out), device))
+    return out
+
+
+def fft_c2r(X: np.ndarray, device: int = 0) -> np.ndarray:
+    """Unnormalised 3D C2R of half spectra X[count, n, n, n//2+1] -> float32 [count, n, n, n]."""
+    X = np.ascontiguousarray(X, dtype=np.complex64)
+    count, n = X.shape[0], X.shape[1]
+    out = np.empty((count, n, n, n), dtype=np.float32)
+    _check(lib().gsk_fft_c2r(n, count, _ptr(X), _ptr(out), device))
+    return out
+
+
+def version() -> str:
+    return lib().gsk_version().decode()

@@ -1,0 +1,132 @@
+/*
+ * gsk.h — C ABI of the MI355X phase-field time-step library (libgsk.so).
+ *
+ * The library owns device memory, HIP streams and kernels; callers pass plain
+ * host pointers and sizes. Every function returns an int status (GSK_OK = 0,
+ * negative on error, mirroring Code_* in src/pub.h:17-22 of General-Simulator)
+ * and records a message readable with gsk_last_error().
+ *
+ * Threading: a handle is used by one host thread at a time (the reference runs
+ * one host thread on the default stream); distinct handles may be used from
+ * distinct threads. Calls are synchronous at the boundary.
+ *
+ * Reference interfaces replaced (General-Simulator, file:line):
+ *   gsk_create            Dynamics_mart::Initialize / Dynamics_precipitate::Initialize
+ *                         (src/dynamics_mart.cu:14-173, src/dynamics_precipitate.cu:17-110)
+ *   gsk_build_btensor     GTensorB::InitB (src/gtensorb.cu:55-108)
+ *   gsk_set/get_field     Data<T>::HostToDevice / DeviceToHost (include/datamain.th:283-294)
+ *   gsk_init_uniform      initial random order parameters (Random::InitRandom, src/random.cu:52-63)
+ *   gsk_step              Dynamics_*::Calculate per step (src/dynamics_mart.cu:348-417,
+ *                         src/dynamics_precipitate.cu:278-289) driven by GS::Run (src/gs.cu:201-209)
+ *   gsk_reduce            Data<T>::TotalDevice used by Get() (include/datamain.th:782-806,
+ *                         src/dynamics_mart.cu:443-456), here in FP64
+ *   gsk_noise_field       Random::NewNormal_device (src/random.cu:79-82), Philox4x32-10 here
+ *   gsk_destroy           ~Dynamics_* (cufftDestroy, src/dynamics_mart.cu:177-179)
+ */
+#ifndef GSK_H
+#define GSK_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSK_OK 0
+#define GSK_ERR (-1)          /* generic failure (Code_ERR)             */
+#define GSK_ERR_ARG (-2)      /* invalid argument                       */
+#define GSK_ERR_HIP (-3)      /* HIP runtime error                      */
+#define GSK_ERR_STATE (-4)    /* call out of order (e.g. no B tensor)   */
+#define GSK_ERR_UNSUPPORTED (-5)
+
+/* physics */
+#define GSK_PHYS_MART 0   /* martensite: semi-implicit Allen-Cahn, Khachaturyan elasticity on eta^2 */
+#define GSK_PHYS_PREC 1   /* precipitate: Cahn-Hilliard c + Allen-Cahn eta, c-sourced elasticity    */
+/* execution path */
+#define GSK_PATH_FUSED32 0  /* FP32 fused spectral kernels (R2C, 3 axis passes per transform) */
+#define GSK_PATH_PLAIN64 1  /* FP64 plain path: one kernel per operation, C2C transforms       */
+/* wave-vector convention */
+#define GSK_K_EXACT 0       /* k = 2 pi m / (n dx)                                  */
+#define GSK_K_REFERENCE 1   /* k = 2 * 3.14 * dx * m / n  (src/gtensorb.cu:9-12)     */
+/* host data types */
+#define GSK_F32 0
+#define GSK_F64 1
+/* reductions */
+#define GSK_RED_ENERGY 1    /* out[0..3] = chemical, gradient, elastic, total */
+#define GSK_RED_MASS 2      /* out[0] = sum of c over voxels                  */
+#define GSK_RED_MEAN_ETA 3  /* out[v] = mean of eta_v, v < nvar              */
+#define GSK_RED_SUM_ETA2 4  /* out[0] = sum over voxels and variants of eta^2 */
+
+typedef struct gsk_sim gsk_sim;
+
+typedef struct gsk_config {
+  int n;                 /* grid edge (periodic n^3): 16..512, 2^a or 3*2^a               */
+  int nvar;              /* number of order parameters eta_v                              */
+  int physics;           /* GSK_PHYS_*                                                    */
+  int path;              /* GSK_PATH_*                                                    */
+  int kconv;             /* GSK_K_*                                                       */
+  int device;            /* HIP device ordinal                                            */
+  double dx;             /* grid spacing                                                  */
+  unsigned long long seed; /* Philox key for the noise                                    */
+  /* Allen-Cahn (both physics) */
+  double L, kappa, dt, noise;
+  /* martensite Landau f = A/2 sum eta^2 - B/3 sum eta^3 + C/4 (sum eta^2)^2 */
+  double A, B, C;
+  double w_el;           /* elastic weight                                                */
+  /* precipitate (src/dynamics_precipitate.cu:20-29): a[1..7], c1, c2, M, kappa_c, xi */
+  double a[8];
+  double c1, c2, M, kappa_c, xi;
+  /* Cahn-Hilliard stabilisation S: the term S k^2 c_hat is moved to the implicit
+   * side of the concentration update; S = 0 is the reference rule
+   * (src/dynamics_precipitate.cu:203-205). */
+  double S;
+} gsk_config;
+
+/* Fill cfg with the defaults of the given physics. */
+int gsk_config_default(gsk_config* cfg, int physics);
+
+int gsk_create(const gsk_config* cfg, gsk_sim** out);
+int gsk_destroy(gsk_sim* sim);
+
+/* Khachaturyan B tensor from elastic constants C[3][3][3][3] (row-major, 81
+ * doubles) and eigenstrains eps[neps][3][3]. Martensite: neps = nvar, B_pq acts
+ * on eta^2. Precipitate: neps = 1, the eigenstrain per unit concentration. */
+int gsk_build_btensor(gsk_sim* sim, const double* C81, const double* eps, int neps);
+
+/* Fields: "eta" (nvar*n^3) and "concentration" (n^3), layout [v][x][y][z]. */
+int gsk_set_field(gsk_sim* sim, const char* name, const void* host, int dtype);
+int gsk_get_field(gsk_sim* sim, const char* name, void* host, int dtype);
+/* Philox initial field lo + (hi-lo)*u, u = (x>>8)*2^-24, counter (voxel/4, field, 0, 1). */
+int gsk_init_uniform(gsk_sim* sim, const char* name, double lo, double hi, unsigned long long seed);
+
+int gsk_step(gsk_sim* sim, int nsteps);
+int gsk_set_step(gsk_sim* sim, long long step);
+long long gsk_get_step(const gsk_sim* sim);
+int gsk_set_param(gsk_sim* sim, const char* key, double value);
+
+int gsk_reduce(gsk_sim* sim, int what, double* out);
+
+/* Timing on the handle's stream with HIP events: ms over nsteps steps. */
+int gsk_time_steps(gsk_sim* sim, int nsteps, double* ms);
+/* Per-kernel timing of nsteps steps: fills up to maxk entries of ms (total ms
+ * per kernel slot) and names (comma separated into a caller buffer). */
+int gsk_kernel_times(gsk_sim* sim, int nsteps, double* ms, int maxk, int* nk, char* names, int names_len);
+/* Algorithmic HBM bytes per step of each kernel slot (same order as gsk_kernel_times). */
+int gsk_kernel_bytes(gsk_sim* sim, double* bytes, int maxk, int* nk);
+
+/* Philox normal field (device generated, copied to host), for bit-exact checks. */
+int gsk_noise_field(unsigned long long seed, unsigned int step, unsigned int field, long long nvox, float* host_out,
+                    int device);
+/* FP64 B tensor evaluated on the device in the full [p][q][x][y][z] layout. */
+int gsk_btensor_full(int n, double dx, int kconv, const double* C81, const double* eps1, int V1,
+                     const double* eps2, int V2, double* host_out, int device);
+/* 3D R2C / C2R of `count` real n^3 fields with the library's FP32 transforms.
+ * Spectrum layout [f][kx][ky][kz], kz in [0, n/2] (interleaved re, im floats). */
+int gsk_fft_r2c(int n, int count, const float* host_in, float* host_out, int device);
+int gsk_fft_c2r(int n, int count, const float* host_in, float* host_out, int device);
+
+const char* gsk_last_error(void);
+const char* gsk_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSK_H */

@@ -1,0 +1,570 @@
+// FP64 CPU oracle — TEST INFRASTRUCTURE ONLY (see oracle.h for scope and citations).
+// Built with -ffp-contract=off so that the fp32 normal transform rounds exactly
+// as written, operation by operation.
+#include "oracle.h"
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef std::complex<double> cd;
+static const double kPi = 3.14159265358979323846;
+
+static int g_threads = 0;
+extern "C" void or_set_threads(int nt) {
+  g_threads = nt;
+#ifdef _OPENMP
+  if (nt > 0) omp_set_num_threads(nt);
+#endif
+}
+extern "C" int or_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// FFT: recursive decimation-in-time Cooley-Tukey with radices 4, 2 and 3.
+// tw[j] = exp(sign*2*pi*i*j/N0) for the top-level length N0.
+// ---------------------------------------------------------------------------
+static void fft_rec(const cd* in, cd* out, int n, int s, const cd* tw, int N0) {
+  if (n == 1) { out[0] = in[0]; return; }
+  int p = (n % 4 == 0) ? 4 : (n % 2 == 0 ? 2 : (n % 3 == 0 ? 3 : 0));
+  if (p == 0) {  // plain DFT for any other factor (not used for supported sizes)
+    for (int k = 0; k < n; ++k) {
+      cd acc = 0;
+      for (int j = 0; j < n; ++j) acc += in[(int64_t)j * s] * tw[((int64_t)j * k % n) * (N0 / n)];
+      out[k] = acc;
+    }
+    return;
+  }
+  int m = n / p;
+  for (int r = 0; r < p; ++r) fft_rec(in + (int64_t)r * s, out + (int64_t)r * m, m, s * p, tw, N0);
+  int step = N0 / n;  // tw index of exp(sign*2*pi*i/n)
+  cd t[4];
+  for (int k = 0; k < m; ++k) {
+    for (int r = 0; r < p; ++r) t[r] = out[r * m + k] * tw[(int64_t)r * k * step % N0];
+    for (int q = 0; q < p; ++q) {
+      cd acc = 0;
+      for (int r = 0; r < p; ++r) acc += t[r] * tw[((int64_t)r * q * m * step) % N0];
+      out[k + q * m] = acc;
+    }
+  }
+}
+
+struct Plan1D {
+  int n;
+  std::vector<cd> tw;
+  Plan1D(int n_, int sign) : n(n_), tw(n_) {
+    for (int j = 0; j < n; ++j) {
+      double a = sign * 2.0 * kPi * (double)j / (double)n;
+      tw[j] = cd(std::cos(a), std::sin(a));
+    }
+  }
+  void run(cd* data, cd* scratch) const {
+    std::memcpy(scratch, data, sizeof(cd) * n);
+    fft_rec(scratch, data, n, 1, tw.data(), n);
+  }
+};
+
+extern "C" void or_fft1d(int n, double* data, int sign) {
+  Plan1D pl(n, sign);
+  std::vector<cd> s(n);
+  pl.run(reinterpret_cast<cd*>(data), s.data());
+}
+
+static void fft3d(int n, cd* a, int sign) {
+  Plan1D pl(n, sign);
+  const int64_t n2 = (int64_t)n * n;
+  // z lines (contiguous)
+#pragma omp parallel
+  {
+    std::vector<cd> s(n), line(n);
+#pragma omp for schedule(static)
+    for (int64_t xy = 0; xy < n2; ++xy) pl.run(a + xy * n, s.data());
+    // y lines: stride n
+#pragma omp for schedule(static)
+    for (int64_t xz = 0; xz < n2; ++xz) {
+      int64_t x = xz / n, z = xz % n;
+      cd* base = a + x * n2 + z;
+      for (int y = 0; y < n; ++y) line[y] = base[(int64_t)y * n];
+      pl.run(line.data(), s.data());
+      for (int y = 0; y < n; ++y) base[(int64_t)y * n] = line[y];
+    }
+    // x lines: stride n^2
+#pragma omp for schedule(static)
+    for (int64_t yz = 0; yz < n2; ++yz) {
+      cd* base = a + yz;
+      for (int x = 0; x < n; ++x) line[x] = base[(int64_t)x * n2];
+      pl.run(line.data(), s.data());
+      for (int x = 0; x < n; ++x) base[(int64_t)x * n2] = line[x];
+    }
+  }
+}
+
+extern "C" void or_fft3d(int n, double* data, int sign) { fft3d(n, reinterpret_cast<cd*>(data), sign); }
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of
+//   (hi0,lo0)=M0*c0, (hi1,lo1)=M1*c2, c=(hi1^c1^k0, lo1, hi0^c3^k1, lo0),
+// key bumped by the Weyl constants between rounds.
+// ---------------------------------------------------------------------------
+extern "C" void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// ---------------------------------------------------------------------------
+// fp32 Box-Muller built from + - * and integer bit operations only, so that any
+// IEEE-754 binary32 implementation with round-to-nearest-even and no operation
+// contraction produces the same bits. Constants are given as bit patterns.
+// ---------------------------------------------------------------------------
+static inline float f_of(uint32_t b) { float f; std::memcpy(&f, &b, 4); return f; }
+static inline uint32_t b_of(float f) { uint32_t b; std::memcpy(&b, &f, 4); return b; }
+
+extern "C" float or_log_f(float u) {
+  // u in [2^-24, 1]; ln(u) = e*ln2 + ln(1+f), m = 1+f in [sqrt(1/2), sqrt(2)]
+  uint32_t i = b_of(u);
+  int e = (int)((i >> 23) & 0xFFu) - 127;
+  float m = f_of((i & 0x007FFFFFu) | 0x3F800000u);
+  if (m > f_of(0x3FB504F3u)) { m = m * 0.5f; e = e + 1; }
+  float f = m - 1.0f;
+  // ln(1+f) = f * q(f), q of degree 9 (least-squares fit on [sqrt(1/2)-1, sqrt(2)-1])
+  float q = f_of(0xBD9C70BBu);
+  q = q * f + f_of(0x3E043E25u);
+  q = q * f + f_of(0xBE07962Fu);
+  q = q * f + f_of(0x3E11342Cu);
+  q = q * f + f_of(0xBE2A13CFu);
+  q = q * f + f_of(0x3E4CD23Cu);
+  q = q * f + f_of(0xBE800214u);
+  q = q * f + f_of(0x3EAAAAA8u);
+  q = q * f + f_of(0xBEFFFFFCu);
+  q = q * f + f_of(0x3F800000u);
+  float p = f * q;
+  float t = (float)e;
+  return t * f_of(0x3F317200u) + (p + t * f_of(0x35BFBE8Eu));
+}
+
+extern "C" float or_sqrt_f(float x) {
+  if (!(x > 0.0f)) return 0.0f;
+  uint32_t i = b_of(x);
+  float y = f_of(0x5F3759DFu - (i >> 1));
+  float h = 0.5f * x;
+  for (int it = 0; it < 3; ++it) {
+    float t = y * y;
+    t = h * t;
+    t = 1.5f - t;
+    y = y * t;
+  }
+  return x * y;
+}
+
+extern "C" void or_sincos_f(float th, float* s, float* c) {
+  // theta in [0, pi/2): Taylor series to theta^13 (sin) and theta^14 (cos)
+  float z = th * th;
+  float ps = f_of(0x2F309231u);
+  ps = ps * z + f_of(0xB2D7322Bu);
+  ps = ps * z + f_of(0x3638EF1Du);
+  ps = ps * z + f_of(0xB9500D01u);
+  ps = ps * z + f_of(0x3C088889u);
+  ps = ps * z + f_of(0xBE2AAAABu);
+  ps = ps * z + 1.0f;
+  *s = th * ps;
+  float pc = f_of(0xAD49CBA5u);
+  pc = pc * z + f_of(0x310F76C7u);
+  pc = pc * z + f_of(0xB493F27Eu);
+  pc = pc * z + f_of(0x37D00D01u);
+  pc = pc * z + f_of(0xBAB60B61u);
+  pc = pc * z + f_of(0x3D2AAAABu);
+  pc = pc * z + f_of(0xBF000000u);
+  *c = pc * z + 1.0f;
+}
+
+static void normal_pair(uint32_t a, uint32_t b, float* z0, float* z1) {
+  float u1 = (float)((a >> 8) + 1u) * f_of(0x33800000u);  // * 2^-24, in (0,1]
+  float L = or_log_f(u1);
+  float r = or_sqrt_f(-2.0f * L);
+  uint32_t q = b >> 30;
+  float fr = (float)((b >> 8) & 0x3FFFFFu) * f_of(0x34800000u);  // * 2^-22, in [0,1)
+  float th = fr * f_of(0x3FC90FDBu);                                // * pi/2
+  float s, c;
+  or_sincos_f(th, &s, &c);
+  float cc, ss;
+  switch (q) {
+    case 0: cc = c; ss = s; break;
+    case 1: cc = -s; ss = c; break;
+    case 2: cc = -c; ss = -s; break;
+    default: cc = s; ss = -c; break;
+  }
+  *z0 = r * cc;
+  *z1 = r * ss;
+}
+
+extern "C" void or_normal4(const uint32_t x[4], float out[4]) {
+  normal_pair(x[0], x[1], &out[0], &out[1]);
+  normal_pair(x[2], x[3], &out[2], &out[3]);
+}
+
+// counter = (voxel/4, field, step, purpose); key = (seed low, seed high)
+extern "C" void or_noise_field(uint64_t seed, uint32_t step, uint32_t field, int64_t nvox, float* out) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+#pragma omp parallel for schedule(static)
+  for (int64_t g = 0; g < nvox / 4; ++g) {
+    uint32_t ctr[4] = {(uint32_t)g, field, step, 0u}, x[4];
+    or_philox4x32_10(ctr, key, x);
+    or_normal4(x, out + 4 * g);
+  }
+}
+
+extern "C" void or_uniform_field(uint64_t seed, uint32_t field, int64_t nvox, double lo, double hi, float* out) {
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+#pragma omp parallel for schedule(static)
+  for (int64_t g = 0; g < nvox / 4; ++g) {
+    uint32_t ctr[4] = {(uint32_t)g, field, 0u, 1u}, x[4];
+    or_philox4x32_10(ctr, key, x);
+    for (int i = 0; i < 4; ++i) {
+      double u = (double)(x[i] >> 8) * (1.0 / 16777216.0);
+      out[4 * g + i] = (float)(lo + (hi - lo) * u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wave vectors and the Khachaturyan tensor (src/gtensorb.cu:9-34, 110-182)
+// ---------------------------------------------------------------------------
+extern "C" double or_kcomp(int m, int n, double dx, int kconv) {
+  int t = (m < n / 2) ? m : m - n;  // src/gtensorb.cu:10
+  if (kconv == 1) return 2.0 * 3.14 * dx / n * t;  // src/gtensorb.cu:11
+  return 2.0 * kPi * t / (n * dx);
+}
+
+static void inv3(const double a[9], double o[9]) {
+  double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+               a[2] * (a[3] * a[7] - a[4] * a[6]);
+  if (std::fabs(det) < 1e-8) {  // src/gtensorb.cu:145,155-156
+    for (int i = 0; i < 9; ++i) o[i] = 0.0;
+    return;
+  }
+  o[0] = (a[4] * a[8] - a[5] * a[7]) / det;
+  o[1] = (a[2] * a[7] - a[1] * a[8]) / det;
+  o[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+  o[3] = (a[5] * a[6] - a[3] * a[8]) / det;
+  o[4] = (a[0] * a[8] - a[2] * a[6]) / det;
+  o[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+  o[6] = (a[3] * a[7] - a[4] * a[6]) / det;
+  o[7] = (a[1] * a[6] - a[0] * a[7]) / det;
+  o[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+}
+
+extern "C" void or_btensor(int n, double dx, int kconv, const double C[81], const double* eps1, int V1,
+                           const double* eps2, int V2, double* B) {
+  const int64_t N = (int64_t)n * n * n;
+  // sigma = C : eps  (src/gtensorb.cu:72-84)
+  std::vector<double> s1(V1 * 9, 0.0), s2(V2 * 9, 0.0), cee(V1 * V2, 0.0);
+  for (int p = 0; p < V1; ++p)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) s1[p * 9 + i * 3 + j] += C[((i * 3 + j) * 3 + k) * 3 + l] * eps1[p * 9 + k * 3 + l];
+  for (int q = 0; q < V2; ++q)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) s2[q * 9 + i * 3 + j] += C[((i * 3 + j) * 3 + k) * 3 + l] * eps2[q * 9 + k * 3 + l];
+  for (int p = 0; p < V1; ++p)
+    for (int q = 0; q < V2; ++q) {
+      double acc = 0;
+      for (int ij = 0; ij < 9; ++ij)
+        for (int kl = 0; kl < 9; ++kl) acc += C[ij * 9 + kl] * eps1[p * 9 + ij] * eps2[q * 9 + kl];
+      cee[p * V2 + q] = acc;
+    }
+#pragma omp parallel for schedule(static)
+  for (int64_t idx = 0; idx < N; ++idx) {
+    int x = (int)(idx / ((int64_t)n * n)), y = (int)((idx / n) % n), z = (int)(idx % n);
+    double g[3] = {or_kcomp(x, n, dx, kconv), or_kcomp(y, n, dx, kconv), or_kcomp(z, n, dx, kconv)};
+    double mo = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    double u[3] = {0, 0, 0};
+    if (mo > 5e-7) { u[0] = g[0] / mo; u[1] = g[1] / mo; u[2] = g[2] / mo; }  // src/gtensorb.cu:25-33
+    double om_inv[9], om[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double acc = 0;
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) acc += C[((i * 3 + k) * 3 + l) * 3 + j] * u[k] * u[l];  // :127-130
+        om_inv[i * 3 + j] = acc;
+      }
+    inv3(om_inv, om);
+    for (int p = 0; p < V1; ++p) {
+      double a[3];  // a_j = n_i sigma^p_ij
+      for (int j = 0; j < 3; ++j) a[j] = u[0] * s1[p * 9 + j] + u[1] * s1[p * 9 + 3 + j] + u[2] * s1[p * 9 + 6 + j];
+      for (int q = 0; q < V2; ++q) {
+        double b[3];  // b_k = sigma^q_kl n_l
+        for (int k = 0; k < 3; ++k) b[k] = s2[q * 9 + k * 3] * u[0] + s2[q * 9 + k * 3 + 1] * u[1] + s2[q * 9 + k * 3 + 2] * u[2];
+        double t = 0;
+        for (int j = 0; j < 3; ++j)
+          for (int k = 0; k < 3; ++k) t += a[j] * om[j * 3 + k] * b[k];
+        B[(int64_t)(p * V2 + q) * N + idx] = cee[p * V2 + q] - t;  // :168-178
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Martensite: semi-implicit spectral Allen-Cahn with Khachaturyan elasticity
+// ---------------------------------------------------------------------------
+static std::vector<double> ksq_table(int n, double dx, int kconv) {
+  std::vector<double> k(n);
+  for (int m = 0; m < n; ++m) { double v = or_kcomp(m, n, dx, kconv); k[m] = v * v; }
+  return k;
+}
+
+// R_p = IFFT[ sum_q B_pq FFT(eta_q^2) ] / N   (src/dynamics_mart.cu:283-289, HEAD: all q)
+static void elastic_R(int n, int V, const double* B, const double* eta, std::vector<cd>& R) {
+  const int64_t N = (int64_t)n * n * n;
+  std::vector<cd> S((size_t)V * N);
+  for (int q = 0; q < V; ++q) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) { double e = eta[q * N + i]; S[q * N + i] = cd(e * e, 0.0); }
+    fft3d(n, S.data() + q * N, -1);
+  }
+  R.assign((size_t)V * N, cd(0, 0));
+  for (int p = 0; p < V; ++p) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      cd acc = 0;
+      for (int q = 0; q < V; ++q) acc += B[(int64_t)(p * V + q) * N + i] * S[q * N + i];
+      R[p * N + i] = acc / (double)N;
+    }
+    fft3d(n, R.data() + p * N, +1);
+  }
+}
+
+extern "C" void or_mart_force(int n, int V, const or_mart_params* p, const double* B, const double* eta, double* F) {
+  const int64_t N = (int64_t)n * n * n;
+  std::vector<cd> R;
+  elastic_R(n, V, B, eta, R);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < N; ++i) {
+    double s2 = 0;
+    for (int v = 0; v < V; ++v) s2 += eta[v * N + i] * eta[v * N + i];
+    for (int v = 0; v < V; ++v) {
+      double e = eta[v * N + i];
+      F[v * N + i] = p->A * e - p->B * e * e + p->C * e * s2 + 2.0 * p->w_el * e * R[v * N + i].real();
+    }
+  }
+}
+
+extern "C" void or_mart_run(int n, int V, const or_mart_params* p, const double* B, double* eta, int step0,
+                            int nsteps) {
+  const int64_t N = (int64_t)n * n * n;
+  std::vector<double> F((size_t)V * N), k2 = ksq_table(n, p->dx, p->kconv);
+  std::vector<float> xi(N);
+  std::vector<cd> eh(N), fh(N);
+  for (int s = 0; s < nsteps; ++s) {
+    or_mart_force(n, V, p, B, eta, F.data());
+    for (int v = 0; v < V; ++v) {
+      if (p->noise != 0.0) {
+        or_noise_field(p->seed, (uint32_t)(step0 + s), (uint32_t)v, N, xi.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; ++i) F[v * N + i] += p->noise * (double)xi[i];
+      }
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < N; ++i) { eh[i] = cd(eta[v * N + i], 0); fh[i] = cd(F[v * N + i], 0); }
+      fft3d(n, eh.data(), -1);
+      fft3d(n, fh.data(), -1);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < N; ++i) {
+        int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
+        double kk = k2[x] + k2[y] + k2[z];
+        eh[i] = (eh[i] - p->dt * p->L * fh[i]) / (1.0 + p->dt * p->L * p->kappa * kk) / (double)N;
+      }
+      fft3d(n, eh.data(), +1);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < N; ++i) eta[v * N + i] = eh[i].real();
+    }
+  }
+}
+
+extern "C" void or_mart_energy(int n, int V, const or_mart_params* p, const double* B, const double* eta,
+                               double out[4]) {
+  const int64_t N = (int64_t)n * n * n;
+  double ech = 0, egr = 0, eel = 0;
+  std::vector<cd> R;
+  elastic_R(n, V, B, eta, R);
+#pragma omp parallel for reduction(+ : ech, eel) schedule(static)
+  for (int64_t i = 0; i < N; ++i) {
+    double s2 = 0, s3 = 0, el = 0;
+    for (int v = 0; v < V; ++v) {
+      double e = eta[v * N + i];
+      s2 += e * e; s3 += e * e * e;
+      el += e * e * R[v * N + i].real();
+    }
+    ech += 0.5 * p->A * s2 - p->B / 3.0 * s3 + 0.25 * p->C * s2 * s2;
+    eel += 0.5 * p->w_el * el;
+  }
+  std::vector<double> k2 = ksq_table(n, p->dx, p->kconv);
+  std::vector<cd> eh(N);
+  for (int v = 0; v < V; ++v) {
+    for (int64_t i = 0; i < N; ++i) eh[i] = cd(eta[v * N + i], 0);
+    fft3d(n, eh.data(), -1);
+    double acc = 0;
+#pragma omp parallel for reduction(+ : acc) schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
+      acc += (k2[x] + k2[y] + k2[z]) * std::norm(eh[i]);
+    }
+    egr += 0.5 * p->kappa * acc / (double)N;
+  }
+  out[0] = ech; out[1] = egr; out[2] = eel; out[3] = ech + egr + eel;
+}
+
+// ---------------------------------------------------------------------------
+// Precipitate: CH for c (k^4 implicit) + AC for eta (k^2 implicit)
+//   mu_c (src/dynamics_precipitate.cu:118-127), F_v (:136-162),
+//   c update (:197-206), eta update (:234-248).
+// ---------------------------------------------------------------------------
+extern "C" void or_prec_local(int V, const or_prec_params* p, double c, const double* eta, double* mu_c, double* F,
+                              double* f) {
+  const double* A = p->A;
+  double s2 = 0;
+  for (int v = 0; v < V; ++v) s2 += eta[v] * eta[v];
+  if (mu_c) *mu_c = 0.5 * A[2] * s2 + A[1] * (c - p->c1);
+  if (F) {
+    for (int v = 0; v < V; ++v) {
+      double e = eta[v], t2 = 0, t4 = 0, t22 = 0;
+      for (int i = 0; i < V; ++i)
+        if (i != v) { double q = eta[i] * eta[i]; t2 += q; t4 += q * q; }
+      for (int i = 0; i < V; ++i)
+        for (int j = 0; j < V; ++j)
+          if (v != i && v != j && i != j) t22 += eta[i] * eta[i] * eta[j] * eta[j];
+      F[v] = (2.0 * A[5] * e + 4.0 * A[6] * e * e * e) * t2 + 2.0 * A[6] * e * t4 + 2.0 * A[7] * e * t22 +
+             A[2] * e * (c - p->c2) - A[3] * e * e * e + A[4] * e * e * e * e * e;
+    }
+  }
+  if (f) {
+    double v0 = 0.5 * A[1] * (c - p->c1) * (c - p->c1) + 0.5 * A[2] * (c - p->c2) * s2;
+    for (int v = 0; v < V; ++v) {
+      double q = eta[v] * eta[v];
+      v0 += -0.25 * A[3] * q * q + A[4] / 6.0 * q * q * q;
+    }
+    for (int i = 0; i < V; ++i)
+      for (int j = i + 1; j < V; ++j) {
+        double a = eta[i] * eta[i], b = eta[j] * eta[j];
+        v0 += A[5] * a * b + A[6] * (a * a * b + a * b * b);
+        for (int k = j + 1; k < V; ++k) v0 += 2.0 * A[7] * a * b * eta[k] * eta[k];
+      }
+    *f = v0;
+  }
+}
+
+extern "C" void or_prec_run(int n, int V, const or_prec_params* p, const double* Bc, double* c, double* eta,
+                            int step0, int nsteps) {
+  const int64_t N = (int64_t)n * n * n;
+  std::vector<double> k2 = ksq_table(n, p->dx, p->kconv), mu(N), F((size_t)V * N);
+  std::vector<float> xi(N);
+  std::vector<cd> ch(N), mh(N), eh(N), fh(N);
+  for (int s = 0; s < nsteps; ++s) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      double e[16], Fl[16];
+      for (int v = 0; v < V; ++v) e[v] = eta[v * N + i];
+      or_prec_local(V, p, c[i], e, &mu[i], Fl, nullptr);
+      for (int v = 0; v < V; ++v) F[v * N + i] = Fl[v];
+    }
+    // concentration
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) { ch[i] = cd(c[i], 0); mh[i] = cd(mu[i], 0); }
+    fft3d(n, ch.data(), -1);
+    fft3d(n, mh.data(), -1);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
+      double kk = k2[x] + k2[y] + k2[z];
+      cd mt = mh[i] + (p->xi * Bc[i] - p->S) * ch[i];
+      ch[i] = (ch[i] - p->dt * p->M * kk * mt) / (1.0 + p->dt * p->M * (p->S * kk + p->kappa_c * kk * kk)) /
+              (double)N;
+    }
+    fft3d(n, ch.data(), +1);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) c[i] = ch[i].real();
+    // order parameters
+    for (int v = 0; v < V; ++v) {
+      if (p->noise != 0.0) {
+        or_noise_field(p->seed, (uint32_t)(step0 + s), (uint32_t)v, N, xi.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; ++i) F[v * N + i] += p->noise * (double)xi[i];
+      }
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < N; ++i) { eh[i] = cd(eta[v * N + i], 0); fh[i] = cd(F[v * N + i], 0); }
+      fft3d(n, eh.data(), -1);
+      fft3d(n, fh.data(), -1);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < N; ++i) {
+        int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
+        double kk = k2[x] + k2[y] + k2[z];
+        eh[i] = (eh[i] - p->dt * p->L * fh[i]) / (1.0 + p->dt * p->L * p->kappa_eta * kk) / (double)N;
+      }
+      fft3d(n, eh.data(), +1);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < N; ++i) eta[v * N + i] = eh[i].real();
+    }
+  }
+}
+
+extern "C" void or_prec_energy(int n, int V, const or_prec_params* p, const double* Bc, const double* c,
+                               const double* eta, double out[4]) {
+  const int64_t N = (int64_t)n * n * n;
+  double ech = 0;
+#pragma omp parallel for reduction(+ : ech) schedule(static)
+  for (int64_t i = 0; i < N; ++i) {
+    double e[16], f;
+    for (int v = 0; v < V; ++v) e[v] = eta[v * N + i];
+    or_prec_local(V, p, c[i], e, nullptr, nullptr, &f);
+    ech += f;
+  }
+  std::vector<double> k2 = ksq_table(n, p->dx, p->kconv);
+  std::vector<cd> h(N);
+  double egr = 0, eel = 0;
+  for (int v = -1; v < V; ++v) {
+    const double* src = (v < 0) ? c : eta + (int64_t)v * N;
+    for (int64_t i = 0; i < N; ++i) h[i] = cd(src[i], 0);
+    fft3d(n, h.data(), -1);
+    double acc = 0, accel = 0;
+#pragma omp parallel for reduction(+ : acc, accel) schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
+      double kk = k2[x] + k2[y] + k2[z], a2 = std::norm(h[i]);
+      acc += kk * a2;
+      if (v < 0) accel += Bc[i] * a2;
+    }
+    egr += 0.5 * (v < 0 ? p->kappa_c : p->kappa_eta) * acc / (double)N;
+    if (v < 0) eel = 0.5 * p->xi * accel / (double)N;
+  }
+  out[0] = ech; out[1] = egr; out[2] = eel; out[3] = ech + egr + eel;
+}
+
+extern "C" double or_sum(const double* a, int64_t n) {
+  double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (int64_t i = 0; i < n; ++i) s += a[i];
+  return s;
+}

@@ -1,0 +1,286 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Bars: bit-exact for the Philox noise and the initial fields; 1e-12 relative for
+the FP64 paths (B tensor, plain step); FP32 paths within the tolerances written
+in each test (single-precision FFT rounding, ~1e-6 relative per transform).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gsk
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def cubic(c11, c12, c44):
+    return gsk.cubic_modulus(c11, c12, c44)
+
+
+C_CFG0 = None
+
+
+def c_cfg0():
+    # C11/C12/C44 = 350/150/100 scaled by 1/500
+    return cubic(350 / 500, 150 / 500, 100 / 500)
+
+
+def mart_sim(n, V=3, path=gsk.PATH_FUSED32, **kw):
+    cfg = gsk.default_config(gsk.PHYS_MART, n=n, nvar=V, path=path, **kw)
+    s = gsk.Sim(cfg)
+    s.build_btensor(c_cfg0(), gsk.mart_eigenstrains(V))
+    return s
+
+
+def oracle_mart_params(cfg):
+    return O.mart_params(A=cfg.A, B=cfg.B, C=cfg.C, kappa=cfg.kappa, L=cfg.L, dt=cfg.dt, w_el=cfg.w_el,
+                         noise=cfg.noise, dx=cfg.dx, kconv=cfg.kconv, seed=cfg.seed)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+# ------------------------------------------------------------------ noise and initial fields
+@pytest.mark.parametrize("seed,step,field", [(0, 0, 0), (7, 3, 1), (2**40 + 5, 123456, 2), (0xFFFFFFFF, 1, 7)])
+def test_noise_bit_exact(gpu, seed, step, field):
+    say(f"noise seed={seed} step={step} field={field}")
+    nv = 1 << 20
+    g = gsk.noise_field(seed, step, field, nv)
+    c = O.noise_field(seed, step, field, nv)
+    assert np.array_equal(g.view(np.uint32), c.view(np.uint32))
+
+
+def test_uniform_init_bit_exact(gpu):
+    say("init_uniform mart n=32")
+    n, V = 32, 3
+    s = mart_sim(n, V)
+    s.init_uniform("eta", -0.01, 0.01, 0)
+    got = s.get_field("eta")
+    for v in range(V):
+        want = O.uniform_field(0, v, n**3, -0.01, 0.01).reshape(n, n, n)
+        assert np.array_equal(got[v], want)
+    s.close()
+
+
+# ------------------------------------------------------------------ B tensor (FP64 on the device)
+@pytest.mark.parametrize("kconv,dx", [(0, 1.0), (1, 1.0), (0, 0.5)])
+def test_btensor_fp64_matches_oracle(gpu, kconv, dx):
+    say(f"btensor kconv={kconv} dx={dx}")
+    n = 16
+    C = cubic(3.5, 1.5, 1.0)
+    e = gsk.mart_eigenstrains(4)
+    g = gsk.btensor_full(n, dx, kconv, C, e)
+    o = O.btensor(n, C, e, dx=dx, kconv=kconv)
+    np.testing.assert_allclose(g, o, rtol=0, atol=1e-14)
+    # non-square (6 unit base strains x V), as the stress plugin uses
+    base = np.zeros((6, 3, 3))
+    for i, (a, b) in enumerate([(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]):
+        base[i, a, b] = base[i, b, a] = 1.0
+    g2 = gsk.btensor_full(n, dx, kconv, C, base, e)
+    o2 = O.btensor(n, C, base, e, dx=dx, kconv=kconv)
+    np.testing.assert_allclose(g2, o2, rtol=0, atol=1e-13)
+
+
+# ------------------------------------------------------------------ FFT
+@pytest.mark.parametrize("n", [16, 32, 64, 96, 128, 192, 256])
+def test_fft_r2c_c2r_vs_numpy(gpu, n):
+    say(f"fft n={n}")
+    rng = np.random.default_rng(n)
+    cnt = 2 if n <= 128 else 1
+    x = rng.standard_normal((cnt, n, n, n)).astype(np.float32)
+    X = gsk.fft_r2c(x)
+    want = np.fft.rfftn(x.astype(np.float64), axes=(1, 2, 3))
+    err = np.abs(X - want).max() / np.abs(want).max()
+    assert err < 2e-6, err
+    y = gsk.fft_c2r(X) / n**3
+    assert np.abs(y - x).max() < 2e-5
+    # C2R of an exact spectrum (Hermitian-consistent input)
+    z = gsk.fft_c2r(want.astype(np.complex64)) / n**3
+    assert np.abs(z - x).max() < 2e-5
+
+
+def test_fft_512_roundtrip_and_impulse(gpu):
+    say("fft n=512")
+    n = 512
+    x = np.zeros((1, n, n, n), np.float32)
+    x[0, 3, 5, 7] = 1.0
+    X = gsk.fft_r2c(x)
+    k = np.arange(n)
+    # impulse -> plane wave, checked on a few slices
+    for kx in (0, 1, 255, 256, 511):
+        ph = np.exp(-2j * np.pi * (3 * kx + 5 * k[:, None] + 7 * np.arange(n // 2 + 1)[None, :]) / n)
+        assert np.abs(X[0, kx] - ph).max() < 5e-6
+    y = gsk.fft_c2r(X) / n**3
+    assert np.abs(y - x).max() < 1e-6
+
+
+# ------------------------------------------------------------------ martensite
+def run_mart_pair(n, V, steps, path, **kw):
+    s = mart_sim(n, V, path=path, **kw)
+    s.init_uniform("eta", -0.01, 0.01, 0)
+    eta0 = s.get_field("eta").astype(np.float64)
+    s.step(steps)
+    got = s.get_field("eta", np.float64)
+    E = s.energy()
+    cfg = s.cfg
+    s.close()
+    p = oracle_mart_params(cfg)
+    B = O.btensor(n, c_cfg0(), gsk.mart_eigenstrains(V))
+    want = O.mart_run(eta0, B, p, steps)
+    Ew = O.mart_energy(want, B, p)
+    return got, want, E, Ew
+
+
+@pytest.mark.parametrize("n,steps", [(16, 50), (32, 200)])
+def test_mart_fused32_matches_oracle(gpu, n, steps):
+    say(f"mart fused n={n} steps={steps}")
+    got, want, E, Ew = run_mart_pair(n, 3, steps, gsk.PATH_FUSED32)
+    scale = np.abs(want).max()
+    assert np.abs(got - want).max() / scale < 2e-5
+    assert rel_l2(got, want) < 2e-5
+    np.testing.assert_allclose(E, Ew, rtol=1e-4, atol=1e-9 * abs(Ew[3]))
+
+
+@pytest.mark.parametrize("n,steps", [(16, 50), (32, 100)])
+def test_mart_plain64_matches_oracle(gpu, n, steps):
+    say(f"mart plain64 n={n} steps={steps}")
+    got, want, E, Ew = run_mart_pair(n, 3, steps, gsk.PATH_PLAIN64)
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-11
+    np.testing.assert_allclose(E, Ew, rtol=1e-10)
+
+
+def test_mart_strong_dynamics_with_noise(gpu):
+    """Martensite below the transition (A < 0) with noise: nonlinear and elastic terms matter."""
+    say("mart fused n=32 A=-0.5 noise")
+    n, V, steps = 32, 3, 100
+    kw = dict(A=-0.5, B=3.0, C=3.0, noise=0.002, seed=9, w_el=20.0)
+    for path, tol in ((gsk.PATH_FUSED32, 5e-5), (gsk.PATH_PLAIN64, 1e-10)):
+        got, want, E, Ew = run_mart_pair(n, V, steps, path, **kw)
+        assert np.abs(got - want).max() / np.abs(want).max() < tol, path
+        np.testing.assert_allclose(E, Ew, rtol=max(tol * 10, 1e-10), atol=1e-9 * abs(Ew[3]))
+
+
+def test_mart_linear_mode_at_256(gpu):
+    """Full-size property: with B = C = 0 and no elasticity, a single mode decays by
+    ((1 - dt L A)/(1 + dt L kappa k^2))^steps exactly."""
+    say("mart linear mode n=256")
+    n, steps = 256, 20
+    s = mart_sim(n, 1, w_el=0.0, B=0.0, C=0.0, A=0.3, kappa=1.5, L=0.7, dt=0.05)
+    x = np.arange(n, dtype=np.float64)
+    m = (5, 17, 100)
+    wave = 1e-3 * np.cos(2 * np.pi * (m[0] * x[:, None, None] + m[1] * x[None, :, None] + m[2] * x[None, None, :]) / n)
+    s.set_field("eta", wave[None].astype(np.float32))
+    s.step(steps)
+    got = s.get_field("eta", np.float64)[0]
+    s.close()
+    k2 = sum((2 * np.pi * mi / n) ** 2 for mi in m)
+    g = (1 - 0.05 * 0.7 * 0.3) / (1 + 0.05 * 0.7 * 1.5 * k2)
+    assert np.abs(got - wave * g**steps).max() < 1e-3 * 1e-4
+
+
+def test_mart_translation_equivariance_at_256(gpu):
+    say("mart translation n=256")
+    n, V = 256, 3
+    s = mart_sim(n, V, A=-0.5, w_el=20.0)
+    s.init_uniform("eta", -0.05, 0.05, 3)
+    e0 = s.get_field("eta")
+    s.step(5)
+    a = s.get_field("eta")
+    sh = (17, 3, 101)
+    s.set_field("eta", np.roll(e0, sh, axis=(1, 2, 3)))
+    s.set_step(0)
+    s.step(5)
+    b = s.get_field("eta")
+    s.close()
+    assert np.abs(np.roll(a, sh, axis=(1, 2, 3)) - b).max() < 1e-5 * np.abs(a).max()
+
+
+# ------------------------------------------------------------------ precipitate
+def prec_sim(n, V=4, **kw):
+    base = dict(n=n, nvar=V, M=1.0, L=1.0, kappa=1.5, kappa_c=1.5, dt=0.005, noise=0.001, seed=5, S=54.0)
+    base.update(kw)
+    cfg = gsk.default_config(gsk.PHYS_PREC, **base)
+    s = gsk.Sim(cfg)
+    s.build_btensor(cubic(3.5, 1.5, 1.0), 0.02 * np.eye(3)[None])
+    return s
+
+
+def oracle_prec_params(cfg):
+    return O.prec_params(A=list(cfg.a), c1=cfg.c1, c2=cfg.c2, M=cfg.M, L=cfg.L, kappa_c=cfg.kappa_c,
+                         kappa_eta=cfg.kappa, xi=cfg.xi, S=cfg.S, dt=cfg.dt, noise=cfg.noise, dx=cfg.dx,
+                         kconv=cfg.kconv, seed=cfg.seed)
+
+
+def prec_init(s, seed=11):
+    s.init_uniform("concentration", 0.245, 0.255, seed)
+    s.init_uniform("eta", 0.0, 0.01, seed + 1)
+
+
+@pytest.mark.parametrize("n,steps", [(16, 50), (32, 200)])
+def test_prec_fused32_matches_oracle(gpu, n, steps):
+    say(f"prec fused n={n} steps={steps}")
+    s = prec_sim(n)
+    prec_init(s)
+    c0 = s.get_field("concentration", np.float64)
+    e0 = s.get_field("eta", np.float64)
+    m0 = s.mass()
+    s.step(steps)
+    c1 = s.get_field("concentration", np.float64)
+    e1 = s.get_field("eta", np.float64)
+    E = s.energy()
+    m1 = s.mass()
+    cfg = s.cfg
+    s.close()
+    p = oracle_prec_params(cfg)
+    Bc = O.btensor(n, cubic(3.5, 1.5, 1.0), 0.02 * np.eye(3)[None])[0, 0]
+    cw, ew = O.prec_run(c0, e0, Bc, p, steps)
+    assert np.abs(c1 - cw).max() < 2e-6
+    assert np.abs(e1 - ew).max() / np.abs(ew).max() < 2e-5
+    assert abs(m1 - m0) / abs(m0) < 1e-12
+    assert abs(m0 - c0.sum()) / c0.sum() < 1e-12
+    Ew = O.prec_energy(cw, ew, Bc, p)
+    np.testing.assert_allclose(E, Ew, rtol=1e-5, atol=1e-9 * abs(Ew[3]))
+
+
+def test_prec_mass_conservation_at_256(gpu):
+    say("prec mass n=256")
+    s = prec_sim(256)
+    prec_init(s)
+    m0 = s.mass()
+    s.step(40)
+    m1 = s.mass()
+    e = s.get_field("eta")
+    s.close()
+    assert np.isfinite(e).all()
+    assert abs(m1 - m0) / m0 < 1e-12
+
+
+def test_resume_is_bit_exact(gpu):
+    """Counter-based noise: stopping, copying the fields out and back, and resuming at the
+    same step index reproduces an uninterrupted run bit for bit (FP64 plain path)."""
+    say("mart plain64 resume n=32")
+    n, V = 32, 3
+    a = mart_sim(n, V, path=gsk.PATH_PLAIN64, noise=0.002, seed=4, A=-0.5)
+    a.init_uniform("eta", -0.01, 0.01, 0)
+    e0 = a.get_field("eta", np.float64)
+    a.step(10)
+    full = a.get_field("eta", np.float64)
+    b = mart_sim(n, V, path=gsk.PATH_PLAIN64, noise=0.002, seed=4, A=-0.5)
+    b.set_field("eta", e0)
+    b.step(6)
+    mid = b.get_field("eta", np.float64)
+    c = mart_sim(n, V, path=gsk.PATH_PLAIN64, noise=0.002, seed=4, A=-0.5)
+    c.set_field("eta", mid)
+    c.set_step(6)
+    c.step(4)
+    assert np.array_equal(c.get_field("eta", np.float64), full)
+    for x in (a, b, c):
+        x.close()
