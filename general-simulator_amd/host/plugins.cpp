@@ -1,0 +1,328 @@
+// Phase-field plugins of the script host, running on the GPU through the C ABI.
+//
+// sys mart        -> Dynamics_mart        (General-Simulator src/dynamics_mart.cu)
+// sys precipitate -> Dynamics_precipitate (General-Simulator src/dynamics_precipitate.cu)
+// Keys keep the reference names and defaults where the physics is the same; keys
+// the reference does not read are stored by its `set` unchanged (src/gs.cu:106),
+// so scripts stay valid for both programs.
+#include <cmath>
+#include <sstream>
+
+#include "dynamics.h"
+#include "gsk.h"
+
+namespace gs {
+namespace {
+
+std::string fmt(double v) {
+  std::ostringstream os;
+  os << v;
+  return os.str();
+}
+
+// C[i][j][k][l] of a cubic crystal: the reference's cijkl (src/dynamics_mart.cu:80-89)
+void cubic_modulus(double c11, double c12, double c44, double* C) {
+  for (int i = 0; i < 81; ++i) C[i] = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) {
+          double v = 0.0;
+          if (i == j && k == l) v = (i == k) ? c11 : c12;
+          else if ((i == k && j == l) || (i == l && j == k)) v = c44;
+          C[((i * 3 + j) * 3 + k) * 3 + l] = v;
+        }
+}
+
+// Common part of both plugins: grid, handle, fields.
+class GpuDynamics : public Dynamics {
+ public:
+  ~GpuDynamics() override {
+    if (sim_) gsk_destroy(sim_);
+  }
+  int RunFunc(const std::string& /*func*/) override { return Code_COMMAND_UNKNOW; }
+  bool FixIsInert() const override { return true; }
+
+  // "set fix temperature Ts Te": linear temperature route (src/dynamics_mart.cu:421-441)
+  int Fix(double progress) override {
+    std::vector<std::string> w;
+    if (!var_values(*Vars, "fix", w)) return Code_NORMAL;
+    for (size_t i = 0; i < w.size();) {
+      if (w[i] == "temperature" && i + 2 < w.size()) {
+        double st = std::stod(w[i + 1]), et = std::stod(w[i + 2]);
+        (*Vars)["temperature"] = fmt(st + progress * (et - st));
+        i += 3;
+      } else if (w[i] == "pressure") {
+        i += 1;
+      } else {
+        log_line("Error: fix style " + w[i] + " does not find!");
+        return Code_NORMAL;
+      }
+    }
+    return Code_NORMAL;
+  }
+
+  int Calculate() override { return step(1); }
+  int CalculateMany(int n) override { return step(n); }
+
+ protected:
+  gsk_sim* sim_ = nullptr;
+  gsk_config cfg_{};
+  int nvar_ = 0;
+
+  int step(int n) {
+    if (gsk_step(sim_, n) != GSK_OK) {
+      log_line(std::string("Error: ") + gsk_last_error());
+      return Code_ERR;
+    }
+    return Code_NORMAL;
+  }
+
+  int read_grid() {
+    std::vector<std::string> w;
+    if (var_values(*Vars, "gridsize", w) && w.size() >= 3) {
+      nx = std::stoi(w[0]); ny = std::stoi(w[1]); nz = std::stoi(w[2]);
+      if (w.size() >= 6) { dx = std::stod(w[3]); dy = std::stod(w[4]); dz = std::stod(w[5]); }
+    }
+    if (nx != ny || ny != nz || nx <= 0) {
+      log_line("Error: gridsize must describe a periodic n x n x n grid");
+      return Code_ERR;
+    }
+    if (dx != dy || dy != dz) {
+      log_line("Error: gridsize needs equal spacings dx = dy = dz");
+      return Code_ERR;
+    }
+    return Code_NORMAL;
+  }
+
+  void read_common() {
+    cfg_.n = nx;
+    cfg_.dx = dx;
+    double v;
+    if (var_get(*Vars, "noiseamp", &v, 1)) cfg_.noise = v;
+    unsigned long long seed;
+    if (var_get(*Vars, "seed", &seed, 1)) cfg_.seed = seed;
+    int dev;
+    if (var_get(*Vars, "device", &dev, 1)) cfg_.device = dev;
+    std::vector<std::string> w;
+    if (var_values(*Vars, "wavevector", w) && w[0] == "reference") cfg_.kconv = GSK_K_REFERENCE;
+  }
+
+  int create() {
+    if (gsk_create(&cfg_, &sim_) != GSK_OK) {
+      log_line(std::string("Error: ") + gsk_last_error());
+      return Code_ERR;
+    }
+    long long ahead = 0;
+    if (var_get(*Vars, "ahead_steps", &ahead, 1)) gsk_set_step(sim_, ahead);
+    return Code_NORMAL;
+  }
+
+  int check(int rc) {
+    if (rc != GSK_OK) {
+      log_line(std::string("Error: ") + gsk_last_error());
+      return Code_ERR;
+    }
+    return Code_NORMAL;
+  }
+
+  // Upload a named field from the map, or initialise it.
+  // `init<name> uniform lo hi [seed]` draws Philox uniforms; otherwise `fill` is used.
+  int load_field(const std::string& name, const std::string& key, std::vector<int> dims, double fill) {
+    auto it = Datas->find(name);
+    const long long n3 = (long long)nx * ny * nz;
+    if (it != Datas->end() && !it->second.empty()) {
+      long long cnt = it->second.count();
+      long long want = 1;
+      for (int d : dims) want *= d;
+      if (cnt != want) {
+        log_line("Error: field " + name + " has " + std::to_string(cnt) + " values, expected " + std::to_string(want));
+        return Code_ERR;
+      }
+      return check(gsk_set_field(sim_, name.c_str(), it->second.data.data(), GSK_F64));
+    }
+    std::vector<std::string> w;
+    if (var_values(*Vars, key, w) && w[0] == "uniform" && w.size() >= 3) {
+      unsigned long long seed = w.size() >= 4 ? std::stoull(w[3]) : 0ull;
+      if (check(gsk_init_uniform(sim_, name.c_str(), std::stod(w[1]), std::stod(w[2]), seed))) return Code_ERR;
+    } else {
+      long long cnt = 1;
+      for (int d : dims) cnt *= d;
+      std::vector<double> f((size_t)cnt, fill);
+      if (check(gsk_set_field(sim_, name.c_str(), f.data(), GSK_F64))) return Code_ERR;
+    }
+    (void)n3;
+    Field& fd = (*Datas)[name];
+    fd.init(dims);
+    return check(gsk_get_field(sim_, name.c_str(), fd.data.data(), GSK_F64));
+  }
+
+  int pull(const std::string& name, std::vector<int> dims) {
+    Field& f = (*Datas)[name];
+    f.init(dims);
+    return check(gsk_get_field(sim_, name.c_str(), f.data.data(), GSK_F64));
+  }
+
+  std::string energy_col(const std::string& col) {
+    double out[16];
+    if (gsk_reduce(sim_, GSK_RED_ENERGY, out) != GSK_OK) return "nan";
+    if (col == "energy") return fmt(out[3]);
+    if (col == "chemical_energy") return fmt(out[0]);
+    if (col == "gradient_energy") return fmt(out[1]);
+    return fmt(out[2]);
+  }
+};
+
+// ------------------------------------------------------------------------------ martensite
+// Semi-implicit Allen-Cahn with Landau f = A/2 sum eta^2 - B/3 sum eta^3 + C/4 (sum eta^2)^2
+// and Khachaturyan elasticity on eta^2 (the HEAD contraction of src/dynamics_mart.cu:270-303
+// with the variational factor 2 of src/dynamics_precipitate.cu:193).
+class Dynamics_mart : public GpuDynamics {
+ public:
+  int Initialize() override {
+    if (read_grid()) return Code_ERR;
+    gsk_config_default(&cfg_, GSK_PHYS_MART);
+    read_common();
+    auto vt = Datas->find("varianttensor");
+    if (vt == Datas->end() || vt->second.empty() || vt->second.dims.size() != 3 || vt->second.dims[1] != 3 ||
+        vt->second.dims[2] != 3) {
+      log_line("Error: variants' strain tensor does not set while initialize dynamics");  // src/dynamics_mart.cu:44-47
+      return Code_ERR;
+    }
+    nvar_ = vt->second.dims[0];
+    cfg_.nvar = nvar_;
+    var_get(*Vars, "deltatime", &cfg_.dt, 1);
+    double abc[3] = {cfg_.A, cfg_.B, cfg_.C};
+    var_get(*Vars, "coefficient", abc, 3);
+    cfg_.A = abc[0]; cfg_.B = abc[1]; cfg_.C = abc[2];
+    var_get(*Vars, "kappa", &cfg_.kappa, 1);
+    var_get(*Vars, "mobility", &cfg_.L, 1);
+    var_get(*Vars, "weightelastic", &cfg_.w_el, 1);
+    std::vector<std::string> w;
+    if (var_values(*Vars, "precision", w) && w[0] == "fp64") cfg_.path = GSK_PATH_PLAIN64;
+    double C[81];
+    auto md = Datas->find("modulus");
+    if (md != Datas->end() && md->second.count() == 81) {
+      for (int i = 0; i < 81; ++i) C[i] = md->second.data[i];
+    } else {
+      double c[4] = {3.5, 1.5, 1.0, 1.0};  // src/dynamics_mart.cu:80
+      int k = var_get(*Vars, "cubicmodulus", c, 4);
+      if (k < 4) c[3] = 1.0;
+      cubic_modulus(c[0] * c[3], c[1] * c[3], c[2] * c[3], C);
+    }
+    if (create()) return Code_ERR;
+    if (check(gsk_build_btensor(sim_, C, vt->second.data.data(), nvar_))) return Code_ERR;
+    return load_field("eta", "initeta", {nvar_, nx, ny, nz}, 0.0);
+  }
+
+  std::string Get(const std::string& col) override {
+    if (col == "temperature") {
+      auto it = Vars->find("temperature");
+      return it == Vars->end() ? "nan" : it->second;
+    }
+    if (col == "eta" || col == "eta_average") {
+      double m[16];
+      if (gsk_reduce(sim_, GSK_RED_MEAN_ETA, m) != GSK_OK) return "nan";
+      double s = 0;
+      for (int v = 0; v < nvar_; ++v) s += m[v];
+      return fmt(s / nvar_);
+    }
+    if (col == "energy" || col == "chemical_energy" || col == "gradient_energy" || col == "elastic_energy")
+      return energy_col(col);
+    if (col == "step") return fmt((double)gsk_get_step(sim_));
+    return "nan";
+  }
+
+  int Pull(const std::string& name) override {
+    if (name == "eta" && sim_) return pull("eta", {nvar_, nx, ny, nz});
+    return Code_NORMAL;
+  }
+};
+
+// ------------------------------------------------------------------------------ precipitate
+// Cahn-Hilliard for c (k^4 implicit, optional stabilisation S k^2) and Allen-Cahn for eta
+// (src/dynamics_precipitate.cu:118-275) with the dilatational c-sourced elasticity.
+class Dynamics_precipitate : public GpuDynamics {
+ public:
+  int Initialize() override {
+    if (read_grid()) return Code_ERR;
+    gsk_config_default(&cfg_, GSK_PHYS_PREC);
+    read_common();
+    auto vt = Datas->find("varianttensor");
+    if (vt == Datas->end() || vt->second.empty()) {
+      log_line("Error: variants' strain tensor deoos not set while initialize dynamics");  // :32-35
+      return Code_ERR;
+    }
+    nvar_ = vt->second.dims[0];
+    cfg_.nvar = nvar_;
+    var_get(*Vars, "deltatime", &cfg_.dt, 1);
+    double A[7];
+    if (var_get(*Vars, "coefficient", A, 7) == 7)
+      for (int i = 0; i < 7; ++i) cfg_.a[i + 1] = A[i];
+    var_get(*Vars, "arfi", &cfg_.kappa, 1);
+    var_get(*Vars, "beta", &cfg_.kappa_c, 1);
+    var_get(*Vars, "meta", &cfg_.M, 1);
+    var_get(*Vars, "lpp", &cfg_.L, 1);
+    var_get(*Vars, "xi", &cfg_.xi, 1);
+    var_get(*Vars, "stabilization", &cfg_.S, 1);
+    double cc[2] = {cfg_.c1, cfg_.c2};
+    var_get(*Vars, "concentration", cc, 2);
+    cfg_.c1 = cc[0]; cfg_.c2 = cc[1];
+    std::vector<std::string> w;
+    if (var_values(*Vars, "elastic_source", w) && w[0] != "c") {
+      log_line("Error: elastic_source " + w[0] + " is not supported (only c)");
+      return Code_ERR;
+    }
+    double mod[3] = {3.5, 1.5, 1.0};  // src/dynamics_precipitate.cu:55-56
+    var_get(*Vars, "modulus", mod, 3);
+    double C[81];
+    cubic_modulus(mod[0], mod[1], mod[2], C);
+    double d0 = 0.02;
+    var_get(*Vars, "dilatation", &d0, 1);
+    double eps[9] = {d0, 0, 0, 0, d0, 0, 0, 0, d0};
+    if (create()) return Code_ERR;
+    if (check(gsk_build_btensor(sim_, C, eps, 1))) return Code_ERR;
+    if (load_field("concentration", "initconcentration", {nx, ny, nz}, cfg_.c1 + (cfg_.c2 - cfg_.c1) / 3))
+      return Code_ERR;  // default c (src/dynamics_precipitate.cu:46)
+    return load_field("eta", "initeta", {nvar_, nx, ny, nz}, 0.0);
+  }
+
+  std::string Get(const std::string& col) override {
+    if (col == "temperature") {
+      auto it = Vars->find("temperature");
+      return it == Vars->end() ? "nan" : it->second;
+    }
+    if (col == "eta_average" || col == "eta") {
+      double m[16];
+      if (gsk_reduce(sim_, GSK_RED_MEAN_ETA, m) != GSK_OK) return "nan";
+      double s = 0;
+      for (int v = 0; v < nvar_; ++v) s += m[v];
+      return fmt(s / nvar_);
+    }
+    if (col == "mass") {
+      double m[16];
+      if (gsk_reduce(sim_, GSK_RED_MASS, m) != GSK_OK) return "nan";
+      std::ostringstream os;
+      os.precision(17);
+      os << m[0];
+      return os.str();
+    }
+    if (col == "energy" || col == "chemical_energy" || col == "gradient_energy" || col == "elastic_energy")
+      return energy_col(col);
+    if (col == "step") return fmt((double)gsk_get_step(sim_));
+    return "nan";
+  }
+
+  int Pull(const std::string& name) override {
+    if (!sim_) return Code_NORMAL;
+    if (name == "eta") return pull("eta", {nvar_, nx, ny, nz});
+    if (name == "concentration") return pull("concentration", {nx, ny, nz});
+    return Code_NORMAL;
+  }
+};
+
+Register reg_mart("mart", [] { return std::unique_ptr<Dynamics>(new Dynamics_mart); });
+Register reg_prec("precipitate", [] { return std::unique_ptr<Dynamics>(new Dynamics_precipitate); });
+
+}  // namespace
+}  // namespace gs

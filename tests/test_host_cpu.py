@@ -1,0 +1,88 @@
+"""The gsi script host on the CPU: interpreter, expressions, field formats and the
+error paths the reference scripts can hit before any GPU work starts
+(src/input.cu:14-108, src/variable.cu:66-214, src/gs.cu:57-212)."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GSI = os.path.join(ROOT, "general-simulator_amd", "gsi")
+
+
+def run(tmp_path, script, *args):
+    p = tmp_path / "in.test"
+    p.write_text(script)
+    r = subprocess.run([GSI, str(p), *args], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    return r.returncode, r.stdout
+
+
+def test_expressions_and_print(tmp_path):
+    rc, out = run(tmp_path, ":steps=3000\nprint {steps/5} {2^3} {(1+2)*3}\n:x=steps\n:x+=1\nprint {x} {x>3000}\n"
+                            "set gridsize 64 64 64\nprint {$gridsize}\n")
+    assert rc == 0, out
+    assert "600 8 9" in out and "3001 1" in out
+
+
+def test_comments_semicolons_commas(tmp_path):
+    rc, out = run(tmp_path, "print a # comment\n:y=2; print {y*y}\nprint 1,2,3\n")
+    assert rc == 0
+    lines = [l for l in out.splitlines() if not l.startswith(">>>")]
+    assert "a" in lines and "4" in lines and "1 2 3" in lines
+
+
+def test_readhere_writehere_and_text_format(tmp_path):
+    script = ("readhere varianttensor\n3 2 3 3\n0.00, 0.0063, 0.0063 , 0.0063, 0.0, 0.0063 , 0.0063, 0.0063, 0.0\n"
+              "1 2 3 4 5 6 7 8 9\nwrite varianttensor vt.data\n")
+    rc, out = run(tmp_path, script)
+    assert rc == 0, out
+    lines = (tmp_path / "vt.data").read_text().split("\n")
+    assert lines[0].split() == ["3", "2", "3", "3"]
+    vals = [float(x) for x in lines[1:] if x.strip()]
+    assert vals == [0.0, 0.0063, 0.0063, 0.0063, 0.0, 0.0063, 0.0063, 0.0063, 0.0, 1, 2, 3, 4, 5, 6, 7, 8, 9]
+
+
+def test_binary_round_trip(tmp_path):
+    # binary format: int32 rank, int32 dims, float32 values (include/datamain.th:356-418)
+    vals = np.arange(6, dtype=np.float32) * 0.5
+    with open(tmp_path / "f.bin", "wb") as f:
+        f.write(struct.pack("iii", 2, 2, 3))
+        f.write(vals.tobytes())
+    rc, out = run(tmp_path, "set binary on\nread field f.bin\nwrite field g.bin\nset binary off\nwrite field g.txt\n")
+    assert rc == 0, out
+    raw = (tmp_path / "g.bin").read_bytes()
+    assert struct.unpack("iii", raw[:12]) == (2, 2, 3)
+    np.testing.assert_array_equal(np.frombuffer(raw[12:], np.float32), vals)
+    assert (tmp_path / "g.txt").read_text().split("\n")[0].split() == ["2", "2", "3"]
+
+
+def test_shell_and_backticks(tmp_path):
+    rc, out = run(tmp_path, "shell echo hello\nprint `echo 7`\n")
+    assert rc == 0 and "hello" in out and "\n7\n" in out
+
+
+def test_errors_stop_the_script(tmp_path):
+    rc, out = run(tmp_path, "sys nosuch\nprint never\n")
+    assert rc == 1 and 'unknown system "nosuch"' in out and "never" not in out.replace(">>>print never", "")
+    rc, out = run(tmp_path, "frobnicate 3\n")
+    assert rc == 1
+    rc, out = run(tmp_path, "run 10\n")
+    assert rc == 1 and "system not set" in out
+    rc, out = run(tmp_path, "read eta nofile.data\n")
+    assert rc == 1 and "not found" in out
+    rc, out = run(tmp_path, "print {undefined_var+1}\n")
+    assert rc == 1
+
+
+def test_missing_varianttensor_aborts_run(tmp_path):
+    # the reference logs and carries on (src/gs.cu:180); here the failed Initialize stops the run
+    rc, out = run(tmp_path, "sys mart\nset gridsize 16 16 16 1 1 1\nrun 10\n")
+    assert rc == 1 and "strain tensor does not set" in out
+
+
+def test_non_cubic_grid_rejected(tmp_path):
+    rc, out = run(tmp_path, "sys mart\nset gridsize 16 16 32 1 1 1\nreadhere varianttensor\n3 1 3 3\n"
+                            "1 0 0 0 1 0 0 0 1\nrun 1\n")
+    assert rc == 1 and "periodic n x n x n" in out
