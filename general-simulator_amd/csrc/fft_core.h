@@ -186,4 +186,122 @@ __device__ __forceinline__ void fft_lines(T2* lds, const Lines& L, const T2* tw)
 // Threads needed to transform nl lines of length N in one call.
 template <int N> constexpr int fft_threads(int nl) { return nl * (N / PlanOf<N>::rmin); }
 
+// Block size of the strided (y/x) passes: 16 lines per tile at most, 64..1024 threads.
+// Used as the exact launch bound so small blocks get the full register file.
+template <int N> constexpr int xy_threads() {
+  return fft_threads<N>(16) < 64 ? 64 : (fft_threads<N>(16) > 1024 ? 1024 : fft_threads<N>(16));
+}
+
+// ----------------------------------------------------------------- global-edge variants
+// Forward transforms use PlanOf<N> in order; inverse transforms use the reversed plan.
+// The last forward pass of radix Rl leaves thread (l, j) holding positions
+// j + r*N/Rl, which are exactly the operands of the first inverse pass (radix Rl,
+// Ns = 1), so a pointwise k-space operation can sit in registers between them.
+// Operands of the first pass are read straight from global memory through `ld`,
+// results of the last pass are written straight to global memory through `st`;
+// only the interior exchanges go through LDS.
+template <typename A, typename B> struct RevImpl;
+template <int... Done> struct RevImpl<Radices<>, Radices<Done...>> { using type = Radices<Done...>; };
+template <int R0, int... Rs, int... Done> struct RevImpl<Radices<R0, Rs...>, Radices<Done...>> {
+  using type = typename RevImpl<Radices<Rs...>, Radices<R0, Done...>>::type;
+};
+template <int N> using FwdPlan = typename PlanOf<N>::type;
+template <int N> using InvPlan = typename RevImpl<typename PlanOf<N>::type, Radices<>>::type;
+
+template <int R0, int... Rs> constexpr int first_radix(Radices<R0, Rs...>) { return R0; }
+template <int R0> constexpr int last_radix(Radices<R0>) { return R0; }
+template <int R0, int R1, int... Rs> constexpr int last_radix(Radices<R0, R1, Rs...>) {
+  return last_radix(Radices<R1, Rs...>{});
+}
+// radix of the last forward pass (= first inverse pass)
+template <int N> constexpr int edge_radix() { return last_radix(FwdPlan<N>{}); }
+
+template <typename T2, int N, int R, bool FWD>
+__device__ __forceinline__ void pass_math(T2* v, int j, int Ns, const T2* tw) {
+  if (Ns > 1) {
+    const int k = j % Ns;
+    const int st = k * (N / (Ns * R));
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      T2 w = tw[r * st];
+      if (!FWD) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+  }
+  dftR<R, FWD>(v);
+}
+
+// One pass with selectable operand source and result sink.
+// SRC: 0 = LDS, 1 = global via ld, 2 = registers vin.  DST: 0 = LDS, 1 = global via st, 2 = registers vout.
+template <typename T2, int N, int R, bool FWD, int SRC, int DST, class LD, class ST>
+__device__ __forceinline__ void gpass(T2* lds, const Lines& L, int Ns, const T2* tw, LD& ld, ST& st, T2* vreg) {
+  const int nb = L.nl * (N / R);
+  const int b = threadIdx.x;
+  const bool act = b < nb;
+  const int l = act ? b % L.nl : 0;
+  const int j = act ? b / L.nl : 0;
+  const int base = L.base(l);
+  T2 v[R];
+  if (act) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int pos = j + r * (N / R);
+      if constexpr (SRC == 0) v[r] = lds[base + pos * L.ps];
+      else if constexpr (SRC == 1) v[r] = ld(l, pos);
+      else v[r] = vreg[r];
+    }
+  }
+  // LDS must not be overwritten while other threads still read it
+  // LDS must not be overwritten while other threads still read it; an in-place
+  // global -> global single pass must finish all reads before any write
+  if constexpr (DST == 0 || (SRC == 1 && DST == 1)) __syncthreads();
+  if (act) {
+    pass_math<T2, N, R, FWD>(v, j, Ns, tw);
+    const int d = (j / Ns) * Ns * R + j % Ns;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int pos = d + r * Ns;
+      if constexpr (DST == 0) lds[base + pos * L.ps] = v[r];
+      else if constexpr (DST == 1) st(l, pos, v[r]);
+      else vreg[r] = v[r];
+    }
+  }
+  if constexpr (DST == 0) __syncthreads();
+}
+
+template <typename T2, int N, bool FWD, int SRC, int DST, int Ns, class LD, class ST, int R0, int... Rs>
+__device__ __forceinline__ void gchain(T2* lds, const Lines& L, const T2* tw, LD& ld, ST& st, T2* vreg,
+                                       Radices<R0, Rs...>) {
+  if constexpr (sizeof...(Rs) == 0) {
+    gpass<T2, N, R0, FWD, SRC, DST>(lds, L, Ns, tw, ld, st, vreg);
+  } else {
+    gpass<T2, N, R0, FWD, SRC, 0>(lds, L, Ns, tw, ld, st, vreg);
+    gchain<T2, N, FWD, 0, DST, Ns * R0>(lds, L, tw, ld, st, vreg, Radices<Rs...>{});
+  }
+}
+
+// global -> global transform of the lines in L (LDS used for interior passes only)
+template <typename T2, int N, bool FWD, class LD, class ST>
+__device__ __forceinline__ void fft_g2g(T2* lds, const Lines& L, const T2* tw, LD ld, ST st) {
+  T2* none = nullptr;
+  if constexpr (FWD) gchain<T2, N, true, 1, 1, 1>(lds, L, tw, ld, st, none, FwdPlan<N>{});
+  else gchain<T2, N, false, 1, 1, 1>(lds, L, tw, ld, st, none, InvPlan<N>{});
+}
+
+// global -> registers forward transform; vreg[edge_radix<N>()] receives positions j + r*N/Rl
+template <typename T2, int N, class LD>
+__device__ __forceinline__ void fft_g2r(T2* lds, const Lines& L, const T2* tw, LD ld, T2* vreg) {
+  auto st = [](int, int, T2) {};
+  gchain<T2, N, true, 1, 2, 1>(lds, L, tw, ld, st, vreg, FwdPlan<N>{});
+}
+
+// registers -> global inverse transform starting from the positions left by fft_g2r
+template <typename T2, int N, class ST>
+__device__ __forceinline__ void fft_r2g(T2* lds, const Lines& L, const T2* tw, T2* vreg, ST st) {
+  auto ld = [](int, int) { return T2{}; };
+  // in-place use: every thread has read its operands (fft_g2r) before anyone writes
+  __syncthreads();
+  gchain<T2, N, false, 2, 1, 1>(lds, L, tw, ld, st, vreg, InvPlan<N>{});
+}
+
 }  // namespace gsk

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <type_traits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -110,6 +111,7 @@ struct gsk_sim {
   int rblocks = 512;
   // state
   double cbar = 0.0;
+  double c_k0 = 0.0;  // FP64 sum of the stored concentration fluctuation (its k = 0 mode)
   double cee0 = 0.0;  // B(k = 0) of the c-sourced tensor
   bool haveB = false, dirty = true;
   long long step = 0;
@@ -124,6 +126,8 @@ namespace {
 
 size_t nvox(const gsk_sim* s) { return (size_t)s->n * s->n * s->n; }
 size_t hs_count(const gsk_sim* s) { return (size_t)s->n * s->n * (s->n / 2 + 1); }
+// State element of field 0 at k = 0: first main x tile (after the n/W Nyquist tiles), kx = 0, column 0.
+size_t k0_index(const gsk_sim* s) { return (size_t)(s->n / s->g.W) * s->F * s->n * s->g.W; }
 
 int mark(gsk_sim* s) {
   if (s->prof) {
@@ -164,6 +168,7 @@ PrecK prec_consts(const gsk_sim* s) {
   k.A5 = (float)c.a[5]; k.A6 = (float)c.a[6]; k.A7 = (float)c.a[7];
   k.c1 = (float)c.c1; k.c2 = (float)c.c2;
   k.cbar = (float)s->cbar;
+  k.k0 = (float)s->c_k0;
   k.noise = (float)c.noise;
   k.dtM = (float)(c.dt * c.M);
   k.dtMkc = (float)(c.dt * c.M * c.kappa_c);
@@ -202,32 +207,56 @@ int grid_1d(long work, int threads = 256) {
   return (int)b;
 }
 
+// Allow a kernel the device's whole LDS for dynamic shared memory (minus its static part).
+int lds_limit(const void* f) {
+  int dev = 0, optin = 0, maxs = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&maxs, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && optin > maxs)
+    maxs = optin;
+  hipFuncAttributes a;
+  HIPCHK(hipFuncGetAttributes(&a, f));
+  HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, maxs - (int)a.sharedSizeBytes));
+  return GSK_OK;
+}
+
 // ------------------------------------------------------------------ per-size implementation
 template <int N>
 struct Impl {
   static constexpr int M = N / 2;
 
   static int set_attrs() {
-    const int lim = 160 * 1024;
-    HIPCHK(hipFuncSetAttribute((const void*)k_zfwd<M, Z_COPY>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_zfwd<M, Z_SQUARE>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_zfwd<M, Z_PREC_LOCAL>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_zmid_mart<M>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_zinv<M>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_ypass<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_ypass<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_xpass<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_xpass<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_xcontract<N>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_xupdate_mart<N, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_xupdate_mart<N, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_xupdate_prec<N, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)k_xupdate_prec<N, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)r_xenergy_mart<N>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)p_zrows<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)p_zrows<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)p_strided<N, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
-    HIPCHK(hipFuncSetAttribute((const void*)p_strided<N, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lim));
+    CHK(lds_limit((const void*)k_zfwd<M, Z_COPY>));
+    CHK(lds_limit((const void*)k_zfwd<M, Z_SQUARE>));
+    CHK(lds_limit((const void*)k_zfwd<M, Z_PREC_LOCAL>));
+    CHK(lds_limit((const void*)k_zmid_mart<M>));
+    CHK(lds_limit((const void*)k_zinv<M>));
+    CHK(lds_limit((const void*)k_ypass<N, true>));
+    CHK(lds_limit((const void*)k_ypass<N, false>));
+    CHK(lds_limit((const void*)k_xpass<N, true>));
+    CHK(lds_limit((const void*)k_xpass<N, false>));
+    CHK(lds_limit((const void*)k_xcontract<N>));
+    CHK(lds_limit((const void*)k_xcontract_r<N, 1>));
+    CHK(lds_limit((const void*)k_xcontract_r<N, 2>));
+    CHK(lds_limit((const void*)k_xcontract_r<N, 3>));
+    CHK(lds_limit((const void*)k_xcontract_r<N, 4>));
+    CHK(lds_limit((const void*)k_xupdate_mart<N, 1, 0>));
+    CHK(lds_limit((const void*)k_xupdate_mart<N, 2, 0>));
+    CHK(lds_limit((const void*)k_xupdate_mart<N, 1, 1>));
+    CHK(lds_limit((const void*)k_xupdate_mart<N, 2, 1>));
+    CHK(lds_limit((const void*)k_xupdate_prec<N, 1, 0>));
+    CHK(lds_limit((const void*)k_xupdate_prec<N, 2, 0>));
+    CHK(lds_limit((const void*)k_xupdate_prec<N, 1, 1>));
+    CHK(lds_limit((const void*)k_xupdate_prec<N, 2, 1>));
+    CHK(lds_limit((const void*)r_xenergy_mart<N>));
+    CHK(lds_limit((const void*)r_xenergy_mart_r<N, 1>));
+    CHK(lds_limit((const void*)r_xenergy_mart_r<N, 2>));
+    CHK(lds_limit((const void*)r_xenergy_mart_r<N, 3>));
+    CHK(lds_limit((const void*)r_xenergy_mart_r<N, 4>));
+    CHK(lds_limit((const void*)p_zrows<N, true>));
+    CHK(lds_limit((const void*)p_zrows<N, false>));
+    CHK(lds_limit((const void*)p_strided<N, true>));
+    CHK(lds_limit((const void*)p_strided<N, false>));
     return GSK_OK;
   }
 
@@ -246,7 +275,7 @@ struct Impl {
     return GSK_OK;
   }
   static int ypass(gsk_sim* s, bool fwd, int F) {
-    const int grid = s->n * s->g.nch;
+    const int grid = s->g.ntile;
     if (fwd) LAUNCH((k_ypass<N, true>), grid, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, s->twN);
     else LAUNCH((k_ypass<N, false>), grid, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, s->twN);
     return GSK_OK;
@@ -258,29 +287,61 @@ struct Impl {
     return GSK_OK;
   }
 
+  // k-space updates in groups of 2 fields (register-resident x transforms)
+  template <int MODE>
+  static int xupdate_mart(gsk_sim* s, const MartK& mk) {
+    const int gx = s->g.ntile;
+    for (int f0 = 0; f0 < s->F; f0 += 2) {
+      if (s->F - f0 >= 2)
+        LAUNCH((k_xupdate_mart<N, 2, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN, f0);
+      else
+        LAUNCH((k_xupdate_mart<N, 1, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN, f0);
+    }
+    return GSK_OK;
+  }
+  template <int MODE>
+  static int xupdate_prec(gsk_sim* s, const PrecK& pk) {
+    const int gx = s->g.ntile;
+    for (int f0 = 0; f0 < s->F; f0 += 2) {
+      if (s->F - f0 >= 2)
+        LAUNCH((k_xupdate_prec<N, 2, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->Bt, s->g, pk, s->twN,
+               f0);
+      else
+        LAUNCH((k_xupdate_prec<N, 1, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->Bt, s->g, pk, s->twN,
+               f0);
+    }
+    return GSK_OK;
+  }
+
   static int init_state(gsk_sim* s) {
-    const int gx = s->n * s->g.nch;
+    const int gx = s->g.ntile;
     if (s->cfg.physics == GSK_PHYS_MART) {
       CHK(zfwd(s, Z_COPY, s->fld, s->F));
       CHK(ypass(s, true, s->F));
       MartK mk = mart_consts(s);
-      LAUNCH((k_xupdate_mart<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN);
+      CHK(xupdate_mart<1>(s, mk));
     } else {
       CHK(zfwd(s, Z_COPY, s->fld, s->F));
       CHK(ypass(s, true, s->F));
       PrecK pk = prec_consts(s);
-      LAUNCH((k_xupdate_prec<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->Bt, s->g, pk, s->twN);
+      CHK(xupdate_prec<1>(s, pk));
     }
     return GSK_OK;
   }
 
   static int step_mart_fused(gsk_sim* s) {
-    const int gx = s->n * s->g.nch;
+    const int gx = s->g.ntile;
     const int F = s->F;
     MartK mk = mart_consts(s);
     CHK(zfwd(s, Z_SQUARE, s->fld, F));
     CHK(ypass(s, true, F));
-    LAUNCH((k_xcontract<N>), gx, s->xthreads, s->lds_xc, s, s->kbuf, s->Bt, s->g, mk, s->twN);
+    switch (F) {  // register-resident contraction for up to 4 variants
+      case 1: LAUNCH((k_xcontract_r<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
+      case 2: LAUNCH((k_xcontract_r<N, 2>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
+      case 3: LAUNCH((k_xcontract_r<N, 3>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
+      case 4: LAUNCH((k_xcontract_r<N, 4>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
+      default: LAUNCH((k_xcontract<N>), gx, s->xthreads, s->lds_xc, s, s->kbuf, s->Bt, s->g, mk, s->twN);
+    }
     CHK(ypass(s, false, F));
     {
       const int grid = s->n * s->n / s->g.L;
@@ -288,19 +349,19 @@ struct Impl {
       LAUNCH((k_zmid_mart<M>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn);
     }
     CHK(ypass(s, true, F));
-    LAUNCH((k_xupdate_mart<N, 0>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN);
+    CHK(xupdate_mart<0>(s, mk));
     CHK(ypass(s, false, F));
     CHK(zinv(s, s->fld, F));
     return GSK_OK;
   }
 
   static int step_prec_fused(gsk_sim* s) {
-    const int gx = s->n * s->g.nch;
+    const int gx = s->g.ntile;
     const int F = s->F;
     PrecK pk = prec_consts(s);
     CHK(zfwd(s, Z_PREC_LOCAL, s->fld, F));
     CHK(ypass(s, true, F));
-    LAUNCH((k_xupdate_prec<N, 0>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->Bt, s->g, pk, s->twN);
+    CHK(xupdate_prec<0>(s, pk));
     CHK(ypass(s, false, F));
     CHK(zinv(s, s->fld, F));
     return GSK_OK;
@@ -347,9 +408,19 @@ struct Impl {
     // S_hat = FFT(eta^2) through z and y; the energy kernel applies x and contracts.
     CHK(zfwd(s, Z_SQUARE, s->fld, s->F));
     CHK(ypass(s, true, s->F));
-    const int gx = s->n * s->g.nch;
+    const int gx = s->g.ntile;
     if (gx > s->rblocks) return fail(GSK_ERR, "reduction scratch too small");
-    LAUNCH((r_xenergy_mart<N>), gx, s->xthreads, s->lds_xc, s, s->kbuf, s->Bt, s->g, s->F, s->twN, s->part);
+    switch (s->F) {
+      case 1: LAUNCH((r_xenergy_mart_r<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, s->twN, s->part); break;
+      case 2: LAUNCH((r_xenergy_mart_r<N, 2>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, s->twN, s->part); break;
+      case 3: LAUNCH((r_xenergy_mart_r<N, 3>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, s->twN, s->part); break;
+      case 4: LAUNCH((r_xenergy_mart_r<N, 4>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, s->twN, s->part); break;
+      default: {
+        const size_t lds = (size_t)(N + (size_t)s->F * N * s->g.W) * sizeof(float2);
+        if (lds > 160 * 1024) return fail(GSK_ERR_UNSUPPORTED, "elastic energy: too many variants for this grid");
+        LAUNCH((r_xenergy_mart<N>), gx, s->xthreads, lds, s, s->kbuf, s->Bt, s->g, s->F, s->twN, s->part);
+      }
+    }
     LAUNCH(r_finish, 1, 1024, 0, s, s->part, gx, 1, s->rout);
     HIPCHK(hipMemcpyAsync(el, s->rout, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -407,7 +478,7 @@ template <int N> struct DZfwd { static int run(gsk_sim* s, const float* in, int 
 template <int N> struct DY { static int run(gsk_sim* s, bool fwd, int F) { return Impl<N>::ypass(s, fwd, F); } };
 template <int N> struct DX {
   static int run(gsk_sim* s, bool fwd, int F, float scale) {
-    const int gx = s->n * s->g.nch;
+    const int gx = s->g.ntile;
     if (fwd) LAUNCH((k_xpass<N, true>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, scale, s->twN);
     else LAUNCH((k_xpass<N, false>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, scale, s->twN);
     return GSK_OK;
@@ -431,20 +502,27 @@ void choose_geometry(gsk_sim* s, int Fz, int Fx_resident) {
   g.n = n; g.M = M; g.Pz = M + 16;
   g.nvox = (long long)n * n * n;
   g.kfs = (long long)n * n * g.Pz;
-  // z tiles: rows L, threads L*M/rmin <= 256, LDS <= 96 KB
+  // z tiles: rows L, threads L*M/rmin <= zmax (256; GSK_ZTHREADS overrides), LDS <= 96 KB
   const int rz = rmin_of(M);
+  int zmax = 256;
+  if (const char* e = std::getenv("GSK_ZTHREADS")) zmax = std::max(64, std::atoi(e));
   int L = 1;
-  while (L * 2 <= n * n && (L * 2) * M / rz <= 256 &&
+  while (L * 2 <= n * n && (L * 2) * M / rz <= zmax &&
          (2.0 * M + (double)Fz * (L * 2) * (M + 1)) * 8.0 <= 96 * 1024)
     L *= 2;
   g.L = L;
   s->zthreads = round64(std::max(L * M / rz, 64));
-  // x/y tiles: width W <= 16, threads W*n/rmin <= 1024, resident tiles <= 120 KB
+  // x/y tiles: width W <= 16 (GSK_W overrides) dividing n/2, threads W*n/rmin <= 1024,
+  // resident tiles <= 120 KB
   const int rx = rmin_of(n);
   int W = 16;
-  while (W > 1 && (W * n / rx > 1024 || (double)(n + (double)Fx_resident * n * W) * 8.0 > 120 * 1024)) W /= 2;
+  // (the strided kernels are compiled for at most 16 lines per tile, xy_threads<N>())
+  if (const char* e = std::getenv("GSK_W")) W = std::max(1, std::min(16, std::atoi(e)));
+  while (W > 1 && (M % W || W * n / rx > 1024 || (double)(n + (double)Fx_resident * n * W) * 8.0 > 120 * 1024))
+    W /= 2;
   g.W = W;
-  g.nch = (M + 1 + W - 1) / W;
+  g.nchm = M / W;
+  g.ntile = n * g.nchm + n / W;
   s->xthreads = round64(std::max(W * n / rx, 64));
   s->lds_y = (size_t)(n + n * W) * sizeof(float2);
   s->lds_xc = (size_t)(n + (size_t)Fx_resident * n * W) * sizeof(float2);
@@ -501,13 +579,20 @@ void build_slots(gsk_sim* s) {
     s->slots.push_back({"y_inv_R", F * 2 * hs});
     s->slots.push_back({"zmid_force", F * (2 * hs + rv)});
     s->slots.push_back({"y_fwd_F", F * 2 * hs});
-    s->slots.push_back({"x_update", F * 4 * hs});
+    for (int f0 = 0; f0 < F; f0 += 2) {  // one launch per group of 2 fields
+      const int c = std::min(2, F - f0);
+      s->slots.push_back({f0 == 0 ? "x_update_g0" : (f0 == 2 ? "x_update_g1" : "x_update_g2"), c * 4 * hs});
+    }
     s->slots.push_back({"y_inv_eta", F * 2 * hs});
     s->slots.push_back({"zinv_eta", F * (hs + rv)});
   } else {
     s->slots.push_back({"zfwd_local", F * (rv + hs)});
     s->slots.push_back({"y_fwd", F * 2 * hs});
-    s->slots.push_back({"x_update", F * 4 * hs + hs / 2});
+    for (int f0 = 0; f0 < F; f0 += 2) {
+      const int c = std::min(2, F - f0);
+      s->slots.push_back({f0 == 0 ? "x_update_g0" : (f0 == 2 ? "x_update_g1" : (f0 == 4 ? "x_update_g2" : "x_update_g3")),
+                          c * 4 * hs + (f0 == 0 ? hs / 2 : 0.0)});
+    }
     s->slots.push_back({"y_inv", F * 2 * hs});
     s->slots.push_back({"zinv", F * (hs + rv)});
   }
@@ -591,7 +676,7 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
     return bail(fail(GSK_ERR_HIP, "hipStreamCreate failed"));
   const int n = s->n, M = n / 2;
   const size_t N3 = nvox(s);
-  choose_geometry(s, s->F, s->cfg.physics == GSK_PHYS_MART ? s->F : 1);
+  choose_geometry(s, s->F, (s->cfg.physics == GSK_PHYS_MART && s->F > 4) ? s->F : 1);
   choose_plain(s);
   auto up = [&](auto*& dst, const auto& host) -> int {
     using T = std::remove_reference_t<decltype(*dst)>;
@@ -611,7 +696,7 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
     if (hipMemset(p, 0, count * sizeof(T)) != hipSuccess) return fail(GSK_ERR_HIP, "hipMemset failed");
     return GSK_OK;
   };
-  const size_t T = (size_t)n * s->g.W, ntile = (size_t)n * s->g.nch;
+  const size_t T = (size_t)n * s->g.W, ntile = (size_t)s->g.ntile;
   if (cfg->path == GSK_PATH_FUSED32) {
     if ((rc = alloc(s->fld, (size_t)s->F * N3)) != GSK_OK) return bail(rc);
     if ((rc = alloc(s->kbuf, (size_t)s->F * s->g.kfs)) != GSK_OK) return bail(rc);
@@ -687,7 +772,7 @@ int gsk_build_btensor(gsk_sim* s, const double* C81, const double* eps, int neps
   CHK(fill_bargs(a, C81, eps, neps, eps, neps, kfac));
   s->cee0 = a.cee[0];
   if (s->cfg.path == GSK_PATH_FUSED32) {
-    const long total = (long)s->n * s->g.nch * s->n * s->g.W;
+    const long total = (long)(long)s->g.ntile * s->n * s->g.W;
     hipLaunchKernelGGL(k_btensor_tiles, dim3(grid_1d(total)), dim3(256), 0, s->stream, s->Bt, s->g, a);
   } else {
     hipLaunchKernelGGL(k_btensor_full, dim3(grid_1d((long)nvox(s))), dim3(256), 0, s->stream, s->B64, s->n, a);
@@ -725,26 +810,40 @@ int gsk_set_field(gsk_sim* s, const char* name, const void* host, int dtype) {
     std::vector<double> h(count);
     for (size_t i = 0; i < count; ++i)
       h[i] = dtype == GSK_F32 ? (double)((const float*)host)[i] : ((const double*)host)[i];
+    CHK(sync(s));  // earlier work on the handle's stream may still read the field
     HIPCHK(hipMemcpy(s->eta64, h.data(), count * sizeof(double), hipMemcpyHostToDevice));
     return GSK_OK;
   }
   std::vector<float> h(count);
   const bool conc = std::strcmp(name, "concentration") == 0;
   if (conc) {
-    // keep the mean in FP64 and store the fluctuation in FP32
-    double sum = 0.0;
-    for (size_t i = 0; i < count; ++i) sum += dtype == GSK_F32 ? (double)((const float*)host)[i] : ((const double*)host)[i];
-    double mean = sum / (double)count;
-    double resid = 0.0;
+    // keep the mean in FP64 (compensated sum) and store the fluctuation in FP32;
+    // the state's k = 0 mode of the fluctuation is held at exactly zero
+    double sum = 0.0, comp = 0.0;
     for (size_t i = 0; i < count; ++i) {
-      double v = dtype == GSK_F32 ? (double)((const float*)host)[i] : ((const double*)host)[i];
-      h[i] = (float)(v - mean);
-      resid += (double)h[i];
+      const double v = dtype == GSK_F32 ? (double)((const float*)host)[i] : ((const double*)host)[i];
+      const double y = v - comp, t = sum + y;
+      comp = (t - sum) - y;
+      sum = t;
     }
-    s->cbar = mean + resid / (double)count;
+    // The mean is rounded to binary32 so that v - cbar is exact in FP32 for inputs near
+    // the mean (no correlated rounding of the fluctuations); the FP64 sum of the
+    // fluctuations becomes the k = 0 mode of the state.
+    const double cbar = (double)(float)(sum / (double)count);
+    double k0 = 0.0, ck = 0.0;
+    for (size_t i = 0; i < count; ++i) {
+      const double v = dtype == GSK_F32 ? (double)((const float*)host)[i] : ((const double*)host)[i];
+      h[i] = (float)(v - cbar);
+      const double y = (double)h[i] - ck, t = k0 + y;
+      ck = (t - k0) - y;
+      k0 = t;
+    }
+    s->cbar = cbar;
+    s->c_k0 = k0;
   } else {
     for (size_t i = 0; i < count; ++i) h[i] = dtype == GSK_F32 ? ((const float*)host)[i] : (float)((const double*)host)[i];
   }
+  CHK(sync(s));  // earlier work on the handle's stream may still read the field
   HIPCHK(hipMemcpy(s->fld + (size_t)off * N3, h.data(), count * sizeof(float), hipMemcpyHostToDevice));
   s->dirty = true;
   return GSK_OK;
@@ -867,8 +966,15 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
   }
   if (what == GSK_RED_MASS) {
     if (c.physics != GSK_PHYS_PREC) return fail(GSK_ERR_UNSUPPORTED, "mass is a precipitate reduction");
+    CHK(prepare(s));
+    // out[0]: N * mean + Re of the k = 0 mode of the fluctuation state (the conserved quantity)
+    // out[1]: FP64 sum of the FP32 real field (includes one IFFT's rounding, not accumulated)
+    float2 k0 = make_float2(0.f, 0.f);
+    HIPCHK(hipMemcpyAsync(&k0, s->state + k0_index(s), sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+    CHK(sync(s));
     CHK(real_pass());
-    out[0] = s->cbar * (double)N3 + h[1];
+    out[0] = s->cbar * (double)N3 + (double)k0.x;
+    out[1] = s->cbar * (double)N3 + h[1];
     return GSK_OK;
   }
   if (what == GSK_RED_ENERGY) {
@@ -881,7 +987,7 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
       gr = 0.5 * c.kappa * g2 / (double)N3;
       el = 0.5 * c.w_el * e2 / (double)N3;
     } else {
-      const int ntile = s->n * s->g.nch;
+      const int ntile = s->g.ntile;
       const double kfac = c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx);
       LAUNCH(r_state_grad, nb, 256, 0, s, s->state, s->g, s->F, (float)kfac, s->part);
       LAUNCH(r_finish, 1, 1024, 0, s, s->part, nb, s->F, s->rout);
@@ -904,9 +1010,12 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
         double eh;
         HIPCHK(hipMemcpyAsync(&eh, s->rout, sizeof(double), hipMemcpyDeviceToHost, s->stream));
         CHK(sync(s));
-        // the k = 0 mode of c is N * cbar (the FP32 state holds the fluctuation only)
-        const double c0 = (double)N3 * s->cbar;
-        el = 0.5 * c.xi * (eh + s->cee0 * c0 * c0) / (double)N3;
+        // the k = 0 mode of c is N * cbar + the state's k = 0 mode of the fluctuation
+        float2 k0 = make_float2(0.f, 0.f);
+        HIPCHK(hipMemcpyAsync(&k0, s->state + k0_index(s), sizeof(float2), hipMemcpyDeviceToHost, s->stream));
+    CHK(sync(s));
+        const double f0 = (double)k0.x, c0 = (double)N3 * s->cbar + f0;
+        el = 0.5 * c.xi * (eh - s->cee0 * f0 * f0 + s->cee0 * c0 * c0) / (double)N3;
         (void)ntile;
       }
     }
@@ -1028,6 +1137,52 @@ static int fft_sim(int n, int count, int device, gsk_sim** out) {
   return gsk_create(&c, out);
 }
 
+int gsk_fail_(int code, const char* msg) { return fail(code, msg ? msg : "error"); }
+
+int gsk_fft_prepare_(int n, int count, int device, void** sim, float** real, float2** kbuf, hipStream_t* stream) {
+  gsk_sim* s = nullptr;
+  CHK(fft_sim(n, count, device, &s));
+  *sim = s;
+  *real = s->fld;
+  *kbuf = s->kbuf;
+  *stream = s->stream;
+  return GSK_OK;
+}
+
+// dir > 0: R2C of the real fields into the half spectrum; dir < 0: C2R back (unnormalised)
+int gsk_fft_run_(void* sim, int dir) {
+  gsk_sim* s = static_cast<gsk_sim*>(sim);
+  const int n = s->n, F = s->V;
+  if (dir > 0) {
+    CHK(dispatch_n<DZfwd>(n, s, (const float*)s->fld, F));
+    CHK(dispatch_n<DY>(n, s, true, F));
+    return dispatch_n<DX>(n, s, true, F, 1.0f);
+  }
+  CHK(dispatch_n<DX>(n, s, false, F, 1.0f));
+  CHK(dispatch_n<DY>(n, s, false, F));
+  return dispatch_n<DZinv>(n, s, s->fld, F);
+}
+
+int gsk_fft_release_(void* sim) { return gsk_destroy(static_cast<gsk_sim*>(sim)); }
+
+// Copy the half spectra of the handle's fields to host in the compact [f][kx][ky][kz] layout.
+int gsk_fft_spectrum_(void* sim, float* out) {
+  gsk_sim* s = static_cast<gsk_sim*>(sim);
+  const int n = s->n, M = n / 2, count = s->V;
+  CHK(sync(s));
+  std::vector<float2> h((size_t)count * s->g.kfs);
+  HIPCHK(hipMemcpy(h.data(), s->kbuf, h.size() * sizeof(float2), hipMemcpyDeviceToHost));
+  for (size_t f = 0; f < (size_t)count; ++f)
+    for (size_t r = 0; r < (size_t)n * n; ++r)
+      for (size_t k = 0; k <= (size_t)M; ++k) {
+        const float2 v = h[kidx(s->g, (int)f, (long long)r, (int)k)];
+        const size_t o = ((f * n * n + r) * (M + 1) + k) * 2;
+        out[o] = v.x;
+        out[o + 1] = v.y;
+      }
+  return GSK_OK;
+}
+
 int gsk_fft_r2c(int n, int count, const float* in, float* out, int device) {
   if (!in || !out) return fail(GSK_ERR_ARG, "null argument");
   gsk_sim* s = nullptr;
@@ -1040,19 +1195,8 @@ int gsk_fft_r2c(int n, int count, const float* in, float* out, int device) {
   if (rc == GSK_OK) rc = dispatch_n<DY>(n, s, true, count);
   if (rc == GSK_OK) rc = dispatch_n<DX>(n, s, true, count, 1.0f);
   if (rc == GSK_OK) rc = sync(s);
-  if (rc == GSK_OK) {
-    std::vector<float2> h((size_t)count * s->g.kfs);
-    if (hipMemcpy(h.data(), s->kbuf, h.size() * sizeof(float2), hipMemcpyDeviceToHost) != hipSuccess)
-      rc = fail(GSK_ERR_HIP, "copy out failed");
-    for (size_t f = 0; rc == GSK_OK && f < (size_t)count; ++f)
-      for (size_t r = 0; r < (size_t)n * n; ++r)
-        for (size_t k = 0; k <= M; ++k) {
-          float2 v = h[f * s->g.kfs + r * s->g.Pz + k];
-          size_t o = ((f * n * n + r) * (M + 1) + k) * 2;
-          out[o] = v.x;
-          out[o + 1] = v.y;
-        }
-  }
+  if (rc == GSK_OK) rc = gsk_fft_spectrum_(s, out);
+  (void)M;
   gsk_destroy(s);
   return rc;
 }
@@ -1067,7 +1211,7 @@ int gsk_fft_c2r(int n, int count, const float* in, float* out, int device) {
     for (size_t r = 0; r < (size_t)n * n; ++r)
       for (size_t k = 0; k <= M; ++k) {
         size_t o = ((f * n * n + r) * (M + 1) + k) * 2;
-        h[f * s->g.kfs + r * s->g.Pz + k] = make_float2(in[o], in[o + 1]);
+        h[kidx(s->g, f, r, k)] = make_float2(in[o], in[o + 1]);
       }
   int rc = GSK_OK;
   if (hipMemcpy(s->kbuf, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)

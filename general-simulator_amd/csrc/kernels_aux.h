@@ -1,6 +1,6 @@
 // B-tensor builder (FP64), FP64 plain-path kernels and FP64 reductions.
 #pragma once
-#include "kernels_fused.h"
+#include "kernels_strided.h"
 
 namespace gsk {
 
@@ -85,18 +85,19 @@ __global__ void k_btensor_full(double* __restrict__ B, int n, BArgs a) {
 __global__ void k_btensor_tiles(float* __restrict__ Bt, Geo g, BArgs a) {
   const int n = g.n, W = g.W, T = n * W;
   const int V = a.V1, npq = V * (V + 1) / 2;
-  const long total = (long)n * g.nch * T;
+  const long total = (long)g.ntile * T;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     const int e = (int)(i % T);
     const long t = i / T;
-    const int ky = (int)(t / g.nch), ch = (int)(t % g.nch);
-    const int kx = e / W, kz = ch * W + e % W;
+    const STile s = xtile((int)t, g);
+    const int c = e % W, kx = e / W;
+    const int ky = s.ky0 + s.dky * c, kz = s.kz0 + s.dkz * c;
     double out[64];
-    if (kz <= g.M) btensor_at(kx, ky, kz, n, a, out);
+    btensor_at(kx, ky, kz, n, a, out);
     for (int p = 0; p < V; ++p)
       for (int q = p; q < V; ++q) {
         const int pq = p * V - (p * (p - 1)) / 2 + (q - p);
-        Bt[(t * npq + pq) * T + e] = (kz <= g.M) ? (float)out[p * V + q] : 0.f;
+        Bt[(t * npq + pq) * T + e] = (float)out[p * V + q];
       }
   }
 }
@@ -319,21 +320,21 @@ __global__ void r_prec_real(const float* __restrict__ fld, long N3, PrecE pe, do
 __global__ void r_state_grad(const float2* __restrict__ state, Geo g, int F, float kfac, double* __restrict__ part) {
   __shared__ double sh[16];
   const int n = g.n, W = g.W, T = n * W;
-  const long total = (long)n * g.nch * F * T;
+  const long total = (long)g.ntile * F * T;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     const int e = (int)(i % T);
     const long tf = i / T;
     const int f = (int)(tf % F);
     const long t = tf / F;
-    const int ky = (int)(t / g.nch), ch = (int)(t % g.nch);
-    const int kx = e / W, kz = ch * W + e % W;
-    if (kz > g.M) continue;
+    const STile s = xtile((int)t, g);
+    const int c = e % W, kx = e / W;
+    const int ky = s.ky0 + s.dky * c, kz = s.kz0 + s.dkz * c;
     const double kxv = (double)kfac * ((kx < n / 2) ? kx : kx - n), kyv = (double)kfac * ((ky < n / 2) ? ky : ky - n),
                  kzv = (double)kfac * kz;
     const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
-    const float2 s = state[i];
-    acc[f] += w * (kxv * kxv + kyv * kyv + kzv * kzv) * ((double)s.x * s.x + (double)s.y * s.y);
+    const float2 v = state[i];
+    acc[f] += w * (kxv * kxv + kyv * kyv + kzv * kzv) * ((double)v.x * v.x + (double)v.y * v.y);
   }
   for (int f = 0; f < F; ++f) {
     double r = block_sum(acc[f], sh);
@@ -346,13 +347,13 @@ __global__ void r_state_el_prec(const float2* __restrict__ state, const float* _
                                 double* __restrict__ part) {
   __shared__ double sh[16];
   const int W = g.W, T = g.n * W;
-  const long total = (long)g.n * g.nch * T;
+  const long total = (long)g.ntile * T;
   double acc = 0.0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     const int e = (int)(i % T);
     const long t = i / T;
-    const int kz = (int)(t % g.nch) * W + e % W;
-    if (kz > g.M) continue;
+    const STile st = xtile((int)t, g);
+    const int kz = st.kz0 + st.dkz * (e % W);
     const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
     const float2 s = state[(t * F + 0) * T + e];
     acc += w * (double)Bt[t * T + e] * ((double)s.x * s.x + (double)s.y * s.y);
@@ -363,7 +364,7 @@ __global__ void r_state_el_prec(const float2* __restrict__ state, const float* _
 
 // Martensite elastic energy sum_k w sum_pq B_pq Re(conj(S_p) S_q) after the forward x transform of S.
 template <int N>
-__global__ void __launch_bounds__(1024) r_xenergy_mart(const float2* __restrict__ kbuf, const float* __restrict__ Bt,
+__global__ void __launch_bounds__(xy_threads<N>()) r_xenergy_mart(const float2* __restrict__ kbuf, const float* __restrict__ Bt,
                                                       Geo g, int F, const float2* __restrict__ twN,
                                                       double* __restrict__ part) {
   extern __shared__ float2 smem[];
@@ -371,9 +372,13 @@ __global__ void __launch_bounds__(1024) r_xenergy_mart(const float2* __restrict_
   float2* tw = smem;
   float2* tiles = smem + N;
   const int W = g.W, T = N * W;
-  const int t = blockIdx.x, ky = t / g.nch, ch = t % g.nch;
+  const int t = blockIdx.x;
+  const STile st = xtile(t, g);
   load_tw<N>(tw, twN);
-  for (int f = 0; f < F; ++f) x_load(tiles + f * T, kbuf + f * g.kfs, g, N, ky, ch);
+  for (int f = 0; f < F; ++f) {
+    const TileIO io = tio(const_cast<float2*>(kbuf) + f * g.kfs, st);
+    for (int e = threadIdx.x; e < T; e += blockDim.x) tiles[f * T + e] = io.get(e % W, e / W);
+  }
   __syncthreads();
   Lines Ls{W, W, 1, 0, W};
   for (int f = 0; f < F; ++f) fft_lines<float2, N, true>(tiles + f * T, Ls, tw);
@@ -381,8 +386,7 @@ __global__ void __launch_bounds__(1024) r_xenergy_mart(const float2* __restrict_
   const float* Bb = Bt + (long)t * npq * T;
   double acc = 0.0;
   for (int e = threadIdx.x; e < T; e += blockDim.x) {
-    const int kz = ch * W + e % W;
-    if (kz > g.M) continue;
+    const int kz = st.kz0 + st.dkz * (e % W);
     const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
     double s = 0.0;
     for (int p = 0; p < F; ++p)
@@ -396,6 +400,55 @@ __global__ void __launch_bounds__(1024) r_xenergy_mart(const float2* __restrict_
   }
   double r = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// Same sum with the transformed tiles held in registers (V <= 4).
+template <int N, int V>
+__global__ void __launch_bounds__(xy_threads<N>()) r_xenergy_mart_r(float2* __restrict__ kbuf, const float* __restrict__ Bt,
+                                                        Geo g, const float2* __restrict__ twN,
+                                                        double* __restrict__ part) {
+  constexpr int RL = edge_radix<N>();
+  constexpr int NPQ = V * (V + 1) / 2;
+  extern __shared__ float2 smem[];
+  __shared__ double sh[16];
+  float2* tw = smem;
+  float2* tile = smem + N;
+  const int W = g.W, T = N * W;
+  const int t = blockIdx.x;
+  const STile st = xtile(t, g);
+  load_tw<N>(tw, twN);
+  __syncthreads();
+  Lines Ls{W, W, 1, 0, W};
+  float2 S[V][RL];
+#pragma unroll
+  for (int f = 0; f < V; ++f) {
+    const TileIO io = tio(kbuf + f * g.kfs, st);
+    fft_g2r<float2, N>(tile, Ls, tw, [&](int c, int kx) { return io.get(c, kx); }, S[f]);
+  }
+  const int b = threadIdx.x;
+  double acc = 0.0;
+  const int kz = st.kz0 + st.dkz * (b % W);
+  if (b < W * (N / RL)) {
+    const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
+    const float* Bb = Bt + (long)t * NPQ * T + b;
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int e = r * (N / RL) * W;
+      double s = 0.0;
+#pragma unroll
+      for (int p = 0; p < V; ++p)
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          const int lo = p < q ? p : q, hi = p < q ? q : p;
+          const int pq = lo * V - (lo * (lo - 1)) / 2 + (hi - lo);
+          s += (double)Bb[(long)pq * T + e] *
+               ((double)S[p][r].x * S[q][r].x + (double)S[p][r].y * S[q][r].y);
+        }
+      acc += w * s;
+    }
+  }
+  double rr = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = rr;
 }
 
 // Plain-path energy pieces on full complex spectra (FP64): gradient sum k^2 |E|^2, elastic B_pq Re(S_p* S_q).

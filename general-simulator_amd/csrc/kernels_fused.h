@@ -5,7 +5,7 @@
 //   real fields     f * n^3 + (x*n + y)*n + z                       (float)
 //   half spectrum   f * n*n*Pz + (kx*n + ky)*Pz + kz, kz in [0, n/2] (float2)
 //                   Pz = n/2 + 16 so every row starts on a 128-byte line.
-//   x-pass state    tile-ordered: ((t*F + f)*n + kx)*W + c, t = ky*nch + kz/W
+//   x-pass state    tile-ordered: ((t*F + f)*n + kx)*W + c, t = x-pass tile (kernels_strided.h)
 //   B tensor        tile-ordered: ((t*npq + pq)*n + kx)*W + c, packed p<=q
 //
 // Transform convention: forward R2C then C2C along y and x (unnormalised);
@@ -20,10 +20,16 @@ namespace gsk {
 struct Geo {
   int n, M, Pz;      // edge, n/2, padded row pitch
   int L;             // rows per z-pass tile
-  int W, nch;        // x/y-pass tile width and number of kz chunks
+  int W, nchm;       // x/y-pass tile width and number of full kz chunks (n/2 = nchm*W)
+  int ntile;         // x/y-pass tiles: n*nchm main tiles + n/W Nyquist tiles
   long long nvox;    // n^3
   long long kfs;     // complex elements per k-space field
 };
+
+// Index of half-spectrum element (field f, row r = kx*n + ky, kz) in the k-space buffer.
+__host__ __device__ __forceinline__ long long kidx(const Geo& g, int f, long long r, int kz) {
+  return f * g.kfs + r * g.Pz + kz;
+}
 
 // Physics constants for one launch (plain values, by value in kernel args).
 struct MartK {
@@ -42,7 +48,8 @@ struct MartK {
 struct PrecK {
   float A1, A2, A3, A4, A5, A6, A7;
   float c1, c2;
-  float cbar;        // mean concentration (fp32 copy of the fp64 value)
+  float cbar;        // mean concentration (binary32 value kept in fp64 on the host)
+  float k0;          // k = 0 mode of the concentration fluctuation
   float noise;
   float dtM, dtMkc;  // dt*M, dt*M*kappa_c
   float dtL, dtLke;  // dt*L, dt*L*kappa_eta
@@ -291,209 +298,6 @@ __global__ void __launch_bounds__(1024) k_zinv(const float2* __restrict__ kin, f
   for (int e = threadIdx.x; e < F * g.L * M; e += blockDim.x) {
     const int m = e % M, fl = e / M, l = fl % g.L, f = fl / g.L;
     *reinterpret_cast<float2*>(out + f * g.nvox + (r0 + l) * g.n + 2 * m) = tile[(long)fl * (M + 1) + m];
-  }
-}
-
-// ====================================================================== strided passes
-// Tile: all n positions along the transformed axis x W consecutive kz columns.
-// Y pass: blockIdx -> (x, chunk), element (y, c) at ((x*n + y)*Pz + kz).
-template <int N, bool FWD>
-__global__ void __launch_bounds__(1024) k_ypass(float2* __restrict__ kbuf, Geo g, int F,
-                                               const float2* __restrict__ twN) {
-  extern __shared__ float2 smem[];
-  float2* tw = smem;
-  float2* tile = smem + N;
-  const int W = g.W, M = g.M;
-  const int x = blockIdx.x / g.nch, ch = blockIdx.x % g.nch;
-  load_tw<N>(tw, twN);
-  const long rowbase = (long)x * N;
-  Lines Ls{W, W, 1, 0, W};
-  for (int f = 0; f < F; ++f) {
-    float2* src = kbuf + f * g.kfs;
-    for (int e = threadIdx.x; e < N * W; e += blockDim.x) {
-      const int y = e / W, c = e % W, kz = ch * W + c;
-      tile[e] = (kz <= M) ? src[(rowbase + y) * g.Pz + kz] : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    fft_lines<float2, N, FWD>(tile, Ls, tw);
-    for (int e = threadIdx.x; e < N * W; e += blockDim.x) {
-      const int y = e / W, c = e % W, kz = ch * W + c;
-      if (kz <= M) src[(rowbase + y) * g.Pz + kz] = tile[e];
-    }
-    __syncthreads();
-  }
-}
-
-// X pass tiles: blockIdx -> (ky, chunk) = tile t; element (kx, c) at ((kx*n + ky)*Pz + kz).
-__device__ __forceinline__ void x_load(float2* tile, const float2* __restrict__ src, const Geo& g, int N, int ky,
-                                       int ch) {
-  const int W = g.W;
-  for (int e = threadIdx.x; e < N * W; e += blockDim.x) {
-    const int kx = e / W, c = e % W, kz = ch * W + c;
-    tile[e] = (kz <= g.M) ? src[((long)kx * N + ky) * g.Pz + kz] : make_float2(0.f, 0.f);
-  }
-}
-__device__ __forceinline__ void x_store(const float2* tile, float2* __restrict__ dst, const Geo& g, int N, int ky,
-                                        int ch) {
-  const int W = g.W;
-  for (int e = threadIdx.x; e < N * W; e += blockDim.x) {
-    const int kx = e / W, c = e % W, kz = ch * W + c;
-    if (kz <= g.M) dst[((long)kx * N + ky) * g.Pz + kz] = tile[e];
-  }
-}
-
-// Martensite contraction: S_hat -> x FFT -> R_hat_p = sum_q B_pq S_hat_q / n^3 -> inverse x FFT.
-template <int N>
-__global__ void __launch_bounds__(1024) k_xcontract(float2* __restrict__ kbuf, const float* __restrict__ Bt, Geo g,
-                                                   MartK mk, const float2* __restrict__ twN) {
-  extern __shared__ float2 smem[];
-  float2* tw = smem;
-  float2* tiles = smem + N;
-  const int W = g.W, F = mk.F, T = N * W;
-  const int t = blockIdx.x, ky = t / g.nch, ch = t % g.nch;
-  load_tw<N>(tw, twN);
-  for (int f = 0; f < F; ++f) x_load(tiles + f * T, kbuf + f * g.kfs, g, N, ky, ch);
-  __syncthreads();
-  Lines Ls{W, W, 1, 0, W};
-  for (int f = 0; f < F; ++f) fft_lines<float2, N, true>(tiles + f * T, Ls, tw);
-  const float* Bb = Bt + (long)t * mk.npq * T;
-  for (int e = threadIdx.x; e < T; e += blockDim.x) {
-    float2 S[8];
-    for (int q = 0; q < F; ++q) S[q] = tiles[q * T + e];
-    for (int p = 0; p < F; ++p) {
-      float2 acc = make_float2(0.f, 0.f);
-      for (int q = 0; q < F; ++q) {
-        const int a = p < q ? p : q, b = p < q ? q : p;
-        const int pq = a * F - (a * (a - 1)) / 2 + (b - a);
-        const float bv = Bb[(long)pq * T + e];
-        acc.x += bv * S[q].x;
-        acc.y += bv * S[q].y;
-      }
-      tiles[p * T + e] = make_float2(acc.x * mk.inv_n3, acc.y * mk.inv_n3);
-    }
-  }
-  __syncthreads();
-  for (int f = 0; f < F; ++f) fft_lines<float2, N, false>(tiles + f * T, Ls, tw);
-  for (int f = 0; f < F; ++f) x_store(tiles + f * T, kbuf + f * g.kfs, g, N, ky, ch);
-}
-
-// Martensite update: F_hat -> x FFT -> eta_hat = (eta_hat - dt L F_hat)/(1 + dt L kappa k^2) -> inverse x FFT.
-// MODE 0: step update; MODE 1: initialise state from the forward transform of eta.
-template <int N, int MODE>
-__global__ void __launch_bounds__(1024) k_xupdate_mart(float2* __restrict__ kbuf, float2* __restrict__ state, Geo g,
-                                                      MartK mk, const float2* __restrict__ twN) {
-  extern __shared__ float2 smem[];
-  float2* tw = smem;
-  float2* tile = smem + N;
-  const int W = g.W, F = mk.F, T = N * W;
-  const int t = blockIdx.x, ky = t / g.nch, ch = t % g.nch;
-  load_tw<N>(tw, twN);
-  Lines Ls{W, W, 1, 0, W};
-  const float kyv = kcomp(ky, N, mk.kfac);
-  for (int f = 0; f < F; ++f) {
-    x_load(tile, kbuf + f * g.kfs, g, N, ky, ch);
-    __syncthreads();
-    fft_lines<float2, N, true>(tile, Ls, tw);
-    float2* st = state + ((long)t * F + f) * T;
-    for (int e = threadIdx.x; e < T; e += blockDim.x) {
-      if (MODE == 1) {
-        st[e] = tile[e];
-      } else {
-        const int kx = e / W, kz = ch * W + e % W;
-        const float kxv = kcomp(kx, N, mk.kfac), kzv = mk.kfac * (float)kz;
-        const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
-        const float2 fh = tile[e];
-        const float2 eh = st[e];
-        const float den = 1.f / (1.f + mk.dtLk * k2);
-        float2 nh = make_float2((eh.x - mk.dtL * fh.x) * den, (eh.y - mk.dtL * fh.y) * den);
-        st[e] = nh;
-        tile[e] = make_float2(nh.x * mk.inv_n3, nh.y * mk.inv_n3);
-      }
-    }
-    __syncthreads();
-    if (MODE == 0) {
-      fft_lines<float2, N, false>(tile, Ls, tw);
-      x_store(tile, kbuf + f * g.kfs, g, N, ky, ch);
-      __syncthreads();
-    }
-  }
-}
-
-// Precipitate update on fields (mu_c, F_1..F_V) with state (dc_hat, eta_hat_1..V).
-// c: dc_hat <- (dc_hat - dt M k^2 (mu_hat + xi B(k) dc_hat)) / (1 + dt M kappa_c k^4)
-// eta: eta_hat <- (eta_hat - dt L F_hat) / (1 + dt L kappa_eta k^2)
-// MODE 1 initialises the state (and zeroes the k = 0 mode of dc).
-template <int N, int MODE>
-__global__ void __launch_bounds__(1024) k_xupdate_prec(float2* __restrict__ kbuf, float2* __restrict__ state,
-                                                      const float* __restrict__ Bt, Geo g, PrecK pk,
-                                                      const float2* __restrict__ twN) {
-  extern __shared__ float2 smem[];
-  float2* tw = smem;
-  float2* tile = smem + N;
-  const int W = g.W, F = pk.V + 1, T = N * W;
-  const int t = blockIdx.x, ky = t / g.nch, ch = t % g.nch;
-  load_tw<N>(tw, twN);
-  Lines Ls{W, W, 1, 0, W};
-  const float kyv = kcomp(ky, N, pk.kfac);
-  const float* Bb = Bt + (long)t * T;
-  for (int f = 0; f < F; ++f) {
-    x_load(tile, kbuf + f * g.kfs, g, N, ky, ch);
-    __syncthreads();
-    fft_lines<float2, N, true>(tile, Ls, tw);
-    float2* st = state + ((long)t * F + f) * T;
-    for (int e = threadIdx.x; e < T; e += blockDim.x) {
-      const int kx = e / W, kz = ch * W + e % W;
-      if (MODE == 1) {
-        float2 v = tile[e];
-        if (f == 0 && kx == 0 && ky == 0 && kz == 0) v = make_float2(0.f, 0.f);
-        st[e] = v;
-      } else {
-        const float kxv = kcomp(kx, N, pk.kfac), kzv = pk.kfac * (float)kz;
-        const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
-        const float2 fh = tile[e];
-        const float2 sh = st[e];
-        float2 nh;
-        if (f == 0) {
-          const float bk = pk.xi * Bb[e];
-          const float mx = fh.x + (bk - pk.S) * sh.x, my = fh.y + (bk - pk.S) * sh.y;
-          const float den = 1.f / (1.f + pk.dtM * pk.S * k2 + pk.dtMkc * k2 * k2);
-          nh = make_float2((sh.x - pk.dtM * k2 * mx) * den, (sh.y - pk.dtM * k2 * my) * den);
-        } else {
-          const float den = 1.f / (1.f + pk.dtLke * k2);
-          nh = make_float2((sh.x - pk.dtL * fh.x) * den, (sh.y - pk.dtL * fh.y) * den);
-        }
-        st[e] = nh;
-        tile[e] = make_float2(nh.x * pk.inv_n3, nh.y * pk.inv_n3);
-      }
-    }
-    __syncthreads();
-    if (MODE == 0) {
-      fft_lines<float2, N, false>(tile, Ls, tw);
-      x_store(tile, kbuf + f * g.kfs, g, N, ky, ch);
-      __syncthreads();
-    }
-  }
-}
-
-// x-pass that only applies a forward or inverse C2C transform (used by the FFT test entry points).
-template <int N, bool FWD>
-__global__ void __launch_bounds__(1024) k_xpass(float2* __restrict__ kbuf, Geo g, int F, float scale,
-                                               const float2* __restrict__ twN) {
-  extern __shared__ float2 smem[];
-  float2* tw = smem;
-  float2* tile = smem + N;
-  const int W = g.W, T = N * W;
-  const int t = blockIdx.x, ky = t / g.nch, ch = t % g.nch;
-  load_tw<N>(tw, twN);
-  Lines Ls{W, W, 1, 0, W};
-  for (int f = 0; f < F; ++f) {
-    x_load(tile, kbuf + f * g.kfs, g, N, ky, ch);
-    __syncthreads();
-    fft_lines<float2, N, FWD>(tile, Ls, tw);
-    for (int e = threadIdx.x; e < T; e += blockDim.x) tile[e] = make_float2(tile[e].x * scale, tile[e].y * scale);
-    __syncthreads();
-    x_store(tile, kbuf + f * g.kfs, g, N, ky, ch);
-    __syncthreads();
   }
 }
 

@@ -208,7 +208,12 @@ class Sim:
         return self.reduce(RED_ENERGY)[:4].copy()
 
     def mass(self) -> float:
+        """Sum of c: FP64 mean times n^3 plus the k = 0 mode of the FP32 fluctuation state."""
         return float(self.reduce(RED_MASS)[0])
+
+    def mass_real(self) -> float:
+        """FP64 sum of the stored FP32 concentration field."""
+        return float(self.reduce(RED_MASS)[1])
 
     def mean_eta(self) -> np.ndarray:
         return self.reduce(RED_MEAN_ETA)[: self.nvar].copy()

@@ -51,7 +51,8 @@ extern "C" {
 #define GSK_F64 1
 /* reductions */
 #define GSK_RED_ENERGY 1    /* out[0..3] = chemical, gradient, elastic, total */
-#define GSK_RED_MASS 2      /* out[0] = sum of c over voxels                  */
+#define GSK_RED_MASS 2      /* out[0] = sum of c (FP64 mean x N + k=0 state),
+                               out[1] = FP64 sum of the stored FP32 real field */
 #define GSK_RED_MEAN_ETA 3  /* out[v] = mean of eta_v, v < nvar              */
 #define GSK_RED_SUM_ETA2 4  /* out[0] = sum over voxels and variants of eta^2 */
 
@@ -122,6 +123,11 @@ int gsk_btensor_full(int n, double dx, int kconv, const double* C81, const doubl
  * Spectrum layout [f][kx][ky][kz], kz in [0, n/2] (interleaved re, im floats). */
 int gsk_fft_r2c(int n, int count, const float* host_in, float* host_out, int device);
 int gsk_fft_c2r(int n, int count, const float* host_in, float* host_out, int device);
+
+/* Same transforms timed against rocFFT (out-of-place, single precision, batch = count).
+ * out[0] library R2C ms, out[1] rocFFT R2C ms, out[2] library C2R ms, out[3] rocFFT C2R ms,
+ * out[4] max |library - rocFFT| / max |rocFFT| over the R2C spectra. */
+int gsk_fft_compare(int n, int count, int reps, double* out, int device);
 
 const char* gsk_last_error(void);
 const char* gsk_version(void);
