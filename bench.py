@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this size once (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
@@ -139,6 +141,17 @@ def main():
     if not a.no_cpu:
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(n, V, a.cpu_steps)
+    if a.calib_copy_mib:
+        # a plain device-to-device copy of known size: reads and writes exactly this many bytes
+        nb = a.calib_copy_mib << 20
+        src = torch.empty(nb // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        total = float(src.sum())  # a read of exactly nb bytes
+        torch.cuda.synchronize()
+        result["calib_copy_bytes"] = nb
+        result["calib_sum"] = total
     print(json.dumps(result), flush=True)
 
 
