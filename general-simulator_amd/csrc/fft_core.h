@@ -304,4 +304,21 @@ __device__ __forceinline__ void fft_r2g(T2* lds, const Lines& L, const T2* tw, T
   gchain<T2, N, false, 2, 1, 1>(lds, L, tw, ld, st, vreg, InvPlan<N>{});
 }
 
+// All-LDS transform with the pass stride Ns known at compile time (constant index math);
+// forward uses the plan in order, inverse the reversed plan.
+template <typename T2, int N, bool FWD, int Ns, int R0, int... Rs>
+__device__ __forceinline__ void fft_chain_c(T2* lds, const Lines& L, const T2* tw) {
+  fft_pass<T2, N, R0, FWD>(lds, L, Ns, tw);
+  if constexpr (sizeof...(Rs) > 0) fft_chain_c<T2, N, FWD, Ns * R0, Rs...>(lds, L, tw);
+}
+template <typename T2, int N, bool FWD, int... Rs>
+__device__ __forceinline__ void fft_lines_c_(T2* lds, const Lines& L, const T2* tw, Radices<Rs...>) {
+  fft_chain_c<T2, N, FWD, 1, Rs...>(lds, L, tw);
+}
+template <typename T2, int N, bool FWD>
+__device__ __forceinline__ void fft_lines_c(T2* lds, const Lines& L, const T2* tw) {
+  if constexpr (FWD) fft_lines_c_<T2, N, true>(lds, L, tw, FwdPlan<N>{});
+  else fft_lines_c_<T2, N, false>(lds, L, tw, InvPlan<N>{});
+}
+
 }  // namespace gsk

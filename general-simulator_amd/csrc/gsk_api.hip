@@ -229,7 +229,11 @@ struct Impl {
     CHK(lds_limit((const void*)k_zfwd<M, Z_COPY>));
     CHK(lds_limit((const void*)k_zfwd<M, Z_SQUARE>));
     CHK(lds_limit((const void*)k_zfwd<M, Z_PREC_LOCAL>));
-    CHK(lds_limit((const void*)k_zmid_mart<M>));
+    CHK(lds_limit((const void*)k_zmid_mart<M, 0>));
+    CHK(lds_limit((const void*)k_zmid_mart<M, 1>));
+    CHK(lds_limit((const void*)k_zmid_mart<M, 2>));
+    CHK(lds_limit((const void*)k_zmid_mart<M, 3>));
+    CHK(lds_limit((const void*)k_zmid_mart<M, 4>));
     CHK(lds_limit((const void*)k_zinv<M>));
     CHK(lds_limit((const void*)k_ypass<N, true>));
     CHK(lds_limit((const void*)k_ypass<N, false>));
@@ -346,7 +350,13 @@ struct Impl {
     {
       const int grid = s->n * s->n / s->g.L;
       const size_t lds = (2 * M + (size_t)F * s->g.L * (M + 1)) * sizeof(float2);
-      LAUNCH((k_zmid_mart<M>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn);
+      switch (F) {  // eta prefetched into registers for up to 4 variants
+        case 1: LAUNCH((k_zmid_mart<M, 1>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
+        case 2: LAUNCH((k_zmid_mart<M, 2>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
+        case 3: LAUNCH((k_zmid_mart<M, 3>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
+        case 4: LAUNCH((k_zmid_mart<M, 4>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
+        default: LAUNCH((k_zmid_mart<M, 0>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn);
+      }
     }
     CHK(ypass(s, true, F));
     CHK(xupdate_mart<0>(s, mk));
@@ -502,16 +512,10 @@ void choose_geometry(gsk_sim* s, int Fz, int Fx_resident) {
   g.n = n; g.M = M; g.Pz = M + 16;
   g.nvox = (long long)n * n * n;
   g.kfs = (long long)n * n * g.Pz;
-  // z tiles: rows L, threads L*M/rmin <= zmax (256; GSK_ZTHREADS overrides), LDS <= 96 KB
-  const int rz = rmin_of(M);
-  int zmax = 256;
-  if (const char* e = std::getenv("GSK_ZTHREADS")) zmax = std::max(64, std::atoi(e));
-  int L = 1;
-  while (L * 2 <= n * n && (L * 2) * M / rz <= zmax &&
-         (2.0 * M + (double)Fz * (L * 2) * (M + 1)) * 8.0 <= 96 * 1024)
-    L *= 2;
+  // z tiles: the compile-time shape of the z kernels (zrows rows, zthreads threads)
+  const int L = zrows(M);
   g.L = L;
-  s->zthreads = round64(std::max(L * M / rz, 64));
+  s->zthreads = zthreads_of(M, rmin_of(M));
   // x/y tiles: width W <= 16 (GSK_W overrides) dividing n/2, threads W*n/rmin <= 1024,
   // resident tiles <= 120 KB
   const int rx = rmin_of(n);

@@ -1,0 +1,74 @@
+// FP32 fused spectral step: shared geometry, physics constants and helpers.
+//
+// Field layout in HBM (matches the reference's Data<Real> [v][x][y][z], z
+// fastest, include/datamain.th:70-76):
+//   real fields     f * n^3 + (x*n + y)*n + z                       (float)
+//   half spectrum   f * n*n*Pz + (kx*n + ky)*Pz + kz, kz in [0, n/2] (float2)
+//                   Pz = n/2 + 16 so every row starts on a 128-byte line.
+//   x-pass state    tile-ordered: ((t*F + f)*n + kx)*W + c, t = x-pass tile (kernels_strided.h)
+//   B tensor        tile-ordered: ((t*npq + pq)*n + kx)*W + c, packed p<=q
+//
+// Transform convention: forward R2C then C2C along y and x (unnormalised);
+// the inverse C2C along x and y then C2R are unnormalised too, and the single
+// 1/n^3 is applied in the x-pass between the two directions.
+#pragma once
+#include "fft_core.h"
+#include "philox.h"
+
+namespace gsk {
+
+struct Geo {
+  int n, M, Pz;      // edge, n/2, padded row pitch
+  int L;             // rows per z-pass tile
+  int W, nchm;       // x/y-pass tile width and number of full kz chunks (n/2 = nchm*W)
+  int ntile;         // x/y-pass tiles: n*nchm main tiles + n/W Nyquist tiles
+  long long nvox;    // n^3
+  long long kfs;     // complex elements per k-space field
+};
+
+// Index of half-spectrum element (field f, row r = kx*n + ky, kz) in the k-space buffer.
+__host__ __device__ __forceinline__ long long kidx(const Geo& g, int f, long long r, int kz) {
+  return f * g.kfs + r * g.Pz + kz;
+}
+
+// Physics constants for one launch (plain values, by value in kernel args).
+struct MartK {
+  float A, B, C;     // Landau
+  float two_wel;     // 2 * elastic weight
+  float noise;       // noise amplitude
+  float dtL, dtLk;   // dt*L, dt*L*kappa
+  float kfac;        // wave-number factor per index (2 pi/(n dx) or 2*3.14*dx/n)
+  float inv_n3;
+  int F;             // number of variants
+  int npq;
+  uint32_t step;
+  unsigned long long seed;
+};
+
+struct PrecK {
+  float A1, A2, A3, A4, A5, A6, A7;
+  float c1, c2;
+  float cbar;        // mean concentration (binary32 value kept in fp64 on the host)
+  float k0;          // k = 0 mode of the concentration fluctuation
+  float noise;
+  float dtM, dtMkc;  // dt*M, dt*M*kappa_c
+  float dtL, dtLke;  // dt*L, dt*L*kappa_eta
+  float xi;          // elastic weight
+  float S;           // Cahn-Hilliard stabilisation (implicit S k^2)
+  float kfac, inv_n3;
+  int V;             // number of order parameters (fields = V + 1)
+  uint32_t step;
+  unsigned long long seed;
+};
+
+__device__ __forceinline__ float kcomp(int m, int n, float kfac) { return kfac * (float)((m < n / 2) ? m : m - n); }
+
+// --------------------------------------------------------------- twiddles
+// tw[m] = exp(-2 pi i m/N) for m < N  (N-point FFT)
+// rt[k] = exp(-2 pi i k/n) for k < n/2 (R2C/C2R split)
+template <int N>
+__device__ __forceinline__ void load_tw(float2* dst, const float2* __restrict__ src) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[i];
+}
+
+}  // namespace gsk

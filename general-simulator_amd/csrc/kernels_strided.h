@@ -14,7 +14,7 @@
 // once per interior pass only, and the k-space pointwise work of the x pass runs on
 // registers between the forward and the inverse transform.
 #pragma once
-#include "kernels_fused.h"
+#include "kernels_z.h"
 
 namespace gsk {
 
