@@ -82,6 +82,7 @@ def lib() -> ctypes.CDLL:
     L.gsk_btensor_full.argtypes = [I, D, I, P, P, I, P, I, P, I]
     L.gsk_fft_r2c.argtypes = [I, I, P, P, I]
     L.gsk_fft_c2r.argtypes = [I, I, P, P, I]
+    L.gsk_fft_compare.argtypes = [I, I, I, P, I]
     _lib = L
     return L
 
@@ -270,6 +271,14 @@ def fft_c2r(X: np.ndarray, device: int = 0) -> np.ndarray:
     out = np.empty((count, n, n, n), dtype=np.float32)
     _check(lib().gsk_fft_c2r(n, count, _ptr(X), _ptr(out), device))
     return out
+
+
+def fft_compare(n: int, count: int = 1, reps: int = 10, device: int = 0) -> dict:
+    """Time the library's 3D R2C/C2R against rocFFT on the same data (ms per call of `count` fields)."""
+    out = np.zeros(8)
+    _check(lib().gsk_fft_compare(n, count, reps, _ptr(out), device))
+    return {"hand_r2c_ms": out[0], "rocfft_r2c_ms": out[1], "hand_c2r_ms": out[2], "rocfft_c2r_ms": out[3],
+            "max_rel_diff": out[4]}
 
 
 def version() -> str:

@@ -289,3 +289,12 @@ def test_resume_is_bit_exact(gpu):
     assert np.array_equal(c.get_field("eta", np.float64), full)
     for x in (a, b, c):
         x.close()
+
+
+@pytest.mark.parametrize("n", [64, 256])
+def test_fft_agrees_with_rocfft(gpu, n):
+    """The library's R2C matches rocFFT's on the same data (rocFFT is only a comparator)."""
+    say(f"fft vs rocfft n={n}")
+    r = gsk.fft_compare(n, count=1, reps=3)
+    assert r["max_rel_diff"] < 2e-6, r
+    assert r["hand_r2c_ms"] > 0 and r["rocfft_r2c_ms"] > 0

@@ -25,6 +25,17 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
+def chunked(run, state, steps, chunk=100, label=""):
+    """Advance an oracle run in chunks, printing progress between them."""
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        state = run(state, done, k)
+        done += k
+        print(f"[validate] {label} oracle {done}/{steps}", flush=True)
+    return state
+
+
 def mart_case(n, steps, path, noise=0.0, seed=0, V=3):
     C = gsk.cubic_modulus(350 / 500, 150 / 500, 100 / 500)
     eps = gsk.mart_eigenstrains(V)
@@ -41,7 +52,7 @@ def mart_case(n, steps, path, noise=0.0, seed=0, V=3):
     B = O.btensor(n, C, eps)
     p = O.mart_params(noise=noise, seed=seed)
     t0 = time.perf_counter()
-    want = O.mart_run(eta0, B, p, steps)
+    want = chunked(lambda e, s0, k: O.mart_run(e, B, p, k, step0=s0), eta0, steps, label=f"mart {n}^3")
     tc = time.perf_counter() - t0
     Ew = O.mart_energy(want, B, p)
     return {
@@ -74,7 +85,8 @@ def prec_case(n, steps, V=4):
     p = O.prec_params(A=list(cfg.a), c1=cfg.c1, c2=cfg.c2, M=1.0, L=1.0, kappa_c=1.5, kappa_eta=1.5, xi=cfg.xi,
                       S=54.0, dt=0.005, noise=0.001, seed=5)
     Bc = O.btensor(n, C, e0)[0, 0]
-    cw, hw = O.prec_run(c0, h0, Bc, p, steps)
+    cw, hw = chunked(lambda st, s0, k: O.prec_run(st[0], st[1], Bc, p, k, step0=s0), (c0, h0), steps,
+                     label=f"prec {n}^3")
     Ew = O.prec_energy(cw, hw, Bc, p)
     return {
         "n": n, "V": V, "steps": steps, "path": "fp32_fused",
@@ -126,6 +138,15 @@ def main():
         print(json.dumps(rep["config2_noise_64"]), flush=True)
         rep["config2_512"] = mart_512_property()
         print(json.dumps(rep["config2_512"]), flush=True)
+        for n in (256, 512):
+            r = gsk.fft_compare(n, count=1, reps=10)
+            r3 = gsk.fft_compare(n, count=3, reps=5)
+            alg = 3 * 8 * n**3  # three axis passes, 8 B/voxel each, per field
+            rep[f"fft_compare_{n}"] = {
+                "one_field": r, "three_fields": r3,
+                "hand_r2c_GBps_algorithmic": alg / (r["hand_r2c_ms"] / 1e3) / 1e9,
+                "rocfft_r2c_GBps_algorithmic": alg / (r["rocfft_r2c_ms"] / 1e3) / 1e9}
+            print(json.dumps({f"fft_compare_{n}": rep[f"fft_compare_{n}"]}), flush=True)
     if a.out:
         with open(a.out, "w") as f:
             json.dump(rep, f, indent=1)
