@@ -320,5 +320,11 @@ __device__ __forceinline__ void fft_lines_c(T2* lds, const Lines& L, const T2* t
   if constexpr (FWD) fft_lines_c_<T2, N, true>(lds, L, tw, FwdPlan<N>{});
   else fft_lines_c_<T2, N, false>(lds, L, tw, InvPlan<N>{});
 }
+// Same with an explicit radix plan (forward in order, inverse reversed).
+template <typename T2, int N, bool FWD, class Plan>
+__device__ __forceinline__ void fft_lines_p(T2* lds, const Lines& L, const T2* tw) {
+  if constexpr (FWD) fft_lines_c_<T2, N, true>(lds, L, tw, Plan{});
+  else fft_lines_c_<T2, N, false>(lds, L, tw, typename RevImpl<Plan, Radices<>>::type{});
+}
 
 }  // namespace gsk

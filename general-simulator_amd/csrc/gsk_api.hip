@@ -66,6 +66,36 @@ int rmin_of(int N) {
 
 int round64(int t) { return ((t + 63) / 64) * 64; }
 
+// Launch shapes of the strided passes: GSK_Y_PERFIELD gives every field its own y-pass
+// work groups; GSK_XUPD_G is the largest field group of one x-update launch.
+#ifndef GSK_Y_PERFIELD
+#define GSK_Y_PERFIELD 1
+#endif
+#ifndef GSK_XUPD_G
+#define GSK_XUPD_G 0
+#endif
+
+// fields in the next martensite x-update launch when `left` fields remain
+int xupd_group(int left) {
+  if (GSK_XUPD_G == 0) return left;  // a single launch covers every field
+  return (GSK_XUPD_G >= 3 && left >= 3) ? 3 : ((GSK_XUPD_G >= 2 && left >= 2) ? 2 : 1);
+}
+
+// block size of the z kernels for half-length M
+int zthreads_host(int M) {
+  switch (M) {
+    case 8: return zthreads<8>();
+    case 16: return zthreads<16>();
+    case 32: return zthreads<32>();
+    case 48: return zthreads<48>();
+    case 64: return zthreads<64>();
+    case 96: return zthreads<96>();
+    case 128: return zthreads<128>();
+    case 256: return zthreads<256>();
+  }
+  return 0;
+}
+
 template <typename T2>
 std::vector<T2> twiddles(int N, int denom) {
   std::vector<T2> t(N);
@@ -228,7 +258,7 @@ struct Impl {
   static int set_attrs() {
     CHK(lds_limit((const void*)k_zfwd<M, Z_COPY>));
     CHK(lds_limit((const void*)k_zfwd<M, Z_SQUARE>));
-    CHK(lds_limit((const void*)k_zfwd<M, Z_PREC_LOCAL>));
+    CHK(lds_limit((const void*)k_zfwd_local<M>));
     CHK(lds_limit((const void*)k_zmid_mart<M, 0>));
     CHK(lds_limit((const void*)k_zmid_mart<M, 1>));
     CHK(lds_limit((const void*)k_zmid_mart<M, 2>));
@@ -248,6 +278,8 @@ struct Impl {
     CHK(lds_limit((const void*)k_xupdate_mart<N, 2, 0>));
     CHK(lds_limit((const void*)k_xupdate_mart<N, 1, 1>));
     CHK(lds_limit((const void*)k_xupdate_mart<N, 2, 1>));
+    CHK(lds_limit((const void*)k_xupdate_mart<N, 3, 0>));
+    CHK(lds_limit((const void*)k_xupdate_mart<N, 3, 1>));
     CHK(lds_limit((const void*)k_xupdate_prec<N, 1, 0>));
     CHK(lds_limit((const void*)k_xupdate_prec<N, 2, 0>));
     CHK(lds_limit((const void*)k_xupdate_prec<N, 1, 1>));
@@ -265,29 +297,29 @@ struct Impl {
   }
 
   // ---- fused building blocks
+  // LDS of a z kernel holding nf field tiles
+  static size_t zlds(const gsk_sim* s, int nf) { return (2 * M + (size_t)nf * s->g.L * (M + 1)) * sizeof(float2); }
   static int zfwd(gsk_sim* s, int op, const float* in, int F) {
-    const int grid = s->n * s->n / s->g.L;
-    MartK mk = mart_consts(s);
-    PrecK pk = prec_consts(s);
-    const size_t lds = (2 * M + (size_t)F * s->g.L * (M + 1)) * sizeof(float2);
-    if (op == Z_COPY)
-      LAUNCH((k_zfwd<M, Z_COPY>), grid, s->zthreads, lds, s, in, s->kbuf, s->g, F, mk, pk, s->twM, s->rtn);
-    else if (op == Z_SQUARE)
-      LAUNCH((k_zfwd<M, Z_SQUARE>), grid, s->zthreads, lds, s, in, s->kbuf, s->g, F, mk, pk, s->twM, s->rtn);
-    else
-      LAUNCH((k_zfwd<M, Z_PREC_LOCAL>), grid, s->zthreads, lds, s, in, s->kbuf, s->g, F, mk, pk, s->twM, s->rtn);
+    const int rows = s->n * s->n / s->g.L;
+    if (op == Z_COPY) {
+      LAUNCH((k_zfwd<M, Z_COPY>), dim3(rows, F), s->zthreads, zlds(s, 1), s, in, s->kbuf, s->g, s->twM, s->rtn);
+    } else if (op == Z_SQUARE) {
+      LAUNCH((k_zfwd<M, Z_SQUARE>), dim3(rows, F), s->zthreads, zlds(s, 1), s, in, s->kbuf, s->g, s->twM, s->rtn);
+    } else {
+      PrecK pk = prec_consts(s);
+      LAUNCH((k_zfwd_local<M>), rows, s->zthreads, zlds(s, F), s, in, s->kbuf, s->g, pk, s->twM, s->rtn);
+    }
     return GSK_OK;
   }
   static int ypass(gsk_sim* s, bool fwd, int F) {
-    const int grid = s->g.ntile;
+    const dim3 grid(s->g.ntile, GSK_Y_PERFIELD ? F : 1);
     if (fwd) LAUNCH((k_ypass<N, true>), grid, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, s->twN);
     else LAUNCH((k_ypass<N, false>), grid, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, s->twN);
     return GSK_OK;
   }
   static int zinv(gsk_sim* s, float* out, int F) {
-    const int grid = s->n * s->n / s->g.L;
-    const size_t lds = (2 * M + (size_t)F * s->g.L * (M + 1)) * sizeof(float2);
-    LAUNCH((k_zinv<M>), grid, s->zthreads, lds, s, s->kbuf, out, s->g, F, s->twM, s->rtn);
+    const int rows = s->n * s->n / s->g.L;
+    LAUNCH((k_zinv<M>), dim3(rows, F), s->zthreads, zlds(s, 1), s, s->kbuf, out, s->g, s->twM, s->rtn);
     return GSK_OK;
   }
 
@@ -295,11 +327,20 @@ struct Impl {
   template <int MODE>
   static int xupdate_mart(gsk_sim* s, const MartK& mk) {
     const int gx = s->g.ntile;
-    for (int f0 = 0; f0 < s->F; f0 += 2) {
-      if (s->F - f0 >= 2)
+    if (GSK_XUPD_G == 0) {  // one field per work group, one launch
+      LAUNCH((k_xupdate_mart<N, 1, MODE>), dim3(gx, s->F), s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk,
+             s->twN, 0);
+      return GSK_OK;
+    }
+    for (int f0 = 0; f0 < s->F;) {
+      const int G = xupd_group(s->F - f0);
+      if (G == 3)
+        LAUNCH((k_xupdate_mart<N, 3, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN, f0);
+      else if (G == 2)
         LAUNCH((k_xupdate_mart<N, 2, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN, f0);
       else
         LAUNCH((k_xupdate_mart<N, 1, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN, f0);
+      f0 += G;
     }
     return GSK_OK;
   }
@@ -318,7 +359,6 @@ struct Impl {
   }
 
   static int init_state(gsk_sim* s) {
-    const int gx = s->g.ntile;
     if (s->cfg.physics == GSK_PHYS_MART) {
       CHK(zfwd(s, Z_COPY, s->fld, s->F));
       CHK(ypass(s, true, s->F));
@@ -349,13 +389,13 @@ struct Impl {
     CHK(ypass(s, false, F));
     {
       const int grid = s->n * s->n / s->g.L;
-      const size_t lds = (2 * M + (size_t)F * s->g.L * (M + 1)) * sizeof(float2);
+      const size_t lds = zlds(s, 1);
       switch (F) {  // eta prefetched into registers for up to 4 variants
-        case 1: LAUNCH((k_zmid_mart<M, 1>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
-        case 2: LAUNCH((k_zmid_mart<M, 2>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
-        case 3: LAUNCH((k_zmid_mart<M, 3>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
-        case 4: LAUNCH((k_zmid_mart<M, 4>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
-        default: LAUNCH((k_zmid_mart<M, 0>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn);
+        case 1: LAUNCH((k_zmid_mart<M, 1>), grid, GSK_ZMID_TM * s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
+        case 2: LAUNCH((k_zmid_mart<M, 2>), grid, GSK_ZMID_TM * s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
+        case 3: LAUNCH((k_zmid_mart<M, 3>), grid, GSK_ZMID_TM * s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
+        case 4: LAUNCH((k_zmid_mart<M, 4>), grid, GSK_ZMID_TM * s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
+        default: LAUNCH((k_zmid_mart<M, 0>), grid, GSK_ZMID_TM * s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn);
       }
     }
     CHK(ypass(s, true, F));
@@ -366,7 +406,6 @@ struct Impl {
   }
 
   static int step_prec_fused(gsk_sim* s) {
-    const int gx = s->g.ntile;
     const int F = s->F;
     PrecK pk = prec_consts(s);
     CHK(zfwd(s, Z_PREC_LOCAL, s->fld, F));
@@ -515,7 +554,7 @@ void choose_geometry(gsk_sim* s, int Fz, int Fx_resident) {
   // z tiles: the compile-time shape of the z kernels (zrows rows, zthreads threads)
   const int L = zrows(M);
   g.L = L;
-  s->zthreads = zthreads_of(M, rmin_of(M));
+  s->zthreads = zthreads_host(M);
   // x/y tiles: width W <= 16 (GSK_W overrides) dividing n/2, threads W*n/rmin <= 1024,
   // resident tiles <= 120 KB
   const int rx = rmin_of(n);
@@ -583,9 +622,12 @@ void build_slots(gsk_sim* s) {
     s->slots.push_back({"y_inv_R", F * 2 * hs});
     s->slots.push_back({"zmid_force", F * (2 * hs + rv)});
     s->slots.push_back({"y_fwd_F", F * 2 * hs});
-    for (int f0 = 0; f0 < F; f0 += 2) {  // one launch per group of 2 fields
-      const int c = std::min(2, F - f0);
-      s->slots.push_back({f0 == 0 ? "x_update_g0" : (f0 == 2 ? "x_update_g1" : "x_update_g2"), c * 4 * hs});
+    static const char* const xname[] = {"x_update_g0", "x_update_g1", "x_update_g2", "x_update_g3",
+                                        "x_update_g4", "x_update_g5", "x_update_g6", "x_update_g7"};
+    for (int f0 = 0, gi = 0; f0 < F; ++gi) {  // one launch per field group, as Impl::xupdate_mart
+      const int c = xupd_group(F - f0);
+      s->slots.push_back({xname[std::min(gi, 7)], c * 4 * hs});
+      f0 += c;
     }
     s->slots.push_back({"y_inv_eta", F * 2 * hs});
     s->slots.push_back({"zinv_eta", F * (hs + rv)});

@@ -16,6 +16,11 @@
 #pragma once
 #include "kernels_z.h"
 
+// GSK_XUPD_PF: the martensite x-update loads its k-space state before the forward transform
+#ifndef GSK_XUPD_PF
+#define GSK_XUPD_PF 1
+#endif
+
 namespace gsk {
 
 struct STile {
@@ -66,7 +71,9 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_ypass(float2* __restrict__ 
   load_tw<N>(tw, twN);
   __syncthreads();
   Lines Ls{W, W, 1, 0, W};
-  for (int f = 0; f < F; ++f) {
+  // a grid with gridDim.y > 1 gives each field its own work groups
+  const int fa = gridDim.y > 1 ? (int)blockIdx.y : 0, fb = gridDim.y > 1 ? fa + 1 : F;
+  for (int f = fa; f < fb; ++f) {
     const TileIO io = tio(kbuf + f * g.kfs, s);
     fft_g2g<float2, N, FWD>(
         tile, Ls, tw, [&](int c, int y) { return io.get(c, y); }, [&](int c, int y, float2 v) { io.put(c, y, v); });
@@ -209,13 +216,24 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_mart(float2* __rest
   float2* tile = smem + N;
   const int W = g.W, F = mk.F, T = N * W;
   const int t = blockIdx.x;
+  if (gridDim.y > 1) f0 += blockIdx.y * G;  // one group of G fields per blockIdx.y
   const STile s = xtile(t, g);
-  load_tw<N>(tw, twN);
-  __syncthreads();
-  Lines Ls{W, W, 1, 0, W};
   const int b = threadIdx.x;
   const bool act = b < W * (N / RL);
   const int c = b % W, j = b / W;
+  // the state this thread updates does not depend on the transform: fetch it first
+  float2 sv[GSK_XUPD_PF ? G : 1][RL];
+  if (GSK_XUPD_PF && MODE == 0 && act) {
+#pragma unroll
+    for (int gi = 0; gi < (GSK_XUPD_PF ? G : 1); ++gi) {
+      const float2* st = state + ((long)t * F + f0 + gi) * T + b;
+#pragma unroll
+      for (int r = 0; r < RL; ++r) sv[gi][r] = st[r * (N / RL) * W];
+    }
+  }
+  load_tw<N>(tw, twN);
+  __syncthreads();
+  Lines Ls{W, W, 1, 0, W};
   const float kyv = kcomp(s.ky0 + s.dky * c, N, mk.kfac);
   const float kzv = mk.kfac * (float)(s.kz0 + s.dkz * c);
   {
@@ -238,7 +256,7 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_mart(float2* __rest
               const float kxv = kcomp(j + r * (N / RL), N, mk.kfac);
               const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
               const float den = 1.f / (1.f + mk.dtLk * k2);
-              const float2 eh = st[e];
+              const float2 eh = GSK_XUPD_PF ? sv[GSK_XUPD_PF ? gi : 0][r] : st[e];
               const float2 nh = make_float2((eh.x - mk.dtL * v[gi][r].x) * den, (eh.y - mk.dtL * v[gi][r].y) * den);
               st[e] = nh;
               v[gi][r] = make_float2(nh.x * mk.inv_n3, nh.y * mk.inv_n3);

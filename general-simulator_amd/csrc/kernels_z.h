@@ -1,82 +1,168 @@
 // Contiguous-axis (z) passes of the FP32 fused step: real rows <-> half-spectrum rows.
 //
-// A tile is ZL consecutive rows (x*n + y) of every field. The complex M-point FFT
-// (M = n/2) of the packed real row runs in LDS (row pitch M+1, compile-time tile
-// shape, compile-time pass strides). The R2C / C2R split between the packed
-// M-point spectrum and the n/2+1 half-spectrum values is fused with the global
-// store / load: each thread owns the pair (k, M-k) of one row, reads or writes it
-// straight from or to HBM, so the spectrum never takes an extra LDS round trip.
-// Real rows are moved with 16-byte vector accesses (4 voxels per thread).
+// A z tile is L = zrows(M) consecutive rows (x*n + y) of one field. The complex
+// M-point FFT (M = n/2) of the packed real row runs in LDS (row pitch M+1,
+// compile-time tile shape and pass strides). The R2C / C2R split between the
+// packed M-point spectrum and the n/2+1 half-spectrum values is fused with the
+// global store / load: task (l, k), k < M/2, of a tile owns the spectrum pair
+// (k, M-k) of row l (task k = 0 owns 0, M/2 and M), and task (l, q) of the real
+// side owns the four voxels 4q..4q+3 of row l (one 16-byte access). Both task
+// sets have L*M/2 members, so one thread handles the same (l, k = q) slots on
+// both sides.
+//
+// Every thread issues all of its global loads for a tile before it touches LDS,
+// so a work group keeps L*M*8 bytes in flight while it waits; one field per LDS
+// tile keeps the footprint at about (L*(M+1) + 2M)*8 bytes so several work
+// groups share a CU. The middle martensite pass walks the variants one after the
+// other over the same LDS tile and prefetches the next variant's spectrum while
+// the current one is transformed.
 #pragma once
 #include "kernels_common.h"
 
 namespace gsk {
 
+// Compile-time shape of the z kernels. GSK_ZROWS_128: rows per tile at M = 128;
+// GSK_ZPLAN_128: radix plan of the 128-point transform (0: 16x8, 1: 8x4x4);
+// GSK_ZMID_TM / GSK_ZMID_WPE: thread multiple and waves-per-SIMD floor of the middle pass.
+#ifndef GSK_ZROWS_128
+#define GSK_ZROWS_128 8
+#endif
+#ifndef GSK_ZPLAN_128
+#define GSK_ZPLAN_128 1
+#endif
+#ifndef GSK_ZMID_TM
+#define GSK_ZMID_TM 1
+#endif
+#ifndef GSK_ZMID_WPE
+#define GSK_ZMID_WPE 0
+#endif
+
+template <int M> struct ZPlanOf {
+  using type = typename PlanOf<M>::type;
+  static constexpr int rmin = PlanOf<M>::rmin;
+};
+#if GSK_ZPLAN_128 == 1
+template <> struct ZPlanOf<128> {
+  using type = Radices<8, 4, 4>;
+  static constexpr int rmin = 4;
+};
+#endif
+
 // rows per z tile and the exact block size (also used by the host)
-__host__ __device__ constexpr int zrows(int M) { return M <= 32 ? 32 : (M <= 128 ? 16 : 8); }
+__host__ __device__ constexpr int zrows(int M) { return M <= 32 ? 32 : (M < 128 ? 16 : (M == 128 ? GSK_ZROWS_128 : 8)); }
 __host__ __device__ constexpr int zthreads_of(int M, int rmin) {
   return zrows(M) * (M / rmin) < 64 ? 64 : (zrows(M) * (M / rmin) + 63) / 64 * 64;
 }
-template <int M> constexpr int zthreads() { return zthreads_of(M, PlanOf<M>::rmin); }
+template <int M> constexpr int zthreads() { return zthreads_of(M, ZPlanOf<M>::rmin); }
 
 enum ZOp { Z_COPY = 0, Z_SQUARE = 1, Z_PREC_LOCAL = 2 };
 
-// --------------------------------------------------------------------- split + global I/O
-// half spectrum rows (global) -> packed M-point spectra Z (LDS), C2R direction
-template <int M, int L>
-__device__ __forceinline__ void z_load_c2r(float2* tile, const float2* __restrict__ kin, int F, const Geo& g,
-                                           long r0, const float2* rt) {
-  constexpr int H = M / 2 + 1;
-  for (int e = threadIdx.x; e < F * L * H; e += blockDim.x) {
-    const int k = e % H, fl = e / H, l = fl % L, f = fl / L;
-    const float2* row = kin + kidx(g, f, r0 + l, 0);
-    float2* z = tile + fl * (M + 1);
-    if (k == 0) {
-      const float a = row[0].x, b = row[M].x;
-      z[0] = make_float2(a + b, a - b);
-    } else {
-      const float2 xk = row[k], xm = row[M - k];
-      // E' = Xk + conj Xm ; O' = (Xk - conj Xm) * conj(W^k) ; Z = E' + i O'
-      const float2 E = make_float2(xk.x + xm.x, xk.y - xm.y);
-      const float2 D = make_float2(xk.x - xm.x, xk.y + xm.y);
-      const float2 w = rt[k];
-      const float2 O = cmul(D, make_float2(w.x, -w.y));
-      z[k] = make_float2(E.x - O.y, E.y + O.x);
-      if (k != M - k) z[M - k] = make_float2(E.x + O.y, -E.y + O.x);
-    }
-  }
-  __syncthreads();
+// --------------------------------------------------------------------- split formulas
+// C2R: half-spectrum pair (X_k, X_{M-k}) -> packed spectrum pair (Z_k, Z_{M-k}), w = exp(-2 pi i k/n):
+//   E' = X_k + conj X_{M-k},  O' = (X_k - conj X_{M-k}) conj(w),  Z_k = E' + i O',  Z_{M-k} = conj(E') + i conj(O')
+__device__ __forceinline__ void c2r_pair(float2 xk, float2 xm, float2 w, float2& zk, float2& zm) {
+  const float2 E = make_float2(xk.x + xm.x, xk.y - xm.y);
+  const float2 D = make_float2(xk.x - xm.x, xk.y + xm.y);
+  const float2 O = cmul(D, make_float2(w.x, -w.y));
+  zk = make_float2(E.x - O.y, E.y + O.x);
+  zm = make_float2(E.x + O.y, -E.y + O.x);
+}
+// R2C: packed pair (Z_k, Z_{M-k}) -> (X_k, X_{M-k}):
+//   E = (Z_k + conj Z_{M-k})/2,  O = (Z_k - conj Z_{M-k})/(2i),  X_k = E + w O,  X_{M-k} = conj(E - w O)
+__device__ __forceinline__ void r2c_pair(float2 zk, float2 zm, float2 w, float2& xk, float2& xm) {
+  const float2 E = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+  const float2 O = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+  const float2 WO = cmul(w, O);
+  xk = make_float2(E.x + WO.x, E.y + WO.y);
+  xm = make_float2(E.x - WO.x, -(E.y - WO.y));
 }
 
-// packed M-point spectra Z (LDS) -> half spectrum rows (global), R2C direction
-template <int M, int L>
-__device__ __forceinline__ void z_store_r2c(const float2* tile, float2* __restrict__ kout, int F, const Geo& g,
-                                            long r0, const float2* rt) {
-  constexpr int H = M / 2 + 1;
-  for (int e = threadIdx.x; e < F * L * H; e += blockDim.x) {
-    const int k = e % H, fl = e / H, l = fl % L, f = fl / L;
-    float2* row = kout + kidx(g, f, r0 + l, 0);
-    const float2* z = tile + fl * (M + 1);
-    if (k == 0) {
-      const float2 z0 = z[0];
-      row[0] = make_float2(z0.x + z0.y, 0.f);
-      row[M] = make_float2(z0.x - z0.y, 0.f);
-    } else {
-      const float2 zk = z[k], zm = z[M - k];
-      // E = (Zk + conj Zm)/2 ; O = (Zk - conj Zm)/(2i) ; X[k] = E + W^k O, X[M-k] = conj(E - W^k O)
-      const float2 E = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-      const float2 O = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-      const float2 WO = cmul(rt[k], O);
-      row[k] = make_float2(E.x + WO.x, E.y + WO.y);
-      if (k != M - k) row[M - k] = make_float2(E.x - WO.x, -(E.y - WO.y));
+// --------------------------------------------------------------------- one-field tile I/O
+// TM: work-group size as a multiple of the FFT's thread count (more threads, fewer tasks each)
+template <int M, int TM = 1>
+struct ZShape {
+  static constexpr int L = zrows(M), T = zthreads<M>() * TM, H2 = M / 2;
+  static constexpr int NT = (L * H2 + T - 1) / T;  // tasks per thread
+  static constexpr bool FULL = (L * H2) % T == 0;
+  __device__ static bool valid(int e) { return FULL || e < L * H2; }
+};
+
+// The half-spectrum values of one tile that a thread owns, held in registers.
+template <int M, int TM = 1>
+struct ZSpec {
+  using S = ZShape<M, TM>;
+  float2 a[S::NT], b[S::NT], c[S::NT];
+  // src: row 0 of the tile in the half-spectrum buffer
+  __device__ __forceinline__ void load(const float2* __restrict__ src, int Pz) {
+#pragma unroll
+    for (int i = 0; i < S::NT; ++i) {
+      const int e = threadIdx.x + i * S::T;
+      if (S::valid(e)) {
+        const int l = e / S::H2, k = e % S::H2;
+        const float2* p = src + l * Pz;
+        a[i] = p[k];
+        b[i] = p[M - k];
+        if (k == 0) c[i] = p[S::H2];
+      }
+    }
+  }
+  // C2R split into the packed spectra of the LDS tile (row pitch M+1)
+  __device__ __forceinline__ void split(float2* tile, const float2* rt) const {
+#pragma unroll
+    for (int i = 0; i < S::NT; ++i) {
+      const int e = threadIdx.x + i * S::T;
+      if (S::valid(e)) {
+        const int l = e / S::H2, k = e % S::H2;
+        float2* z = tile + l * (M + 1);
+        if (k == 0) {
+          z[0] = make_float2(a[i].x + b[i].x, a[i].x - b[i].x);
+          float2 zh, unused;
+          c2r_pair(c[i], c[i], rt[S::H2], zh, unused);
+          z[S::H2] = zh;
+        } else {
+          float2 zk, zm;
+          c2r_pair(a[i], b[i], rt[k], zk, zm);
+          z[k] = zk;
+          z[M - k] = zm;
+        }
+      }
+    }
+  }
+};
+
+// packed spectra of the LDS tile -> half-spectrum rows of one field (R2C direction)
+template <int M, int TM = 1>
+__device__ __forceinline__ void z_store_spec(const float2* tile, float2* __restrict__ dst, int Pz, const float2* rt) {
+  using S = ZShape<M, TM>;
+#pragma unroll
+  for (int i = 0; i < S::NT; ++i) {
+    const int e = threadIdx.x + i * S::T;
+    if (S::valid(e)) {
+      const int l = e / S::H2, k = e % S::H2;
+      const float2* z = tile + l * (M + 1);
+      float2* row = dst + l * Pz;
+      if (k == 0) {
+        const float2 z0 = z[0], zh = z[S::H2];
+        row[0] = make_float2(z0.x + z0.y, 0.f);
+        row[M] = make_float2(z0.x - z0.y, 0.f);
+        row[S::H2] = make_float2(zh.x, -zh.y);
+      } else {
+        float2 xk, xm;
+        r2c_pair(z[k], z[M - k], rt[k], xk, xm);
+        row[k] = xk;
+        row[M - k] = xm;
+      }
     }
   }
 }
 
-template <int M, int L, bool FWD>
-__device__ __forceinline__ void z_fft(float2* tile, int F, const float2* tw) {
-  const Lines Ls{L, L, M + 1, 0, 1};
-  for (int f = 0; f < F; ++f) fft_lines_c<float2, M, FWD>(tile + f * L * (M + 1), Ls, tw);
+// quad (l, q) of the tile: voxels 4q..4q+3 of row l
+template <int M, int TM = 1>
+__device__ __forceinline__ void z_quad(int i, int& l, int& q) {
+  using S = ZShape<M, TM>;
+  const int e = threadIdx.x + i * S::T;
+  l = e / S::H2;
+  q = e % S::H2;
 }
 
 template <int M>
@@ -89,173 +175,233 @@ __device__ __forceinline__ void z_twiddles(float2* tw, float2* rt, const float2*
   __syncthreads();
 }
 
-// --------------------------------------------------------------------- forward z pass
-// real fields -> pointwise op -> R2C -> half-spectrum rows.
-// Martensite (Z_SQUARE): eta_p -> eta_p^2.  Copy (Z_COPY): unchanged.
-// Precipitate (Z_PREC_LOCAL): fields (dc, eta_1..V) -> (mu_c, F_1..F_V), noise on eta.
+template <int M, bool FWD>
+__device__ __forceinline__ void z_fft1(float2* tile, const float2* tw) {
+  constexpr int L = zrows(M);
+  const Lines Ls{L, L, M + 1, 0, 1};
+  fft_lines_p<float2, M, FWD, typename ZPlanOf<M>::type>(tile, Ls, tw);
+}
+
+// --------------------------------------------------------------------- forward z pass (one field per block)
+// real field f = blockIdx.y -> (square) -> R2C -> half-spectrum rows.
+// OP = Z_SQUARE: eta_p -> eta_p^2;  OP = Z_COPY: unchanged.
 template <int M, int OP>
 __global__ void __launch_bounds__(zthreads<M>()) k_zfwd(const float* __restrict__ in, float2* __restrict__ kout, Geo g,
-                                                       int F, MartK mk, PrecK pk, const float2* __restrict__ twM,
-                                                       const float2* __restrict__ rtn) {
-  constexpr int L = zrows(M), Q = M / 2;  // Q quads of 4 voxels per row
+                                                       const float2* __restrict__ twM, const float2* __restrict__ rtn) {
+  using S = ZShape<M>;
+  constexpr int n = 2 * M;
   extern __shared__ float2 smem[];
   float2* tw = smem;
   float2* rt = smem + M;
   float2* tile = smem + 2 * M;
-  const long r0 = (long)blockIdx.x * L;
-  const int n = 2 * M;
+  const int f = blockIdx.y;
+  const long r0 = (long)blockIdx.x * S::L;
+  const float* src = in + f * g.nvox + r0 * n;
+  float4 v[S::NT];
+#pragma unroll
+  for (int i = 0; i < S::NT; ++i) {
+    int l, q;
+    z_quad<M>(i, l, q);
+    if (S::valid(threadIdx.x + i * S::T)) v[i] = *reinterpret_cast<const float4*>(src + l * n + 4 * q);
+  }
   z_twiddles<M>(tw, rt, twM, rtn);
-  if constexpr (OP == Z_PREC_LOCAL) {
-    for (int e = threadIdx.x; e < L * Q; e += blockDim.x) {
-      const int l = e / Q, q = e % Q;
-      const long vox = (r0 + l) * n + 4 * q;
-      float v[8][4];
-      for (int f = 0; f < F; ++f) {
-        const float4 t = *reinterpret_cast<const float4*>(in + f * g.nvox + vox);
-        v[f][0] = t.x; v[f][1] = t.y; v[f][2] = t.z; v[f][3] = t.w;
-      }
-      const int V = pk.V;
-      float out[8][4];
-      for (int i = 0; i < 4; ++i) {
-        const float c = pk.cbar + v[0][i];
-        float s2 = 0.f;
-        for (int a = 1; a <= V; ++a) s2 += v[a][i] * v[a][i];
-        out[0][i] = 0.5f * pk.A2 * s2 + pk.A1 * (c - pk.c1);
-      }
-      for (int a = 1; a <= V; ++a) {
-        float xi[4] = {0.f, 0.f, 0.f, 0.f};
-        if (pk.noise != 0.f) normal4(pk.seed, (uint32_t)(vox >> 2), (uint32_t)(a - 1), pk.step, xi);
-        for (int i = 0; i < 4; ++i) {
-          const float et = v[a][i];
-          float t2 = 0.f, t4 = 0.f;
-          for (int b = 1; b <= V; ++b) {
-            if (b == a) continue;
-            const float qq = v[b][i] * v[b][i];
-            t2 += qq;
-            t4 += qq * qq;
-          }
-          // ordered pairs i != j, both != a: (sum q)^2 - sum q^2
-          const float t22 = t2 * t2 - t4;
-          const float c = pk.cbar + v[0][i];
-          const float e2 = et * et;
-          out[a][i] = (2.f * pk.A5 * et + 4.f * pk.A6 * e2 * et) * t2 + 2.f * pk.A6 * et * t4 +
-                      2.f * pk.A7 * et * t22 + pk.A2 * et * (c - pk.c2) - pk.A3 * e2 * et + pk.A4 * e2 * e2 * et +
-                      pk.noise * xi[i];
-        }
-      }
-      for (int f = 0; f < F; ++f) {
-        float2* row = tile + (f * L + l) * (M + 1);
-        row[2 * q] = make_float2(out[f][0], out[f][1]);
-        row[2 * q + 1] = make_float2(out[f][2], out[f][3]);
-      }
-    }
-  } else {
-    for (int e = threadIdx.x; e < F * L * Q; e += blockDim.x) {
-      const int q = e % Q, fl = e / Q, l = fl % L, f = fl / L;
-      float4 t = *reinterpret_cast<const float4*>(in + f * g.nvox + (r0 + l) * n + 4 * q);
+#pragma unroll
+  for (int i = 0; i < S::NT; ++i) {
+    int l, q;
+    z_quad<M>(i, l, q);
+    if (S::valid(threadIdx.x + i * S::T)) {
+      float4 t = v[i];
       if (OP == Z_SQUARE) t = make_float4(t.x * t.x, t.y * t.y, t.z * t.z, t.w * t.w);
-      float2* row = tile + fl * (M + 1);
+      float2* row = tile + l * (M + 1);
       row[2 * q] = make_float2(t.x, t.y);
       row[2 * q + 1] = make_float2(t.z, t.w);
     }
   }
   __syncthreads();
-  z_fft<M, L, true>(tile, F, tw);
-  z_store_r2c<M, L>(tile, kout, F, g, r0, rt);
-  (void)mk;
+  z_fft1<M, true>(tile, tw);
+  z_store_spec<M>(tile, kout + f * g.kfs + r0 * g.Pz, g.Pz, rt);
+}
+
+// --------------------------------------------------------------------- inverse z pass (one field per block)
+// half-spectrum rows of field f = blockIdx.y -> C2R -> real field.
+template <int M>
+__global__ void __launch_bounds__(zthreads<M>()) k_zinv(const float2* __restrict__ kin, float* __restrict__ out, Geo g,
+                                                       const float2* __restrict__ twM, const float2* __restrict__ rtn) {
+  using S = ZShape<M>;
+  constexpr int n = 2 * M;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  float2* rt = smem + M;
+  float2* tile = smem + 2 * M;
+  const int f = blockIdx.y;
+  const long r0 = (long)blockIdx.x * S::L;
+  ZSpec<M> sp;
+  sp.load(kin + f * g.kfs + r0 * g.Pz, g.Pz);
+  z_twiddles<M>(tw, rt, twM, rtn);
+  sp.split(tile, rt);
+  __syncthreads();
+  z_fft1<M, false>(tile, tw);
+  float* dst = out + f * g.nvox + r0 * n;
+#pragma unroll
+  for (int i = 0; i < S::NT; ++i) {
+    int l, q;
+    z_quad<M>(i, l, q);
+    if (S::valid(threadIdx.x + i * S::T)) {
+      const float2* row = tile + l * (M + 1);
+      const float2 a = row[2 * q], b = row[2 * q + 1];
+      *reinterpret_cast<float4*>(dst + l * n + 4 * q) = make_float4(a.x, a.y, b.x, b.y);
+    }
+  }
 }
 
 // --------------------------------------------------------------------- martensite middle z pass
-// R_hat rows -> C2R -> F = f'(eta) + 2 w eta R + a xi (Philox in registers) -> R2C.
-// VT > 0: the variant count is a compile-time constant and eta is prefetched into
-// registers before the spectra are loaded; VT = 0: any variant count, eta read late.
+// For each variant p in turn: R_hat_p rows -> C2R -> F_p = f'(eta) + 2 w eta_p R_p + a xi (Philox in
+// registers) -> R2C, in place. VT > 0: the variant count is a compile-time constant and every
+// eta_q of the tile is prefetched into registers; VT = 0: any variant count, eta read when needed.
 template <int M, int VT>
-__global__ void __launch_bounds__(zthreads<M>()) k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta,
+__global__ void __launch_bounds__(GSK_ZMID_TM * zthreads<M>())
+#if GSK_ZMID_WPE > 0
+__attribute__((amdgpu_waves_per_eu(GSK_ZMID_WPE)))
+#endif
+k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta,
                                                             Geo g, MartK mk, const float2* __restrict__ twM,
                                                             const float2* __restrict__ rtn) {
-  constexpr int L = zrows(M), Q = M / 2, ZT = zthreads<M>();
-  constexpr int QPT = (L * Q + ZT - 1) / ZT;  // quads per thread
+  constexpr int TM = GSK_ZMID_TM;
+  using S = ZShape<M, TM>;
+  constexpr int n = 2 * M;
   constexpr int VR = VT > 0 ? VT : 1;
   extern __shared__ float2 smem[];
   float2* tw = smem;
   float2* rt = smem + M;
   float2* tile = smem + 2 * M;
-  const long r0 = (long)blockIdx.x * L;
-  const int F = VT > 0 ? VT : mk.F, n = 2 * M;
-  float4 pre[QPT][VR];
+  const long r0 = (long)blockIdx.x * S::L;
+  const int F = VT > 0 ? VT : mk.F;
+  const float* esrc = eta + r0 * n;
+  float2* kt = kbuf + r0 * g.Pz;
+  float4 pre[S::NT][VR];
   if constexpr (VT > 0) {
 #pragma unroll
-    for (int i = 0; i < QPT; ++i) {
-      const int e = threadIdx.x + i * ZT;
-      if (e < L * Q) {
-        const long vox = (r0 + e / Q) * n + 4 * (e % Q);
+    for (int i = 0; i < S::NT; ++i) {
+      int l, q;
+      z_quad<M, TM>(i, l, q);
+      if (S::valid(threadIdx.x + i * S::T)) {
 #pragma unroll
-        for (int f = 0; f < VR; ++f) pre[i][f] = *reinterpret_cast<const float4*>(eta + f * g.nvox + vox);
+        for (int f = 0; f < VR; ++f) pre[i][f] = *reinterpret_cast<const float4*>(esrc + f * g.nvox + l * n + 4 * q);
       }
     }
   }
+  ZSpec<M, TM> sp;
+  sp.load(kt, g.Pz);
   z_twiddles<M>(tw, rt, twM, rtn);
-  z_load_c2r<M, L>(tile, kbuf, F, g, r0, rt);
-  z_fft<M, L, false>(tile, F, tw);
+#pragma unroll 1
+  for (int p = 0; p < F; ++p) {
+    sp.split(tile, rt);
+    __syncthreads();
+    if (p + 1 < F) sp.load(kt + (p + 1) * g.kfs, g.Pz);
+    z_fft1<M, false>(tile, tw);
 #pragma unroll
-  for (int i = 0; i < QPT; ++i) {
-    const int e = threadIdx.x + i * ZT;
-    if (e >= L * Q) continue;
-    const int l = e / Q, q = e % Q;
-    const long vox = (r0 + l) * n + 4 * q;
-    float et[8][4];
-    for (int f = 0; f < F; ++f) {
-      const float4 t = VT > 0 ? pre[i][f < VR ? f : 0] : *reinterpret_cast<const float4*>(eta + f * g.nvox + vox);
-      et[f][0] = t.x; et[f][1] = t.y; et[f][2] = t.z; et[f][3] = t.w;
-    }
-    float s2[4];
-    for (int k = 0; k < 4; ++k) {
-      float s = 0.f;
-      for (int f = 0; f < F; ++f) s += et[f][k] * et[f][k];
-      s2[k] = s;
-    }
-    for (int f = 0; f < F; ++f) {
-      float2* row = tile + (f * L + l) * (M + 1);
+    for (int i = 0; i < S::NT; ++i) {
+      int l, q;
+      z_quad<M, TM>(i, l, q);
+      if (!S::valid(threadIdx.x + i * S::T)) continue;
+      const long vox = (r0 + l) * n + 4 * q;
+      float ep[4], s2[4] = {0.f, 0.f, 0.f, 0.f};
+      if constexpr (VT > 0) {
+#pragma unroll
+        for (int f = 0; f < VR; ++f) {
+          const float4 t = pre[i][f];
+          s2[0] += t.x * t.x; s2[1] += t.y * t.y; s2[2] += t.z * t.z; s2[3] += t.w * t.w;
+          if (f == p) { ep[0] = t.x; ep[1] = t.y; ep[2] = t.z; ep[3] = t.w; }
+        }
+      } else {
+        for (int f = 0; f < F; ++f) {
+          const float4 t = *reinterpret_cast<const float4*>(eta + f * g.nvox + vox);
+          s2[0] += t.x * t.x; s2[1] += t.y * t.y; s2[2] += t.z * t.z; s2[3] += t.w * t.w;
+          if (f == p) { ep[0] = t.x; ep[1] = t.y; ep[2] = t.z; ep[3] = t.w; }
+        }
+      }
+      float2* row = tile + l * (M + 1);
       const float2 ra = row[2 * q], rb = row[2 * q + 1];
       const float R[4] = {ra.x, ra.y, rb.x, rb.y};
       float xi[4] = {0.f, 0.f, 0.f, 0.f};
-      if (mk.noise != 0.f) normal4(mk.seed, (uint32_t)(vox >> 2), (uint32_t)f, mk.step, xi);
+      if (mk.noise != 0.f) normal4(mk.seed, (uint32_t)(vox >> 2), (uint32_t)p, mk.step, xi);
       float o[4];
+#pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const float e1 = et[f][k];
+        const float e1 = ep[k];
         o[k] = mk.A * e1 - mk.B * e1 * e1 + mk.C * e1 * s2[k] + mk.two_wel * e1 * R[k] + mk.noise * xi[k];
       }
       row[2 * q] = make_float2(o[0], o[1]);
       row[2 * q + 1] = make_float2(o[2], o[3]);
     }
+    __syncthreads();
+    z_fft1<M, true>(tile, tw);
+    z_store_spec<M, TM>(tile, kt + p * g.kfs, g.Pz, rt);
+    __syncthreads();  // the next variant's split overwrites the tile
   }
-  __syncthreads();
-  z_fft<M, L, true>(tile, F, tw);
-  z_store_r2c<M, L>(tile, kbuf, F, g, r0, rt);
 }
 
-// --------------------------------------------------------------------- inverse z pass
-// half-spectrum rows -> C2R -> real fields.
+// --------------------------------------------------------------------- precipitate forward z pass
+// All fields of a tile in LDS: (dc, eta_1..V) -> (mu_c, F_1..F_V) with noise on eta, then R2C.
 template <int M>
-__global__ void __launch_bounds__(zthreads<M>()) k_zinv(const float2* __restrict__ kin, float* __restrict__ out, Geo g,
-                                                       int F, const float2* __restrict__ twM,
-                                                       const float2* __restrict__ rtn) {
-  constexpr int L = zrows(M), Q = M / 2;
+__global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local(const float* __restrict__ in, float2* __restrict__ kout,
+                                                             Geo g, PrecK pk, const float2* __restrict__ twM,
+                                                             const float2* __restrict__ rtn) {
+  using S = ZShape<M>;
+  constexpr int L = S::L, Q = M / 2, n = 2 * M;
   extern __shared__ float2 smem[];
   float2* tw = smem;
   float2* rt = smem + M;
   float2* tile = smem + 2 * M;
   const long r0 = (long)blockIdx.x * L;
-  const int n = 2 * M;
+  const int F = pk.V + 1;
   z_twiddles<M>(tw, rt, twM, rtn);
-  z_load_c2r<M, L>(tile, kin, F, g, r0, rt);
-  z_fft<M, L, false>(tile, F, tw);
-  for (int e = threadIdx.x; e < F * L * Q; e += blockDim.x) {
-    const int q = e % Q, fl = e / Q, l = fl % L, f = fl / L;
-    const float2* row = tile + fl * (M + 1);
-    const float2 a = row[2 * q], b = row[2 * q + 1];
-    *reinterpret_cast<float4*>(out + f * g.nvox + (r0 + l) * n + 4 * q) = make_float4(a.x, a.y, b.x, b.y);
+  for (int e = threadIdx.x; e < L * Q; e += blockDim.x) {
+    const int l = e / Q, q = e % Q;
+    const long vox = (r0 + l) * n + 4 * q;
+    float v[8][4];
+    for (int f = 0; f < F; ++f) {
+      const float4 t = *reinterpret_cast<const float4*>(in + f * g.nvox + vox);
+      v[f][0] = t.x; v[f][1] = t.y; v[f][2] = t.z; v[f][3] = t.w;
+    }
+    const int V = pk.V;
+    float out[8][4];
+    for (int i = 0; i < 4; ++i) {
+      const float c = pk.cbar + v[0][i];
+      float s2 = 0.f;
+      for (int a = 1; a <= V; ++a) s2 += v[a][i] * v[a][i];
+      out[0][i] = 0.5f * pk.A2 * s2 + pk.A1 * (c - pk.c1);
+    }
+    for (int a = 1; a <= V; ++a) {
+      float xi[4] = {0.f, 0.f, 0.f, 0.f};
+      if (pk.noise != 0.f) normal4(pk.seed, (uint32_t)(vox >> 2), (uint32_t)(a - 1), pk.step, xi);
+      for (int i = 0; i < 4; ++i) {
+        const float et = v[a][i];
+        float t2 = 0.f, t4 = 0.f;
+        for (int b = 1; b <= V; ++b) {
+          if (b == a) continue;
+          const float qq = v[b][i] * v[b][i];
+          t2 += qq;
+          t4 += qq * qq;
+        }
+        // ordered pairs i != j, both != a: (sum q)^2 - sum q^2
+        const float t22 = t2 * t2 - t4;
+        const float c = pk.cbar + v[0][i];
+        const float e2 = et * et;
+        out[a][i] = (2.f * pk.A5 * et + 4.f * pk.A6 * e2 * et) * t2 + 2.f * pk.A6 * et * t4 +
+                    2.f * pk.A7 * et * t22 + pk.A2 * et * (c - pk.c2) - pk.A3 * e2 * et + pk.A4 * e2 * e2 * et +
+                    pk.noise * xi[i];
+      }
+    }
+    for (int f = 0; f < F; ++f) {
+      float2* row = tile + (f * L + l) * (M + 1);
+      row[2 * q] = make_float2(out[f][0], out[f][1]);
+      row[2 * q + 1] = make_float2(out[f][2], out[f][3]);
+    }
   }
+  __syncthreads();
+  for (int f = 0; f < F; ++f) z_fft1<M, true>(tile + f * L * (M + 1), tw);
+  for (int f = 0; f < F; ++f) z_store_spec<M>(tile + f * L * (M + 1), kout + f * g.kfs + r0 * g.Pz, g.Pz, rt);
 }
 
 }  // namespace gsk

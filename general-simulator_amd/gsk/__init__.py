@@ -19,7 +19,8 @@ from typing import Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libgsk.so")
+# GSK_LIBRARY names another build of the same library (for example one compiled with other tile shapes)
+LIB_PATH = os.environ.get("GSK_LIBRARY") or os.path.join(os.path.dirname(_HERE), "libgsk.so")
 
 GSK_OK = 0
 PHYS_MART, PHYS_PREC = 0, 1
