@@ -75,6 +75,13 @@ int round64(int t) { return ((t + 63) / 64) * 64; }
 #define GSK_XUPD_G 0
 #endif
 
+#ifndef GSK_FREV
+#define GSK_FREV 1
+#endif
+#ifndef GSK_TREV
+#define GSK_TREV 0
+#endif
+
 // fields in the next martensite x-update launch when `left` fields remain
 int xupd_group(int left) {
   if (GSK_XUPD_G == 0) return left;  // a single launch covers every field
@@ -144,6 +151,7 @@ struct gsk_sim {
   double c_k0 = 0.0;  // FP64 sum of the stored concentration fluctuation (its k = 0 mode)
   double cee0 = 0.0;  // B(k = 0) of the c-sourced tensor
   bool haveB = false, dirty = true;
+  int order = 0;  // visiting order of the last per-field launch
   long long step = 0;
   // profiling
   bool prof = false;
@@ -153,6 +161,15 @@ struct gsk_sim {
 };
 
 namespace {
+
+// Geometry for the next per-field launch: flips the field / tile visiting order each time.
+Geo geo_next(gsk_sim* s) {
+  Geo g = s->g;
+  s->order ^= 1;
+  g.frev = GSK_FREV ? s->order : 0;
+  g.trev = GSK_TREV ? s->order : 0;
+  return g;
+}
 
 size_t nvox(const gsk_sim* s) { return (size_t)s->n * s->n * s->n; }
 size_t hs_count(const gsk_sim* s) { return (size_t)s->n * s->n * (s->n / 2 + 1); }
@@ -302,9 +319,9 @@ struct Impl {
   static int zfwd(gsk_sim* s, int op, const float* in, int F) {
     const int rows = s->n * s->n / s->g.L;
     if (op == Z_COPY) {
-      LAUNCH((k_zfwd<M, Z_COPY>), dim3(rows, F), s->zthreads, zlds(s, 1), s, in, s->kbuf, s->g, s->twM, s->rtn);
+      LAUNCH((k_zfwd<M, Z_COPY>), dim3(rows, F), s->zthreads, zlds(s, 1), s, in, s->kbuf, geo_next(s), s->twM, s->rtn);
     } else if (op == Z_SQUARE) {
-      LAUNCH((k_zfwd<M, Z_SQUARE>), dim3(rows, F), s->zthreads, zlds(s, 1), s, in, s->kbuf, s->g, s->twM, s->rtn);
+      LAUNCH((k_zfwd<M, Z_SQUARE>), dim3(rows, F), s->zthreads, zlds(s, 1), s, in, s->kbuf, geo_next(s), s->twM, s->rtn);
     } else {
       PrecK pk = prec_consts(s);
       LAUNCH((k_zfwd_local<M>), rows, s->zthreads, zlds(s, F), s, in, s->kbuf, s->g, pk, s->twM, s->rtn);
@@ -313,13 +330,14 @@ struct Impl {
   }
   static int ypass(gsk_sim* s, bool fwd, int F) {
     const dim3 grid(s->g.ntile, GSK_Y_PERFIELD ? F : 1);
-    if (fwd) LAUNCH((k_ypass<N, true>), grid, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, s->twN);
-    else LAUNCH((k_ypass<N, false>), grid, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, s->twN);
+    const Geo g = GSK_Y_PERFIELD ? geo_next(s) : s->g;
+    if (fwd) LAUNCH((k_ypass<N, true>), grid, s->xthreads, s->lds_y, s, s->kbuf, g, F, s->twN);
+    else LAUNCH((k_ypass<N, false>), grid, s->xthreads, s->lds_y, s, s->kbuf, g, F, s->twN);
     return GSK_OK;
   }
   static int zinv(gsk_sim* s, float* out, int F) {
     const int rows = s->n * s->n / s->g.L;
-    LAUNCH((k_zinv<M>), dim3(rows, F), s->zthreads, zlds(s, 1), s, s->kbuf, out, s->g, s->twM, s->rtn);
+    LAUNCH((k_zinv<M>), dim3(rows, F), s->zthreads, zlds(s, 1), s, s->kbuf, out, geo_next(s), s->twM, s->rtn);
     return GSK_OK;
   }
 
@@ -328,7 +346,7 @@ struct Impl {
   static int xupdate_mart(gsk_sim* s, const MartK& mk) {
     const int gx = s->g.ntile;
     if (GSK_XUPD_G == 0) {  // one field per work group, one launch
-      LAUNCH((k_xupdate_mart<N, 1, MODE>), dim3(gx, s->F), s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk,
+      LAUNCH((k_xupdate_mart<N, 1, MODE>), dim3(gx, s->F), s->xthreads, s->lds_y, s, s->kbuf, s->state, geo_next(s), mk,
              s->twN, 0);
       return GSK_OK;
     }

@@ -24,7 +24,18 @@ struct Geo {
   int ntile;         // x/y-pass tiles: n*nchm main tiles + n/W Nyquist tiles
   long long nvox;    // n^3
   long long kfs;     // complex elements per k-space field
+  int frev, trev;    // visit fields / tiles of a per-field launch in reverse order
 };
+
+// Field and tile handled by this work group of a per-field launch (grid = tiles x fields).
+// Consecutive launches alternate the visiting order so a kernel starts on the data its
+// predecessor wrote last, which is the most likely to still sit in the last-level cache.
+__device__ __forceinline__ int geo_field(const Geo& g) {
+  return g.frev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
+}
+__device__ __forceinline__ int geo_tile(const Geo& g) {
+  return g.trev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
+}
 
 // Index of half-spectrum element (field f, row r = kx*n + ky, kz) in the k-space buffer.
 __host__ __device__ __forceinline__ long long kidx(const Geo& g, int f, long long r, int kz) {

@@ -67,12 +67,12 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_ypass(float2* __restrict__ 
   float2* tw = smem;
   float2* tile = smem + N;
   const int W = g.W;
-  const STile s = ytile(blockIdx.x, g);
+  const STile s = ytile(geo_tile(g), g);
   load_tw<N>(tw, twN);
   __syncthreads();
   Lines Ls{W, W, 1, 0, W};
   // a grid with gridDim.y > 1 gives each field its own work groups
-  const int fa = gridDim.y > 1 ? (int)blockIdx.y : 0, fb = gridDim.y > 1 ? fa + 1 : F;
+  const int fa = gridDim.y > 1 ? geo_field(g) : 0, fb = gridDim.y > 1 ? fa + 1 : F;
   for (int f = fa; f < fb; ++f) {
     const TileIO io = tio(kbuf + f * g.kfs, s);
     fft_g2g<float2, N, FWD>(
@@ -215,8 +215,8 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_mart(float2* __rest
   float2* tw = smem;
   float2* tile = smem + N;
   const int W = g.W, F = mk.F, T = N * W;
-  const int t = blockIdx.x;
-  if (gridDim.y > 1) f0 += blockIdx.y * G;  // one group of G fields per blockIdx.y
+  const int t = geo_tile(g);
+  if (gridDim.y > 1) f0 += geo_field(g) * G;  // one group of G fields per blockIdx.y
   const STile s = xtile(t, g);
   const int b = threadIdx.x;
   const bool act = b < W * (N / RL);

@@ -194,8 +194,8 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd(const float* __restrict_
   float2* tw = smem;
   float2* rt = smem + M;
   float2* tile = smem + 2 * M;
-  const int f = blockIdx.y;
-  const long r0 = (long)blockIdx.x * S::L;
+  const int f = geo_field(g);
+  const long r0 = (long)geo_tile(g) * S::L;
   const float* src = in + f * g.nvox + r0 * n;
   float4 v[S::NT];
 #pragma unroll
@@ -233,8 +233,8 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zinv(const float2* __restrict
   float2* tw = smem;
   float2* rt = smem + M;
   float2* tile = smem + 2 * M;
-  const int f = blockIdx.y;
-  const long r0 = (long)blockIdx.x * S::L;
+  const int f = geo_field(g);
+  const long r0 = (long)geo_tile(g) * S::L;
   ZSpec<M> sp;
   sp.load(kin + f * g.kfs + r0 * g.Pz, g.Pz);
   z_twiddles<M>(tw, rt, twM, rtn);
