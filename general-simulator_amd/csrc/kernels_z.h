@@ -47,6 +47,11 @@ template <> struct ZPlanOf<128> {
   static constexpr int rmin = 4;
 };
 #endif
+// a radix-4 pass halves the tasks per thread (fewer live registers in the middle pass)
+template <> struct ZPlanOf<256> {
+  using type = Radices<4, 8, 8>;
+  static constexpr int rmin = 4;
+};
 
 // rows per z tile and the exact block size (also used by the host)
 __host__ __device__ constexpr int zrows(int M) { return M <= 32 ? 32 : (M < 128 ? 16 : (M == 128 ? GSK_ZROWS_128 : 8)); }
