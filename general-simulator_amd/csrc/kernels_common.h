@@ -53,6 +53,7 @@ struct MartK {
   int F;             // number of variants
   int npq;
   uint32_t step;
+  const uint32_t* stepp;  // device step counter (read instead of `step` when set)
   unsigned long long seed;
 };
 
@@ -69,8 +70,14 @@ struct PrecK {
   float kfac, inv_n3;
   int V;             // number of order parameters (fields = V + 1)
   uint32_t step;
+  const uint32_t* stepp;  // device step counter (read instead of `step` when set)
   unsigned long long seed;
 };
+
+// Step index of the noise counter: the device counter when the launch carries one
+// (it is advanced on the device at the end of every fused step, so a captured graph
+// replays with fresh noise), else the value passed at launch.
+__device__ __forceinline__ uint32_t step_of(uint32_t step, const uint32_t* stepp) { return stepp ? *stepp : step; }
 
 __device__ __forceinline__ float kcomp(int m, int n, float kfac) { return kfac * (float)((m < n / 2) ? m : m - n); }
 

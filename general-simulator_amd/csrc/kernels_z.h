@@ -229,9 +229,12 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd(const float* __restrict_
 
 // --------------------------------------------------------------------- inverse z pass (one field per block)
 // half-spectrum rows of field f = blockIdx.y -> C2R -> real field.
+// adv (may be null): step counter advanced once by the launch (the last kernel of a fused step).
 template <int M>
 __global__ void __launch_bounds__(zthreads<M>()) k_zinv(const float2* __restrict__ kin, float* __restrict__ out, Geo g,
-                                                       const float2* __restrict__ twM, const float2* __restrict__ rtn) {
+                                                       const float2* __restrict__ twM, const float2* __restrict__ rtn,
+                                                       uint32_t* adv) {
+  if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *adv += 1u;
   using S = ZShape<M>;
   constexpr int n = 2 * M;
   extern __shared__ float2 smem[];
@@ -329,7 +332,7 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta,
       const float2 ra = row[2 * q], rb = row[2 * q + 1];
       const float R[4] = {ra.x, ra.y, rb.x, rb.y};
       float xi[4] = {0.f, 0.f, 0.f, 0.f};
-      if (mk.noise != 0.f) normal4(mk.seed, (uint32_t)(vox >> 2), (uint32_t)p, mk.step, xi);
+      if (mk.noise != 0.f) normal4(mk.seed, (uint32_t)(vox >> 2), (uint32_t)p, step_of(mk.step, mk.stepp), xi);
       float o[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -379,7 +382,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local(const float* __res
     }
     for (int a = 1; a <= V; ++a) {
       float xi[4] = {0.f, 0.f, 0.f, 0.f};
-      if (pk.noise != 0.f) normal4(pk.seed, (uint32_t)(vox >> 2), (uint32_t)(a - 1), pk.step, xi);
+      if (pk.noise != 0.f) normal4(pk.seed, (uint32_t)(vox >> 2), (uint32_t)(a - 1), step_of(pk.step, pk.stepp), xi);
       for (int i = 0; i < 4; ++i) {
         const float et = v[a][i];
         float t2 = 0.f, t4 = 0.f;

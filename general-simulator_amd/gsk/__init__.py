@@ -84,6 +84,16 @@ def lib() -> ctypes.CDLL:
     L.gsk_fft_r2c.argtypes = [I, I, P, P, I]
     L.gsk_fft_c2r.argtypes = [I, I, P, P, I]
     L.gsk_fft_compare.argtypes = [I, I, I, P, I]
+    L.gsk_philox4x32.argtypes = [P, U, P]
+    L.gsk_ensemble_create.argtypes = [P, I, ctypes.POINTER(P)]
+    L.gsk_ensemble_destroy.argtypes = [P]
+    L.gsk_ensemble_size.argtypes = [P]
+    L.gsk_ensemble_member.argtypes = [P, I, ctypes.POINTER(P)]
+    L.gsk_ensemble_build_btensor.argtypes = [P, P, P, I]
+    L.gsk_ensemble_set_schedule.argtypes = [P, I, I, I]
+    L.gsk_ensemble_get_schedule.argtypes = [P, P, P, P]
+    L.gsk_ensemble_step.argtypes = [P, I]
+    L.gsk_ensemble_autotune.argtypes = [P, I, P, I, P]
     _lib = L
     return L
 
@@ -136,17 +146,22 @@ def default_config(physics: int = PHYS_MART, **kw) -> Config:
 class Sim:
     """One phase-field simulation on one GPU (a gsk_sim handle)."""
 
-    def __init__(self, cfg: Config):
+    def __init__(self, cfg: Config, _borrowed: Optional[ctypes.c_void_p] = None, _owner=None):
         self.cfg = cfg
-        h = ctypes.c_void_p()
-        _check(lib().gsk_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._own = _borrowed is None
+        self._owner = _owner  # keeps an ensemble alive while its member is in use
+        if _borrowed is None:
+            h = ctypes.c_void_p()
+            _check(lib().gsk_create(ctypes.byref(cfg), ctypes.byref(h)))
+        else:
+            h = _borrowed
         self._h = h
         self.n, self.nvar, self.physics = cfg.n, cfg.nvar, cfg.physics
 
     def close(self) -> None:
-        if self._h:
+        if self._h and self._own:
             lib().gsk_destroy(self._h)
-            self._h = ctypes.c_void_p()
+        self._h = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -280,6 +295,85 @@ def fft_compare(n: int, count: int = 1, reps: int = 10, device: int = 0) -> dict
     _check(lib().gsk_fft_compare(n, count, reps, _ptr(out), device))
     return {"hand_r2c_ms": out[0], "rocfft_r2c_ms": out[1], "hand_c2r_ms": out[2], "rocfft_c2r_ms": out[3],
             "max_rel_diff": out[4]}
+
+
+SCHED_SEQUENTIAL, SCHED_STREAMS, SCHED_GRAPH = 0, 1, 2
+
+
+class Ensemble:
+    """Many independent simulations on one GPU stepped together (gsk_ensemble).
+
+    ``members[i]`` are ordinary :class:`Sim` objects borrowed from the ensemble.
+    """
+
+    def __init__(self, cfgs):
+        arr = (Config * len(cfgs))(*cfgs)
+        h = ctypes.c_void_p()
+        _check(lib().gsk_ensemble_create(arr, len(cfgs), ctypes.byref(h)))
+        self._h = h
+        self.members = []
+        for i, c in enumerate(cfgs):
+            m = ctypes.c_void_p()
+            _check(lib().gsk_ensemble_member(h, i, ctypes.byref(m)))
+            self.members.append(Sim(c, _borrowed=m, _owner=self))
+
+    def close(self) -> None:
+        if self._h:
+            for m in self.members:
+                m._h = ctypes.c_void_p()
+            lib().gsk_ensemble_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __len__(self) -> int:
+        return len(self.members)
+
+    def build_btensor(self, C: np.ndarray, eps: np.ndarray) -> None:
+        C = np.ascontiguousarray(C, dtype=np.float64).reshape(81)
+        eps = np.ascontiguousarray(eps, dtype=np.float64).reshape(-1, 9)
+        _check(lib().gsk_ensemble_build_btensor(self._h, _ptr(C), _ptr(eps), eps.shape[0]))
+
+    def set_schedule(self, mode: int, branches: int = 0, steps_per_graph: int = 10) -> None:
+        _check(lib().gsk_ensemble_set_schedule(self._h, mode, branches, steps_per_graph))
+
+    def schedule(self):
+        m, b, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().gsk_ensemble_get_schedule(self._h, ctypes.byref(m), ctypes.byref(b), ctypes.byref(k)))
+        return m.value, b.value, k.value
+
+    def step(self, nsteps: int = 1) -> None:
+        _check(lib().gsk_ensemble_step(self._h, nsteps))
+
+    def autotune(self, trial_steps: int = 5):
+        """Time every schedule candidate, keep the fastest; returns rows (mode, branches, steps_per_graph, ms)."""
+        tab = np.zeros((64, 4))
+        rows = ctypes.c_int()
+        _check(lib().gsk_ensemble_autotune(self._h, trial_steps, _ptr(tab), 64, ctypes.byref(rows)))
+        return [tuple(r) for r in tab[: rows.value]]
+
+
+def philox4x32(ctr, seed: int) -> np.ndarray:
+    """One Philox4x32-10 block (the library's generator) for counter ctr[4] and a 64-bit seed."""
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    _check(lib().gsk_philox4x32(_ptr(c), ctypes.c_ulonglong(seed), _ptr(out)))
+    return out
+
+
+def philox_uniform(seed: int, i: int, j: int) -> float:
+    """u in [0, 1) from word 0 of the block at counter (i, j, 0, 2): (x >> 8) * 2^-24."""
+    return float(philox4x32([i, j, 0, 2], seed)[0] >> 8) * 2.0**-24
 
 
 def version() -> str:

@@ -113,6 +113,9 @@ int gsk_kernel_times(gsk_sim* sim, int nsteps, double* ms, int maxk, int* nk, ch
 /* Algorithmic HBM bytes per step of each kernel slot (same order as gsk_kernel_times). */
 int gsk_kernel_bytes(gsk_sim* sim, double* bytes, int maxk, int* nk);
 
+/* Philox4x32-10 block on the host (the generator every noise and initial field uses):
+ * out = philox(ctr, key = {seed low word, seed high word}). For documented parameter draws. */
+int gsk_philox4x32(const unsigned int ctr[4], unsigned long long seed, unsigned int out[4]);
 /* Philox normal field (device generated, copied to host), for bit-exact checks. */
 int gsk_noise_field(unsigned long long seed, unsigned int step, unsigned int field, long long nvox, float* host_out,
                     int device);
@@ -128,6 +131,32 @@ int gsk_fft_c2r(int n, int count, const float* host_in, float* host_out, int dev
  * out[0] library R2C ms, out[1] rocFFT R2C ms, out[2] library C2R ms, out[3] rocFFT C2R ms,
  * out[4] max |library - rocFFT| / max |rocFFT| over the R2C spectra. */
 int gsk_fft_compare(int n, int count, int reps, double* out, int device);
+
+/* ---- ensembles: many independent instances stepped together on one GPU.
+ * No counterpart in the reference (GS::Run drives one plugin instance at a time,
+ * src/gs.cu:168-212); this is the batched driver of the same step. Members are ordinary
+ * handles (borrowed from the ensemble: use every gsk_* call on them except gsk_destroy)
+ * and must use the FP32 fused path. Schedules: */
+#define GSK_ENS_SEQUENTIAL 0 /* each member runs its steps alone, one after another */
+#define GSK_ENS_STREAMS 1    /* members spread over `branches` streams, launched eagerly */
+#define GSK_ENS_GRAPH 2      /* the branches captured as one hipGraph of steps_per_graph steps */
+typedef struct gsk_ensemble gsk_ensemble;
+int gsk_ensemble_create(const gsk_config* cfgs, int count, gsk_ensemble** out);
+int gsk_ensemble_destroy(gsk_ensemble* ens);
+int gsk_ensemble_size(const gsk_ensemble* ens);
+int gsk_ensemble_member(gsk_ensemble* ens, int i, gsk_sim** out);
+/* One B tensor per distinct (n, nvar, physics, dx, kconv), shared by the members. */
+int gsk_ensemble_build_btensor(gsk_ensemble* ens, const double* C81, const double* eps, int neps);
+/* branches = 0: min(members, 8). Defaults: GSK_ENS_GRAPH, 0, 10. */
+int gsk_ensemble_set_schedule(gsk_ensemble* ens, int mode, int branches, int steps_per_graph);
+int gsk_ensemble_get_schedule(const gsk_ensemble* ens, int* mode, int* branches, int* steps_per_graph);
+/* Advance every member by nsteps (synchronous). Results are identical to stepping the
+ * members one by one with gsk_step, whatever the schedule. */
+int gsk_ensemble_step(gsk_ensemble* ens, int nsteps);
+/* Time every candidate schedule over trial_steps steps, keep the fastest, and restore the
+ * members' fields, state and step counters. table (optional) receives rows of
+ * {mode, branches, steps_per_graph, ms per ensemble step}; *rows their count. */
+int gsk_ensemble_autotune(gsk_ensemble* ens, int trial_steps, double* table, int max_rows, int* rows);
 
 const char* gsk_last_error(void);
 const char* gsk_version(void);

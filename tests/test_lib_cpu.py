@@ -62,3 +62,14 @@ def test_create_without_gpu_fails_cleanly():
     rc = gsk.lib().gsk_create(ctypes.byref(c), ctypes.byref(h))
     assert rc < 0 and not h
     assert gsk.lib().gsk_last_error()
+
+
+def test_host_philox_known_answers():
+    # Random123 known-answer vectors for Philox4x32-10 (Salmon et al., SC'11)
+    assert [hex(x) for x in gsk.philox4x32([0, 0, 0, 0], 0)] == ["0x6627e8d5", "0xe169c58d", "0xbc57ac4c", "0x9b00dbd8"]
+    ones = gsk.philox4x32([0xFFFFFFFF] * 4, 0xFFFFFFFFFFFFFFFF)
+    assert [hex(x) for x in ones] == ["0x408f276d", "0x41c83b0e", "0xa20bc7c6", "0x6d5451fd"]
+    pi = gsk.philox4x32([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], 0x299F31D0A4093822)
+    assert [hex(x) for x in pi] == ["0xd16cfe09", "0x94fdcceb", "0x5001e420", "0x24126ea1"]
+    u = gsk.philox_uniform(3, 0, 0)
+    assert 0.0 <= u < 1.0
