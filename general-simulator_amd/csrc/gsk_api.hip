@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "kernels_aux.h"
+#include "kernels_adjoint.h"
 
 using namespace gsk;
 
@@ -122,6 +123,8 @@ struct Slot {
 
 }  // namespace
 
+struct AdjWS;  // adjoint workspace (checkpoints and buffers), defined with the adjoint entry points
+
 struct gsk_sim {
   gsk_config cfg{};
   int n = 0, V = 0, F = 0, dev = 0;
@@ -148,6 +151,9 @@ struct gsk_sim {
   double* rout = nullptr;
   uint32_t* dstep = nullptr;  // fused path: device copy of the step index (noise counter)
   unsigned gen = 0;           // bumped whenever a value baked into kernel arguments changes
+  BArgs bargs{};              // inputs of the last B tensor build (the adjoint rebuilds B in its own layout)
+  bool haveBargs = false;
+  AdjWS* adj = nullptr;
   bool sharedB = false;       // Bt belongs to an ensemble, not to this handle
   int rblocks = 512;
   // state
@@ -429,6 +435,48 @@ struct Impl {
     return GSK_OK;
   }
 
+  // ---- 3D transforms of F real fields <-> natural-layout half spectra in an explicit buffer
+  static int r2c_buf(gsk_sim* s, const float* in, float2* kb, int F) {
+    const int rows = s->n * s->n / s->g.L;
+    LAUNCH((k_zfwd<M, Z_COPY>), dim3(rows, F), s->zthreads, zlds(s, 1), s, in, kb, s->g, s->twM, s->rtn);
+    LAUNCH((k_ypass<N, true>), dim3(s->g.ntile, F), s->xthreads, s->lds_y, s, kb, s->g, F, s->twN);
+    LAUNCH((k_xpass<N, true>), s->g.ntile, s->xthreads, s->lds_y, s, kb, s->g, F, 1.0f, s->twN);
+    return GSK_OK;
+  }
+  static int c2r_buf(gsk_sim* s, float2* kb, float* out, int F) {
+    LAUNCH((k_xpass<N, false>), s->g.ntile, s->xthreads, s->lds_y, s, kb, s->g, F, 1.0f, s->twN);
+    LAUNCH((k_ypass<N, false>), dim3(s->g.ntile, F), s->xthreads, s->lds_y, s, kb, s->g, F, s->twN);
+    const int rows = s->n * s->n / s->g.L;
+    LAUNCH((k_zinv<M>), dim3(rows, F), s->zthreads, zlds(s, 1), s, kb, out, s->g, s->twM, s->rtn,
+           (uint32_t*)nullptr);
+    return GSK_OK;
+  }
+
+  // One reverse step of the martensite update at eta (step index k.step), eta1 = next state.
+  // lam (in: dLoss/deta1, out: dLoss/deta); partial sums into part (nb X, nb*V*V G, nb <mu,F>).
+  static int adj_step(gsk_sim* s, const float* eta, const float* eta1, float* lam, float* mu, float* rq, float2* kb,
+                      float2* k2, const float* bh, double* part, int nb, const AdjK& k) {
+    const int V = s->V;
+    const long N3 = (long)s->n * s->n * s->n;
+    CHK(r2c_buf(s, lam, kb, V));
+    CHK(r2c_buf(s, eta1, k2, V));
+    LAUNCH(k_adj_dmul, nb, 256, 0, s, kb, k2, s->g, k, part);
+    CHK(c2r_buf(s, kb, mu, V));
+    LAUNCH(k_adj_prep, nb, 256, 0, s, eta, mu, rq, V, N3);
+    CHK(r2c_buf(s, rq, kb, 2 * V));
+    double* pg = part + nb;
+    switch (V) {
+      case 1: LAUNCH((k_adj_contract<1>), nb, 256, 0, s, kb, bh, s->g, k.inv_n3, pg); break;
+      case 2: LAUNCH((k_adj_contract<2>), nb, 256, 0, s, kb, bh, s->g, k.inv_n3, pg); break;
+      case 3: LAUNCH((k_adj_contract<3>), nb, 256, 0, s, kb, bh, s->g, k.inv_n3, pg); break;
+      case 4: LAUNCH((k_adj_contract<4>), nb, 256, 0, s, kb, bh, s->g, k.inv_n3, pg); break;
+      default: return fail(GSK_ERR_UNSUPPORTED, "the adjoint supports up to 4 variants");
+    }
+    CHK(c2r_buf(s, kb, rq, 2 * V));
+    LAUNCH(k_adj_lambda, nb, 256, 0, s, eta, mu, rq, lam, N3, k, part + nb + nb * V * V);
+    return GSK_OK;
+  }
+
   static int step_prec_fused(gsk_sim* s) {
     const int F = s->F;
     PrecK pk = prec_consts(s);
@@ -555,6 +603,12 @@ template <int N> struct DX {
     if (fwd) LAUNCH((k_xpass<N, true>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, scale, s->twN);
     else LAUNCH((k_xpass<N, false>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->g, F, scale, s->twN);
     return GSK_OK;
+  }
+};
+template <int N> struct DAdj {
+  static int run(gsk_sim* s, const float* eta, const float* eta1, float* lam, float* mu, float* rq, float2* kb,
+                 float2* k2, const float* bh, double* part, int nb, const AdjK& k) {
+    return Impl<N>::adj_step(s, eta, eta1, lam, mu, rq, kb, k2, bh, part, nb, k);
   }
 };
 template <int N> struct DZinv { static int run(gsk_sim* s, float* out, int F) { return Impl<N>::zinv(s, out, F); } };
@@ -688,6 +742,8 @@ void free_dev(T*& p) {
   p = nullptr;
 }
 
+void adj_free(AdjWS* w);
+
 }  // namespace
 
 // ======================================================================== C ABI
@@ -803,6 +859,8 @@ int gsk_destroy(gsk_sim* s) {
   if (!s) return GSK_OK;
   (void)hipSetDevice(s->dev);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  adj_free(s->adj);
+  s->adj = nullptr;
   free_dev(s->twN); free_dev(s->twM); free_dev(s->rtn); free_dev(s->twN64);
   if (s->sharedB) s->Bt = nullptr;
   free_dev(s->fld); free_dev(s->kbuf); free_dev(s->state); free_dev(s->Bt); free_dev(s->dstep);
@@ -851,6 +909,8 @@ int gsk_build_btensor(gsk_sim* s, const double* C81, const double* eps, int neps
   BArgs a;
   CHK(fill_bargs(a, C81, eps, neps, eps, neps, kfac));
   s->cee0 = a.cee[0];
+  s->bargs = a;
+  s->haveBargs = true;
   if (s->cfg.path == GSK_PATH_FUSED32) {
     const long total = (long)(long)s->g.ntile * s->n * s->g.W;
     hipLaunchKernelGGL(k_btensor_tiles, dim3(grid_1d(total)), dim3(256), 0, s->stream, s->Bt, s->g, a);
@@ -1584,6 +1644,8 @@ int gsk_ensemble_build_btensor(gsk_ensemble* e, const double* C81, const double*
     s->Bt = o->Bt;
     s->sharedB = true;
     s->cee0 = o->cee0;
+    s->bargs = o->bargs;
+    s->haveBargs = o->haveBargs;
     s->haveB = true;
     s->dirty = true;
     ++s->gen;
@@ -1662,6 +1724,219 @@ int gsk_ensemble_autotune(gsk_ensemble* e, int trial_steps, double* table, int m
       table[4 * i + 2] = cand[i].spg;
       table[4 * i + 3] = cand[i].ms;
     }
+  return GSK_OK;
+}
+
+}  // extern "C"
+
+// ======================================================================== adjoint
+// Reverse mode of the FP32 martensite step with checkpointing. gsk_adjoint_forward steps
+// the handle from its current state and keeps (real field, k-space state, step) every K
+// steps plus the final state; gsk_adjoint_backward walks the segments backwards: it
+// restores a checkpoint, recomputes the segment's K states into a buffer (the restored
+// state reproduces the original trajectory bit for bit), and runs the reverse steps over
+// it. K = 1 keeps every state and recomputes nothing (store-all).
+struct AdjWS {
+  int T = 0, K = 0;
+  long long step_end = 0;
+  std::vector<float*> ck_fld;
+  std::vector<float2*> ck_state;
+  std::vector<long long> ck_step;
+  float* fin_fld = nullptr;
+  float2* fin_state = nullptr;
+  float *seg = nullptr, *lam = nullptr, *mu = nullptr, *rq = nullptr, *bh = nullptr;
+  float2 *kb = nullptr, *k2 = nullptr;
+  double* part = nullptr;
+  std::vector<double> hpart;
+  int nb = 512;
+  double ms_fwd = 0.0, ms_bwd = 0.0;
+  double bytes_ck = 0.0, bytes_seg = 0.0, bytes_work = 0.0;
+};
+
+namespace {
+
+void adj_free(AdjWS* w) {
+  if (!w) return;
+  for (float* p : w->ck_fld) (void)hipFree(p);
+  for (float2* p : w->ck_state) (void)hipFree(p);
+  free_dev(w->fin_fld); free_dev(w->fin_state);
+  free_dev(w->seg); free_dev(w->lam); free_dev(w->mu); free_dev(w->rq); free_dev(w->bh);
+  free_dev(w->kb); free_dev(w->k2); free_dev(w->part);
+  delete w;
+}
+
+// copy the evolving state (real fields, k-space state) of the handle into / out of a slot
+int adj_put(gsk_sim* s, float* fld, float2* st) {
+  HIPCHK(hipMemcpyAsync(fld, s->fld, (size_t)s->F * nvox(s) * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(st, s->state, state_count(s) * sizeof(float2), hipMemcpyDeviceToDevice, s->stream));
+  return GSK_OK;
+}
+int adj_get(gsk_sim* s, const float* fld, const float2* st, long long step) {
+  HIPCHK(hipMemcpyAsync(s->fld, fld, (size_t)s->F * nvox(s) * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(s->state, st, state_count(s) * sizeof(float2), hipMemcpyDeviceToDevice, s->stream));
+  const uint32_t v = (uint32_t)step;
+  HIPCHK(hipMemcpyAsync(s->dstep, &v, sizeof(v), hipMemcpyHostToDevice, s->stream));
+  CHK(sync(s));
+  s->step = step;
+  return GSK_OK;
+}
+
+template <typename T>
+int dev_alloc(T*& p, size_t count, double& tally) {
+  if (hipMalloc((void**)&p, count * sizeof(T)) != hipSuccess)
+    return fail(GSK_ERR_HIP, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+  tally += (double)count * sizeof(T);
+  return GSK_OK;
+}
+
+AdjK adj_consts(const gsk_sim* s, uint32_t step) {
+  const MartK m = mart_consts(s);
+  AdjK k{};
+  k.A = m.A; k.B = m.B; k.C = m.C; k.two_wel = m.two_wel; k.noise = m.noise;
+  k.dtL = m.dtL; k.dtLk = m.dtLk; k.kfac = m.kfac; k.inv_n3 = m.inv_n3;
+  k.V = s->V;
+  k.step = step;
+  k.seed = m.seed;
+  return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gsk_adjoint_forward(gsk_sim* s, int nsteps, int K) {
+  if (!s || nsteps < 1 || K < 1) return fail(GSK_ERR_ARG, "bad argument");
+  if (s->cfg.path != GSK_PATH_FUSED32 || s->cfg.physics != GSK_PHYS_MART)
+    return fail(GSK_ERR_UNSUPPORTED, "the adjoint covers the FP32 fused martensite step");
+  if (s->V > 4) return fail(GSK_ERR_UNSUPPORTED, "the adjoint supports up to 4 variants");
+  CHK(ensure_device(s->dev));
+  CHK(prepare(s));
+  adj_free(s->adj);
+  s->adj = new AdjWS();
+  AdjWS& w = *s->adj;
+  w.T = nsteps;
+  w.K = K;
+  const int nck = (nsteps + K - 1) / K;
+  const size_t fcount = (size_t)s->F * nvox(s), scount = state_count(s);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int j = 0; j < nck; ++j) {
+    float* f = nullptr;
+    float2* t = nullptr;
+    CHK(dev_alloc(f, fcount, w.bytes_ck));
+    w.ck_fld.push_back(f);
+    CHK(dev_alloc(t, scount, w.bytes_ck));
+    w.ck_state.push_back(t);
+    w.ck_step.push_back(s->step);
+    CHK(adj_put(s, f, t));
+    const int len = std::min(K, nsteps - j * K);
+    for (int i = 0; i < len; ++i) {
+      CHK(dispatch_n<DStep>(s->n, s));
+      ++s->step;
+    }
+  }
+  CHK(dev_alloc(w.fin_fld, fcount, w.bytes_ck));
+  CHK(dev_alloc(w.fin_state, scount, w.bytes_ck));
+  CHK(adj_put(s, w.fin_fld, w.fin_state));
+  CHK(sync(s));
+  w.step_end = s->step;
+  w.ms_fwd = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return GSK_OK;
+}
+
+int gsk_adjoint_backward(gsk_sim* s, const float* dL_detaT, float* grad_eta0, double* grad_params) {
+  if (!s || !dL_detaT) return fail(GSK_ERR_ARG, "null argument");
+  if (!s->adj || s->adj->T == 0) return fail(GSK_ERR_STATE, "no recorded forward run (call gsk_adjoint_forward)");
+  if (!s->haveBargs) return fail(GSK_ERR_STATE, "B tensor not built");
+  CHK(ensure_device(s->dev));
+  AdjWS& w = *s->adj;
+  const int V = s->V, K = w.K, T = w.T, nck = (T + K - 1) / K, nb = w.nb;
+  const size_t N3 = nvox(s), VN = (size_t)V * N3;
+  if (!w.lam) {
+    if (K > 1) CHK(dev_alloc(w.seg, (size_t)(K + 1) * VN, w.bytes_seg));
+    CHK(dev_alloc(w.lam, VN, w.bytes_work));
+    CHK(dev_alloc(w.mu, VN, w.bytes_work));
+    CHK(dev_alloc(w.rq, 2 * VN, w.bytes_work));
+    CHK(dev_alloc(w.kb, (size_t)(2 * V) * s->g.kfs, w.bytes_work));
+    CHK(dev_alloc(w.k2, (size_t)V * s->g.kfs, w.bytes_work));
+    CHK(dev_alloc(w.bh, (size_t)(V * (V + 1) / 2) * s->g.kfs, w.bytes_work));
+    CHK(dev_alloc(w.part, (size_t)nb * (2 + V * V), w.bytes_work));
+    w.hpart.resize((size_t)nb * (2 + V * V));
+    hipLaunchKernelGGL(k_btensor_half, dim3(grid_1d((long)s->n * s->n * (s->n / 2 + 1))), dim3(256), 0, s->stream,
+                       w.bh, s->g, s->bargs);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipMemcpyAsync(w.lam, dL_detaT, VN * sizeof(float), hipMemcpyHostToDevice, s->stream));
+  const double dt = s->cfg.dt, dtL = s->cfg.dt * s->cfg.L, kap = s->cfg.kappa, wel = s->cfg.w_el;
+  const double inv_n3 = 1.0 / (double)N3;
+  double gL = 0.0, gk = 0.0;
+  std::vector<double> gs(V, 0.0);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int j = nck - 1; j >= 0; --j) {
+    const int len = std::min(K, T - j * K);
+    CHK(adj_get(s, w.ck_fld[j], w.ck_state[j], w.ck_step[j]));
+    if (K > 1) {  // recompute the segment's states
+      HIPCHK(hipMemcpyAsync(w.seg, s->fld, VN * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+      for (int i = 1; i <= len; ++i) {
+        CHK(dispatch_n<DStep>(s->n, s));
+        ++s->step;
+        HIPCHK(hipMemcpyAsync(w.seg + (size_t)i * VN, s->fld, VN * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
+      }
+    }
+    for (int i = len - 1; i >= 0; --i) {
+      const float* eta = K > 1 ? w.seg + (size_t)i * VN : w.ck_fld[j];
+      const float* eta1 = K > 1 ? w.seg + (size_t)(i + 1) * VN : (j + 1 < nck ? w.ck_fld[j + 1] : w.fin_fld);
+      const AdjK k = adj_consts(s, (uint32_t)(w.ck_step[j] + i));
+      CHK(dispatch_n<DAdj>(s->n, s, eta, eta1, w.lam, w.mu, w.rq, w.kb, w.k2, (const float*)w.bh, w.part, nb, k));
+      HIPCHK(hipMemcpyAsync(w.hpart.data(), w.part, w.hpart.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            s->stream));
+      CHK(sync(s));
+      double X = 0.0, muF = 0.0;
+      std::vector<double> G((size_t)V * V, 0.0);
+      for (int b = 0; b < nb; ++b) {
+        X += w.hpart[b];
+        for (int r = 0; r < V * V; ++r) G[r] += w.hpart[nb + (size_t)b * V * V + r];
+        muF += w.hpart[(size_t)nb + (size_t)nb * V * V + b];
+      }
+      X *= inv_n3;
+      gL += -dt * kap * X - dt * muF;
+      gk += -dtL * X;
+      for (int r = 0; r < V; ++r)
+        for (int q = 0; q < V; ++q) {
+          const double t = -2.0 * dtL * wel * G[r * V + q] * inv_n3;
+          gs[r] += t;
+          gs[q] += t;
+        }
+    }
+  }
+  if (grad_eta0) HIPCHK(hipMemcpyAsync(grad_eta0, w.lam, VN * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+  CHK(adj_get(s, w.fin_fld, w.fin_state, w.step_end));
+  w.ms_bwd = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (grad_params) {
+    for (int v = 0; v < V; ++v) grad_params[v] = gs[v];
+    grad_params[V] = gL;
+    grad_params[V + 1] = gk;
+  }
+  return GSK_OK;
+}
+
+int gsk_adjoint_info(const gsk_sim* s, double* out) {
+  if (!s || !out) return fail(GSK_ERR_ARG, "null argument");
+  if (!s->adj) return fail(GSK_ERR_STATE, "no recorded forward run");
+  const AdjWS& w = *s->adj;
+  out[0] = w.bytes_ck;
+  out[1] = w.bytes_seg;
+  out[2] = w.bytes_work;
+  out[3] = w.ms_fwd;
+  out[4] = w.ms_bwd;
+  out[5] = (double)w.ck_fld.size() + 1.0;  // checkpoints incl. the final state
+  return GSK_OK;
+}
+
+int gsk_adjoint_release(gsk_sim* s) {
+  if (!s) return fail(GSK_ERR_ARG, "null handle");
+  CHK(ensure_device(s->dev));
+  adj_free(s->adj);
+  s->adj = nullptr;
   return GSK_OK;
 }
 

@@ -94,6 +94,10 @@ def lib() -> ctypes.CDLL:
     L.gsk_ensemble_get_schedule.argtypes = [P, P, P, P]
     L.gsk_ensemble_step.argtypes = [P, I]
     L.gsk_ensemble_autotune.argtypes = [P, I, P, I, P]
+    L.gsk_adjoint_forward.argtypes = [P, I, I]
+    L.gsk_adjoint_backward.argtypes = [P, P, P, P]
+    L.gsk_adjoint_info.argtypes = [P, P]
+    L.gsk_adjoint_release.argtypes = [P]
     _lib = L
     return L
 
@@ -255,6 +259,39 @@ class Sim:
 
 
 # ----------------------------------------------------------------------------- free functions
+def _adjoint_forward(self, nsteps: int, K: int = 20) -> None:
+    """Advance nsteps like step(), recording a checkpoint every K steps for adjoint_backward."""
+    _check(lib().gsk_adjoint_forward(self._h, nsteps, K))
+
+
+def _adjoint_backward(self, dL_detaT: np.ndarray):
+    """Reverse sweep over the recorded run: returns (dLoss/deta0, [dLoss/ds_1..s_V, dLoss/dL, dLoss/dkappa])."""
+    g = np.ascontiguousarray(dL_detaT, dtype=np.float32)
+    if g.size != int(np.prod(self._shape("eta"))):
+        raise ValueError("dL_detaT must have the shape of eta")
+    ge = np.empty(self._shape("eta"), dtype=np.float32)
+    gp = np.zeros(self.nvar + 2)
+    _check(lib().gsk_adjoint_backward(self._h, _ptr(g), _ptr(ge), _ptr(gp)))
+    return ge, gp
+
+
+def _adjoint_info(self) -> dict:
+    out = np.zeros(6)
+    _check(lib().gsk_adjoint_info(self._h, _ptr(out)))
+    return {"checkpoint_bytes": out[0], "recompute_bytes": out[1], "work_bytes": out[2], "forward_ms": out[3],
+            "backward_ms": out[4], "stored_states": int(out[5])}
+
+
+def _adjoint_release(self) -> None:
+    _check(lib().gsk_adjoint_release(self._h))
+
+
+Sim.adjoint_forward = _adjoint_forward
+Sim.adjoint_backward = _adjoint_backward
+Sim.adjoint_info = _adjoint_info
+Sim.adjoint_release = _adjoint_release
+
+
 def noise_field(seed: int, step: int, field: int, nvox: int, device: int = 0) -> np.ndarray:
     out = np.empty(nvox, dtype=np.float32)
     _check(lib().gsk_noise_field(seed, step, field, nvox, _ptr(out), device))

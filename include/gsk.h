@@ -132,6 +132,22 @@ int gsk_fft_c2r(int n, int count, const float* host_in, float* host_out, int dev
  * out[4] max |library - rocFFT| / max |rocFFT| over the R2C spectra. */
 int gsk_fft_compare(int n, int count, int reps, double* out, int device);
 
+/* ---- adjoint of the FP32 fused martensite step (nvar <= 4). No counterpart in the
+ * reference (it has no reverse mode); it differentiates the step gsk_step runs.
+ * gsk_adjoint_forward advances the handle by nsteps from its current state like gsk_step
+ * and records a checkpoint (real fields, k-space state, step) every K steps plus the final
+ * state; K = 1 stores every state. gsk_adjoint_backward takes the upstream gradient
+ * dLoss/deta(T) (nvar*n^3 floats, layout of gsk_get_field) and returns dLoss/deta(0)
+ * (grad_eta0, may be null) and grad_params = {dLoss/ds_1..s_nvar, dLoss/dL, dLoss/dkappa}
+ * (s_p scales eigenstrain p, so B_pq ~ s_p s_q; evaluated at the built tensor, s = 1).
+ * The handle is left at the final state. Noise is included (same Philox stream). */
+int gsk_adjoint_forward(gsk_sim* sim, int nsteps, int K);
+int gsk_adjoint_backward(gsk_sim* sim, const float* dL_detaT, float* grad_eta0, double* grad_params);
+/* out[0] checkpoint bytes (incl. final state), out[1] recompute-buffer bytes, out[2] other
+ * work bytes, out[3] forward ms, out[4] backward ms, out[5] stored states. */
+int gsk_adjoint_info(const gsk_sim* sim, double* out);
+int gsk_adjoint_release(gsk_sim* sim);
+
 /* ---- ensembles: many independent instances stepped together on one GPU.
  * No counterpart in the reference (GS::Run drives one plugin instance at a time,
  * src/gs.cu:168-212); this is the batched driver of the same step. Members are ordinary

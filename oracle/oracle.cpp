@@ -404,6 +404,118 @@ extern "C" void or_mart_run(int n, int V, const or_mart_params* p, const double*
   }
 }
 
+// Store-all adjoint of or_mart_run (FP64 reference gradient for the GPU adjoint; the
+// reference itself has no adjoint). One step is eta' = D * (eta - dt L F(eta)) with
+// D(k) = 1/(1 + dt L kappa k^2); for the upstream gradient lam = dLoss/deta':
+//   mu      = D * lam
+//   dLoss/deta_p = mu_p - dt L [ (A - 2B eta_p + C S + 2w R_p) mu_p + 2C eta_p sum_q eta_q mu_q
+//                               + 4w eta_p Q_p ],   Q_p = IFFT[sum_r B_pr FFT(eta_r mu_r)]/N
+//   dLoss/dL     += <lam, dD/dL D^-1 eta'> - dt <mu, F>,  dD/dL = -dt kappa k^2 D^2
+//   dLoss/dkappa += <lam, dD/dkappa D^-1 eta'>,           dD/dkappa = -dt L k^2 D^2
+//   dLoss/ds_p   += -2 dt L w sum_q (G_pq + G_qp),  G_rq = <eta_r mu_r, B_rq * eta_q^2>
+// where s_p scales eigenstrain p (B_pq proportional to s_p s_q), evaluated at s = 1.
+// g_par = {ds_1..ds_V, dL, dkappa}.
+extern "C" void or_mart_adjoint(int n, int V, const or_mart_params* p, const double* B, const double* eta0, int step0,
+                                int nsteps, const double* gT, double* g_eta0, double* g_par) {
+  const int64_t N = (int64_t)n * n * n, VN = (int64_t)V * N;
+  std::vector<double> traj((size_t)(nsteps + 1) * VN);
+  std::memcpy(traj.data(), eta0, VN * sizeof(double));
+  for (int s = 0; s < nsteps; ++s) {
+    std::memcpy(traj.data() + (s + 1) * VN, traj.data() + s * VN, VN * sizeof(double));
+    or_mart_run(n, V, p, B, traj.data() + (s + 1) * VN, step0 + s, 1);
+  }
+  const std::vector<double> k2 = ksq_table(n, p->dx, p->kconv);
+  auto kk_of = [&](int64_t i) {
+    const int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
+    return k2[x] + k2[y] + k2[z];
+  };
+  const double dtL = p->dt * p->L, w = p->w_el;
+  std::vector<double> lam(gT, gT + VN), mu(VN), F(VN), gs(V, 0.0);
+  std::vector<float> xi(N);
+  std::vector<cd> a(N), b(N), R, Q((size_t)VN), Sh((size_t)VN), Ph((size_t)VN);
+  double gL = 0.0, gk = 0.0;
+  for (int s = nsteps - 1; s >= 0; --s) {
+    const double* e = traj.data() + s * VN;
+    const double* e1 = traj.data() + (s + 1) * VN;
+    double X = 0.0;  // (1/N) sum_k Re(conj(lam_hat) eta1_hat) k^2 D
+    for (int v = 0; v < V; ++v) {
+      for (int64_t i = 0; i < N; ++i) { a[i] = cd(lam[v * N + i], 0); b[i] = cd(e1[v * N + i], 0); }
+      fft3d(n, a.data(), -1);
+      fft3d(n, b.data(), -1);
+      double x = 0.0;
+#pragma omp parallel for reduction(+ : x) schedule(static)
+      for (int64_t i = 0; i < N; ++i) {
+        const double kk = kk_of(i), D = 1.0 / (1.0 + dtL * p->kappa * kk);
+        x += (std::conj(a[i]) * b[i]).real() * kk * D;
+        a[i] *= D / (double)N;
+      }
+      X += x / (double)N;
+      fft3d(n, a.data(), +1);
+      for (int64_t i = 0; i < N; ++i) mu[v * N + i] = a[i].real();
+    }
+    gL += -p->dt * p->kappa * X;
+    gk += -dtL * X;
+    elastic_R(n, V, B, e, R);
+    // spectra of eta_q^2 and eta_r mu_r
+    for (int q = 0; q < V; ++q) {
+      for (int64_t i = 0; i < N; ++i) {
+        const double ev = e[q * N + i];
+        Sh[q * N + i] = cd(ev * ev, 0);
+        Ph[q * N + i] = cd(ev * mu[q * N + i], 0);
+      }
+      fft3d(n, Sh.data() + q * N, -1);
+      fft3d(n, Ph.data() + q * N, -1);
+    }
+    for (int r = 0; r < V; ++r)
+      for (int q = 0; q < V; ++q) {
+        double G = 0.0;
+        const double* Brq = B + (int64_t)(r * V + q) * N;
+#pragma omp parallel for reduction(+ : G) schedule(static)
+        for (int64_t i = 0; i < N; ++i) G += (std::conj(Ph[r * N + i]) * Brq[i] * Sh[q * N + i]).real();
+        G /= (double)N;
+        gs[r] += -2.0 * dtL * w * G;
+        gs[q] += -2.0 * dtL * w * G;
+      }
+    for (int pp = 0; pp < V; ++pp) {
+      for (int64_t i = 0; i < N; ++i) {
+        cd acc = 0;
+        for (int r = 0; r < V; ++r) acc += B[(int64_t)(pp * V + r) * N + i] * Ph[r * N + i];
+        Q[pp * N + i] = acc / (double)N;
+      }
+      fft3d(n, Q.data() + pp * N, +1);
+    }
+    double muF = 0.0;
+    for (int v = 0; v < V; ++v) {
+      if (p->noise != 0.0) or_noise_field(p->seed, (uint32_t)(step0 + s), (uint32_t)v, N, xi.data());
+      for (int64_t i = 0; i < N; ++i) {
+        double s2 = 0.0;
+        for (int u = 0; u < V; ++u) s2 += e[u * N + i] * e[u * N + i];
+        const double ev = e[v * N + i];
+        double f = p->A * ev - p->B * ev * ev + p->C * ev * s2 + 2.0 * w * ev * R[v * N + i].real();
+        if (p->noise != 0.0) f += p->noise * (double)xi[i];
+        F[v * N + i] = f;
+        muF += mu[v * N + i] * f;
+      }
+    }
+    gL += -p->dt * muF;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      double s2 = 0.0, t = 0.0;
+      for (int u = 0; u < V; ++u) { s2 += e[u * N + i] * e[u * N + i]; t += e[u * N + i] * mu[u * N + i]; }
+      for (int v = 0; v < V; ++v) {
+        const double ev = e[v * N + i], m = mu[v * N + i];
+        const double jt = (p->A - 2.0 * p->B * ev + p->C * s2 + 2.0 * w * R[v * N + i].real()) * m +
+                          2.0 * p->C * ev * t + 4.0 * w * ev * Q[v * N + i].real();
+        lam[v * N + i] = m - dtL * jt;
+      }
+    }
+  }
+  std::memcpy(g_eta0, lam.data(), VN * sizeof(double));
+  for (int v = 0; v < V; ++v) g_par[v] = gs[v];
+  g_par[V] = gL;
+  g_par[V + 1] = gk;
+}
+
 extern "C" void or_mart_energy(int n, int V, const or_mart_params* p, const double* B, const double* eta,
                                double out[4]) {
   const int64_t N = (int64_t)n * n * n;

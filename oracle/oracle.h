@@ -82,6 +82,11 @@ void or_mart_run(int n, int V, const or_mart_params* p, const double* B,
 /* out[0]=chemical, out[1]=gradient, out[2]=elastic, out[3]=total */
 void or_mart_energy(int n, int V, const or_mart_params* p, const double* B,
                     const double* eta, double out[4]);
+/* Store-all adjoint of or_mart_run over nsteps from eta0 (step index step0) for the upstream
+ * gradient gT = dLoss/deta(T): g_eta0 = dLoss/deta0 (V*n^3), g_par = {dLoss/ds_1..s_V
+ * (eigenstrain magnitudes, B_pq ~ s_p s_q, at s = 1), dLoss/dL, dLoss/dkappa}. */
+void or_mart_adjoint(int n, int V, const or_mart_params* p, const double* B, const double* eta0, int step0,
+                     int nsteps, const double* gT, double* g_eta0, double* g_par);
 /* Force F_p of the martensite rule (no noise) for checking against energy derivatives. */
 void or_mart_force(int n, int V, const or_mart_params* p, const double* B,
                    const double* eta, double* F);

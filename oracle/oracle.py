@@ -64,6 +64,7 @@ def lib():
         L.or_mart_run.argtypes = [I, I, ctypes.POINTER(MartParams), P, P, I, I]
         L.or_mart_energy.argtypes = [I, I, ctypes.POINTER(MartParams), P, P, P]
         L.or_mart_force.argtypes = [I, I, ctypes.POINTER(MartParams), P, P, P]
+        L.or_mart_adjoint.argtypes = [I, I, ctypes.POINTER(MartParams), P, P, I, I, P, P, P]
         L.or_prec_run.argtypes = [I, I, ctypes.POINTER(PrecParams), P, P, P, I, I]
         L.or_prec_energy.argtypes = [I, I, ctypes.POINTER(PrecParams), P, P, P, P]
         L.or_prec_local.argtypes = [I, ctypes.POINTER(PrecParams), D, P, P, P, P]
@@ -191,6 +192,18 @@ def mart_force(eta: np.ndarray, B: np.ndarray, p: MartParams) -> np.ndarray:
     F = np.empty_like(e)
     lib().or_mart_force(e.shape[1], e.shape[0], ctypes.byref(p), _p(np.ascontiguousarray(B)), _p(e), _p(F))
     return F
+
+
+def mart_adjoint(eta0: np.ndarray, B: np.ndarray, p: MartParams, nsteps: int, gT: np.ndarray, step0: int = 0):
+    """Store-all FP64 adjoint: returns (dLoss/deta0, [dLoss/ds_1..s_V, dLoss/dL, dLoss/dkappa])."""
+    e = np.ascontiguousarray(eta0, dtype=np.float64)
+    V, n = e.shape[0], e.shape[1]
+    g = np.ascontiguousarray(gT, dtype=np.float64)
+    ge = np.empty_like(e)
+    gp = np.zeros(V + 2)
+    lib().or_mart_adjoint(n, V, ctypes.byref(p), _p(np.ascontiguousarray(B, dtype=np.float64)), _p(e), step0, nsteps,
+                          _p(g), _p(ge), _p(gp))
+    return ge, gp
 
 
 def prec_run(c: np.ndarray, eta: np.ndarray, Bc: np.ndarray, p: PrecParams, nsteps: int, step0: int = 0):

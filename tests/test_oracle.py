@@ -317,3 +317,68 @@ def test_prec_reference_rule_unstable_without_stabilisation():
     # mu = A1 (c - c1) + A2/2 * 0 is linear: each mode evolves by its exact factor
     np.testing.assert_allclose(c0 - 0.25, wave * g0**10, atol=1e-14)
     np.testing.assert_allclose(cS - 0.25, wave * gS**10, atol=1e-14)
+
+
+# ------------------------------------------------------------------------- adjoint
+def _cubic(c11, c12, c44):
+    C = np.zeros((3, 3, 3, 3))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                for l in range(3):
+                    if i == j and k == l:
+                        C[i, j, k, l] = c11 if i == k else c12
+                    elif (i == k and j == l) or (i == l and j == k):
+                        C[i, j, k, l] = c44
+    return C
+
+
+def _eps(V, scale=None):
+    e = np.zeros((V, 3, 3))
+    for p in range(V):
+        e[p] = np.eye(3) * -0.04
+        e[p, p % 3, p % 3] += 0.12
+        if scale is not None:
+            e[p] *= scale[p]
+    return e
+
+
+def test_mart_adjoint_matches_central_differences():
+    """The store-all adjoint against central differences of the FP64 forward model for every
+    gradient it returns: eta0 (random direction), eigenstrain magnitudes s_p, L, kappa.
+    The target sits 1e-3 from eta(T) so the loss is small next to its gradients and the
+    differences resolve them (h = 1e-6; h = 1e-4 for the s_p gradients, ~1e-7 in size,
+    where rounding of the loss would otherwise dominate)."""
+    n, V, steps = 16, 3, 8
+    rng = np.random.default_rng(4)
+    C = _cubic(0.7, 0.3, 0.2)
+    p = O.mart_params(A=-0.3, B=3.0, C=3.0, kappa=1.2, L=0.8, dt=0.05, w_el=20.0, noise=0.01, seed=9)
+    eta0 = rng.uniform(-0.1, 0.1, (V, n, n, n))
+    target = O.mart_run(eta0, O.btensor(n, C, _eps(V)), p, steps) + 1e-3 * rng.standard_normal(eta0.shape)
+
+    def loss(e0=eta0, s=np.ones(V), **kw):
+        q = O.mart_params(**{f: getattr(p, f) for f, _ in O.MartParams._fields_})
+        for k, v in kw.items():
+            setattr(q, k, v)
+        eT = O.mart_run(e0, O.btensor(n, C, _eps(V, s)), q, steps)
+        return 0.5 * float(((eT - target) ** 2).sum()), eT
+
+    _, eT = loss()
+    g_eta, g_par = O.mart_adjoint(eta0, O.btensor(n, C, _eps(V)), p, steps, eT - target)
+    h = 1e-6
+    d = rng.standard_normal(eta0.shape)
+    fd = (loss(e0=eta0 + h * d)[0] - loss(e0=eta0 - h * d)[0]) / (2 * h)
+    assert abs(fd - float((g_eta * d).sum())) < 1e-7 * abs(fd), (fd, float((g_eta * d).sum()))
+    fd = (loss(L=p.L + h)[0] - loss(L=p.L - h)[0]) / (2 * h)
+    assert abs(fd - g_par[V]) < 1e-7 * abs(fd), (fd, g_par[V])
+    fd = (loss(kappa=p.kappa + h)[0] - loss(kappa=p.kappa - h)[0]) / (2 * h)
+    assert abs(fd - g_par[V + 1]) < 1e-7 * abs(fd), (fd, g_par[V + 1])
+    hs = 1e-4
+    for q in range(V):
+        sp, sm = np.ones(V), np.ones(V)
+        sp[q] += hs
+        sm[q] -= hs
+        fd = (loss(s=sp)[0] - loss(s=sm)[0]) / (2 * hs)
+        assert abs(fd - g_par[q]) < 1e-5 * abs(fd), (q, fd, g_par[q])
+    # every gradient is non-trivial, so a dropped term cannot hide behind a zero
+    assert np.all(np.abs(g_par) > 1e-9) and np.abs(g_eta).max() > 1e-6

@@ -113,9 +113,91 @@ def mart_512_property(steps=500):
             "energy": E.tolist(), "wall_s": dt}
 
 
+def adjoint_case(n=32, steps=200, K=20, fd=True):
+    """Config 4 physics: 3 variants, noise off; loss = 1/2 |eta(T) - eta_target|^2 with
+    eta_target from the forward model (initial field seed 7) at perturbed parameters
+    (s = 1.05/0.95/1.0, L x1.1, kappa x0.9). GPU checkpointed adjoint vs the oracle's FP64
+    store-all adjoint, and the oracle adjoint vs central differences (h = 1e-6)."""
+    V = 3
+    C = gsk.cubic_modulus(0.7, 0.3, 0.2)
+    eps = gsk.mart_eigenstrains(V)
+    cfg = gsk.default_config(gsk.PHYS_MART, n=n, nvar=V, w_el=20.0)
+    pert = dict(s=np.array([1.05, 0.95, 1.0]), L=cfg.L * 1.1, kappa=cfg.kappa * 0.9)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(C, eps)
+        s.init_uniform("eta", -0.05, 0.05, 7)
+        eta0 = s.get_field("eta")
+    p = O.mart_params(A=cfg.A, B=cfg.B, C=cfg.C, kappa=cfg.kappa, L=cfg.L, dt=cfg.dt, w_el=cfg.w_el)
+    pp = O.mart_params(A=cfg.A, B=cfg.B, C=cfg.C, kappa=pert["kappa"], L=pert["L"], dt=cfg.dt, w_el=cfg.w_el)
+    target = O.mart_run(eta0.astype(np.float64), O.btensor(n, C, eps * pert["s"][:, None, None]), pp, steps)
+    B = O.btensor(n, C, eps)
+    eT = O.mart_run(eta0.astype(np.float64), B, p, steps)
+    t0 = time.perf_counter()
+    we, wp = O.mart_adjoint(eta0.astype(np.float64), B, p, steps, eT - target)
+    t_or = time.perf_counter() - t0
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(C, eps)
+        s.set_field("eta", eta0)
+        s.adjoint_forward(steps, K)
+        gT = (s.get_field("eta", np.float64) - target).astype(np.float32)
+        ge, gp = s.adjoint_backward(gT)
+        info = s.adjoint_info()
+    out = {"n": n, "steps": steps, "K": K, "loss": 0.5 * float(((eT - target) ** 2).sum()),
+           "rel_l2_grad_eta0": rel_l2(ge, we), "grad_params_gpu": gp.tolist(), "grad_params_oracle": wp.tolist(),
+           "rel_err_params": [abs(a - b) / abs(b) for a, b in zip(gp, wp)], "oracle_adjoint_s": t_or,
+           "gpu_info": info}
+    if fd:
+        h = 1e-6
+
+        def loss(**kw):
+            q = O.mart_params(A=cfg.A, B=cfg.B, C=cfg.C, kappa=kw.get("kappa", cfg.kappa), L=kw.get("L", cfg.L),
+                              dt=cfg.dt, w_el=cfg.w_el)
+            sc = kw.get("s", np.ones(V))
+            e = O.mart_run(kw.get("e0", eta0.astype(np.float64)), O.btensor(n, C, eps * sc[:, None, None]), q, steps)
+            return 0.5 * float(((e - target) ** 2).sum())
+
+        fdv = {}
+        fdv["L"] = (loss(L=cfg.L + h) - loss(L=cfg.L - h)) / (2 * h)
+        fdv["kappa"] = (loss(kappa=cfg.kappa + h) - loss(kappa=cfg.kappa - h)) / (2 * h)
+        for q in range(V):
+            sp, sm = np.ones(V), np.ones(V)
+            sp[q] += h
+            sm[q] -= h
+            fdv[f"s{q + 1}"] = (loss(s=sp) - loss(s=sm)) / (2 * h)
+        d = np.random.default_rng(1).standard_normal(eta0.shape)
+        fdv["eta0_dir"] = (loss(e0=eta0 + h * d) - loss(e0=eta0 - h * d)) / (2 * h)
+        adj = {"L": wp[V], "kappa": wp[V + 1], "eta0_dir": float((we * d).sum())}
+        adj.update({f"s{q + 1}": wp[q] for q in range(V)})
+        out["fd_vs_oracle_adjoint_rel"] = {k: abs(fdv[k] - adj[k]) / abs(fdv[k]) for k in fdv}
+    return out
+
+
+def adjoint_memory_case(n=128, steps=200):
+    """Config 4 at full size: checkpointed (K = 20) vs store-all (K = 1) memory and time."""
+    V = 3
+    C = gsk.cubic_modulus(0.7, 0.3, 0.2)
+    cfg = gsk.default_config(gsk.PHYS_MART, n=n, nvar=V, w_el=20.0)
+    res = {}
+    grads = {}
+    for K in (20, 1):
+        with gsk.Sim(cfg) as s:
+            s.build_btensor(C, gsk.mart_eigenstrains(V))
+            s.init_uniform("eta", -0.05, 0.05, 7)
+            s.adjoint_forward(steps, K)
+            gT = s.get_field("eta")  # d/deta(T) of 1/2 |eta(T)|^2
+            ge, gp = s.adjoint_backward(gT)
+            res[f"K{K}"] = s.adjoint_info()
+            grads[K] = (ge, gp)
+        print(f"[validate] adjoint {n}^3 K={K}: {res[f'K{K}']}", flush=True)
+    res["bit_identical_K20_vs_K1"] = bool(np.array_equal(grads[20][0], grads[1][0]) and
+                                          np.array_equal(grads[20][1], grads[1][1]))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--only", default="0,1,2")
+    ap.add_argument("--only", default="0,1,2,4")
+    ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--out", default="")
     ap.add_argument("--threads", type=int, default=min(os.cpu_count() or 1, 16))
     a = ap.parse_args()
@@ -147,6 +229,12 @@ def main():
                 "hand_r2c_GBps_algorithmic": alg / (r["hand_r2c_ms"] / 1e3) / 1e9,
                 "rocfft_r2c_GBps_algorithmic": alg / (r["rocfft_r2c_ms"] / 1e3) / 1e9}
             print(json.dumps({f"fft_compare_{n}": rep[f"fft_compare_{n}"]}), flush=True)
+    if 4 in only:
+        print("[validate] config 4: adjoint at 32^3 vs oracle, FD; memory/time at 128^3", flush=True)
+        rep["config4_adjoint_32"] = adjoint_case(32, a.adj_steps)
+        print(json.dumps(rep["config4_adjoint_32"]), flush=True)
+        rep["config4_memory_128"] = adjoint_memory_case(128, 200)
+        print(json.dumps(rep["config4_memory_128"]), flush=True)
     if a.out:
         with open(a.out, "w") as f:
             json.dump(rep, f, indent=1)
