@@ -15,6 +15,7 @@
 
 #include "kernels_aux.h"
 #include "kernels_adjoint.h"
+#include "kernels_ref.h"
 
 using namespace gsk;
 
@@ -154,6 +155,12 @@ struct gsk_sim {
   BArgs bargs{};              // inputs of the last B tensor build (the adjoint rebuilds B in its own layout)
   bool haveBargs = false;
   AdjWS* adj = nullptr;
+  // reference rules
+  float* rbuf = nullptr;      // V real fields: elastic R, then the new eta (martensite)
+  float* defect = nullptr;    // n^3 blocking field (martensite reference rule), optional
+  float* Kd = nullptr;        // V*n^3 dislocation force constants, optional
+  double eps[72] = {};        // eigenstrains of the last B build
+  bool haveEps = false;
   bool sharedB = false;       // Bt belongs to an ensemble, not to this handle
   int rblocks = 512;
   // state
@@ -233,7 +240,8 @@ PrecK prec_consts(const gsk_sim* s) {
   k.dtL = (float)(c.dt * c.L);
   k.dtLke = (float)(c.dt * c.L * c.kappa);
   k.xi = (float)c.xi;
-  k.S = (float)c.S;
+  k.S = c.rule == GSK_RULE_REFERENCE ? 0.f : (float)c.S;
+  k.celastic = c.rule == GSK_RULE_REFERENCE ? 0 : 1;
   k.kfac = (float)(c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx));
   k.inv_n3 = (float)(1.0 / ((double)s->n * s->n * s->n));
   k.V = s->V;
@@ -336,7 +344,10 @@ struct Impl {
       LAUNCH((k_zfwd<M, Z_SQUARE>), dim3(rows, F), s->zthreads, zlds(s, 1), s, in, s->kbuf, geo_next(s), s->twM, s->rtn);
     } else {
       PrecK pk = prec_consts(s);
-      LAUNCH((k_zfwd_local<M>), rows, s->zthreads, zlds(s, F), s, in, s->kbuf, s->g, pk, s->twM, s->rtn);
+      const bool ref = s->cfg.rule == GSK_RULE_REFERENCE;
+      LAUNCH((k_zfwd_local<M>), rows, s->zthreads, zlds(s, F), s, in, s->kbuf, s->g, pk, s->twM, s->rtn,
+             ref ? (const float*)s->rbuf : (const float*)nullptr, ref ? (float)(2.0 * s->cfg.xi) : 0.f,
+             ref ? (float)s->cfg.noise : 0.f);
     }
     return GSK_OK;
   }
@@ -477,6 +488,52 @@ struct Impl {
     return GSK_OK;
   }
 
+  // R_p = IFFT[sum_q B_pq FFT(eta_q^2)]/N of the V fields at eta into out (real, V fields)
+  static int elastic_R_real(gsk_sim* s, const float* eta, float* out) {
+    const int V = s->V, gx = s->g.ntile;
+    MartK mk = mart_consts(s);
+    CHK(zfwd(s, Z_SQUARE, eta, V));
+    CHK(ypass(s, true, V));
+    switch (V) {
+      case 1: LAUNCH((k_xcontract_r<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
+      case 2: LAUNCH((k_xcontract_r<N, 2>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
+      case 3: LAUNCH((k_xcontract_r<N, 3>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
+      case 4: LAUNCH((k_xcontract_r<N, 4>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
+      default: LAUNCH((k_xcontract<N>), gx, s->xthreads, s->lds_xc, s, s->kbuf, s->Bt, s->g, mk, s->twN);
+    }
+    CHK(ypass(s, false, V));
+    CHK(zinv(s, out, V));
+    return GSK_OK;
+  }
+
+  static int step_mart_ref(gsk_sim* s) {
+    const gsk_config& c = s->cfg;
+    CHK(elastic_R_real(s, s->fld, s->rbuf));
+    RefMartK k{};
+    k.a1 = (float)(0.02 * (c.temperature - c.T0));
+    k.a2 = (float)c.B;
+    k.a3 = (float)c.C;
+    k.sc = (float)std::sqrt(std::fabs((0.02 * (c.temperature - c.T0)) / (c.B - c.C)));
+    k.wG = (float)c.w_grad; k.wC = (float)c.w_chem; k.wE = (float)c.w_el; k.wD = (float)c.w_dis;
+    k.wN = (float)c.noise; k.dt = (float)c.dt;
+    k.g = (float)(1.0 / (2.0 * c.dx) / 3.0);
+    k.V = s->V;
+    k.chem_form = c.chem_form;
+    k.step = (uint32_t)s->step;
+    k.stepp = s->dstep;
+    k.seed = c.seed;
+    const long N3 = (long)s->n * s->n * s->n;
+    LAUNCH(k_ref_mart, grid_1d(N3 / 4), 256, 0, s, s->fld, s->rbuf, s->defect, s->Kd, s->n, k);
+    LAUNCH(k_copy4, grid_1d((long)s->V * N3 / 4), 256, 0, s, s->fld, s->rbuf, (long)s->V * N3);
+    LAUNCH(k_advance, 1, 64, 0, s, s->dstep);
+    return GSK_OK;
+  }
+
+  static int step_prec_ref(gsk_sim* s) {
+    CHK(elastic_R_real(s, s->fld + nvox(s), s->rbuf));
+    return step_prec_fused(s);
+  }
+
   static int step_prec_fused(gsk_sim* s) {
     const int F = s->F;
     PrecK pk = prec_consts(s);
@@ -587,6 +644,8 @@ template <int N> struct DInit { static int run(gsk_sim* s) { return Impl<N>::ini
 template <int N> struct DStep {
   static int run(gsk_sim* s) {
     if (s->cfg.path == GSK_PATH_PLAIN64) return Impl<N>::step_mart_plain(s);
+    if (s->cfg.rule == GSK_RULE_REFERENCE)
+      return s->cfg.physics == GSK_PHYS_MART ? Impl<N>::step_mart_ref(s) : Impl<N>::step_prec_ref(s);
     if (s->cfg.physics == GSK_PHYS_MART) return Impl<N>::step_mart_fused(s);
     return Impl<N>::step_prec_fused(s);
   }
@@ -729,7 +788,8 @@ int sync(gsk_sim* s) {
 
 int prepare(gsk_sim* s) {
   if (!s->haveB) return fail(GSK_ERR_STATE, "B tensor not built (call gsk_build_btensor first)");
-  if (s->dirty && s->cfg.path == GSK_PATH_FUSED32) {
+  const bool stateless = s->cfg.rule == GSK_RULE_REFERENCE && s->cfg.physics == GSK_PHYS_MART;
+  if (s->dirty && s->cfg.path == GSK_PATH_FUSED32 && !stateless) {
     CHK(dispatch_n<DInit>(s->n, s));
   }
   s->dirty = false;
@@ -782,6 +842,26 @@ int gsk_config_default(gsk_config* c, int physics) {
   } else {
     return fail(GSK_ERR_ARG, "unknown physics");
   }
+  // reference-rule weights (src/dynamics_mart.cu:20-25 HEAD side); unused by the spectral rule
+  c->rule = GSK_RULE_SPECTRAL;
+  c->w_grad = 2.5; c->w_chem = 1.0; c->w_dis = 1.0;
+  c->T0 = 450.0; c->temperature = 450.0;
+  return GSK_OK;
+}
+
+int gsk_config_reference(gsk_config* c, int physics) {
+  CHK(gsk_config_default(c, physics));
+  c->rule = GSK_RULE_REFERENCE;
+  c->kconv = GSK_K_REFERENCE;
+  if (physics == GSK_PHYS_MART) {
+    // src/dynamics_mart.cu:20-25 (HEAD), LPC[2], LPC[3] at :41
+    c->dt = 0.01; c->w_el = 100000.0; c->noise = 0.0001;
+    c->B = 32.05; c->C = 37.5;
+  } else {
+    // src/dynamics_precipitate.cu:21-29; weightnoise 1 -> amplitude 1e-4 (:203, :244)
+    c->noise = 0.0001;
+    c->S = 0.0;
+  }
   return GSK_OK;
 }
 
@@ -793,6 +873,9 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
   if (cfg->path != GSK_PATH_FUSED32 && cfg->path != GSK_PATH_PLAIN64) return fail(GSK_ERR_ARG, "unknown path");
   if (cfg->path == GSK_PATH_PLAIN64 && cfg->physics != GSK_PHYS_MART)
     return fail(GSK_ERR_UNSUPPORTED, "the FP64 plain path implements the martensite rule only");
+  if (cfg->rule != GSK_RULE_SPECTRAL && cfg->rule != GSK_RULE_REFERENCE) return fail(GSK_ERR_ARG, "unknown rule");
+  if (cfg->rule == GSK_RULE_REFERENCE && cfg->path != GSK_PATH_FUSED32)
+    return fail(GSK_ERR_UNSUPPORTED, "the reference rules run on the FP32 fused path");
   const int maxv = cfg->physics == GSK_PHYS_MART ? 8 : 7;
   if (cfg->nvar < 1 || cfg->nvar > maxv) return fail(GSK_ERR_ARG, "nvar out of range");
   if (!(cfg->dx > 0)) return fail(GSK_ERR_ARG, "dx must be positive");
@@ -835,8 +918,10 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
     if ((rc = alloc(s->fld, (size_t)s->F * N3)) != GSK_OK) return bail(rc);
     if ((rc = alloc(s->kbuf, (size_t)s->F * s->g.kfs)) != GSK_OK) return bail(rc);
     if ((rc = alloc(s->state, ntile * s->F * T)) != GSK_OK) return bail(rc);
-    const int npq = cfg->physics == GSK_PHYS_MART ? s->V * (s->V + 1) / 2 : 1;
+    const bool vv = cfg->physics == GSK_PHYS_MART || cfg->rule == GSK_RULE_REFERENCE;
+    const int npq = vv ? s->V * (s->V + 1) / 2 : 1;
     if ((rc = alloc(s->Bt, ntile * npq * T)) != GSK_OK) return bail(rc);
+    if (cfg->rule == GSK_RULE_REFERENCE && (rc = alloc(s->rbuf, (size_t)s->V * N3)) != GSK_OK) return bail(rc);
   } else {
     const size_t VN = (size_t)s->V * N3;
     if ((rc = alloc(s->eta64, VN)) != GSK_OK) return bail(rc);
@@ -866,6 +951,7 @@ int gsk_destroy(gsk_sim* s) {
   free_dev(s->fld); free_dev(s->kbuf); free_dev(s->state); free_dev(s->Bt); free_dev(s->dstep);
   free_dev(s->eta64); free_dev(s->S64); free_dev(s->R64); free_dev(s->E64); free_dev(s->F64); free_dev(s->B64);
   free_dev(s->part); free_dev(s->rout);
+  free_dev(s->rbuf); free_dev(s->defect); free_dev(s->Kd);
   for (auto& e : s->ev) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -901,7 +987,7 @@ static int fill_bargs(BArgs& a, const double* C81, const double* e1, int V1, con
 
 int gsk_build_btensor(gsk_sim* s, const double* C81, const double* eps, int neps) {
   if (!s || !C81 || !eps) return fail(GSK_ERR_ARG, "null argument");
-  const int need = s->cfg.physics == GSK_PHYS_MART ? s->V : 1;
+  const int need = (s->cfg.physics == GSK_PHYS_MART || s->cfg.rule == GSK_RULE_REFERENCE) ? s->V : 1;
   if (neps != need) return fail(GSK_ERR_ARG, "eigenstrain count must be " + std::to_string(need));
   CHK(ensure_device(s->dev));
   const double kfac =
@@ -911,6 +997,8 @@ int gsk_build_btensor(gsk_sim* s, const double* C81, const double* eps, int neps
   s->cee0 = a.cee[0];
   s->bargs = a;
   s->haveBargs = true;
+  for (int i = 0; i < neps * 9; ++i) s->eps[i] = eps[i];
+  s->haveEps = true;
   if (s->cfg.path == GSK_PATH_FUSED32) {
     const long total = (long)(long)s->g.ntile * s->n * s->g.W;
     hipLaunchKernelGGL(k_btensor_tiles, dim3(grid_1d(total)), dim3(256), 0, s->stream, s->Bt, s->g, a);
@@ -940,9 +1028,45 @@ static int field_info(gsk_sim* s, const char* name, int* offset_fields, int* cou
   return fail(GSK_ERR_ARG, std::string("unknown field \"") + name + "\"");
 }
 
+static int set_ref_field(gsk_sim* s, const char* name, const void* host, int dtype) {
+  const size_t N3 = nvox(s);
+  auto val = [&](size_t i) { return dtype == GSK_F32 ? (double)((const float*)host)[i] : ((const double*)host)[i]; };
+  if (s->cfg.rule != GSK_RULE_REFERENCE || s->cfg.physics != GSK_PHYS_MART)
+    return fail(GSK_ERR_UNSUPPORTED, std::string("field \"") + name + "\" belongs to the martensite reference rule");
+  std::vector<float> h;
+  float** dst;
+  if (std::strcmp(name, "defect") == 0) {
+    h.resize(N3);
+    for (size_t i = 0; i < N3; ++i) h[i] = (float)val(i);
+    dst = &s->defect;
+  } else {
+    // sigma from the 6 stored components (src/dynamics_mart.cu:125-139), then
+    // K_p = 25 sum_ij sigma_ij eps^p_ij (src/dynamics_mart.cu:305-318)
+    if (!s->haveEps) return fail(GSK_ERR_STATE, "build the B tensor before setting the dislocation stress");
+    static const int comp[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    h.resize((size_t)s->V * N3);
+    for (int p = 0; p < s->V; ++p)
+      for (size_t i = 0; i < N3; ++i) {
+        double k = 0.0;
+        for (int ij = 0; ij < 9; ++ij) k += 25.0 * val(comp[ij] * N3 + i) * s->eps[p * 9 + ij];
+        h[(size_t)p * N3 + i] = (float)k;
+      }
+    dst = &s->Kd;
+  }
+  CHK(sync(s));
+  if (!*dst) HIPCHK(hipMalloc((void**)dst, h.size() * sizeof(float)));
+  HIPCHK(hipMemcpy(*dst, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  ++s->gen;
+  return GSK_OK;
+}
+
 int gsk_set_field(gsk_sim* s, const char* name, const void* host, int dtype) {
   if (!s || !host) return fail(GSK_ERR_ARG, "null argument");
   if (dtype != GSK_F32 && dtype != GSK_F64) return fail(GSK_ERR_ARG, "bad dtype");
+  if (name && (std::strcmp(name, "defect") == 0 || std::strcmp(name, "dislocationstress") == 0)) {
+    CHK(ensure_device(s->dev));
+    return set_ref_field(s, name, host, dtype);
+  }
   int off, cnt;
   CHK(field_info(s, name, &off, &cnt));
   CHK(ensure_device(s->dev));
@@ -1074,6 +1198,9 @@ int gsk_set_param(gsk_sim* s, const char* key, double v) {
   else if (k == "noise") c.noise = v; else if (k == "w_el") c.w_el = v; else if (k == "M") c.M = v;
   else if (k == "kappa_c") c.kappa_c = v; else if (k == "xi") c.xi = v; else if (k == "c1") c.c1 = v;
   else if (k == "c2") c.c2 = v; else if (k == "S") c.S = v;
+  else if (k == "w_grad") c.w_grad = v; else if (k == "w_chem") c.w_chem = v; else if (k == "w_dis") c.w_dis = v;
+  else if (k == "temperature") c.temperature = v; else if (k == "T0") c.T0 = v;
+  else if (k == "chem_form") c.chem_form = (int)v;
   else if (k.size() == 2 && k[0] == 'a' && k[1] >= '1' && k[1] <= '7') c.a[k[1] - '0'] = v;
   else return fail(GSK_ERR_ARG, "unknown parameter " + k);
   ++s->gen;
@@ -1127,6 +1254,8 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
     return GSK_OK;
   }
   if (what == GSK_RED_ENERGY) {
+    if (s->cfg.rule == GSK_RULE_REFERENCE)
+      return fail(GSK_ERR_UNSUPPORTED, "the reference rules define no free energy (the plugins report forces)");
     CHK(real_pass());
     const double chem = h[0];
     double gr = 0.0, el = 0.0;
@@ -1806,8 +1935,8 @@ extern "C" {
 
 int gsk_adjoint_forward(gsk_sim* s, int nsteps, int K) {
   if (!s || nsteps < 1 || K < 1) return fail(GSK_ERR_ARG, "bad argument");
-  if (s->cfg.path != GSK_PATH_FUSED32 || s->cfg.physics != GSK_PHYS_MART)
-    return fail(GSK_ERR_UNSUPPORTED, "the adjoint covers the FP32 fused martensite step");
+  if (s->cfg.path != GSK_PATH_FUSED32 || s->cfg.physics != GSK_PHYS_MART || s->cfg.rule != GSK_RULE_SPECTRAL)
+    return fail(GSK_ERR_UNSUPPORTED, "the adjoint covers the FP32 fused spectral martensite step");
   if (s->V > 4) return fail(GSK_ERR_UNSUPPORTED, "the adjoint supports up to 4 variants");
   CHK(ensure_device(s->dev));
   CHK(prepare(s));

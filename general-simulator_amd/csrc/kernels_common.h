@@ -67,6 +67,7 @@ struct PrecK {
   float dtL, dtLke;  // dt*L, dt*L*kappa_eta
   float xi;          // elastic weight
   float S;           // Cahn-Hilliard stabilisation (implicit S k^2)
+  int celastic;      // 1: c-sourced elasticity in the concentration update
   float kfac, inv_n3;
   int V;             // number of order parameters (fields = V + 1)
   uint32_t step;

@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_prec(float2* __rest
               const float2 fh = v[gi][r];
               float2 nh;
               if (f == 0) {
-                const float a = pk.xi * Bb[e] - pk.S;
+                const float a = (pk.celastic ? pk.xi * Bb[e] : 0.f) - pk.S;
                 const float mx = fh.x + a * sh.x, my = fh.y + a * sh.y;
                 const float den = 1.f / (1.f + pk.dtM * pk.S * k2 + pk.dtMkc * k2 * k2);
                 nh = make_float2((sh.x - pk.dtM * k2 * mx) * den, (sh.y - pk.dtM * k2 * my) * den);

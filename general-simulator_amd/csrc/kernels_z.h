@@ -351,10 +351,13 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta,
 
 // --------------------------------------------------------------------- precipitate forward z pass
 // All fields of a tile in LDS: (dc, eta_1..V) -> (mu_c, F_1..F_V) with noise on eta, then R2C.
+// R (may be null): real elastic fields R_v of the eta^2 source, adding el * R_v eta_v to F_v;
+// cnoise: amplitude of noise on mu_c (Philox field 0x8000), both used by the reference rule.
 template <int M>
 __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local(const float* __restrict__ in, float2* __restrict__ kout,
                                                              Geo g, PrecK pk, const float2* __restrict__ twM,
-                                                             const float2* __restrict__ rtn) {
+                                                             const float2* __restrict__ rtn,
+                                                             const float* __restrict__ R, float el, float cnoise) {
   using S = ZShape<M>;
   constexpr int L = S::L, Q = M / 2, n = 2 * M;
   extern __shared__ float2 smem[];
@@ -374,11 +377,13 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local(const float* __res
     }
     const int V = pk.V;
     float out[8][4];
+    float xc[4] = {0.f, 0.f, 0.f, 0.f};
+    if (cnoise != 0.f) normal4(pk.seed, (uint32_t)(vox >> 2), 0x8000u, step_of(pk.step, pk.stepp), xc);
     for (int i = 0; i < 4; ++i) {
       const float c = pk.cbar + v[0][i];
       float s2 = 0.f;
       for (int a = 1; a <= V; ++a) s2 += v[a][i] * v[a][i];
-      out[0][i] = 0.5f * pk.A2 * s2 + pk.A1 * (c - pk.c1);
+      out[0][i] = 0.5f * pk.A2 * s2 + pk.A1 * (c - pk.c1) + cnoise * xc[i];
     }
     for (int a = 1; a <= V; ++a) {
       float xi[4] = {0.f, 0.f, 0.f, 0.f};
@@ -399,6 +404,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local(const float* __res
         out[a][i] = (2.f * pk.A5 * et + 4.f * pk.A6 * e2 * et) * t2 + 2.f * pk.A6 * et * t4 +
                     2.f * pk.A7 * et * t22 + pk.A2 * et * (c - pk.c2) - pk.A3 * e2 * et + pk.A4 * e2 * e2 * et +
                     pk.noise * xi[i];
+        if (R) out[a][i] += el * R[(a - 1) * g.nvox + vox + i] * et;
       }
     }
     for (int f = 0; f < F; ++f) {

@@ -25,6 +25,7 @@ LIB_PATH = os.environ.get("GSK_LIBRARY") or os.path.join(os.path.dirname(_HERE),
 GSK_OK = 0
 PHYS_MART, PHYS_PREC = 0, 1
 PATH_FUSED32, PATH_PLAIN64 = 0, 1
+RULE_SPECTRAL, RULE_REFERENCE = 0, 1
 K_EXACT, K_REFERENCE = 0, 1
 F32, F64 = 0, 1
 RED_ENERGY, RED_MASS, RED_MEAN_ETA, RED_SUM_ETA2 = 1, 2, 3, 4
@@ -46,6 +47,8 @@ class Config(ctypes.Structure):
         ("A", ctypes.c_double), ("B", ctypes.c_double), ("C", ctypes.c_double), ("w_el", ctypes.c_double),
         ("a", ctypes.c_double * 8), ("c1", ctypes.c_double), ("c2", ctypes.c_double), ("M", ctypes.c_double),
         ("kappa_c", ctypes.c_double), ("xi", ctypes.c_double), ("S", ctypes.c_double),
+        ("rule", ctypes.c_int), ("chem_form", ctypes.c_int), ("w_grad", ctypes.c_double), ("w_chem", ctypes.c_double),
+        ("w_dis", ctypes.c_double), ("temperature", ctypes.c_double), ("T0", ctypes.c_double),
     ]
 
 
@@ -64,6 +67,7 @@ def lib() -> ctypes.CDLL:
     L.gsk_last_error.restype = ctypes.c_char_p
     L.gsk_version.restype = ctypes.c_char_p
     L.gsk_config_default.argtypes = [ctypes.POINTER(Config), I]
+    L.gsk_config_reference.argtypes = [ctypes.POINTER(Config), I]
     L.gsk_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(P)]
     L.gsk_destroy.argtypes = [P]
     L.gsk_build_btensor.argtypes = [P, P, P, I]
@@ -147,6 +151,19 @@ def default_config(physics: int = PHYS_MART, **kw) -> Config:
     return c
 
 
+def reference_config(physics: int = PHYS_MART, **kw) -> Config:
+    """Config of the plugins' own update rule with the reference's Initialize defaults."""
+    c = Config()
+    _check(lib().gsk_config_reference(ctypes.byref(c), physics))
+    for k, v in kw.items():
+        if k == "a":
+            for i, x in enumerate(v):
+                c.a[i] = x
+        else:
+            setattr(c, k, v)
+    return c
+
+
 class Sim:
     """One phase-field simulation on one GPU (a gsk_sim handle)."""
 
@@ -187,7 +204,11 @@ class Sim:
 
     def _shape(self, name: str):
         n = self.n
-        return (self.nvar, n, n, n) if name == "eta" else (n, n, n)
+        if name == "eta":
+            return (self.nvar, n, n, n)
+        if name == "dislocationstress":
+            return (6, n, n, n)
+        return (n, n, n)
 
     def set_field(self, name: str, a: np.ndarray) -> None:
         a = np.asarray(a)

@@ -79,10 +79,29 @@ typedef struct gsk_config {
    * side of the concentration update; S = 0 is the reference rule
    * (src/dynamics_precipitate.cu:203-205). */
   double S;
+  /* Rule. GSK_RULE_SPECTRAL: the semi-implicit spectral rules above. GSK_RULE_REFERENCE:
+   * the plugins' own updates (FP32 fused path only):
+   *   martensite (src/dynamics_mart.cu:181-417): eta += dt (w_grad Grad + Chem + El + Dis
+   *     + noise xi), eta *= (1 - defect); Chem from a1 = 0.02 (temperature - T0), a2 = B,
+   *     a3 = C (the reference's LPC[1..3]), El = -w_el R eta (no factor 2), Dis = -w_dis K eta;
+   *   precipitate (src/dynamics_precipitate.cu:118-289): eta^2 elastic source (B built from
+   *     nvar eigenstrains), F_v += xi 2 R_v eta_v, no elasticity and no stabilisation in the
+   *     c update, noise amplitude `noise` on both c and eta (the reference's 1e-4 weightnoise). */
+  int rule;
+  int chem_form;         /* martensite reference chemical force: 0 HEAD side, 1 master side */
+  double w_grad, w_chem, w_dis; /* martensite reference weights                           */
+  double temperature, T0;       /* martensite reference: a1 = 0.02 (temperature - T0)       */
 } gsk_config;
+
+#define GSK_RULE_SPECTRAL 0
+#define GSK_RULE_REFERENCE 1
 
 /* Fill cfg with the defaults of the given physics. */
 int gsk_config_default(gsk_config* cfg, int physics);
+/* Fill cfg with the defaults of the reference rule of the given physics (rule =
+ * GSK_RULE_REFERENCE, kconv = GSK_K_REFERENCE, the plugins' Initialize defaults:
+ * src/dynamics_mart.cu:20-41 HEAD side, src/dynamics_precipitate.cu:21-29). */
+int gsk_config_reference(gsk_config* cfg, int physics);
 
 int gsk_create(const gsk_config* cfg, gsk_sim** out);
 int gsk_destroy(gsk_sim* sim);
@@ -92,7 +111,10 @@ int gsk_destroy(gsk_sim* sim);
  * on eta^2. Precipitate: neps = 1, the eigenstrain per unit concentration. */
 int gsk_build_btensor(gsk_sim* sim, const double* C81, const double* eps, int neps);
 
-/* Fields: "eta" (nvar*n^3) and "concentration" (n^3), layout [v][x][y][z]. */
+/* Fields: "eta" (nvar*n^3) and "concentration" (n^3), layout [v][x][y][z]. Martensite
+ * reference rule, set only: "defect" (n^3, src/dynamics_mart.cu:333-346) and
+ * "dislocationstress" (6*n^3 components 00,01,02,11,12,22, src/dynamics_mart.cu:125-139;
+ * needs the B tensor built first: K_p = 25 sum_ij sigma_ij eps^p_ij uses its eigenstrains). */
 int gsk_set_field(gsk_sim* sim, const char* name, const void* host, int dtype);
 int gsk_get_field(gsk_sim* sim, const char* name, void* host, int dtype);
 /* Philox initial field lo + (hi-lo)*u, u = (x>>8)*2^-24, counter (voxel/4, field, 0, 1). */

@@ -642,6 +642,125 @@ extern "C" void or_prec_run(int n, int V, const or_prec_params* p, const double*
   }
 }
 
+// ---------------------------------------------------------------------------
+// Reference rules: the plugins' own updates, transcribed.
+//   Martensite (src/dynamics_mart.cu:181-417): explicit Euler
+//     eta += dt (w_grad Grad + Chem + El + Dis + w_noise xi), then eta *= (1 - defect)
+//     Grad  = sum_d (eta(x+e_d) + eta(x-e_d) - 2 eta(x)) / (2 dx) / 3           (:181-188)
+//     Chem  = HEAD: s = sqrt|a1/(a2-a3)|, eta >= 0: -w_chem (s eta)(a1 - a2 (s eta)^2 + a3 sum_i (s eta_i)^2),
+//                   eta < 0: 1000 w_chem a2 eta^2                              (:222-250 HEAD side)
+//             master: -w_chem eta (a1 - a2 eta + a3 sum_i eta_i)                (:222-250 master side,
+//                   whose eta <= 0 test compares a pointer and is never taken)
+//     El    = -w_el R_p eta_p, R_p = IFFT[sum_q B_pq FFT(eta_q^2)]/N, all q       (:270-289 HEAD)
+//     Dis   = -w_dis K_p eta_p, K_p = 25 sum_ij sigma_ij eps^p_ij (given)        (:305-331)
+//   Precipitate (src/dynamics_precipitate.cu:118-289): the spectral rule of or_prec_run with
+//     the eta^2 elastic source El_v = 2 R_v eta_v added as xi El_v to F_v (:170-195, :249),
+//     no elasticity and no stabilisation in the c update (:197-206), and noise of amplitude
+//     p->noise on both fields (the reference's 1e-4 weightnoise, :203, :244): eta_v uses
+//     Philox field v, c uses field 0x8000.
+// ---------------------------------------------------------------------------
+extern "C" void or_mart_ref_run(int n, int V, const or_mart_ref_params* p, const double* B, double* eta,
+                                const double* defect, const double* K, int step0, int nsteps) {
+  const int64_t N = (int64_t)n * n * n, VN = (int64_t)V * N;
+  std::vector<double> out((size_t)VN);
+  std::vector<float> xi((size_t)VN, 0.0f);
+  std::vector<cd> R;
+  const double g = 1.0 / (2.0 * p->dx) / 3.0;
+  const double sc = std::sqrt(std::fabs(p->a1 / (p->a2 - p->a3)));
+  for (int s = 0; s < nsteps; ++s) {
+    elastic_R(n, V, B, eta, R);
+    if (p->w_noise != 0.0)
+      for (int v = 0; v < V; ++v) or_noise_field(p->seed, (uint32_t)(step0 + s), (uint32_t)v, N, xi.data() + v * N);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      const int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
+      const int64_t ox = (int64_t)n * n, oy = n;
+      const int64_t xp = i + ((x + 1) % n - x) * ox, xm = i + ((x + n - 1) % n - x) * ox;
+      const int64_t yp = i + ((y + 1) % n - y) * oy, ym = i + ((y + n - 1) % n - y) * oy;
+      const int64_t zp = i + ((z + 1) % n - z), zm = i + ((z + n - 1) % n - z);
+      double S2 = 0.0, S1 = 0.0;
+      for (int u = 0; u < V; ++u) {
+        const double e = eta[u * N + i];
+        S2 += (sc * e) * (sc * e);
+        S1 += e;
+      }
+      for (int v = 0; v < V; ++v) {
+        const double* ev = eta + v * N;
+        const double e = ev[i];
+        const double grad = (ev[xp] + ev[xm] - 2.0 * e) * g + (ev[yp] + ev[ym] - 2.0 * e) * g +
+                            (ev[zp] + ev[zm] - 2.0 * e) * g;
+        double chem;
+        if (p->chem_form == 0)
+          chem = e >= 0.0 ? -p->w_chem * (sc * e) * (p->a1 - p->a2 * (sc * e) * (sc * e) + p->a3 * S2)
+                          : 1000.0 * p->w_chem * p->a2 * e * e;
+        else
+          chem = -p->w_chem * e * (p->a1 - p->a2 * e + p->a3 * S1);
+        const double el = -p->w_el * R[v * N + i].real() * e;
+        const double dis = K ? -p->w_dis * K[v * N + i] * e : 0.0;
+        double nv = e + p->dt * (p->w_grad * grad + chem + el + dis + p->w_noise * (double)xi[v * N + i]);
+        if (defect) nv *= 1.0 - defect[i];
+        out[v * N + i] = nv;
+      }
+    }
+    std::memcpy(eta, out.data(), VN * sizeof(double));
+  }
+}
+
+extern "C" void or_prec_ref_run(int n, int V, const or_prec_params* p, const double* B, double* c, double* eta,
+                                int step0, int nsteps) {
+  const int64_t N = (int64_t)n * n * n;
+  std::vector<double> k2 = ksq_table(n, p->dx, p->kconv), mu(N), F((size_t)V * N);
+  std::vector<float> xi(N);
+  std::vector<cd> ch(N), mh(N), eh(N), fh(N), R;
+  for (int s = 0; s < nsteps; ++s) {
+    elastic_R(n, V, B, eta, R);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      double e[16], Fl[16];
+      for (int v = 0; v < V; ++v) e[v] = eta[v * N + i];
+      or_prec_local(V, p, c[i], e, &mu[i], Fl, nullptr);
+      for (int v = 0; v < V; ++v) F[v * N + i] = Fl[v] + p->xi * 2.0 * R[v * N + i].real() * e[v];
+    }
+    if (p->noise != 0.0) {
+      or_noise_field(p->seed, (uint32_t)(step0 + s), 0x8000u, N, xi.data());
+      for (int64_t i = 0; i < N; ++i) mu[i] += p->noise * (double)xi[i];
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) { ch[i] = cd(c[i], 0); mh[i] = cd(mu[i], 0); }
+    fft3d(n, ch.data(), -1);
+    fft3d(n, mh.data(), -1);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
+      double kk = k2[x] + k2[y] + k2[z];
+      ch[i] = (ch[i] - p->dt * p->M * kk * mh[i]) / (1.0 + p->dt * p->M * p->kappa_c * kk * kk) / (double)N;
+    }
+    fft3d(n, ch.data(), +1);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) c[i] = ch[i].real();
+    for (int v = 0; v < V; ++v) {
+      if (p->noise != 0.0) {
+        or_noise_field(p->seed, (uint32_t)(step0 + s), (uint32_t)v, N, xi.data());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; ++i) F[v * N + i] += p->noise * (double)xi[i];
+      }
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < N; ++i) { eh[i] = cd(eta[v * N + i], 0); fh[i] = cd(F[v * N + i], 0); }
+      fft3d(n, eh.data(), -1);
+      fft3d(n, fh.data(), -1);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < N; ++i) {
+        int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
+        double kk = k2[x] + k2[y] + k2[z];
+        eh[i] = (eh[i] - p->dt * p->L * fh[i]) / (1.0 + p->dt * p->L * p->kappa_eta * kk) / (double)N;
+      }
+      fft3d(n, eh.data(), +1);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < N; ++i) eta[v * N + i] = eh[i].real();
+    }
+  }
+}
+
 extern "C" void or_prec_energy(int n, int V, const or_prec_params* p, const double* Bc, const double* c,
                                const double* eta, double out[4]) {
   const int64_t N = (int64_t)n * n * n;

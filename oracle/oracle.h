@@ -53,6 +53,22 @@ typedef struct or_prec_params {
   uint64_t seed;
 } or_prec_params;
 
+/* Reference rule of the martensite plugin (explicit Euler, src/dynamics_mart.cu:181-417). */
+typedef struct or_mart_ref_params {
+  double a1, a2, a3;     /* LPC[1..3]; a1 = 0.02 (T - T0) at the step                    */
+  double w_grad, w_chem, w_el, w_dis, w_noise;
+  double dt, dx;
+  int kconv;
+  int chem_form;         /* 0: HEAD side of ChemiFor_Mart_Kernel, 1: master side         */
+  uint64_t seed;
+} or_mart_ref_params;
+/* defect (n^3) and K (dislocation force constants, V*n^3) may be null. */
+void or_mart_ref_run(int n, int V, const or_mart_ref_params* p, const double* B, double* eta,
+                     const double* defect, const double* K, int step0, int nsteps);
+/* Reference rule of the precipitate plugin: eta^2 elastic source with the full V x V B. */
+void or_prec_ref_run(int n, int V, const or_prec_params* p, const double* B, double* c, double* eta,
+                     int step0, int nsteps);
+
 void or_set_threads(int nt);
 int or_get_threads(void);
 

@@ -23,6 +23,14 @@ class MartParams(ctypes.Structure):
                 ("kconv", ctypes.c_int), ("seed", ctypes.c_uint64)]
 
 
+class MartRefParams(ctypes.Structure):
+    _fields_ = [("a1", ctypes.c_double), ("a2", ctypes.c_double), ("a3", ctypes.c_double),
+                ("w_grad", ctypes.c_double), ("w_chem", ctypes.c_double), ("w_el", ctypes.c_double),
+                ("w_dis", ctypes.c_double), ("w_noise", ctypes.c_double), ("dt", ctypes.c_double),
+                ("dx", ctypes.c_double), ("kconv", ctypes.c_int), ("chem_form", ctypes.c_int),
+                ("seed", ctypes.c_uint64)]
+
+
 class PrecParams(ctypes.Structure):
     _fields_ = [("A", ctypes.c_double * 8), ("c1", ctypes.c_double), ("c2", ctypes.c_double),
                 ("M", ctypes.c_double), ("L", ctypes.c_double), ("kappa_c", ctypes.c_double),
@@ -65,6 +73,8 @@ def lib():
         L.or_mart_energy.argtypes = [I, I, ctypes.POINTER(MartParams), P, P, P]
         L.or_mart_force.argtypes = [I, I, ctypes.POINTER(MartParams), P, P, P]
         L.or_mart_adjoint.argtypes = [I, I, ctypes.POINTER(MartParams), P, P, I, I, P, P, P]
+        L.or_mart_ref_run.argtypes = [I, I, ctypes.POINTER(MartRefParams), P, P, P, P, I, I]
+        L.or_prec_ref_run.argtypes = [I, I, ctypes.POINTER(PrecParams), P, P, P, I, I]
         L.or_prec_run.argtypes = [I, I, ctypes.POINTER(PrecParams), P, P, P, I, I]
         L.or_prec_energy.argtypes = [I, I, ctypes.POINTER(PrecParams), P, P, P, P]
         L.or_prec_local.argtypes = [I, ctypes.POINTER(PrecParams), D, P, P, P, P]
@@ -204,6 +214,34 @@ def mart_adjoint(eta0: np.ndarray, B: np.ndarray, p: MartParams, nsteps: int, gT
     lib().or_mart_adjoint(n, V, ctypes.byref(p), _p(np.ascontiguousarray(B, dtype=np.float64)), _p(e), step0, nsteps,
                           _p(g), _p(ge), _p(gp))
     return ge, gp
+
+
+def mart_ref_params(**kw) -> MartRefParams:
+    # reference defaults (src/dynamics_mart.cu:20-39 HEAD side, LPC[2..3] :41)
+    p = MartRefParams(a1=0.0, a2=32.05, a3=37.5, w_grad=2.5, w_chem=1.0, w_el=1e5, w_dis=1.0, w_noise=1e-4, dt=0.01,
+                      dx=1.0, kconv=1, chem_form=0, seed=0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def mart_ref_run(eta, B, p: MartRefParams, nsteps: int, step0: int = 0, defect=None, K=None) -> np.ndarray:
+    e = np.ascontiguousarray(eta, dtype=np.float64).copy()
+    V, n = e.shape[0], e.shape[1]
+    d = None if defect is None else np.ascontiguousarray(defect, dtype=np.float64)
+    k = None if K is None else np.ascontiguousarray(K, dtype=np.float64)
+    lib().or_mart_ref_run(n, V, ctypes.byref(p), _p(np.ascontiguousarray(B, dtype=np.float64)), _p(e),
+                          None if d is None else _p(d), None if k is None else _p(k), step0, nsteps)
+    return e
+
+
+def prec_ref_run(c, eta, B, p: PrecParams, nsteps: int, step0: int = 0):
+    cc = np.ascontiguousarray(c, dtype=np.float64).copy()
+    e = np.ascontiguousarray(eta, dtype=np.float64).copy()
+    V, n = e.shape[0], e.shape[1]
+    lib().or_prec_ref_run(n, V, ctypes.byref(p), _p(np.ascontiguousarray(B, dtype=np.float64)), _p(cc), _p(e),
+                          step0, nsteps)
+    return cc, e
 
 
 def prec_run(c: np.ndarray, eta: np.ndarray, Bc: np.ndarray, p: PrecParams, nsteps: int, step0: int = 0):

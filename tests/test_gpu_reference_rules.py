@@ -1,0 +1,102 @@
+"""GPU: the reference rules of both plugins (rule = GSK_RULE_REFERENCE) against the FP64 oracle.
+
+The oracle restates src/dynamics_mart.cu:181-417 and src/dynamics_precipitate.cu:118-289
+(or_mart_ref_run / or_prec_ref_run, pinned in test_oracle.py). The GPU computes in FP32 like
+the reference; tolerances (relative L2 after the stated steps) are written in each test.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gsk
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
+
+
+@pytest.mark.parametrize("chem_form,noise", [(0, 1e-4), (1, 0.0)])
+def test_mart_reference_rule_matches_oracle(gpu, chem_form, noise):
+    say(f"mart reference rule chem_form={chem_form} noise={noise}")
+    n, V, steps = 32, 3, 40
+    C = gsk.cubic_modulus(3.5, 1.5, 1.0)
+    eps = gsk.mart_eigenstrains(V, base=0.0, axis=0.01)
+    cfg = gsk.reference_config(gsk.PHYS_MART, n=n, nvar=V, temperature=300.0, chem_form=chem_form, noise=noise,
+                               w_el=1e3, seed=6, dt=0.005)
+    rng = np.random.default_rng(1)
+    defect = (rng.random((n, n, n)) < 0.01).astype(np.float32)
+    stress = rng.uniform(-0.01, 0.01, (6, n, n, n)).astype(np.float32)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(C, eps)
+        s.init_uniform("eta", 0.0, 0.2, 3)
+        s.set_field("defect", defect)
+        s.set_field("dislocationstress", stress)
+        eta0 = s.get_field("eta").astype(np.float64)
+        s.step(steps)
+        got = s.get_field("eta", np.float64)
+    a1 = 0.02 * (cfg.temperature - cfg.T0)
+    p = O.mart_ref_params(a1=a1, a2=cfg.B, a3=cfg.C, w_grad=cfg.w_grad, w_chem=cfg.w_chem, w_el=cfg.w_el,
+                          w_dis=cfg.w_dis, w_noise=noise, dt=cfg.dt, dx=cfg.dx, kconv=cfg.kconv, chem_form=chem_form,
+                          seed=cfg.seed)
+    comp = [0, 1, 2, 1, 3, 4, 2, 4, 5]
+    sig = stress.astype(np.float64)[comp].reshape(3, 3, n, n, n)
+    K = 25.0 * np.einsum("ijxyz,pij->pxyz", sig, eps)
+    want = O.mart_ref_run(eta0, O.btensor(n, C, eps, kconv=cfg.kconv), p, steps, defect=defect.astype(np.float64),
+                          K=K)
+    err = rel(got, want)
+    say(f"  rel L2 {err:.2e}, |eta| max {np.abs(want).max():.3f}")
+    assert np.isfinite(got).all()
+    assert np.all(got[:, defect > 0] == 0.0)
+    assert err < 2e-5
+
+
+def test_prec_reference_rule_matches_oracle(gpu):
+    say("precipitate reference rule")
+    n, V, steps = 32, 3, 30
+    C = gsk.cubic_modulus(3.5, 1.5, 1.0)
+    eps = gsk.mart_eigenstrains(V, base=0.0, axis=0.02)
+    cfg = gsk.reference_config(gsk.PHYS_PREC, n=n, nvar=V, M=1.0, L=1.0, kappa=1.5, kappa_c=1.5, dt=0.002, seed=8,
+                               xi=50.0)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(C, eps)
+        s.init_uniform("concentration", 0.245, 0.255, 11)
+        s.init_uniform("eta", 0.0, 0.1, 12)
+        c0 = s.get_field("concentration", np.float64)
+        h0 = s.get_field("eta", np.float64)
+        m0 = s.mass()
+        s.step(steps)
+        c1 = s.get_field("concentration", np.float64)
+        h1 = s.get_field("eta", np.float64)
+        m1 = s.mass()
+    p = O.prec_params(A=list(cfg.a), c1=cfg.c1, c2=cfg.c2, M=cfg.M, L=cfg.L, kappa_c=cfg.kappa_c,
+                      kappa_eta=cfg.kappa, xi=cfg.xi, S=0.0, dt=cfg.dt, noise=cfg.noise, kconv=cfg.kconv,
+                      seed=cfg.seed)
+    cw, hw = O.prec_ref_run(c0, h0, O.btensor(n, C, eps, kconv=cfg.kconv), p, steps)
+    say(f"  rel L2 eta {rel(h1, hw):.2e}  c-cbar {rel(c1 - c0.mean(), cw - c0.mean()):.2e}")
+    assert rel(h1, hw) < 1e-4
+    assert rel(c1 - c0.mean(), cw - c0.mean()) < 1e-4
+    assert m1 == m0
+
+
+def test_reference_rule_errors(gpu):
+    cfg = gsk.reference_config(gsk.PHYS_MART, n=16, nvar=2)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(gsk.cubic_modulus(3.5, 1.5, 1.0), gsk.mart_eigenstrains(2))
+        s.init_uniform("eta", 0.0, 0.1, 1)
+        with pytest.raises(gsk.GskError):
+            s.energy()
+        with pytest.raises(gsk.GskError):
+            s.adjoint_forward(2, 1)
+    spec = gsk.default_config(gsk.PHYS_MART, n=16, nvar=2)
+    with gsk.Sim(spec) as s:
+        with pytest.raises(gsk.GskError):
+            s.set_field("defect", np.zeros((16, 16, 16), np.float32))

@@ -382,3 +382,95 @@ def test_mart_adjoint_matches_central_differences():
         assert abs(fd - g_par[q]) < 1e-5 * abs(fd), (q, fd, g_par[q])
     # every gradient is non-trivial, so a dropped term cannot hide behind a zero
     assert np.all(np.abs(g_par) > 1e-9) and np.abs(g_eta).max() > 1e-6
+
+
+# ------------------------------------------------------------------------- reference rules
+def test_mart_ref_plane_waves_decay_by_exact_factor():
+    """Linear regime of the reference martensite rule (master chemical form with a2 = a3 = 0,
+    no elasticity, no noise): a cosine along each axis is an eigenvector of the Grad stencil
+    of src/dynamics_mart.cu:181-188 with eigenvalue (2 cos(theta) - 2)/(2 dx)/3, so one step
+    multiplies it by 1 + dt (w_grad (2 cos(theta) - 2)/(6 dx) - w_chem a1) exactly."""
+    n, dx, dt, wG, wC, a1 = 16, 0.7, 0.01, 2.5, 1.3, 0.4
+    x = np.arange(n)
+    modes = [(0, 1, 0.3), (1, 2, -0.2), (2, 5, 0.1)]
+    eta = np.zeros((1, n, n, n))
+    want = np.zeros_like(eta)
+    for axis, m, amp in modes:
+        th = 2 * np.pi * m / n
+        shape = [1, 1, 1]
+        shape[axis] = n
+        wave = amp * np.cos(th * x).reshape(shape) * np.ones((n, n, n))
+        eta[0] += wave
+        want[0] += wave * (1 + dt * (wG * (2 * np.cos(th) - 2) / (6 * dx) - wC * a1))
+    p = O.mart_ref_params(a1=a1, a2=0.0, a3=0.0, w_grad=wG, w_chem=wC, w_el=0.0, w_noise=0.0, dt=dt, dx=dx,
+                          chem_form=1)
+    got = O.mart_ref_run(eta, np.zeros((1, 1, n, n, n)), p, 1)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-14)
+
+
+def test_mart_ref_uniform_states_defect_and_dislocation():
+    """Uniform fields (Grad = 0, no elasticity): the HEAD chemical force on each branch,
+    defect blocking to zero and linear dislocation decay."""
+    n, dt = 8, 0.01
+    a1, a2, a3, wC = -1.5, 32.05, 37.5, 1.0
+    sc = math.sqrt(abs(a1 / (a2 - a3)))
+    e1, e2 = 0.3, -0.2
+    eta = np.stack([np.full((n, n, n), e1), np.full((n, n, n), e2)])
+    S2 = (sc * e1) ** 2 + (sc * e2) ** 2
+    f1 = -wC * (sc * e1) * (a1 - a2 * (sc * e1) ** 2 + a3 * S2)    # eta >= 0 branch
+    f2 = 1000.0 * wC * a2 * e2 * e2                                 # eta < 0 branch
+    p = O.mart_ref_params(a1=a1, a2=a2, a3=a3, w_grad=2.5, w_chem=wC, w_el=0.0, w_noise=0.0, dt=dt, chem_form=0)
+    B0 = np.zeros((2, 2, n, n, n))
+    got = O.mart_ref_run(eta, B0, p, 1)
+    np.testing.assert_allclose(got[0], e1 + dt * f1, rtol=1e-14)
+    np.testing.assert_allclose(got[1], e2 + dt * f2, rtol=1e-14)
+    defect = np.zeros((n, n, n))
+    defect[1, 2, 3] = 1.0
+    got = O.mart_ref_run(eta, B0, p, 1, defect=defect)
+    assert got[0, 1, 2, 3] == 0.0 and got[1, 1, 2, 3] == 0.0 and got[0, 0, 0, 0] != 0.0
+    q = O.mart_ref_params(a1=0.0, a2=1.0, a3=0.0, w_grad=0.0, w_chem=0.0, w_el=0.0, w_dis=2.0, w_noise=0.0, dt=dt,
+                          chem_form=1)
+    K = np.full((2, n, n, n), 3.0)
+    got = O.mart_ref_run(eta, B0, q, 1, K=K)
+    np.testing.assert_allclose(got[:, 0, 0, 0], [e1 * (1 - dt * 2.0 * 3.0), e2 * (1 - dt * 2.0 * 3.0)], rtol=1e-14)
+
+
+def test_prec_ref_elastic_matches_martensite_rule():
+    """With every Landau coefficient zero the reference precipitate eta update is the
+    semi-implicit step with F = xi * 2 R eta, which is the martensite rule with w_el = xi."""
+    n, V, steps = 16, 2, 3
+    rng = np.random.default_rng(8)
+    C = _cubic(0.7, 0.3, 0.2)
+    B = O.btensor(n, C, _eps(V))
+    eta = rng.uniform(-0.1, 0.1, (V, n, n, n))
+    c = rng.uniform(0.2, 0.3, (n, n, n))
+    pp = O.prec_params(A=[0.0] * 8, xi=5.0, L=0.7, kappa_eta=1.1, dt=0.02, M=1.0, kappa_c=1.0)
+    _, got = O.prec_ref_run(c, eta, B, pp, steps)
+    pm = O.mart_params(A=0.0, B=0.0, C=0.0, w_el=5.0, L=0.7, kappa=1.1, dt=0.02)
+    np.testing.assert_allclose(got, O.mart_run(eta, B, pm, steps), rtol=0, atol=1e-13)
+
+
+def test_prec_ref_without_order_parameters_is_the_plain_ch_step():
+    """eta = 0 removes the eta^2 source: the reference rule equals the spectral rule with no
+    elasticity and no stabilisation (noise off)."""
+    n, V = 16, 2
+    rng = np.random.default_rng(9)
+    c = rng.uniform(0.2, 0.3, (n, n, n))
+    eta = np.zeros((V, n, n, n))
+    pp = O.prec_params(xi=400.0, S=0.0, dt=0.001, M=1.0, L=1.0, kappa_c=1.5, kappa_eta=1.5)
+    cr, er = O.prec_ref_run(c, eta, O.btensor(n, _cubic(3.5, 1.5, 1.0), _eps(V)), pp, 4)
+    q = O.prec_params(xi=0.0, S=0.0, dt=0.001, M=1.0, L=1.0, kappa_c=1.5, kappa_eta=1.5)
+    cs, es = O.prec_run(c, eta, np.zeros((n, n, n)), q, 4)
+    np.testing.assert_allclose(cr, cs, rtol=0, atol=1e-13)
+    assert np.abs(er).max() == 0.0
+
+
+def test_prec_ref_conserves_mass_with_noise():
+    n, V = 16, 3
+    rng = np.random.default_rng(10)
+    c = rng.uniform(0.2, 0.3, (n, n, n))
+    eta = rng.uniform(0.0, 0.05, (V, n, n, n))
+    pp = O.prec_params(noise=1e-4, seed=4, dt=0.001, M=1.0, L=1.0, kappa_c=1.5, kappa_eta=1.5)
+    c1, _ = O.prec_ref_run(c, eta, O.btensor(n, _cubic(3.5, 1.5, 1.0), _eps(V)), pp, 5)
+    assert abs(c1.sum() - c.sum()) < 1e-11 * c.sum()
+    assert np.abs(c1 - c).max() > 0
