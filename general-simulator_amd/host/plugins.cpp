@@ -41,7 +41,8 @@ class GpuDynamics : public Dynamics {
     if (sim_) gsk_destroy(sim_);
   }
   int RunFunc(const std::string& /*func*/) override { return Code_COMMAND_UNKNOW; }
-  bool FixIsInert() const override { return true; }
+  // the reference martensite rule reads the temperature every step; everything else ignores Fix
+  bool FixIsInert() const override { return !(ref_ && cfg_.physics == GSK_PHYS_MART); }
 
   // "set fix temperature Ts Te": linear temperature route (src/dynamics_mart.cu:421-441)
   int Fix(double progress) override {
@@ -62,13 +63,45 @@ class GpuDynamics : public Dynamics {
     return Code_NORMAL;
   }
 
-  int Calculate() override { return step(1); }
-  int CalculateMany(int n) override { return step(n); }
+  int Calculate() override {
+    if (ref_ && cfg_.physics == GSK_PHYS_MART) {  // (*Vars)["temperature"] >>= Temperature (:354)
+      double T;
+      if (var_get(*Vars, "temperature", &T, 1) && check(gsk_set_param(sim_, "temperature", T))) return Code_ERR;
+    }
+    return step(1);
+  }
+  int CalculateMany(int n) override {
+    if (!FixIsInert()) {
+      for (int i = 0; i < n; ++i)
+        if (Calculate() != Code_NORMAL) return Code_ERR;
+      return Code_NORMAL;
+    }
+    return step(n);
+  }
 
  protected:
   gsk_sim* sim_ = nullptr;
   gsk_config cfg_{};
   int nvar_ = 0;
+  bool ref_ = false;  // `set rule reference`: the plugin's own update rule
+
+  // `set rule reference|spectral` (spectral by default)
+  bool read_rule() {
+    std::vector<std::string> w;
+    return var_values(*Vars, "rule", w) && w[0] == "reference";
+  }
+
+  // optional field from the map (e.g. defect, dislocationstress)
+  int upload_optional(const std::string& name, long long want) {
+    auto it = Datas->find(name);
+    if (it == Datas->end() || it->second.empty()) return Code_NORMAL;
+    if (it->second.count() != want) {
+      log_line("Error: field " + name + " has " + std::to_string(it->second.count()) + " values, expected " +
+               std::to_string(want));
+      return Code_ERR;
+    }
+    return check(gsk_set_field(sim_, name.c_str(), it->second.data.data(), GSK_F64));
+  }
 
   int step(int n) {
     if (gsk_step(sim_, n) != GSK_OK) {
@@ -181,7 +214,9 @@ class Dynamics_mart : public GpuDynamics {
  public:
   int Initialize() override {
     if (read_grid()) return Code_ERR;
-    gsk_config_default(&cfg_, GSK_PHYS_MART);
+    ref_ = read_rule();
+    if (ref_) gsk_config_reference(&cfg_, GSK_PHYS_MART);
+    else gsk_config_default(&cfg_, GSK_PHYS_MART);
     read_common();
     auto vt = Datas->find("varianttensor");
     if (vt == Datas->end() || vt->second.empty() || vt->second.dims.size() != 3 || vt->second.dims[1] != 3 ||
@@ -200,6 +235,15 @@ class Dynamics_mart : public GpuDynamics {
     var_get(*Vars, "weightelastic", &cfg_.w_el, 1);
     std::vector<std::string> w;
     if (var_values(*Vars, "precision", w) && w[0] == "fp64") cfg_.path = GSK_PATH_PLAIN64;
+    if (ref_) {  // src/dynamics_mart.cu:18-41
+      var_get(*Vars, "transitiontemperature", &cfg_.T0, 1);
+      var_get(*Vars, "weightgradient", &cfg_.w_grad, 1);
+      var_get(*Vars, "weightchemical", &cfg_.w_chem, 1);
+      var_get(*Vars, "weightdislocation", &cfg_.w_dis, 1);
+      var_get(*Vars, "weightnoise", &cfg_.noise, 1);
+      var_get(*Vars, "temperature", &cfg_.temperature, 1);
+      if (var_values(*Vars, "chemical_form", w) && w[0] == "master") cfg_.chem_form = 1;
+    }
     double C[81];
     auto md = Datas->find("modulus");
     if (md != Datas->end() && md->second.count() == 81) {
@@ -212,6 +256,10 @@ class Dynamics_mart : public GpuDynamics {
     }
     if (create()) return Code_ERR;
     if (check(gsk_build_btensor(sim_, C, vt->second.data.data(), nvar_))) return Code_ERR;
+    if (ref_) {  // src/dynamics_mart.cu:117-142
+      const long long n3 = (long long)nx * ny * nz;
+      if (upload_optional("defect", n3) || upload_optional("dislocationstress", 6 * n3)) return Code_ERR;
+    }
     return load_field("eta", "initeta", {nvar_, nx, ny, nz}, 0.0);
   }
 
@@ -246,7 +294,9 @@ class Dynamics_precipitate : public GpuDynamics {
  public:
   int Initialize() override {
     if (read_grid()) return Code_ERR;
-    gsk_config_default(&cfg_, GSK_PHYS_PREC);
+    ref_ = read_rule();
+    if (ref_) gsk_config_reference(&cfg_, GSK_PHYS_PREC);
+    else gsk_config_default(&cfg_, GSK_PHYS_PREC);
     read_common();
     auto vt = Datas->find("varianttensor");
     if (vt == Datas->end() || vt->second.empty()) {
@@ -269,8 +319,17 @@ class Dynamics_precipitate : public GpuDynamics {
     var_get(*Vars, "concentration", cc, 2);
     cfg_.c1 = cc[0]; cfg_.c2 = cc[1];
     std::vector<std::string> w;
-    if (var_values(*Vars, "elastic_source", w) && w[0] != "c") {
-      log_line("Error: elastic_source " + w[0] + " is not supported (only c)");
+    if (ref_) {
+      // weightnoise w gives noise 1e-4 w on c and eta (src/dynamics_precipitate.cu:30, :203, :244)
+      double wn = 1.0;
+      var_get(*Vars, "weightnoise", &wn, 1);
+      cfg_.noise = 1e-4 * wn;
+      if (var_values(*Vars, "elastic_source", w) && w[0] != "eta2") {
+        log_line("Error: the reference rule uses the eta^2 elastic source");
+        return Code_ERR;
+      }
+    } else if (var_values(*Vars, "elastic_source", w) && w[0] != "c") {
+      log_line("Error: elastic_source " + w[0] + " is not supported by the spectral rule (only c)");
       return Code_ERR;
     }
     double mod[3] = {3.5, 1.5, 1.0};  // src/dynamics_precipitate.cu:55-56
@@ -281,7 +340,15 @@ class Dynamics_precipitate : public GpuDynamics {
     var_get(*Vars, "dilatation", &d0, 1);
     double eps[9] = {d0, 0, 0, 0, d0, 0, 0, 0, d0};
     if (create()) return Code_ERR;
-    if (check(gsk_build_btensor(sim_, C, eps, 1))) return Code_ERR;
+    if (ref_) {  // eta^2 source with the variant tensors (src/dynamics_precipitate.cu:57-67)
+      if (vt->second.count() != (long long)nvar_ * 9) {
+        log_line("Error: varianttensor must hold nvar 3x3 tensors");
+        return Code_ERR;
+      }
+      if (check(gsk_build_btensor(sim_, C, vt->second.data.data(), nvar_))) return Code_ERR;
+    } else if (check(gsk_build_btensor(sim_, C, eps, 1))) {
+      return Code_ERR;
+    }
     if (load_field("concentration", "initconcentration", {nx, ny, nz}, cfg_.c1 + (cfg_.c2 - cfg_.c1) / 3))
       return Code_ERR;  // default c (src/dynamics_precipitate.cu:46)
     return load_field("eta", "initeta", {nvar_, nx, ny, nz}, 0.0);

@@ -1,0 +1,89 @@
+"""GPU: the gsi script host driving the plugins end to end (src/gs.cu:168-212 run loop,
+src/dynamics.h:9-25 plugin interface) against the same steps taken through the C ABI.
+
+Bar: bit-exact. The host only forwards parameters and fields; the dumped binary fields
+(float32, include/datamain.th:356-418) must equal the library's output for the same
+configuration, including the per-step temperature of `set fix temperature`.
+"""
+import os
+import struct
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import gsk
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GSI = os.path.join(ROOT, "general-simulator_amd", "gsi")
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def run_gsi(tmp_path, script):
+    p = tmp_path / "in.test"
+    p.write_text(script)
+    r = subprocess.run([GSI, str(p)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def read_binary(path):
+    raw = open(path, "rb").read()
+    rank = struct.unpack("i", raw[:4])[0]
+    dims = struct.unpack("i" * rank, raw[4:4 + 4 * rank])
+    return np.frombuffer(raw[4 + 4 * rank:], np.float32).reshape(dims)
+
+
+def tensor_lines(eps):
+    V = eps.shape[0]
+    return f"3 {V} 3 3\n" + " ".join(f"{x:.17g}" for x in eps.ravel()) + "\n"
+
+
+def test_gsi_spectral_mart_equals_library(gpu, tmp_path):
+    say("gsi spectral martensite vs library")
+    n, V, steps = 32, 3, 12
+    eps = gsk.mart_eigenstrains(V)
+    script = (f"sys mart\nset gridsize {n} {n} {n} 1 1 1\nset deltatime 0.01\nset coefficient -0.5 3 3\n"
+              f"set weightelastic 20\nset noiseamp 0.002\nset seed 3\nset cubicmodulus 0.7 0.3 0.2\n"
+              f"readhere varianttensor\n{tensor_lines(eps)}set initeta uniform -0.05 0.05 7\n"
+              f"set binary on\ndump out 1 eta\ninfo 3 eta step\nrun {steps}\n")
+    out = run_gsi(tmp_path, script)
+    got = read_binary(tmp_path / "out" / f"eta.{steps}.data")
+    cfg = gsk.default_config(gsk.PHYS_MART, n=n, nvar=V, dt=0.01, A=-0.5, B=3.0, C=3.0, w_el=20.0, noise=0.002,
+                             seed=3)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(gsk.cubic_modulus(0.7, 0.3, 0.2), eps)
+        s.init_uniform("eta", -0.05, 0.05, 7)
+        s.step(steps)
+        want = s.get_field("eta")
+    assert "Info" in out
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_gsi_reference_mart_with_temperature_ramp(gpu, tmp_path):
+    say("gsi reference martensite with a temperature ramp")
+    n, V, steps = 32, 3, 10
+    eps = gsk.mart_eigenstrains(V, base=0.0, axis=0.01)
+    script = (f"sys mart\nset gridsize {n} {n} {n} 1 1 1\nset rule reference\nset deltatime 0.005\n"
+              f"set coefficient 0 32.05 37.5\nset weightelastic 1000\nset weightnoise 0.0001\nset seed 6\n"
+              f"readhere varianttensor\n{tensor_lines(eps)}set initeta uniform 0 0.2 3\n"
+              f"set fix temperature 300 250\nset binary on\ndump out 1 eta\ninfo 2 eta temperature\nrun {steps}\n")
+    run_gsi(tmp_path, script)
+    got = read_binary(tmp_path / "out" / f"eta.{steps}.data")
+    cfg = gsk.reference_config(gsk.PHYS_MART, n=n, nvar=V, dt=0.005, B=32.05, C=37.5, w_el=1000.0, noise=1e-4, seed=6)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(gsk.cubic_modulus(3.5, 1.5, 1.0), eps)
+        s.init_uniform("eta", 0.0, 0.2, 3)
+        for i in range(1, steps + 1):
+            # GS::Run calls Fix(i / total) before step i; the temperature travels as text (6 digits)
+            s.set_param("temperature", float(f"{300 + (i / steps) * (250 - 300):.6g}"))
+            s.step(1)
+        want = s.get_field("eta")
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
