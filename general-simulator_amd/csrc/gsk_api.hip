@@ -296,6 +296,10 @@ struct Impl {
     CHK(lds_limit((const void*)k_zfwd<M, Z_COPY>));
     CHK(lds_limit((const void*)k_zfwd<M, Z_SQUARE>));
     CHK(lds_limit((const void*)k_zfwd_local<M>));
+    CHK(lds_limit((const void*)k_zfwd_local_r<M, 2>));
+    CHK(lds_limit((const void*)k_zfwd_local_r<M, 3>));
+    CHK(lds_limit((const void*)k_zfwd_local_r<M, 4>));
+    CHK(lds_limit((const void*)k_zfwd_local_r<M, 5>));
     CHK(lds_limit((const void*)k_zmid_mart<M, 0>));
     CHK(lds_limit((const void*)k_zmid_mart<M, 1>));
     CHK(lds_limit((const void*)k_zmid_mart<M, 2>));
@@ -345,9 +349,15 @@ struct Impl {
     } else {
       PrecK pk = prec_consts(s);
       const bool ref = s->cfg.rule == GSK_RULE_REFERENCE;
-      LAUNCH((k_zfwd_local<M>), rows, s->zthreads, zlds(s, F), s, in, s->kbuf, s->g, pk, s->twM, s->rtn,
-             ref ? (const float*)s->rbuf : (const float*)nullptr, ref ? (float)(2.0 * s->cfg.xi) : 0.f,
-             ref ? (float)s->cfg.noise : 0.f);
+      const float* R = ref ? (const float*)s->rbuf : (const float*)nullptr;
+      const float el = ref ? (float)(2.0 * s->cfg.xi) : 0.f, cn = ref ? (float)s->cfg.noise : 0.f;
+      switch (F) {  // every field of a tile in registers for up to 5 fields
+        case 2: LAUNCH((k_zfwd_local_r<M, 2>), rows, s->zthreads, zlds(s, 1), s, in, s->kbuf, s->g, pk, s->twM, s->rtn, R, el, cn); break;
+        case 3: LAUNCH((k_zfwd_local_r<M, 3>), rows, s->zthreads, zlds(s, 1), s, in, s->kbuf, s->g, pk, s->twM, s->rtn, R, el, cn); break;
+        case 4: LAUNCH((k_zfwd_local_r<M, 4>), rows, s->zthreads, zlds(s, 1), s, in, s->kbuf, s->g, pk, s->twM, s->rtn, R, el, cn); break;
+        case 5: LAUNCH((k_zfwd_local_r<M, 5>), rows, s->zthreads, zlds(s, 1), s, in, s->kbuf, s->g, pk, s->twM, s->rtn, R, el, cn); break;
+        default: LAUNCH((k_zfwd_local<M>), rows, s->zthreads, zlds(s, F), s, in, s->kbuf, s->g, pk, s->twM, s->rtn, R, el, cn);
+      }
     }
     return GSK_OK;
   }
@@ -388,6 +398,11 @@ struct Impl {
   template <int MODE>
   static int xupdate_prec(gsk_sim* s, const PrecK& pk) {
     const int gx = s->g.ntile;
+    if (GSK_XUPD_G == 0) {  // one field per work group, one launch
+      LAUNCH((k_xupdate_prec<N, 1, MODE>), dim3(gx, s->F), s->xthreads, s->lds_y, s, s->kbuf, s->state, s->Bt,
+             geo_next(s), pk, s->twN, 0);
+      return GSK_OK;
+    }
     for (int f0 = 0; f0 < s->F; f0 += 2) {
       if (s->F - f0 >= 2)
         LAUNCH((k_xupdate_prec<N, 2, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->Bt, s->g, pk, s->twN,
@@ -771,10 +786,14 @@ void build_slots(gsk_sim* s) {
   } else {
     s->slots.push_back({"zfwd_local", F * (rv + hs)});
     s->slots.push_back({"y_fwd", F * 2 * hs});
-    for (int f0 = 0; f0 < F; f0 += 2) {
-      const int c = std::min(2, F - f0);
-      s->slots.push_back({f0 == 0 ? "x_update_g0" : (f0 == 2 ? "x_update_g1" : (f0 == 4 ? "x_update_g2" : "x_update_g3")),
-                          c * 4 * hs + (f0 == 0 ? hs / 2 : 0.0)});
+    if (GSK_XUPD_G == 0) {  // one launch, as Impl::xupdate_prec
+      s->slots.push_back({"x_update", F * 4 * hs + hs / 2});
+    } else {
+      for (int f0 = 0; f0 < F; f0 += 2) {
+        const int c = std::min(2, F - f0);
+        s->slots.push_back({f0 == 0 ? "x_update_g0" : (f0 == 2 ? "x_update_g1" : (f0 == 4 ? "x_update_g2" : "x_update_g3")),
+                            c * 4 * hs + (f0 == 0 ? hs / 2 : 0.0)});
+      }
     }
     s->slots.push_back({"y_inv", F * 2 * hs});
     s->slots.push_back({"zinv", F * (hs + rv)});

@@ -288,18 +288,29 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_prec(float2* __rest
   float2* tw = smem;
   float2* tile = smem + N;
   const int W = g.W, F = pk.V + 1, T = N * W;
-  const int t = blockIdx.x;
+  const int t = geo_tile(g);
+  if (gridDim.y > 1) f0 += geo_field(g) * G;  // one group of G fields per blockIdx.y
   const STile s = xtile(t, g);
-  load_tw<N>(tw, twN);
-  __syncthreads();
-  Lines Ls{W, W, 1, 0, W};
   const int b = threadIdx.x;
   const bool act = b < W * (N / RL);
   const int c = b % W, j = b / W;
+  // the state does not depend on the transform: fetch it first
+  float2 sv[GSK_XUPD_PF ? G : 1][RL];
+  const float* Bb = Bt + (long)t * T + b;
+  if (GSK_XUPD_PF && MODE == 0 && act) {
+#pragma unroll
+    for (int gi = 0; gi < (GSK_XUPD_PF ? G : 1); ++gi) {
+      const float2* st = state + ((long)t * F + f0 + gi) * T + b;
+#pragma unroll
+      for (int r = 0; r < RL; ++r) sv[gi][r] = st[r * (N / RL) * W];
+    }
+  }
+  load_tw<N>(tw, twN);
+  __syncthreads();
+  Lines Ls{W, W, 1, 0, W};
   const int ky = s.ky0 + s.dky * c, kz = s.kz0 + s.dkz * c;
   const float kyv = kcomp(ky, N, pk.kfac);
   const float kzv = pk.kfac * (float)kz;
-  const float* Bb = Bt + (long)t * T + b;
   {
     float2 v[G][RL];
 #pragma unroll
@@ -323,11 +334,12 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_prec(float2* __rest
             } else {
               const float kxv = kcomp(kx, N, pk.kfac);
               const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
-              const float2 sh = st[e];
+              const float2 sh = GSK_XUPD_PF ? sv[GSK_XUPD_PF ? gi : 0][r] : st[e];
               const float2 fh = v[gi][r];
               float2 nh;
               if (f == 0) {
-                const float a = (pk.celastic ? pk.xi * Bb[e] : 0.f) - pk.S;
+                const float bv = pk.celastic ? Bb[e] : 0.f;
+                const float a = pk.xi * bv - pk.S;
                 const float mx = fh.x + a * sh.x, my = fh.y + a * sh.y;
                 const float den = 1.f / (1.f + pk.dtM * pk.S * k2 + pk.dtMkc * k2 * k2);
                 nh = make_float2((sh.x - pk.dtM * k2 * mx) * den, (sh.y - pk.dtM * k2 * my) * den);

@@ -418,4 +418,94 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local(const float* __res
   for (int f = 0; f < F; ++f) z_store_spec<M>(tile + f * L * (M + 1), kout + f * g.kfs + r0 * g.Pz, g.Pz, rt);
 }
 
+// Same as k_zfwd_local for a compile-time field count FT = V + 1 (2..5): every field of the
+// tile is loaded into registers first (16-byte quads), then the output fields are built and
+// transformed one after the other over a single LDS tile.
+template <int M, int FT>
+__global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local_r(const float* __restrict__ in, float2* __restrict__ kout,
+                                                               Geo g, PrecK pk, const float2* __restrict__ twM,
+                                                               const float2* __restrict__ rtn,
+                                                               const float* __restrict__ R, float el, float cnoise) {
+  using S = ZShape<M>;
+  constexpr int n = 2 * M, V = FT - 1;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  float2* rt = smem + M;
+  float2* tile = smem + 2 * M;
+  const long r0 = (long)blockIdx.x * S::L;
+  const float* src = in + r0 * n;
+  float4 v[S::NT][FT];
+#pragma unroll
+  for (int i = 0; i < S::NT; ++i) {
+    int l, q;
+    z_quad<M>(i, l, q);
+    if (S::valid(threadIdx.x + i * S::T)) {
+#pragma unroll
+      for (int f = 0; f < FT; ++f) v[i][f] = *reinterpret_cast<const float4*>(src + f * g.nvox + l * n + 4 * q);
+    }
+  }
+  z_twiddles<M>(tw, rt, twM, rtn);
+  const uint32_t step = step_of(pk.step, pk.stepp);
+#pragma unroll 1
+  for (int f = 0; f < FT; ++f) {
+#pragma unroll
+    for (int i = 0; i < S::NT; ++i) {
+      int l, q;
+      z_quad<M>(i, l, q);
+      if (!S::valid(threadIdx.x + i * S::T)) continue;
+      const long vox = (r0 + l) * n + 4 * q;
+      float e[FT][4];
+#pragma unroll
+      for (int u = 0; u < FT; ++u) {
+        e[u][0] = v[i][u].x; e[u][1] = v[i][u].y; e[u][2] = v[i][u].z; e[u][3] = v[i][u].w;
+      }
+      float out[4];
+      if (f == 0) {
+        float xc[4] = {0.f, 0.f, 0.f, 0.f};
+        if (cnoise != 0.f) normal4(pk.seed, (uint32_t)(vox >> 2), 0x8000u, step, xc);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          float s2 = 0.f;
+#pragma unroll
+          for (int a = 1; a <= V; ++a) s2 += e[a][k] * e[a][k];
+          out[k] = 0.5f * pk.A2 * s2 + pk.A1 * (pk.cbar + e[0][k] - pk.c1) + cnoise * xc[k];
+        }
+      } else {
+        float xi[4] = {0.f, 0.f, 0.f, 0.f};
+        if (pk.noise != 0.f) normal4(pk.seed, (uint32_t)(vox >> 2), (uint32_t)(f - 1), step, xi);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          float et = 0.f, t2 = 0.f, t4 = 0.f;
+#pragma unroll
+          for (int b = 1; b <= V; ++b) {
+            const float x = e[b][k];
+            if (b == f) {
+              et = x;
+            } else {
+              const float qq = x * x;
+              t2 += qq;
+              t4 += qq * qq;
+            }
+          }
+          // ordered pairs i != j, both != f: (sum q)^2 - sum q^2
+          const float t22 = t2 * t2 - t4;
+          const float c = pk.cbar + e[0][k];
+          const float e2 = et * et;
+          float o = (2.f * pk.A5 * et + 4.f * pk.A6 * e2 * et) * t2 + 2.f * pk.A6 * et * t4 + 2.f * pk.A7 * et * t22 +
+                    pk.A2 * et * (c - pk.c2) - pk.A3 * e2 * et + pk.A4 * e2 * e2 * et + pk.noise * xi[k];
+          if (R) o += el * R[(f - 1) * g.nvox + vox + k] * et;
+          out[k] = o;
+        }
+      }
+      float2* row = tile + l * (M + 1);
+      row[2 * q] = make_float2(out[0], out[1]);
+      row[2 * q + 1] = make_float2(out[2], out[3]);
+    }
+    __syncthreads();
+    z_fft1<M, true>(tile, tw);
+    z_store_spec<M>(tile, kout + f * g.kfs + r0 * g.Pz, g.Pz, rt);
+    __syncthreads();  // the next field overwrites the tile
+  }
+}
+
 }  // namespace gsk
