@@ -17,7 +17,14 @@
 namespace gs {
 
 // return codes mirror src/pub.h:17-22
-enum Code { Code_NORMAL = 0, Code_ERR = -1, Code_QUIT = -2, Code_COMMAND_UNKNOW = -3 };
+enum Code {
+  Code_NORMAL = 0,
+  Code_ERR = -1,
+  Code_BREAK = -2,
+  Code_QUIT = -3,
+  Code_COMMAND_UNKNOW = -4,
+  Code_INPUT_UNCOMPLETE = -5
+};
 
 class Dynamics {
  public:

@@ -610,8 +610,72 @@ int Interpreter::readhere(const std::string& rest, std::string& script) {
   return gs_.ReadHere(name, vals);
 }
 
+// The lines of a loop / while / if block up to its matching `end` (nested blocks count),
+// with the `else` part of an if split off (src/input.cu:254-336). Consumes them from script.
+static int split_block(std::string& script, std::string& body, std::string* other) {
+  int depth = 1;
+  bool in_else = false;
+  std::string line;
+  body.clear();
+  if (other) other->clear();
+  while (next_line(script, line)) {
+    std::istringstream ls(line);
+    std::string w;
+    ls >> w;
+    if (w == "loop" || w == "while" || w == "if") {
+      ++depth;
+    } else if (w == "end") {
+      if (--depth == 0) return Code_NORMAL;
+    } else if (w == "else" && depth == 1 && other) {
+      in_else = true;
+      continue;
+    }
+    (in_else ? *other : body) += line + "\n";
+  }
+  return Code_INPUT_UNCOMPLETE;
+}
+
+// loop N / while expr / if expr [else] ... end (src/input.cu:269-336)
+int Interpreter::control(const std::string& cmd, const std::string& rest, std::string& script) {
+  std::string body, other;
+  if (split_block(script, body, cmd == "if" ? &other : nullptr) != Code_NORMAL) {
+    log_line("Error: " + cmd + " without end");
+    return Code_INPUT_UNCOMPLETE;
+  }
+  auto value = [&](double* v) -> int {
+    std::string e = rest;
+    if (substitute(e)) return Code_ERR;
+    return eval_expr(e, vars_, gs_.Vars, v);
+  };
+  double v = 0.0;
+  if (cmd == "loop") {
+    if (value(&v)) return Code_ERR;
+    for (long i = 0; i < (long)v; ++i) {
+      std::string b = body;
+      const int e = exec(b);
+      if (e != Code_NORMAL) return e;
+    }
+    return Code_NORMAL;
+  }
+  if (cmd == "while") {
+    for (;;) {
+      if (value(&v)) return Code_ERR;
+      if (!(v > 0)) return Code_NORMAL;
+      std::string b = body;
+      const int e = exec(b);
+      if (e != Code_NORMAL) return e;
+    }
+  }
+  if (value(&v)) return Code_ERR;
+  return exec(v > 0 ? body : other);
+}
+
 int Interpreter::run(std::string script) {
   script = standardize(script);
+  return exec(script);
+}
+
+int Interpreter::exec(std::string& script) {
   std::string line;
   while (next_line(script, line)) {
     size_t b = line.find_first_not_of(' ');
@@ -627,6 +691,16 @@ int Interpreter::run(std::string script) {
     if (!cmd.empty() && cmd[0] == ':') {
       rest = cmd.substr(1) + rest;
       cmd = "expr";
+    }
+    if (cmd == "loop" || cmd == "while" || cmd == "if") {
+      if (log_echo_on()) log_line(">>>" + raw);
+      const int e = control(cmd, rest, script);
+      if (e == Code_QUIT) return Code_QUIT;
+      if (e != Code_NORMAL) {
+        log_line("Error: error in command \"" + raw + "\"");
+        return Code_ERR;
+      }
+      continue;
     }
     if (cmd != "print" && cmd != "expr") {
       if (substitute(rest)) return Code_ERR;
@@ -679,9 +753,6 @@ int Interpreter::run(std::string script) {
       gs_.Vars["device"] = std::to_string(device_);
     } else if (cmd == "runfunc") {
       err = gs_.RunFunc(rest);
-    } else if (cmd == "loop" || cmd == "while" || cmd == "if") {
-      log_line("Error: control flow \"" + cmd + "\" is not supported by this host");
-      err = Code_ERR;
     } else {
       err = gs_.RunFunc(cmd + " " + rest);
       if (err == Code_COMMAND_UNKNOW) {

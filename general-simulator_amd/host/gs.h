@@ -57,6 +57,8 @@ class Interpreter {
   VarMap& vars() { return vars_; }
 
  private:
+  int exec(std::string& script);  // runs standardized lines, consuming script
+  int control(const std::string& cmd, const std::string& rest, std::string& script);
   int readhere(const std::string& rest, std::string& script);
   int shell_capture(const std::string& cmd, std::string& out);
   GS& gs_;

@@ -86,3 +86,20 @@ def test_non_cubic_grid_rejected(tmp_path):
     rc, out = run(tmp_path, "sys mart\nset gridsize 16 16 32 1 1 1\nreadhere varianttensor\n3 1 3 3\n"
                             "1 0 0 0 1 0 0 0 1\nrun 1\n")
     assert rc == 1 and "periodic n x n x n" in out
+
+
+def test_control_flow_loop_while_if(tmp_path):
+    """loop / while / if-else-end with nesting (src/input.cu:254-336): the block runs again
+    for every pass, `{}` is substituted per pass, `else` belongs to the innermost if."""
+    script = ("set log off\n:s=0\nloop 4\n  :s=s+1\n  if (s>2)\n    print big{s}\n  else\n    print small{s}\n  end\nend\n"
+              ":k=5\nwhile k>1\n  :k=k-2\n  loop 2\n    print k{k}\n  end\nend\n"
+              "if (k<0)\n  print neg\nend\nprint done{s}{k}\n")
+    rc, out = run(tmp_path, script)
+    assert rc == 0, out
+    lines = [l for l in out.splitlines() if not l.startswith(">>>") and not l.startswith("Input script")]
+    assert lines == ["small1", "small2", "big3", "big4", "k3", "k3", "k1", "k1", "done41"]
+
+
+def test_control_flow_without_end_is_an_error(tmp_path):
+    rc, out = run(tmp_path, "loop 3\nprint x\n")
+    assert rc != 0 and "without end" in out
