@@ -42,7 +42,7 @@ __global__ void k_btensor_half(float* __restrict__ Bh, Geo g, BArgs a) {
     long row;
     half_index(i, g, kx, ky, kz, row);
     double out[64];
-    btensor_at(kx, ky, kz, g.n, a, out);
+    btensor_half_spec(kx, ky, kz, g.n, a, out);
     for (int p = 0; p < V; ++p)
       for (int q = p; q < V; ++q) {
         const int pq = p * V - (p * (p - 1)) / 2 + (q - p);
@@ -166,6 +166,52 @@ __global__ void k_adj_lambda(const float* __restrict__ eta, const float* __restr
   }
   const double r = block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// Full (not packed) rectangular B_pq, p < V1, q < V2, on the half spectrum in natural layout:
+// Bh[(p*V2 + q)*kfs + row*Pz + kz] (the stress plugin's 6 base strains x nvar variants).
+__global__ void k_btensor_half_rect(float* __restrict__ Bh, Geo g, BArgs a) {
+  const long total = (long)g.n * g.n * (g.M + 1);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    int kx, ky, kz;
+    long row;
+    half_index(i, g, kx, ky, kz, row);
+    double out[64];
+    btensor_half_spec(kx, ky, kz, g.n, a, out);
+    for (int pq = 0; pq < a.V1 * a.V2; ++pq) Bh[pq * g.kfs + row * g.Pz + kz] = (float)out[pq];
+  }
+}
+
+// In place on natural-layout spectra: fields 0..V2-1 hold X_q; fields 0..V1-1 receive
+// scale * sum_q B_pq X_q (every input of a wave vector is read before any output is written).
+__global__ void k_contract_rect(float2* __restrict__ kb, const float* __restrict__ Bh, Geo g, int V1, int V2,
+                                float scale) {
+  const long per = (long)g.n * g.n * (g.M + 1);
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < per; i += (long)gridDim.x * blockDim.x) {
+    int kx, ky, kz;
+    long row;
+    half_index(i, g, kx, ky, kz, row);
+    const long off = row * g.Pz + kz;
+    float2 X[8];
+    for (int q = 0; q < V2; ++q) X[q] = kb[q * g.kfs + off];
+    float2 Y[8];
+    for (int p = 0; p < V1; ++p) {
+      float2 acc = make_float2(0.f, 0.f);
+      for (int q = 0; q < V2; ++q) {
+        const float bv = Bh[(p * V2 + q) * g.kfs + off];
+        acc.x += bv * X[q].x;
+        acc.y += bv * X[q].y;
+      }
+      Y[p] = make_float2(acc.x * scale, acc.y * scale);
+    }
+    for (int p = 0; p < V1; ++p) kb[p * g.kfs + off] = Y[p];
+  }
+}
+
+// x <- x^2 on count floats (the eta^2 source)
+__global__ void k_square(float* __restrict__ x, long count) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x)
+    x[i] = x[i] * x[i];
 }
 
 }  // namespace gsk

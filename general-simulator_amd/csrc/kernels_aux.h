@@ -17,8 +17,8 @@ struct BArgs {
   double kfac;
 };
 
-__device__ __forceinline__ void btensor_at(int kx, int ky, int kz, int n, const BArgs& a, double* out) {
-  const int mx = (kx < n / 2) ? kx : kx - n, my = (ky < n / 2) ? ky : ky - n, mz = (kz < n / 2) ? kz : kz - n;
+// B at the integer wave vector (mx, my, mz)
+__device__ __forceinline__ void btensor_m(int mx, int my, int mz, const BArgs& a, double* out) {
   const double g0 = a.kfac * mx, g1 = a.kfac * my, g2 = a.kfac * mz;
   const double mo = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
   double u[3] = {0.0, 0.0, 0.0};
@@ -70,6 +70,29 @@ __device__ __forceinline__ void btensor_at(int kx, int ky, int kz, int n, const 
   }
 }
 
+// B at FFT index (kx, ky, kz) with the reference's mapping m -> m - n for m >= n/2
+// (src/gtensorb.cu:9-12): the Nyquist index n/2 is taken as -n/2.
+__device__ __forceinline__ void btensor_at(int kx, int ky, int kz, int n, const BArgs& a, double* out) {
+  btensor_m((kx < n / 2) ? kx : kx - n, (ky < n / 2) ? ky : ky - n, (kz < n / 2) ? kz : kz - n, a, out);
+}
+
+// B for products with half (R2C) spectra. The reference multiplies the full complex
+// spectrum by B at index k and keeps the real part of the inverse transform, which equals
+// multiplying by (B(m) + B(m'))/2, m' = m with its Nyquist components (index n/2) negated:
+// the index -k of a Nyquist component is itself, and B is even. A half spectrum is
+// implicitly Hermitian, so the half-spectrum kernels use this average (it differs from
+// B(m) only on Nyquist planes, and only when B is not symmetric under a reflection).
+__device__ __forceinline__ void btensor_half_spec(int kx, int ky, int kz, int n, const BArgs& a, double* out) {
+  const int h = n / 2;
+  const int mx = (kx < h) ? kx : kx - n, my = (ky < h) ? ky : ky - n, mz = (kz < h) ? kz : kz - n;
+  btensor_m(mx, my, mz, a, out);
+  if (kx == h || ky == h || kz == h) {
+    double o2[64];
+    btensor_m(kx == h ? -mx : mx, ky == h ? -my : my, kz == h ? -mz : mz, a, o2);
+    for (int i = 0; i < a.V1 * a.V2; ++i) out[i] = 0.5 * (out[i] + o2[i]);
+  }
+}
+
 // Full layout B[p][q][x][y][z], FP64.
 __global__ void k_btensor_full(double* __restrict__ B, int n, BArgs a) {
   const long N3 = (long)n * n * n;
@@ -93,7 +116,7 @@ __global__ void k_btensor_tiles(float* __restrict__ Bt, Geo g, BArgs a) {
     const int c = e % W, kx = e / W;
     const int ky = s.ky0 + s.dky * c, kz = s.kz0 + s.dkz * c;
     double out[64];
-    btensor_at(kx, ky, kz, n, a, out);
+    btensor_half_spec(kx, ky, kz, n, a, out);
     for (int p = 0; p < V; ++p)
       for (int q = p; q < V; ++q) {
         const int pq = p * V - (p * (p - 1)) / 2 + (q - p);

@@ -102,6 +102,7 @@ def lib() -> ctypes.CDLL:
     L.gsk_adjoint_backward.argtypes = [P, P, P, P]
     L.gsk_adjoint_info.argtypes = [P, P]
     L.gsk_adjoint_release.argtypes = [P]
+    L.gsk_stress.argtypes = [I, D, I, P, P, I, D, I, P, P, P, I]
     _lib = L
     return L
 
@@ -432,6 +433,19 @@ def philox4x32(ctr, seed: int) -> np.ndarray:
 def philox_uniform(seed: int, i: int, j: int) -> float:
     """u in [0, 1) from word 0 of the block at counter (i, j, 0, 2): (x >> 8) * 2^-24."""
     return float(philox4x32([i, j, 0, 2], seed)[0] >> 8) * 2.0**-24
+
+
+def stress(eta: np.ndarray, C: np.ndarray, eps: np.ndarray, xi: float = 4000.0, source: int = 2, dx: float = 1.0,
+           kconv: int = K_EXACT, device: int = 0):
+    """The stress plugin on the GPU: returns (stress [6,n,n,n], defect [n,n,n]) as float32."""
+    e = np.ascontiguousarray(eta, dtype=np.float32)
+    V, n = e.shape[0], e.shape[1]
+    Cc = np.ascontiguousarray(C, dtype=np.float64).reshape(81)
+    ep = np.ascontiguousarray(eps, dtype=np.float64).reshape(-1, 9)
+    out = np.empty((6, n, n, n), np.float32)
+    d = np.empty((n, n, n), np.float32)
+    _check(lib().gsk_stress(n, dx, kconv, _ptr(Cc), _ptr(ep), V, xi, source, _ptr(e), _ptr(out), _ptr(d), device))
+    return out, d
 
 
 def version() -> str:

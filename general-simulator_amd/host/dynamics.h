@@ -39,6 +39,9 @@ class Dynamics {
   virtual int Fix(double progress) = 0;
   virtual std::string Get(const std::string& col) = 0;
   virtual int Pull(const std::string& /*name*/) { return Code_NORMAL; }
+  // Push() is the reverse of Pull(): it uploads a field of the map into the plugin's device copy
+  // (the reference's systems share one set of device arrays; here each GPU plugin holds its own).
+  virtual int Push(const std::string& /*name*/) { return Code_NORMAL; }
   // Several steps at once when nothing between them needs the host (Fix is a no-op
   // for the physics); the default calls Calculate n times.
   virtual int CalculateMany(int n) {

@@ -295,8 +295,17 @@ GS::GS() = default;
 GS::~GS() = default;
 
 int GS::SetSys(const std::string& name) {
+  // hand the shared fields over between systems (one set of arrays in the reference)
+  static const char* const shared[] = {"eta", "concentration"};
+  if (!current_.empty() && current_ != name)
+    for (const char* f : shared)
+      if (int e = sys_[current_]->Pull(f)) return e;
   if (sys_.count(name)) {
+    const bool handover = current_ != name && inited_[name];
     current_ = name;
+    if (handover)
+      for (const char* f : shared)
+        if (int e = sys_[name]->Push(f)) return e;
     return Code_NORMAL;
   }
   auto it = registry().find(name);

@@ -190,6 +190,16 @@ class GpuDynamics : public Dynamics {
     return check(gsk_get_field(sim_, name.c_str(), fd.data.data(), GSK_F64));
   }
 
+  int Push(const std::string& name) override {
+    if (!sim_) return Code_NORMAL;
+    const bool ok = name == "eta" || (name == "concentration" && cfg_.physics == GSK_PHYS_PREC);
+    auto it = Datas->find(name);
+    if (!ok || it == Datas->end() || it->second.empty()) return Code_NORMAL;
+    const long long want = (long long)(name == "eta" ? nvar_ : 1) * nx * ny * nz;
+    if (it->second.count() != want) return Code_NORMAL;  // not this system's field
+    return check(gsk_set_field(sim_, name.c_str(), it->second.data.data(), GSK_F64));
+  }
+
   int pull(const std::string& name, std::vector<int> dims) {
     Field& f = (*Datas)[name];
     f.init(dims);
@@ -388,7 +398,190 @@ class Dynamics_precipitate : public GpuDynamics {
   }
 };
 
+// ------------------------------------------------------------------------------ stress
+// Elastic stress of the order parameters (src/dynamics_stress.cu:19-194), the input of the
+// martensite plugin's dislocation term: `stress` (6,n,n,n; components 00,01,02,11,12,22)
+// and `defect` (n,n,n; 1 where any |eta_v| > 0.9), computed on the GPU by gsk_stress.
+class Dynamics_stress : public GpuDynamics {
+ public:
+  int Initialize() override {
+    if (read_grid()) return Code_ERR;
+    gsk_config_default(&cfg_, GSK_PHYS_MART);
+    read_common();
+    var_get(*Vars, "xi", &xi_, 1);  // default 4000 (:25)
+    std::vector<std::string> w;
+    if (var_values(*Vars, "stress_source", w) && w[0] == "eta") source_ = 1;  // master side of :143-147
+    auto vt = Datas->find("varianttensor");
+    if (vt == Datas->end() || vt->second.empty() || vt->second.count() % 9 != 0) {
+      log_line("Error: variants' strain tensor does not set while initialize dynamics");  // :27-30
+      return Code_ERR;
+    }
+    nvar_ = (int)(vt->second.count() / 9);
+    eps_ = vt->second.data;
+    auto md = Datas->find("modulus");
+    if (md != Datas->end() && md->second.count() == 81) {
+      C_ = md->second.data;
+    } else {  // the plugin's own default modulus (:44-64)
+      C_.assign(81, 0.0);
+      auto c = [&](int i, int j, int k, int l, double v) { C_[((i * 3 + j) * 3 + k) * 3 + l] = v; };
+      c(0, 0, 0, 0, 5.39); c(1, 1, 1, 1, 5.39); c(2, 2, 2, 2, 5.22);
+      c(0, 0, 1, 1, 3.39); c(1, 1, 0, 0, 3.39);
+      c(1, 1, 2, 2, 3.56); c(2, 2, 1, 1, 3.56); c(2, 2, 0, 0, 3.56); c(0, 0, 2, 2, 3.56);
+      for (auto p : {std::make_pair(0, 1), std::make_pair(1, 0)})
+        for (auto q : {std::make_pair(0, 1), std::make_pair(1, 0)}) c(p.first, p.second, q.first, q.second, 0.6);
+      for (auto p : {std::make_pair(1, 2), std::make_pair(2, 1)})
+        for (auto q : {std::make_pair(1, 2), std::make_pair(2, 1)}) c(p.first, p.second, q.first, q.second, 0.77);
+      for (auto p : {std::make_pair(0, 2), std::make_pair(2, 0)})
+        for (auto q : {std::make_pair(0, 2), std::make_pair(2, 0)}) c(p.first, p.second, q.first, q.second, 0.77);
+    }
+    Field& eta = (*Datas)["eta"];
+    if (eta.empty()) eta.init({nvar_, nx, ny, nz});  // zero when not read (:36-41)
+    if (eta.count() != (long long)nvar_ * nx * ny * nz) {
+      log_line("Error: eta does not match gridsize and the variant count");
+      return Code_ERR;
+    }
+    return Code_NORMAL;
+  }
+
+  int Calculate() override {
+    const Field& eta = (*Datas)["eta"];
+    std::vector<float> e(eta.data.begin(), eta.data.end());
+    const long long n3 = (long long)nx * ny * nz;
+    std::vector<float> st((size_t)(6 * n3)), df((size_t)n3);
+    if (check(gsk_stress(nx, dx, cfg_.kconv, C_.data(), eps_.data(), nvar_, xi_, source_, e.data(), st.data(),
+                         df.data(), cfg_.device)))
+      return Code_ERR;
+    Field& s = (*Datas)["stress"];
+    s.init({6, nx, ny, nz});
+    for (size_t i = 0; i < st.size(); ++i) s.data[i] = st[i];
+    Field& d = (*Datas)["defect"];
+    d.init({nx, ny, nz});
+    for (size_t i = 0; i < df.size(); ++i) d.data[i] = df[i];
+    return Code_NORMAL;
+  }
+  int CalculateMany(int) override { return Calculate(); }  // the result does not change with steps
+
+  std::string Get(const std::string&) override { return "nan"; }
+  int Pull(const std::string&) override { return Code_NORMAL; }
+
+ private:
+  double xi_ = 4000.0;
+  int source_ = 2;  // eta^2 (HEAD side)
+  std::vector<double> eps_, C_;
+};
+
+// ------------------------------------------------------------------------------ cores
+// Initial microstructure of spherical nuclei (src/dynamics_cores.cu:11-195): eta_v = 1 and
+// c = c2 inside each core, c = c1 elsewhere; periodic images along each axis. The reference
+// draws positions and variants with libc random(); here Philox4x32-10 at counter
+// (core, attempt, call, 3) keyed by `seed` draws them (x, y, z, variant from words 0-3).
+class Dynamics_cores : public GpuDynamics {
+ public:
+  int Initialize() override {
+    if (Vars->find("gridsize") == Vars->end()) {  // :14 defaults
+      nx = ny = nz = 16;
+      dx = dy = dz = 0.1;
+    } else if (read_grid()) {
+      return Code_ERR;
+    }
+    var_get(*Vars, "variantn", &V_, 1);
+    var_get(*Vars, "coresn", &cores_, 1);
+    var_get(*Vars, "radius", &radius_, 1);
+    double cc[2] = {c1_, c2_};
+    var_get(*Vars, "concentration", cc, 2);
+    c1_ = cc[0];
+    c2_ = cc[1];
+    std::vector<std::string> w;
+    if (var_values(*Vars, "method", w)) method_ = w[0];
+    var_get(*Vars, "seed", &seed_, 1);
+    if (V_ < 1 || cores_ < 0 || radius_ < 0) {
+      log_line("Error: cores needs variantn >= 1, coresn >= 0, radius >= 0");
+      return Code_ERR;
+    }
+    (*Datas)["eta"].init({V_, nx, ny, nz});
+    (*Datas)["concentration"].init({nx, ny, nz}, c1_);
+    return Code_NORMAL;
+  }
+
+  int Calculate() override {
+    Field& eta = (*Datas)["eta"];
+    Field& c = (*Datas)["concentration"];
+    mark_.assign((size_t)nx * ny * nz, 0);
+    if (method_ == "random") {
+      eta.init({V_, nx, ny, nz});
+      c.init({nx, ny, nz}, c1_);
+      for (int cn = 0; cn < cores_; ++cn) {
+        unsigned int o[4] = {0, 0, 0, 0};
+        for (unsigned int t = 0; t < 100000; ++t) {
+          const unsigned int ctr[4] = {(unsigned)cn, t, (unsigned)calls_, 3u};
+          gsk_philox4x32(ctr, seed_, o);
+          if (!mark_[((size_t)(o[0] % nx) * ny + o[1] % ny) * nz + o[2] % nz]) break;
+        }
+        place(eta, c, (int)(o[0] % nx), (int)(o[1] % ny), (int)(o[2] % nz), (int)(o[3] % V_));
+      }
+    } else if (method_ == "regular") {  // :86-115, variant cn % V
+      for (int cn = 0; cn < cores_; ++cn) place(eta, c, nx / 2, ny / 2, nz / cores_ / 2 + nx / cores_ * cn, cn % V_);
+    } else if (method_ == "regular2d" || method_ == "regular3d") {  // :116-190
+      const int d3 = method_ == "regular3d";
+      int id = 0;
+      for (int cn1 = 0; cn1 < cores_; ++cn1)
+        for (int cn2 = 0; cn2 < cores_; ++cn2)
+          for (int cn3 = 0; cn3 < (d3 ? cores_ : 1); ++cn3, ++id) {
+            const int ox = d3 ? nx / cores_ / 2 + nx / cores_ * cn3 : nx / 2;
+            const int oy = ny / cores_ / 2 + ny / cores_ * cn2;
+            const int oz = nz / cores_ / 2 + nx / cores_ * cn1;
+            unsigned int o[4];
+            const unsigned int ctr[4] = {(unsigned)id, 0u, (unsigned)calls_, 3u};
+            gsk_philox4x32(ctr, seed_, o);
+            place(eta, c, ox, oy, oz, (int)(o[3] % V_));
+          }
+    } else {
+      log_line("Error: cores method " + method_ + " is unknown");
+      return Code_ERR;
+    }
+    ++calls_;
+    return Code_NORMAL;
+  }
+  int CalculateMany(int n) override {
+    for (int i = 0; i < n; ++i)
+      if (Calculate() != Code_NORMAL) return Code_ERR;
+    return Code_NORMAL;
+  }
+
+  std::string Get(const std::string&) override { return "nan"; }
+  int Pull(const std::string&) override { return Code_NORMAL; }
+
+ private:
+  int V_ = 4, cores_ = 5, radius_ = 5;
+  double c1_ = 0.2, c2_ = 0.44;
+  std::string method_ = "random";
+  unsigned long long seed_ = 0;
+  int calls_ = 0;
+  std::vector<char> mark_;
+
+  // a sphere of radius r about (ox, oy, oz) and its images shifted by one period along
+  // one axis (the seven tests of the reference)
+  void place(Field& eta, Field& c, int ox, int oy, int oz, int v) {
+    const long long r2 = (long long)radius_ * radius_;
+    auto in = [&](long long a, long long b, long long d) { return a * a + b * b + d * d <= r2; };
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) {
+          const long long a = i - ox, b = j - oy, d = k - oz;
+          if (in(a, b, d) || in(a + nx, b, d) || in(a - nx, b, d) || in(a, b + ny, d) || in(a, b - ny, d) ||
+              in(a, b, d + nz) || in(a, b, d - nz)) {
+            const size_t idx = ((size_t)i * ny + j) * nz + k;
+            mark_[idx] = 1;
+            c.data[idx] = c2_;
+            eta.data[(size_t)v * nx * ny * nz + idx] = 1.0;
+          }
+        }
+  }
+};
+
 Register reg_mart("mart", [] { return std::unique_ptr<Dynamics>(new Dynamics_mart); });
+Register reg_stress("stress", [] { return std::unique_ptr<Dynamics>(new Dynamics_stress); });
+Register reg_cores("cores", [] { return std::unique_ptr<Dynamics>(new Dynamics_cores); });
 Register reg_prec("precipitate", [] { return std::unique_ptr<Dynamics>(new Dynamics_precipitate); });
 
 }  // namespace

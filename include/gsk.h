@@ -154,6 +154,15 @@ int gsk_fft_c2r(int n, int count, const float* host_in, float* host_out, int dev
  * out[4] max |library - rocFFT| / max |rocFFT| over the R2C spectra. */
 int gsk_fft_compare(int n, int count, int reps, double* out, int device);
 
+/* Elastic stress of the order parameters, the stress plugin (src/dynamics_stress.cu:19-194):
+ * with the six unit base strains b (single entries 00, 01, 02, 11, 12, 22) and the nvar
+ * variant eigenstrains eps, stress_b = -xi IFFT[ sum_v B_bv(k) FFT(s_v) ] / n^3 with
+ * s_v = eta_v^2 (source 2, HEAD side) or eta_v (source 1, master side); stress is 6*n^3
+ * floats in the component order the martensite plugin reads as "dislocationstress";
+ * defect (n^3, may be null) is 1 where any |eta_v| > 0.9. Host buffers, synchronous. */
+int gsk_stress(int n, double dx, int kconv, const double* C81, const double* eps, int nvar, double xi, int source,
+               const float* eta, float* stress, float* defect, int device);
+
 /* ---- adjoint of the FP32 fused martensite step (nvar <= 4). No counterpart in the
  * reference (it has no reverse mode); it differentiates the step gsk_step runs.
  * gsk_adjoint_forward advances the handle by nsteps from its current state like gsk_step

@@ -761,6 +761,38 @@ extern "C" void or_prec_ref_run(int n, int V, const or_prec_params* p, const dou
   }
 }
 
+// Stress plugin (src/dynamics_stress.cu:126-185): stress_b = -xi IFFT[sum_v B_bv FFT(s_v)]/N for
+// the 6 base strains b, s_v = eta_v^2 (source 2) or eta_v (source 1); defect = any |eta_v| > 0.9.
+extern "C" void or_stress(int n, int V, const double* B, double xi, int source, const double* eta, double* stress,
+                          double* defect) {
+  const int64_t N = (int64_t)n * n * n;
+  std::vector<cd> S((size_t)V * N), R(N);
+  for (int v = 0; v < V; ++v) {
+    for (int64_t i = 0; i < N; ++i) {
+      const double e = eta[v * N + i];
+      S[v * N + i] = cd(source == 2 ? e * e : e, 0.0);
+    }
+    fft3d(n, S.data() + v * N, -1);
+  }
+  for (int b = 0; b < 6; ++b) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      cd acc = 0;
+      for (int v = 0; v < V; ++v) acc += B[(int64_t)(b * V + v) * N + i] * S[v * N + i];
+      R[i] = acc / (double)N;
+    }
+    fft3d(n, R.data(), +1);
+    for (int64_t i = 0; i < N; ++i) stress[b * N + i] = -xi * R[i].real();
+  }
+  if (defect)
+    for (int64_t i = 0; i < N; ++i) {
+      double d = 0.0;
+      for (int v = 0; v < V; ++v)
+        if (std::fabs(eta[v * N + i]) > 0.9) d = 1.0;
+      defect[i] = d;
+    }
+}
+
 extern "C" void or_prec_energy(int n, int V, const or_prec_params* p, const double* Bc, const double* c,
                                const double* eta, double out[4]) {
   const int64_t N = (int64_t)n * n * n;

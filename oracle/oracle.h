@@ -69,6 +69,10 @@ void or_mart_ref_run(int n, int V, const or_mart_ref_params* p, const double* B,
 void or_prec_ref_run(int n, int V, const or_prec_params* p, const double* B, double* c, double* eta,
                      int step0, int nsteps);
 
+/* Stress plugin: B is the (6 base strains) x V tensor [6][V][n^3]; stress 6*n^3, defect n^3. */
+void or_stress(int n, int V, const double* B, double xi, int source, const double* eta, double* stress,
+               double* defect);
+
 void or_set_threads(int nt);
 int or_get_threads(void);
 

@@ -74,6 +74,7 @@ def lib():
         L.or_mart_force.argtypes = [I, I, ctypes.POINTER(MartParams), P, P, P]
         L.or_mart_adjoint.argtypes = [I, I, ctypes.POINTER(MartParams), P, P, I, I, P, P, P]
         L.or_mart_ref_run.argtypes = [I, I, ctypes.POINTER(MartRefParams), P, P, P, P, I, I]
+        L.or_stress.argtypes = [I, I, P, ctypes.c_double, I, P, P, P]
         L.or_prec_ref_run.argtypes = [I, I, ctypes.POINTER(PrecParams), P, P, P, I, I]
         L.or_prec_run.argtypes = [I, I, ctypes.POINTER(PrecParams), P, P, P, I, I]
         L.or_prec_energy.argtypes = [I, I, ctypes.POINTER(PrecParams), P, P, P, P]
@@ -242,6 +243,22 @@ def prec_ref_run(c, eta, B, p: PrecParams, nsteps: int, step0: int = 0):
     lib().or_prec_ref_run(n, V, ctypes.byref(p), _p(np.ascontiguousarray(B, dtype=np.float64)), _p(cc), _p(e),
                           step0, nsteps)
     return cc, e
+
+
+BASE_STRAINS = np.zeros((6, 3, 3))
+for _b, (_i, _j) in enumerate([(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]):
+    BASE_STRAINS[_b, _i, _j] = 1.0  # single unit entries (src/dynamics_stress.cu:70-75)
+
+
+def stress(eta, C, eps, xi=4000.0, source=2, dx=1.0, kconv=0):
+    """Stress plugin: returns (stress [6,n,n,n], defect [n,n,n])."""
+    e = np.ascontiguousarray(eta, dtype=np.float64)
+    V, n = e.shape[0], e.shape[1]
+    B = np.ascontiguousarray(btensor(n, C, BASE_STRAINS, eps, dx=dx, kconv=kconv))
+    out = np.empty((6, n, n, n))
+    d = np.empty((n, n, n))
+    lib().or_stress(n, V, _p(B), xi, source, _p(e), _p(out), _p(d))
+    return out, d
 
 
 def prec_run(c: np.ndarray, eta: np.ndarray, Bc: np.ndarray, p: PrecParams, nsteps: int, step0: int = 0):

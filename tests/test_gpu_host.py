@@ -87,3 +87,65 @@ def test_gsi_reference_mart_with_temperature_ramp(gpu, tmp_path):
             s.step(1)
         want = s.get_field("eta")
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_stress_matches_oracle(gpu):
+    """gsk_stress (the stress plugin on the GPU) against the FP64 oracle, both sources."""
+    import oracle as O
+
+    say("stress vs oracle")
+    n, V = 32, 4
+    rng = np.random.default_rng(4)
+    eta = rng.uniform(-1.0, 1.0, (V, n, n, n)).astype(np.float32)
+    eps = rng.uniform(-0.02, 0.02, (V, 3, 3))
+    C = gsk.cubic_modulus(5.39, 3.39, 0.6)
+    for source in (2, 1):
+        st, d = gsk.stress(eta, C, eps, xi=4000.0, source=source, dx=0.2, kconv=gsk.K_REFERENCE)
+        wst, wd = O.stress(eta.astype(np.float64), C, eps, xi=4000.0, source=source, dx=0.2, kconv=gsk.K_REFERENCE)
+        err = float(np.linalg.norm((st - wst).ravel()) / np.linalg.norm(wst.ravel()))
+        say(f"  source {source}: rel L2 {err:.2e}")
+        assert err < 1e-5
+        assert np.array_equal(d, wd.astype(np.float32))
+
+
+def test_gsi_pipeline_cores_precipitate_stress_mart(gpu, tmp_path):
+    """The reference's script chain in one gsi run: cores seeds c and eta, the precipitate
+    (reference rule) evolves them, stress turns eta into the dislocation stress and the defect
+    map, and the martensite reference rule consumes both. Checks the hand-offs: stress equals
+    gsk_stress of the dumped eta, and the martensite eta is zero on every defect voxel."""
+    say("gsi pipeline")
+    n, V = 32, 3
+    eps = gsk.mart_eigenstrains(V, base=0.0, axis=0.01)
+    vt = tensor_lines(eps)
+    script = (f"set gridsize {n} {n} {n} 1 1 1\nset binary on\n"
+              f"sys cores\nset variantn {V}\nset coresn 2\nset radius 5\nset method regular\nrun\n"
+              f"sys precipitate\nset rule reference\nset deltatime 0.002\nset meta 1\nset lpp 1\nset arfi 1.5\n"
+              f"set beta 1.5\nset xi 50\nreadhere varianttensor\n{vt}run 20\n"
+              f"dump pre 1 eta\nrun 1\n"
+              f"sys stress\nset xi 4000\ndump st 1 stress defect\nrun\n"
+              f"shell cp st/stress.1.data ds.data\nread dislocationstress ds.data\n"
+              f"sys mart\nset rule reference\nset deltatime 0.005\nset weightelastic 1000\nset temperature 300\n"
+              f"dump mt 1 eta\nrun 5\n")
+    out = run_gsi(tmp_path, script)
+    pre = read_binary(tmp_path / "pre" / "eta.21.data")
+    st = read_binary(tmp_path / "st" / "stress.1.data")
+    df = read_binary(tmp_path / "st" / "defect.1.data")
+    mt = read_binary(tmp_path / "mt" / "eta.5.data")
+    assert np.isfinite(pre).all() and np.isfinite(st).all() and np.isfinite(mt).all(), out
+    want, wd = gsk.stress(pre, _stress_default_modulus(), eps, xi=4000.0)
+    assert np.array_equal(df, wd)
+    assert float(np.abs(st - want).max()) <= 1e-6 * float(np.abs(want).max())
+    assert df.sum() > 0 and np.all(mt[:, df > 0] == 0.0)
+
+
+def _stress_default_modulus():
+    C = np.zeros((3, 3, 3, 3))
+    for (i, j, k, l), v in {(0, 0, 0, 0): 5.39, (1, 1, 1, 1): 5.39, (2, 2, 2, 2): 5.22, (0, 0, 1, 1): 3.39,
+                            (1, 1, 0, 0): 3.39, (1, 1, 2, 2): 3.56, (2, 2, 1, 1): 3.56, (2, 2, 0, 0): 3.56,
+                            (0, 0, 2, 2): 3.56}.items():
+        C[i, j, k, l] = v
+    for (a, b), v in {(0, 1): 0.6, (1, 2): 0.77, (0, 2): 0.77}.items():
+        for p in ((a, b), (b, a)):
+            for q in ((a, b), (b, a)):
+                C[p + q] = v
+    return C

@@ -103,3 +103,56 @@ def test_control_flow_loop_while_if(tmp_path):
 def test_control_flow_without_end_is_an_error(tmp_path):
     rc, out = run(tmp_path, "loop 3\nprint x\n")
     assert rc != 0 and "without end" in out
+
+
+def read_bin(path):
+    raw = open(path, "rb").read()
+    rank = struct.unpack("i", raw[:4])[0]
+    dims = struct.unpack("i" * rank, raw[4:4 + 4 * rank])
+    return np.frombuffer(raw[4 + 4 * rank:], np.float32).reshape(dims)
+
+
+def _sphere_count(n, ox, oy, oz, r):
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    a, b, d = i - ox, j - oy, k - oz
+    inside = np.zeros((n, n, n), bool)
+    for sa, sb, sd in [(0, 0, 0), (n, 0, 0), (-n, 0, 0), (0, n, 0), (0, -n, 0), (0, 0, n), (0, 0, -n)]:
+        inside |= (a + sa) ** 2 + (b + sb) ** 2 + (d + sd) ** 2 <= r * r
+    return inside
+
+
+def test_cores_regular_places_spheres(tmp_path):
+    """cores, regular method (src/dynamics_cores.cu:86-115): core cn at (n/2, n/2, n/C/2 + n/C cn),
+    variant cn % V, eta = 1 and c = c2 inside (sphere plus one-period images), c = c1 outside."""
+    n, C, r, V = 16, 2, 3, 3
+    script = (f"sys cores\nset gridsize {n} {n} {n} 0.1 0.1 0.1\nset variantn {V}\nset coresn {C}\nset radius {r}\n"
+              "set concentration 0.2 0.44\nset method regular\nset binary on\ndump out 1 eta concentration\nrun\n")
+    rc, out = run(tmp_path, script)
+    assert rc == 0, out
+    eta = read_bin(tmp_path / "out" / "eta.1.data")
+    c = read_bin(tmp_path / "out" / "concentration.1.data")
+    assert eta.shape == (V, n, n, n)
+    want_c = np.full((n, n, n), 0.2)
+    for cn in range(C):
+        ins = _sphere_count(n, n // 2, n // 2, n // C // 2 + n // C * cn, r)
+        assert np.array_equal(eta[cn % V] > 0.5, ins) or cn % V != cn
+        want_c[ins] = 0.44
+    np.testing.assert_allclose(c, want_c.astype(np.float32), rtol=0, atol=0)
+
+
+def test_cores_random_is_reproducible(tmp_path):
+    script = ("sys cores\nset gridsize 16 16 16 0.1 0.1 0.1\nset variantn 4\nset coresn 3\nset radius 2\n"
+              "set seed {s}\nset binary on\ndump o{s} 1 eta concentration\nrun\n")
+    outs = []
+    for s in (7, 7, 8):
+        d = tmp_path / f"r{len(outs)}"
+        d.mkdir()
+        rc, out = run(d, script.replace("{s}", str(s)))
+        assert rc == 0, out
+        outs.append((read_bin(d / f"o{s}" / "eta.1.data"), read_bin(d / f"o{s}" / "concentration.1.data")))
+    assert np.array_equal(outs[0][0], outs[1][0]) and not np.array_equal(outs[0][0], outs[2][0])
+    eta, c = outs[0]
+    inside = eta.sum(axis=0) > 0
+    assert set(np.unique(eta)) <= {0.0, 1.0}
+    assert np.all(c[inside] == np.float32(0.44)) and np.all(c[~inside] == np.float32(0.2))
+    assert 0 < inside.sum() <= 3 * 33  # at most three spheres of radius 2 (33 lattice points each)

@@ -474,3 +474,24 @@ def test_prec_ref_conserves_mass_with_noise():
     c1, _ = O.prec_ref_run(c, eta, O.btensor(n, _cubic(3.5, 1.5, 1.0), _eps(V)), pp, 5)
     assert abs(c1.sum() - c.sum()) < 1e-11 * c.sum()
     assert np.abs(c1 - c).max() > 0
+
+
+def test_stress_uniform_state_is_minus_xi_sigma():
+    """A uniform eta has only k = 0, where B_bv(0) = C_ijkl e^b_ij eps^v_kl = sigma^v_ab for the
+    unit base strain b = (a, b): the stress plugin must return -xi sum_v sigma^v_ab eta_v^2
+    (HEAD source) or -xi sum_v sigma^v_ab eta_v (master source), component order 00,01,02,11,12,22."""
+    n, V, xi = 8, 3, 4000.0
+    C = _cubic(5.39, 3.39, 0.6)
+    rng = np.random.default_rng(12)
+    eps = rng.uniform(-0.02, 0.02, (V, 3, 3))
+    vals = np.array([0.3, -0.5, 0.95])
+    eta = vals[:, None, None, None] * np.ones((V, n, n, n))
+    sig = np.einsum("ijkl,vkl->vij", C, eps)
+    for source, s_v in ((2, vals**2), (1, vals)):
+        st, d = O.stress(eta, C, eps, xi=xi, source=source)
+        want = [-xi * np.dot(sig[:, a, b], s_v) for a, b in [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]]
+        np.testing.assert_allclose(st[:, 0, 0, 0], want, rtol=1e-12)
+        np.testing.assert_allclose(st.std(axis=(1, 2, 3)), 0.0, atol=1e-9 * xi)
+        assert np.all(d == 1.0)  # |0.95| > 0.9 everywhere
+    st, d = O.stress(eta * 0.5, C, eps, xi=xi)
+    assert np.all(d == 0.0)
