@@ -27,8 +27,8 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_sim(gsk, n, nvar, path, seed=0):
-    cfg = gsk.default_config(gsk.PHYS_MART, n=n, nvar=nvar, path=path, device=0)
+def make_sim(gsk, n, nvar, path, seed=0, noise=0.0):
+    cfg = gsk.default_config(gsk.PHYS_MART, n=n, nvar=nvar, path=path, device=0, noise=noise, seed=seed)
     s = gsk.Sim(cfg)
     s.build_btensor(gsk.cubic_modulus(0.7, 0.3, 0.2), gsk.mart_eigenstrains(nvar))
     s.init_uniform("eta", -0.01, 0.01, seed)
@@ -188,6 +188,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--nvar", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--noise", type=float, default=0.0, help="noise amplitude (0.002 for the 512^3 configuration)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -219,7 +220,7 @@ def main():
 
     n, V = a.n, a.nvar
     log(f"fused32 n={n} V={V}")
-    s = make_sim(gsk, n, V, gsk.PATH_FUSED32)
+    s = make_sim(gsk, n, V, gsk.PATH_FUSED32, noise=a.noise)
     s.step(a.warmup)
     wall_ms, ev_ms = timed(s, a.steps, torch)
     ms = wall_ms / a.steps
@@ -248,7 +249,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic (Philox uniform initial fields, seed 0)",
         "config": {"workload": f"martensite {n}^3, {V} variants, semi-implicit AC + Khachaturyan elasticity, "
-                               "FP32 fused spectral path", "n": n, "nvar": V, "dt": 0.01,
+                               "FP32 fused spectral path", "n": n, "nvar": V, "dt": 0.01, "noise": a.noise,
                    "parallelism": "single GPU"},
         "hbm_frac_canonical": round(canon_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "event_ms_per_step": round(ev_ms / a.steps, 4),
