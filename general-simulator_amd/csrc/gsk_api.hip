@@ -706,9 +706,10 @@ int ensure_device(int dev) {
 void choose_geometry(gsk_sim* s, int Fz, int Fx_resident) {
   const int n = s->n, M = n / 2;
   Geo& g = s->g;
-  g.n = n; g.M = M; g.Pz = M + 16;
+  g.n = n; g.M = M;
   g.nvox = (long long)n * n * n;
-  g.kfs = (long long)n * n * g.Pz;
+  g.nq = (long long)n * n * M;
+  g.kfs = ((long long)n * n * (M + 1) + 15) / 16 * 16;
   // z tiles: the compile-time shape of the z kernels (zrows rows, zthreads threads)
   const int L = zrows(M);
   g.L = L;

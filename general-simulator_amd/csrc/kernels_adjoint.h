@@ -11,7 +11,7 @@
 //   X = (1/N) sum_k Re(conj(lam_hat) eta'_hat) k^2 D
 //   dLoss/ds_p   += -2 dt L w sum_q (G_pq + G_qp),  G_rq = (1/N) sum_k Re(conj(P_hat_r) B_rq S_hat_q)
 // The transforms are the library's FP32 R2C/C2R passes on natural-layout spectra
-// [f][kx][ky][kz], kz <= n/2, row pitch Pz; sums over the full spectrum use the
+// [f][kx][ky][kz], kz <= n/2 (index kidx); sums over the full spectrum use the
 // half-spectrum weights w(kz) = 1 for kz in {0, n/2}, 2 otherwise.
 #pragma once
 #include "kernels_aux.h"
@@ -33,7 +33,7 @@ __device__ __forceinline__ void half_index(long i, const Geo& g, int& kx, int& k
   ky = (int)(row % g.n);
 }
 
-// Packed B_pq (p <= q) on the half spectrum in natural layout, FP32: Bh[pq*kfs + row*Pz + kz].
+// Packed B_pq (p <= q) on the half spectrum in natural layout, FP32: Bh[kidx(g, pq, row, kz)].
 __global__ void k_btensor_half(float* __restrict__ Bh, Geo g, BArgs a) {
   const long total = (long)g.n * g.n * (g.M + 1);
   const int V = a.V1;
@@ -46,7 +46,7 @@ __global__ void k_btensor_half(float* __restrict__ Bh, Geo g, BArgs a) {
     for (int p = 0; p < V; ++p)
       for (int q = p; q < V; ++q) {
         const int pq = p * V - (p * (p - 1)) / 2 + (q - p);
-        Bh[pq * g.kfs + row * g.Pz + kz] = (float)out[p * V + q];
+        Bh[kidx(g, pq, row, kz)] = (float)out[p * V + q];
       }
   }
 }
@@ -62,7 +62,7 @@ __global__ void k_adj_dmul(float2* __restrict__ lam, const float2* __restrict__ 
     int kx, ky, kz;
     long row;
     half_index(i % per, g, kx, ky, kz, row);
-    const long idx = f * g.kfs + row * g.Pz + kz;
+    const long idx = kidx(g, f, row, kz);
     const float kxv = kcomp(kx, g.n, k.kfac), kyv = kcomp(ky, g.n, k.kfac), kzv = k.kfac * (float)kz;
     const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
     const float D = 1.f / (1.f + k.dtLk * k2);
@@ -90,7 +90,7 @@ __global__ void k_adj_contract(float2* __restrict__ kb, const float* __restrict_
     int kx, ky, kz;
     long row;
     half_index(i, g, kx, ky, kz, row);
-    const long off = row * g.Pz + kz;
+    const long off = kidx(g, 0, row, kz);
     const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
     float2 S[V], P[V];
     float b[V * (V + 1) / 2];
@@ -169,7 +169,7 @@ __global__ void k_adj_lambda(const float* __restrict__ eta, const float* __restr
 }
 
 // Full (not packed) rectangular B_pq, p < V1, q < V2, on the half spectrum in natural layout:
-// Bh[(p*V2 + q)*kfs + row*Pz + kz] (the stress plugin's 6 base strains x nvar variants).
+// Bh[kidx(g, p*V2 + q, row, kz)] (the stress plugin's 6 base strains x nvar variants).
 __global__ void k_btensor_half_rect(float* __restrict__ Bh, Geo g, BArgs a) {
   const long total = (long)g.n * g.n * (g.M + 1);
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
@@ -178,7 +178,7 @@ __global__ void k_btensor_half_rect(float* __restrict__ Bh, Geo g, BArgs a) {
     half_index(i, g, kx, ky, kz, row);
     double out[64];
     btensor_half_spec(kx, ky, kz, g.n, a, out);
-    for (int pq = 0; pq < a.V1 * a.V2; ++pq) Bh[pq * g.kfs + row * g.Pz + kz] = (float)out[pq];
+    for (int pq = 0; pq < a.V1 * a.V2; ++pq) Bh[kidx(g, pq, row, kz)] = (float)out[pq];
   }
 }
 
@@ -191,7 +191,7 @@ __global__ void k_contract_rect(float2* __restrict__ kb, const float* __restrict
     int kx, ky, kz;
     long row;
     half_index(i, g, kx, ky, kz, row);
-    const long off = row * g.Pz + kz;
+    const long off = kidx(g, 0, row, kz);
     float2 X[8];
     for (int q = 0; q < V2; ++q) X[q] = kb[q * g.kfs + off];
     float2 Y[8];

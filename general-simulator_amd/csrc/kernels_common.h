@@ -3,8 +3,10 @@
 // Field layout in HBM (matches the reference's Data<Real> [v][x][y][z], z
 // fastest, include/datamain.th:70-76):
 //   real fields     f * n^3 + (x*n + y)*n + z                       (float)
-//   half spectrum   f * n*n*Pz + (kx*n + ky)*Pz + kz, kz in [0, n/2] (float2)
-//                   Pz = n/2 + 16 so every row starts on a 128-byte line.
+//   half spectrum   field f at f * kfs (float2), row r = kx*n + ky:
+//                     kz <  n/2   r*(n/2) + kz        (rows of exactly n/2 values, so every
+//                                                      row is a whole number of 128-byte lines)
+//                     kz == n/2   nq + r, nq = n*n*n/2 (the Nyquist plane, contiguous)
 //   x-pass state    tile-ordered: ((t*F + f)*n + kx)*W + c, t = x-pass tile (kernels_strided.h)
 //   B tensor        tile-ordered: ((t*npq + pq)*n + kx)*W + c, packed p<=q
 //
@@ -18,12 +20,13 @@
 namespace gsk {
 
 struct Geo {
-  int n, M, Pz;      // edge, n/2, padded row pitch
+  int n, M;          // edge, n/2 (also the row pitch of the half spectrum)
   int L;             // rows per z-pass tile
   int W, nchm;       // x/y-pass tile width and number of full kz chunks (n/2 = nchm*W)
   int ntile;         // x/y-pass tiles: n*nchm main tiles + n/W Nyquist tiles
   long long nvox;    // n^3
-  long long kfs;     // complex elements per k-space field
+  long long kfs;     // complex elements per k-space field (>= n*n*(n/2+1), a multiple of 16)
+  long long nq;      // offset of the Nyquist plane (kz = n/2) inside a field
   int frev, trev;    // visit fields / tiles of a per-field launch in reverse order
 };
 
@@ -39,7 +42,7 @@ __device__ __forceinline__ int geo_tile(const Geo& g) {
 
 // Index of half-spectrum element (field f, row r = kx*n + ky, kz) in the k-space buffer.
 __host__ __device__ __forceinline__ long long kidx(const Geo& g, int f, long long r, int kz) {
-  return f * g.kfs + r * g.Pz + kz;
+  return f * g.kfs + (kz < g.M ? r * g.M + kz : g.nq + r);
 }
 
 // Physics constants for one launch (plain values, by value in kernel args).

@@ -6,8 +6,8 @@
 //   main x tile  (ky, chunk):  W consecutive kz < n/2 of one ky
 //   Nyquist tiles:             the kz = n/2 column for W consecutive x (y pass) or ky (x pass)
 // Main tiles are therefore always full and every row segment they touch is a
-// whole 128-byte line (W = 16 complex); the Nyquist plane, 1/(n/2+1) of the data,
-// is handled by n/W extra tiles with strided columns.
+// whole 128-byte line (W = 16 complex); the Nyquist plane, 1/(n/2+1) of the data and
+// stored contiguously after the rows, is handled by n/W extra tiles.
 //
 // The first Stockham pass reads its operands straight from HBM and the last pass
 // writes straight to HBM (fft_g2g / fft_g2r / fft_r2g), so a y pass touches LDS
@@ -31,24 +31,26 @@ struct STile {
 
 // Tile numbering: the n/W Nyquist tiles (scattered, latency-bound) come first so they
 // run while the main tiles stream, then the n*nchm main tiles.
+// Main tiles read W consecutive kz of a row (one 128-byte line at W = 16); a Nyquist tile
+// covers W whole rows (x or ky) of the contiguous Nyquist plane.
 __device__ __forceinline__ STile ytile(int bid, const Geo& g) {
-  const long Pz = g.Pz;
+  const long M = g.M;
   const int nnyq = g.n / g.W;
   if (bid >= nnyq) {
     const int m = bid - nnyq, x = m / g.nchm, ch = m % g.nchm;
-    return STile{(long)x * g.n * Pz + (long)ch * g.W, 1, Pz, 0, 0, ch * g.W, 1};
+    return STile{(long)x * g.n * M + (long)ch * g.W, 1, M, 0, 0, ch * g.W, 1};
   }
-  return STile{(long)bid * g.W * g.n * Pz + g.M, (long)g.n * Pz, Pz, 0, 0, g.M, 0};
+  return STile{(long)g.nq + (long)bid * g.W * g.n, g.n, 1, 0, 0, g.M, 0};
 }
 
 __device__ __forceinline__ STile xtile(int t, const Geo& g) {
-  const long Pz = g.Pz;
+  const long M = g.M;
   const int nnyq = g.n / g.W;
   if (t >= nnyq) {
     const int m = t - nnyq, ky = m / g.nchm, ch = m % g.nchm;
-    return STile{(long)ky * Pz + (long)ch * g.W, 1, (long)g.n * Pz, ky, 0, ch * g.W, 1};
+    return STile{(long)ky * M + (long)ch * g.W, 1, (long)g.n * M, ky, 0, ch * g.W, 1};
   }
-  return STile{(long)t * g.W * Pz + g.M, Pz, (long)g.n * Pz, t * g.W, 1, g.M, 0};
+  return STile{(long)g.nq + (long)t * g.W, 1, g.n, t * g.W, 1, g.M, 0};
 }
 
 struct TileIO {

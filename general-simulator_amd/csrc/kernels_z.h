@@ -23,7 +23,8 @@ namespace gsk {
 
 // Compile-time shape of the z kernels. GSK_ZROWS_128: rows per tile at M = 128;
 // GSK_ZPLAN_128: radix plan of the 128-point transform (0: 16x8, 1: 8x4x4);
-// GSK_ZMID_TM / GSK_ZMID_WPE: thread multiple and waves-per-SIMD floor of the middle pass.
+// GSK_ZMID_TM / GSK_ZMID_WPE: thread multiple and waves-per-SIMD floor of the middle pass
+// (GSK_ZMID_WPE = 0: the per-shape floor of zmid_wpe).
 #ifndef GSK_ZROWS_128
 #define GSK_ZROWS_128 8
 #endif
@@ -52,6 +53,12 @@ template <> struct ZPlanOf<256> {
   using type = Radices<4, 8, 8>;
   static constexpr int rmin = 4;
 };
+
+// Waves-per-SIMD floor of the middle pass: 4 (at most 128 VGPRs) for the shapes that fit
+// there without spilling (n >= 256 with three variants), else no floor.
+template <int M, int VT> constexpr int zmid_wpe() {
+  return GSK_ZMID_WPE > 0 ? GSK_ZMID_WPE : ((M >= 128 && VT == 3) ? 4 : 1);
+}
 
 // rows per z tile and the exact block size (also used by the host)
 __host__ __device__ constexpr int zrows(int M) { return M <= 32 ? 32 : (M < 128 ? 16 : (M == 128 ? GSK_ZROWS_128 : 8)); }
@@ -97,19 +104,26 @@ template <int M, int TM = 1>
 struct ZSpec {
   using S = ZShape<M, TM>;
   float2 a[S::NT], b[S::NT], c[S::NT];
-  // src: row 0 of the tile in the half-spectrum buffer
-  __device__ __forceinline__ void load(const float2* __restrict__ src, int Pz) {
+  // rows: row 0 of the tile (kz < M, pitch M); nyq: its kz = M value in the Nyquist plane
+  __device__ __forceinline__ void load(const float2* __restrict__ rows, const float2* __restrict__ nyq) {
 #pragma unroll
     for (int i = 0; i < S::NT; ++i) {
       const int e = threadIdx.x + i * S::T;
       if (S::valid(e)) {
         const int l = e / S::H2, k = e % S::H2;
-        const float2* p = src + l * Pz;
+        const float2* p = rows + l * M;
         a[i] = p[k];
-        b[i] = p[M - k];
-        if (k == 0) c[i] = p[S::H2];
+        if (k == 0) {
+          b[i] = nyq[l];
+          c[i] = p[S::H2];
+        } else {
+          b[i] = p[M - k];
+        }
       }
     }
+  }
+  __device__ __forceinline__ void load(const float2* __restrict__ field, const Geo& g, long r0) {
+    load(field + r0 * M, field + g.nq + r0);
   }
   // C2R split into the packed spectra of the LDS tile (row pitch M+1)
   __device__ __forceinline__ void split(float2* tile, const float2* rt) const {
@@ -135,9 +149,12 @@ struct ZSpec {
   }
 };
 
-// packed spectra of the LDS tile -> half-spectrum rows of one field (R2C direction)
+// packed spectra of the LDS tile -> half-spectrum rows r0.. of one field (R2C direction)
 template <int M, int TM = 1>
-__device__ __forceinline__ void z_store_spec(const float2* tile, float2* __restrict__ dst, int Pz, const float2* rt) {
+__device__ __forceinline__ void z_store_spec(const float2* tile, float2* __restrict__ field, const Geo& g, long r0,
+                                             const float2* rt) {
+  float2* __restrict__ dst = field + r0 * M;
+  float2* __restrict__ nyq = field + g.nq + r0;
   using S = ZShape<M, TM>;
 #pragma unroll
   for (int i = 0; i < S::NT; ++i) {
@@ -145,11 +162,11 @@ __device__ __forceinline__ void z_store_spec(const float2* tile, float2* __restr
     if (S::valid(e)) {
       const int l = e / S::H2, k = e % S::H2;
       const float2* z = tile + l * (M + 1);
-      float2* row = dst + l * Pz;
+      float2* row = dst + l * M;
       if (k == 0) {
         const float2 z0 = z[0], zh = z[S::H2];
         row[0] = make_float2(z0.x + z0.y, 0.f);
-        row[M] = make_float2(z0.x - z0.y, 0.f);
+        nyq[l] = make_float2(z0.x - z0.y, 0.f);
         row[S::H2] = make_float2(zh.x, -zh.y);
       } else {
         float2 xk, xm;
@@ -224,7 +241,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd(const float* __restrict_
   }
   __syncthreads();
   z_fft1<M, true>(tile, tw);
-  z_store_spec<M>(tile, kout + f * g.kfs + r0 * g.Pz, g.Pz, rt);
+  z_store_spec<M>(tile, kout + f * g.kfs, g, r0, rt);
 }
 
 // --------------------------------------------------------------------- inverse z pass (one field per block)
@@ -244,7 +261,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zinv(const float2* __restrict
   const int f = geo_field(g);
   const long r0 = (long)geo_tile(g) * S::L;
   ZSpec<M> sp;
-  sp.load(kin + f * g.kfs + r0 * g.Pz, g.Pz);
+  sp.load(kin + f * g.kfs, g, r0);
   z_twiddles<M>(tw, rt, twM, rtn);
   sp.split(tile, rt);
   __syncthreads();
@@ -268,9 +285,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zinv(const float2* __restrict
 // eta_q of the tile is prefetched into registers; VT = 0: any variant count, eta read when needed.
 template <int M, int VT>
 __global__ void __launch_bounds__(GSK_ZMID_TM * zthreads<M>())
-#if GSK_ZMID_WPE > 0
-__attribute__((amdgpu_waves_per_eu(GSK_ZMID_WPE)))
-#endif
+__attribute__((amdgpu_waves_per_eu(zmid_wpe<M, VT>())))
 k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta,
                                                             Geo g, MartK mk, const float2* __restrict__ twM,
                                                             const float2* __restrict__ rtn) {
@@ -285,7 +300,7 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta,
   const long r0 = (long)blockIdx.x * S::L;
   const int F = VT > 0 ? VT : mk.F;
   const float* esrc = eta + r0 * n;
-  float2* kt = kbuf + r0 * g.Pz;
+
   float4 pre[S::NT][VR];
   if constexpr (VT > 0) {
 #pragma unroll
@@ -299,13 +314,13 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta,
     }
   }
   ZSpec<M, TM> sp;
-  sp.load(kt, g.Pz);
+  sp.load(kbuf, g, r0);
   z_twiddles<M>(tw, rt, twM, rtn);
 #pragma unroll 1
   for (int p = 0; p < F; ++p) {
     sp.split(tile, rt);
     __syncthreads();
-    if (p + 1 < F) sp.load(kt + (p + 1) * g.kfs, g.Pz);
+    if (p + 1 < F) sp.load(kbuf + (p + 1) * g.kfs, g, r0);
     z_fft1<M, false>(tile, tw);
 #pragma unroll
     for (int i = 0; i < S::NT; ++i) {
@@ -344,7 +359,7 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta,
     }
     __syncthreads();
     z_fft1<M, true>(tile, tw);
-    z_store_spec<M, TM>(tile, kt + p * g.kfs, g.Pz, rt);
+    z_store_spec<M, TM>(tile, kbuf + p * g.kfs, g, r0, rt);
     __syncthreads();  // the next variant's split overwrites the tile
   }
 }
@@ -415,7 +430,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local(const float* __res
   }
   __syncthreads();
   for (int f = 0; f < F; ++f) z_fft1<M, true>(tile + f * L * (M + 1), tw);
-  for (int f = 0; f < F; ++f) z_store_spec<M>(tile + f * L * (M + 1), kout + f * g.kfs + r0 * g.Pz, g.Pz, rt);
+  for (int f = 0; f < F; ++f) z_store_spec<M>(tile + f * L * (M + 1), kout + f * g.kfs, g, r0, rt);
 }
 
 // Same as k_zfwd_local for a compile-time field count FT = V + 1 (2..5): every field of the
@@ -503,7 +518,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local_r(const float* __r
     }
     __syncthreads();
     z_fft1<M, true>(tile, tw);
-    z_store_spec<M>(tile, kout + f * g.kfs + r0 * g.Pz, g.Pz, rt);
+    z_store_spec<M>(tile, kout + f * g.kfs, g, r0, rt);
     __syncthreads();  // the next field overwrites the tile
   }
 }
