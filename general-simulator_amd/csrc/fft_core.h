@@ -234,9 +234,9 @@ __device__ __forceinline__ void pass_math(T2* v, int j, int Ns, const T2* tw) {
 // One pass with selectable operand source and result sink.
 // SRC: 0 = LDS, 1 = global via ld, 2 = registers vin.  DST: 0 = LDS, 1 = global via st, 2 = registers vout.
 template <typename T2, int N, int R, bool FWD, int SRC, int DST, class LD, class ST>
-__device__ __forceinline__ void gpass(T2* lds, const Lines& L, int Ns, const T2* tw, LD& ld, ST& st, T2* vreg) {
+__device__ __forceinline__ void gpass(T2* lds, const Lines& L, int Ns, const T2* tw, LD& ld, ST& st, T2* vreg,
+                                      int b = threadIdx.x) {
   const int nb = L.nl * (N / R);
-  const int b = threadIdx.x;
   const bool act = b < nb;
   const int l = act ? b % L.nl : 0;
   const int j = act ? b / L.nl : 0;
@@ -271,12 +271,12 @@ __device__ __forceinline__ void gpass(T2* lds, const Lines& L, int Ns, const T2*
 
 template <typename T2, int N, bool FWD, int SRC, int DST, int Ns, class LD, class ST, int R0, int... Rs>
 __device__ __forceinline__ void gchain(T2* lds, const Lines& L, const T2* tw, LD& ld, ST& st, T2* vreg,
-                                       Radices<R0, Rs...>) {
+                                       Radices<R0, Rs...>, int b = threadIdx.x) {
   if constexpr (sizeof...(Rs) == 0) {
-    gpass<T2, N, R0, FWD, SRC, DST>(lds, L, Ns, tw, ld, st, vreg);
+    gpass<T2, N, R0, FWD, SRC, DST>(lds, L, Ns, tw, ld, st, vreg, b);
   } else {
-    gpass<T2, N, R0, FWD, SRC, 0>(lds, L, Ns, tw, ld, st, vreg);
-    gchain<T2, N, FWD, 0, DST, Ns * R0>(lds, L, tw, ld, st, vreg, Radices<Rs...>{});
+    gpass<T2, N, R0, FWD, SRC, 0>(lds, L, Ns, tw, ld, st, vreg, b);
+    gchain<T2, N, FWD, 0, DST, Ns * R0>(lds, L, tw, ld, st, vreg, Radices<Rs...>{}, b);
   }
 }
 

@@ -447,11 +447,11 @@ struct Impl {
       const int grid = s->n * s->n / s->g.L;
       const size_t lds = zlds(s, 1);
       switch (F) {  // eta prefetched into registers for up to 4 variants
-        case 1: LAUNCH((k_zmid_mart<M, 1>), grid, GSK_ZMID_TM * s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
-        case 2: LAUNCH((k_zmid_mart<M, 2>), grid, GSK_ZMID_TM * s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
-        case 3: LAUNCH((k_zmid_mart<M, 3>), grid, GSK_ZMID_TM * s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
-        case 4: LAUNCH((k_zmid_mart<M, 4>), grid, GSK_ZMID_TM * s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
-        default: LAUNCH((k_zmid_mart<M, 0>), grid, GSK_ZMID_TM * s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn);
+        case 1: LAUNCH((k_zmid_mart<M, 1>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
+        case 2: LAUNCH((k_zmid_mart<M, 2>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
+        case 3: LAUNCH((k_zmid_mart<M, 3>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
+        case 4: LAUNCH((k_zmid_mart<M, 4>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
+        default: LAUNCH((k_zmid_mart<M, 0>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn);
       }
     }
     CHK(ypass(s, true, F));

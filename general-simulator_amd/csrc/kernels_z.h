@@ -17,22 +17,19 @@
 // other over the same LDS tile and prefetches the next variant's spectrum while
 // the current one is transformed.
 #pragma once
+#include <type_traits>
 #include "kernels_common.h"
 
 namespace gsk {
 
 // Compile-time shape of the z kernels. GSK_ZROWS_128: rows per tile at M = 128;
 // GSK_ZPLAN_128: radix plan of the 128-point transform (0: 16x8, 1: 8x4x4);
-// GSK_ZMID_TM / GSK_ZMID_WPE: thread multiple and waves-per-SIMD floor of the middle pass
-// (GSK_ZMID_WPE = 0: the per-shape floor of zmid_wpe).
+// GSK_ZMID_WPE: waves-per-SIMD floor of the middle pass (0: the per-shape floor of zmid_wpe).
 #ifndef GSK_ZROWS_128
 #define GSK_ZROWS_128 8
 #endif
 #ifndef GSK_ZPLAN_128
 #define GSK_ZPLAN_128 1
-#endif
-#ifndef GSK_ZMID_TM
-#define GSK_ZMID_TM 1
 #endif
 #ifndef GSK_ZMID_WPE
 #define GSK_ZMID_WPE 0
@@ -52,6 +49,23 @@ template <> struct ZPlanOf<128> {
 template <> struct ZPlanOf<256> {
   using type = Radices<4, 8, 8>;
   static constexpr int rmin = 4;
+};
+
+// Plan of the middle pass: the z plan with its smallest radix moved to the front, so the
+// first forward pass and the last inverse pass (the reversed plan) run on every thread of the
+// block and the real-space work between them stays in registers.
+template <int X, class In, class Out> struct DropOne;
+template <int X, int... Out> struct DropOne<X, Radices<>, Radices<Out...>> { using type = Radices<Out...>; };
+template <int X, int R0, int... Rs, int... Out> struct DropOne<X, Radices<R0, Rs...>, Radices<Out...>> {
+  using type = typename std::conditional<R0 == X, Radices<Out..., Rs...>,
+                                         typename DropOne<X, Radices<Rs...>, Radices<Out..., R0>>::type>::type;
+};
+template <int X, class P> struct Prepend;
+template <int X, int... Rs> struct Prepend<X, Radices<Rs...>> { using type = Radices<X, Rs...>; };
+template <int M> struct ZMidPlan {
+  static constexpr int r0 = ZPlanOf<M>::rmin;
+  using fwd = typename Prepend<r0, typename DropOne<r0, typename ZPlanOf<M>::type, Radices<>>::type>::type;
+  using inv = typename RevImpl<fwd, Radices<>>::type;
 };
 
 // Waves-per-SIMD floor of the middle pass: 4 (at most 128 VGPRs) for the shapes that fit
@@ -89,6 +103,12 @@ __device__ __forceinline__ void r2c_pair(float2 zk, float2 zm, float2 w, float2&
   xm = make_float2(E.x - WO.x, -(E.y - WO.y));
 }
 
+// Element `elem` of `base` through a 32-bit byte offset (uniform base + one offset register).
+template <class T> __device__ __forceinline__ T& at32(T* base, unsigned elem) {
+  using C = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return *reinterpret_cast<T*>(reinterpret_cast<C*>(base) + (unsigned)(elem * (unsigned)sizeof(T)));
+}
+
 // --------------------------------------------------------------------- one-field tile I/O
 // TM: work-group size as a multiple of the FFT's thread count (more threads, fewer tasks each)
 template <int M, int TM = 1>
@@ -104,32 +124,26 @@ template <int M, int TM = 1>
 struct ZSpec {
   using S = ZShape<M, TM>;
   float2 a[S::NT], b[S::NT], c[S::NT];
-  // rows: row 0 of the tile (kz < M, pitch M); nyq: its kz = M value in the Nyquist plane
-  __device__ __forceinline__ void load(const float2* __restrict__ rows, const float2* __restrict__ nyq) {
+  // Rows r0.. of one field: a uniform field base and 32-bit element offsets (a field holds
+  // fewer than 2^29 elements), so the loads use a scalar base and one offset register each.
+  __device__ __forceinline__ void load(const float2* __restrict__ field, const Geo& g, long r0) {
+    const unsigned rb = (unsigned)r0 * M, nb = (unsigned)(g.nq + r0);
 #pragma unroll
     for (int i = 0; i < S::NT; ++i) {
-      const int e = threadIdx.x + i * S::T;
+      const unsigned e = threadIdx.x + i * S::T;
       if (S::valid(e)) {
-        const int l = e / S::H2, k = e % S::H2;
-        const float2* p = rows + l * M;
-        a[i] = p[k];
-        if (k == 0) {
-          b[i] = nyq[l];
-          c[i] = p[S::H2];
-        } else {
-          b[i] = p[M - k];
-        }
+        const unsigned l = e / S::H2, k = e % S::H2, ro = rb + l * M;
+        a[i] = at32(field, ro + k);
+        b[i] = at32(field, k == 0 ? nb + l : ro + (M - k));  // k = 0: the kz = M value (Nyquist plane)
+        if (k == 0) c[i] = at32(field, ro + S::H2);
       }
     }
   }
-  __device__ __forceinline__ void load(const float2* __restrict__ field, const Geo& g, long r0) {
-    load(field + r0 * M, field + g.nq + r0);
-  }
   // C2R split into the packed spectra of the LDS tile (row pitch M+1)
-  __device__ __forceinline__ void split(float2* tile, const float2* rt) const {
+  __device__ __forceinline__ void split(float2* tile, const float2* rt, int tid = threadIdx.x) const {
 #pragma unroll
     for (int i = 0; i < S::NT; ++i) {
-      const int e = threadIdx.x + i * S::T;
+      const int e = tid + i * S::T;
       if (S::valid(e)) {
         const int l = e / S::H2, k = e % S::H2;
         float2* z = tile + l * (M + 1);
@@ -149,31 +163,30 @@ struct ZSpec {
   }
 };
 
-// packed spectra of the LDS tile -> half-spectrum rows r0.. of one field (R2C direction)
+// packed spectra of the LDS tile -> half-spectrum rows r0.. of one field (R2C direction);
+// offsets as in ZSpec::load
 template <int M, int TM = 1>
 __device__ __forceinline__ void z_store_spec(const float2* tile, float2* __restrict__ field, const Geo& g, long r0,
-                                             const float2* rt) {
-  float2* __restrict__ dst = field + r0 * M;
-  float2* __restrict__ nyq = field + g.nq + r0;
+                                             const float2* rt, int tid = threadIdx.x) {
   using S = ZShape<M, TM>;
+  const unsigned rb = (unsigned)r0 * M, nb = (unsigned)(g.nq + r0);
 #pragma unroll
   for (int i = 0; i < S::NT; ++i) {
-    const int e = threadIdx.x + i * S::T;
+    const unsigned e = tid + i * S::T;
     if (S::valid(e)) {
-      const int l = e / S::H2, k = e % S::H2;
+      const unsigned l = e / S::H2, k = e % S::H2, ro = rb + l * M;
       const float2* z = tile + l * (M + 1);
-      float2* row = dst + l * M;
+      float2 xk, xm;
       if (k == 0) {
         const float2 z0 = z[0], zh = z[S::H2];
-        row[0] = make_float2(z0.x + z0.y, 0.f);
-        nyq[l] = make_float2(z0.x - z0.y, 0.f);
-        row[S::H2] = make_float2(zh.x, -zh.y);
+        xk = make_float2(z0.x + z0.y, 0.f);
+        xm = make_float2(z0.x - z0.y, 0.f);
+        at32(field, ro + S::H2) = make_float2(zh.x, -zh.y);
       } else {
-        float2 xk, xm;
         r2c_pair(z[k], z[M - k], rt[k], xk, xm);
-        row[k] = xk;
-        row[M - k] = xm;
       }
+      at32(field, ro + k) = xk;
+      at32(field, k == 0 ? nb + l : ro + (M - k)) = xm;
     }
   }
 }
@@ -281,17 +294,18 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zinv(const float2* __restrict
 
 // --------------------------------------------------------------------- martensite middle z pass
 // For each variant p in turn: R_hat_p rows -> C2R -> F_p = f'(eta) + 2 w eta_p R_p + a xi (Philox in
-// registers) -> R2C, in place. VT > 0: the variant count is a compile-time constant and every
-// eta_q of the tile is prefetched into registers; VT = 0: any variant count, eta read when needed.
+// registers) -> R2C, in place. The last inverse Stockham pass leaves thread (l, j) = (b % L, b / L)
+// holding packed positions m = j + r*M/R0 of row l, i.e. the voxel pairs (2m, 2m+1); the force is
+// formed on those registers and the first forward pass starts from them. VT > 0: the variant count
+// is a compile-time constant and every eta_q the thread needs is prefetched into registers;
+// VT = 0: any variant count, eta read when needed.
 template <int M, int VT>
-__global__ void __launch_bounds__(GSK_ZMID_TM * zthreads<M>())
-__attribute__((amdgpu_waves_per_eu(zmid_wpe<M, VT>())))
-k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta,
-                                                            Geo g, MartK mk, const float2* __restrict__ twM,
-                                                            const float2* __restrict__ rtn) {
-  constexpr int TM = GSK_ZMID_TM;
-  using S = ZShape<M, TM>;
-  constexpr int n = 2 * M;
+__global__ void __launch_bounds__(zthreads<M>()) __attribute__((amdgpu_waves_per_eu(zmid_wpe<M, VT>())))
+k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, MartK mk, const float2* __restrict__ twM,
+            const float2* __restrict__ rtn) {
+  using S = ZShape<M>;
+  using P = ZMidPlan<M>;
+  constexpr int n = 2 * M, R0 = P::r0, NB = S::L * (M / R0);
   constexpr int VR = VT > 0 ? VT : 1;
   extern __shared__ float2 smem[];
   float2* tw = smem;
@@ -299,67 +313,76 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta,
   float2* tile = smem + 2 * M;
   const long r0 = (long)blockIdx.x * S::L;
   const int F = VT > 0 ? VT : mk.F;
-  const float* esrc = eta + r0 * n;
-
-  float4 pre[S::NT][VR];
+  const int b = threadIdx.x;
+  const bool act = b < NB;
+  const int l = b % S::L, j = b / S::L;
+  // voxel of register r: row r0 + l, pair m = j + r*M/R0
+  const float* erow = eta + r0 * n;        // uniform base of the tile's rows
+  const unsigned vloc = l * n + 2 * j;      // voxel of register 0 relative to erow
+  float2 pre[R0][VR];
   if constexpr (VT > 0) {
+    if (act) {
 #pragma unroll
-    for (int i = 0; i < S::NT; ++i) {
-      int l, q;
-      z_quad<M, TM>(i, l, q);
-      if (S::valid(threadIdx.x + i * S::T)) {
+      for (int r = 0; r < R0; ++r)
 #pragma unroll
-        for (int f = 0; f < VR; ++f) pre[i][f] = *reinterpret_cast<const float4*>(esrc + f * g.nvox + l * n + 4 * q);
-      }
+        for (int f = 0; f < VR; ++f)
+          pre[r][f] = at32(reinterpret_cast<const float2*>(erow + f * g.nvox), vloc / 2 + r * (M / R0));
     }
   }
-  ZSpec<M, TM> sp;
+  ZSpec<M> sp;
   sp.load(kbuf, g, r0);
   z_twiddles<M>(tw, rt, twM, rtn);
+  const Lines Ls{S::L, S::L, M + 1, 0, 1};
+  auto no_ld = [](int, int) { return float2{}; };
+  auto no_st = [](int, int, float2) {};
+  const uint32_t step = step_of(mk.step, mk.stepp);
 #pragma unroll 1
   for (int p = 0; p < F; ++p) {
-    sp.split(tile, rt);
+    // an opaque copy of the thread index per variant: the LDS addresses of the passes are
+    // recomputed in each iteration instead of being held in registers across the loop
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    sp.split(tile, rt, tid);
     __syncthreads();
     if (p + 1 < F) sp.load(kbuf + (p + 1) * g.kfs, g, r0);
-    z_fft1<M, false>(tile, tw);
+    float2 v[R0];
+    gchain<float2, M, false, 0, 2, 1>(tile, Ls, tw, no_ld, no_st, v, typename P::inv{}, tid);
+    if (act) {
 #pragma unroll
-    for (int i = 0; i < S::NT; ++i) {
-      int l, q;
-      z_quad<M, TM>(i, l, q);
-      if (!S::valid(threadIdx.x + i * S::T)) continue;
-      const long vox = (r0 + l) * n + 4 * q;
-      float ep[4], s2[4] = {0.f, 0.f, 0.f, 0.f};
-      if constexpr (VT > 0) {
+      for (int r = 0; r < R0; ++r) {
+        const unsigned vo = vloc + 2 * r * (M / R0);
+        const unsigned vox = (unsigned)r0 * n + vo;  // < n^3 <= 2^27
+        float e0 = 0.f, e1 = 0.f, s0 = 0.f, s1 = 0.f;
+        if constexpr (VT > 0) {
 #pragma unroll
-        for (int f = 0; f < VR; ++f) {
-          const float4 t = pre[i][f];
-          s2[0] += t.x * t.x; s2[1] += t.y * t.y; s2[2] += t.z * t.z; s2[3] += t.w * t.w;
-          if (f == p) { ep[0] = t.x; ep[1] = t.y; ep[2] = t.z; ep[3] = t.w; }
+          for (int f = 0; f < VR; ++f) {
+            const float2 t = pre[r][f];
+            s0 += t.x * t.x; s1 += t.y * t.y;
+            if (f == p) { e0 = t.x; e1 = t.y; }
+          }
+        } else {
+          for (int f = 0; f < F; ++f) {
+            const float2 t = at32(reinterpret_cast<const float2*>(erow + f * g.nvox), vo / 2);
+            s0 += t.x * t.x; s1 += t.y * t.y;
+            if (f == p) { e0 = t.x; e1 = t.y; }
+          }
         }
-      } else {
-        for (int f = 0; f < F; ++f) {
-          const float4 t = *reinterpret_cast<const float4*>(eta + f * g.nvox + vox);
-          s2[0] += t.x * t.x; s2[1] += t.y * t.y; s2[2] += t.z * t.z; s2[3] += t.w * t.w;
-          if (f == p) { ep[0] = t.x; ep[1] = t.y; ep[2] = t.z; ep[3] = t.w; }
+        float xi[2] = {0.f, 0.f};
+        if (mk.noise != 0.f) {
+          // opaque per variant: keeps the compiler from hoisting the variant-independent part
+          // of the Philox rounds out of the variant loop (it would hold it in registers)
+          uint32_t gq = vox >> 2;
+          asm volatile("" : "+v"(gq));
+          normal2(mk.seed, gq, (uint32_t)p, step, (vox >> 1) & 1u, xi);
         }
+        const float o0 = mk.A * e0 - mk.B * e0 * e0 + mk.C * e0 * s0 + mk.two_wel * e0 * v[r].x + mk.noise * xi[0];
+        const float o1 = mk.A * e1 - mk.B * e1 * e1 + mk.C * e1 * s1 + mk.two_wel * e1 * v[r].y + mk.noise * xi[1];
+        v[r] = make_float2(o0, o1);
+        __builtin_amdgcn_sched_barrier(0);
       }
-      float2* row = tile + l * (M + 1);
-      const float2 ra = row[2 * q], rb = row[2 * q + 1];
-      const float R[4] = {ra.x, ra.y, rb.x, rb.y};
-      float xi[4] = {0.f, 0.f, 0.f, 0.f};
-      if (mk.noise != 0.f) normal4(mk.seed, (uint32_t)(vox >> 2), (uint32_t)p, step_of(mk.step, mk.stepp), xi);
-      float o[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const float e1 = ep[k];
-        o[k] = mk.A * e1 - mk.B * e1 * e1 + mk.C * e1 * s2[k] + mk.two_wel * e1 * R[k] + mk.noise * xi[k];
-      }
-      row[2 * q] = make_float2(o[0], o[1]);
-      row[2 * q + 1] = make_float2(o[2], o[3]);
     }
-    __syncthreads();
-    z_fft1<M, true>(tile, tw);
-    z_store_spec<M, TM>(tile, kbuf + p * g.kfs, g, r0, rt);
+    gchain<float2, M, true, 2, 0, 1>(tile, Ls, tw, no_ld, no_st, v, typename P::fwd{}, tid);
+    z_store_spec<M>(tile, kbuf + p * g.kfs, g, r0, rt, tid);
     __syncthreads();  // the next variant's split overwrites the tile
   }
 }

@@ -122,6 +122,15 @@ __host__ __device__ __forceinline__ void normal4(uint64_t seed, uint32_t g, uint
   box_muller(c[2], c[3], out[2], out[3]);
 }
 
+// The two normals of normal4 for voxels 4g + 2h, 4g + 2h + 1 (h = 0 or 1), bit-identical to
+// out[2h], out[2h + 1] of normal4.
+__host__ __device__ __forceinline__ void normal2(uint64_t seed, uint32_t g, uint32_t field, uint32_t step, uint32_t h,
+                                                 float out[2]) {
+  uint32_t c[4] = {g, field, step, 0u};
+  philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  box_muller(h ? c[2] : c[0], h ? c[3] : c[1], out[0], out[1]);
+}
+
 // Four uniforms in [0,1) (24-bit) for the initial-field generator.
 __host__ __device__ __forceinline__ void uniform4_bits(uint64_t seed, uint32_t g, uint32_t field, uint32_t out[4]) {
   uint32_t c[4] = {g, field, 0u, 1u};

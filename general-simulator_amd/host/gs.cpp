@@ -333,6 +333,21 @@ int GS::Set(const std::string& rest) {
   return Code_NORMAL;
 }
 
+// `link target alias` (src/gs.cu:110-115): the string variable `alias` becomes a name for
+// `target`. As in the reference, an unknown target leaves the variables unchanged and the
+// command still succeeds; here a warning is logged.
+int GS::Link(const std::string& rest) {
+  std::istringstream ss(rest);
+  std::string target, alias;
+  ss >> target >> alias;
+  if (target.empty() || alias.empty()) {
+    log_line("Error: link needs a target and an alias");
+    return Code_ERR;
+  }
+  if (Vars.link(target, alias)) log_line("Warning: link target \"" + target + "\" has no value; not linked");
+  return Code_NORMAL;
+}
+
 int GS::Read(const std::string& rest) {
   std::istringstream ss(rest);
   std::string var, file;
@@ -739,6 +754,8 @@ int Interpreter::exec(std::string& script) {
       err = gs_.Set(rest);
     } else if (cmd == "read") {
       err = gs_.Read(rest);
+    } else if (cmd == "link") {
+      err = gs_.Link(rest);
     } else if (cmd == "readhere") {
       err = readhere(rest, script);
     } else if (cmd == "write") {

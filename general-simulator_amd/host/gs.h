@@ -27,6 +27,7 @@ class GS {
   int SetSys(const std::string& name);
   int Set(const std::string& rest);
   int Read(const std::string& rest);
+  int Link(const std::string& rest);
   int ReadHere(const std::string& name, const std::vector<double>& vals);
   int Write(const std::string& rest);
   int WriteHere(const std::string& rest);

@@ -100,6 +100,18 @@ def test_control_flow_loop_while_if(tmp_path):
     assert lines == ["small1", "small2", "big3", "big4", "k3", "k3", "k1", "k1", "done41"]
 
 
+def test_link_aliases_a_variable(tmp_path):
+    # link target alias (src/gs.cu:110-115, include/datamap.th:126-139): the alias reads and
+    # writes the target; an unknown target leaves everything as it was
+    script = ("set deltatime 0.5\nset dt 7\nlink deltatime dt\nprint {$dt}\n"
+              "set dt 0.25\nprint {$deltatime}\nlink nosuch alias2\nset alias2 3\nprint {$alias2} {$deltatime}\n")
+    rc, out = run(tmp_path, script)
+    assert rc == 0, out
+    lines = [l for l in out.splitlines() if not l.startswith(">>>")]
+    assert "0.5" in lines and "0.25" in lines and "3 0.25" in lines
+    assert any("nosuch" in l for l in lines)
+
+
 def test_control_flow_without_end_is_an_error(tmp_path):
     rc, out = run(tmp_path, "loop 3\nprint x\n")
     assert rc != 0 and "without end" in out
