@@ -53,13 +53,17 @@ __device__ __forceinline__ STile xtile(int t, const Geo& g) {
   return STile{(long)g.nq + (long)t * g.W, 1, g.n, t * g.W, 1, g.M, 0};
 }
 
+// Element (c, pos) of a tile: a uniform base plus a 32-bit offset (a field holds fewer than
+// 2^29 elements), so the index math stays in 32-bit registers and the accesses use a scalar base.
 struct TileIO {
   float2* p;
-  long cs, ps;
-  __device__ __forceinline__ float2 get(int c, int pos) const { return p[c * cs + pos * ps]; }
-  __device__ __forceinline__ void put(int c, int pos, float2 v) const { p[c * cs + pos * ps] = v; }
+  unsigned cs, ps;
+  __device__ __forceinline__ float2 get(int c, int pos) const { return at32(p, (unsigned)c * cs + (unsigned)pos * ps); }
+  __device__ __forceinline__ void put(int c, int pos, float2 v) const { at32(p, (unsigned)c * cs + (unsigned)pos * ps) = v; }
 };
-__device__ __forceinline__ TileIO tio(float2* field, const STile& s) { return TileIO{field + s.off, s.cs, s.ps}; }
+__device__ __forceinline__ TileIO tio(float2* field, const STile& s) {
+  return TileIO{field + s.off, (unsigned)s.cs, (unsigned)s.ps};
+}
 
 // ---------------------------------------------------------------- y pass (C2C along y, in place)
 template <int N, bool FWD>
@@ -126,13 +130,13 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xcontract_r(float2* __restr
   }
   const int b = threadIdx.x;
   if (b < W * (N / RL)) {
-    const float* Bb = Bt + (long)t * NPQ * T + b;
+    const float* Bb = Bt + (long)t * NPQ * T;  // uniform base, lane offsets below
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const int e = r * (N / RL) * W;
       float bv[NPQ];
 #pragma unroll
-      for (int pq = 0; pq < NPQ; ++pq) bv[pq] = Bb[(long)pq * T + e];
+      for (int pq = 0; pq < NPQ; ++pq) bv[pq] = at32(Bb, (unsigned)(pq * T + b + e));
       float2 R[V];
 #pragma unroll
       for (int p = 0; p < V; ++p) {
@@ -185,7 +189,7 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xcontract(float2* __restric
       for (int q = 0; q < F; ++q) {
         const int a = p < q ? p : q, c = p < q ? q : p;
         const int pq = a * F - (a * (a - 1)) / 2 + (c - a);
-        const float bv = Bb[(long)pq * T + e];
+        const float bv = at32(Bb, (unsigned)(pq * T + e));
         acc.x += bv * S[q].x;
         acc.y += bv * S[q].y;
       }
@@ -228,9 +232,9 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_mart(float2* __rest
   if (GSK_XUPD_PF && MODE == 0 && act) {
 #pragma unroll
     for (int gi = 0; gi < (GSK_XUPD_PF ? G : 1); ++gi) {
-      const float2* st = state + ((long)t * F + f0 + gi) * T + b;
+      const float2* st = state + ((long)t * F + f0 + gi) * T;
 #pragma unroll
-      for (int r = 0; r < RL; ++r) sv[gi][r] = st[r * (N / RL) * W];
+      for (int r = 0; r < RL; ++r) sv[gi][r] = at32(st, (unsigned)(b + r * (N / RL) * W));
     }
   }
   load_tw<N>(tw, twN);
@@ -248,19 +252,19 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_mart(float2* __rest
     if (act) {
 #pragma unroll
       for (int gi = 0; gi < G; ++gi) {
-          float2* st = state + ((long)t * F + f0 + gi) * T + b;
+          float2* st = state + ((long)t * F + f0 + gi) * T;
 #pragma unroll
           for (int r = 0; r < RL; ++r) {
             const int e = r * (N / RL) * W;
             if (MODE == 1) {
-              st[e] = v[gi][r];
+              at32(st, (unsigned)(b + e)) = v[gi][r];
             } else {
               const float kxv = kcomp(j + r * (N / RL), N, mk.kfac);
               const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
               const float den = 1.f / (1.f + mk.dtLk * k2);
-              const float2 eh = GSK_XUPD_PF ? sv[GSK_XUPD_PF ? gi : 0][r] : st[e];
+              const float2 eh = GSK_XUPD_PF ? sv[GSK_XUPD_PF ? gi : 0][r] : at32(st, (unsigned)(b + e));
               const float2 nh = make_float2((eh.x - mk.dtL * v[gi][r].x) * den, (eh.y - mk.dtL * v[gi][r].y) * den);
-              st[e] = nh;
+              at32(st, (unsigned)(b + e)) = nh;
               v[gi][r] = make_float2(nh.x * mk.inv_n3, nh.y * mk.inv_n3);
             }
           }
@@ -298,13 +302,13 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_prec(float2* __rest
   const int c = b % W, j = b / W;
   // the state does not depend on the transform: fetch it first
   float2 sv[GSK_XUPD_PF ? G : 1][RL];
-  const float* Bb = Bt + (long)t * T + b;
+  const float* Bb = Bt + (long)t * T;
   if (GSK_XUPD_PF && MODE == 0 && act) {
 #pragma unroll
     for (int gi = 0; gi < (GSK_XUPD_PF ? G : 1); ++gi) {
-      const float2* st = state + ((long)t * F + f0 + gi) * T + b;
+      const float2* st = state + ((long)t * F + f0 + gi) * T;
 #pragma unroll
-      for (int r = 0; r < RL; ++r) sv[gi][r] = st[r * (N / RL) * W];
+      for (int r = 0; r < RL; ++r) sv[gi][r] = at32(st, (unsigned)(b + r * (N / RL) * W));
     }
   }
   load_tw<N>(tw, twN);
@@ -324,7 +328,7 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_prec(float2* __rest
 #pragma unroll
       for (int gi = 0; gi < G; ++gi) {
           const int f = f0 + gi;
-          float2* st = state + ((long)t * F + f) * T + b;
+          float2* st = state + ((long)t * F + f) * T;
 #pragma unroll
           for (int r = 0; r < RL; ++r) {
             const int e = r * (N / RL) * W;
@@ -332,15 +336,15 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_prec(float2* __rest
             if (MODE == 1) {
               float2 z = v[gi][r];
               if (f == 0 && kx == 0 && ky == 0 && kz == 0) z = make_float2(pk.k0, 0.f);
-              st[e] = z;
+              at32(st, (unsigned)(b + e)) = z;
             } else {
               const float kxv = kcomp(kx, N, pk.kfac);
               const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
-              const float2 sh = GSK_XUPD_PF ? sv[GSK_XUPD_PF ? gi : 0][r] : st[e];
+              const float2 sh = GSK_XUPD_PF ? sv[GSK_XUPD_PF ? gi : 0][r] : at32(st, (unsigned)(b + e));
               const float2 fh = v[gi][r];
               float2 nh;
               if (f == 0) {
-                const float bv = pk.celastic ? Bb[e] : 0.f;
+                const float bv = pk.celastic ? at32(Bb, (unsigned)(b + e)) : 0.f;
                 const float a = pk.xi * bv - pk.S;
                 const float mx = fh.x + a * sh.x, my = fh.y + a * sh.y;
                 const float den = 1.f / (1.f + pk.dtM * pk.S * k2 + pk.dtMkc * k2 * k2);
@@ -349,7 +353,7 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_prec(float2* __rest
                 const float den = 1.f / (1.f + pk.dtLke * k2);
                 nh = make_float2((sh.x - pk.dtL * fh.x) * den, (sh.y - pk.dtL * fh.y) * den);
               }
-              st[e] = nh;
+              at32(st, (unsigned)(b + e)) = nh;
               v[gi][r] = make_float2(nh.x * pk.inv_n3, nh.y * pk.inv_n3);
             }
           }
