@@ -339,7 +339,7 @@ struct Impl {
 
   // ---- fused building blocks
   // LDS of a z kernel holding nf field tiles
-  static size_t zlds(const gsk_sim* s, int nf) { return (2 * M + (size_t)nf * s->g.L * (M + 1)) * sizeof(float2); }
+  static size_t zlds(const gsk_sim* s, int nf) { return (2 * M + (size_t)nf * s->g.L * zpitch(M)) * sizeof(float2); }
   static int zfwd(gsk_sim* s, int op, const float* in, int F) {
     const int rows = s->n * s->n / s->g.L;
     if (op == Z_COPY) {
@@ -728,7 +728,7 @@ void choose_geometry(gsk_sim* s, int Fz, int Fx_resident) {
   s->xthreads = round64(std::max(W * n / rx, 64));
   s->lds_y = (size_t)(n + n * W) * sizeof(float2);
   s->lds_xc = (size_t)(n + (size_t)Fx_resident * n * W) * sizeof(float2);
-  s->lds_z = (2 * M + (size_t)Fz * L * (M + 1)) * sizeof(float2);
+  s->lds_z = (2 * M + (size_t)Fz * L * zpitch(M)) * sizeof(float2);
 }
 
 void choose_plain(gsk_sim* s) {

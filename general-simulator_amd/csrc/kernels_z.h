@@ -1,7 +1,7 @@
 // Contiguous-axis (z) passes of the FP32 fused step: real rows <-> half-spectrum rows.
 //
 // A z tile is L = zrows(M) consecutive rows (x*n + y) of one field. The complex
-// M-point FFT (M = n/2) of the packed real row runs in LDS (row pitch M+1,
+// M-point FFT (M = n/2) of the packed real row runs in LDS (row pitch zpitch(M),
 // compile-time tile shape and pass strides). The R2C / C2R split between the
 // packed M-point spectrum and the n/2+1 half-spectrum values is fused with the
 // global store / load: task (l, k), k < M/2, of a tile owns the spectrum pair
@@ -12,7 +12,7 @@
 //
 // Every thread issues all of its global loads for a tile before it touches LDS,
 // so a work group keeps L*M*8 bytes in flight while it waits; one field per LDS
-// tile keeps the footprint at about (L*(M+1) + 2M)*8 bytes so several work
+// tile keeps the footprint at about (L*zpitch(M) + 2M)*8 bytes so several work
 // groups share a CU. The middle martensite pass walks the variants one after the
 // other over the same LDS tile and prefetches the next variant's spectrum while
 // the current one is transformed.
@@ -81,6 +81,23 @@ __host__ __device__ constexpr int zthreads_of(int M, int rmin) {
 }
 template <int M> constexpr int zthreads() { return zthreads_of(M, ZPlanOf<M>::rmin); }
 
+// Row pitch of the z tile in LDS (complex elements): the smallest P >= M + 1 with
+// P = 16/L (mod 16) for L <= 16 rows per tile, P odd for L = 32. An FFT pass maps lanes
+// line fastest, and the LDS serves an 8-byte access 16 lanes at a time from 16 bank pairs:
+// those 16 lanes (L lines x 16/L consecutive positions) then hit 16 distinct pairs.
+// (Measured at n = 256, middle pass: P = 130 135 us, 129 141 us, 132 147 us, 136 173 us.)
+// GSK_ZPITCH_R overrides the residue mod 16.
+__host__ __device__ constexpr int zpitch(int M) {
+#ifdef GSK_ZPITCH_R
+  const int r = GSK_ZPITCH_R;
+#else
+  const int r = zrows(M) >= 16 ? 1 : 16 / zrows(M);
+#endif
+  int p = M + 1;
+  while (p % 16 != r % 16) ++p;
+  return p;
+}
+
 enum ZOp { Z_COPY = 0, Z_SQUARE = 1, Z_PREC_LOCAL = 2 };
 
 // --------------------------------------------------------------------- split formulas
@@ -139,14 +156,14 @@ struct ZSpec {
       }
     }
   }
-  // C2R split into the packed spectra of the LDS tile (row pitch M+1)
+  // C2R split into the packed spectra of the LDS tile (row pitch zpitch(M))
   __device__ __forceinline__ void split(float2* tile, const float2* rt, int tid = threadIdx.x) const {
 #pragma unroll
     for (int i = 0; i < S::NT; ++i) {
       const int e = tid + i * S::T;
       if (S::valid(e)) {
         const int l = e / S::H2, k = e % S::H2;
-        float2* z = tile + l * (M + 1);
+        float2* z = tile + l * zpitch(M);
         if (k == 0) {
           z[0] = make_float2(a[i].x + b[i].x, a[i].x - b[i].x);
           float2 zh, unused;
@@ -175,7 +192,7 @@ __device__ __forceinline__ void z_store_spec(const float2* tile, float2* __restr
     const unsigned e = tid + i * S::T;
     if (S::valid(e)) {
       const unsigned l = e / S::H2, k = e % S::H2, ro = rb + l * M;
-      const float2* z = tile + l * (M + 1);
+      const float2* z = tile + l * zpitch(M);
       float2 xk, xm;
       if (k == 0) {
         const float2 z0 = z[0], zh = z[S::H2];
@@ -213,7 +230,7 @@ __device__ __forceinline__ void z_twiddles(float2* tw, float2* rt, const float2*
 template <int M, bool FWD>
 __device__ __forceinline__ void z_fft1(float2* tile, const float2* tw) {
   constexpr int L = zrows(M);
-  const Lines Ls{L, L, M + 1, 0, 1};
+  const Lines Ls{L, L, zpitch(M), 0, 1};
   fft_lines_p<float2, M, FWD, typename ZPlanOf<M>::type>(tile, Ls, tw);
 }
 
@@ -247,7 +264,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd(const float* __restrict_
     if (S::valid(threadIdx.x + i * S::T)) {
       float4 t = v[i];
       if (OP == Z_SQUARE) t = make_float4(t.x * t.x, t.y * t.y, t.z * t.z, t.w * t.w);
-      float2* row = tile + l * (M + 1);
+      float2* row = tile + l * zpitch(M);
       row[2 * q] = make_float2(t.x, t.y);
       row[2 * q + 1] = make_float2(t.z, t.w);
     }
@@ -285,7 +302,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zinv(const float2* __restrict
     int l, q;
     z_quad<M>(i, l, q);
     if (S::valid(threadIdx.x + i * S::T)) {
-      const float2* row = tile + l * (M + 1);
+      const float2* row = tile + l * zpitch(M);
       const float2 a = row[2 * q], b = row[2 * q + 1];
       *reinterpret_cast<float4*>(dst + l * n + 4 * q) = make_float4(a.x, a.y, b.x, b.y);
     }
@@ -332,7 +349,7 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, Mar
   ZSpec<M> sp;
   sp.load(kbuf, g, r0);
   z_twiddles<M>(tw, rt, twM, rtn);
-  const Lines Ls{S::L, S::L, M + 1, 0, 1};
+  const Lines Ls{S::L, S::L, zpitch(M), 0, 1};
   auto no_ld = [](int, int) { return float2{}; };
   auto no_st = [](int, int, float2) {};
   const uint32_t step = step_of(mk.step, mk.stepp);
@@ -446,14 +463,14 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local(const float* __res
       }
     }
     for (int f = 0; f < F; ++f) {
-      float2* row = tile + (f * L + l) * (M + 1);
+      float2* row = tile + (f * L + l) * zpitch(M);
       row[2 * q] = make_float2(out[f][0], out[f][1]);
       row[2 * q + 1] = make_float2(out[f][2], out[f][3]);
     }
   }
   __syncthreads();
-  for (int f = 0; f < F; ++f) z_fft1<M, true>(tile + f * L * (M + 1), tw);
-  for (int f = 0; f < F; ++f) z_store_spec<M>(tile + f * L * (M + 1), kout + f * g.kfs, g, r0, rt);
+  for (int f = 0; f < F; ++f) z_fft1<M, true>(tile + f * L * zpitch(M), tw);
+  for (int f = 0; f < F; ++f) z_store_spec<M>(tile + f * L * zpitch(M), kout + f * g.kfs, g, r0, rt);
 }
 
 // Same as k_zfwd_local for a compile-time field count FT = V + 1 (2..5): every field of the
@@ -535,7 +552,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local_r(const float* __r
           out[k] = o;
         }
       }
-      float2* row = tile + l * (M + 1);
+      float2* row = tile + l * zpitch(M);
       row[2 * q] = make_float2(out[0], out[1]);
       row[2 * q + 1] = make_float2(out[2], out[3]);
     }
