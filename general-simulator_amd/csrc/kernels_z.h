@@ -475,9 +475,14 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local(const float* __res
 
 // Same as k_zfwd_local for a compile-time field count FT = V + 1 (2..5): every field of the
 // tile is loaded into registers first (16-byte quads), then the output fields are built and
-// transformed one after the other over a single LDS tile.
+// transformed one after the other over a single LDS tile. Inside the field loop the thread
+// index and the Philox counter are made opaque, so the compiler recomputes the LDS addresses
+// and the field-independent Philox work per field instead of holding them in registers.
+// Waves-per-SIMD floor 4 (at most 128 VGPRs) for n >= 256.
+template <int M, int FT> constexpr int zlocal_wpe() { return M >= 128 ? 4 : 1; }
 template <int M, int FT>
-__global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local_r(const float* __restrict__ in, float2* __restrict__ kout,
+__global__ void __launch_bounds__(zthreads<M>()) __attribute__((amdgpu_waves_per_eu(zlocal_wpe<M, FT>())))
+k_zfwd_local_r(const float* __restrict__ in, float2* __restrict__ kout,
                                                                Geo g, PrecK pk, const float2* __restrict__ twM,
                                                                const float2* __restrict__ rtn,
                                                                const float* __restrict__ R, float el, float cnoise) {
@@ -496,19 +501,27 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local_r(const float* __r
     z_quad<M>(i, l, q);
     if (S::valid(threadIdx.x + i * S::T)) {
 #pragma unroll
-      for (int f = 0; f < FT; ++f) v[i][f] = *reinterpret_cast<const float4*>(src + f * g.nvox + l * n + 4 * q);
+      for (int f = 0; f < FT; ++f)
+        v[i][f] = at32(reinterpret_cast<const float4*>(src + f * g.nvox), (unsigned)(l * n + 4 * q) / 4);
     }
   }
+  const Lines Ls{S::L, S::L, zpitch(M), 0, 1};
+  auto no_ld = [](int, int) { return float2{}; };
+  auto no_st = [](int, int, float2) {};
   z_twiddles<M>(tw, rt, twM, rtn);
   const uint32_t step = step_of(pk.step, pk.stepp);
 #pragma unroll 1
   for (int f = 0; f < FT; ++f) {
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
 #pragma unroll
     for (int i = 0; i < S::NT; ++i) {
-      int l, q;
-      z_quad<M>(i, l, q);
-      if (!S::valid(threadIdx.x + i * S::T)) continue;
-      const long vox = (r0 + l) * n + 4 * q;
+      const int e0 = tid + i * S::T;
+      if (!S::valid(e0)) continue;
+      const int l = e0 / S::H2, q = e0 % S::H2;
+      const unsigned vox = (unsigned)r0 * n + l * n + 4 * q;  // < n^3 <= 2^27
+      uint32_t gq = vox >> 2;
+      asm volatile("" : "+v"(gq));
       float e[FT][4];
 #pragma unroll
       for (int u = 0; u < FT; ++u) {
@@ -517,7 +530,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local_r(const float* __r
       float out[4];
       if (f == 0) {
         float xc[4] = {0.f, 0.f, 0.f, 0.f};
-        if (cnoise != 0.f) normal4(pk.seed, (uint32_t)(vox >> 2), 0x8000u, step, xc);
+        if (cnoise != 0.f) normal4(pk.seed, gq, 0x8000u, step, xc);
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           float s2 = 0.f;
@@ -527,7 +540,10 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local_r(const float* __r
         }
       } else {
         float xi[4] = {0.f, 0.f, 0.f, 0.f};
-        if (pk.noise != 0.f) normal4(pk.seed, (uint32_t)(vox >> 2), (uint32_t)(f - 1), step, xi);
+        if (pk.noise != 0.f) normal4(pk.seed, gq, (uint32_t)(f - 1), step, xi);
+        float4 rq = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (R) rq = at32(reinterpret_cast<const float4*>(R + (f - 1) * g.nvox + r0 * n), (unsigned)(l * n + 4 * q) / 4);
+        const float rk[4] = {rq.x, rq.y, rq.z, rq.w};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           float et = 0.f, t2 = 0.f, t4 = 0.f;
@@ -548,7 +564,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local_r(const float* __r
           const float e2 = et * et;
           float o = (2.f * pk.A5 * et + 4.f * pk.A6 * e2 * et) * t2 + 2.f * pk.A6 * et * t4 + 2.f * pk.A7 * et * t22 +
                     pk.A2 * et * (c - pk.c2) - pk.A3 * e2 * et + pk.A4 * e2 * e2 * et + pk.noise * xi[k];
-          if (R) o += el * R[(f - 1) * g.nvox + vox + k] * et;
+          if (R) o += el * rk[k] * et;
           out[k] = o;
         }
       }
@@ -557,8 +573,8 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local_r(const float* __r
       row[2 * q + 1] = make_float2(out[2], out[3]);
     }
     __syncthreads();
-    z_fft1<M, true>(tile, tw);
-    z_store_spec<M>(tile, kout + f * g.kfs, g, r0, rt);
+    gchain<float2, M, true, 0, 0, 1>(tile, Ls, tw, no_ld, no_st, (float2*)nullptr, typename ZPlanOf<M>::type{}, tid);
+    z_store_spec<M>(tile, kout + f * g.kfs, g, r0, rt, tid);
     __syncthreads();  // the next field overwrites the tile
   }
 }
