@@ -936,7 +936,11 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
     using T = std::remove_reference_t<decltype(*p)>;
     if (hipMalloc((void**)&p, count * sizeof(T)) != hipSuccess)
       return fail(GSK_ERR_HIP, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
-    if (hipMemset(p, 0, count * sizeof(T)) != hipSuccess) return fail(GSK_ERR_HIP, "hipMemset failed");
+    // zeroed on the handle's stream: the stream is non-blocking, so a memset on the null
+    // stream would not be ordered before the handle's kernels (it could land after the B
+    // tensor build has written the buffer)
+    if (hipMemsetAsync(p, 0, count * sizeof(T), s->stream) != hipSuccess)
+      return fail(GSK_ERR_HIP, "hipMemsetAsync failed");
     return GSK_OK;
   };
   const size_t T = (size_t)n * s->g.W, ntile = (size_t)s->g.ntile;
@@ -961,6 +965,7 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
   if ((rc = alloc(s->part, (size_t)s->rblocks * 16)) != GSK_OK) return bail(rc);
   if ((rc = alloc(s->rout, 16)) != GSK_OK) return bail(rc);
   if (cfg->path == GSK_PATH_FUSED32 && (rc = alloc(s->dstep, 1)) != GSK_OK) return bail(rc);
+  if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(GSK_ERR_HIP, "stream synchronisation failed"));
   build_slots(s);
   *out = s;
   return GSK_OK;

@@ -204,6 +204,41 @@ def test_mart_translation_equivariance_at_256(gpu):
 
 
 # ------------------------------------------------------------------ precipitate
+def test_mart_fused_vs_plain_at_512(gpu):
+    """512^3 with the 512^3 configuration's noise (0.002), strong dynamics: the FP32 fused step
+    against the FP64 plain path, an independent implementation (C2C transforms, full-layout
+    FP64 B tensor). Catches any wrong B entry or transform at the largest grid, where the oracle
+    is too slow. FP32 tolerance: a few steps keep the relative error near 1e-6."""
+    say("mart fused vs plain n=512")
+    n, V, steps = 512, 3, 4
+    kw = dict(A=-0.5, w_el=20.0, noise=0.002, seed=11)
+    out = []
+    for path in (gsk.PATH_FUSED32, gsk.PATH_PLAIN64):
+        s = mart_sim(n, V, path=path, **kw)
+        s.init_uniform("eta", -0.05, 0.05, 4)
+        s.step(steps)
+        out.append(s.get_field("eta"))
+        s.close()
+    got, want = out
+    assert np.isfinite(got).all()
+    assert rel_l2(got.astype(np.float64), want.astype(np.float64)) < 2e-6
+    assert float(np.abs(got - want).max()) < 2e-6 * float(np.abs(want).max())
+
+
+def test_mart_two_handles_bit_identical_at_512(gpu):
+    """Two handles created one after the other give bit-identical trajectories (every buffer
+    is ordered on the handle's own stream, including its zero fill at creation)."""
+    say("mart determinism n=512")
+    res = []
+    for _ in range(2):
+        s = mart_sim(512, 3, A=-0.5, w_el=20.0)
+        s.init_uniform("eta", -0.05, 0.05, 4)
+        s.step(2)
+        res.append(s.get_field("eta"))
+        s.close()
+    assert np.array_equal(res[0].view(np.uint32), res[1].view(np.uint32))
+
+
 def prec_sim(n, V=4, **kw):
     base = dict(n=n, nvar=V, M=1.0, L=1.0, kappa=1.5, kappa_c=1.5, dt=0.005, noise=0.001, seed=5, S=54.0)
     base.update(kw)
