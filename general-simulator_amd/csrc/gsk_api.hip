@@ -168,6 +168,10 @@ struct gsk_sim {
   double c_k0 = 0.0;  // FP64 sum of the stored concentration fluctuation (its k = 0 mode)
   double cee0 = 0.0;  // B(k = 0) of the c-sourced tensor
   bool haveB = false, dirty = true;
+  // martensite fused step: the k-space buffer holds the z-transformed eta^2 of the current
+  // fields (left there by the last step's fused inverse/square pass); cleared by anything that
+  // writes the fields or uses the buffer otherwise
+  bool sq_ready = false;
   int order = 0;  // visiting order of the last per-field launch
   long long step = 0;
   // profiling
@@ -306,6 +310,7 @@ struct Impl {
     CHK(lds_limit((const void*)k_zmid_mart<M, 3>));
     CHK(lds_limit((const void*)k_zmid_mart<M, 4>));
     CHK(lds_limit((const void*)k_zinv<M>));
+    CHK(lds_limit((const void*)k_zinv_sq<M>));
     CHK(lds_limit((const void*)k_ypass<N, true>));
     CHK(lds_limit((const void*)k_ypass<N, false>));
     CHK(lds_limit((const void*)k_xpass<N, true>));
@@ -429,11 +434,19 @@ struct Impl {
     return GSK_OK;
   }
 
+  // z-transformed eta^2 of the current fields into the k-space buffer (the first kernel of a
+  // step, unless the previous step's fused last pass has left it there)
+  static int square_fwd(gsk_sim* s) {
+    CHK(zfwd(s, Z_SQUARE, s->fld, s->F));
+    s->sq_ready = true;
+    return GSK_OK;
+  }
+
   static int step_mart_fused(gsk_sim* s) {
     const int gx = s->g.ntile;
     const int F = s->F;
     MartK mk = mart_consts(s);
-    CHK(zfwd(s, Z_SQUARE, s->fld, F));
+    if (!s->sq_ready) CHK(square_fwd(s));
     CHK(ypass(s, true, F));
     switch (F) {  // register-resident contraction for up to 4 variants
       case 1: LAUNCH((k_xcontract_r<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
@@ -457,7 +470,12 @@ struct Impl {
     CHK(ypass(s, true, F));
     CHK(xupdate_mart<0>(s, mk));
     CHK(ypass(s, false, F));
-    CHK(zinv(s, s->fld, F, s->dstep));
+    {  // eta back to real space, fused with the next step's forward pass of eta^2
+      const int rows = s->n * s->n / s->g.L;
+      LAUNCH((k_zinv_sq<M>), dim3(rows, F), s->zthreads, zlds(s, 1), s, s->kbuf, s->fld, geo_next(s), s->twM, s->rtn,
+             s->dstep);
+    }
+    s->sq_ready = true;
     return GSK_OK;
   }
 
@@ -599,7 +617,8 @@ struct Impl {
 
   static int energy_mart_el_fused(gsk_sim* s, double* el) {
     // S_hat = FFT(eta^2) through z and y; the energy kernel applies x and contracts.
-    CHK(zfwd(s, Z_SQUARE, s->fld, s->F));
+    if (!s->sq_ready) CHK(zfwd(s, Z_SQUARE, s->fld, s->F));
+    s->sq_ready = false;  // the y pass below transforms the buffer further
     CHK(ypass(s, true, s->F));
     const int gx = s->g.ntile;
     if (gx > s->rblocks) return fail(GSK_ERR, "reduction scratch too small");
@@ -656,6 +675,7 @@ int dispatch_n(int n, Args&&... args) {
 
 template <int N> struct DAttrs { static int run() { return Impl<N>::set_attrs(); } };
 template <int N> struct DInit { static int run(gsk_sim* s) { return Impl<N>::init_state(s); } };
+template <int N> struct DSquare { static int run(gsk_sim* s) { return Impl<N>::square_fwd(s); } };
 template <int N> struct DStep {
   static int run(gsk_sim* s) {
     if (s->cfg.path == GSK_PATH_PLAIN64) return Impl<N>::step_mart_plain(s);
@@ -775,7 +795,6 @@ void build_slots(gsk_sim* s) {
   }
   if (s->cfg.physics == GSK_PHYS_MART) {
     const int npq = F * (F + 1) / 2;
-    s->slots.push_back({"zfwd_square", F * (rv + hs)});
     s->slots.push_back({"y_fwd_S", F * 2 * hs});
     s->slots.push_back({"x_contract", F * 2 * hs + npq * hs / 2});
     s->slots.push_back({"y_inv_R", F * 2 * hs});
@@ -789,7 +808,7 @@ void build_slots(gsk_sim* s) {
       f0 += c;
     }
     s->slots.push_back({"y_inv_eta", F * 2 * hs});
-    s->slots.push_back({"zinv_eta", F * (hs + rv)});
+    s->slots.push_back({"zinv_eta_sq", F * (2 * hs + rv)});  // C2R, write eta, R2C of eta^2
   } else {
     s->slots.push_back({"zfwd_local", F * (rv + hs)});
     s->slots.push_back({"y_fwd", F * 2 * hs});
@@ -816,9 +835,15 @@ int prepare(gsk_sim* s) {
   if (!s->haveB) return fail(GSK_ERR_STATE, "B tensor not built (call gsk_build_btensor first)");
   const bool stateless = s->cfg.rule == GSK_RULE_REFERENCE && s->cfg.physics == GSK_PHYS_MART;
   if (s->dirty && s->cfg.path == GSK_PATH_FUSED32 && !stateless) {
+    s->sq_ready = false;  // the initialisation uses the k-space buffer
     CHK(dispatch_n<DInit>(s->n, s));
   }
   s->dirty = false;
+  // the spectral martensite step starts from eta^2 in k-space: computed here, so that every
+  // step of a timed or profiled run has the same launches
+  if (s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL &&
+      !s->sq_ready)
+    CHK(dispatch_n<DSquare>(s->n, s));
   return GSK_OK;
 }
 
@@ -1727,6 +1752,7 @@ int ens_restore(gsk_ensemble* e, const EnsSnapshot& snap) {
     HIPCHK(hipMemcpyAsync(s->dstep, &v, sizeof(v), hipMemcpyHostToDevice, s->stream));
     CHK(sync(s));
     s->step = snap.step[i];
+    s->sq_ready = false;  // restored fields: the next step transforms eta^2 afresh
   }
   return GSK_OK;
 }
@@ -1938,6 +1964,7 @@ int adj_get(gsk_sim* s, const float* fld, const float2* st, long long step) {
   HIPCHK(hipMemcpyAsync(s->dstep, &v, sizeof(v), hipMemcpyHostToDevice, s->stream));
   CHK(sync(s));
   s->step = step;
+  s->sq_ready = false;  // new fields: the next step transforms eta^2 afresh
   return GSK_OK;
 }
 

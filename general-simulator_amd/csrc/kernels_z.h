@@ -309,6 +309,54 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zinv(const float2* __restrict
   }
 }
 
+// --------------------------------------------------------------------- inverse z pass + next forward square
+// The last kernel of a martensite step fused with the first kernel of the next one: half-spectrum
+// rows of eta_f -> C2R -> eta_f written to the real field -> eta_f^2 -> R2C -> half-spectrum rows
+// written back in place (the z-transformed S_f = eta_f^2 the next step starts from). The squares
+// are taken of the FP32 values written to the real field, so S_f is bit-identical to a separate
+// forward pass over that field.
+// adv (may be null): step counter advanced once by the launch.
+template <int M>
+__global__ void __launch_bounds__(zthreads<M>()) k_zinv_sq(float2* __restrict__ kbuf, float* __restrict__ out, Geo g,
+                                                          const float2* __restrict__ twM,
+                                                          const float2* __restrict__ rtn, uint32_t* adv) {
+  if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *adv += 1u;
+  using S = ZShape<M>;
+  constexpr int n = 2 * M;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  float2* rt = smem + M;
+  float2* tile = smem + 2 * M;
+  const int f = geo_field(g);
+  const long r0 = (long)geo_tile(g) * S::L;
+  float2* field = kbuf + f * g.kfs;
+  {
+    ZSpec<M> sp;
+    sp.load(field, g, r0);
+    z_twiddles<M>(tw, rt, twM, rtn);
+    sp.split(tile, rt);
+  }
+  __syncthreads();
+  z_fft1<M, false>(tile, tw);
+  float* dst = out + f * g.nvox + r0 * n;
+#pragma unroll
+  for (int i = 0; i < S::NT; ++i) {
+    int l, q;
+    z_quad<M>(i, l, q);
+    if (S::valid(threadIdx.x + i * S::T)) {
+      float2* row = tile + l * zpitch(M);
+      const float2 a = row[2 * q], b = row[2 * q + 1];
+      at32(reinterpret_cast<float4*>(dst), (unsigned)(l * n + 4 * q) / 4) = make_float4(a.x, a.y, b.x, b.y);
+      // each thread squares the very elements it has just read: no barrier between
+      row[2 * q] = make_float2(a.x * a.x, a.y * a.y);
+      row[2 * q + 1] = make_float2(b.x * b.x, b.y * b.y);
+    }
+  }
+  __syncthreads();
+  z_fft1<M, true>(tile, tw);
+  z_store_spec<M>(tile, field, g, r0, rt);
+}
+
 // --------------------------------------------------------------------- martensite middle z pass
 // For each variant p in turn: R_hat_p rows -> C2R -> F_p = f'(eta) + 2 w eta_p R_p + a xi (Philox in
 // registers) -> R2C, in place. The last inverse Stockham pass leaves thread (l, j) = (b % L, b / L)

@@ -225,6 +225,26 @@ def test_mart_fused_vs_plain_at_512(gpu):
     assert float(np.abs(got - want).max()) < 2e-6 * float(np.abs(want).max())
 
 
+def test_mart_square_carry_bit_identical(gpu):
+    """A step leaves the z-transformed eta^2 of its result for the next step. Steps that run
+    back to back must give bit for bit the trajectory of steps separated by energy reductions
+    (which use the k-space buffer, so the next step transforms eta^2 afresh)."""
+    say("mart eta^2 carry-over n=64")
+    kw = dict(A=-0.5, w_el=20.0, noise=0.002, seed=5)
+    a = mart_sim(64, 3, **kw)
+    b = mart_sim(64, 3, **kw)
+    for s in (a, b):
+        s.init_uniform("eta", -0.05, 0.05, 2)
+    a.step(6)
+    for k in (2, 1, 3):
+        b.step(k)
+        b.energy()
+    ea, eb = a.get_field("eta"), b.get_field("eta")
+    a.close()
+    b.close()
+    assert np.array_equal(ea.view(np.uint32), eb.view(np.uint32))
+
+
 def test_mart_two_handles_bit_identical_at_512(gpu):
     """Two handles created one after the other give bit-identical trajectories (every buffer
     is ordered on the handle's own stream, including its zero fill at creation)."""
