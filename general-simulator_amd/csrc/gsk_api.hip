@@ -291,6 +291,17 @@ int lds_limit(const void* f) {
   return GSK_OK;
 }
 
+// Steps whose last z pass leaves the first k-space stage of the next step in the k-space buffer
+// (the spectral rules on the fused path; the precipitate with 2..5 fields).
+bool prec_carries(const gsk_sim* s) {
+  return s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_PREC && s->cfg.rule == GSK_RULE_SPECTRAL &&
+         s->F >= 2 && s->F <= 5;
+}
+bool carries(const gsk_sim* s) {
+  return prec_carries(s) || (s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART &&
+                             s->cfg.rule == GSK_RULE_SPECTRAL);
+}
+
 // ------------------------------------------------------------------ per-size implementation
 template <int N>
 struct Impl {
@@ -311,6 +322,10 @@ struct Impl {
     CHK(lds_limit((const void*)k_zmid_mart<M, 4>));
     CHK(lds_limit((const void*)k_zinv<M>));
     CHK(lds_limit((const void*)k_zinv_sq<M>));
+    CHK(lds_limit((const void*)k_zinv_local_r<M, 2>));
+    CHK(lds_limit((const void*)k_zinv_local_r<M, 3>));
+    CHK(lds_limit((const void*)k_zinv_local_r<M, 4>));
+    CHK(lds_limit((const void*)k_zinv_local_r<M, 5>));
     CHK(lds_limit((const void*)k_ypass<N, true>));
     CHK(lds_limit((const void*)k_ypass<N, false>));
     CHK(lds_limit((const void*)k_xpass<N, true>));
@@ -408,13 +423,15 @@ struct Impl {
              geo_next(s), pk, s->twN, 0);
       return GSK_OK;
     }
+    PrecK pg = pk;
     for (int f0 = 0; f0 < s->F; f0 += 2) {
       if (s->F - f0 >= 2)
-        LAUNCH((k_xupdate_prec<N, 2, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->Bt, s->g, pk, s->twN,
+        LAUNCH((k_xupdate_prec<N, 2, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->Bt, s->g, pg, s->twN,
                f0);
       else
-        LAUNCH((k_xupdate_prec<N, 1, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->Bt, s->g, pk, s->twN,
+        LAUNCH((k_xupdate_prec<N, 1, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->Bt, s->g, pg, s->twN,
                f0);
+      pg.adv = nullptr;  // one advance per step
     }
     return GSK_OK;
   }
@@ -437,7 +454,7 @@ struct Impl {
   // z-transformed eta^2 of the current fields into the k-space buffer (the first kernel of a
   // step, unless the previous step's fused last pass has left it there)
   static int square_fwd(gsk_sim* s) {
-    CHK(zfwd(s, Z_SQUARE, s->fld, s->F));
+    CHK(zfwd(s, s->cfg.physics == GSK_PHYS_MART ? Z_SQUARE : Z_PREC_LOCAL, s->fld, s->F));
     s->sq_ready = true;
     return GSK_OK;
   }
@@ -567,14 +584,27 @@ struct Impl {
     return step_prec_fused(s);
   }
 
+  // With 2..5 fields the last z pass is fused with the next step's first one (k_zinv_local_r)
+  // and the x update advances the noise counter.
   static int step_prec_fused(gsk_sim* s) {
     const int F = s->F;
     PrecK pk = prec_consts(s);
-    CHK(zfwd(s, Z_PREC_LOCAL, s->fld, F));
+    const bool fuse = prec_carries(s);
+    if (!fuse || !s->sq_ready) CHK(zfwd(s, Z_PREC_LOCAL, s->fld, F));
     CHK(ypass(s, true, F));
+    if (fuse) pk.adv = s->dstep;
     CHK(xupdate_prec<0>(s, pk));
     CHK(ypass(s, false, F));
-    CHK(zinv(s, s->fld, F, s->dstep));
+    if (!fuse) return zinv(s, s->fld, F, s->dstep);
+    pk.adv = nullptr;
+    const int rows = s->n * s->n / s->g.L;
+    switch (F) {
+      case 2: LAUNCH((k_zinv_local_r<M, 2>), rows, s->zthreads, zlds(s, 1), s, s->kbuf, s->fld, s->g, pk, s->twM, s->rtn); break;
+      case 3: LAUNCH((k_zinv_local_r<M, 3>), rows, s->zthreads, zlds(s, 1), s, s->kbuf, s->fld, s->g, pk, s->twM, s->rtn); break;
+      case 4: LAUNCH((k_zinv_local_r<M, 4>), rows, s->zthreads, zlds(s, 1), s, s->kbuf, s->fld, s->g, pk, s->twM, s->rtn); break;
+      default: LAUNCH((k_zinv_local_r<M, 5>), rows, s->zthreads, zlds(s, 1), s, s->kbuf, s->fld, s->g, pk, s->twM, s->rtn);
+    }
+    s->sq_ready = true;
     return GSK_OK;
   }
 
@@ -810,7 +840,8 @@ void build_slots(gsk_sim* s) {
     s->slots.push_back({"y_inv_eta", F * 2 * hs});
     s->slots.push_back({"zinv_eta_sq", F * (2 * hs + rv)});  // C2R, write eta, R2C of eta^2
   } else {
-    s->slots.push_back({"zfwd_local", F * (rv + hs)});
+    const bool fuse = prec_carries(s);
+    if (!fuse) s->slots.push_back({"zfwd_local", F * (rv + hs)});
     s->slots.push_back({"y_fwd", F * 2 * hs});
     if (GSK_XUPD_G == 0) {  // one launch, as Impl::xupdate_prec
       s->slots.push_back({"x_update", F * 4 * hs + hs / 2});
@@ -822,7 +853,8 @@ void build_slots(gsk_sim* s) {
       }
     }
     s->slots.push_back({"y_inv", F * 2 * hs});
-    s->slots.push_back({"zinv", F * (hs + rv)});
+    if (fuse) s->slots.push_back({"zinv_local", F * (2 * hs + rv)});  // C2R, write fields, local forces, R2C
+    else s->slots.push_back({"zinv", F * (hs + rv)});
   }
 }
 
@@ -839,11 +871,10 @@ int prepare(gsk_sim* s) {
     CHK(dispatch_n<DInit>(s->n, s));
   }
   s->dirty = false;
-  // the spectral martensite step starts from eta^2 in k-space: computed here, so that every
-  // step of a timed or profiled run has the same launches
-  if (s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL &&
-      !s->sq_ready)
-    CHK(dispatch_n<DSquare>(s->n, s));
+  // the spectral steps start from the z-transformed eta^2 (martensite) or local forces
+  // (precipitate) in k-space: computed here, so that every step of a timed or profiled run
+  // has the same launches
+  if (carries(s) && !s->sq_ready) CHK(dispatch_n<DSquare>(s->n, s));
   return GSK_OK;
 }
 
@@ -1235,6 +1266,7 @@ int gsk_step(gsk_sim* s, int nsteps) {
 int gsk_set_step(gsk_sim* s, long long step) {
   if (!s || step < 0) return fail(GSK_ERR_ARG, "bad argument");
   s->step = step;
+  s->sq_ready = false;  // carried-over noise was drawn for the old step index
   if (s->dstep) {
     CHK(ensure_device(s->dev));
     CHK(sync(s));
@@ -1260,6 +1292,8 @@ int gsk_set_param(gsk_sim* s, const char* key, double v) {
   else if (k.size() == 2 && k[0] == 'a' && k[1] >= '1' && k[1] <= '7') c.a[k[1] - '0'] = v;
   else return fail(GSK_ERR_ARG, "unknown parameter " + k);
   ++s->gen;
+  // the precipitate's carried-over local forces depend on the parameters
+  if (s->cfg.physics == GSK_PHYS_PREC) s->sq_ready = false;
   return GSK_OK;
 }
 

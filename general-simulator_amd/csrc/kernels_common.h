@@ -75,6 +75,7 @@ struct PrecK {
   int V;             // number of order parameters (fields = V + 1)
   uint32_t step;
   const uint32_t* stepp;  // device step counter (read instead of `step` when set)
+  uint32_t* adv;          // device step counter advanced by the x update (fused step), or null
   unsigned long long seed;
 };
 

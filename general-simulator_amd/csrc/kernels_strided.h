@@ -294,6 +294,9 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_prec(float2* __rest
   float2* tw = smem;
   float2* tile = smem + N;
   const int W = g.W, F = pk.V + 1, T = N * W;
+  // the fused precipitate step advances the noise counter here: the fused last z pass draws
+  // the next step's noise
+  if (MODE == 0 && pk.adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *pk.adv += 1u;
   const int t = geo_tile(g);
   if (gridDim.y > 1) f0 += geo_field(g) * G;  // one group of G fields per blockIdx.y
   const STile s = xtile(t, g);

@@ -521,42 +521,25 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local(const float* __res
   for (int f = 0; f < F; ++f) z_store_spec<M>(tile + f * L * zpitch(M), kout + f * g.kfs, g, r0, rt);
 }
 
-// Same as k_zfwd_local for a compile-time field count FT = V + 1 (2..5): every field of the
-// tile is loaded into registers first (16-byte quads), then the output fields are built and
-// transformed one after the other over a single LDS tile. Inside the field loop the thread
-// index and the Philox counter are made opaque, so the compiler recomputes the LDS addresses
-// and the field-independent Philox work per field instead of holding them in registers.
-// Waves-per-SIMD floor 4 (at most 128 VGPRs) for n >= 256.
+// Waves-per-SIMD floor of the register-resident precipitate z passes: 4 (at most 128 VGPRs)
+// for n >= 256.
 template <int M, int FT> constexpr int zlocal_wpe() { return M >= 128 ? 4 : 1; }
+
+// Local precipitate forces of one tile from its FT = V + 1 fields held in registers (v[i][f]:
+// quad i of the thread, field f), then the R2C of each output field over a single LDS tile.
+// Inside the field loop the thread index and the Philox counter are made opaque, so the
+// compiler recomputes the LDS addresses and the field-independent Philox work per field
+// instead of holding them in registers.
 template <int M, int FT>
-__global__ void __launch_bounds__(zthreads<M>()) __attribute__((amdgpu_waves_per_eu(zlocal_wpe<M, FT>())))
-k_zfwd_local_r(const float* __restrict__ in, float2* __restrict__ kout,
-                                                               Geo g, PrecK pk, const float2* __restrict__ twM,
-                                                               const float2* __restrict__ rtn,
-                                                               const float* __restrict__ R, float el, float cnoise) {
+__device__ __forceinline__ void zlocal_forces_r(const float4 (&v)[ZShape<M>::NT][FT], float2* tile, const float2* tw,
+                                                const float2* rt, float2* __restrict__ kout, const Geo& g,
+                                                const PrecK& pk, long r0, const float* __restrict__ R, float el,
+                                                float cnoise) {
   using S = ZShape<M>;
   constexpr int n = 2 * M, V = FT - 1;
-  extern __shared__ float2 smem[];
-  float2* tw = smem;
-  float2* rt = smem + M;
-  float2* tile = smem + 2 * M;
-  const long r0 = (long)blockIdx.x * S::L;
-  const float* src = in + r0 * n;
-  float4 v[S::NT][FT];
-#pragma unroll
-  for (int i = 0; i < S::NT; ++i) {
-    int l, q;
-    z_quad<M>(i, l, q);
-    if (S::valid(threadIdx.x + i * S::T)) {
-#pragma unroll
-      for (int f = 0; f < FT; ++f)
-        v[i][f] = at32(reinterpret_cast<const float4*>(src + f * g.nvox), (unsigned)(l * n + 4 * q) / 4);
-    }
-  }
   const Lines Ls{S::L, S::L, zpitch(M), 0, 1};
   auto no_ld = [](int, int) { return float2{}; };
   auto no_st = [](int, int, float2) {};
-  z_twiddles<M>(tw, rt, twM, rtn);
   const uint32_t step = step_of(pk.step, pk.stepp);
 #pragma unroll 1
   for (int f = 0; f < FT; ++f) {
@@ -625,6 +608,82 @@ k_zfwd_local_r(const float* __restrict__ in, float2* __restrict__ kout,
     z_store_spec<M>(tile, kout + f * g.kfs, g, r0, rt, tid);
     __syncthreads();  // the next field overwrites the tile
   }
+}
+
+// k_zfwd_local for a compile-time field count FT = V + 1 (2..5): every field of the tile is
+// loaded into registers first (16-byte quads), then the output fields are built and
+// transformed one after the other over a single LDS tile.
+// Waves-per-SIMD floor 4 (at most 128 VGPRs) for n >= 256.
+template <int M, int FT>
+__global__ void __launch_bounds__(zthreads<M>()) __attribute__((amdgpu_waves_per_eu(zlocal_wpe<M, FT>())))
+k_zfwd_local_r(const float* __restrict__ in, float2* __restrict__ kout, Geo g, PrecK pk,
+               const float2* __restrict__ twM, const float2* __restrict__ rtn, const float* __restrict__ R, float el,
+               float cnoise) {
+  using S = ZShape<M>;
+  constexpr int n = 2 * M;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  float2* rt = smem + M;
+  float2* tile = smem + 2 * M;
+  const long r0 = (long)blockIdx.x * S::L;
+  const float* src = in + r0 * n;
+  float4 v[S::NT][FT];
+#pragma unroll
+  for (int i = 0; i < S::NT; ++i) {
+    int l, q;
+    z_quad<M>(i, l, q);
+    if (S::valid(threadIdx.x + i * S::T)) {
+#pragma unroll
+      for (int f = 0; f < FT; ++f)
+        v[i][f] = at32(reinterpret_cast<const float4*>(src + f * g.nvox), (unsigned)(l * n + 4 * q) / 4);
+    }
+  }
+  z_twiddles<M>(tw, rt, twM, rtn);
+  zlocal_forces_r<M, FT>(v, tile, tw, rt, kout, g, pk, r0, R, el, cnoise);
+}
+
+// The last kernel of a precipitate step fused with the first kernel of the next one: for each
+// field in turn, half-spectrum rows -> C2R -> the real field (written out and kept in registers;
+// the next field's spectrum is loaded during the transform); then the local forces of the next
+// step from the registers, R2C, written back in place. The forces are formed from the FP32
+// values written to the real fields, so the result is bit-identical to k_zfwd_local_r over them.
+// The noise counter is read as the next step's (the step's x update has advanced it).
+template <int M, int FT>
+__global__ void __launch_bounds__(zthreads<M>()) __attribute__((amdgpu_waves_per_eu(zlocal_wpe<M, FT>())))
+k_zinv_local_r(float2* __restrict__ kbuf, float* __restrict__ out, Geo g, PrecK pk, const float2* __restrict__ twM,
+               const float2* __restrict__ rtn) {
+  using S = ZShape<M>;
+  constexpr int n = 2 * M;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  float2* rt = smem + M;
+  float2* tile = smem + 2 * M;
+  const long r0 = (long)blockIdx.x * S::L;
+  float* dst = out + r0 * n;
+  float4 v[S::NT][FT];
+  ZSpec<M> sp;
+  sp.load(kbuf, g, r0);
+  z_twiddles<M>(tw, rt, twM, rtn);
+#pragma unroll
+  for (int f = 0; f < FT; ++f) {
+    sp.split(tile, rt);
+    __syncthreads();
+    if (f + 1 < FT) sp.load(kbuf + (f + 1) * g.kfs, g, r0);
+    z_fft1<M, false>(tile, tw);
+#pragma unroll
+    for (int i = 0; i < S::NT; ++i) {
+      int l, q;
+      z_quad<M>(i, l, q);
+      if (S::valid(threadIdx.x + i * S::T)) {
+        const float2* row = tile + l * zpitch(M);
+        const float2 a = row[2 * q], b = row[2 * q + 1];
+        v[i][f] = make_float4(a.x, a.y, b.x, b.y);
+        at32(reinterpret_cast<float4*>(dst + f * g.nvox), (unsigned)(l * n + 4 * q) / 4) = v[i][f];
+      }
+    }
+    __syncthreads();  // the next field's split overwrites the tile
+  }
+  zlocal_forces_r<M, FT>(v, tile, tw, rt, kbuf, g, pk, r0, nullptr, 0.f, 0.f);
 }
 
 }  // namespace gsk

@@ -245,6 +245,27 @@ def test_mart_square_carry_bit_identical(gpu):
     assert np.array_equal(ea.view(np.uint32), eb.view(np.uint32))
 
 
+def test_prec_local_force_carry_bit_identical(gpu):
+    """The precipitate step's last z pass also forms the next step's local forces (with the
+    next step's noise). Steps back to back must give bit for bit the trajectory of steps
+    whose carried-over forces are dropped in between (a parameter set to its own value)."""
+    say("prec local-force carry-over n=64")
+    a, b = prec_sim(64), prec_sim(64)
+    for s in (a, b):
+        s.init_uniform("concentration", 0.245, 0.255, 11)
+        s.init_uniform("eta", 0.0, 0.01, 12)
+    a.step(5)
+    for k in (2, 1, 2):
+        b.step(k)
+        b.set_param("M", 1.0)
+    fa = [a.get_field(nm) for nm in ("concentration", "eta")]
+    fb = [b.get_field(nm) for nm in ("concentration", "eta")]
+    a.close()
+    b.close()
+    for x, y in zip(fa, fb):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
 def test_mart_two_handles_bit_identical_at_512(gpu):
     """Two handles created one after the other give bit-identical trajectories (every buffer
     is ordered on the handle's own stream, including its zero fill at creation)."""
