@@ -22,9 +22,78 @@ for p in (os.path.join(ROOT, "general-simulator_amd"), os.path.join(ROOT, "oracl
 
 HBM_PEAK_GBS = 8000.0
 
+# kernel symbol (a regular expression over the demangled name) behind each per-kernel timing
+# slot of the fused steps; the x updates in step mode (last template argument 0), not the
+# state initialisation
+SLOT_KERNEL = {
+    "y_fwd_S": r"k_ypass<", "y_inv_R": r"k_ypass<", "y_fwd_F": r"k_ypass<", "y_inv_eta": r"k_ypass<",
+    "y_fwd": r"k_ypass<", "y_inv": r"k_ypass<", "x_contract": r"k_xcontract", "zmid_force": r"k_zmid_mart<",
+    "x_update_g0": r"k_xupdate_mart<\d+, \d+, 0>", "x_update": r"k_xupdate_prec<\d+, \d+, 0>",
+    "zinv_eta_sq": r"k_zinv_sq<", "zinv_local": r"k_zinv_local_r<",
+}
+CALIB_MIB = 256
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def calib_copy(torch, mib):
+    """A device-to-device copy of `mib` MiB: reads and writes exactly that many bytes (the
+    reference the HBM counters are scaled by)."""
+    nb = mib << 20
+    src = torch.empty(nb // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    return nb
+
+
+def measure_traffic(a, slot):
+    """HBM bytes per launch of the kernel behind timing slot `slot`: two short rocprofv3 counter
+    runs of the same workload (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, averaged over the
+    kernel's dispatches), each scaled by what that counter reads for a 256 MiB device-to-device
+    copy in the same run. None when rocprofv3 is missing or a run fails."""
+    import csv
+    import glob
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    under_profiler = any(k.startswith("ROCPROF") for k in os.environ) or "rocprof" in os.environ.get("LD_PRELOAD", "")
+    if not prof or slot not in SLOT_KERNEL or under_profiler:
+        return None
+    out = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="gsk_pmc_", dir="/tmp")
+        cmd = [prof, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+               os.path.abspath(__file__), "--workload", a.workload, "--n", str(a.n), "--nvar", str(a.nvar),
+               "--noise", str(a.noise), "--steps", "2", "--warmup", "1", "--profile-steps", "1", "--no-cpu",
+               "--no-plain", "--no-traffic", "--calib-copy-mib", str(CALIB_MIB)]
+        try:
+            subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, timeout=150, check=True)
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            kern, calib = [], []
+            for r in csv.DictReader(open(files[0])):
+                v = float(r["Counter_Value"])
+                if re.search(SLOT_KERNEL[slot], r["Kernel_Name"]):
+                    kern.append(v)
+                elif "copyBuffer" in r["Kernel_Name"]:
+                    calib.append(v)
+        except Exception as e:  # noqa: BLE001 - the counter run is optional
+            log(f"traffic: {ctr} run failed ({type(e).__name__})")
+            return None
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        if not kern or not calib:
+            return None
+        # the largest copy dispatch is the calibration copy: CALIB_MIB MiB read and written
+        out[ctr] = sum(kern) / len(kern) * (CALIB_MIB * 1024.0 / max(calib)) * 1024.0
+    return round(out["FETCH_SIZE"] + out["WRITE_SIZE"])
 
 
 def make_sim(gsk, n, nvar, path, seed=0, noise=0.0):
@@ -113,7 +182,7 @@ def run_ensemble(a):
     seq = vox * a.ens_seq_steps / ts / 1e6
     finite = all(bool(__import__("numpy").isfinite(m.get_field("eta")).all()) for m in e.members)
     e.close()
-    names = {0: "sequential", 1: "streams", 2: "graph"}
+    names = {0: "sequential", 1: "streams", 2: "graph", 3: "batched", 4: "batched_graph"}
     ms = t / a.steps * 1e3
     result = {
         "metric": "aggregate Mvoxel-steps/s, 96-instance martensite ensemble (32 each at 64^3, 128^3, 192^3), FP32",
@@ -162,7 +231,7 @@ def run_prec(a):
     s.close()
     canon = (48 * (V + 1) + 2) * n**3
     result = {
-        "metric": "Mvoxel-steps/s, 256^3 precipitate (CH + 4 AC order parameters, c-sourced elasticity), FP32 fused",
+        "metric": f"Mvoxel-steps/s, {n}^3 precipitate (CH + 4 AC order parameters, c-sourced elasticity), FP32 fused",
         "value": round(n**3 * a.steps / (wall_ms / 1e3) / 1e6, 2), "unit": "Mvoxel-steps/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (Philox uniform c and eta)",
@@ -177,6 +246,10 @@ def run_prec(a):
         "mass_drift_rel": abs(m1 - m0) / m0,
         "finite": finite,
     }
+    if a.calib_copy_mib:
+        result["calib_copy_bytes"] = calib_copy(torch, a.calib_copy_mib)
+    if not a.no_traffic:
+        result["roofline"]["traffic"] = measure_traffic(a, dom[0])
     print(json.dumps(result), flush=True)
 
 
@@ -193,6 +266,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs for roofline.traffic")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size once (counter calibration)")
     ap.add_argument("--workload", choices=["mart256", "ensemble", "prec256"], default="mart256",
@@ -237,7 +311,7 @@ def main():
     canon_bytes = (84 * V + V * (V + 1)) * n**3  # canonical fused schedule of nine kernels, bytes per step
     sched_bytes = (80 * V + V * (V + 1)) * n**3  # this schedule: the last z pass also transforms eta^2
     result = {
-        "metric": "Mvoxel-steps/s, 256^3 3-variant AC+elasticity, FP32 fused spectral step",
+        "metric": f"Mvoxel-steps/s, {n}^3 {V}-variant AC+elasticity, FP32 fused spectral step",
         "value": round(mvox, 2),
         "unit": "Mvoxel-steps/s",
         "n_gpus": 1,
@@ -274,16 +348,10 @@ def main():
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(n, V, a.cpu_steps)
     if a.calib_copy_mib:
-        # a plain device-to-device copy of known size: reads and writes exactly this many bytes
-        nb = a.calib_copy_mib << 20
-        src = torch.empty(nb // 4, dtype=torch.float32, device="cuda").fill_(1.0)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        total = float(src.sum())  # a read of exactly nb bytes
-        torch.cuda.synchronize()
-        result["calib_copy_bytes"] = nb
-        result["calib_sum"] = total
+        result["calib_copy_bytes"] = calib_copy(torch, a.calib_copy_mib)
+    if not a.no_traffic:
+        log("traffic (rocprofv3 counters)")
+        result["roofline"]["traffic"] = measure_traffic(a, dom[0])
     print(json.dumps(result), flush=True)
 
 

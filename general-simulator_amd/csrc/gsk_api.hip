@@ -384,8 +384,8 @@ struct Impl {
   static int ypass(gsk_sim* s, bool fwd, int F) {
     const dim3 grid(s->g.ntile, GSK_Y_PERFIELD ? F : 1);
     const Geo g = GSK_Y_PERFIELD ? geo_next(s) : s->g;
-    if (fwd) LAUNCH((k_ypass<N, true>), grid, s->xthreads, s->lds_y, s, s->kbuf, g, F, s->twN);
-    else LAUNCH((k_ypass<N, false>), grid, s->xthreads, s->lds_y, s, s->kbuf, g, F, s->twN);
+    if (fwd) LAUNCH((k_ypass<N, true>), grid, s->xthreads, s->lds_y, s, s->kbuf, g, F, s->twN, (const MemberPtrs*)nullptr);
+    else LAUNCH((k_ypass<N, false>), grid, s->xthreads, s->lds_y, s, s->kbuf, g, F, s->twN, (const MemberPtrs*)nullptr);
     return GSK_OK;
   }
   static int zinv(gsk_sim* s, float* out, int F, uint32_t* adv = nullptr) {
@@ -400,17 +400,17 @@ struct Impl {
     const int gx = s->g.ntile;
     if (GSK_XUPD_G == 0) {  // one field per work group, one launch
       LAUNCH((k_xupdate_mart<N, 1, MODE>), dim3(gx, s->F), s->xthreads, s->lds_y, s, s->kbuf, s->state, geo_next(s), mk,
-             s->twN, 0);
+             s->twN, 0, (const MemberPtrs*)nullptr);
       return GSK_OK;
     }
     for (int f0 = 0; f0 < s->F;) {
       const int G = xupd_group(s->F - f0);
       if (G == 3)
-        LAUNCH((k_xupdate_mart<N, 3, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN, f0);
+        LAUNCH((k_xupdate_mart<N, 3, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN, f0, (const MemberPtrs*)nullptr);
       else if (G == 2)
-        LAUNCH((k_xupdate_mart<N, 2, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN, f0);
+        LAUNCH((k_xupdate_mart<N, 2, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN, f0, (const MemberPtrs*)nullptr);
       else
-        LAUNCH((k_xupdate_mart<N, 1, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN, f0);
+        LAUNCH((k_xupdate_mart<N, 1, MODE>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->state, s->g, mk, s->twN, f0, (const MemberPtrs*)nullptr);
       f0 += G;
     }
     return GSK_OK;
@@ -466,10 +466,10 @@ struct Impl {
     if (!s->sq_ready) CHK(square_fwd(s));
     CHK(ypass(s, true, F));
     switch (F) {  // register-resident contraction for up to 4 variants
-      case 1: LAUNCH((k_xcontract_r<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
-      case 2: LAUNCH((k_xcontract_r<N, 2>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
-      case 3: LAUNCH((k_xcontract_r<N, 3>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
-      case 4: LAUNCH((k_xcontract_r<N, 4>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
+      case 1: LAUNCH((k_xcontract_r<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN, (const MemberPtrs*)nullptr); break;
+      case 2: LAUNCH((k_xcontract_r<N, 2>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN, (const MemberPtrs*)nullptr); break;
+      case 3: LAUNCH((k_xcontract_r<N, 3>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN, (const MemberPtrs*)nullptr); break;
+      case 4: LAUNCH((k_xcontract_r<N, 4>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN, (const MemberPtrs*)nullptr); break;
       default: LAUNCH((k_xcontract<N>), gx, s->xthreads, s->lds_xc, s, s->kbuf, s->Bt, s->g, mk, s->twN);
     }
     CHK(ypass(s, false, F));
@@ -477,11 +477,11 @@ struct Impl {
       const int grid = s->n * s->n / s->g.L;
       const size_t lds = zlds(s, 1);
       switch (F) {  // eta prefetched into registers for up to 4 variants
-        case 1: LAUNCH((k_zmid_mart<M, 1>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
-        case 2: LAUNCH((k_zmid_mart<M, 2>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
-        case 3: LAUNCH((k_zmid_mart<M, 3>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
-        case 4: LAUNCH((k_zmid_mart<M, 4>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn); break;
-        default: LAUNCH((k_zmid_mart<M, 0>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn);
+        case 1: LAUNCH((k_zmid_mart<M, 1>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+        case 2: LAUNCH((k_zmid_mart<M, 2>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+        case 3: LAUNCH((k_zmid_mart<M, 3>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+        case 4: LAUNCH((k_zmid_mart<M, 4>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+        default: LAUNCH((k_zmid_mart<M, 0>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr);
       }
     }
     CHK(ypass(s, true, F));
@@ -490,9 +490,63 @@ struct Impl {
     {  // eta back to real space, fused with the next step's forward pass of eta^2
       const int rows = s->n * s->n / s->g.L;
       LAUNCH((k_zinv_sq<M>), dim3(rows, F), s->zthreads, zlds(s, 1), s, s->kbuf, s->fld, geo_next(s), s->twM, s->rtn,
-             s->dstep);
+             s->dstep, (const MemberPtrs*)nullptr);
     }
     s->sq_ready = true;
+    return GSK_OK;
+  }
+
+  // One spectral martensite step of `cnt` members of this grid size in every launch
+  // (grid.z = member). The members' buffers and constants come from the device table `tab`;
+  // s0, one of them, supplies the geometry, launch shapes and twiddle tables, which members
+  // of one size share. Every member must hold its z-transformed eta^2 (sq_ready).
+  static int step_mart_batched(gsk_sim* s0, int cnt, const MemberPtrs* tab, hipStream_t st) {
+    const int F = s0->F, gx = s0->g.ntile, rows = s0->n * s0->n / s0->g.L;
+    const MartK mk{};  // replaced by the table entry in every work group
+    const size_t lz = zlds(s0, 1);
+    float2* nk = nullptr;
+    auto y = [&](bool fwd) -> int {
+      if (fwd)
+        hipLaunchKernelGGL((k_ypass<N, true>), dim3(gx, F, cnt), dim3(s0->xthreads), s0->lds_y, st, nk, geo_next(s0),
+                           F, s0->twN, tab);
+      else
+        hipLaunchKernelGGL((k_ypass<N, false>), dim3(gx, F, cnt), dim3(s0->xthreads), s0->lds_y, st, nk, geo_next(s0),
+                           F, s0->twN, tab);
+      HIPCHK(hipGetLastError());
+      return GSK_OK;
+    };
+    CHK(y(true));
+    switch (F) {
+#define GSK_XC(V)                                                                                                  \
+  case V:                                                                                                          \
+    hipLaunchKernelGGL((k_xcontract_r<N, V>), dim3(gx, 1, cnt), dim3(s0->xthreads), s0->lds_y, st, nk,            \
+                       (const float*)nullptr, s0->g, mk, s0->twN, tab);                                            \
+    break;
+      GSK_XC(1) GSK_XC(2) GSK_XC(3) GSK_XC(4)
+#undef GSK_XC
+      default: return fail(GSK_ERR_UNSUPPORTED, "batched ensemble launches take 1..4 variants");
+    }
+    HIPCHK(hipGetLastError());
+    CHK(y(false));
+    switch (F) {
+#define GSK_ZM(V)                                                                                                  \
+  case V:                                                                                                          \
+    hipLaunchKernelGGL((k_zmid_mart<M, V>), dim3(rows, 1, cnt), dim3(s0->zthreads), lz, st, nk,                   \
+                       (const float*)nullptr, s0->g, mk, s0->twM, s0->rtn, tab);                                   \
+    break;
+      GSK_ZM(1) GSK_ZM(2) GSK_ZM(3) GSK_ZM(4)
+#undef GSK_ZM
+      default: return fail(GSK_ERR_UNSUPPORTED, "batched ensemble launches take 1..4 variants");
+    }
+    HIPCHK(hipGetLastError());
+    CHK(y(true));
+    hipLaunchKernelGGL((k_xupdate_mart<N, 1, 0>), dim3(gx, F, cnt), dim3(s0->xthreads), s0->lds_y, st, nk,
+                       (float2*)nullptr, geo_next(s0), mk, s0->twN, 0, tab);
+    HIPCHK(hipGetLastError());
+    CHK(y(false));
+    hipLaunchKernelGGL((k_zinv_sq<M>), dim3(rows, F, cnt), dim3(s0->zthreads), lz, st, nk, (float*)nullptr,
+                       geo_next(s0), s0->twM, s0->rtn, (uint32_t*)nullptr, tab);
+    HIPCHK(hipGetLastError());
     return GSK_OK;
   }
 
@@ -500,13 +554,13 @@ struct Impl {
   static int r2c_buf(gsk_sim* s, const float* in, float2* kb, int F) {
     const int rows = s->n * s->n / s->g.L;
     LAUNCH((k_zfwd<M, Z_COPY>), dim3(rows, F), s->zthreads, zlds(s, 1), s, in, kb, s->g, s->twM, s->rtn);
-    LAUNCH((k_ypass<N, true>), dim3(s->g.ntile, F), s->xthreads, s->lds_y, s, kb, s->g, F, s->twN);
+    LAUNCH((k_ypass<N, true>), dim3(s->g.ntile, F), s->xthreads, s->lds_y, s, kb, s->g, F, s->twN, (const MemberPtrs*)nullptr);
     LAUNCH((k_xpass<N, true>), s->g.ntile, s->xthreads, s->lds_y, s, kb, s->g, F, 1.0f, s->twN);
     return GSK_OK;
   }
   static int c2r_buf(gsk_sim* s, float2* kb, float* out, int F) {
     LAUNCH((k_xpass<N, false>), s->g.ntile, s->xthreads, s->lds_y, s, kb, s->g, F, 1.0f, s->twN);
-    LAUNCH((k_ypass<N, false>), dim3(s->g.ntile, F), s->xthreads, s->lds_y, s, kb, s->g, F, s->twN);
+    LAUNCH((k_ypass<N, false>), dim3(s->g.ntile, F), s->xthreads, s->lds_y, s, kb, s->g, F, s->twN, (const MemberPtrs*)nullptr);
     const int rows = s->n * s->n / s->g.L;
     LAUNCH((k_zinv<M>), dim3(rows, F), s->zthreads, zlds(s, 1), s, kb, out, s->g, s->twM, s->rtn,
            (uint32_t*)nullptr);
@@ -545,10 +599,10 @@ struct Impl {
     CHK(zfwd(s, Z_SQUARE, eta, V));
     CHK(ypass(s, true, V));
     switch (V) {
-      case 1: LAUNCH((k_xcontract_r<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
-      case 2: LAUNCH((k_xcontract_r<N, 2>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
-      case 3: LAUNCH((k_xcontract_r<N, 3>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
-      case 4: LAUNCH((k_xcontract_r<N, 4>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN); break;
+      case 1: LAUNCH((k_xcontract_r<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN, (const MemberPtrs*)nullptr); break;
+      case 2: LAUNCH((k_xcontract_r<N, 2>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN, (const MemberPtrs*)nullptr); break;
+      case 3: LAUNCH((k_xcontract_r<N, 3>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN, (const MemberPtrs*)nullptr); break;
+      case 4: LAUNCH((k_xcontract_r<N, 4>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN, (const MemberPtrs*)nullptr); break;
       default: LAUNCH((k_xcontract<N>), gx, s->xthreads, s->lds_xc, s, s->kbuf, s->Bt, s->g, mk, s->twN);
     }
     CHK(ypass(s, false, V));
@@ -706,6 +760,11 @@ int dispatch_n(int n, Args&&... args) {
 template <int N> struct DAttrs { static int run() { return Impl<N>::set_attrs(); } };
 template <int N> struct DInit { static int run(gsk_sim* s) { return Impl<N>::init_state(s); } };
 template <int N> struct DSquare { static int run(gsk_sim* s) { return Impl<N>::square_fwd(s); } };
+template <int N> struct DBatch {
+  static int run(gsk_sim* s0, int cnt, const MemberPtrs* tab, hipStream_t st) {
+    return Impl<N>::step_mart_batched(s0, cnt, tab, st);
+  }
+};
 template <int N> struct DStep {
   static int run(gsk_sim* s) {
     if (s->cfg.path == GSK_PATH_PLAIN64) return Impl<N>::step_mart_plain(s);
@@ -1617,8 +1676,16 @@ struct gsk_ensemble {
   std::vector<hipStream_t> bs;
   std::vector<hipEvent_t> join;
   hipGraphExec_t exec = nullptr;
-  int g_branches = -1, g_spg = -1;
+  int g_mode = -1, g_branches = -1, g_spg = -1;
   std::vector<unsigned> g_gen;
+  // batched schedules: members of one grid size and variant count share every launch
+  struct Cls {
+    std::vector<int> idx;          // members of the class
+    MemberPtrs* dtab = nullptr;    // their device table (buffers and constants)
+  };
+  std::vector<Cls> cls;
+  std::vector<int> loose;          // members that run one by one (other rules, physics or V > 4)
+  std::vector<unsigned> cls_gen;   // member generations the tables were built from
 };
 
 namespace {
@@ -1654,6 +1721,13 @@ int ens_streams(gsk_ensemble* e, int B) {
     HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     e->join.push_back(ev);
   }
+  return GSK_OK;
+}
+
+int sync_all(gsk_ensemble* e) {
+  if (e->master) HIPCHK(hipStreamSynchronize(e->master));
+  for (hipStream_t st : e->bs) HIPCHK(hipStreamSynchronize(st));
+  for (gsk_sim* s : e->m) CHK(sync(s));
   return GSK_OK;
 }
 
@@ -1694,17 +1768,99 @@ int ens_launch(gsk_ensemble* e, const std::vector<std::vector<int>>& asg, int st
 void ens_drop_graph(gsk_ensemble* e) {
   if (e->exec) (void)hipGraphExecDestroy(e->exec);
   e->exec = nullptr;
-  e->g_branches = e->g_spg = -1;
+  e->g_mode = e->g_branches = e->g_spg = -1;
   e->g_gen.clear();
 }
+
+bool batchable(const gsk_sim* s) {
+  return s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL &&
+         s->V >= 1 && s->V <= 4;
+}
+
+void ens_drop_classes(gsk_ensemble* e) {
+  for (auto& c : e->cls)
+    if (c.dtab) (void)hipFree(c.dtab);
+  e->cls.clear();
+  e->loose.clear();
+  e->cls_gen.clear();
+}
+
+// Size classes of the batched schedules and their device tables, rebuilt when a member's
+// constants change (the tables hold each member's buffers and physics constants).
+int ens_classes(gsk_ensemble* e) {
+  std::vector<unsigned> gen;
+  for (gsk_sim* s : e->m) gen.push_back(s->gen);
+  if (!e->cls_gen.empty() && gen == e->cls_gen) return GSK_OK;
+  ens_drop_graph(e);  // a captured graph refers to the old tables
+  CHK(sync_all(e));
+  ens_drop_classes(e);
+  std::vector<std::pair<int, int>> keys;  // (n, V) of each class
+  for (int i = 0; i < (int)e->m.size(); ++i) {
+    const gsk_sim* s = e->m[i];
+    if (!batchable(s)) {
+      e->loose.push_back(i);
+      continue;
+    }
+    size_t c = 0;
+    while (c < keys.size() && keys[c] != std::make_pair(s->n, s->V)) ++c;
+    if (c == keys.size()) {
+      keys.push_back({s->n, s->V});
+      e->cls.emplace_back();
+    }
+    e->cls[c].idx.push_back(i);
+  }
+  for (auto& c : e->cls) {
+    std::vector<MemberPtrs> h;
+    for (int i : c.idx) {
+      gsk_sim* s = e->m[i];
+      h.push_back(MemberPtrs{s->kbuf, s->state, s->fld, s->Bt, s->dstep, mart_consts(s)});
+    }
+    HIPCHK(hipMalloc((void**)&c.dtab, h.size() * sizeof(MemberPtrs)));
+    HIPCHK(hipMemcpy(c.dtab, h.data(), h.size() * sizeof(MemberPtrs), hipMemcpyHostToDevice));
+  }
+  e->cls_gen = gen;
+  return GSK_OK;
+}
+
+// Enqueue `steps` batched steps: one branch stream per size class (every launch covers the
+// class's members), the other members spread over further branches as in ens_launch.
+int ens_launch_batched(gsk_ensemble* e, int steps) {
+  const int C = (int)e->cls.size();
+  const int Lb = e->loose.empty() ? 0 : std::min((int)e->loose.size(), ens_branches(e));
+  CHK(ens_streams(e, C + Lb));
+  HIPCHK(hipEventRecord(e->fork, e->master));
+  for (int b = 0; b < C + Lb; ++b) HIPCHK(hipStreamWaitEvent(e->bs[b], e->fork, 0));
+  for (int c = 0; c < C; ++c) {
+    gsk_sim* s0 = e->m[e->cls[c].idx[0]];
+    const int cnt = (int)e->cls[c].idx.size();
+    for (int k = 0; k < steps; ++k) CHK(dispatch_n<DBatch>(s0->n, s0, cnt, (const MemberPtrs*)e->cls[c].dtab, e->bs[c]));
+  }
+  for (int j = 0; j < (int)e->loose.size(); ++j) {
+    gsk_sim* s = e->m[e->loose[j]];
+    hipStream_t own = s->stream;
+    s->stream = e->bs[C + j % Lb];
+    int rc = GSK_OK;
+    for (int k = 0; k < steps && rc == GSK_OK; ++k) rc = dispatch_n<DStep>(s->n, s);
+    s->stream = own;
+    CHK(rc);
+  }
+  for (int b = 0; b < C + Lb; ++b) {
+    HIPCHK(hipEventRecord(e->join[b], e->bs[b]));
+    HIPCHK(hipStreamWaitEvent(e->master, e->join[b], 0));
+  }
+  return GSK_OK;
+}
+
+bool batched_mode(int mode) { return mode == GSK_ENS_BATCHED || mode == GSK_ENS_BATCHED_GRAPH; }
 
 int ens_graph(gsk_ensemble* e, const std::vector<std::vector<int>>& asg, int spg) {
   std::vector<unsigned> gen;
   for (gsk_sim* s : e->m) gen.push_back(s->gen);
-  if (e->exec && e->g_branches == (int)asg.size() && e->g_spg == spg && gen == e->g_gen) return GSK_OK;
+  if (e->exec && e->g_mode == e->mode && e->g_branches == (int)asg.size() && e->g_spg == spg && gen == e->g_gen)
+    return GSK_OK;
   ens_drop_graph(e);
   HIPCHK(hipStreamBeginCapture(e->master, hipStreamCaptureModeThreadLocal));
-  const int rc = ens_launch(e, asg, spg);
+  const int rc = batched_mode(e->mode) ? ens_launch_batched(e, spg) : ens_launch(e, asg, spg);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(e->master, &graph);
   if (rc != GSK_OK || ec != hipSuccess) {
@@ -1717,6 +1873,7 @@ int ens_graph(gsk_ensemble* e, const std::vector<std::vector<int>>& asg, int spg
     e->exec = nullptr;
     return fail(GSK_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
   }
+  e->g_mode = e->mode;
   e->g_branches = (int)asg.size();
   e->g_spg = spg;
   e->g_gen = gen;
@@ -1732,15 +1889,17 @@ int ens_run(gsk_ensemble* e, int nsteps) {
       CHK(sync(s));
     }
   } else {
+    const bool bat = batched_mode(e->mode);
     const int B = ens_branches(e);
     CHK(ens_streams(e, B));
+    if (bat) CHK(ens_classes(e));
     const auto asg = ens_assign(e, B);
     int left = nsteps;
-    if (e->mode == GSK_ENS_GRAPH && nsteps >= e->spg) {
+    if ((e->mode == GSK_ENS_GRAPH || e->mode == GSK_ENS_BATCHED_GRAPH) && nsteps >= e->spg) {
       CHK(ens_graph(e, asg, e->spg));
       for (; left >= e->spg; left -= e->spg) HIPCHK(hipGraphLaunch(e->exec, e->master));
     }
-    if (left > 0) CHK(ens_launch(e, asg, left));
+    if (left > 0) CHK(bat ? ens_launch_batched(e, left) : ens_launch(e, asg, left));
     HIPCHK(hipStreamSynchronize(e->master));
   }
   for (gsk_sim* s : e->m) s->step += nsteps;
@@ -1823,6 +1982,7 @@ int gsk_ensemble_destroy(gsk_ensemble* e) {
   (void)hipSetDevice(e->dev);
   if (e->master) (void)hipStreamSynchronize(e->master);
   ens_drop_graph(e);
+  ens_drop_classes(e);
   for (gsk_sim* s : e->m) gsk_destroy(s);
   for (float* p : e->sharedB) (void)hipFree(p);
   for (hipStream_t st : e->bs) (void)hipStreamDestroy(st);
@@ -1875,7 +2035,7 @@ int gsk_ensemble_build_btensor(gsk_ensemble* e, const double* C81, const double*
 
 int gsk_ensemble_set_schedule(gsk_ensemble* e, int mode, int branches, int steps_per_graph) {
   if (!e) return fail(GSK_ERR_ARG, "null handle");
-  if (mode < GSK_ENS_SEQUENTIAL || mode > GSK_ENS_GRAPH) return fail(GSK_ERR_ARG, "unknown schedule");
+  if (mode < GSK_ENS_SEQUENTIAL || mode > GSK_ENS_BATCHED_GRAPH) return fail(GSK_ERR_ARG, "unknown schedule");
   if (branches < 0 || steps_per_graph < 1) return fail(GSK_ERR_ARG, "bad schedule parameters");
   e->mode = mode;
   e->branches = branches;
@@ -1912,6 +2072,12 @@ int gsk_ensemble_autotune(gsk_ensemble* e, int trial_steps, double* table, int m
       for (int k : {1, 5, 10})
         if (k <= trial_steps) cand.push_back({GSK_ENS_GRAPH, b, k, 0.0});
     }
+  {  // batched launches per size class; branches serve the members outside the classes
+    const int b = std::min(cnt, 8);
+    cand.push_back({GSK_ENS_BATCHED, b, 1, 0.0});
+    for (int k : {1, 5, 10})
+      if (k <= trial_steps) cand.push_back({GSK_ENS_BATCHED_GRAPH, b, k, 0.0});
+  }
   const int mode0 = e->mode, b0 = e->branches, k0 = e->spg;
   int rc = GSK_OK;
   for (Cand& c : cand) {

@@ -79,6 +79,17 @@ struct PrecK {
   unsigned long long seed;
 };
 
+// One member of a batched ensemble launch (grid.z = member): its buffers and constants. The
+// batched kernels read entry blockIdx.z instead of their pointer and constant arguments.
+struct MemberPtrs {
+  float2* kbuf;
+  float2* state;
+  float* fld;
+  const float* Bt;
+  uint32_t* dstep;
+  MartK mk;
+};
+
 // Step index of the noise counter: the device counter when the launch carries one
 // (it is advanced on the device at the end of every fused step, so a captured graph
 // replays with fresh noise), else the value passed at launch.

@@ -68,7 +68,8 @@ __device__ __forceinline__ TileIO tio(float2* field, const STile& s) {
 // ---------------------------------------------------------------- y pass (C2C along y, in place)
 template <int N, bool FWD>
 __global__ void __launch_bounds__(xy_threads<N>()) k_ypass(float2* __restrict__ kbuf, Geo g, int F,
-                                               const float2* __restrict__ twN) {
+                                               const float2* __restrict__ twN, const MemberPtrs* __restrict__ mt) {
+  if (mt) kbuf = mt[blockIdx.z].kbuf;  // batched ensemble launch
   extern __shared__ float2 smem[];
   float2* tw = smem;
   float2* tile = smem + N;
@@ -110,7 +111,13 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xpass(float2* __restrict__ 
 // S_hat -> x FFT (to registers) -> R_hat_p = sum_q B_pq S_hat_q / n^3 -> inverse x FFT.
 template <int N, int V>
 __global__ void __launch_bounds__(xy_threads<N>()) k_xcontract_r(float2* __restrict__ kbuf, const float* __restrict__ Bt, Geo g,
-                                                     MartK mk, const float2* __restrict__ twN) {
+                                                     MartK mk, const float2* __restrict__ twN,
+                                                     const MemberPtrs* __restrict__ mt) {
+  if (mt) {  // batched ensemble launch
+    kbuf = mt[blockIdx.z].kbuf;
+    Bt = mt[blockIdx.z].Bt;
+    mk = mt[blockIdx.z].mk;
+  }
   constexpr int RL = edge_radix<N>();
   constexpr int NPQ = V * (V + 1) / 2;
   extern __shared__ float2 smem[];
@@ -215,7 +222,13 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xcontract(float2* __restric
 // MODE 0: step update; MODE 1: initialise the state from the forward transform of eta.
 template <int N, int G, int MODE>
 __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_mart(float2* __restrict__ kbuf, float2* __restrict__ state, Geo g,
-                                                      MartK mk, const float2* __restrict__ twN, int f0) {
+                                                      MartK mk, const float2* __restrict__ twN, int f0,
+                                                      const MemberPtrs* __restrict__ mt) {
+  if (mt) {  // batched ensemble launch
+    kbuf = mt[blockIdx.z].kbuf;
+    state = mt[blockIdx.z].state;
+    mk = mt[blockIdx.z].mk;
+  }
   constexpr int RL = edge_radix<N>();
   extern __shared__ float2 smem[];
   float2* tw = smem;

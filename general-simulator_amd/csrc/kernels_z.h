@@ -319,7 +319,13 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zinv(const float2* __restrict
 template <int M>
 __global__ void __launch_bounds__(zthreads<M>()) k_zinv_sq(float2* __restrict__ kbuf, float* __restrict__ out, Geo g,
                                                           const float2* __restrict__ twM,
-                                                          const float2* __restrict__ rtn, uint32_t* adv) {
+                                                          const float2* __restrict__ rtn, uint32_t* adv,
+                                                          const MemberPtrs* __restrict__ mt) {
+  if (mt) {  // batched ensemble launch
+    kbuf = mt[blockIdx.z].kbuf;
+    out = mt[blockIdx.z].fld;
+    adv = mt[blockIdx.z].dstep;
+  }
   if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *adv += 1u;
   using S = ZShape<M>;
   constexpr int n = 2 * M;
@@ -367,7 +373,12 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zinv_sq(float2* __restrict__ 
 template <int M, int VT>
 __global__ void __launch_bounds__(zthreads<M>()) __attribute__((amdgpu_waves_per_eu(zmid_wpe<M, VT>())))
 k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, MartK mk, const float2* __restrict__ twM,
-            const float2* __restrict__ rtn) {
+            const float2* __restrict__ rtn, const MemberPtrs* __restrict__ mt) {
+  if (mt) {  // batched ensemble launch
+    kbuf = mt[blockIdx.z].kbuf;
+    eta = mt[blockIdx.z].fld;
+    mk = mt[blockIdx.z].mk;
+  }
   using S = ZShape<M>;
   using P = ZMidPlan<M>;
   constexpr int n = 2 * M, R0 = P::r0, NB = S::L * (M / R0);

@@ -356,7 +356,7 @@ def fft_compare(n: int, count: int = 1, reps: int = 10, device: int = 0) -> dict
             "max_rel_diff": out[4]}
 
 
-SCHED_SEQUENTIAL, SCHED_STREAMS, SCHED_GRAPH = 0, 1, 2
+SCHED_SEQUENTIAL, SCHED_STREAMS, SCHED_GRAPH, SCHED_BATCHED, SCHED_BATCHED_GRAPH = 0, 1, 2, 3, 4
 
 
 class Ensemble:

@@ -55,7 +55,8 @@ def alone(cfgs, steps):
 
 @pytest.mark.parametrize("mode,branches,spg", [(gsk.SCHED_SEQUENTIAL, 1, 1), (gsk.SCHED_STREAMS, 2, 1),
                                                (gsk.SCHED_GRAPH, 2, 3), (gsk.SCHED_GRAPH, 5, 1),
-                                               (gsk.SCHED_GRAPH, 1, 4)])
+                                               (gsk.SCHED_GRAPH, 1, 4), (gsk.SCHED_BATCHED, 2, 1),
+                                               (gsk.SCHED_BATCHED_GRAPH, 2, 3), (gsk.SCHED_BATCHED_GRAPH, 1, 4)])
 def test_ensemble_bit_exact_vs_alone(gpu, mode, branches, spg):
     say(f"ensemble mode={mode} branches={branches} steps/graph={spg}")
     cfgs = member_configs()
@@ -82,7 +83,8 @@ def test_ensemble_autotune_restores_state(gpu):
         before = [s.get_field("eta") for s in e.members]
         table = e.autotune(trial_steps=2)
         modes = {int(r[0]) for r in table}
-        assert modes == {gsk.SCHED_SEQUENTIAL, gsk.SCHED_STREAMS, gsk.SCHED_GRAPH}
+        assert modes == {gsk.SCHED_SEQUENTIAL, gsk.SCHED_STREAMS, gsk.SCHED_GRAPH, gsk.SCHED_BATCHED,
+                         gsk.SCHED_BATCHED_GRAPH}
         assert all(r[3] > 0 for r in table)
         best = min(table, key=lambda r: r[3])
         m, b, k = e.schedule()
