@@ -59,7 +59,7 @@ extern "C" {
 typedef struct gsk_sim gsk_sim;
 
 typedef struct gsk_config {
-  int n;                 /* grid edge (periodic n^3): 16..512, 2^a or 3*2^a               */
+  int n;                 /* grid edge (periodic n^3): 16, 32, 64, 96, 128, 192, 256 or 512  */
   int nvar;              /* number of order parameters eta_v                              */
   int physics;           /* GSK_PHYS_*                                                    */
   int path;              /* GSK_PATH_*                                                    */

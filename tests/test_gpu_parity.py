@@ -266,6 +266,24 @@ def test_prec_local_force_carry_bit_identical(gpu):
         assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
 
 
+@pytest.mark.parametrize("n", [96, 128, 192])
+def test_mart_fused_vs_plain_mixed_sizes(gpu, n):
+    """The fused step at the radix-3 sizes (96 = 3*32, 192 = 3*64) and at 128 against the FP64
+    plain path (pinned by the oracle), strong dynamics with noise, a few steps."""
+    say(f"mart fused vs plain n={n}")
+    kw = dict(A=-0.5, w_el=20.0, noise=0.002, seed=13)
+    out = []
+    for path in (gsk.PATH_FUSED32, gsk.PATH_PLAIN64):
+        s = mart_sim(n, 3, path=path, **kw)
+        s.init_uniform("eta", -0.05, 0.05, 6)
+        s.step(6)
+        out.append(s.get_field("eta").astype(np.float64))
+        s.close()
+    got, want = out
+    assert rel_l2(got, want) < 2e-6
+    assert float(np.abs(got - want).max()) < 2e-6 * float(np.abs(want).max())
+
+
 def test_mart_two_handles_bit_identical_at_512(gpu):
     """Two handles created one after the other give bit-identical trajectories (every buffer
     is ordered on the handle's own stream, including its zero fill at creation)."""
@@ -300,7 +318,7 @@ def prec_init(s, seed=11):
     s.init_uniform("eta", 0.0, 0.01, seed + 1)
 
 
-@pytest.mark.parametrize("n,steps", [(16, 50), (32, 200)])
+@pytest.mark.parametrize("n,steps", [(16, 50), (32, 200), (96, 20)])
 def test_prec_fused32_matches_oracle(gpu, n, steps):
     say(f"prec fused n={n} steps={steps}")
     s = prec_sim(n)
