@@ -20,6 +20,11 @@
 #ifndef GSK_XUPD_PF
 #define GSK_XUPD_PF 1
 #endif
+// GSK_XC_PFB: the contraction loads its B tensor before the forward transforms at the sizes
+// where that was measured to pay (64, 96, 192: -7 to -12 %; 128 and 512: slower)
+#ifndef GSK_XC_PFB
+#define GSK_XC_PFB 1
+#endif
 
 namespace gsk {
 
@@ -120,12 +125,24 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xcontract_r(float2* __restr
   }
   constexpr int RL = edge_radix<N>();
   constexpr int NPQ = V * (V + 1) / 2;
+  // B does not depend on the transforms: with at most 48 values per thread (last radix <= 8,
+  // V <= 3) it can be fetched before them, so its loads are in flight during the forward FFTs
+  constexpr bool PFB = GSK_XC_PFB && RL * NPQ <= 48 && (N == 64 || N == 96 || N == 192);
   extern __shared__ float2 smem[];
   float2* tw = smem;
   float2* tile = smem + N;
   const int W = g.W, T = N * W;
   const int t = blockIdx.x;
   const STile s = xtile(t, g);
+  const float* Bb = Bt + (long)t * NPQ * T;  // uniform base, lane offsets below
+  float bpre[PFB ? RL : 1][PFB ? NPQ : 1];
+  if (PFB && threadIdx.x < W * (N / RL)) {
+#pragma unroll
+    for (int r = 0; r < (PFB ? RL : 1); ++r)
+#pragma unroll
+      for (int pq = 0; pq < (PFB ? NPQ : 1); ++pq)
+        bpre[r][pq] = at32(Bb, (unsigned)(pq * T + threadIdx.x + r * (N / RL) * W));
+  }
   load_tw<N>(tw, twN);
   __syncthreads();
   Lines Ls{W, W, 1, 0, W};
@@ -137,13 +154,12 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xcontract_r(float2* __restr
   }
   const int b = threadIdx.x;
   if (b < W * (N / RL)) {
-    const float* Bb = Bt + (long)t * NPQ * T;  // uniform base, lane offsets below
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const int e = r * (N / RL) * W;
       float bv[NPQ];
 #pragma unroll
-      for (int pq = 0; pq < NPQ; ++pq) bv[pq] = at32(Bb, (unsigned)(pq * T + b + e));
+      for (int pq = 0; pq < NPQ; ++pq) bv[pq] = PFB ? bpre[PFB ? r : 0][PFB ? pq : 0] : at32(Bb, (unsigned)(pq * T + b + e));
       float2 R[V];
 #pragma unroll
       for (int p = 0; p < V; ++p) {
