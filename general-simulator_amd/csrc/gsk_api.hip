@@ -2246,8 +2246,9 @@ int gsk_adjoint_backward(gsk_sim* s, const float* dL_detaT, float* grad_eta0, do
     CHK(dev_alloc(w.kb, (size_t)(2 * V) * s->g.kfs, w.bytes_work));
     CHK(dev_alloc(w.k2, (size_t)V * s->g.kfs, w.bytes_work));
     CHK(dev_alloc(w.bh, (size_t)(V * (V + 1) / 2) * s->g.kfs, w.bytes_work));
-    CHK(dev_alloc(w.part, (size_t)nb * (2 + V * V), w.bytes_work));
-    w.hpart.resize((size_t)nb * (2 + V * V));
+    // FP64 partial sums of every reverse step of a segment, copied to the host once per segment
+    CHK(dev_alloc(w.part, (size_t)K * nb * (2 + V * V), w.bytes_work));
+    w.hpart.resize((size_t)K * nb * (2 + V * V));
     hipLaunchKernelGGL(k_btensor_half, dim3(grid_1d((long)s->n * s->n * (s->n / 2 + 1))), dim3(256), 0, s->stream,
                        w.bh, s->g, s->bargs);
     HIPCHK(hipGetLastError());
@@ -2269,20 +2270,26 @@ int gsk_adjoint_backward(gsk_sim* s, const float* dL_detaT, float* grad_eta0, do
         HIPCHK(hipMemcpyAsync(w.seg + (size_t)i * VN, s->fld, VN * sizeof(float), hipMemcpyDeviceToDevice, s->stream));
       }
     }
+    const size_t stride = (size_t)nb * (2 + V * V);
     for (int i = len - 1; i >= 0; --i) {
       const float* eta = K > 1 ? w.seg + (size_t)i * VN : w.ck_fld[j];
       const float* eta1 = K > 1 ? w.seg + (size_t)(i + 1) * VN : (j + 1 < nck ? w.ck_fld[j + 1] : w.fin_fld);
       const AdjK k = adj_consts(s, (uint32_t)(w.ck_step[j] + i));
-      CHK(dispatch_n<DAdj>(s->n, s, eta, eta1, w.lam, w.mu, w.rq, w.kb, w.k2, (const float*)w.bh, w.part, nb, k));
-      HIPCHK(hipMemcpyAsync(w.hpart.data(), w.part, w.hpart.size() * sizeof(double), hipMemcpyDeviceToHost,
-                            s->stream));
-      CHK(sync(s));
+      CHK(dispatch_n<DAdj>(s->n, s, eta, eta1, w.lam, w.mu, w.rq, w.kb, w.k2, (const float*)w.bh,
+                           w.part + (size_t)i * stride, nb, k));
+    }
+    HIPCHK(hipMemcpyAsync(w.hpart.data(), w.part, (size_t)len * stride * sizeof(double), hipMemcpyDeviceToHost,
+                          s->stream));
+    CHK(sync(s));
+    // the step's sums in the same order as the reverse sweep: gradients do not depend on K
+    for (int i = len - 1; i >= 0; --i) {
+      const double* hp = w.hpart.data() + (size_t)i * stride;
       double X = 0.0, muF = 0.0;
       std::vector<double> G((size_t)V * V, 0.0);
       for (int b = 0; b < nb; ++b) {
-        X += w.hpart[b];
-        for (int r = 0; r < V * V; ++r) G[r] += w.hpart[nb + (size_t)b * V * V + r];
-        muF += w.hpart[(size_t)nb + (size_t)nb * V * V + b];
+        X += hp[b];
+        for (int r = 0; r < V * V; ++r) G[r] += hp[nb + (size_t)b * V * V + r];
+        muF += hp[(size_t)nb + (size_t)nb * V * V + b];
       }
       X *= inv_n3;
       gL += -dt * kap * X - dt * muF;
