@@ -187,10 +187,11 @@ int gsk_adjoint_release(gsk_sim* sim);
 #define GSK_ENS_SEQUENTIAL 0 /* each member runs its steps alone, one after another */
 #define GSK_ENS_STREAMS 1    /* members spread over `branches` streams, launched eagerly */
 #define GSK_ENS_GRAPH 2      /* the branches captured as one hipGraph of steps_per_graph steps */
-#define GSK_ENS_BATCHED 3    /* members with one grid size and variant count (spectral martensite,
-                                1..4 variants) share every kernel launch (one launch per kernel and
-                                size class, grid.z = member); size classes run on parallel streams,
-                                other members as in GSK_ENS_STREAMS over `branches` streams */
+#define GSK_ENS_BATCHED 3    /* members with one grid size (edge <= 96) and variant count (spectral
+                                martensite, 1..4 variants) share every kernel launch (one launch per
+                                kernel and size class, grid.z = member); size classes run on parallel
+                                streams, the other members as in GSK_ENS_STREAMS over `branches`
+                                streams */
 #define GSK_ENS_BATCHED_GRAPH 4 /* the batched launches of steps_per_graph steps captured as one
                                    hipGraph and replayed */
 typedef struct gsk_ensemble gsk_ensemble;
