@@ -424,6 +424,37 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, Mar
     float2 v[R0];
     gchain<float2, M, false, 0, 2, 1>(tile, Ls, tw, no_ld, no_st, v, typename P::inv{}, tid);
     if (act) {
+      // Noise of the thread's voxel pairs. A Philox block covers 4 voxels, i.e. the pair of this
+      // thread and that of lane tid ^ L (j of opposite parity, same row). With ND each lane
+      // computes half of the blocks the two share and hands the partner the half it needs,
+      // so every block is computed once; bit-identical to normal2 for each pair.
+      constexpr bool ND = R0 % 2 == 0 && (M / R0) % 2 == 0 && S::L <= 32;
+      float xin[R0][2];
+      if constexpr (ND) {
+        if (mk.noise != 0.f) {
+          const uint32_t h = (uint32_t)(tid / S::L) & 1u;
+#pragma unroll
+          for (int q = 0; q < R0 / 2; ++q) {
+            const unsigned ro = h * (R0 / 2) + q;
+            uint32_t gq = ((unsigned)r0 * n + vloc + 2 * ro * (M / R0)) >> 2;
+            asm volatile("" : "+v"(gq));  // opaque per variant (see below)
+            uint32_t c[4] = {gq, (uint32_t)p, step, 0u};
+            philox10(c, (uint32_t)mk.seed, (uint32_t)(mk.seed >> 32));
+            const uint32_t a0 = h ? c[2] : c[0], a1 = h ? c[3] : c[1];
+            const uint32_t b0 = __shfl_xor(h ? c[0] : c[2], S::L), b1 = __shfl_xor(h ? c[1] : c[3], S::L);
+            float own[2], oth[2];
+            box_muller(a0, a1, own[0], own[1]);
+            box_muller(b0, b1, oth[0], oth[1]);
+            xin[q][0] = h ? oth[0] : own[0];
+            xin[q][1] = h ? oth[1] : own[1];
+            xin[R0 / 2 + q][0] = h ? own[0] : oth[0];
+            xin[R0 / 2 + q][1] = h ? own[1] : oth[1];
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < R0; ++r) xin[r][0] = xin[r][1] = 0.f;
+        }
+      }
 #pragma unroll
       for (int r = 0; r < R0; ++r) {
         const unsigned vo = vloc + 2 * r * (M / R0);
@@ -444,7 +475,10 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, Mar
           }
         }
         float xi[2] = {0.f, 0.f};
-        if (mk.noise != 0.f) {
+        if constexpr (ND) {
+          xi[0] = xin[r][0];
+          xi[1] = xin[r][1];
+        } else if (mk.noise != 0.f) {
           // opaque per variant: keeps the compiler from hoisting the variant-independent part
           // of the Philox rounds out of the variant loop (it would hold it in registers)
           uint32_t gq = vox >> 2;

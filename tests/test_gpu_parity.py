@@ -168,6 +168,24 @@ def test_mart_strong_dynamics_with_noise(gpu):
         np.testing.assert_allclose(E, Ew, rtol=max(tol * 10, 1e-10), atol=1e-9 * abs(Ew[3]))
 
 
+@pytest.mark.parametrize("n", [64, 96, 256, 512])
+def test_mart_noise_enters_exactly(gpu, n):
+    """With every other force zero (A = B = C = 0, no elasticity, kappa = 0), eta = 0 and
+    dt = L = 1, one step gives eta = -a xi with xi the Philox normals of (seed, step 0,
+    variant, voxel), which noise_field reproduces bit for bit. Checks that each voxel of the
+    middle z pass gets its own normal (its lanes share Philox blocks at 64, 256, 512)."""
+    say(f"mart noise placement n={n}")
+    V, seed = 3, 21
+    s = mart_sim(n, V, A=0.0, B=0.0, C=0.0, w_el=0.0, kappa=0.0, L=1.0, dt=1.0, noise=1.0, seed=seed)
+    s.set_field("eta", np.zeros((V, n, n, n), np.float32))
+    s.step(1)
+    got = s.get_field("eta")
+    s.close()
+    for v in range(V):
+        want = -gsk.noise_field(seed, 0, v, n**3).reshape(n, n, n)
+        assert float(np.abs(got[v] - want).max()) < 1e-5 * float(np.abs(want).max()), v
+
+
 def test_mart_linear_mode_at_256(gpu):
     """Full-size property: with B = C = 0 and no elasticity, a single mode decays by
     ((1 - dt L A)/(1 + dt L kappa k^2))^steps exactly."""
