@@ -22,11 +22,17 @@
 
 namespace gsk {
 
-// Compile-time shape of the z kernels. GSK_ZROWS_128, GSK_ZROWS_96: rows per tile at M = 128, 96;
+// Compile-time shape of the z kernels. GSK_ZROWS_{128,96,64,32}: rows per tile at M = 128, 96, 64, 32;
 // GSK_ZPLAN_128: radix plan of the 128-point transform (0: 16x8, 1: 8x4x4);
 // GSK_ZMID_WPE: waves-per-SIMD floor of the middle pass (0: the per-shape floor of zmid_wpe).
 #ifndef GSK_ZROWS_128
 #define GSK_ZROWS_128 8
+#endif
+#ifndef GSK_ZROWS_64
+#define GSK_ZROWS_64 16  // 8 rows: no change at 128^3
+#endif
+#ifndef GSK_ZROWS_32
+#define GSK_ZROWS_32 16  // 64^3 step 0.0535 -> 0.0518 ms against 32 rows
 #endif
 #ifndef GSK_ZROWS_96
 #define GSK_ZROWS_96 8  // 192^3 step 0.418 -> 0.406 ms against 16 rows
@@ -79,7 +85,10 @@ template <int M, int VT> constexpr int zmid_wpe() {
 
 // rows per z tile and the exact block size (also used by the host)
 __host__ __device__ constexpr int zrows(int M) {
-  return M <= 32 ? 32 : (M == 96 ? GSK_ZROWS_96 : (M < 128 ? 16 : (M == 128 ? GSK_ZROWS_128 : 8)));
+  return M == 32 ? GSK_ZROWS_32
+                 : (M < 32 ? 32
+                           : (M == 64 ? GSK_ZROWS_64
+                                      : (M == 96 ? GSK_ZROWS_96 : (M < 128 ? 16 : (M == 128 ? GSK_ZROWS_128 : 8)))));
 }
 __host__ __device__ constexpr int zthreads_of(int M, int rmin) {
   return zrows(M) * (M / rmin) < 64 ? 64 : (zrows(M) * (M / rmin) + 63) / 64 * 64;
