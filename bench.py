@@ -277,8 +277,8 @@ def main():
     ap.add_argument("--ens-seq-steps", type=int, default=20)
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported in this build "
-              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the library runs on one GPU "
+              "(how the step would shard across GPUs is a design note only)", file=sys.stderr)
         sys.exit(2)
 
     if a.workload == "ensemble":

@@ -1,7 +1,10 @@
-"""Accuracy report for the BASELINE configurations against the FP64 oracle.
+"""Accuracy report of the benchmark workloads against the FP64 oracle: the 64^3 martensite
+step (FP64 plain and FP32 fused, 1000 steps), the precipitate physics at 64^3 (2000 steps),
+the martensite step with noise (64^3 against the oracle, 512^3 run, hand FFT vs rocFFT) and
+the adjoint (32^3 against the oracle and finite differences, memory and time at 128^3).
 
 Not collected by pytest (long oracle runs). Usage on a GPU machine:
-    python tests/validate_configs.py [--only 0,1,2] [--out report.json]
+    python tests/validate_configs.py [--only mart64,prec64,noise,adjoint] [--out report.json]
 Prints one JSON object per configuration: max|d eta|, relative L2, energy error,
 and for the precipitate the FP64 drift of sum(c).
 """
@@ -98,7 +101,7 @@ def prec_case(n, steps, V=4):
 
 
 def mart_512_property(steps=500):
-    """Config 2 at full size: finite, bounded, energy non-increasing without noise is not
+    """The 512^3 martensite step with noise 0.002: finite, bounded; energy non-increasing is not
     expected (noise on), so report the state statistics and the mean-field drift."""
     cfg = gsk.default_config(gsk.PHYS_MART, n=512, nvar=3, noise=0.002, seed=2)
     with gsk.Sim(cfg) as s:
@@ -114,7 +117,7 @@ def mart_512_property(steps=500):
 
 
 def adjoint_case(n=32, steps=200, K=20, fd=True):
-    """Config 4 physics: 3 variants, noise off; loss = 1/2 |eta(T) - eta_target|^2 with
+    """Adjoint of the martensite step: 3 variants, noise off; loss = 1/2 |eta(T) - eta_target|^2 with
     eta_target from the forward model (initial field seed 7) at perturbed parameters
     (s = 1.05/0.95/1.0, L x1.1, kappa x0.9). GPU checkpointed adjoint vs the oracle's FP64
     store-all adjoint, and the oracle adjoint vs central differences (h = 1e-6)."""
@@ -173,7 +176,7 @@ def adjoint_case(n=32, steps=200, K=20, fd=True):
 
 
 def adjoint_memory_case(n=128, steps=200):
-    """Config 4 at full size: checkpointed (K = 20) vs store-all (K = 1) memory and time."""
+    """Adjoint at 128^3: checkpointed (K = 20) vs store-all (K = 1) memory and time."""
     V = 3
     C = gsk.cubic_modulus(0.7, 0.3, 0.2)
     cfg = gsk.default_config(gsk.PHYS_MART, n=n, nvar=V, w_el=20.0)
@@ -196,30 +199,30 @@ def adjoint_memory_case(n=128, steps=200):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--only", default="0,1,2,4")
+    ap.add_argument("--only", default="mart64,prec64,noise,adjoint")
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--out", default="")
     ap.add_argument("--threads", type=int, default=min(os.cpu_count() or 1, 16))
     a = ap.parse_args()
     O.set_threads(a.threads)
-    only = set(int(x) for x in a.only.split(","))
+    only = set(a.only.split(","))
     rep = {}
-    if 0 in only:
-        print("[validate] config 0: 64^3, 1000 steps, FP64 plain and FP32 fused", flush=True)
-        rep["config0_fp64_plain"] = mart_case(64, 1000, gsk.PATH_PLAIN64)
-        print(json.dumps(rep["config0_fp64_plain"]), flush=True)
-        rep["config0_fp32_fused"] = mart_case(64, 1000, gsk.PATH_FUSED32)
-        print(json.dumps(rep["config0_fp32_fused"]), flush=True)
-    if 1 in only:
-        print("[validate] config 1 physics at 64^3, 2000 steps", flush=True)
-        rep["config1_prec_64"] = prec_case(64, 2000)
-        print(json.dumps(rep["config1_prec_64"]), flush=True)
-    if 2 in only:
-        print("[validate] config 2 physics (noise 0.002) at 64^3 vs oracle, and 512^3 run", flush=True)
-        rep["config2_noise_64"] = mart_case(64, 500, gsk.PATH_FUSED32, noise=0.002, seed=2)
-        print(json.dumps(rep["config2_noise_64"]), flush=True)
-        rep["config2_512"] = mart_512_property()
-        print(json.dumps(rep["config2_512"]), flush=True)
+    if "mart64" in only:
+        print("[validate] martensite 64^3, 1000 steps, FP64 plain and FP32 fused", flush=True)
+        rep["mart64_fp64_plain"] = mart_case(64, 1000, gsk.PATH_PLAIN64)
+        print(json.dumps(rep["mart64_fp64_plain"]), flush=True)
+        rep["mart64_fp32_fused"] = mart_case(64, 1000, gsk.PATH_FUSED32)
+        print(json.dumps(rep["mart64_fp32_fused"]), flush=True)
+    if "prec64" in only:
+        print("[validate] precipitate physics at 64^3, 2000 steps", flush=True)
+        rep["prec64"] = prec_case(64, 2000)
+        print(json.dumps(rep["prec64"]), flush=True)
+    if "noise" in only:
+        print("[validate] martensite with noise 0.002: 64^3 vs oracle, and a 512^3 run", flush=True)
+        rep["mart64_noise"] = mart_case(64, 500, gsk.PATH_FUSED32, noise=0.002, seed=2)
+        print(json.dumps(rep["mart64_noise"]), flush=True)
+        rep["mart512_noise"] = mart_512_property()
+        print(json.dumps(rep["mart512_noise"]), flush=True)
         for n in (256, 512):
             r = gsk.fft_compare(n, count=1, reps=10)
             r3 = gsk.fft_compare(n, count=3, reps=5)
@@ -229,12 +232,12 @@ def main():
                 "hand_r2c_GBps_algorithmic": alg / (r["hand_r2c_ms"] / 1e3) / 1e9,
                 "rocfft_r2c_GBps_algorithmic": alg / (r["rocfft_r2c_ms"] / 1e3) / 1e9}
             print(json.dumps({f"fft_compare_{n}": rep[f"fft_compare_{n}"]}), flush=True)
-    if 4 in only:
-        print("[validate] config 4: adjoint at 32^3 vs oracle, FD; memory/time at 128^3", flush=True)
-        rep["config4_adjoint_32"] = adjoint_case(32, a.adj_steps)
-        print(json.dumps(rep["config4_adjoint_32"]), flush=True)
-        rep["config4_memory_128"] = adjoint_memory_case(128, 200)
-        print(json.dumps(rep["config4_memory_128"]), flush=True)
+    if "adjoint" in only:
+        print("[validate] adjoint at 32^3 vs oracle and finite differences; memory/time at 128^3", flush=True)
+        rep["adjoint32"] = adjoint_case(32, a.adj_steps)
+        print(json.dumps(rep["adjoint32"]), flush=True)
+        rep["adjoint128_memory"] = adjoint_memory_case(128, 200)
+        print(json.dumps(rep["adjoint128_memory"]), flush=True)
     if a.out:
         with open(a.out, "w") as f:
             json.dump(rep, f, indent=1)
