@@ -73,6 +73,26 @@ def test_ensemble_bit_exact_vs_alone(gpu, mode, branches, spg):
             assert np.array_equal(s.get_field("eta").view(np.uint32), want[i].view(np.uint32)), f"member {i}"
 
 
+@pytest.mark.parametrize("mode", [gsk.SCHED_BATCHED, gsk.SCHED_BATCHED_GRAPH])
+def test_ensemble_batched_with_large_members(gpu, mode):
+    """Batched schedules share launches only among small grids; members above 96^3 run on the
+    branch streams beside the batched classes. Both kinds must match members stepped alone."""
+    say(f"ensemble batched mode={mode} with 128^3 members")
+    spec = [(64, 0.2, 1.0, 0.01), (128, 0.18, 0.8, 0.006), (64, 0.22, 1.2, 0.008), (128, 0.25, 1.3, 0.007)]
+    cfgs = [gsk.default_config(gsk.PHYS_MART, n=n, nvar=V, A=A, kappa=k, dt=dt, noise=0.003, seed=200 + i, w_el=5.0)
+            for i, (n, A, k, dt) in enumerate(spec)]
+    steps = 5
+    want = alone(cfgs, steps)
+    with gsk.Ensemble(cfgs) as e:
+        e.build_btensor(C, gsk.mart_eigenstrains(V))
+        init_members(e.members)
+        e.set_schedule(mode, 2, 2)
+        e.step(steps)
+        for i, s in enumerate(e.members):
+            assert s.step_index == steps
+            assert np.array_equal(s.get_field("eta").view(np.uint32), want[i].view(np.uint32)), f"member {i}"
+
+
 def test_ensemble_autotune_restores_state(gpu):
     say("ensemble autotune")
     cfgs = member_configs()
