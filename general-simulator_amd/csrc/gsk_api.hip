@@ -1456,21 +1456,26 @@ int gsk_time_steps(gsk_sim* s, int nsteps, double* ms) {
   if (!s || !ms || nsteps <= 0) return fail(GSK_ERR_ARG, "bad argument");
   CHK(ensure_device(s->dev));
   CHK(prepare(s));
-  hipEvent_t a, b;
+  hipEvent_t a = nullptr, b = nullptr;
+  struct Events {  // destroyed on every return path
+    hipEvent_t* a;
+    hipEvent_t* b;
+    ~Events() {
+      if (*a) (void)hipEventDestroy(*a);
+      if (*b) (void)hipEventDestroy(*b);
+    }
+  } guard{&a, &b};
   HIPCHK(hipEventCreate(&a));
   HIPCHK(hipEventCreate(&b));
   HIPCHK(hipEventRecord(a, s->stream));
   for (int i = 0; i < nsteps; ++i) {
-    int rc = dispatch_n<DStep>(s->n, s);
-    if (rc != GSK_OK) return rc;
+    CHK(dispatch_n<DStep>(s->n, s));
     ++s->step;
   }
   HIPCHK(hipEventRecord(b, s->stream));
   HIPCHK(hipEventSynchronize(b));
   float t = 0.f;
   HIPCHK(hipEventElapsedTime(&t, a, b));
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
   *ms = (double)t;
   return GSK_OK;
 }
