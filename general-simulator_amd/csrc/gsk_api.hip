@@ -360,6 +360,9 @@ struct Impl {
   // ---- fused building blocks
   // LDS of a z kernel holding nf field tiles
   static size_t zlds(const gsk_sim* s, int nf) { return (2 * M + (size_t)nf * s->g.L * zpitch(M)) * sizeof(float2); }
+  // shape of the middle martensite pass (its own rows per tile)
+  static constexpr int LZ = zmid_rows(M), ZMT = zmid_threads<M>();
+  static constexpr size_t ZM_LDS = (2 * M + (size_t)LZ * zpitch_l(M, LZ)) * sizeof(float2);
   static int zfwd(gsk_sim* s, int op, const float* in, int F) {
     const int rows = s->n * s->n / s->g.L;
     if (op == Z_COPY) {
@@ -474,14 +477,14 @@ struct Impl {
     }
     CHK(ypass(s, false, F));
     {
-      const int grid = s->n * s->n / s->g.L;
-      const size_t lds = zlds(s, 1);
+      const int grid = s->n * s->n / LZ;
+      const size_t lds = ZM_LDS;
       switch (F) {  // eta prefetched into registers for up to 4 variants
-        case 1: LAUNCH((k_zmid_mart<M, 1>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
-        case 2: LAUNCH((k_zmid_mart<M, 2>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
-        case 3: LAUNCH((k_zmid_mart<M, 3>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
-        case 4: LAUNCH((k_zmid_mart<M, 4>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
-        default: LAUNCH((k_zmid_mart<M, 0>), grid, s->zthreads, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr);
+        case 1: LAUNCH((k_zmid_mart<M, 1>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+        case 2: LAUNCH((k_zmid_mart<M, 2>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+        case 3: LAUNCH((k_zmid_mart<M, 3>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+        case 4: LAUNCH((k_zmid_mart<M, 4>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+        default: LAUNCH((k_zmid_mart<M, 0>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr);
       }
     }
     CHK(ypass(s, true, F));
@@ -531,7 +534,7 @@ struct Impl {
     switch (F) {
 #define GSK_ZM(V)                                                                                                  \
   case V:                                                                                                          \
-    hipLaunchKernelGGL((k_zmid_mart<M, V>), dim3(rows, 1, cnt), dim3(s0->zthreads), lz, st, nk,                   \
+    hipLaunchKernelGGL((k_zmid_mart<M, V>), dim3(s0->n * s0->n / LZ, 1, cnt), dim3(ZMT), ZM_LDS, st, nk,       \
                        (const float*)nullptr, s0->g, mk, s0->twM, s0->rtn, tab);                                   \
     break;
       GSK_ZM(1) GSK_ZM(2) GSK_ZM(3) GSK_ZM(4)

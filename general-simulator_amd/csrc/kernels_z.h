@@ -1,6 +1,7 @@
 // Contiguous-axis (z) passes of the FP32 fused step: real rows <-> half-spectrum rows.
 //
-// A z tile is L = zrows(M) consecutive rows (x*n + y) of one field. The complex
+// A z tile is L = zrows(M) consecutive rows (x*n + y) of one field (zmid_rows(M) in the
+// middle martensite pass). The complex
 // M-point FFT (M = n/2) of the packed real row runs in LDS (row pitch zpitch(M),
 // compile-time tile shape and pass strides). The R2C / C2R split between the
 // packed M-point spectrum and the n/2+1 half-spectrum values is fused with the
@@ -39,6 +40,18 @@ namespace gsk {
 #endif
 #ifndef GSK_ZPLAN_128
 #define GSK_ZPLAN_128 1
+#endif
+// Rows per tile of the middle martensite pass at M = 256, 128, 64 (the other sizes use zrows):
+// 512^3 middle pass 1.26 ms at 8 rows, 1.10 at 4, 1.09 at 2; 256^3 141, 137, 135 us;
+// 128^3 step 0.1213 -> 0.1202 ms at 8 rows against 16; 4 rows at 192^3 and 8 at 64^3: no gain.
+#ifndef GSK_ZMID_ROWS_256
+#define GSK_ZMID_ROWS_256 2
+#endif
+#ifndef GSK_ZMID_ROWS_128
+#define GSK_ZMID_ROWS_128 2
+#endif
+#ifndef GSK_ZMID_ROWS_64
+#define GSK_ZMID_ROWS_64 8
 #endif
 #ifndef GSK_ZMID_WPE
 #define GSK_ZMID_WPE 0
@@ -90,27 +103,34 @@ __host__ __device__ constexpr int zrows(int M) {
                            : (M == 64 ? GSK_ZROWS_64
                                       : (M == 96 ? GSK_ZROWS_96 : (M < 128 ? 16 : (M == 128 ? GSK_ZROWS_128 : 8)))));
 }
-__host__ __device__ constexpr int zthreads_of(int M, int rmin) {
-  return zrows(M) * (M / rmin) < 64 ? 64 : (zrows(M) * (M / rmin) + 63) / 64 * 64;
+__host__ __device__ constexpr int zmid_rows(int M) {
+  return M == 256 ? GSK_ZMID_ROWS_256 : (M == 128 ? GSK_ZMID_ROWS_128 : (M == 64 ? GSK_ZMID_ROWS_64 : zrows(M)));
 }
+// threads of a tile of L rows: one per butterfly of the smallest radix
+__host__ __device__ constexpr int zthreads_lr(int L, int M, int rmin) {
+  return L * (M / rmin) < 64 ? 64 : (L * (M / rmin) + 63) / 64 * 64;
+}
+__host__ __device__ constexpr int zthreads_of(int M, int rmin) { return zthreads_lr(zrows(M), M, rmin); }
 template <int M> constexpr int zthreads() { return zthreads_of(M, ZPlanOf<M>::rmin); }
+template <int M> constexpr int zmid_threads() { return zthreads_lr(zmid_rows(M), M, ZPlanOf<M>::rmin); }
 
-// Row pitch of the z tile in LDS (complex elements): the smallest P >= M + 1 with
+// Row pitch of a z tile of L rows in LDS (complex elements): the smallest P >= M + 1 with
 // P = 16/L (mod 16) for L <= 16 rows per tile, P odd for L = 32. An FFT pass maps lanes
 // line fastest, and the LDS serves an 8-byte access 16 lanes at a time from 16 bank pairs:
 // those 16 lanes (L lines x 16/L consecutive positions) then hit 16 distinct pairs.
 // (Measured at n = 256, middle pass: P = 130 135 us, 129 141 us, 132 147 us, 136 173 us.)
 // GSK_ZPITCH_R overrides the residue mod 16.
-__host__ __device__ constexpr int zpitch(int M) {
+__host__ __device__ constexpr int zpitch_l(int M, int L) {
 #ifdef GSK_ZPITCH_R
   const int r = GSK_ZPITCH_R;
 #else
-  const int r = zrows(M) >= 16 ? 1 : 16 / zrows(M);
+  const int r = L >= 16 ? 1 : 16 / L;
 #endif
   int p = M + 1;
   while (p % 16 != r % 16) ++p;
   return p;
 }
+__host__ __device__ constexpr int zpitch(int M) { return zpitch_l(M, zrows(M)); }
 
 enum ZOp { Z_COPY = 0, Z_SQUARE = 1, Z_PREC_LOCAL = 2 };
 
@@ -141,19 +161,20 @@ template <class T> __device__ __forceinline__ T& at32(T* base, unsigned elem) {
 }
 
 // --------------------------------------------------------------------- one-field tile I/O
-// TM: work-group size as a multiple of the FFT's thread count (more threads, fewer tasks each)
-template <int M, int TM = 1>
+// TM: work-group size as a multiple of the FFT's thread count (more threads, fewer tasks each);
+// LR: rows per tile, P: their LDS row pitch
+template <int M, int TM = 1, int LR = zrows(M)>
 struct ZShape {
-  static constexpr int L = zrows(M), T = zthreads<M>() * TM, H2 = M / 2;
+  static constexpr int L = LR, T = zthreads_lr(LR, M, ZPlanOf<M>::rmin) * TM, H2 = M / 2, P = zpitch_l(M, LR);
   static constexpr int NT = (L * H2 + T - 1) / T;  // tasks per thread
   static constexpr bool FULL = (L * H2) % T == 0;
   __device__ static bool valid(int e) { return FULL || e < L * H2; }
 };
 
 // The half-spectrum values of one tile that a thread owns, held in registers.
-template <int M, int TM = 1>
+template <int M, int TM = 1, int LR = zrows(M)>
 struct ZSpec {
-  using S = ZShape<M, TM>;
+  using S = ZShape<M, TM, LR>;
   float2 a[S::NT], b[S::NT], c[S::NT];
   // Rows r0.. of one field: a uniform field base and 32-bit element offsets (a field holds
   // fewer than 2^29 elements), so the loads use a scalar base and one offset register each.
@@ -170,14 +191,14 @@ struct ZSpec {
       }
     }
   }
-  // C2R split into the packed spectra of the LDS tile (row pitch zpitch(M))
+  // C2R split into the packed spectra of the LDS tile (row pitch S::P)
   __device__ __forceinline__ void split(float2* tile, const float2* rt, int tid = threadIdx.x) const {
 #pragma unroll
     for (int i = 0; i < S::NT; ++i) {
       const int e = tid + i * S::T;
       if (S::valid(e)) {
         const int l = e / S::H2, k = e % S::H2;
-        float2* z = tile + l * zpitch(M);
+        float2* z = tile + l * S::P;
         if (k == 0) {
           z[0] = make_float2(a[i].x + b[i].x, a[i].x - b[i].x);
           float2 zh, unused;
@@ -196,17 +217,17 @@ struct ZSpec {
 
 // packed spectra of the LDS tile -> half-spectrum rows r0.. of one field (R2C direction);
 // offsets as in ZSpec::load
-template <int M, int TM = 1>
+template <int M, int TM = 1, int LR = zrows(M)>
 __device__ __forceinline__ void z_store_spec(const float2* tile, float2* __restrict__ field, const Geo& g, long r0,
                                              const float2* rt, int tid = threadIdx.x) {
-  using S = ZShape<M, TM>;
+  using S = ZShape<M, TM, LR>;
   const unsigned rb = (unsigned)r0 * M, nb = (unsigned)(g.nq + r0);
 #pragma unroll
   for (int i = 0; i < S::NT; ++i) {
     const unsigned e = tid + i * S::T;
     if (S::valid(e)) {
       const unsigned l = e / S::H2, k = e % S::H2, ro = rb + l * M;
-      const float2* z = tile + l * zpitch(M);
+      const float2* z = tile + l * S::P;
       float2 xk, xm;
       if (k == 0) {
         const float2 z0 = z[0], zh = z[S::H2];
@@ -385,7 +406,7 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zinv_sq(float2* __restrict__ 
 // is a compile-time constant and every eta_q the thread needs is prefetched into registers;
 // VT = 0: any variant count, eta read when needed.
 template <int M, int VT>
-__global__ void __launch_bounds__(zthreads<M>()) __attribute__((amdgpu_waves_per_eu(zmid_wpe<M, VT>())))
+__global__ void __launch_bounds__(zmid_threads<M>()) __attribute__((amdgpu_waves_per_eu(zmid_wpe<M, VT>())))
 k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, MartK mk, const float2* __restrict__ twM,
             const float2* __restrict__ rtn, const MemberPtrs* __restrict__ mt) {
   if (mt) {  // batched ensemble launch
@@ -393,7 +414,8 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, Mar
     eta = mt[blockIdx.z].fld;
     mk = mt[blockIdx.z].mk;
   }
-  using S = ZShape<M>;
+  constexpr int LZ = zmid_rows(M);
+  using S = ZShape<M, 1, LZ>;
   using P = ZMidPlan<M>;
   constexpr int n = 2 * M, R0 = P::r0, NB = S::L * (M / R0);
   constexpr int VR = VT > 0 ? VT : 1;
@@ -419,10 +441,10 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, Mar
           pre[r][f] = at32(reinterpret_cast<const float2*>(erow + f * g.nvox), vloc / 2 + r * (M / R0));
     }
   }
-  ZSpec<M> sp;
+  ZSpec<M, 1, LZ> sp;
   sp.load(kbuf, g, r0);
   z_twiddles<M>(tw, rt, twM, rtn);
-  const Lines Ls{S::L, S::L, zpitch(M), 0, 1};
+  const Lines Ls{S::L, S::L, S::P, 0, 1};
   auto no_ld = [](int, int) { return float2{}; };
   auto no_st = [](int, int, float2) {};
   const uint32_t step = step_of(mk.step, mk.stepp);
@@ -506,7 +528,7 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, Mar
       }
     }
     gchain<float2, M, true, 2, 0, 1>(tile, Ls, tw, no_ld, no_st, v, typename P::fwd{}, tid);
-    z_store_spec<M>(tile, kbuf + p * g.kfs, g, r0, rt, tid);
+    z_store_spec<M, 1, LZ>(tile, kbuf + p * g.kfs, g, r0, rt, tid);
     __syncthreads();  // the next variant's split overwrites the tile
   }
 }
