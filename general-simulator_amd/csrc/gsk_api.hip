@@ -16,6 +16,7 @@
 #include "kernels_aux.h"
 #include "kernels_adjoint.h"
 #include "kernels_ref.h"
+#include "kernels_plane.h"
 
 using namespace gsk;
 
@@ -192,6 +193,18 @@ Geo geo_next(gsk_sim* s) {
   return g;
 }
 
+// The spectral martensite step's y inverse -> middle z pass -> y forward as one plane-resident
+// launch (k_plane_mid: n = 32, 64, 128 and 1..4 variants); GSK_PLANE=0 in the environment keeps
+// the three kernels.
+bool plane_mid_on(const gsk_sim* s) {
+  static const bool off = [] {
+    const char* e = std::getenv("GSK_PLANE");
+    return e && e[0] == '0';
+  }();
+  return !off && (s->n == 32 || s->n == 64 || s->n == 128) && s->F >= 1 && s->F <= 4 &&
+         s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL;
+}
+
 size_t nvox(const gsk_sim* s) { return (size_t)s->n * s->n * s->n; }
 size_t hs_count(const gsk_sim* s) { return (size_t)s->n * s->n * (s->n / 2 + 1); }
 // State element of field 0 at k = 0: first main x tile (after the n/W Nyquist tiles), kx = 0, column 0.
@@ -354,7 +367,35 @@ struct Impl {
     CHK(lds_limit((const void*)p_zrows<N, false>));
     CHK(lds_limit((const void*)p_strided<N, true>));
     CHK(lds_limit((const void*)p_strided<N, false>));
+    if constexpr (plane_mid_size<N>()) {
+      CHK(lds_limit((const void*)k_plane_mid<N, 1>));
+      CHK(lds_limit((const void*)k_plane_mid<N, 2>));
+      CHK(lds_limit((const void*)k_plane_mid<N, 3>));
+      CHK(lds_limit((const void*)k_plane_mid<N, 4>));
+    }
     return GSK_OK;
+  }
+
+  // y inverse of R, middle z pass and y forward of F in one launch, one x-plane of one variant
+  // per work group (plane_mid_on)
+  static int plane_mid(gsk_sim* s, const MartK& mk) {
+    if constexpr (plane_mid_size<N>()) {
+      const dim3 grid(N, s->F);
+      const Geo g = geo_next(s);
+      constexpr int T = plane_threads<N>();
+      constexpr size_t lds = plane_lds<N>();
+      switch (s->F) {
+        case 1: LAUNCH((k_plane_mid<N, 1>), grid, T, lds, s, s->kbuf, s->fld, g, mk, s->twN, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+        case 2: LAUNCH((k_plane_mid<N, 2>), grid, T, lds, s, s->kbuf, s->fld, g, mk, s->twN, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+        case 3: LAUNCH((k_plane_mid<N, 3>), grid, T, lds, s, s->kbuf, s->fld, g, mk, s->twN, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+        case 4: LAUNCH((k_plane_mid<N, 4>), grid, T, lds, s, s->kbuf, s->fld, g, mk, s->twN, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+        default: return fail(GSK_ERR_UNSUPPORTED, "the plane pass takes 1..4 variants");
+      }
+      return GSK_OK;
+    } else {
+      (void)s; (void)mk;
+      return fail(GSK_ERR_UNSUPPORTED, "the plane pass takes n = 32, 64, 128");
+    }
   }
 
   // ---- fused building blocks
@@ -475,19 +516,23 @@ struct Impl {
       case 4: LAUNCH((k_xcontract_r<N, 4>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN, (const MemberPtrs*)nullptr); break;
       default: LAUNCH((k_xcontract<N>), gx, s->xthreads, s->lds_xc, s, s->kbuf, s->Bt, s->g, mk, s->twN);
     }
-    CHK(ypass(s, false, F));
-    {
-      const int grid = s->n * s->n / LZ;
-      const size_t lds = ZM_LDS;
-      switch (F) {  // eta prefetched into registers for up to 4 variants
-        case 1: LAUNCH((k_zmid_mart<M, 1>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
-        case 2: LAUNCH((k_zmid_mart<M, 2>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
-        case 3: LAUNCH((k_zmid_mart<M, 3>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
-        case 4: LAUNCH((k_zmid_mart<M, 4>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
-        default: LAUNCH((k_zmid_mart<M, 0>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr);
+    if (plane_mid_on(s)) {
+      CHK(plane_mid(s, mk));
+    } else {
+      CHK(ypass(s, false, F));
+      {
+        const int grid = s->n * s->n / LZ;
+        const size_t lds = ZM_LDS;
+        switch (F) {  // eta prefetched into registers for up to 4 variants
+          case 1: LAUNCH((k_zmid_mart<M, 1>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+          case 2: LAUNCH((k_zmid_mart<M, 2>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+          case 3: LAUNCH((k_zmid_mart<M, 3>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+          case 4: LAUNCH((k_zmid_mart<M, 4>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr); break;
+          default: LAUNCH((k_zmid_mart<M, 0>), grid, ZMT, lds, s, s->kbuf, s->fld, s->g, mk, s->twM, s->rtn, (const MemberPtrs*)nullptr);
+        }
       }
+      CHK(ypass(s, true, F));
     }
-    CHK(ypass(s, true, F));
     CHK(xupdate_mart<0>(s, mk));
     CHK(ypass(s, false, F));
     {  // eta back to real space, fused with the next step's forward pass of eta^2
@@ -530,19 +575,35 @@ struct Impl {
       default: return fail(GSK_ERR_UNSUPPORTED, "batched ensemble launches take 1..4 variants");
     }
     HIPCHK(hipGetLastError());
-    CHK(y(false));
-    switch (F) {
+    if (plane_mid_on(s0)) {  // the members' own step uses the plane pass: keep them bit-identical
+      if constexpr (plane_mid_size<N>()) {
+        switch (F) {
+#define GSK_PM(V)                                                                                                  \
+  case V:                                                                                                          \
+    hipLaunchKernelGGL((k_plane_mid<N, V>), dim3(N, F, cnt), dim3(plane_threads<N>()), plane_lds<N>(), st, nk,    \
+                       (const float*)nullptr, geo_next(s0), mk, s0->twN, s0->twM, s0->rtn, tab);                   \
+    break;
+          GSK_PM(1) GSK_PM(2) GSK_PM(3) GSK_PM(4)
+#undef GSK_PM
+          default: return fail(GSK_ERR_UNSUPPORTED, "batched ensemble launches take 1..4 variants");
+        }
+        HIPCHK(hipGetLastError());
+      }
+    } else {
+      CHK(y(false));
+      switch (F) {
 #define GSK_ZM(V)                                                                                                  \
   case V:                                                                                                          \
-    hipLaunchKernelGGL((k_zmid_mart<M, V>), dim3(s0->n * s0->n / LZ, 1, cnt), dim3(ZMT), ZM_LDS, st, nk,       \
+    hipLaunchKernelGGL((k_zmid_mart<M, V>), dim3(s0->n * s0->n / LZ, 1, cnt), dim3(ZMT), ZM_LDS, st, nk,         \
                        (const float*)nullptr, s0->g, mk, s0->twM, s0->rtn, tab);                                   \
     break;
-      GSK_ZM(1) GSK_ZM(2) GSK_ZM(3) GSK_ZM(4)
+        GSK_ZM(1) GSK_ZM(2) GSK_ZM(3) GSK_ZM(4)
 #undef GSK_ZM
-      default: return fail(GSK_ERR_UNSUPPORTED, "batched ensemble launches take 1..4 variants");
+        default: return fail(GSK_ERR_UNSUPPORTED, "batched ensemble launches take 1..4 variants");
+      }
+      HIPCHK(hipGetLastError());
+      CHK(y(true));
     }
-    HIPCHK(hipGetLastError());
-    CHK(y(true));
     hipLaunchKernelGGL((k_xupdate_mart<N, 1, 0>), dim3(gx, F, cnt), dim3(s0->xthreads), s0->lds_y, st, nk,
                        (float2*)nullptr, geo_next(s0), mk, s0->twN, 0, tab);
     HIPCHK(hipGetLastError());
@@ -889,9 +950,13 @@ void build_slots(gsk_sim* s) {
     const int npq = F * (F + 1) / 2;
     s->slots.push_back({"y_fwd_S", F * 2 * hs});
     s->slots.push_back({"x_contract", F * 2 * hs + npq * hs / 2});
-    s->slots.push_back({"y_inv_R", F * 2 * hs});
-    s->slots.push_back({"zmid_force", F * (2 * hs + rv)});
-    s->slots.push_back({"y_fwd_F", F * 2 * hs});
+    if (plane_mid_on(s)) {  // R and F spectra, eta of every variant per variant
+      s->slots.push_back({"plane_mid", F * (2 * hs + F * rv)});
+    } else {
+      s->slots.push_back({"y_inv_R", F * 2 * hs});
+      s->slots.push_back({"zmid_force", F * (2 * hs + rv)});
+      s->slots.push_back({"y_fwd_F", F * 2 * hs});
+    }
     static const char* const xname[] = {"x_update_g0", "x_update_g1", "x_update_g2", "x_update_g3",
                                         "x_update_g4", "x_update_g5", "x_update_g6", "x_update_g7"};
     for (int f0 = 0, gi = 0; f0 < F; ++gi) {  // one launch per field group, as Impl::xupdate_mart
