@@ -193,10 +193,12 @@ Geo geo_next(gsk_sim* s) {
   return g;
 }
 
-// The spectral martensite step's y inverse -> middle z pass -> y forward as one plane-resident
-// launch (k_plane_mid: n = 32, 64, 128 and 1..4 variants); GSK_PLANE=0 in the environment keeps
-// the three kernels.
-bool plane_mid_on(const gsk_sim* s) {
+// Plane-resident passes of the spectral martensite step (n = 32, 64, 128 and 1..4 variants):
+// k_plane_mid replaces y inverse -> middle z pass -> y forward, k_plane_sq the y inverse -> last
+// z pass -> next step's y forward, so the k-space buffer carries the y,z-transformed eta^2
+// between steps (z-transformed only at the other sizes). GSK_PLANE=0 in the environment keeps
+// the chain of y and z kernels.
+bool plane_on(const gsk_sim* s) {
   static const bool off = [] {
     const char* e = std::getenv("GSK_PLANE");
     return e && e[0] == '0';
@@ -367,7 +369,9 @@ struct Impl {
     CHK(lds_limit((const void*)p_zrows<N, false>));
     CHK(lds_limit((const void*)p_strided<N, true>));
     CHK(lds_limit((const void*)p_strided<N, false>));
-    if constexpr (plane_mid_size<N>()) {
+    if constexpr (plane_size<N>()) {
+      CHK(lds_limit((const void*)k_plane_sq<N>));
+      CHK(lds_limit((const void*)k_plane_sq0<N>));
       CHK(lds_limit((const void*)k_plane_mid<N, 1>));
       CHK(lds_limit((const void*)k_plane_mid<N, 2>));
       CHK(lds_limit((const void*)k_plane_mid<N, 3>));
@@ -376,10 +380,23 @@ struct Impl {
     return GSK_OK;
   }
 
+  // y inverse of eta, last z pass (eta written, squared) and y forward in one launch, one x-plane
+  // of one field per work group (plane_on); adv: the step counter the launch advances
+  static int plane_sq(gsk_sim* s, uint32_t* adv) {
+    if constexpr (plane_size<N>()) {
+      LAUNCH((k_plane_sq<N>), dim3(N, s->F), plane_threads<N>(), plane_lds<N>(), s, s->kbuf, s->fld, geo_next(s), s->twN,
+             s->twM, s->rtn, adv, (const MemberPtrs*)nullptr);
+      return GSK_OK;
+    } else {
+      (void)s; (void)adv;
+      return fail(GSK_ERR_UNSUPPORTED, "the plane pass takes n = 32, 64, 128");
+    }
+  }
+
   // y inverse of R, middle z pass and y forward of F in one launch, one x-plane of one variant
-  // per work group (plane_mid_on)
+  // per work group (plane_on)
   static int plane_mid(gsk_sim* s, const MartK& mk) {
-    if constexpr (plane_mid_size<N>()) {
+    if constexpr (plane_size<N>()) {
       const dim3 grid(N, s->F);
       const Geo g = geo_next(s);
       constexpr int T = plane_threads<N>();
@@ -495,10 +512,18 @@ struct Impl {
     return GSK_OK;
   }
 
-  // z-transformed eta^2 of the current fields into the k-space buffer (the first kernel of a
-  // step, unless the previous step's fused last pass has left it there)
+  // z-transformed eta^2 of the current fields into the k-space buffer (y,z-transformed where
+  // plane_on): the first kernel of a step, unless the previous step's fused last pass has left
+  // it there
   static int square_fwd(gsk_sim* s) {
-    CHK(zfwd(s, s->cfg.physics == GSK_PHYS_MART ? Z_SQUARE : Z_PREC_LOCAL, s->fld, s->F));
+    if (plane_on(s)) {
+      if constexpr (plane_size<N>()) {
+        LAUNCH((k_plane_sq0<N>), dim3(N, s->F), plane_threads<N>(), plane_lds<N>(), s, s->fld, s->kbuf, geo_next(s),
+               s->twN, s->twM, s->rtn);
+      }
+    } else {
+      CHK(zfwd(s, s->cfg.physics == GSK_PHYS_MART ? Z_SQUARE : Z_PREC_LOCAL, s->fld, s->F));
+    }
     s->sq_ready = true;
     return GSK_OK;
   }
@@ -507,8 +532,9 @@ struct Impl {
     const int gx = s->g.ntile;
     const int F = s->F;
     MartK mk = mart_consts(s);
+    const bool pl = plane_on(s);
     if (!s->sq_ready) CHK(square_fwd(s));
-    CHK(ypass(s, true, F));
+    if (!pl) CHK(ypass(s, true, F));
     switch (F) {  // register-resident contraction for up to 4 variants
       case 1: LAUNCH((k_xcontract_r<N, 1>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN, (const MemberPtrs*)nullptr); break;
       case 2: LAUNCH((k_xcontract_r<N, 2>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN, (const MemberPtrs*)nullptr); break;
@@ -516,7 +542,7 @@ struct Impl {
       case 4: LAUNCH((k_xcontract_r<N, 4>), gx, s->xthreads, s->lds_y, s, s->kbuf, s->Bt, s->g, mk, s->twN, (const MemberPtrs*)nullptr); break;
       default: LAUNCH((k_xcontract<N>), gx, s->xthreads, s->lds_xc, s, s->kbuf, s->Bt, s->g, mk, s->twN);
     }
-    if (plane_mid_on(s)) {
+    if (pl) {
       CHK(plane_mid(s, mk));
     } else {
       CHK(ypass(s, false, F));
@@ -534,8 +560,10 @@ struct Impl {
       CHK(ypass(s, true, F));
     }
     CHK(xupdate_mart<0>(s, mk));
-    CHK(ypass(s, false, F));
-    {  // eta back to real space, fused with the next step's forward pass of eta^2
+    if (pl) {  // eta back to real space, fused with the next step's forward passes of eta^2
+      CHK(plane_sq(s, s->dstep));
+    } else {  // eta back to real space, fused with the next step's forward z pass of eta^2
+      CHK(ypass(s, false, F));
       const int rows = s->n * s->n / s->g.L;
       LAUNCH((k_zinv_sq<M>), dim3(rows, F), s->zthreads, zlds(s, 1), s, s->kbuf, s->fld, geo_next(s), s->twM, s->rtn,
              s->dstep, (const MemberPtrs*)nullptr);
@@ -547,7 +575,8 @@ struct Impl {
   // One spectral martensite step of `cnt` members of this grid size in every launch
   // (grid.z = member). The members' buffers and constants come from the device table `tab`;
   // s0, one of them, supplies the geometry, launch shapes and twiddle tables, which members
-  // of one size share. Every member must hold its z-transformed eta^2 (sq_ready).
+  // of one size share. Every member must hold its transformed eta^2 (sq_ready; y,z-transformed
+  // where plane_on, which holds for all members of one size alike).
   static int step_mart_batched(gsk_sim* s0, int cnt, const MemberPtrs* tab, hipStream_t st) {
     const int F = s0->F, gx = s0->g.ntile, rows = s0->n * s0->n / s0->g.L;
     const MartK mk{};  // replaced by the table entry in every work group
@@ -563,7 +592,8 @@ struct Impl {
       HIPCHK(hipGetLastError());
       return GSK_OK;
     };
-    CHK(y(true));
+    const bool pl = plane_on(s0);
+    if (!pl) CHK(y(true));
     switch (F) {
 #define GSK_XC(V)                                                                                                  \
   case V:                                                                                                          \
@@ -575,8 +605,8 @@ struct Impl {
       default: return fail(GSK_ERR_UNSUPPORTED, "batched ensemble launches take 1..4 variants");
     }
     HIPCHK(hipGetLastError());
-    if (plane_mid_on(s0)) {  // the members' own step uses the plane pass: keep them bit-identical
-      if constexpr (plane_mid_size<N>()) {
+    if (pl) {  // the members' own step uses the plane passes: keep them bit-identical
+      if constexpr (plane_size<N>()) {
         switch (F) {
 #define GSK_PM(V)                                                                                                  \
   case V:                                                                                                          \
@@ -607,9 +637,16 @@ struct Impl {
     hipLaunchKernelGGL((k_xupdate_mart<N, 1, 0>), dim3(gx, F, cnt), dim3(s0->xthreads), s0->lds_y, st, nk,
                        (float2*)nullptr, geo_next(s0), mk, s0->twN, 0, tab);
     HIPCHK(hipGetLastError());
-    CHK(y(false));
-    hipLaunchKernelGGL((k_zinv_sq<M>), dim3(rows, F, cnt), dim3(s0->zthreads), lz, st, nk, (float*)nullptr,
-                       geo_next(s0), s0->twM, s0->rtn, (uint32_t*)nullptr, tab);
+    if (pl) {
+      if constexpr (plane_size<N>()) {
+        hipLaunchKernelGGL((k_plane_sq<N>), dim3(N, F, cnt), dim3(plane_threads<N>()), plane_lds<N>(), st, nk,
+                           (float*)nullptr, geo_next(s0), s0->twN, s0->twM, s0->rtn, (uint32_t*)nullptr, tab);
+      }
+    } else {
+      CHK(y(false));
+      hipLaunchKernelGGL((k_zinv_sq<M>), dim3(rows, F, cnt), dim3(s0->zthreads), lz, st, nk, (float*)nullptr,
+                         geo_next(s0), s0->twM, s0->rtn, (uint32_t*)nullptr, tab);
+    }
     HIPCHK(hipGetLastError());
     return GSK_OK;
   }
@@ -765,9 +802,13 @@ struct Impl {
 
   static int energy_mart_el_fused(gsk_sim* s, double* el) {
     // S_hat = FFT(eta^2) through z and y; the energy kernel applies x and contracts.
-    if (!s->sq_ready) CHK(zfwd(s, Z_SQUARE, s->fld, s->F));
-    s->sq_ready = false;  // the y pass below transforms the buffer further
-    CHK(ypass(s, true, s->F));
+    if (plane_on(s)) {
+      if (!s->sq_ready) CHK(square_fwd(s));  // y,z-transformed already
+    } else {
+      if (!s->sq_ready) CHK(zfwd(s, Z_SQUARE, s->fld, s->F));
+      CHK(ypass(s, true, s->F));
+    }
+    s->sq_ready = false;  // the buffer is no longer what a step starts from
     const int gx = s->g.ntile;
     if (gx > s->rblocks) return fail(GSK_ERR, "reduction scratch too small");
     switch (s->F) {
@@ -948,9 +989,10 @@ void build_slots(gsk_sim* s) {
   }
   if (s->cfg.physics == GSK_PHYS_MART) {
     const int npq = F * (F + 1) / 2;
-    s->slots.push_back({"y_fwd_S", F * 2 * hs});
+    const bool pl = plane_on(s);
+    if (!pl) s->slots.push_back({"y_fwd_S", F * 2 * hs});
     s->slots.push_back({"x_contract", F * 2 * hs + npq * hs / 2});
-    if (plane_mid_on(s)) {  // R and F spectra, eta of every variant per variant
+    if (pl) {  // R and F spectra, eta of every variant per variant
       s->slots.push_back({"plane_mid", F * (2 * hs + F * rv)});
     } else {
       s->slots.push_back({"y_inv_R", F * 2 * hs});
@@ -964,8 +1006,12 @@ void build_slots(gsk_sim* s) {
       s->slots.push_back({xname[std::min(gi, 7)], c * 4 * hs});
       f0 += c;
     }
-    s->slots.push_back({"y_inv_eta", F * 2 * hs});
-    s->slots.push_back({"zinv_eta_sq", F * (2 * hs + rv)});  // C2R, write eta, R2C of eta^2
+    if (pl) {
+      s->slots.push_back({"plane_sq", F * (2 * hs + rv)});  // eta and S spectra, eta written
+    } else {
+      s->slots.push_back({"y_inv_eta", F * 2 * hs});
+      s->slots.push_back({"zinv_eta_sq", F * (2 * hs + rv)});  // C2R, write eta, R2C of eta^2
+    }
   } else {
     const bool fuse = prec_carries(s);
     if (!fuse) s->slots.push_back({"zfwd_local", F * (rv + hs)});

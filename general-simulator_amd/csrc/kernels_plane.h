@@ -1,23 +1,27 @@
-// Plane-resident middle pass of the FP32 fused martensite step, edges n = 32, 64, 128.
+// Plane-resident passes of the FP32 fused martensite step, edges n = 32, 64, 128.
 //
-// One work group holds one x-plane of one variant's half spectrum in LDS: n rows (ky) of
-// n/2 + 1 values (kz < n/2, then the Nyquist value), at an odd row pitch. On it the group runs
-// what the y inverse -> middle z pass -> y forward chain does through HBM:
-//   y inverse of R_p on the n/2 + 1 columns;
-//   C2R split and M-point inverse along each row (M = n/2), giving R_p in real space;
-//   F_p = A eta_p - B eta_p^2 + C eta_p sum_q eta_q^2 + 2 w eta_p R_p + a xi_p on every voxel,
-//     eta of all variants read as 16-byte quads, xi from the same Philox4x32-10 blocks;
-//   M-point forward along each row and R2C split; y forward; F_p written back in place.
-// HBM per voxel and variant: the R_p and F_p spectra and the V real eta fields, 8 + 4V bytes
-// against 28 for the three kernels. At n = 128 a plane takes 66.5 KB of LDS (two groups per CU).
-// The passes are the engine's LDS Stockham passes (fft_core.h) with the y and z plans of the
-// strided and contiguous kernels.
+// One work group holds one x-plane of one field's half spectrum in LDS: n rows (ky) of
+// n/2 + 1 values (kz < n/2, then the Nyquist value), at an odd row pitch, and runs on it what
+// a y pass, a z pass and a y pass do through HBM. At n = 128 a plane takes 66.5 KB of LDS (two
+// groups per CU). The passes are the engine's LDS Stockham passes (fft_core.h) with the plans
+// of the strided (y) and contiguous (z) kernels.
+//
+// k_plane_mid (middle of a step, per variant p): y inverse of R_p, C2R split and M-point inverse
+//   along each row (M = n/2), F_p = A eta_p - B eta_p^2 + C eta_p sum_q eta_q^2 + 2 w eta_p R_p
+//   + a xi_p on every voxel (eta of all variants read as 16-byte quads, xi from the same
+//   Philox4x32-10 blocks), M-point forward along each row and R2C split, y forward, F_p
+//   written back in place. HBM per voxel and variant: 8 + 4V bytes against 28 for the chain.
+// k_plane_sq (end of a step, per field): y inverse of eta_hat, C2R, eta written to the real
+//   field, squared, R2C, y forward: the y,z-transformed eta^2 the next step starts from (the
+//   chain leaves it z-transformed only). 12 bytes per voxel and field against 28.
+// k_plane_sq0: the same y,z-transformed eta^2 from the real eta fields, bit-identical to what
+//   k_plane_sq leaves (the squares are taken of the same FP32 values, then the same code).
 #pragma once
 #include "kernels_z.h"
 
 namespace gsk {
 
-template <int N> constexpr bool plane_mid_size() { return N == 32 || N == 64 || N == 128; }
+template <int N> constexpr bool plane_size() { return N == 32 || N == 64 || N == 128; }
 
 // Row pitch of the plane (complex elements): odd and >= n/2 + 1. Column passes put consecutive
 // lanes on consecutive columns (unit stride); row passes put them on consecutive rows, which an
@@ -36,6 +40,84 @@ template <int N> constexpr size_t plane_lds() {
   return (size_t)(N + 2 * (N / 2) + N * plane_pitch(N)) * sizeof(float2);
 }
 
+// LDS of a plane work group: the N-point (y) and M-point (z) twiddles, the split factors, the plane.
+template <int N> struct PlaneLds {
+  static constexpr int M = N / 2, P = plane_pitch(N), Q = M / 2, T = plane_threads<N>();
+  float2 *tw, *twz, *rt, *pl;
+  __device__ PlaneLds(float2* smem) : tw(smem), twz(smem + N), rt(smem + N + M), pl(smem + N + 2 * M) {}
+  __device__ void tables(const float2* __restrict__ twN, const float2* __restrict__ twM,
+                         const float2* __restrict__ rtn) const {
+    for (int i = threadIdx.x; i < N; i += T) tw[i] = twN[i];
+    for (int i = threadIdx.x; i < M; i += T) {
+      twz[i] = twM[i];
+      rt[i] = rtn[i];
+    }
+  }
+  // half-spectrum rows rb .. rb + n - 1 of one field (rb = x*n) <-> the plane
+  __device__ void load(const float2* __restrict__ field, const Geo& g, unsigned rb) const {
+    for (int e = threadIdx.x; e < N * M; e += T) pl[(e / M) * P + e % M] = at32(field, rb * M + (unsigned)e);
+    for (int y = threadIdx.x; y < N; y += T) pl[y * P + M] = at32(field, (unsigned)g.nq + rb + y);
+  }
+  __device__ void store(float2* __restrict__ field, const Geo& g, unsigned rb) const {
+    for (int e = threadIdx.x; e < N * M; e += T) at32(field, rb * M + (unsigned)e) = pl[(e / M) * P + e % M];
+    for (int y = threadIdx.x; y < N; y += T) at32(field, (unsigned)g.nq + rb + y) = pl[y * P + M];
+  }
+  // y transform of the n/2 main columns, then of the Nyquist column
+  template <bool FWD> __device__ void ypass() const {
+    fft_lines_p<float2, N, FWD, typename PlanOf<N>::type>(pl, Lines{M, M, 1, 0, P}, tw);
+    fft_lines_p<float2, N, FWD, typename PlanOf<N>::type>(pl + M, Lines{1, 1, 0, 0, P}, tw);
+  }
+  // C2R: split of every row into the packed M-point spectrum (task (y, k) owns the pair k, M - k;
+  // k = 0 also owns M/2 and M), then the M-point inverse along the rows
+  __device__ void c2r() const {
+    for (int e = threadIdx.x; e < N * Q; e += T) {
+      float2* z = pl + (e / Q) * P;
+      const int k = e % Q;
+      if (k == 0) {
+        const float2 a = z[0], b = z[M], c = z[Q];
+        float2 zh, unused;
+        c2r_pair(c, c, rt[Q], zh, unused);
+        z[0] = make_float2(a.x + b.x, a.x - b.x);
+        z[Q] = zh;
+      } else {
+        float2 zk, zm;
+        c2r_pair(z[k], z[M - k], rt[k], zk, zm);
+        z[k] = zk;
+        z[M - k] = zm;
+      }
+    }
+    __syncthreads();
+    fft_lines_p<float2, M, false, typename ZPlanOf<M>::type>(pl, Lines{N, N, P, 0, 1}, twz);
+  }
+  // R2C: M-point forward along the rows, then the split back to n/2 + 1 values per row
+  __device__ void r2c() const {
+    fft_lines_p<float2, M, true, typename ZPlanOf<M>::type>(pl, Lines{N, N, P, 0, 1}, twz);
+    for (int e = threadIdx.x; e < N * Q; e += T) {
+      float2* z = pl + (e / Q) * P;
+      const int k = e % Q;
+      if (k == 0) {
+        const float2 z0 = z[0], zh = z[Q];
+        z[0] = make_float2(z0.x + z0.y, 0.f);
+        z[M] = make_float2(z0.x - z0.y, 0.f);
+        z[Q] = make_float2(zh.x, -zh.y);
+      } else {
+        float2 xk, xm;
+        r2c_pair(z[k], z[M - k], rt[k], xk, xm);
+        z[k] = xk;
+        z[M - k] = xm;
+      }
+    }
+    __syncthreads();
+  }
+  // the squared real rows (packed: element m of row y holds voxels 2m, 2m+1) -> y,z-transformed
+  // eta^2 written to the field
+  __device__ void square_out(float2* __restrict__ field, const Geo& g, unsigned rb) const {
+    r2c();
+    ypass<true>();
+    store(field, g, rb);
+  }
+};
+
 // Grid: (n planes, V variants[, members]) through geo_tile / geo_field. VT: the variant count
 // (1..4). mt (may be null): batched ensemble launch, member blockIdx.z's buffers and constants.
 template <int N, int VT>
@@ -44,62 +126,31 @@ __global__ void __launch_bounds__(plane_threads<N>()) k_plane_mid(float2* __rest
                                                               const float2* __restrict__ twM,
                                                               const float2* __restrict__ rtn,
                                                               const MemberPtrs* __restrict__ mt) {
+  static_assert(plane_size<N>() && VT >= 1 && VT <= 4, "plane pass: n = 32, 64, 128 and 1..4 variants");
   if (mt) {
     kbuf = mt[blockIdx.z].kbuf;
     eta = mt[blockIdx.z].fld;
     mk = mt[blockIdx.z].mk;
   }
-  static_assert(plane_mid_size<N>() && VT >= 1 && VT <= 4, "plane pass: n = 32, 64, 128 and 1..4 variants");
-  constexpr int M = N / 2, P = plane_pitch(N), Q = M / 2;
-  constexpr int T = plane_threads<N>();
+  using PL = PlaneLds<N>;
+  constexpr int Q = PL::Q, P = PL::P, T = PL::T;
   extern __shared__ float2 smem[];
-  float2* tw = smem;           // N-point table (y)
-  float2* twz = smem + N;      // M-point table (z)
-  float2* rt = smem + N + M;   // R2C / C2R split factors
-  float2* pl = smem + N + 2 * M;
+  const PL s(smem);
   const int p = geo_field(g);
   const unsigned rb = (unsigned)geo_tile(g) * N;  // row index (x*n + y) of the plane's ky = 0
-  const int tid = threadIdx.x;
   float2* field = kbuf + p * g.kfs;
-  for (int e = tid; e < N * M; e += T) pl[(e / M) * P + e % M] = at32(field, rb * M + (unsigned)e);
-  for (int y = tid; y < N; y += T) pl[y * P + M] = at32(field, (unsigned)g.nq + rb + y);
-  for (int i = tid; i < N; i += T) tw[i] = twN[i];
-  for (int i = tid; i < M; i += T) {
-    twz[i] = twM[i];
-    rt[i] = rtn[i];
-  }
+  s.load(field, g, rb);
+  s.tables(twN, twM, rtn);
   __syncthreads();
-
-  const Lines cols{M, M, 1, 0, P}, nyq{1, 1, 0, 0, P}, rows{N, N, P, 0, 1};
-  fft_lines_p<float2, N, false, typename PlanOf<N>::type>(pl, cols, tw);
-  fft_lines_p<float2, N, false, typename PlanOf<N>::type>(pl + M, nyq, tw);
-
-  // C2R split of every row (task (y, k) owns the pair k, M - k; k = 0 also owns M/2 and M)
-  for (int e = tid; e < N * Q; e += T) {
-    float2* z = pl + (e / Q) * P;
-    const int k = e % Q;
-    if (k == 0) {
-      const float2 a = z[0], b = z[M], c = z[Q];
-      float2 zh, unused;
-      c2r_pair(c, c, rt[Q], zh, unused);
-      z[0] = make_float2(a.x + b.x, a.x - b.x);
-      z[Q] = zh;
-    } else {
-      float2 zk, zm;
-      c2r_pair(z[k], z[M - k], rt[k], zk, zm);
-      z[k] = zk;
-      z[M - k] = zm;
-    }
-  }
-  __syncthreads();
-  fft_lines_p<float2, M, false, typename ZPlanOf<M>::type>(pl, rows, twz);
+  s.template ypass<false>();
+  s.c2r();
 
   // forces on quads of voxels: packed element 2q, 2q+1 of row y holds R at voxels 4q..4q+3
   const uint32_t step = step_of(mk.step, mk.stepp);
   const float* eb = eta + (size_t)rb * N;
-  for (int e = tid; e < N * Q; e += T) {
+  for (int e = threadIdx.x; e < N * Q; e += T) {
     const int y = e / Q, q = e % Q;
-    float2* z = pl + y * P;
+    float2* z = s.pl + y * P;
     const unsigned vo = (unsigned)(y * N + 4 * q);  // voxel offset inside the plane
     float4 ep = make_float4(0.f, 0.f, 0.f, 0.f), s2 = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -122,30 +173,76 @@ __global__ void __launch_bounds__(plane_threads<N>()) k_plane_mid(float2* __rest
     z[2 * q + 1] = make_float2(o[2], o[3]);
   }
   __syncthreads();
-  fft_lines_p<float2, M, true, typename ZPlanOf<M>::type>(pl, rows, twz);
+  s.r2c();
+  s.template ypass<true>();
+  s.store(field, g, rb);
+}
 
-  // R2C split of every row back to n/2 + 1 half-spectrum values
-  for (int e = tid; e < N * Q; e += T) {
-    float2* z = pl + (e / Q) * P;
-    const int k = e % Q;
-    if (k == 0) {
-      const float2 z0 = z[0], zh = z[Q];
-      z[0] = make_float2(z0.x + z0.y, 0.f);
-      z[M] = make_float2(z0.x - z0.y, 0.f);
-      z[Q] = make_float2(zh.x, -zh.y);
-    } else {
-      float2 xk, xm;
-      r2c_pair(z[k], z[M - k], rt[k], xk, xm);
-      z[k] = xk;
-      z[M - k] = xm;
-    }
+// Grid: (n planes, F fields[, members]). adv (may be null): step counter advanced once by the
+// launch (per member in a batched launch, whose table also supplies kbuf and the real fields).
+template <int N>
+__global__ void __launch_bounds__(plane_threads<N>()) k_plane_sq(float2* __restrict__ kbuf, float* __restrict__ out, Geo g,
+                                                             const float2* __restrict__ twN,
+                                                             const float2* __restrict__ twM,
+                                                             const float2* __restrict__ rtn, uint32_t* adv,
+                                                             const MemberPtrs* __restrict__ mt) {
+  static_assert(plane_size<N>(), "plane pass: n = 32, 64, 128");
+  if (mt) {
+    kbuf = mt[blockIdx.z].kbuf;
+    out = mt[blockIdx.z].fld;
+    adv = mt[blockIdx.z].dstep;
+  }
+  if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *adv += 1u;
+  using PL = PlaneLds<N>;
+  constexpr int Q = PL::Q, P = PL::P, T = PL::T;
+  extern __shared__ float2 smem[];
+  const PL s(smem);
+  const int f = geo_field(g);
+  const unsigned rb = (unsigned)geo_tile(g) * N;
+  float2* field = kbuf + f * g.kfs;
+  s.load(field, g, rb);
+  s.tables(twN, twM, rtn);
+  __syncthreads();
+  s.template ypass<false>();
+  s.c2r();
+  float* dst = out + f * g.nvox + (size_t)rb * N;
+  for (int e = threadIdx.x; e < N * Q; e += T) {
+    const int y = e / Q, q = e % Q;
+    float2* z = s.pl + y * P;
+    const float2 a = z[2 * q], b = z[2 * q + 1];
+    at32(reinterpret_cast<float4*>(dst), (unsigned)(y * N + 4 * q) / 4) = make_float4(a.x, a.y, b.x, b.y);
+    // each task squares the very elements it has just read: no barrier between
+    z[2 * q] = make_float2(a.x * a.x, a.y * a.y);
+    z[2 * q + 1] = make_float2(b.x * b.x, b.y * b.y);
   }
   __syncthreads();
-  fft_lines_p<float2, N, true, typename PlanOf<N>::type>(pl, cols, tw);
-  fft_lines_p<float2, N, true, typename PlanOf<N>::type>(pl + M, nyq, tw);
+  s.square_out(field, g, rb);
+}
 
-  for (int e = tid; e < N * M; e += T) at32(field, rb * M + (unsigned)e) = pl[(e / M) * P + e % M];
-  for (int y = tid; y < N; y += T) at32(field, (unsigned)g.nq + rb + y) = pl[y * P + M];
+// Grid: (n planes, F fields). The real fields `in` -> y,z-transformed squares in kbuf.
+template <int N>
+__global__ void __launch_bounds__(plane_threads<N>()) k_plane_sq0(const float* __restrict__ in, float2* __restrict__ kbuf,
+                                                              Geo g, const float2* __restrict__ twN,
+                                                              const float2* __restrict__ twM,
+                                                              const float2* __restrict__ rtn) {
+  static_assert(plane_size<N>(), "plane pass: n = 32, 64, 128");
+  using PL = PlaneLds<N>;
+  constexpr int Q = PL::Q, P = PL::P, T = PL::T;
+  extern __shared__ float2 smem[];
+  const PL s(smem);
+  const int f = geo_field(g);
+  const unsigned rb = (unsigned)geo_tile(g) * N;
+  const float* src = in + f * g.nvox + (size_t)rb * N;
+  for (int e = threadIdx.x; e < N * Q; e += T) {
+    const int y = e / Q, q = e % Q;
+    float2* z = s.pl + y * P;
+    const float4 v = at32(reinterpret_cast<const float4*>(src), (unsigned)(y * N + 4 * q) / 4);
+    z[2 * q] = make_float2(v.x * v.x, v.y * v.y);
+    z[2 * q + 1] = make_float2(v.z * v.z, v.w * v.w);
+  }
+  s.tables(twN, twM, rtn);
+  __syncthreads();
+  s.square_out(kbuf + f * g.kfs, g, rb);
 }
 
 }  // namespace gsk
