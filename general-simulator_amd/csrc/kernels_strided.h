@@ -21,10 +21,16 @@
 #define GSK_XUPD_PF 1
 #endif
 // GSK_XC_PFB: the contraction loads its B tensor before the forward transforms at the sizes
-// where that was measured to pay (64, 96, 192: -7 to -12 %; 128 and 512: slower)
+// where that was measured to pay (64, 96, 192: -7 to -12 %; 128 and 512: slower), except where
+// xc_cap64 holds
 #ifndef GSK_XC_PFB
 #define GSK_XC_PFB 1
 #endif
+// At n = 192 with three variants the contraction's 1024-thread groups are capped at 64 VGPRs
+// (8 waves/SIMD, two groups per CU; 4 registers spill) and skip the B prefetch, which would need
+// 48 more: 84.6 -> 72.5 us against one group per CU with the prefetch.
+template <int N, int V> constexpr bool xc_cap64() { return N == 192 && V == 3; }
+template <int N, int V> constexpr int xc_wpe() { return xc_cap64<N, V>() ? 8 : 1; }
 
 namespace gsk {
 
@@ -115,7 +121,8 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xpass(float2* __restrict__ 
 // ---------------------------------------------------------------- martensite contraction (V <= 4)
 // S_hat -> x FFT (to registers) -> R_hat_p = sum_q B_pq S_hat_q / n^3 -> inverse x FFT.
 template <int N, int V>
-__global__ void __launch_bounds__(xy_threads<N>()) k_xcontract_r(float2* __restrict__ kbuf, const float* __restrict__ Bt, Geo g,
+__global__ void __launch_bounds__(xy_threads<N>()) __attribute__((amdgpu_waves_per_eu(xc_wpe<N, V>())))
+k_xcontract_r(float2* __restrict__ kbuf, const float* __restrict__ Bt, Geo g,
                                                      MartK mk, const float2* __restrict__ twN,
                                                      const MemberPtrs* __restrict__ mt) {
   if (mt) {  // batched ensemble launch
@@ -127,7 +134,7 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xcontract_r(float2* __restr
   constexpr int NPQ = V * (V + 1) / 2;
   // B does not depend on the transforms: with at most 48 values per thread (last radix <= 8,
   // V <= 3) it can be fetched before them, so its loads are in flight during the forward FFTs
-  constexpr bool PFB = GSK_XC_PFB && RL * NPQ <= 48 && (N == 64 || N == 96 || N == 192);
+  constexpr bool PFB = GSK_XC_PFB && RL * NPQ <= 48 && (N == 64 || N == 96 || N == 192) && !xc_cap64<N, V>();
   extern __shared__ float2 smem[];
   float2* tw = smem;
   float2* tile = smem + N;
