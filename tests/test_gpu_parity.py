@@ -243,14 +243,16 @@ def test_mart_fused_vs_plain_at_512(gpu):
     assert float(np.abs(got - want).max()) < 2e-6 * float(np.abs(want).max())
 
 
-def test_mart_square_carry_bit_identical(gpu):
-    """A step leaves the z-transformed eta^2 of its result for the next step. Steps that run
-    back to back must give bit for bit the trajectory of steps separated by energy reductions
-    (which use the k-space buffer, so the next step transforms eta^2 afresh)."""
-    say("mart eta^2 carry-over n=64")
+@pytest.mark.parametrize("n", [64, 96])
+def test_mart_square_carry_bit_identical(gpu, n):
+    """A step leaves the transformed eta^2 of its result for the next step (along z and y at
+    n = 64, where the plane-resident passes run; along z at n = 96). Steps that run back to
+    back must give bit for bit the trajectory of steps separated by energy reductions (which
+    use the k-space buffer, so the next step transforms eta^2 afresh)."""
+    say(f"mart eta^2 carry-over n={n}")
     kw = dict(A=-0.5, w_el=20.0, noise=0.002, seed=5)
-    a = mart_sim(64, 3, **kw)
-    b = mart_sim(64, 3, **kw)
+    a = mart_sim(n, 3, **kw)
+    b = mart_sim(n, 3, **kw)
     for s in (a, b):
         s.init_uniform("eta", -0.05, 0.05, 2)
     a.step(6)
