@@ -168,6 +168,19 @@ def test_mart_strong_dynamics_with_noise(gpu):
         np.testing.assert_allclose(E, Ew, rtol=max(tol * 10, 1e-10), atol=1e-9 * abs(Ew[3]))
 
 
+@pytest.mark.parametrize("n,V", [(32, 1), (32, 2), (32, 4), (64, 2), (128, 1)])
+def test_mart_fused32_variant_counts(gpu, n, V):
+    """The fused step with 1, 2 and 4 variants (at 32^3, 64^3 and 128^3 the plane-resident
+    passes, instantiated per variant count) against the FP64 oracle, below the transition,
+    with elasticity and noise."""
+    say(f"mart fused n={n} V={V} A=-0.5 noise")
+    kw = dict(A=-0.5, B=3.0, C=3.0, noise=0.002, seed=13, w_el=20.0)
+    got, want, E, Ew = run_mart_pair(n, V, 30, gsk.PATH_FUSED32, **kw)
+    assert np.abs(got - want).max() / np.abs(want).max() < 5e-5
+    assert rel_l2(got, want) < 5e-5
+    np.testing.assert_allclose(E, Ew, rtol=5e-4, atol=1e-9 * abs(Ew[3]))
+
+
 @pytest.mark.parametrize("n", [64, 96, 256, 512])
 def test_mart_noise_enters_exactly(gpu, n):
     """With every other force zero (A = B = C = 0, no elasticity, kappa = 0), eta = 0 and
