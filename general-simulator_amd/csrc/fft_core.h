@@ -97,11 +97,41 @@ template <bool FWD, typename T2> __device__ __forceinline__ void dft16(T2* v) {
   }
 }
 
+// 12 = 3 x 4: 4-point DFTs over n1 of x[3 n1 + n2], twiddles w12^(n2 k1), 3-point DFTs over n2
+// giving X[k1 + 4 k2]
+template <bool FWD, typename T2> __device__ __forceinline__ void dft12(T2* v) {
+  using S = decltype(T2::x);
+  const S c = S(0.86602540378443864676), h = S(0.5);  // cos(pi/6), sin(pi/6)
+  T2 a[3][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 3; ++n2) {
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) a[n2][n1] = v[3 * n1 + n2];
+    dft4v<FWD>(a[n2][0], a[n2][1], a[n2][2], a[n2][3]);
+  }
+  const S sg = FWD ? S(-1) : S(1);  // w12^m = (cos(pi m/6), sg sin(pi m/6))
+  a[1][1] = cmul(a[1][1], mk<T2>(c, sg * h));
+  a[1][2] = cmul(a[1][2], mk<T2>(h, sg * c));
+  a[1][3] = rot<FWD>(a[1][3]);
+  a[2][1] = cmul(a[2][1], mk<T2>(h, sg * c));
+  a[2][2] = cmul(a[2][2], mk<T2>(-h, sg * c));
+  a[2][3] = mk<T2>(-a[2][3].x, -a[2][3].y);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    T2 t[3] = {a[0][k1], a[1][k1], a[2][k1]};
+    dft3<FWD>(t);
+    v[k1] = t[0];
+    v[k1 + 4] = t[1];
+    v[k1 + 8] = t[2];
+  }
+}
+
 template <int R, bool FWD, typename T2> __device__ __forceinline__ void dftR(T2* v) {
   if constexpr (R == 2) dft2<FWD>(v);
   else if constexpr (R == 3) dft3<FWD>(v);
   else if constexpr (R == 4) dft4<FWD>(v);
   else if constexpr (R == 8) dft8<FWD>(v);
+  else if constexpr (R == 12) dft12<FWD>(v);
   else if constexpr (R == 16) dft16<FWD>(v);
   else static_assert(R == 2, "unsupported radix");
 }
@@ -118,7 +148,19 @@ template <> struct PlanOf<48> { using type = Radices<3, 16>; static constexpr in
 template <> struct PlanOf<64> { using type = Radices<8, 8>; static constexpr int rmin = 8; };
 template <> struct PlanOf<96> { using type = Radices<3, 4, 8>; static constexpr int rmin = 3; };
 template <> struct PlanOf<128> { using type = Radices<16, 8>; static constexpr int rmin = 8; };
+// GSK_PLAN_192: 0 = 3*8*8, 1 = 12*16, 2 = 16*12. 12*16 takes two passes and 16 butterflies per
+// line like 256 = 16*16 (192^3 step 0.384 -> 0.347 ms against 3*8*8, whose radix-3 pass needs
+// 64 butterflies per line and leaves most threads idle in the radix-8 passes; 16*12: 0.350)
+#ifndef GSK_PLAN_192
+#define GSK_PLAN_192 1
+#endif
+#if GSK_PLAN_192 == 1
+template <> struct PlanOf<192> { using type = Radices<12, 16>; static constexpr int rmin = 12; };
+#elif GSK_PLAN_192 == 2
+template <> struct PlanOf<192> { using type = Radices<16, 12>; static constexpr int rmin = 12; };
+#else
 template <> struct PlanOf<192> { using type = Radices<3, 8, 8>; static constexpr int rmin = 3; };
+#endif
 template <> struct PlanOf<256> { using type = Radices<16, 16>; static constexpr int rmin = 16; };
 template <> struct PlanOf<512> { using type = Radices<8, 8, 8>; static constexpr int rmin = 8; };
 

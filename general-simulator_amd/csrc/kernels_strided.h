@@ -29,7 +29,7 @@
 // At n = 192 with three variants the contraction's 1024-thread groups are capped at 64 VGPRs
 // (8 waves/SIMD, two groups per CU; 4 registers spill) and skip the B prefetch, which would need
 // 48 more: 84.6 -> 72.5 us against one group per CU with the prefetch.
-template <int N, int V> constexpr bool xc_cap64() { return N == 192 && V == 3; }
+template <int N, int V> constexpr bool xc_cap64() { return N == 192 && V == 3 && gsk::PlanOf<N>::rmin == 3; }
 template <int N, int V> constexpr int xc_wpe() { return xc_cap64<N, V>() ? 8 : 1; }
 
 namespace gsk {
