@@ -84,9 +84,19 @@ template <int X, int R0, int... Rs, int... Out> struct DropOne<X, Radices<R0, Rs
 };
 template <int X, class P> struct Prepend;
 template <int X, int... Rs> struct Prepend<X, Radices<Rs...>> { using type = Radices<X, Rs...>; };
+// Base plan of the middle pass: the z plan, except 3*4*8 at M = 96 (with 12*8 the middle pass
+// at 192^3 took 82.5 against 68.7 us, while the other 96-point z passes gain)
+template <int M> struct ZMidBase {
+  using type = typename ZPlanOf<M>::type;
+  static constexpr int rmin = ZPlanOf<M>::rmin;
+};
+template <> struct ZMidBase<96> {
+  using type = Radices<3, 4, 8>;
+  static constexpr int rmin = 3;
+};
 template <int M> struct ZMidPlan {
-  static constexpr int r0 = ZPlanOf<M>::rmin;
-  using fwd = typename Prepend<r0, typename DropOne<r0, typename ZPlanOf<M>::type, Radices<>>::type>::type;
+  static constexpr int r0 = ZMidBase<M>::rmin;
+  using fwd = typename Prepend<r0, typename DropOne<r0, typename ZMidBase<M>::type, Radices<>>::type>::type;
   using inv = typename RevImpl<fwd, Radices<>>::type;
 };
 
@@ -112,7 +122,7 @@ __host__ __device__ constexpr int zthreads_lr(int L, int M, int rmin) {
 }
 __host__ __device__ constexpr int zthreads_of(int M, int rmin) { return zthreads_lr(zrows(M), M, rmin); }
 template <int M> constexpr int zthreads() { return zthreads_of(M, ZPlanOf<M>::rmin); }
-template <int M> constexpr int zmid_threads() { return zthreads_lr(zmid_rows(M), M, ZPlanOf<M>::rmin); }
+template <int M> constexpr int zmid_threads() { return zthreads_lr(zmid_rows(M), M, ZMidBase<M>::rmin); }
 
 // Row pitch of a z tile of L rows in LDS (complex elements): the smallest P >= M + 1 with
 // P = 16/L (mod 16) for L <= 16 rows per tile, P odd for L = 32. An FFT pass maps lanes
@@ -162,19 +172,19 @@ template <class T> __device__ __forceinline__ T& at32(T* base, unsigned elem) {
 
 // --------------------------------------------------------------------- one-field tile I/O
 // TM: work-group size as a multiple of the FFT's thread count (more threads, fewer tasks each);
-// LR: rows per tile, P: their LDS row pitch
-template <int M, int TM = 1, int LR = zrows(M)>
+// LR: rows per tile, P: their LDS row pitch; RM: smallest radix of the tile's plan
+template <int M, int TM = 1, int LR = zrows(M), int RM = ZPlanOf<M>::rmin>
 struct ZShape {
-  static constexpr int L = LR, T = zthreads_lr(LR, M, ZPlanOf<M>::rmin) * TM, H2 = M / 2, P = zpitch_l(M, LR);
+  static constexpr int L = LR, T = zthreads_lr(LR, M, RM) * TM, H2 = M / 2, P = zpitch_l(M, LR);
   static constexpr int NT = (L * H2 + T - 1) / T;  // tasks per thread
   static constexpr bool FULL = (L * H2) % T == 0;
   __device__ static bool valid(int e) { return FULL || e < L * H2; }
 };
 
 // The half-spectrum values of one tile that a thread owns, held in registers.
-template <int M, int TM = 1, int LR = zrows(M)>
+template <int M, int TM = 1, int LR = zrows(M), int RM = ZPlanOf<M>::rmin>
 struct ZSpec {
-  using S = ZShape<M, TM, LR>;
+  using S = ZShape<M, TM, LR, RM>;
   float2 a[S::NT], b[S::NT], c[S::NT];
   // Rows r0.. of one field: a uniform field base and 32-bit element offsets (a field holds
   // fewer than 2^29 elements), so the loads use a scalar base and one offset register each.
@@ -217,10 +227,10 @@ struct ZSpec {
 
 // packed spectra of the LDS tile -> half-spectrum rows r0.. of one field (R2C direction);
 // offsets as in ZSpec::load
-template <int M, int TM = 1, int LR = zrows(M)>
+template <int M, int TM = 1, int LR = zrows(M), int RM = ZPlanOf<M>::rmin>
 __device__ __forceinline__ void z_store_spec(const float2* tile, float2* __restrict__ field, const Geo& g, long r0,
                                              const float2* rt, int tid = threadIdx.x) {
-  using S = ZShape<M, TM, LR>;
+  using S = ZShape<M, TM, LR, RM>;
   const unsigned rb = (unsigned)r0 * M, nb = (unsigned)(g.nq + r0);
 #pragma unroll
   for (int i = 0; i < S::NT; ++i) {
@@ -414,8 +424,8 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, Mar
     eta = mt[blockIdx.z].fld;
     mk = mt[blockIdx.z].mk;
   }
-  constexpr int LZ = zmid_rows(M);
-  using S = ZShape<M, 1, LZ>;
+  constexpr int LZ = zmid_rows(M), RMZ = ZMidBase<M>::rmin;
+  using S = ZShape<M, 1, LZ, RMZ>;
   using P = ZMidPlan<M>;
   constexpr int n = 2 * M, R0 = P::r0, NB = S::L * (M / R0);
   constexpr int VR = VT > 0 ? VT : 1;
@@ -441,7 +451,7 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, Mar
           pre[r][f] = at32(reinterpret_cast<const float2*>(erow + f * g.nvox), vloc / 2 + r * (M / R0));
     }
   }
-  ZSpec<M, 1, LZ> sp;
+  ZSpec<M, 1, LZ, RMZ> sp;
   sp.load(kbuf, g, r0);
   z_twiddles<M>(tw, rt, twM, rtn);
   const Lines Ls{S::L, S::L, S::P, 0, 1};
@@ -528,7 +538,7 @@ k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, Mar
       }
     }
     gchain<float2, M, true, 2, 0, 1>(tile, Ls, tw, no_ld, no_st, v, typename P::fwd{}, tid);
-    z_store_spec<M, 1, LZ>(tile, kbuf + p * g.kfs, g, r0, rt, tid);
+    z_store_spec<M, 1, LZ, RMZ>(tile, kbuf + p * g.kfs, g, r0, rt, tid);
     __syncthreads();  // the next variant's split overwrites the tile
   }
 }
