@@ -21,8 +21,8 @@
 #define GSK_XUPD_PF 1
 #endif
 // GSK_XC_PFB: the contraction loads its B tensor before the forward transforms at the sizes
-// where that was measured to pay (64, 96, 192: -7 to -12 %; 128 and 512: slower), except where
-// xc_cap64 holds
+// where that was measured to pay (64, 96, 192 under a plan ending on radix 8: -7 to -12 %; 128
+// and 512: slower), except where xc_cap64 holds
 #ifndef GSK_XC_PFB
 #define GSK_XC_PFB 1
 #endif
