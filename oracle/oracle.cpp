@@ -3,6 +3,7 @@
 // as written, operation by operation.
 #include "oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -30,80 +31,143 @@ extern "C" int or_get_threads(void) {
 }
 
 // ---------------------------------------------------------------------------
-// FFT: recursive decimation-in-time Cooley-Tukey with radices 4, 2 and 3.
-// tw[j] = exp(sign*2*pi*i*j/N0) for the top-level length N0.
+// FFT: self-sorting Stockham passes with radices 4, 2 and 3, on batches of B lines
+// stored line-fastest ([position][line]) so that the strided y and x passes of a 3D
+// transform read and write runs of B contiguous z values. One pass of radix R with
+// Ns = product of the radices before it:
+//   for j < n/R:  k = j mod Ns,  v_r = in[j + r n/R] * exp(sign 2 pi i k r / (Ns R)),
+//                 X_q = sum_r v_r exp(sign 2 pi i r q / R),  out[(j/Ns) Ns R + k + q Ns] = X_q.
+// Twiddles are evaluated with cos/sin in double precision for every (pass, k, r).
 // ---------------------------------------------------------------------------
-static void fft_rec(const cd* in, cd* out, int n, int s, const cd* tw, int N0) {
-  if (n == 1) { out[0] = in[0]; return; }
-  int p = (n % 4 == 0) ? 4 : (n % 2 == 0 ? 2 : (n % 3 == 0 ? 3 : 0));
-  if (p == 0) {  // plain DFT for any other factor (not used for supported sizes)
-    for (int k = 0; k < n; ++k) {
-      cd acc = 0;
-      for (int j = 0; j < n; ++j) acc += in[(int64_t)j * s] * tw[((int64_t)j * k % n) * (N0 / n)];
-      out[k] = acc;
-    }
-    return;
-  }
-  int m = n / p;
-  for (int r = 0; r < p; ++r) fft_rec(in + (int64_t)r * s, out + (int64_t)r * m, m, s * p, tw, N0);
-  int step = N0 / n;  // tw index of exp(sign*2*pi*i/n)
-  cd t[4];
-  for (int k = 0; k < m; ++k) {
-    for (int r = 0; r < p; ++r) t[r] = out[r * m + k] * tw[(int64_t)r * k * step % N0];
-    for (int q = 0; q < p; ++q) {
-      cd acc = 0;
-      for (int r = 0; r < p; ++r) acc += t[r] * tw[((int64_t)r * q * m * step) % N0];
-      out[k + q * m] = acc;
-    }
-  }
-}
-
-struct Plan1D {
-  int n;
-  std::vector<cd> tw;
-  Plan1D(int n_, int sign) : n(n_), tw(n_) {
-    for (int j = 0; j < n; ++j) {
-      double a = sign * 2.0 * kPi * (double)j / (double)n;
-      tw[j] = cd(std::cos(a), std::sin(a));
+struct FftPlan {
+  int n = 0, sign = -1;
+  std::vector<int> radix;
+  std::vector<std::vector<cd>> tw;  // tw[pass][k * R + r]
+  FftPlan(int n_, int sign_) : n(n_), sign(sign_) {
+    int m = n;
+    while (m % 4 == 0) { radix.push_back(4); m /= 4; }
+    while (m % 2 == 0) { radix.push_back(2); m /= 2; }
+    while (m % 3 == 0) { radix.push_back(3); m /= 3; }
+    if (m != 1) radix.push_back(m);  // any other factor: one direct DFT pass
+    int Ns = 1;
+    for (int R : radix) {
+      std::vector<cd> t((size_t)Ns * R);
+      for (int k = 0; k < Ns; ++k)
+        for (int r = 0; r < R; ++r) {
+          const double a = sign * 2.0 * kPi * (double)k * (double)r / (double)(Ns * R);
+          t[(size_t)k * R + r] = cd(std::cos(a), std::sin(a));
+        }
+      tw.push_back(std::move(t));
+      Ns *= R;
     }
   }
-  void run(cd* data, cd* scratch) const {
-    std::memcpy(scratch, data, sizeof(cd) * n);
-    fft_rec(scratch, data, n, 1, tw.data(), n);
+  // data, scratch: n * B complex values, element (i, b) at i * B + b. Result in data.
+  void run(cd* data, cd* scratch, int B) const {
+    cd* in = data;
+    cd* out = scratch;
+    int Ns = 1;
+    const double s = (double)sign;
+    const double c3 = -0.5, s3 = s * 0.86602540378443864676;  // exp(sign 2 pi i / 3)
+    for (size_t ps = 0; ps < radix.size(); ++ps) {
+      const int R = radix[ps], nR = n / R;
+      const cd* w = tw[ps].data();
+      for (int j = 0; j < nR; ++j) {
+        const int k = j % Ns;
+        const int dst = (j / Ns) * Ns * R + k;
+        const cd* wk = w + (size_t)k * R;
+        for (int b = 0; b < B; ++b) {
+          double vr[16], vi[16];
+          for (int r = 0; r < R; ++r) {
+            const cd x = in[(size_t)(j + r * nR) * B + b];
+            vr[r] = x.real() * wk[r].real() - x.imag() * wk[r].imag();
+            vi[r] = x.real() * wk[r].imag() + x.imag() * wk[r].real();
+          }
+          double xr[16], xi[16];
+          if (R == 4) {
+            const double ar = vr[0] + vr[2], ai = vi[0] + vi[2], br = vr[0] - vr[2], bi = vi[0] - vi[2];
+            const double cr = vr[1] + vr[3], ci = vi[1] + vi[3], dr = vr[1] - vr[3], di = vi[1] - vi[3];
+            // exp(sign 2 pi i / 4) = sign * i:  (sign i) d = (-sign di, sign dr)
+            xr[0] = ar + cr; xi[0] = ai + ci;
+            xr[2] = ar - cr; xi[2] = ai - ci;
+            xr[1] = br - s * di; xi[1] = bi + s * dr;
+            xr[3] = br + s * di; xi[3] = bi - s * dr;
+          } else if (R == 2) {
+            xr[0] = vr[0] + vr[1]; xi[0] = vi[0] + vi[1];
+            xr[1] = vr[0] - vr[1]; xi[1] = vi[0] - vi[1];
+          } else if (R == 3) {
+            const double tr = vr[1] + vr[2], ti = vi[1] + vi[2], ur = vr[1] - vr[2], ui = vi[1] - vi[2];
+            xr[0] = vr[0] + tr; xi[0] = vi[0] + ti;
+            const double mr = vr[0] + c3 * tr, mi = vi[0] + c3 * ti;
+            xr[1] = mr - s3 * ui; xi[1] = mi + s3 * ur;
+            xr[2] = mr + s3 * ui; xi[2] = mi - s3 * ur;
+          } else {
+            for (int q = 0; q < R; ++q) {
+              double accr = 0.0, acci = 0.0;
+              for (int r = 0; r < R; ++r) {
+                const double a = s * 2.0 * kPi * (double)((r * q) % R) / (double)R;
+                const double cr = std::cos(a), ci = std::sin(a);
+                accr += vr[r] * cr - vi[r] * ci;
+                acci += vr[r] * ci + vi[r] * cr;
+              }
+              xr[q] = accr; xi[q] = acci;
+            }
+          }
+          for (int q = 0; q < R; ++q) out[(size_t)(dst + q * Ns) * B + b] = cd(xr[q], xi[q]);
+        }
+      }
+      std::swap(in, out);
+      Ns *= R;
+    }
+    if (in != data) std::memcpy(data, in, sizeof(cd) * (size_t)n * B);
   }
 };
 
 extern "C" void or_fft1d(int n, double* data, int sign) {
-  Plan1D pl(n, sign);
+  FftPlan pl(n, sign);
   std::vector<cd> s(n);
-  pl.run(reinterpret_cast<cd*>(data), s.data());
+  pl.run(reinterpret_cast<cd*>(data), s.data(), 1);
 }
 
 static void fft3d(int n, cd* a, int sign) {
-  Plan1D pl(n, sign);
+  const FftPlan pl(n, sign);
   const int64_t n2 = (int64_t)n * n;
-  // z lines (contiguous)
+  const int B = n < 16 ? n : 16;  // lines per batch (n is a multiple of 16 or smaller than 16 here)
+  const int nb = (n + B - 1) / B;
 #pragma omp parallel
   {
-    std::vector<cd> s(n), line(n);
+    std::vector<cd> buf((size_t)n * B), s((size_t)n * B);
+    // z lines (contiguous): B lines of one (x, y-block) gathered transposed
 #pragma omp for schedule(static)
-    for (int64_t xy = 0; xy < n2; ++xy) pl.run(a + xy * n, s.data());
-    // y lines: stride n
-#pragma omp for schedule(static)
-    for (int64_t xz = 0; xz < n2; ++xz) {
-      int64_t x = xz / n, z = xz % n;
-      cd* base = a + x * n2 + z;
-      for (int y = 0; y < n; ++y) line[y] = base[(int64_t)y * n];
-      pl.run(line.data(), s.data());
-      for (int y = 0; y < n; ++y) base[(int64_t)y * n] = line[y];
+    for (int64_t t = 0; t < (int64_t)n * nb; ++t) {
+      const int64_t x = t / nb, y0 = (t % nb) * B;
+      const int Bt = (int)std::min<int64_t>(B, n - y0);
+      for (int b = 0; b < Bt; ++b)
+        for (int z = 0; z < n; ++z) buf[(size_t)z * Bt + b] = a[x * n2 + (y0 + b) * n + z];
+      pl.run(buf.data(), s.data(), Bt);
+      for (int b = 0; b < Bt; ++b)
+        for (int z = 0; z < n; ++z) a[x * n2 + (y0 + b) * n + z] = buf[(size_t)z * Bt + b];
     }
-    // x lines: stride n^2
+    // y lines: for each x and block of B z values, element (y, b) = a[x][y][z0 + b]
 #pragma omp for schedule(static)
-    for (int64_t yz = 0; yz < n2; ++yz) {
-      cd* base = a + yz;
-      for (int x = 0; x < n; ++x) line[x] = base[(int64_t)x * n2];
-      pl.run(line.data(), s.data());
-      for (int x = 0; x < n; ++x) base[(int64_t)x * n2] = line[x];
+    for (int64_t t = 0; t < (int64_t)n * nb; ++t) {
+      const int64_t x = t / nb, z0 = (t % nb) * B;
+      const int Bt = (int)std::min<int64_t>(B, n - z0);
+      for (int y = 0; y < n; ++y)
+        for (int b = 0; b < Bt; ++b) buf[(size_t)y * Bt + b] = a[x * n2 + (int64_t)y * n + z0 + b];
+      pl.run(buf.data(), s.data(), Bt);
+      for (int y = 0; y < n; ++y)
+        for (int b = 0; b < Bt; ++b) a[x * n2 + (int64_t)y * n + z0 + b] = buf[(size_t)y * Bt + b];
+    }
+    // x lines: for each y and block of B z values, element (x, b) = a[x][y][z0 + b]
+#pragma omp for schedule(static)
+    for (int64_t t = 0; t < (int64_t)n * nb; ++t) {
+      const int64_t y = t / nb, z0 = (t % nb) * B;
+      const int Bt = (int)std::min<int64_t>(B, n - z0);
+      for (int x = 0; x < n; ++x)
+        for (int b = 0; b < Bt; ++b) buf[(size_t)x * Bt + b] = a[(int64_t)x * n2 + y * n + z0 + b];
+      pl.run(buf.data(), s.data(), Bt);
+      for (int x = 0; x < n; ++x)
+        for (int b = 0; b < Bt; ++b) a[(int64_t)x * n2 + y * n + z0 + b] = buf[(size_t)x * Bt + b];
     }
   }
 }

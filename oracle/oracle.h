@@ -19,7 +19,7 @@
  * factor 2 of src/dynamics_precipitate.cu:193.
  *
  * cuFFT / cuRAND (closed NVIDIA libraries linked by the reference) are replaced:
- * the FFT here is an independent recursive mixed-radix (2,3) Cooley-Tukey in
+ * the FFT here is an independent self-sorting Stockham (radices 4, 2, 3) in
  * double precision; noise is Philox4x32-10 (Salmon et al., SC'11) followed by
  * the documented fp32 Box-Muller transform built from + - * only.
  */

@@ -1,0 +1,131 @@
+"""GPU accuracy at the benchmark workloads' full sizes.
+
+The FP32 fused step (the benchmarked path) is compared with the FP64 plain path after the
+benchmark's step counts. The FP64 plain path is an independent implementation (C2C
+transforms, one kernel per operation, the full-layout FP64 B tensor) that the FP64 oracle
+pins to ~1e-12 (tests/test_gpu_parity.py and the 64^3 case below, which runs the oracle
+itself), so at 256^3 and 512^3, where the oracle takes hours, it stands in for the oracle.
+
+Each case prints one `[accuracy]` JSON line: max|d eta|, max|eta|, relative L2 error and
+the relative error of the FP64 total energy (chemical + gradient + elastic).
+
+Tolerances (FP32 arithmetic against FP64):
+* the headline bar max|d eta| <= 1e-4 after 1000 steps (256^3, 3 variants);
+* relative L2 <= 1e-4 on the config-0 physics, whose eta decays to ~1e-4 so that the
+  absolute bar alone would be loose; relative energy error <= 1e-4;
+* strong dynamics (A < 0, eta grows to O(1) domains): relative L2 <= 1e-3.
+"""
+import json
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import gsk
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def report(name, **kw):
+    print("[accuracy] " + json.dumps({"case": name, **kw}), flush=True)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def c_cfg0():
+    return gsk.cubic_modulus(350 / 500, 150 / 500, 100 / 500)
+
+
+def run_mart(n, steps, path, V=3, init=(-0.01, 0.01, 0), **kw):
+    cfg = gsk.default_config(gsk.PHYS_MART, n=n, nvar=V, path=path, **kw)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(c_cfg0(), gsk.mart_eigenstrains(V))
+        s.init_uniform("eta", *init)
+        eta0 = s.get_field("eta", np.float64)
+        t0 = time.perf_counter()
+        s.step(steps)
+        el = time.perf_counter() - t0
+        return eta0, s.get_field("eta", np.float64), s.energy(), el
+
+
+def compare(name, got, want, Eg, Ew, **extra):
+    d = float(np.abs(got - want).max())
+    r = dict(max_abs_d_eta=d, max_abs_eta=float(np.abs(want).max()), rel_l2=rel_l2(got, want),
+             rel_energy_err=float(abs(Eg[3] - Ew[3]) / abs(Ew[3])), energy_fused=Eg.tolist(), energy_ref=Ew.tolist())
+    r.update(extra)
+    report(name, **r)
+    return r
+
+
+def test_headline_accuracy_256_config0_1000_steps(gpu):
+    """256^3, 3 variants, the 64^3 configuration's physics (A = 0.2, B = C = 3, kappa = L = 1,
+    dt = 0.01, C11/C12/C44 = 350/150/100 / 500, eta0 ~ U[-0.01, 0.01] seed 0, noise off),
+    1000 steps: FP32 fused against FP64 plain."""
+    say("accuracy 256^3 config-0 physics, 1000 steps, fused vs FP64 plain")
+    n, steps = 256, 1000
+    e0f, got, Eg, tf = run_mart(n, steps, gsk.PATH_FUSED32)
+    e0p, want, Ew, tp = run_mart(n, steps, gsk.PATH_PLAIN64)
+    assert np.array_equal(e0f, e0p)
+    r = compare("mart256_config0_1000", got, want, Eg, Ew, steps=steps, fused_s=tf, plain64_s=tp)
+    assert r["max_abs_d_eta"] <= 1e-4
+    assert r["rel_l2"] <= 1e-4
+    assert r["rel_energy_err"] <= 1e-4
+
+
+def test_headline_accuracy_256_strong_dynamics_1000_steps(gpu):
+    """256^3, 3 variants, A = -0.5 and w_el = 20 (eta grows into O(1) domains), eta0 ~
+    U[-0.05, 0.05], 1000 steps: FP32 fused against FP64 plain."""
+    say("accuracy 256^3 strong dynamics, 1000 steps, fused vs FP64 plain")
+    n, steps = 256, 1000
+    kw = dict(A=-0.5, w_el=20.0, init=(-0.05, 0.05, 3))
+    _, got, Eg, tf = run_mart(n, steps, gsk.PATH_FUSED32, **kw)
+    _, want, Ew, tp = run_mart(n, steps, gsk.PATH_PLAIN64, **kw)
+    r = compare("mart256_strong_1000", got, want, Eg, Ew, steps=steps, fused_s=tf, plain64_s=tp)
+    assert r["max_abs_eta"] > 0.1  # the dynamics left the linear regime
+    assert r["max_abs_d_eta"] <= 1e-4
+    assert r["rel_l2"] <= 1e-3
+    assert r["rel_energy_err"] <= 1e-4
+
+
+def test_config2_accuracy_512_noise_500_steps(gpu):
+    """512^3, 3 variants, config-0 physics with Philox noise of amplitude 0.002, 500 steps:
+    FP32 fused against FP64 plain (same Philox normals on both paths)."""
+    say("accuracy 512^3 noise 0.002, 500 steps, fused vs FP64 plain")
+    n, steps = 512, 500
+    kw = dict(noise=0.002, seed=2)
+    _, got, Eg, tf = run_mart(n, steps, gsk.PATH_FUSED32, **kw)
+    _, want, Ew, tp = run_mart(n, steps, gsk.PATH_PLAIN64, **kw)
+    r = compare("mart512_noise_500", got, want, Eg, Ew, steps=steps, fused_s=tf, plain64_s=tp)
+    assert np.isfinite(got).all()
+    assert r["max_abs_d_eta"] <= 1e-4
+    assert r["rel_l2"] <= 1e-4
+    assert r["rel_energy_err"] <= 1e-4
+
+
+def test_config0_64_against_oracle_1000_steps(gpu):
+    """The 64^3 configuration itself: FP64 plain and FP32 fused against the FP64 oracle after
+    1000 steps (one oracle run serves both)."""
+    say("accuracy 64^3 config 0, 1000 steps, plain and fused vs oracle")
+    n, steps, V = 64, 1000, 3
+    e0p, plain, Ep, _ = run_mart(n, steps, gsk.PATH_PLAIN64)
+    e0f, fused, Ef, _ = run_mart(n, steps, gsk.PATH_FUSED32)
+    assert np.array_equal(e0p, e0f)
+    B = O.btensor(n, c_cfg0(), gsk.mart_eigenstrains(V))
+    p = O.mart_params()
+    t0 = time.perf_counter()
+    want = O.mart_run(e0p, B, p, steps)
+    to = time.perf_counter() - t0
+    Ew = O.mart_energy(want, B, p)
+    rp = compare("mart64_config0_plain64", plain, want, Ep, Ew, oracle_s=to, oracle_threads=O.threads())
+    rf = compare("mart64_config0_fused32", fused, want, Ef, Ew)
+    assert rp["rel_l2"] < 1e-11 and rp["rel_energy_err"] < 1e-11
+    assert rf["max_abs_d_eta"] <= 1e-4 and rf["rel_l2"] < 1e-4 and rf["rel_energy_err"] < 1e-4
