@@ -141,3 +141,40 @@ def test_ensemble_rejects_plain_path(gpu):
     cfg = gsk.default_config(gsk.PHYS_MART, n=16, nvar=2, path=gsk.PATH_PLAIN64)
     with pytest.raises(gsk.GskError):
         gsk.Ensemble([cfg])
+
+
+@pytest.mark.parametrize("mode", ["batched_graph", "autotuned"])
+def test_ensemble_config3_sizes_bit_exact(gpu, mode):
+    """The 96-member workload's three sizes in one ensemble: 2 x 64^3, 2 x 128^3 and 2 x 192^3
+    (192 = 3 * 64 runs the radix-12 plans), with per-member A, kappa, dt and seeds as the
+    workload draws them. Under the batched hipGraph schedule (64^3 members share launches, the
+    others run on the branch streams) and under the schedule the autotuner picks, every member
+    must equal itself stepped alone bit for bit; the first 64^3 member is also checked against
+    the FP64 oracle."""
+    say(f"ensemble 2x64^3 + 2x128^3 + 2x192^3, {mode}")
+    spec = [(64, 0.15, 0.6, 0.005), (128, 0.2, 1.0, 0.01), (192, 0.25, 1.4, 0.008), (64, 0.18, 0.9, 0.007),
+            (128, 0.22, 1.2, 0.006), (192, 0.16, 0.7, 0.009)]
+    cfgs = [gsk.default_config(gsk.PHYS_MART, n=n, nvar=V, A=A, kappa=k, dt=dt, noise=0.002, seed=300 + i, w_el=1.0)
+            for i, (n, A, k, dt) in enumerate(spec)]
+    steps = 7
+    want = alone(cfgs, steps)
+    with gsk.Ensemble(cfgs) as e:
+        e.build_btensor(C, gsk.mart_eigenstrains(V))
+        init_members(e.members)
+        eta0 = e.members[0].get_field("eta").astype(np.float64)
+        if mode == "autotuned":
+            table = e.autotune(trial_steps=2)
+            say(f"  autotuned schedule {e.schedule()} of {len(table)} candidates")
+        else:
+            e.set_schedule(gsk.SCHED_BATCHED_GRAPH, 4, 3)
+        e.step(steps)
+        for i, s in enumerate(e.members):
+            assert s.step_index == steps
+            assert np.array_equal(s.get_field("eta").view(np.uint32), want[i].view(np.uint32)), f"member {i}"
+        got0 = e.members[0].get_field("eta", np.float64)
+    c = cfgs[0]
+    p = O.mart_params(A=c.A, B=c.B, C=c.C, kappa=c.kappa, L=c.L, dt=c.dt, w_el=c.w_el, noise=c.noise, dx=c.dx,
+                      kconv=c.kconv, seed=c.seed)
+    ref = O.mart_run(eta0, O.btensor(c.n, C, gsk.mart_eigenstrains(V)), p, steps)
+    err = float(np.linalg.norm((got0 - ref).ravel()) / np.linalg.norm(ref.ravel()))
+    assert err < 2e-6, err

@@ -149,3 +149,33 @@ def _stress_default_modulus():
             for q in ((a, b), (b, a)):
                 C[p + q] = v
     return C
+
+
+def test_gsi_restart_with_ahead_steps(gpu, tmp_path):
+    """Restart through a dump (src/gs.cu:183-184): `read eta` of the binary dump at step 5, then
+    `set ahead_steps 5` and `run 5`, must reproduce the uninterrupted run's step-10 dump.
+
+    The fused step's state is the FP32 k-space field, of which the dumped real field is the
+    rounded inverse transform, so the restart agrees to FP32 rounding (a few ulp), not bit for
+    bit. The noise counter must resume at step 5: a restart without ahead_steps draws step 0's
+    normals again and is off by the noise increment, far above the rounding."""
+    say("gsi restart through ahead_steps")
+    n, V = 32, 3
+    eps = gsk.mart_eigenstrains(V)
+    head = (f"sys mart\nset gridsize {n} {n} {n} 1 1 1\nset deltatime 0.01\nset coefficient -0.5 3 3\n"
+            f"set weightelastic 20\nset noiseamp 0.002\nset seed 9\nset cubicmodulus 0.7 0.3 0.2\n"
+            f"readhere varianttensor\n{tensor_lines(eps)}set binary on\n")
+    run_gsi(tmp_path, head + "set initeta uniform -0.05 0.05 4\ndump full 2 eta\nrun 10\n")
+    full5 = read_binary(tmp_path / "full" / "eta.5.data")
+    full10 = read_binary(tmp_path / "full" / "eta.10.data")
+    run_gsi(tmp_path, head + "read eta full/eta.5.data\nset ahead_steps 5\ndump again 1 eta\nrun 5\n")
+    again = read_binary(tmp_path / "again" / "eta.10.data")
+    run_gsi(tmp_path, head + "read eta full/eta.5.data\ndump wrong 1 eta\nrun 5\n")
+    wrong = read_binary(tmp_path / "wrong" / "eta.5.data")
+    scale = float(np.abs(full10).max())
+    d_again = float(np.abs(again - full10).max())
+    d_wrong = float(np.abs(wrong - full10).max())
+    say(f"  restart max|d| {d_again:.3e}, without ahead_steps {d_wrong:.3e}, max|eta| {scale:.3e}")
+    assert not np.array_equal(full5, full10)
+    assert d_again <= 2e-6 * scale
+    assert d_wrong > 1e-4 * scale
