@@ -72,7 +72,7 @@ def measure_traffic(a, slot):
         cmd = [prof, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
                os.path.abspath(__file__), "--workload", a.workload, "--n", str(a.n), "--nvar", str(a.nvar),
                "--noise", str(a.noise), "--steps", "2", "--warmup", "1", "--profile-steps", "1", "--no-cpu",
-               "--no-plain", "--no-traffic", "--calib-copy-mib", str(CALIB_MIB)]
+               "--no-plain", "--no-traffic", "--repeats", "1", "--calib-copy-mib", str(CALIB_MIB)]
         try:
             subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.DEVNULL,
                            stderr=subprocess.DEVNULL, timeout=150, check=True)
@@ -112,25 +112,64 @@ def timed(sim, steps, torch):
     return (time.perf_counter() - t0) * 1e3, ev_ms
 
 
-def cpu_baseline(n, nvar, steps):
+def timed_repeats(sim, steps, repeats, torch):
+    """`repeats` timed passes of exactly `steps` steps each; returns the pass of median wall time
+    (wall ms, event ms) and every pass's wall ms."""
+    runs = [timed(sim, steps, torch) for _ in range(max(1, repeats))]
+    order = sorted(range(len(runs)), key=lambda i: runs[i][0])
+    med = runs[order[len(runs) // 2]]
+    return med[0], med[1], [r[0] for r in runs]
+
+
+def repeat_stats(walls, steps):
+    ms = sorted(w / steps for w in walls)
+    return {"count": len(ms), "ms_per_step": [round(x, 4) for x in ms],
+            "spread": round((ms[-1] - ms[0]) / ms[len(ms) // 2], 4)}
+
+
+def cpu_threads():
+    """Threads the host offers this process: OMP_NUM_THREADS when set (the GPU box sets it to its
+    CPU share), else the CPUs in the process's affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(n, nvar, budget_s=8.0, max_steps=10):
+    """The FP64 oracle (C++/OpenMP, its own FFT) on the host cores, on a bounded sample of the
+    same workload: steps are run one at a time until `budget_s` seconds have passed (at most
+    `max_steps`) on every offered thread, then one step on a single thread."""
     import numpy as np
 
     import gsk
     import oracle as O
 
-    threads = min(os.cpu_count() or 1, 16)
-    O.set_threads(threads)
+    threads = cpu_threads()
     C = gsk.cubic_modulus(0.7, 0.3, 0.2)
     eps = gsk.mart_eigenstrains(nvar)
+    O.set_threads(threads)
     B = O.btensor(n, C, eps)
     eta = np.stack([O.uniform_field(0, v, n**3, -0.01, 0.01).reshape(n, n, n) for v in range(nvar)]).astype(np.float64)
     p = O.mart_params()
+    steps, dt = 0, 0.0
+    while steps < max_steps and dt < budget_s:
+        t0 = time.perf_counter()
+        eta = O.mart_run(eta, B, p, 1, step0=steps)
+        dt += time.perf_counter() - t0
+        steps += 1
+    O.set_threads(1)
     t0 = time.perf_counter()
-    O.mart_run(eta, B, p, steps)
-    dt = time.perf_counter() - t0
+    O.mart_run(eta, B, p, 1, step0=steps)
+    dt1 = time.perf_counter() - t0
+    O.set_threads(threads)
     return {"value": n**3 * steps / dt / 1e6, "unit": "Mvoxel-steps/s", "cores": threads, "kind": "port",
             "sample": f"{steps} steps of the {n}^3 {nvar}-variant martensite step in the FP64 oracle "
-                      f"(C++/OpenMP, own FFT), {dt:.1f} s"}
+                      f"(C++/OpenMP, own FFT) on {threads} threads, {dt:.1f} s; then 1 step on 1 thread, {dt1:.1f} s",
+            "single_thread_value": n**3 / dt1 / 1e6, "nproc": os.cpu_count(), "threads_offered": threads}
 
 
 def ensemble_configs(gsk, master_seed=3, per_size=32, sizes=(64, 128, 192), nvar=3):
@@ -220,7 +259,7 @@ def run_prec(a):
     s.init_uniform("eta", 0.0, 0.01, 12)
     m0 = s.mass()
     s.step(a.warmup)
-    wall_ms, ev_ms = timed(s, a.steps, torch)
+    wall_ms, ev_ms, walls = timed_repeats(s, a.steps, a.repeats, torch)
     ms = wall_ms / a.steps
     names, kms = s.kernel_times(a.profile_steps)
     kbytes = s.kernel_bytes()
@@ -239,6 +278,7 @@ def run_prec(a):
                    "n": n, "nvar": V, "parallelism": "single GPU"},
         "hbm_frac_canonical": round(canon / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "event_ms_per_step": round(ev_ms / a.steps, 4),
+        "repeats": repeat_stats(walls, a.steps),
         "kernels": {nm: {"ms": round(t, 4), "GBps": round(b / (t / 1e3) / 1e9, 1)} for nm, t, b in per},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(dom[2] / (dom[1] / 1e3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom[2] / (dom[1] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -262,7 +302,8 @@ def main():
     ap.add_argument("--nvar", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--noise", type=float, default=0.0, help="noise amplitude (0.002 for the 512^3 configuration)")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="time budget of the multi-threaded CPU sample")
+    ap.add_argument("--repeats", type=int, default=3, help="timed passes of --steps steps; the median is reported")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
@@ -296,7 +337,7 @@ def main():
     log(f"fused32 n={n} V={V}")
     s = make_sim(gsk, n, V, gsk.PATH_FUSED32, noise=a.noise)
     s.step(a.warmup)
-    wall_ms, ev_ms = timed(s, a.steps, torch)
+    wall_ms, ev_ms, walls = timed_repeats(s, a.steps, a.repeats, torch)
     ms = wall_ms / a.steps
     names, kms = s.kernel_times(a.profile_steps)
     kbytes = s.kernel_bytes()
@@ -329,6 +370,7 @@ def main():
         "hbm_frac_canonical": round(canon_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "hbm_frac_schedule": round(sched_bytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "event_ms_per_step": round(ev_ms / a.steps, 4),
+        "repeats": repeat_stats(walls, a.steps),
         "kernels": {nm: {"ms": round(t, 4), "GBps": round(b / (t / 1e3) / 1e9, 1)} for nm, t, b in per},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(dom[2] / (dom[1] / 1e3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -346,7 +388,7 @@ def main():
         result["speedup_vs_fp64_plain"] = round(pms / ms, 2)
     if not a.no_cpu:
         log("cpu baseline")
-        result["cpu_baseline"] = cpu_baseline(n, V, a.cpu_steps)
+        result["cpu_baseline"] = cpu_baseline(n, V, budget_s=a.cpu_seconds)
     if a.calib_copy_mib:
         result["calib_copy_bytes"] = calib_copy(torch, a.calib_copy_mib)
     if not a.no_traffic:

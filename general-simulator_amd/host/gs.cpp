@@ -147,6 +147,17 @@ std::string num_to_string(double v) {
 //   * /
 //   ^ (integer power as the reference's Real^int, src/variable.cu:194)
 //   unary -, parentheses, numbers, names
+// Each value carries its text as the reference's ExprTree does (src/variable.cu:139-209): a
+// variable or a number literal keeps its text verbatim (so `:t=`date +%s`` stores all ten
+// digits and {t} prints them), an operation's text is its double printed with the stream's
+// default six significant digits, and `=` stores the right-hand side's text.
+struct EVal {
+  double v = NAN;
+  std::string t;
+};
+
+static EVal ev_num(double v) { return EVal{v, num_to_string(v)}; }
+
 struct Expr {
   const std::string& s;
   size_t p = 0;
@@ -169,7 +180,7 @@ struct Expr {
     while (p < s.size() && (std::isalnum((unsigned char)s[p]) || s[p] == '_' || s[p] == '.')) ++p;
     return s.substr(b, p - b);
   }
-  double lookup(const std::string& nm) {
+  EVal lookup(const std::string& nm) {
     auto it = vars.find(nm);
     std::string key = nm;
     if (it == vars.end()) {
@@ -177,78 +188,82 @@ struct Expr {
       it = gvars.find(key);
       if (it == gvars.end()) {
         err = "\"" + nm + "\" unknown";
-        return NAN;
+        return EVal{};
       }
     }
     std::istringstream ss(it->second);
-    double v = NAN;
-    ss >> v;
-    return v;
+    EVal r;
+    ss >> r.v;
+    if (ss.fail()) r.v = NAN;
+    std::istringstream ws_(it->second);
+    ws_ >> r.t;  // the first word, as the reference's `>>=` reads it
+    return r;
   }
-  double primary() {
+  EVal primary() {
     ws();
-    if (p >= s.size()) { err = "incomplete expression"; return NAN; }
+    if (p >= s.size()) { err = "incomplete expression"; return EVal{}; }
     if (s[p] == '(') {
       ++p;
-      double v = assign();
+      EVal v = assign();
       if (!eat(")")) err = "missing )";
       return v;
     }
     if (std::isdigit((unsigned char)s[p]) || s[p] == '.') {
       char* end = nullptr;
+      const size_t b = p;
       double v = std::strtod(s.c_str() + p, &end);
       p = end - s.c_str();
-      return v;
+      return EVal{v, s.substr(b, p - b)};
     }
     std::string nm = name();
-    if (nm.empty()) { err = std::string("unexpected '") + s[p] + "'"; ++p; return NAN; }
+    if (nm.empty()) { err = std::string("unexpected '") + s[p] + "'"; ++p; return EVal{}; }
     return lookup(nm);
   }
-  double unary() {
-    if (eat("-")) return -unary();
+  EVal unary() {
+    if (eat("-")) return ev_num(-unary().v);
     if (eat("+")) return unary();
     return primary();
   }
-  double power() {
-    double a = unary();
+  EVal power() {
+    EVal a = unary();
     if (eat("^")) {
-      double b = power();
+      double b = power().v;
       int e = (int)b;
       double r = 1.0;
-      for (int i = 0; i < std::abs(e); ++i) r *= a;
-      return e < 0 ? 1.0 / r : r;
+      for (int i = 0; i < std::abs(e); ++i) r *= a.v;
+      return ev_num(e < 0 ? 1.0 / r : r);
     }
     return a;
   }
-  double mul() {
-    double a = power();
+  EVal mul() {
+    EVal a = power();
     for (;;) {
-      if (eat("*")) a *= power();
-      else if (eat("/")) { double b = power(); a = (b == 0) ? NAN : a / b; }
+      if (eat("*")) a = ev_num(a.v * power().v);
+      else if (eat("/")) { double b = power().v; a = (b == 0) ? EVal{NAN, "nan"} : ev_num(a.v / b); }
       else return a;
     }
   }
-  double add() {
-    double a = mul();
+  EVal add() {
+    EVal a = mul();
     for (;;) {
       ws();
       if (p + 1 < s.size() && (s[p] == '+' || s[p] == '-') && s[p + 1] == '=') return a;
-      if (eat("+")) a += mul();
-      else if (eat("-")) a -= mul();
+      if (eat("+")) a = ev_num(a.v + mul().v);
+      else if (eat("-")) a = ev_num(a.v - mul().v);
       else return a;
     }
   }
-  double cmp() {
-    double a = add();
-    if (eat("==")) return a == add();
-    if (eat("!=")) return a != add();
-    if (eat("<=")) return a <= add();
-    if (eat(">=")) return a >= add();
-    if (eat("<")) return a < add();
-    if (eat(">")) return a > add();
+  EVal cmp() {
+    EVal a = add();
+    if (eat("==")) return ev_num(a.v == add().v);
+    if (eat("!=")) return ev_num(a.v != add().v);
+    if (eat("<=")) return ev_num(a.v <= add().v);
+    if (eat(">=")) return ev_num(a.v >= add().v);
+    if (eat("<")) return ev_num(a.v < add().v);
+    if (eat(">")) return ev_num(a.v > add().v);
     return a;
   }
-  double assign() {
+  EVal assign() {
     size_t save = p;
     std::string nm = name();
     ws();
@@ -256,37 +271,39 @@ struct Expr {
       const char* ops[] = {"+=", "-=", "*=", "/="};
       for (const char* op : ops)
         if (eat(op)) {
-          double cur = lookup(nm), r = assign();
+          double cur = lookup(nm).v, r = assign().v;
           double v = op[0] == '+' ? cur + r : op[0] == '-' ? cur - r : op[0] == '*' ? cur * r : cur / r;
-          store(nm, v);
-          return v;
+          EVal out = ev_num(v);
+          store(nm, out.t);
+          return out;
         }
       if (s[p] == '=' && (p + 1 >= s.size() || s[p + 1] != '=')) {
         ++p;
-        double v = assign();
-        store(nm, v);
+        EVal v = assign();
+        store(nm, v.t);
         return v;
       }
     }
     p = save;
     return cmp();
   }
-  void store(const std::string& nm, double v) {
-    if (!nm.empty() && nm[0] == '$') gvars[nm.substr(1)] = num_to_string(v);
-    else vars[nm] = num_to_string(v);
+  void store(const std::string& nm, const std::string& text) {
+    if (!nm.empty() && nm[0] == '$') gvars[nm.substr(1)] = text;
+    else vars[nm] = text;
   }
 };
 
-int eval_expr(const std::string& text, VarMap& vars, VarMap& gvars, double* out) {
+int eval_expr(const std::string& text, VarMap& vars, VarMap& gvars, double* out, std::string* out_text) {
   Expr e(text, vars, gvars);
-  double v = e.assign();
+  EVal v = e.assign();
   e.ws();
   if (e.err.empty() && e.p != text.size()) e.err = "trailing characters in \"" + text + "\"";
   if (!e.err.empty()) {
     log_line("Error: " + e.err);
     return Code_ERR;
   }
-  *out = v;
+  if (out) *out = v.v;
+  if (out_text) *out_text = v.t;
   return Code_NORMAL;
 }
 
@@ -572,9 +589,9 @@ int Interpreter::substitute(std::string& s) {
       log_line("Error: uncomplete expression");
       return Code_ERR;
     }
-    double v;
-    if (eval_expr(s.substr(l + 1, r - l - 1), vars_, gs_.Vars, &v)) return Code_ERR;
-    s.replace(l, r - l + 1, num_to_string(v));
+    std::string t;
+    if (eval_expr(s.substr(l + 1, r - l - 1), vars_, gs_.Vars, nullptr, &t)) return Code_ERR;
+    s.replace(l, r - l + 1, t);
   }
   return Code_NORMAL;
 }
@@ -713,7 +730,7 @@ int Interpreter::exec(std::string& script) {
     std::getline(ls, rest);
     if (!rest.empty() && rest[0] == ' ') rest.erase(0, 1);
     if (!cmd.empty() && cmd[0] == ':') {
-      rest = cmd.substr(1) + rest;
+      rest = line.substr(1);  // the whole line after the colon, blanks kept
       cmd = "expr";
     }
     if (cmd == "loop" || cmd == "while" || cmd == "if") {
@@ -726,7 +743,9 @@ int Interpreter::exec(std::string& script) {
       }
       continue;
     }
-    if (cmd != "print" && cmd != "expr") {
+    // every line but print has `cmd` and {expr} substituted before dispatch, expressions
+    // (`:x=...` lines) included (src/input.cu:119-120, HEAD side)
+    if (cmd != "print") {
       if (substitute(rest)) return Code_ERR;
     }
     if (log_echo_on()) log_line(">>>" + raw);

@@ -10,7 +10,9 @@
 
 namespace gs {
 
-int eval_expr(const std::string& text, VarMap& vars, VarMap& gvars, double* out);
+// Evaluates `text`; *out gets the value and *out_text (may be null) its text (a variable's stored
+// text or a literal verbatim, an operation's value with six significant digits).
+int eval_expr(const std::string& text, VarMap& vars, VarMap& gvars, double* out, std::string* out_text = nullptr);
 std::string num_to_string(double v);
 void log_open(const std::string& file);
 void log_echo(bool on);

@@ -168,3 +168,48 @@ def test_cores_random_is_reproducible(tmp_path):
     assert set(np.unique(eta)) <= {0.0, 1.0}
     assert np.all(c[inside] == np.float32(0.44)) and np.all(c[~inside] == np.float32(0.2))
     assert 0 < inside.sum() <= 3 * 33  # at most three spheres of radius 2 (33 lattice points each)
+
+
+REF_SCRIPTS = "/root/reference/scripts"
+
+
+def run_ref_script(tmp_path, name):
+    path = os.path.join(REF_SCRIPTS, name)
+    if not os.path.exists(path):
+        pytest.skip(f"reference script {name} not present")
+    r = subprocess.run([GSI, path], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return [l for l in r.stdout.splitlines() if not l.startswith((">>>", "Input script"))]
+
+
+def test_reference_script_if_loop_while(tmp_path):
+    """scripts/in.if.loop.while unmodified: loop (a*2) runs 6 times counting a down from 3, while
+    a>0 counts down to 0, and the if takes its else branch (a^2 = 4)."""
+    out = run_ref_script(tmp_path, "in.if.loop.while")
+    assert out == ["3", "2", "1", "0", "-1", "-2", "-3", "2", "1", "0", "0", "4"]
+
+
+def test_reference_script_time_calculate(tmp_path):
+    """scripts/in.time_calculate unmodified: backticks on `:x=` lines store the shell's output
+    verbatim (all ten digits of `date +%s`), and {x2-x1} is the 3 s of `shell sleep 3`."""
+    import re
+
+    out = run_ref_script(tmp_path, "in.time_calculate")
+    last = out[-1]
+    m = re.match(r"since time point (\d{10}) to time point (\d{10}),?\s+(\d+) seconds have passed\.", last)
+    assert m, out
+    t1, t2, d = (int(g) for g in m.groups())
+    assert t2 - t1 == d and d in (3, 4)
+
+
+def test_reference_script_shell(tmp_path):
+    out = run_ref_script(tmp_path, "in.shell")
+    assert len(out) == 2 and all(l.strip().isdigit() for l in out)
+
+
+def test_expression_text_is_kept_verbatim(tmp_path):
+    """A variable assigned from a literal or a shell result keeps its text; operations print
+    six significant digits, as the reference's stream formatting does."""
+    rc, out = run(tmp_path, ":big=1234567891\nprint {big}\n:c=big\nprint {c} {big+1} {1/3}\n:s=`echo 0012`\nprint {s}\n")
+    assert rc == 0, out
+    assert "\n1234567891\n" in out and "1234567891 1.23457e+09 0.333333" in out and "\n0012\n" in out
