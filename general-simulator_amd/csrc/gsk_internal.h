@@ -1,0 +1,546 @@
+// Internal declarations of the library (libgsk.so) shared by its translation units:
+//   gsk_api.hip       C ABI: handles, fields, steps, reductions, timing, noise, B tensor, FFTs
+//   gsk_impl.hip      the per-size step schedules and kernel launches (one object per grid edge)
+//   gsk_ensemble.hip  ensembles: schedules, batched launches, hipGraph capture, autotuner
+//   gsk_adjoint.hip   checkpointed reverse mode
+//   gsk_stress.hip    the stress plugin's transform
+//   fft_compare.hip   the rocFFT comparator
+#pragma once
+#include "gsk.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "kernels_adjoint.h"
+#include "kernels_aux.h"
+#include "kernels_plane.h"
+#include "kernels_ref.h"
+
+using namespace gsk;
+
+
+// the message of the last failure on this thread (defined once, in gsk_api.hip)
+std::string& err_text();
+
+inline int fail(int code, const std::string& msg) {
+  err_text() = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                                  \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess)                                                                             \
+      return fail(GSK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + \
+                                   ":" + std::to_string(__LINE__) + ")");                             \
+  } while (0)
+
+#define CHK(expr)                  \
+  do {                             \
+    int rc_ = (expr);              \
+    if (rc_ != GSK_OK) return rc_; \
+  } while (0)
+
+constexpr double kPi = 3.14159265358979323846;
+
+inline bool supported_n(int n) {
+  switch (n) {
+    case 16: case 32: case 64: case 96: case 128: case 192: case 256: case 512: return true;
+    default: return false;
+  }
+}
+
+inline int rmin_of(int N) {
+  switch (N) {
+    case 8: return PlanOf<8>::rmin;
+    case 16: return PlanOf<16>::rmin;
+    case 32: return PlanOf<32>::rmin;
+    case 48: return PlanOf<48>::rmin;
+    case 64: return PlanOf<64>::rmin;
+    case 96: return PlanOf<96>::rmin;
+    case 128: return PlanOf<128>::rmin;
+    case 192: return PlanOf<192>::rmin;
+    case 256: return PlanOf<256>::rmin;
+    case 512: return PlanOf<512>::rmin;
+  }
+  return 1;
+}
+
+inline int round64(int t) { return ((t + 63) / 64) * 64; }
+
+// Launch shapes of the strided passes: GSK_Y_PERFIELD gives every field its own y-pass
+// work groups; GSK_XUPD_G is the largest field group of one x-update launch.
+#ifndef GSK_Y_PERFIELD
+#define GSK_Y_PERFIELD 1
+#endif
+#ifndef GSK_XUPD_G
+#define GSK_XUPD_G 0
+#endif
+
+#ifndef GSK_FREV
+#define GSK_FREV 1
+#endif
+#ifndef GSK_TREV
+#define GSK_TREV 0
+#endif
+
+// fields in the next martensite x-update launch when `left` fields remain
+inline int xupd_group(int left) {
+  if (GSK_XUPD_G == 0) return left;  // a single launch covers every field
+  return (GSK_XUPD_G >= 3 && left >= 3) ? 3 : ((GSK_XUPD_G >= 2 && left >= 2) ? 2 : 1);
+}
+
+// block size of the z kernels for half-length M
+inline int zthreads_host(int M) {
+  switch (M) {
+    case 8: return zthreads<8>();
+    case 16: return zthreads<16>();
+    case 32: return zthreads<32>();
+    case 48: return zthreads<48>();
+    case 64: return zthreads<64>();
+    case 96: return zthreads<96>();
+    case 128: return zthreads<128>();
+    case 256: return zthreads<256>();
+  }
+  return 0;
+}
+
+template <typename T2>
+std::vector<T2> twiddles(int N, int denom) {
+  std::vector<T2> t(N);
+  for (int m = 0; m < N; ++m) {
+    double a = -2.0 * kPi * m / denom;
+    t[m].x = (decltype(t[m].x))std::cos(a);
+    t[m].y = (decltype(t[m].x))std::sin(a);
+  }
+  return t;
+}
+
+
+struct Slot {
+  const char* name;
+  double bytes;  // algorithmic HBM bytes per step
+};
+
+struct AdjWS;  // adjoint workspace (checkpoints and buffers), defined with the adjoint entry points
+
+struct gsk_sim {
+  gsk_config cfg{};
+  int n = 0, V = 0, F = 0, dev = 0;
+  Geo g{};
+  int zthreads = 0, xthreads = 0;
+  size_t lds_z = 0, lds_y = 0, lds_xc = 0;
+  hipStream_t stream = nullptr;
+  // twiddles
+  float2 *twN = nullptr, *twM = nullptr, *rtn = nullptr;
+  double2* twN64 = nullptr;
+  // fused FP32 buffers
+  float* fld = nullptr;
+  float2* kbuf = nullptr;
+  float2* state = nullptr;
+  float* Bt = nullptr;
+  // plain FP64 buffers
+  double* eta64 = nullptr;
+  double2 *S64 = nullptr, *R64 = nullptr, *E64 = nullptr, *F64 = nullptr;
+  double* B64 = nullptr;
+  int pL = 0, pW = 0, pzthreads = 0, pxthreads = 0;
+  size_t lds_pz = 0, lds_px = 0;
+  // reductions
+  double* part = nullptr;
+  double* rout = nullptr;
+  uint32_t* dstep = nullptr;  // fused path: device copy of the step index (noise counter)
+  unsigned gen = 0;           // bumped whenever a value baked into kernel arguments changes
+  BArgs bargs{};              // inputs of the last B tensor build (the adjoint rebuilds B in its own layout)
+  bool haveBargs = false;
+  AdjWS* adj = nullptr;
+  // reference rules
+  float* rbuf = nullptr;      // V real fields: elastic R, then the new eta (martensite)
+  float* defect = nullptr;    // n^3 blocking field (martensite reference rule), optional
+  float* Kd = nullptr;        // V*n^3 dislocation force constants, optional
+  double eps[72] = {};        // eigenstrains of the last B build
+  bool haveEps = false;
+  bool sharedB = false;       // Bt belongs to an ensemble, not to this handle
+  int rblocks = 512;
+  // state
+  double cbar = 0.0;
+  double c_k0 = 0.0;  // FP64 sum of the stored concentration fluctuation (its k = 0 mode)
+  double cee0 = 0.0;  // B(k = 0) of the c-sourced tensor
+  bool haveB = false, dirty = true;
+  // martensite fused step: the k-space buffer holds the z-transformed eta^2 of the current
+  // fields (left there by the last step's fused inverse/square pass); cleared by anything that
+  // writes the fields or uses the buffer otherwise
+  bool sq_ready = false;
+  int order = 0;  // visiting order of the last per-field launch
+  long long step = 0;
+  // profiling
+  bool prof = false;
+  std::vector<hipEvent_t> ev;
+  int evi = 0;
+  std::vector<Slot> slots;
+};
+
+
+// Geometry for the next per-field launch: flips the field / tile visiting order each time.
+inline Geo geo_next(gsk_sim* s) {
+  Geo g = s->g;
+  s->order ^= 1;
+  g.frev = GSK_FREV ? s->order : 0;
+  g.trev = GSK_TREV ? s->order : 0;
+  return g;
+}
+
+// Plane-resident passes of the spectral martensite step (n = 32, 64, 128 and 1..4 variants):
+// k_plane_mid replaces y inverse -> middle z pass -> y forward, k_plane_sq the y inverse -> last
+// z pass -> next step's y forward, so the k-space buffer carries the y,z-transformed eta^2
+// between steps (z-transformed only at the other sizes). GSK_PLANE=0 in the environment keeps
+// the chain of y and z kernels.
+inline bool plane_on(const gsk_sim* s) {
+  static const bool off = [] {
+    const char* e = std::getenv("GSK_PLANE");
+    return e && e[0] == '0';
+  }();
+  return !off && (s->n == 32 || s->n == 64 || s->n == 128) && s->F >= 1 && s->F <= 4 &&
+         s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL;
+}
+
+inline size_t nvox(const gsk_sim* s) { return (size_t)s->n * s->n * s->n; }
+inline size_t hs_count(const gsk_sim* s) { return (size_t)s->n * s->n * (s->n / 2 + 1); }
+// State element of field 0 at k = 0: first main x tile (after the n/W Nyquist tiles), kx = 0, column 0.
+inline size_t k0_index(const gsk_sim* s) { return (size_t)(s->n / s->g.W) * s->F * s->n * s->g.W; }
+
+inline int mark(gsk_sim* s) {
+  if (s->prof) {
+    if (s->evi >= (int)s->ev.size()) return fail(GSK_ERR, "profiling event overflow");
+    HIPCHK(hipEventRecord(s->ev[s->evi++], s->stream));
+  }
+  return GSK_OK;
+}
+
+#define LAUNCH(kern, grid, threads, lds, s, ...)                                              \
+  do {                                                                                      \
+    CHK(mark(s));                                                                           \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, (s)->stream, __VA_ARGS__);     \
+    HIPCHK(hipGetLastError());                                                              \
+  } while (0)
+
+inline MartK mart_consts(const gsk_sim* s) {
+  const gsk_config& c = s->cfg;
+  MartK k{};
+  k.A = (float)c.A; k.B = (float)c.B; k.C = (float)c.C;
+  k.two_wel = (float)(2.0 * c.w_el);
+  k.noise = (float)c.noise;
+  k.dtL = (float)(c.dt * c.L);
+  k.dtLk = (float)(c.dt * c.L * c.kappa);
+  k.kfac = (float)(c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx));
+  k.inv_n3 = (float)(1.0 / ((double)s->n * s->n * s->n));
+  k.F = s->V;
+  k.npq = s->V * (s->V + 1) / 2;
+  k.step = (uint32_t)s->step;
+  k.stepp = s->dstep;
+  k.seed = c.seed;
+  return k;
+}
+
+inline PrecK prec_consts(const gsk_sim* s) {
+  const gsk_config& c = s->cfg;
+  PrecK k{};
+  k.A1 = (float)c.a[1]; k.A2 = (float)c.a[2]; k.A3 = (float)c.a[3]; k.A4 = (float)c.a[4];
+  k.A5 = (float)c.a[5]; k.A6 = (float)c.a[6]; k.A7 = (float)c.a[7];
+  k.c1 = (float)c.c1; k.c2 = (float)c.c2;
+  k.cbar = (float)s->cbar;
+  k.k0 = (float)s->c_k0;
+  k.noise = (float)c.noise;
+  k.dtM = (float)(c.dt * c.M);
+  k.dtMkc = (float)(c.dt * c.M * c.kappa_c);
+  k.dtL = (float)(c.dt * c.L);
+  k.dtLke = (float)(c.dt * c.L * c.kappa);
+  k.xi = (float)c.xi;
+  k.S = c.rule == GSK_RULE_REFERENCE ? 0.f : (float)c.S;
+  k.celastic = c.rule == GSK_RULE_REFERENCE ? 0 : 1;
+  k.kfac = (float)(c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx));
+  k.inv_n3 = (float)(1.0 / ((double)s->n * s->n * s->n));
+  k.V = s->V;
+  k.step = (uint32_t)s->step;
+  k.stepp = s->dstep;
+  k.seed = c.seed;
+  return k;
+}
+
+inline PlainK plain_consts(const gsk_sim* s) {
+  const gsk_config& c = s->cfg;
+  PlainK k{};
+  k.A = c.A; k.B = c.B; k.C = c.C;
+  k.two_wel = 2.0 * c.w_el;
+  k.noise = c.noise;
+  k.dtL = c.dt * c.L;
+  k.dtLk = c.dt * c.L * c.kappa;
+  k.kfac = (c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx));
+  k.inv_n3 = 1.0 / ((double)s->n * s->n * s->n);
+  k.V = s->V;
+  k.step = (uint32_t)s->step;
+  k.seed = c.seed;
+  return k;
+}
+
+inline int grid_1d(long work, int threads = 256) {
+  long b = (work + threads - 1) / threads;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+// Allow a kernel the device's whole LDS for dynamic shared memory (minus its static part).
+inline int lds_limit(const void* f) {
+  int dev = 0, optin = 0, maxs = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&maxs, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && optin > maxs)
+    maxs = optin;
+  hipFuncAttributes a;
+  HIPCHK(hipFuncGetAttributes(&a, f));
+  HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, maxs - (int)a.sharedSizeBytes));
+  return GSK_OK;
+}
+
+// Steps whose last z pass leaves the first k-space stage of the next step in the k-space buffer
+// (the spectral rules on the fused path; the precipitate with 2..5 fields).
+inline bool prec_carries(const gsk_sim* s) {
+  return s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_PREC && s->cfg.rule == GSK_RULE_SPECTRAL &&
+         s->F >= 2 && s->F <= 5;
+}
+inline bool carries(const gsk_sim* s) {
+  return prec_carries(s) || (s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART &&
+                             s->cfg.rule == GSK_RULE_SPECTRAL);
+}
+// Entry points of the per-size implementation (gsk_impl.hip, compiled once per grid edge N with
+// -DGSK_IMPL_N=N; each object file holds the explicit instantiations of its N).
+template <int N> int impl_set_attrs();
+template <int N> int impl_init_state(gsk_sim* s);
+template <int N> int impl_square_fwd(gsk_sim* s);
+template <int N> int impl_step_batched(gsk_sim* s0, int cnt, const MemberPtrs* tab, hipStream_t st);
+template <int N> int impl_step(gsk_sim* s);
+template <int N> int impl_energy_el_fused(gsk_sim* s, double* el);
+template <int N> int impl_energy_el_plain(gsk_sim* s, double* gr, double* el);
+template <int N> int impl_zfwd(gsk_sim* s, const float* in, int F);
+template <int N> int impl_ypass(gsk_sim* s, bool fwd, int F);
+template <int N> int impl_xpass(gsk_sim* s, bool fwd, int F, float scale);
+template <int N> int impl_adj_step(gsk_sim* s, const float* eta, const float* eta1, float* lam, float* mu, float* rq,
+                                   float2* kb, float2* k2, const float* bh, double* part, int nb, const AdjK& k);
+template <int N> int impl_r2c(gsk_sim* s, const float* in, float2* kb, int F);
+template <int N> int impl_c2r(gsk_sim* s, float2* kb, float* out, int F);
+template <int N> int impl_zinv(gsk_sim* s, float* out, int F);
+
+template <template <int> class Fn, typename... Args>
+inline int dispatch_n(int n, Args&&... args) {
+  switch (n) {
+    case 16: return Fn<16>::run(args...);
+    case 32: return Fn<32>::run(args...);
+    case 64: return Fn<64>::run(args...);
+    case 96: return Fn<96>::run(args...);
+    case 128: return Fn<128>::run(args...);
+    case 192: return Fn<192>::run(args...);
+    case 256: return Fn<256>::run(args...);
+    case 512: return Fn<512>::run(args...);
+  }
+  return fail(GSK_ERR_UNSUPPORTED, "unsupported grid size " + std::to_string(n));
+}
+
+template <int N> struct DAttrs { static int run() { return impl_set_attrs<N>(); } };
+template <int N> struct DInit { static int run(gsk_sim* s) { return impl_init_state<N>(s); } };
+template <int N> struct DSquare { static int run(gsk_sim* s) { return impl_square_fwd<N>(s); } };
+template <int N> struct DBatch {
+  static int run(gsk_sim* s0, int cnt, const MemberPtrs* tab, hipStream_t st) {
+    return impl_step_batched<N>(s0, cnt, tab, st);
+  }
+};
+template <int N> struct DStep { static int run(gsk_sim* s) { return impl_step<N>(s); } };
+template <int N> struct DElF { static int run(gsk_sim* s, double* el) { return impl_energy_el_fused<N>(s, el); } };
+template <int N> struct DElP {
+  static int run(gsk_sim* s, double* gr, double* el) { return impl_energy_el_plain<N>(s, gr, el); }
+};
+template <int N> struct DZfwd { static int run(gsk_sim* s, const float* in, int F) { return impl_zfwd<N>(s, in, F); } };
+template <int N> struct DY { static int run(gsk_sim* s, bool fwd, int F) { return impl_ypass<N>(s, fwd, F); } };
+template <int N> struct DX {
+  static int run(gsk_sim* s, bool fwd, int F, float scale) { return impl_xpass<N>(s, fwd, F, scale); }
+};
+template <int N> struct DAdj {
+  static int run(gsk_sim* s, const float* eta, const float* eta1, float* lam, float* mu, float* rq, float2* kb,
+                 float2* k2, const float* bh, double* part, int nb, const AdjK& k) {
+    return impl_adj_step<N>(s, eta, eta1, lam, mu, rq, kb, k2, bh, part, nb, k);
+  }
+};
+template <int N> struct DR2C {
+  static int run(gsk_sim* s, const float* in, float2* kb, int F) { return impl_r2c<N>(s, in, kb, F); }
+};
+template <int N> struct DC2R {
+  static int run(gsk_sim* s, float2* kb, float* out, int F) { return impl_c2r<N>(s, kb, out, F); }
+};
+template <int N> struct DZinv { static int run(gsk_sim* s, float* out, int F) { return impl_zinv<N>(s, out, F); } };
+
+inline int ensure_device(int dev) {
+  int cnt = 0;
+  HIPCHK(hipGetDeviceCount(&cnt));
+  if (cnt <= 0) return fail(GSK_ERR_HIP, "no HIP device available");
+  if (dev < 0 || dev >= cnt) return fail(GSK_ERR_ARG, "device ordinal out of range");
+  HIPCHK(hipSetDevice(dev));
+  return GSK_OK;
+}
+
+// Choose tile shapes for a given field count F.
+inline void choose_geometry(gsk_sim* s, int Fz, int Fx_resident) {
+  const int n = s->n, M = n / 2;
+  Geo& g = s->g;
+  g.n = n; g.M = M;
+  g.nvox = (long long)n * n * n;
+  g.nq = (long long)n * n * M;
+  g.kfs = ((long long)n * n * (M + 1) + 15) / 16 * 16;
+  // z tiles: the compile-time shape of the z kernels (zrows rows, zthreads threads)
+  const int L = zrows(M);
+  g.L = L;
+  s->zthreads = zthreads_host(M);
+  // x/y tiles: width W <= 16 (GSK_W overrides) dividing n/2, threads W*n/rmin <= 1024,
+  // resident tiles <= 120 KB
+  const int rx = rmin_of(n);
+  int W = 16;
+  // (the strided kernels are compiled for at most 16 lines per tile, xy_threads<N>())
+  if (const char* e = std::getenv("GSK_W")) W = std::max(1, std::min(16, std::atoi(e)));
+  while (W > 1 && (M % W || W * n / rx > 1024 || (double)(n + (double)Fx_resident * n * W) * 8.0 > 120 * 1024))
+    W /= 2;
+  g.W = W;
+  g.nchm = M / W;
+  g.ntile = n * g.nchm + n / W;
+  s->xthreads = round64(std::max(W * n / rx, 64));
+  s->lds_y = (size_t)(n + n * W) * sizeof(float2);
+  s->lds_xc = (size_t)(n + (size_t)Fx_resident * n * W) * sizeof(float2);
+  s->lds_z = (2 * M + (size_t)Fz * L * zpitch(M)) * sizeof(float2);
+}
+
+inline void choose_plain(gsk_sim* s) {
+  const int n = s->n, r = rmin_of(n);
+  int L = 1;
+  while (L * 2 <= n * n && (L * 2) * n / r <= 256 && (double)(n + (L * 2) * (n + 1)) * 16.0 <= 96 * 1024) L *= 2;
+  s->pL = L;
+  s->pzthreads = round64(std::max(L * n / r, 64));
+  int W = 16;
+  while (W > 1 && (W * n / r > 1024 || (double)(n + n * W) * 16.0 > 96 * 1024 || n % W)) W /= 2;
+  s->pW = W;
+  s->pxthreads = round64(std::max(W * n / r, 64));
+  s->lds_pz = (size_t)(n + L * (n + 1)) * sizeof(double2);
+  s->lds_px = (size_t)(n + n * W) * sizeof(double2);
+}
+
+inline void build_slots(gsk_sim* s) {
+  s->slots.clear();
+  const double hs = (double)hs_count(s) * 8.0;  // one half-spectrum complex field
+  const double rv = (double)nvox(s) * 4.0;      // one real fp32 field
+  const int F = s->F;
+  if (s->cfg.path == GSK_PATH_PLAIN64) {
+    const double cf = (double)nvox(s) * 16.0, r8 = (double)nvox(s) * 8.0;
+    const int V = s->V;
+    auto add_c2c = [&](const char* a, const char* b, const char* c) {
+      for (int f = 0; f < V; ++f) {
+        s->slots.push_back({a, 2 * cf});
+        s->slots.push_back({b, 2 * cf});
+        s->slots.push_back({c, 2 * cf});
+      }
+    };
+    s->slots.push_back({"p_square", V * (r8 + cf)});
+    add_c2c("p_z_fwdS", "p_y_fwdS", "p_x_fwdS");
+    s->slots.push_back({"p_contract", V * 2 * cf + V * V * r8});
+    add_c2c("p_z_invR", "p_y_invR", "p_x_invR");
+    s->slots.push_back({"p_force", V * (r8 + cf + cf)});
+    s->slots.push_back({"p_load", V * (r8 + cf)});
+    add_c2c("p_z_fwdF", "p_y_fwdF", "p_x_fwdF");
+    add_c2c("p_z_fwdE", "p_y_fwdE", "p_x_fwdE");
+    s->slots.push_back({"p_update", V * 3 * cf});
+    add_c2c("p_z_invE", "p_y_invE", "p_x_invE");
+    s->slots.push_back({"p_extract", V * (cf + r8)});
+    return;
+  }
+  if (s->cfg.physics == GSK_PHYS_MART) {
+    const int npq = F * (F + 1) / 2;
+    const bool pl = plane_on(s);
+    if (!pl) s->slots.push_back({"y_fwd_S", F * 2 * hs});
+    s->slots.push_back({"x_contract", F * 2 * hs + npq * hs / 2});
+    if (pl) {  // R and F spectra, eta of every variant per variant
+      s->slots.push_back({"plane_mid", F * (2 * hs + F * rv)});
+    } else {
+      s->slots.push_back({"y_inv_R", F * 2 * hs});
+      s->slots.push_back({"zmid_force", F * (2 * hs + rv)});
+      s->slots.push_back({"y_fwd_F", F * 2 * hs});
+    }
+    static const char* const xname[] = {"x_update_g0", "x_update_g1", "x_update_g2", "x_update_g3",
+                                        "x_update_g4", "x_update_g5", "x_update_g6", "x_update_g7"};
+    for (int f0 = 0, gi = 0; f0 < F; ++gi) {  // one launch per field group, as Impl::xupdate_mart
+      const int c = xupd_group(F - f0);
+      s->slots.push_back({xname[std::min(gi, 7)], c * 4 * hs});
+      f0 += c;
+    }
+    if (pl) {
+      s->slots.push_back({"plane_sq", F * (2 * hs + rv)});  // eta and S spectra, eta written
+    } else {
+      s->slots.push_back({"y_inv_eta", F * 2 * hs});
+      s->slots.push_back({"zinv_eta_sq", F * (2 * hs + rv)});  // C2R, write eta, R2C of eta^2
+    }
+  } else {
+    const bool fuse = prec_carries(s);
+    if (!fuse) s->slots.push_back({"zfwd_local", F * (rv + hs)});
+    s->slots.push_back({"y_fwd", F * 2 * hs});
+    if (GSK_XUPD_G == 0) {  // one launch, as Impl::xupdate_prec
+      s->slots.push_back({"x_update", F * 4 * hs + hs / 2});
+    } else {
+      for (int f0 = 0; f0 < F; f0 += 2) {
+        const int c = std::min(2, F - f0);
+        s->slots.push_back({f0 == 0 ? "x_update_g0" : (f0 == 2 ? "x_update_g1" : (f0 == 4 ? "x_update_g2" : "x_update_g3")),
+                            c * 4 * hs + (f0 == 0 ? hs / 2 : 0.0)});
+      }
+    }
+    s->slots.push_back({"y_inv", F * 2 * hs});
+    if (fuse) s->slots.push_back({"zinv_local", F * (2 * hs + rv)});  // C2R, write fields, local forces, R2C
+    else s->slots.push_back({"zinv", F * (hs + rv)});
+  }
+}
+
+inline int sync(gsk_sim* s) {
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return GSK_OK;
+}
+
+inline int prepare(gsk_sim* s) {
+  if (!s->haveB) return fail(GSK_ERR_STATE, "B tensor not built (call gsk_build_btensor first)");
+  const bool stateless = s->cfg.rule == GSK_RULE_REFERENCE && s->cfg.physics == GSK_PHYS_MART;
+  if (s->dirty && s->cfg.path == GSK_PATH_FUSED32 && !stateless) {
+    s->sq_ready = false;  // the initialisation uses the k-space buffer
+    CHK(dispatch_n<DInit>(s->n, s));
+  }
+  s->dirty = false;
+  // the spectral steps start from the z-transformed eta^2 (martensite) or local forces
+  // (precipitate) in k-space: computed here, so that every step of a timed or profiled run
+  // has the same launches
+  if (carries(s) && !s->sq_ready) CHK(dispatch_n<DSquare>(s->n, s));
+  return GSK_OK;
+}
+
+template <typename T>
+inline void free_dev(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+
+// complex elements of the tile-ordered k-space state of a handle
+inline size_t state_count(const gsk_sim* s) { return (size_t)s->g.ntile * s->F * s->n * s->g.W; }
+
+// Shared by the B tensor builders of gsk_api.hip and gsk_stress.hip.
+int fill_bargs(BArgs& a, const double* C81, const double* e1, int V1, const double* e2, int V2, double kfac);
+// Frees an adjoint workspace (gsk_adjoint.hip).
+void adj_free(AdjWS* w);

@@ -34,6 +34,7 @@ __device__ __forceinline__ void half_index(long i, const Geo& g, int& kx, int& k
 }
 
 // Packed B_pq (p <= q) on the half spectrum in natural layout, FP32: Bh[kidx(g, pq, row, kz)].
+template <int = 0>
 __global__ void k_btensor_half(float* __restrict__ Bh, Geo g, BArgs a) {
   const long total = (long)g.n * g.n * (g.M + 1);
   const int V = a.V1;
@@ -52,6 +53,7 @@ __global__ void k_btensor_half(float* __restrict__ Bh, Geo g, BArgs a) {
 }
 
 // lam_hat <- D lam_hat / N in place (fields 0..V-1); part[block] = sum w Re(conj(lam) eta1) k^2 D.
+template <int = 0>
 __global__ void k_adj_dmul(float2* __restrict__ lam, const float2* __restrict__ eta1, Geo g, AdjK k,
                            double* __restrict__ part) {
   __shared__ double sh[16];
@@ -124,6 +126,7 @@ __global__ void k_adj_contract(float2* __restrict__ kb, const float* __restrict_
 }
 
 // out[q] = eta_q^2, out[V + q] = eta_q mu_q (real fields of N voxels)
+template <int = 0>
 __global__ void k_adj_prep(const float* __restrict__ eta, const float* __restrict__ mu, float* __restrict__ out, int V,
                            long N) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x)
@@ -136,6 +139,7 @@ __global__ void k_adj_prep(const float* __restrict__ eta, const float* __restric
 
 // lam <- mu - dt L J^T mu at eta (RQ: R then Q, real); part[block] = <mu, F(eta)> including noise.
 // One thread per 4 consecutive voxels (one Philox block of noise per variant).
+template <int = 0>
 __global__ void k_adj_lambda(const float* __restrict__ eta, const float* __restrict__ mu, const float* __restrict__ RQ,
                              float* __restrict__ lam, long N, AdjK k, double* __restrict__ part) {
   __shared__ double sh[16];
@@ -170,6 +174,7 @@ __global__ void k_adj_lambda(const float* __restrict__ eta, const float* __restr
 
 // Full (not packed) rectangular B_pq, p < V1, q < V2, on the half spectrum in natural layout:
 // Bh[kidx(g, p*V2 + q, row, kz)] (the stress plugin's 6 base strains x nvar variants).
+template <int = 0>
 __global__ void k_btensor_half_rect(float* __restrict__ Bh, Geo g, BArgs a) {
   const long total = (long)g.n * g.n * (g.M + 1);
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
@@ -184,6 +189,7 @@ __global__ void k_btensor_half_rect(float* __restrict__ Bh, Geo g, BArgs a) {
 
 // In place on natural-layout spectra: fields 0..V2-1 hold X_q; fields 0..V1-1 receive
 // scale * sum_q B_pq X_q (every input of a wave vector is read before any output is written).
+template <int = 0>
 __global__ void k_contract_rect(float2* __restrict__ kb, const float* __restrict__ Bh, Geo g, int V1, int V2,
                                 float scale) {
   const long per = (long)g.n * g.n * (g.M + 1);
@@ -209,6 +215,7 @@ __global__ void k_contract_rect(float2* __restrict__ kb, const float* __restrict
 }
 
 // x <- x^2 on count floats (the eta^2 source)
+template <int = 0>
 __global__ void k_square(float* __restrict__ x, long count) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < count; i += (long)gridDim.x * blockDim.x)
     x[i] = x[i] * x[i];

@@ -94,6 +94,7 @@ __device__ __forceinline__ void btensor_half_spec(int kx, int ky, int kz, int n,
 }
 
 // Full layout B[p][q][x][y][z], FP64.
+template <int = 0>
 __global__ void k_btensor_full(double* __restrict__ B, int n, BArgs a) {
   const long N3 = (long)n * n * n;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
@@ -105,6 +106,7 @@ __global__ void k_btensor_full(double* __restrict__ B, int n, BArgs a) {
 }
 
 // Tile-ordered FP32 layout for the fused x pass, packed p <= q (V1 == V2).
+template <int = 0>
 __global__ void k_btensor_tiles(float* __restrict__ Bt, Geo g, BArgs a) {
   const int n = g.n, W = g.W, T = n * W;
   const int V = a.V1, npq = V * (V + 1) / 2;
@@ -174,6 +176,7 @@ __global__ void __launch_bounds__(1024) p_strided(double2* __restrict__ buf, int
   for (int e = threadIdx.x; e < N * W; e += blockDim.x) base[(e / W) * ps + e % W] = tile[e];
 }
 
+template <int = 0>
 __global__ void p_square(const double* __restrict__ eta, double2* __restrict__ S, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const double e = eta[i];
@@ -181,11 +184,13 @@ __global__ void p_square(const double* __restrict__ eta, double2* __restrict__ S
   }
 }
 
+template <int = 0>
 __global__ void p_load(const double* __restrict__ eta, double2* __restrict__ E, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     E[i] = make_double2(eta[i], 0.0);
 }
 
+template <int = 0>
 __global__ void p_contract(const double2* __restrict__ S, double2* __restrict__ R, const double* __restrict__ B,
                            long N3, PlainK k) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
@@ -202,6 +207,7 @@ __global__ void p_contract(const double2* __restrict__ S, double2* __restrict__ 
   }
 }
 
+template <int = 0>
 __global__ void p_force(const double* __restrict__ eta, const double2* __restrict__ R, double2* __restrict__ Fh,
                         long N3, PlainK k) {
   for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
@@ -225,6 +231,7 @@ __global__ void p_force(const double* __restrict__ eta, const double2* __restric
   }
 }
 
+template <int = 0>
 __global__ void p_update(double2* __restrict__ E, const double2* __restrict__ Fh, int n, PlainK k) {
   const long N3 = (long)n * n * n;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < (long)k.V * N3; i += (long)gridDim.x * blockDim.x) {
@@ -238,6 +245,7 @@ __global__ void p_update(double2* __restrict__ E, const double2* __restrict__ Fh
   }
 }
 
+template <int = 0>
 __global__ void p_extract(const double2* __restrict__ E, double* __restrict__ eta, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) eta[i] = E[i].x;
 }
@@ -257,6 +265,7 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
+template <int = 0>
 __global__ void r_finish(const double* __restrict__ part, int np, int nout, double* __restrict__ out) {
   // part[b*nout + j]
   __shared__ double sh[16];
@@ -316,6 +325,7 @@ __device__ __forceinline__ double prec_f(double c, const double* e, int V, const
 
 struct PrecE { double A[8]; double c1, c2, cbar; int V; };
 
+template <int = 0>
 __global__ void r_prec_real(const float* __restrict__ fld, long N3, PrecE pe, double* __restrict__ part) {
   __shared__ double sh[16];
   double chem = 0.0, sdc = 0.0, sv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -340,6 +350,7 @@ __global__ void r_prec_real(const float* __restrict__ fld, long N3, PrecE pe, do
 
 // Gradient energy from the tile-ordered fused state: sum_k w(kz) k^2 |X(k)|^2 per field
 // (w = 1 for kz in {0, n/2}, 2 otherwise); optional k^2 exponent for Cahn-Hilliard is not needed.
+template <int = 0>
 __global__ void r_state_grad(const float2* __restrict__ state, Geo g, int F, float kfac, double* __restrict__ part) {
   __shared__ double sh[16];
   const int n = g.n, W = g.W, T = n * W;
@@ -366,6 +377,7 @@ __global__ void r_state_grad(const float2* __restrict__ state, Geo g, int F, flo
 }
 
 // Precipitate elastic energy sum_k w B(k) |dc_hat(k)|^2 from the state (field 0).
+template <int = 0>
 __global__ void r_state_el_prec(const float2* __restrict__ state, const float* __restrict__ Bt, Geo g, int F,
                                 double* __restrict__ part) {
   __shared__ double sh[16];
@@ -475,6 +487,7 @@ __global__ void __launch_bounds__(xy_threads<N>()) r_xenergy_mart_r(float2* __re
 }
 
 // Plain-path energy pieces on full complex spectra (FP64): gradient sum k^2 |E|^2, elastic B_pq Re(S_p* S_q).
+template <int = 0>
 __global__ void r_plain_kspace(const double2* __restrict__ E, const double2* __restrict__ S,
                                const double* __restrict__ B, int n, int V, double kfac, double* __restrict__ part) {
   __shared__ double sh[16];
@@ -501,6 +514,7 @@ __global__ void r_plain_kspace(const double2* __restrict__ E, const double2* __r
 }
 
 // Device Philox normal field (for bit-exact host comparison).
+template <int = 0>
 __global__ void k_noise(float* __restrict__ out, long nvox, unsigned long long seed, uint32_t step, uint32_t field) {
   for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < nvox / 4; g4 += (long)gridDim.x * blockDim.x) {
     float xi[4];
@@ -510,6 +524,7 @@ __global__ void k_noise(float* __restrict__ out, long nvox, unsigned long long s
 }
 
 // Philox uniform initial field lo + (hi - lo) * u, rounded to fp32 from fp64.
+template <int = 0>
 __global__ void k_uniform(float* __restrict__ out, long nvox, unsigned long long seed, uint32_t field, double lo,
                           double hi) {
   for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < nvox / 4; g4 += (long)gridDim.x * blockDim.x) {

@@ -22,6 +22,7 @@ struct RefMartK {
 
 // One thread per 4 consecutive z voxels; eta: V fields, RE: V fields holding R on entry and
 // the new eta on exit (each element is read before it is overwritten by the same thread).
+template <int = 0>
 __global__ void k_ref_mart(const float* __restrict__ eta, float* __restrict__ RE, const float* __restrict__ defect,
                            const float* __restrict__ K, int n, RefMartK k) {
   const long N = (long)n * n * n, nq = N / 4;
@@ -67,6 +68,7 @@ __global__ void k_ref_mart(const float* __restrict__ eta, float* __restrict__ RE
 }
 
 // dst[i] = src[i] for count floats (16-byte vectors; count is a multiple of 4)
+template <int = 0>
 __global__ void k_copy4(float* __restrict__ dst, const float* __restrict__ src, long count) {
   const long n4 = count / 4;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x)
@@ -74,6 +76,7 @@ __global__ void k_copy4(float* __restrict__ dst, const float* __restrict__ src, 
 }
 
 // advances the device step counter (the reference rule's last launch of a step)
+template <int = 0>
 __global__ void k_advance(uint32_t* __restrict__ stepp) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *stepp += 1u;
 }
