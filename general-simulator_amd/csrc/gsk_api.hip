@@ -72,8 +72,6 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
   if (!supported_n(cfg->n)) return fail(GSK_ERR_UNSUPPORTED, "unsupported grid size " + std::to_string(cfg->n));
   if (cfg->physics != GSK_PHYS_MART && cfg->physics != GSK_PHYS_PREC) return fail(GSK_ERR_ARG, "unknown physics");
   if (cfg->path != GSK_PATH_FUSED32 && cfg->path != GSK_PATH_PLAIN64) return fail(GSK_ERR_ARG, "unknown path");
-  if (cfg->path == GSK_PATH_PLAIN64 && cfg->physics != GSK_PHYS_MART)
-    return fail(GSK_ERR_UNSUPPORTED, "the FP64 plain path implements the martensite rule only");
   if (cfg->rule != GSK_RULE_SPECTRAL && cfg->rule != GSK_RULE_REFERENCE) return fail(GSK_ERR_ARG, "unknown rule");
   if (cfg->rule == GSK_RULE_REFERENCE && cfg->path != GSK_PATH_FUSED32)
     return fail(GSK_ERR_UNSUPPORTED, "the reference rules run on the FP32 fused path");
@@ -127,7 +125,7 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
     const int npq = vv ? s->V * (s->V + 1) / 2 : 1;
     if ((rc = alloc(s->Bt, ntile * npq * T)) != GSK_OK) return bail(rc);
     if (cfg->rule == GSK_RULE_REFERENCE && (rc = alloc(s->rbuf, (size_t)s->V * N3)) != GSK_OK) return bail(rc);
-  } else {
+  } else if (cfg->physics == GSK_PHYS_MART) {
     const size_t VN = (size_t)s->V * N3;
     if ((rc = alloc(s->eta64, VN)) != GSK_OK) return bail(rc);
     if ((rc = alloc(s->S64, VN)) != GSK_OK) return bail(rc);
@@ -135,6 +133,12 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
     if ((rc = alloc(s->E64, VN)) != GSK_OK) return bail(rc);
     if ((rc = alloc(s->F64, VN)) != GSK_OK) return bail(rc);
     if ((rc = alloc(s->B64, (size_t)s->V * s->V * N3)) != GSK_OK) return bail(rc);
+  } else {  // precipitate: fields [c, eta_1..eta_V], forces, transforms; the scalar B(k) of c
+    const size_t FN = (size_t)s->F * N3;
+    if ((rc = alloc(s->eta64, FN)) != GSK_OK) return bail(rc);
+    if ((rc = alloc(s->E64, FN)) != GSK_OK) return bail(rc);
+    if ((rc = alloc(s->F64, FN)) != GSK_OK) return bail(rc);
+    if ((rc = alloc(s->B64, N3)) != GSK_OK) return bail(rc);
   }
   s->rblocks = std::max<int>(1024, (int)ntile);
   if ((rc = alloc(s->part, (size_t)s->rblocks * 16)) != GSK_OK) return bail(rc);
@@ -286,7 +290,7 @@ int gsk_set_field(gsk_sim* s, const char* name, const void* host, int dtype) {
     for (size_t i = 0; i < count; ++i)
       h[i] = dtype == GSK_F32 ? (double)((const float*)host)[i] : ((const double*)host)[i];
     CHK(sync(s));  // earlier work on the handle's stream may still read the field
-    HIPCHK(hipMemcpy(s->eta64, h.data(), count * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(s->eta64 + (size_t)off * N3, h.data(), count * sizeof(double), hipMemcpyHostToDevice));
     return GSK_OK;
   }
   std::vector<float> h(count);
@@ -335,7 +339,7 @@ int gsk_get_field(gsk_sim* s, const char* name, void* host, int dtype) {
   const size_t N3 = nvox(s), count = (size_t)cnt * N3;
   if (s->cfg.path == GSK_PATH_PLAIN64) {
     std::vector<double> h(count);
-    HIPCHK(hipMemcpy(h.data(), s->eta64, count * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h.data(), s->eta64 + (size_t)off * N3, count * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < count; ++i) {
       if (dtype == GSK_F32) ((float*)host)[i] = (float)h[i];
       else ((double*)host)[i] = h[i];
@@ -429,15 +433,16 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
   const gsk_config& c = s->cfg;
   double h[16];
   auto real_pass = [&]() -> int {
-    if (s->cfg.path == GSK_PATH_PLAIN64) {
-      LAUNCH(r_mart_real<double>, nb, 256, 0, s, s->eta64, N3, s->V, c.A, c.B, c.C, s->part);
-    } else if (c.physics == GSK_PHYS_MART) {
-      LAUNCH(r_mart_real<float>, nb, 256, 0, s, s->fld, N3, s->V, c.A, c.B, c.C, s->part);
+    const bool plain = s->cfg.path == GSK_PATH_PLAIN64;
+    if (c.physics == GSK_PHYS_MART) {
+      if (plain) LAUNCH(r_mart_real<double>, nb, 256, 0, s, s->eta64, N3, s->V, c.A, c.B, c.C, s->part);
+      else LAUNCH(r_mart_real<float>, nb, 256, 0, s, s->fld, N3, s->V, c.A, c.B, c.C, s->part);
     } else {
       PrecE pe{};
       for (int i = 0; i < 8; ++i) pe.A[i] = c.a[i];
-      pe.c1 = c.c1; pe.c2 = c.c2; pe.cbar = s->cbar; pe.V = s->V;
-      LAUNCH(r_prec_real<>, nb, 256, 0, s, s->fld, N3, pe, s->part);
+      pe.c1 = c.c1; pe.c2 = c.c2; pe.cbar = plain ? 0.0 : s->cbar; pe.V = s->V;
+      if (plain) LAUNCH(r_prec_real<double>, nb, 256, 0, s, s->eta64, N3, pe, s->part);
+      else LAUNCH(r_prec_real<float>, nb, 256, 0, s, s->fld, N3, pe, s->part);
     }
     LAUNCH(r_finish<>, 1, 1024, 0, s, s->part, nb, 2 + s->V, s->rout);
     HIPCHK(hipMemcpyAsync(h, s->rout, (2 + s->V) * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -456,6 +461,11 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
   if (what == GSK_RED_MASS) {
     if (c.physics != GSK_PHYS_PREC) return fail(GSK_ERR_UNSUPPORTED, "mass is a precipitate reduction");
     CHK(prepare(s));
+    if (s->cfg.path == GSK_PATH_PLAIN64) {  // the FP64 sum of the FP64 field, twice
+      CHK(real_pass());
+      out[0] = out[1] = h[1];
+      return GSK_OK;
+    }
     // out[0]: N * mean + Re of the k = 0 mode of the fluctuation state (the conserved quantity)
     // out[1]: FP64 sum of the FP32 real field (includes one IFFT's rounding, not accumulated)
     float2 k0 = make_float2(0.f, 0.f);
@@ -472,7 +482,12 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
     CHK(real_pass());
     const double chem = h[0];
     double gr = 0.0, el = 0.0;
-    if (s->cfg.path == GSK_PATH_PLAIN64) {
+    if (s->cfg.path == GSK_PATH_PLAIN64 && c.physics == GSK_PHYS_PREC) {
+      double ks[3];
+      CHK(dispatch_n<DEpP>(s->n, s, ks));
+      gr = 0.5 * (c.kappa_c * ks[0] + c.kappa * ks[1]) / (double)N3;
+      el = 0.5 * c.xi * ks[2] / (double)N3;
+    } else if (s->cfg.path == GSK_PATH_PLAIN64) {
       double g2, e2;
       CHK(dispatch_n<DElP>(s->n, s, &g2, &e2));
       gr = 0.5 * c.kappa * g2 / (double)N3;

@@ -457,10 +457,10 @@ struct Impl {
   }
 
   // ---- plain FP64 3D C2C on V fields of buf
-  static int c2c(gsk_sim* s, double2* buf, bool fwd) {
+  static int c2c(gsk_sim* s, double2* buf, bool fwd, int nf) {
     const long n = s->n;
     const long N3 = n * n * n;
-    for (int f = 0; f < s->V; ++f) {
+    for (int f = 0; f < nf; ++f) {
       double2* b = buf + f * N3;
       const int gz = (int)(n * n / s->pL);
       if (fwd) LAUNCH((p_zrows<N, true>), gz, s->pzthreads, s->lds_pz, s, b, s->pL, s->twN64);
@@ -480,16 +480,45 @@ struct Impl {
     const long N3 = (long)nvox(s), VN = s->V * N3;
     PlainK k = plain_consts(s);
     LAUNCH(p_square<>, grid_1d(VN), 256, 0, s, s->eta64, s->S64, VN);
-    CHK(c2c(s, s->S64, true));
+    CHK(c2c(s, s->S64, true, s->V));
     LAUNCH(p_contract<>, grid_1d(N3), 256, 0, s, s->S64, s->R64, s->B64, N3, k);
-    CHK(c2c(s, s->R64, false));
+    CHK(c2c(s, s->R64, false, s->V));
     LAUNCH(p_force<>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->R64, s->F64, N3, k);
     LAUNCH(p_load<>, grid_1d(VN), 256, 0, s, s->eta64, s->E64, VN);
-    CHK(c2c(s, s->F64, true));
-    CHK(c2c(s, s->E64, true));
+    CHK(c2c(s, s->F64, true, s->V));
+    CHK(c2c(s, s->E64, true, s->V));
     LAUNCH(p_update<>, grid_1d(VN), 256, 0, s, s->E64, s->F64, s->n, k);
-    CHK(c2c(s, s->E64, false));
+    CHK(c2c(s, s->E64, false, s->V));
     LAUNCH(p_extract<>, grid_1d(VN), 256, 0, s, s->E64, s->eta64, VN);
+    return GSK_OK;
+  }
+
+  // Precipitate spectral rule on the FP64 plain path (src/dynamics_precipitate.cu:118-275): local
+  // forces, C2C of forces and fields, the semi-implicit CH and AC updates, inverse C2C.
+  static int step_prec_plain(gsk_sim* s) {
+    const int F = s->F;
+    const long N3 = (long)nvox(s), FN = F * N3;
+    const PlainPK k = plain_prec_consts(s);
+    LAUNCH(p_prec_local<>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->F64, s->E64, N3, k);
+    CHK(c2c(s, s->F64, true, F));
+    CHK(c2c(s, s->E64, true, F));
+    LAUNCH(p_prec_update<>, grid_1d(FN), 256, 0, s, s->E64, s->F64, s->B64, s->n, k);
+    CHK(c2c(s, s->E64, false, F));
+    LAUNCH(p_extract<>, grid_1d(FN), 256, 0, s, s->E64, s->eta64, FN);
+    return GSK_OK;
+  }
+
+  // out = {sum k^2 |c_hat|^2, sum k^2 sum_v |eta_hat_v|^2, sum B |c_hat|^2} of the plain precipitate
+  static int energy_prec_plain(gsk_sim* s, double* out) {
+    const long N3 = (long)nvox(s), FN = s->F * N3;
+    const PlainPK k = plain_prec_consts(s);
+    LAUNCH(p_load<>, grid_1d(FN), 256, 0, s, s->eta64, s->E64, FN);
+    CHK(c2c(s, s->E64, true, s->F));
+    const int nb = 256;
+    LAUNCH(r_plain_kspace_prec<>, nb, 256, 0, s, s->E64, s->B64, s->n, s->V, k.kfac, s->part);
+    LAUNCH(r_finish<>, 1, 1024, 0, s, s->part, nb, 3, s->rout);
+    HIPCHK(hipMemcpyAsync(out, s->rout, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
     return GSK_OK;
   }
 
@@ -525,9 +554,9 @@ struct Impl {
     const long N3 = (long)nvox(s), VN = s->V * N3;
     PlainK k = plain_consts(s);
     LAUNCH(p_square<>, grid_1d(VN), 256, 0, s, s->eta64, s->S64, VN);
-    CHK(c2c(s, s->S64, true));
+    CHK(c2c(s, s->S64, true, s->V));
     LAUNCH(p_load<>, grid_1d(VN), 256, 0, s, s->eta64, s->E64, VN);
-    CHK(c2c(s, s->E64, true));
+    CHK(c2c(s, s->E64, true, s->V));
     const int nb = 256;
     LAUNCH(r_plain_kspace<>, nb, 256, 0, s, s->E64, s->S64, s->B64, s->n, s->V, k.kfac, s->part);
     LAUNCH(r_finish<>, 1, 1024, 0, s, s->part, nb, 2, s->rout);
@@ -549,7 +578,8 @@ template <int N> int impl_step_batched(gsk_sim* s0, int cnt, const MemberPtrs* t
   return Impl<N>::step_mart_batched(s0, cnt, tab, st);
 }
 template <int N> int impl_step(gsk_sim* s) {
-  if (s->cfg.path == GSK_PATH_PLAIN64) return Impl<N>::step_mart_plain(s);
+  if (s->cfg.path == GSK_PATH_PLAIN64)
+    return s->cfg.physics == GSK_PHYS_MART ? Impl<N>::step_mart_plain(s) : Impl<N>::step_prec_plain(s);
   if (s->cfg.rule == GSK_RULE_REFERENCE)
     return s->cfg.physics == GSK_PHYS_MART ? Impl<N>::step_mart_ref(s) : Impl<N>::step_prec_ref(s);
   if (s->cfg.physics == GSK_PHYS_MART) return Impl<N>::step_mart_fused(s);
@@ -559,6 +589,7 @@ template <int N> int impl_energy_el_fused(gsk_sim* s, double* el) { return Impl<
 template <int N> int impl_energy_el_plain(gsk_sim* s, double* gr, double* el) {
   return Impl<N>::energy_mart_el_plain(s, gr, el);
 }
+template <int N> int impl_energy_prec_plain(gsk_sim* s, double* out) { return Impl<N>::energy_prec_plain(s, out); }
 template <int N> int impl_zfwd(gsk_sim* s, const float* in, int F) { return Impl<N>::zfwd(s, Z_COPY, in, F); }
 template <int N> int impl_ypass(gsk_sim* s, bool fwd, int F) { return Impl<N>::ypass(s, fwd, F); }
 template <int N> int impl_xpass(gsk_sim* s, bool fwd, int F, float scale) {
@@ -583,6 +614,7 @@ template <int N> int impl_zinv(gsk_sim* s, float* out, int F) { return Impl<N>::
   template int impl_step<N>(gsk_sim*);                                                                           \
   template int impl_energy_el_fused<N>(gsk_sim*, double*);                                                       \
   template int impl_energy_el_plain<N>(gsk_sim*, double*, double*);                                              \
+  template int impl_energy_prec_plain<N>(gsk_sim*, double*);                                                     \
   template int impl_zfwd<N>(gsk_sim*, const float*, int);                                                        \
   template int impl_ypass<N>(gsk_sim*, bool, int);                                                               \
   template int impl_xpass<N>(gsk_sim*, bool, int, float);                                                        \

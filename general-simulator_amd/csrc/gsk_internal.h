@@ -291,6 +291,26 @@ inline PlainK plain_consts(const gsk_sim* s) {
   return k;
 }
 
+inline PlainPK plain_prec_consts(const gsk_sim* s) {
+  const gsk_config& c = s->cfg;
+  PlainPK k{};
+  for (int i = 0; i < 8; ++i) k.A[i] = c.a[i];
+  k.c1 = c.c1; k.c2 = c.c2;
+  k.dtM = c.dt * c.M;
+  k.dtMkc = c.dt * c.M * c.kappa_c;
+  k.dtL = c.dt * c.L;
+  k.dtLke = c.dt * c.L * c.kappa;
+  k.xi = c.xi;
+  k.S = c.S;
+  k.noise = c.noise;
+  k.kfac = (c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx));
+  k.inv_n3 = 1.0 / ((double)s->n * s->n * s->n);
+  k.V = s->V;
+  k.step = (uint32_t)s->step;
+  k.seed = c.seed;
+  return k;
+}
+
 inline int grid_1d(long work, int threads = 256) {
   long b = (work + threads - 1) / threads;
   if (b > 65536) b = 65536;
@@ -330,6 +350,7 @@ template <int N> int impl_step_batched(gsk_sim* s0, int cnt, const MemberPtrs* t
 template <int N> int impl_step(gsk_sim* s);
 template <int N> int impl_energy_el_fused(gsk_sim* s, double* el);
 template <int N> int impl_energy_el_plain(gsk_sim* s, double* gr, double* el);
+template <int N> int impl_energy_prec_plain(gsk_sim* s, double* out);
 template <int N> int impl_zfwd(gsk_sim* s, const float* in, int F);
 template <int N> int impl_ypass(gsk_sim* s, bool fwd, int F);
 template <int N> int impl_xpass(gsk_sim* s, bool fwd, int F, float scale);
@@ -367,6 +388,7 @@ template <int N> struct DElF { static int run(gsk_sim* s, double* el) { return i
 template <int N> struct DElP {
   static int run(gsk_sim* s, double* gr, double* el) { return impl_energy_el_plain<N>(s, gr, el); }
 };
+template <int N> struct DEpP { static int run(gsk_sim* s, double* out) { return impl_energy_prec_plain<N>(s, out); } };
 template <int N> struct DZfwd { static int run(gsk_sim* s, const float* in, int F) { return impl_zfwd<N>(s, in, F); } };
 template <int N> struct DY { static int run(gsk_sim* s, bool fwd, int F) { return impl_ypass<N>(s, fwd, F); } };
 template <int N> struct DX {
@@ -443,6 +465,23 @@ inline void build_slots(gsk_sim* s) {
   const double hs = (double)hs_count(s) * 8.0;  // one half-spectrum complex field
   const double rv = (double)nvox(s) * 4.0;      // one real fp32 field
   const int F = s->F;
+  if (s->cfg.path == GSK_PATH_PLAIN64 && s->cfg.physics == GSK_PHYS_PREC) {
+    const double cf = (double)nvox(s) * 16.0, r8 = (double)nvox(s) * 8.0;
+    auto add_c2c = [&](const char* a, const char* b, const char* c) {
+      for (int f = 0; f < F; ++f) {
+        s->slots.push_back({a, 2 * cf});
+        s->slots.push_back({b, 2 * cf});
+        s->slots.push_back({c, 2 * cf});
+      }
+    };
+    s->slots.push_back({"p_prec_local", F * (r8 + 2 * cf)});
+    add_c2c("p_z_fwdF", "p_y_fwdF", "p_x_fwdF");
+    add_c2c("p_z_fwdE", "p_y_fwdE", "p_x_fwdE");
+    s->slots.push_back({"p_prec_update", F * 3 * cf + r8});
+    add_c2c("p_z_invE", "p_y_invE", "p_x_invE");
+    s->slots.push_back({"p_extract", F * (cf + r8)});
+    return;
+  }
   if (s->cfg.path == GSK_PATH_PLAIN64) {
     const double cf = (double)nvox(s) * 16.0, r8 = (double)nvox(s) * 8.0;
     const int V = s->V;

@@ -250,6 +250,88 @@ __global__ void p_extract(const double2* __restrict__ E, double* __restrict__ et
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) eta[i] = E[i].x;
 }
 
+// ---- precipitate, FP64 plain path: the spectral rule of the precipitate plugin
+// (src/dynamics_precipitate.cu:118-275 with the c-sourced elasticity and the stabilisation S),
+// one kernel per operation. Fields: [c, eta_1..eta_V], F = V + 1.
+struct PlainPK {
+  double A[8];
+  double c1, c2, dtM, dtMkc, dtL, dtLke, xi, S, noise, kfac, inv_n3;
+  int V;
+  uint32_t step;
+  unsigned long long seed;
+};
+
+// mu_c = A2/2 sum eta^2 + A1 (c - c1) (:118-127) and F_v (:136-162) + noise a xi_v (Philox field v)
+// of every voxel into Fh; the fields themselves into E (both as complex with zero imaginary part).
+template <int = 0>
+__global__ void p_prec_local(const double* __restrict__ fld, double2* __restrict__ Fh, double2* __restrict__ E,
+                             long N3, PlainPK k) {
+  const int V = k.V;
+  for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
+    for (int i = 0; i < 4; ++i) {
+      const long v = g4 * 4 + i;
+      const double c = fld[v];
+      double e[8], s2 = 0.0;
+      for (int a = 0; a < V; ++a) {
+        e[a] = fld[(1 + a) * N3 + v];
+        s2 += e[a] * e[a];
+      }
+      Fh[v] = make_double2(0.5 * k.A[2] * s2 + k.A[1] * (c - k.c1), 0.0);
+      E[v] = make_double2(c, 0.0);
+      for (int a = 0; a < V; ++a) {
+        double t2 = 0.0, t4 = 0.0;
+        for (int b = 0; b < V; ++b)
+          if (b != a) {
+            const double q = e[b] * e[b];
+            t2 += q;
+            t4 += q * q;
+          }
+        const double et = e[a], e2 = et * et;
+        // ordered pairs i != j, both != a: (sum q)^2 - sum q^2
+        const double f = (2.0 * k.A[5] * et + 4.0 * k.A[6] * e2 * et) * t2 + 2.0 * k.A[6] * et * t4 +
+                         2.0 * k.A[7] * et * (t2 * t2 - t4) + k.A[2] * et * (c - k.c2) - k.A[3] * e2 * et +
+                         k.A[4] * e2 * e2 * et;
+        Fh[(1 + a) * N3 + v] = make_double2(f, 0.0);
+        E[(1 + a) * N3 + v] = make_double2(et, 0.0);
+      }
+    }
+    if (k.noise != 0.0) {
+      for (int a = 0; a < V; ++a) {
+        float xi[4];
+        normal4(k.seed, (uint32_t)g4, (uint32_t)a, k.step, xi);
+        for (int i = 0; i < 4; ++i) Fh[(1 + a) * N3 + g4 * 4 + i].x += k.noise * (double)xi[i];
+      }
+    }
+  }
+}
+
+// Semi-implicit updates of the transformed fields (with the 1/n^3 of the inverse transform):
+//   c_hat   <- (c_hat - dt M k^2 (mu_hat + (xi B(k) - S) c_hat)) / (1 + dt M (S k^2 + kappa_c k^4))
+//   eta_hat <- (eta_hat - dt L F_hat) / (1 + dt L kappa_eta k^2)
+template <int = 0>
+__global__ void p_prec_update(double2* __restrict__ E, const double2* __restrict__ Fh, const double* __restrict__ Bc,
+                              int n, PlainPK k) {
+  const long N3 = (long)n * n * n;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < (long)(k.V + 1) * N3;
+       i += (long)gridDim.x * blockDim.x) {
+    const long v = i % N3;
+    const int x = (int)(v / ((long)n * n)), y = (int)((v / n) % n), z = (int)(v % n);
+    const double kx = k.kfac * ((x < n / 2) ? x : x - n), ky = k.kfac * ((y < n / 2) ? y : y - n),
+                 kz = k.kfac * ((z < n / 2) ? z : z - n);
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double2 e = E[i], f = Fh[i];
+    if (i < N3) {
+      const double a = k.xi * Bc[v] - k.S;
+      const double mx = f.x + a * e.x, my = f.y + a * e.y;
+      const double den = k.inv_n3 / (1.0 + k.dtM * k.S * k2 + k.dtMkc * k2 * k2);
+      E[i] = make_double2((e.x - k.dtM * k2 * mx) * den, (e.y - k.dtM * k2 * my) * den);
+    } else {
+      const double den = k.inv_n3 / (1.0 + k.dtLke * k2);
+      E[i] = make_double2((e.x - k.dtL * f.x) * den, (e.y - k.dtL * f.y) * den);
+    }
+  }
+}
+
 // ====================================================================== FP64 reductions
 // Two-stage deterministic sums: each block writes a partial, a single block adds them.
 __device__ __forceinline__ double block_sum(double v, double* sh) {
@@ -325,8 +407,8 @@ __device__ __forceinline__ double prec_f(double c, const double* e, int V, const
 
 struct PrecE { double A[8]; double c1, c2, cbar; int V; };
 
-template <int = 0>
-__global__ void r_prec_real(const float* __restrict__ fld, long N3, PrecE pe, double* __restrict__ part) {
+template <typename T = float>
+__global__ void r_prec_real(const T* __restrict__ fld, long N3, PrecE pe, double* __restrict__ part) {
   __shared__ double sh[16];
   double chem = 0.0, sdc = 0.0, sv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   const int V = pe.V;
@@ -511,6 +593,36 @@ __global__ void r_plain_kspace(const double2* __restrict__ E, const double2* __r
   if (threadIdx.x == 0) part[blockIdx.x * 2] = r;
   r = block_sum(el, sh);
   if (threadIdx.x == 0) part[blockIdx.x * 2 + 1] = r;
+}
+
+// Precipitate, FP64 plain path: sum_k k^2 |c_hat|^2, sum_k k^2 sum_v |eta_hat_v|^2 and
+// sum_k B(k) |c_hat|^2 over the full transformed fields E = [c, eta_1..eta_V].
+template <int = 0>
+__global__ void r_plain_kspace_prec(const double2* __restrict__ E, const double* __restrict__ Bc, int n, int V,
+                                    double kfac, double* __restrict__ part) {
+  __shared__ double sh[16];
+  const long N3 = (long)n * n * n;
+  double gc = 0.0, ge = 0.0, el = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(i / ((long)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
+    const double kx = kfac * ((x < n / 2) ? x : x - n), ky = kfac * ((y < n / 2) ? y : y - n),
+                 kz = kfac * ((z < n / 2) ? z : z - n);
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double2 c = E[i];
+    const double a2 = c.x * c.x + c.y * c.y;
+    gc += k2 * a2;
+    el += Bc[i] * a2;
+    for (int v = 1; v <= V; ++v) {
+      const double2 e = E[v * N3 + i];
+      ge += k2 * (e.x * e.x + e.y * e.y);
+    }
+  }
+  double r = block_sum(gc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x * 3] = r;
+  r = block_sum(ge, sh);
+  if (threadIdx.x == 0) part[blockIdx.x * 3 + 1] = r;
+  r = block_sum(el, sh);
+  if (threadIdx.x == 0) part[blockIdx.x * 3 + 2] = r;
 }
 
 // Device Philox normal field (for bit-exact host comparison).

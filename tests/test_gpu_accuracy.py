@@ -131,3 +131,43 @@ def test_config0_64_against_oracle_1000_steps(gpu):
     rf = compare("mart64_config0_fused32", fused, want, Ef, Ew)
     assert rp["rel_l2"] < 1e-11 and rp["rel_energy_err"] < 1e-11
     assert rf["max_abs_d_eta"] <= 1e-4 and rf["rel_l2"] < 1e-4 and rf["rel_energy_err"] < 1e-4
+
+
+def run_prec(n, steps, path, V=4):
+    """The 256^3 precipitate configuration: c = 0.25 + U[-0.005, 0.005], eta ~ U[0, 0.01],
+    M = L = 1, kappa_c = kappa_eta = 1.5, dt = 0.005, dilatational eigenstrain 0.02 c I, eta noise
+    0.001 (Philox seed 5), S = A1 = 54, the other coefficients the reference's defaults."""
+    cfg = gsk.default_config(gsk.PHYS_PREC, n=n, nvar=V, path=path, M=1.0, L=1.0, kappa=1.5, kappa_c=1.5, dt=0.005,
+                             noise=0.001, seed=5, S=54.0)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(gsk.cubic_modulus(3.5, 1.5, 1.0), 0.02 * np.eye(3)[None])
+        s.init_uniform("concentration", 0.245, 0.255, 11)
+        s.init_uniform("eta", 0.0, 0.01, 12)
+        m0 = s.mass()
+        t0 = time.perf_counter()
+        s.step(steps)
+        el = time.perf_counter() - t0
+        return (s.get_field("concentration", np.float64), s.get_field("eta", np.float64), s.energy(), m0, s.mass(),
+                el)
+
+
+def test_config1_accuracy_256_precipitate_2000_steps(gpu):
+    """256^3 precipitate, 4 order parameters, 2000 steps: FP32 fused against FP64 plain (same
+    Philox noise). Sum of c: conserved to the last bit on the fused path (FP64 mean and exact
+    k = 0 state), to FP64 rounding on the plain path."""
+    say("accuracy 256^3 precipitate, 2000 steps, fused vs FP64 plain")
+    n, steps = 256, 2000
+    cf, ef, Ef, m0f, m1f, tf = run_prec(n, steps, gsk.PATH_FUSED32)
+    cp, ep, Ep, m0p, m1p, tp = run_prec(n, steps, gsk.PATH_PLAIN64)
+    r = compare("prec256_config1_2000", ef, ep, Ef, Ep, steps=steps, fused_s=tf, plain64_s=tp,
+                max_abs_d_c=float(np.abs(cf - cp).max()), rel_l2_c=rel_l2(cf, cp),
+                mass_drift_fused=abs(m1f - m0f) / m0f, mass_drift_plain=abs(m1p - m0p) / m0p,
+                mass_fused_vs_plain=abs(m1f - m1p) / m1p)
+    assert np.isfinite(ef).all() and np.isfinite(cf).all()
+    assert m1f == m0f
+    assert r["mass_drift_plain"] < 1e-12
+    assert r["mass_fused_vs_plain"] < 1e-12
+    assert r["max_abs_d_eta"] <= 1e-4
+    assert r["max_abs_d_c"] <= 1e-4
+    assert r["rel_l2"] <= 1e-4 and r["rel_l2_c"] <= 1e-5
+    assert r["rel_energy_err"] <= 1e-4

@@ -446,3 +446,31 @@ def test_mart_fused32_general_eigenstrains(gpu):
         got = s.get_field("eta", np.float64)
     want = O.mart_run(eta0, O.btensor(n, C, eps), oracle_mart_params(cfg), steps)
     assert rel_l2(got, want) < 2e-5, rel_l2(got, want)
+
+
+@pytest.mark.parametrize("n,steps", [(16, 40), (32, 50)])
+def test_prec_plain64_matches_oracle(gpu, n, steps):
+    """The precipitate spectral rule on the FP64 plain path (one kernel per operation, C2C
+    transforms, the full-layout FP64 B(k)) against the FP64 oracle, with eta noise: 1e-11."""
+    say(f"prec plain64 n={n} steps={steps}")
+    s = prec_sim(n, path=gsk.PATH_PLAIN64)
+    prec_init(s)
+    c0 = s.get_field("concentration", np.float64)
+    e0 = s.get_field("eta", np.float64)
+    m0 = s.mass()
+    s.step(steps)
+    c1 = s.get_field("concentration", np.float64)
+    e1 = s.get_field("eta", np.float64)
+    E = s.energy()
+    m1 = s.mass()
+    cfg = s.cfg
+    s.close()
+    p = oracle_prec_params(cfg)
+    Bc = O.btensor(n, cubic(3.5, 1.5, 1.0), 0.02 * np.eye(3)[None])[0, 0]
+    cw, ew = O.prec_run(c0, e0, Bc, p, steps)
+    assert np.abs(c1 - cw).max() < 1e-11 * np.abs(cw).max()
+    assert np.abs(e1 - ew).max() < 1e-11 * np.abs(ew).max()
+    assert abs(m0 - c0.sum()) / c0.sum() < 1e-14
+    assert abs(m1 - m0) / m0 < 1e-13
+    Ew = O.prec_energy(cw, ew, Bc, p)
+    np.testing.assert_allclose(E, Ew, rtol=1e-10, atol=1e-12 * abs(Ew[3]))

@@ -47,8 +47,8 @@ def test_invalid_arguments_return_errors():
     h = ctypes.c_void_p()
     rc = lib.gsk_create(ctypes.byref(c), ctypes.byref(h))
     assert rc == -5 and b"unsupported grid size" in lib.gsk_last_error()
-    c = gsk.default_config(gsk.PHYS_PREC, path=gsk.PATH_PLAIN64)
-    assert lib.gsk_create(ctypes.byref(c), ctypes.byref(h)) == -5
+    c = gsk.reference_config(gsk.PHYS_PREC, path=gsk.PATH_PLAIN64)
+    assert lib.gsk_create(ctypes.byref(c), ctypes.byref(h)) == -5 and b"FP32 fused path" in lib.gsk_last_error()
     c = gsk.default_config(gsk.PHYS_MART, nvar=0)
     assert lib.gsk_create(ctypes.byref(c), ctypes.byref(h)) == -2
 
