@@ -125,6 +125,9 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
     const int npq = vv ? s->V * (s->V + 1) / 2 : 1;
     if ((rc = alloc(s->Bt, ntile * npq * T)) != GSK_OK) return bail(rc);
     if (cfg->rule == GSK_RULE_REFERENCE && (rc = alloc(s->rbuf, (size_t)s->V * N3)) != GSK_OK) return bail(rc);
+    if (cfg->rule == GSK_RULE_REFERENCE && cfg->physics == GSK_PHYS_MART &&
+        (rc = alloc(s->fstat, (size_t)kRefBlocks * 4)) != GSK_OK)
+      return bail(rc);
   } else if (cfg->physics == GSK_PHYS_MART) {
     const size_t VN = (size_t)s->V * N3;
     if ((rc = alloc(s->eta64, VN)) != GSK_OK) return bail(rc);
@@ -161,7 +164,7 @@ int gsk_destroy(gsk_sim* s) {
   free_dev(s->fld); free_dev(s->kbuf); free_dev(s->state); free_dev(s->Bt); free_dev(s->dstep);
   free_dev(s->eta64); free_dev(s->S64); free_dev(s->R64); free_dev(s->E64); free_dev(s->F64); free_dev(s->B64);
   free_dev(s->part); free_dev(s->rout);
-  free_dev(s->rbuf); free_dev(s->defect); free_dev(s->Kd);
+  free_dev(s->rbuf); free_dev(s->defect); free_dev(s->Kd); free_dev(s->fstat);
   for (auto& e : s->ev) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
@@ -456,6 +459,16 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
     } else {
       for (int v = 0; v < s->V; ++v) out[v] = h[2 + v] / (double)N3;
     }
+    return GSK_OK;
+  }
+  if (what == GSK_RED_MART_FORCES) {
+    if (c.physics != GSK_PHYS_MART || c.rule != GSK_RULE_REFERENCE || !s->fstat)
+      return fail(GSK_ERR_UNSUPPORTED, "the force means belong to the martensite reference rule");
+    const int nbr = std::min(grid_1d(N3 / 4), kRefBlocks);
+    LAUNCH(r_finish<>, 1, 1024, 0, s, s->fstat, nbr, 4, s->rout);
+    HIPCHK(hipMemcpyAsync(h, s->rout, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    CHK(sync(s));
+    for (int j = 0; j < 4; ++j) out[j] = h[j] / ((double)s->V * (double)N3);
     return GSK_OK;
   }
   if (what == GSK_RED_MASS) {

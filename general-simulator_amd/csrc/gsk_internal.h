@@ -166,6 +166,7 @@ struct gsk_sim {
   float* rbuf = nullptr;      // V real fields: elastic R, then the new eta (martensite)
   float* defect = nullptr;    // n^3 blocking field (martensite reference rule), optional
   float* Kd = nullptr;        // V*n^3 dislocation force constants, optional
+  double* fstat = nullptr;    // per-block sums of the four force terms of the last reference step
   double eps[72] = {};        // eigenstrains of the last B build
   bool haveEps = false;
   bool sharedB = false;       // Bt belongs to an ensemble, not to this handle
@@ -310,6 +311,9 @@ inline PlainPK plain_prec_consts(const gsk_sim* s) {
   k.seed = c.seed;
   return k;
 }
+
+// blocks of the martensite reference-rule kernel (its per-block force sums: kRefBlocks * 4 doubles)
+constexpr int kRefBlocks = 2048;
 
 inline int grid_1d(long work, int threads = 256) {
   long b = (work + threads - 1) / threads;

@@ -7,7 +7,7 @@
 //   Dis  = -w_dis K_p eta_p (K_p = 25 sum_ij sigma_ij eps^p_ij, set once, :305-331)
 // The elastic R comes from the fused transforms (zfwd square, y, x contraction, y, z inverse).
 #pragma once
-#include "kernels_strided.h"
+#include "kernels_aux.h"
 
 namespace gsk {
 
@@ -22,9 +22,13 @@ struct RefMartK {
 
 // One thread per 4 consecutive z voxels; eta: V fields, RE: V fields holding R on entry and
 // the new eta on exit (each element is read before it is overwritten by the same thread).
+// fpart (may be null): per-block FP64 sums of the four force terms of this step, w_grad Grad,
+// Chem, El, Dis, over every voxel and variant (the reference's GradientForce, ChemicalForce,
+// ElasticForce, DislocationForce arrays that Get averages, src/dynamics_mart.cu:443-456).
 template <int = 0>
 __global__ void k_ref_mart(const float* __restrict__ eta, float* __restrict__ RE, const float* __restrict__ defect,
-                           const float* __restrict__ K, int n, RefMartK k) {
+                           const float* __restrict__ K, int n, RefMartK k, double* __restrict__ fpart) {
+  double fs[4] = {0.0, 0.0, 0.0, 0.0};
   const long N = (long)n * n * n, nq = N / 4;
   const long nn = (long)n * n;
   const uint32_t step = step_of(k.step, k.stepp);
@@ -62,7 +66,20 @@ __global__ void k_ref_mart(const float* __restrict__ eta, float* __restrict__ RE
         float nv = e + k.dt * (k.wG * grad + chem + el + dis + k.wN * xi[j]);
         if (defect) nv *= 1.f - defect[i];
         RE[v * N + i] = nv;
+        if (fpart) {
+          fs[0] += (double)(k.wG * grad);
+          fs[1] += (double)chem;
+          fs[2] += (double)el;
+          fs[3] += (double)dis;
+        }
       }
+    }
+  }
+  if (fpart) {
+    __shared__ double sh[16];
+    for (int j = 0; j < 4; ++j) {
+      const double r = block_sum(fs[j], sh);
+      if (threadIdx.x == 0) fpart[blockIdx.x * 4 + j] = r;
     }
   }
 }

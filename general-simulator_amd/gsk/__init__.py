@@ -28,7 +28,7 @@ PATH_FUSED32, PATH_PLAIN64 = 0, 1
 RULE_SPECTRAL, RULE_REFERENCE = 0, 1
 K_EXACT, K_REFERENCE = 0, 1
 F32, F64 = 0, 1
-RED_ENERGY, RED_MASS, RED_MEAN_ETA, RED_SUM_ETA2 = 1, 2, 3, 4
+RED_ENERGY, RED_MASS, RED_MEAN_ETA, RED_SUM_ETA2, RED_MART_FORCES = 1, 2, 3, 4, 5
 
 
 class GskError(RuntimeError):
@@ -257,6 +257,11 @@ class Sim:
     def mass_real(self) -> float:
         """FP64 sum of the stored FP32 concentration field."""
         return float(self.reduce(RED_MASS)[1])
+
+    def mart_forces(self) -> np.ndarray:
+        """Martensite reference rule: means of the gradient, chemical, elastic and dislocation force
+        terms of the last step (the reference plugin's Get columns)."""
+        return self.reduce(RED_MART_FORCES)[:4].copy()
 
     def mean_eta(self) -> np.ndarray:
         return self.reduce(RED_MEAN_ETA)[: self.nvar].copy()

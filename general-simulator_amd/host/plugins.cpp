@@ -85,10 +85,22 @@ class GpuDynamics : public Dynamics {
   int nvar_ = 0;
   bool ref_ = false;  // `set rule reference`: the plugin's own update rule
 
-  // `set rule reference|spectral` (spectral by default)
-  bool read_rule() {
+  // `set rule reference|spectral`. A script that names no rule gets the plugin's own update rule,
+  // as the reference program runs it; `set rule spectral` selects the semi-implicit spectral rule.
+  bool read_rule(const char* plugin) {
     std::vector<std::string> w;
-    return var_values(*Vars, "rule", w) && w[0] == "reference";
+    const bool spectral = var_values(*Vars, "rule", w) && w[0] == "spectral";
+    if (!spectral && !(w.empty() || w[0] == "reference")) log_line("Warning: unknown rule \"" + w[0] + "\", using reference");
+    log_line(std::string("Info: ") + plugin + " runs the " + (spectral ? "semi-implicit spectral" : "reference") +
+             " update rule" + (spectral ? "" : " (set rule spectral selects the semi-implicit spectral rule)"));
+    return !spectral;
+  }
+
+  // One warning per key of `keys` the script set that the chosen rule does not read.
+  void warn_unread(std::initializer_list<const char*> keys, const char* plugin, const char* rule) {
+    for (const char* k : keys)
+      if (Vars->count(k))
+        log_line(std::string("Warning: ") + plugin + " ignores \"set " + k + "\" under the " + rule + " rule");
   }
 
   // optional field from the map (e.g. defect, dislocationstress)
@@ -224,9 +236,20 @@ class Dynamics_mart : public GpuDynamics {
  public:
   int Initialize() override {
     if (read_grid()) return Code_ERR;
-    ref_ = read_rule();
-    if (ref_) gsk_config_reference(&cfg_, GSK_PHYS_MART);
-    else gsk_config_default(&cfg_, GSK_PHYS_MART);
+    ref_ = read_rule("mart");
+    if (ref_) {
+      gsk_config_reference(&cfg_, GSK_PHYS_MART);
+      warn_unread({"kappa", "mobility", "precision"}, "mart", "reference");
+    } else {
+      gsk_config_default(&cfg_, GSK_PHYS_MART);
+      warn_unread({"transitiontemperature", "weightgradient", "weightchemical", "weightdislocation", "weightnoise",
+                   "chemical_form", "fix"}, "mart", "spectral");
+      for (const char* f : {"defect", "dislocationstress"}) {
+        auto it = Datas->find(f);
+        if (it != Datas->end() && !it->second.empty())
+          log_line(std::string("Warning: mart ignores the field ") + f + " under the spectral rule");
+      }
+    }
     read_common();
     auto vt = Datas->find("varianttensor");
     if (vt == Datas->end() || vt->second.empty() || vt->second.dims.size() != 3 || vt->second.dims[1] != 3 ||
@@ -244,7 +267,7 @@ class Dynamics_mart : public GpuDynamics {
     var_get(*Vars, "mobility", &cfg_.L, 1);
     var_get(*Vars, "weightelastic", &cfg_.w_el, 1);
     std::vector<std::string> w;
-    if (var_values(*Vars, "precision", w) && w[0] == "fp64") cfg_.path = GSK_PATH_PLAIN64;
+    if (!ref_ && var_values(*Vars, "precision", w) && w[0] == "fp64") cfg_.path = GSK_PATH_PLAIN64;
     if (ref_) {  // src/dynamics_mart.cu:18-41
       var_get(*Vars, "transitiontemperature", &cfg_.T0, 1);
       var_get(*Vars, "weightgradient", &cfg_.w_grad, 1);
@@ -288,6 +311,14 @@ class Dynamics_mart : public GpuDynamics {
     if (col == "energy" || col == "chemical_energy" || col == "gradient_energy" || col == "elastic_energy")
       return energy_col(col);
     if (col == "step") return fmt((double)gsk_get_step(sim_));
+    // means of the force terms of the last step (src/dynamics_mart.cu:446-451), reference rule
+    static const char* const forces[] = {"gradient", "chemical", "elastic", "dislocation"};
+    for (int j = 0; j < 4; ++j)
+      if (col == forces[j]) {
+        double m[16];
+        if (!ref_ || gsk_reduce(sim_, GSK_RED_MART_FORCES, m) != GSK_OK) return "nan";
+        return fmt(m[j]);
+      }
     return "nan";
   }
 
@@ -304,9 +335,14 @@ class Dynamics_precipitate : public GpuDynamics {
  public:
   int Initialize() override {
     if (read_grid()) return Code_ERR;
-    ref_ = read_rule();
-    if (ref_) gsk_config_reference(&cfg_, GSK_PHYS_PREC);
-    else gsk_config_default(&cfg_, GSK_PHYS_PREC);
+    ref_ = read_rule("precipitate");
+    if (ref_) {
+      gsk_config_reference(&cfg_, GSK_PHYS_PREC);
+      warn_unread({"stabilization", "dilatation", "precision"}, "precipitate", "reference");
+    } else {
+      gsk_config_default(&cfg_, GSK_PHYS_PREC);
+      warn_unread({"weightnoise"}, "precipitate", "spectral");
+    }
     read_common();
     auto vt = Datas->find("varianttensor");
     if (vt == Datas->end() || vt->second.empty()) {
@@ -342,6 +378,7 @@ class Dynamics_precipitate : public GpuDynamics {
       log_line("Error: elastic_source " + w[0] + " is not supported by the spectral rule (only c)");
       return Code_ERR;
     }
+    if (!ref_ && var_values(*Vars, "precision", w) && w[0] == "fp64") cfg_.path = GSK_PATH_PLAIN64;
     double mod[3] = {3.5, 1.5, 1.0};  // src/dynamics_precipitate.cu:55-56
     var_get(*Vars, "modulus", mod, 3);
     double C[81];

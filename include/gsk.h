@@ -55,6 +55,11 @@ extern "C" {
                                out[1] = FP64 sum of the stored FP32 real field */
 #define GSK_RED_MEAN_ETA 3  /* out[v] = mean of eta_v, v < nvar              */
 #define GSK_RED_SUM_ETA2 4  /* out[0] = sum over voxels and variants of eta^2 */
+#define GSK_RED_MART_FORCES 5 /* martensite reference rule: out[0..3] = means over voxels and variants of
+                                 the gradient (w_grad Grad), chemical, elastic and dislocation force
+                                 terms of the last step (zero before the first step); the reference's
+                                 Get columns gradient/chemical/elastic/dislocation
+                                 (src/dynamics_mart.cu:443-456) */
 
 typedef struct gsk_sim gsk_sim;
 

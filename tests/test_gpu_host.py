@@ -50,7 +50,7 @@ def test_gsi_spectral_mart_equals_library(gpu, tmp_path):
     say("gsi spectral martensite vs library")
     n, V, steps = 32, 3, 12
     eps = gsk.mart_eigenstrains(V)
-    script = (f"sys mart\nset gridsize {n} {n} {n} 1 1 1\nset deltatime 0.01\nset coefficient -0.5 3 3\n"
+    script = (f"sys mart\nset rule spectral\nset gridsize {n} {n} {n} 1 1 1\nset deltatime 0.01\nset coefficient -0.5 3 3\n"
               f"set weightelastic 20\nset noiseamp 0.002\nset seed 3\nset cubicmodulus 0.7 0.3 0.2\n"
               f"readhere varianttensor\n{tensor_lines(eps)}set initeta uniform -0.05 0.05 7\n"
               f"set binary on\ndump out 1 eta\ninfo 3 eta step\nrun {steps}\n")
@@ -162,7 +162,7 @@ def test_gsi_restart_with_ahead_steps(gpu, tmp_path):
     say("gsi restart through ahead_steps")
     n, V = 32, 3
     eps = gsk.mart_eigenstrains(V)
-    head = (f"sys mart\nset gridsize {n} {n} {n} 1 1 1\nset deltatime 0.01\nset coefficient -0.5 3 3\n"
+    head = (f"sys mart\nset rule spectral\nset gridsize {n} {n} {n} 1 1 1\nset deltatime 0.01\nset coefficient -0.5 3 3\n"
             f"set weightelastic 20\nset noiseamp 0.002\nset seed 9\nset cubicmodulus 0.7 0.3 0.2\n"
             f"readhere varianttensor\n{tensor_lines(eps)}set binary on\n")
     run_gsi(tmp_path, head + "set initeta uniform -0.05 0.05 4\ndump full 2 eta\nrun 10\n")
@@ -179,3 +179,40 @@ def test_gsi_restart_with_ahead_steps(gpu, tmp_path):
     assert not np.array_equal(full5, full10)
     assert d_again <= 2e-6 * scale
     assert d_wrong > 1e-4 * scale
+
+
+def test_gsi_reference_mart_info_force_columns(gpu, tmp_path):
+    """An in.mart-style script (no `set rule`: the plugin's own update rule) with the reference's
+    info columns temperature eta gradient chemical elastic dislocation
+    (scripts/in.mart:26, src/dynamics_mart.cu:443-456): every column is a number, and the force
+    columns equal the library's force means after the same steps."""
+    say("gsi reference martensite info force columns")
+    n, V, steps = 32, 3, 8
+    eps = gsk.mart_eigenstrains(V, base=0.0, axis=0.01)
+    rng = np.random.default_rng(3)
+    stress = rng.uniform(-0.01, 0.01, (6, n, n, n)).astype(np.float32)
+    with open(tmp_path / "ds.data", "wb") as f:
+        f.write(struct.pack("5i", 4, 6, n, n, n))
+        f.write(stress.tobytes())
+    script = (f"sys mart\nset gridsize {n} {n} {n} 1 1 1\nset deltatime 0.005\nset transitiontemperature 450\n"
+              f"set weightdislocation .1\nset weightelastic 1000\nset weightnoise 0\nset binary on\n"
+              f"read dislocationstress ds.data\nreadhere varianttensor\n{tensor_lines(eps)}"
+              f"set initeta uniform 0 0.2 3\ninfo 1 temperature eta gradient chemical elastic dislocation\n"
+              f"set fix temperature 460 300\nrun {steps}\n")
+    out = run_gsi(tmp_path, script)
+    assert "reference update rule" in out
+    rows = [l.split() for l in out.splitlines() if l.startswith("Step")]
+    last = rows[-1]
+    assert int(last[1]) == steps
+    vals = [float(x) for x in last[2:]]
+    assert len(vals) == 6 and all(np.isfinite(vals)), last
+    cfg = gsk.reference_config(gsk.PHYS_MART, n=n, nvar=V, dt=0.005, w_el=1000.0, w_dis=0.1, noise=0.0)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(gsk.cubic_modulus(3.5, 1.5, 1.0), eps)
+        s.set_field("dislocationstress", stress)
+        s.init_uniform("eta", 0.0, 0.2, 3)
+        for i in range(1, steps + 1):
+            s.set_param("temperature", float(f"{460 + (i / steps) * (300 - 460):.6g}"))
+            s.step(1)
+        want = s.mart_forces()
+    np.testing.assert_allclose(vals[2:], want, rtol=2e-5, atol=1e-12)

@@ -100,3 +100,57 @@ def test_reference_rule_errors(gpu):
     with gsk.Sim(spec) as s:
         with pytest.raises(gsk.GskError):
             s.set_field("defect", np.zeros((16, 16, 16), np.float32))
+
+
+def test_mart_reference_force_means(gpu):
+    """The Get columns gradient / chemical / elastic / dislocation (src/dynamics_mart.cu:446-451):
+    means over voxels and variants of the four force terms of the last step.
+    (a) Uniform eta, uniform dislocation stress: Grad = 0, Chem from the HEAD branches,
+        El_p = -w_el (B(0) eta^2)_p eta_p with B(0) = C eps eps, Dis_p = -w_dis K_p eta_p with
+        K_p = 25 sigma : eps_p -- each mean is a closed form.
+    (b) A random state without noise or defects: the four means add up to the mean increment
+        (eta1 - eta0)/dt of the FP64 oracle's step.
+    Before the first step every mean is zero."""
+    say("mart reference force means")
+    n, V = 16, 3
+    C = gsk.cubic_modulus(3.5, 1.5, 1.0)
+    eps = gsk.mart_eigenstrains(V, base=0.0, axis=0.01)
+    cfg = gsk.reference_config(gsk.PHYS_MART, n=n, nvar=V, temperature=300.0, noise=0.0, w_el=1e3, w_dis=2.0, dt=0.005)
+    vals = np.array([0.3, -0.2, 0.5])
+    sig6 = np.array([0.01, 0.002, -0.003, -0.02, 0.001, 0.015])
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(C, eps)
+        s.set_field("eta", (vals[:, None, None, None] * np.ones((V, n, n, n))).astype(np.float32))
+        s.set_field("dislocationstress", (sig6[:, None, None, None] * np.ones((6, n, n, n))).astype(np.float32))
+        assert np.all(s.mart_forces() == 0.0)
+        s.step(1)
+        got = s.mart_forces()
+    a1, a2, a3 = 0.02 * (cfg.temperature - cfg.T0), cfg.B, cfg.C
+    sc = np.sqrt(abs(a1 / (a2 - a3)))
+    S2 = ((sc * vals) ** 2).sum()
+    chem = np.where(vals >= 0, -cfg.w_chem * (sc * vals) * (a1 - a2 * (sc * vals) ** 2 + a3 * S2),
+                    1000.0 * cfg.w_chem * a2 * vals**2)
+    B0 = np.einsum("ijkl,pij,qkl->pq", C, eps, eps)
+    el = -cfg.w_el * (B0 @ vals**2) * vals
+    comp = [0, 1, 2, 1, 3, 4, 2, 4, 5]
+    K = 25.0 * np.einsum("ij,pij->p", sig6[comp].reshape(3, 3), eps)
+    dis = -cfg.w_dis * K * vals
+    want = np.array([0.0, chem.mean(), el.mean(), dis.mean()])
+    say(f"  uniform: got {got}, want {want}")
+    assert abs(got[0]) < 1e-6
+    np.testing.assert_allclose(got[1:], want[1:], rtol=2e-5)
+
+    cfg = gsk.reference_config(gsk.PHYS_MART, n=32, nvar=V, temperature=300.0, noise=0.0, w_el=1e3, dt=0.005)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(C, eps)
+        s.init_uniform("eta", 0.0, 0.2, 5)
+        eta0 = s.get_field("eta").astype(np.float64)
+        s.step(1)
+        got = s.mart_forces()
+    p = O.mart_ref_params(a1=a1, a2=a2, a3=a3, w_grad=cfg.w_grad, w_chem=cfg.w_chem, w_el=cfg.w_el, w_dis=cfg.w_dis,
+                          w_noise=0.0, dt=cfg.dt, dx=cfg.dx, kconv=cfg.kconv)
+    eta1 = O.mart_ref_run(eta0, O.btensor(32, C, eps, kconv=cfg.kconv), p, 1)
+    inc = ((eta1 - eta0) / cfg.dt).mean()
+    say(f"  random: sum of means {got.sum():.6e}, oracle mean increment {inc:.6e}, terms {got}")
+    assert got[3] == 0.0 and abs(got[0]) < 1e-6 * np.abs(got).max()
+    assert abs(got.sum() - inc) < 1e-4 * np.abs(got[1:3]).max()
