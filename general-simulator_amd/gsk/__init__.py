@@ -103,6 +103,7 @@ def lib() -> ctypes.CDLL:
     L.gsk_adjoint_info.argtypes = [P, P]
     L.gsk_adjoint_release.argtypes = [P]
     L.gsk_stress.argtypes = [I, D, I, P, P, I, D, I, P, P, P, I]
+    L.gsk_reduce_eta_noalloc.argtypes = [P, I, ctypes.c_longlong, I, P, I]
     _lib = L
     return L
 
@@ -451,6 +452,14 @@ def stress(eta: np.ndarray, C: np.ndarray, eps: np.ndarray, xi: float = 4000.0, 
     d = np.empty((n, n, n), np.float32)
     _check(lib().gsk_stress(n, dx, kconv, _ptr(Cc), _ptr(ep), V, xi, source, _ptr(e), _ptr(out), _ptr(d), device))
     return out, d
+
+
+def reduce_eta_noalloc(d_eta: int, nvar: int, nvox: int, what: int, device: int = 0) -> np.ndarray:
+    """gsk_reduce_eta_noalloc on nvar fields of nvox floats at device address d_eta: the means of
+    eta_v (RED_MEAN_ETA) or the sum of eta^2 (RED_SUM_ETA2), allocating no device memory."""
+    out = np.zeros(16)
+    _check(lib().gsk_reduce_eta_noalloc(ctypes.c_void_p(d_eta), nvar, nvox, what, _ptr(out), device))
+    return out[: (nvar if what == RED_MEAN_ETA else 1)].copy()
 
 
 def version() -> str:

@@ -217,6 +217,18 @@ int gsk_ensemble_step(gsk_ensemble* ens, int nsteps);
  * {mode, branches, steps_per_graph, ms per ensemble step}; *rows their count. */
 int gsk_ensemble_autotune(gsk_ensemble* ens, int trial_steps, double* table, int max_rows, int* rows);
 
+/* ---- no-allocation variant of gsk_reduce's real-space reductions. No counterpart in the
+ * reference; it serves callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc): it calls no hipMalloc/hipFree and keeps its partial sums in a fixed
+ * 40 KiB device array of the library, one per GPU, that every call on that GPU shares (calls that
+ * overlap on one GPU race; run them from one thread or serialise them).
+ * d_eta: nvar order-parameter fields of nvox values each, in device memory of `device`, layout
+ * [v][voxel] as gsk_get_field's (any nvox > 0, nvar 1..8). what = GSK_RED_MEAN_ETA: out[v] = mean of
+ * eta_v; GSK_RED_SUM_ETA2: out[0] = sum over voxels and variants of eta^2 -- the values gsk_reduce
+ * returns for a handle holding the same fields, bit for bit. Synchronous on the null stream:
+ * the caller's writes to d_eta must be complete. Larger nvar is rejected with GSK_ERR_ARG. */
+int gsk_reduce_eta_noalloc(const float* d_eta, int nvar, long long nvox, int what, double* out, int device);
+
 const char* gsk_last_error(void);
 const char* gsk_version(void);
 
