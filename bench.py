@@ -27,9 +27,10 @@ HBM_PEAK_GBS = 8000.0
 # state initialisation
 SLOT_KERNEL = {
     "y_fwd_S": r"k_ypass<", "y_inv_R": r"k_ypass<", "y_fwd_F": r"k_ypass<", "y_inv_eta": r"k_ypass<",
-    "y_fwd": r"k_ypass<", "y_inv": r"k_ypass<", "x_contract": r"k_xcontract", "zmid_force": r"k_zmid_mart<",
-    "x_update_g0": r"k_xupdate_mart<\d+, \d+, 0>", "x_update": r"k_xupdate_prec<\d+, \d+, 0>",
+    "y_fwd": r"k_ypass<", "y_inv": r"k_ypass<", "x_contract": r"k_xcontract(_r)?<|k_xcontract_split<\d+, \d+, 0>", "zmid_force": r"k_zmid_mart<",
+    "x_update_g0": r"k_xupdate_mart<\d+, \d+, 0>|k_xupdate_split<\d+, 0>", "x_update": r"k_xupdate_prec<\d+, \d+, 0>",
     "zinv_eta_sq": r"k_zinv_sq<", "zinv_local": r"k_zinv_local_r<", "plane_mid": r"k_plane_mid<", "plane_sq": r"k_plane_sq<",
+    "sub_mid": r"k_sub_mid<", "sub_sq": r"k_sub_sq<",
 }
 CALIB_MIB = 256
 

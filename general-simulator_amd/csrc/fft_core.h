@@ -381,4 +381,131 @@ __device__ __forceinline__ void fft_lines_p(T2* lds, const Lines& L, const T2* t
   else fft_lines_c_<T2, N, false>(lds, L, tw, typename RevImpl<Plan, Radices<>>::type{});
 }
 
+// ------------------------------------------------------ passes for blocks of TB threads
+// A pass of the all-LDS transform in a work group of TB threads whatever its butterfly count:
+// thread b takes the butterflies b, b + TB, ... (at most K of them, K * TB >= L.nl * N / R).
+template <typename T2, int N, int R, bool FWD, int Ns, int TB, int K>
+__device__ __forceinline__ void fft_pass_tb(T2* lds, const Lines& L, const T2* tw) {
+  const int nb = L.nl * (N / R);
+  T2 v[K][R];
+  int base[K], j[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const int b = (int)threadIdx.x + q * TB;
+    j[q] = -1;
+    base[q] = 0;
+    if (b < nb) {
+      j[q] = b / L.nl;
+      base[q] = L.base(b % L.nl);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = lds[base[q] + (j[q] + r * (N / R)) * L.ps];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    if (j[q] < 0) continue;
+    const int k = j[q] % Ns;
+    if (Ns > 1) {
+      const int st = k * (N / (Ns * R));
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        T2 w = tw[r * st];
+        if (!FWD) w.y = -w.y;
+        v[q][r] = cmul(v[q][r], w);
+      }
+    }
+    dftR<R, FWD>(v[q]);
+    const int d = (j[q] / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[base[q] + (d + r * Ns) * L.ps] = v[q][r];
+  }
+  __syncthreads();
+}
+template <typename T2, int N, bool FWD, int Ns, int TB, int NL, int R0, int... Rs>
+__device__ __forceinline__ void fft_chain_tb(T2* lds, const Lines& L, const T2* tw) {
+  fft_pass_tb<T2, N, R0, FWD, Ns, TB, (NL * (N / R0) + TB - 1) / TB>(lds, L, tw);
+  if constexpr (sizeof...(Rs) > 0) fft_chain_tb<T2, N, FWD, Ns * R0, TB, NL, Rs...>(lds, L, tw);
+}
+template <typename T2, int N, bool FWD, int TB, int NL, int... Rs>
+__device__ __forceinline__ void fft_lines_tb_(T2* lds, const Lines& L, const T2* tw, Radices<Rs...>) {
+  fft_chain_tb<T2, N, FWD, 1, TB, NL, Rs...>(lds, L, tw);
+}
+// ------------------------------------------------------ passes over one wave's lines
+// Orders the calling wave's LDS accesses: its writes and reads before the call are complete and
+// visible to all its lanes after it, and the compiler moves no LDS access across it.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// A pass of the all-LDS transform over lines that belong to the calling wave alone: lane l takes
+// the butterflies l, l + 64, ... (at most K), numbered line-fastest (consecutive lanes on
+// consecutive lines: lines that lie side by side) or, JF, butterfly-fastest (consecutive lanes
+// along one line: lines whose elements lie side by side). No work-group barrier.
+template <typename T2, int N, int R, bool FWD, int Ns, int K, bool JF>
+__device__ __forceinline__ void fft_wpass(T2* lds, const Lines& L, const T2* tw, int lane) {
+  constexpr int NB = N / R;  // butterflies per line
+  const int nb = L.nl * NB;
+  T2 v[K][R];
+  int base[K], j[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    const int b = lane + q * 64;
+    j[q] = -1;
+    base[q] = 0;
+    if (K * 64 <= nb || b < nb) {
+      j[q] = JF ? b % NB : b / L.nl;
+      base[q] = L.base(JF ? b / NB : b % L.nl);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = lds[base[q] + (j[q] + r * (N / R)) * L.ps];
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    if (j[q] < 0) continue;
+    const int k = j[q] % Ns;
+    if (Ns > 1) {
+      const int st = k * (N / (Ns * R));
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        T2 w = tw[r * st];
+        if (!FWD) w.y = -w.y;
+        v[q][r] = cmul(v[q][r], w);
+      }
+    }
+    dftR<R, FWD>(v[q]);
+    const int d = (j[q] / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) lds[base[q] + (d + r * Ns) * L.ps] = v[q][r];
+  }
+  wave_lds_sync();
+}
+template <typename T2, int N, bool FWD, int Ns, int NL, bool JF, int R0, int... Rs>
+__device__ __forceinline__ void fft_wchain(T2* lds, const Lines& L, const T2* tw, int lane) {
+  fft_wpass<T2, N, R0, FWD, Ns, (NL * (N / R0) + 63) / 64, JF>(lds, L, tw, lane);
+  if constexpr (sizeof...(Rs) > 0) fft_wchain<T2, N, FWD, Ns * R0, NL, JF, Rs...>(lds, L, tw, lane);
+}
+template <typename T2, int N, bool FWD, int NL, bool JF, int... Rs>
+__device__ __forceinline__ void fft_wlines_(T2* lds, const Lines& L, const T2* tw, int lane, Radices<Rs...>) {
+  fft_wchain<T2, N, FWD, 1, NL, JF, Rs...>(lds, L, tw, lane);
+}
+// All lines of L (at most NL of them, touched by no other wave) with an explicit plan (forward in
+// order, inverse reversed), by the calling wave; lane = its lane id; JF as for fft_wpass.
+template <typename T2, int N, bool FWD, class Plan, int NL, bool JF>
+__device__ __forceinline__ void fft_wlines(T2* lds, const Lines& L, const T2* tw, int lane) {
+  if constexpr (FWD) fft_wlines_<T2, N, true, NL, JF>(lds, L, tw, lane, Plan{});
+  else fft_wlines_<T2, N, false, NL, JF>(lds, L, tw, lane, typename RevImpl<Plan, Radices<>>::type{});
+}
+
+// All lines of L (at most NL of them) with an explicit plan (forward in order, inverse reversed)
+// in a work group of TB threads; every thread of the group must call it.
+template <typename T2, int N, bool FWD, class Plan, int TB, int NL>
+__device__ __forceinline__ void fft_lines_tb(T2* lds, const Lines& L, const T2* tw) {
+  if constexpr (FWD) fft_lines_tb_<T2, N, true, TB, NL>(lds, L, tw, Plan{});
+  else fft_lines_tb_<T2, N, false, TB, NL>(lds, L, tw, typename RevImpl<Plan, Radices<>>::type{});
+}
+
 }  // namespace gsk

@@ -216,7 +216,10 @@ int gsk_build_btensor(gsk_sim* s, const double* C81, const double* eps, int neps
   s->haveBargs = true;
   for (int i = 0; i < neps * 9; ++i) s->eps[i] = eps[i];
   s->haveEps = true;
-  if (s->cfg.path == GSK_PATH_FUSED32) {
+  if (s->cfg.path == GSK_PATH_FUSED32 && s->split) {
+    const long total = (long)s->sg.ntile * s->n * kSplitW;
+    hipLaunchKernelGGL(k_btensor_split<>, dim3(grid_1d(total)), dim3(256), 0, s->stream, s->Bt, s->sg, a);
+  } else if (s->cfg.path == GSK_PATH_FUSED32) {
     const long total = (long)(long)s->g.ntile * s->n * s->g.W;
     hipLaunchKernelGGL(k_btensor_tiles<>, dim3(grid_1d(total)), dim3(256), 0, s->stream, s->Bt, s->g, a);
   } else {
@@ -508,7 +511,8 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
     } else {
       const int ntile = s->g.ntile;
       const double kfac = c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx);
-      LAUNCH(r_state_grad<>, nb, 256, 0, s, s->state, s->g, s->F, (float)kfac, s->part);
+      if (s->split) LAUNCH(r_state_grad_split<>, nb, 256, 0, s, s->state, s->sg, s->F, (float)kfac, s->part);
+      else LAUNCH(r_state_grad<>, nb, 256, 0, s, s->state, s->g, s->F, (float)kfac, s->part);
       LAUNCH(r_finish<>, 1, 1024, 0, s, s->part, nb, s->F, s->rout);
       double gh[16];
       HIPCHK(hipMemcpyAsync(gh, s->rout, s->F * sizeof(double), hipMemcpyDeviceToHost, s->stream));

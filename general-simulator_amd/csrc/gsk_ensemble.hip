@@ -127,8 +127,9 @@ void ens_drop_graph(gsk_ensemble* e) {
 #endif
 
 bool batchable(const gsk_sim* s) {
+  // (split-y members keep their state and B tensor in another tile order than the batched kernels read)
   return s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL &&
-         s->V >= 1 && s->V <= 4;
+         s->V >= 1 && s->V <= 4 && !s->split;
 }
 
 void ens_drop_classes(gsk_ensemble* e) {

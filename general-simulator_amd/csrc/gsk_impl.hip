@@ -62,6 +62,25 @@ struct Impl {
     CHK(lds_limit((const void*)p_zrows<N, false>));
     CHK(lds_limit((const void*)p_strided<N, true>));
     CHK(lds_limit((const void*)p_strided<N, false>));
+    if constexpr (split_size<N>()) {
+      CHK(lds_limit((const void*)k_xcontract_split<N, 1, 0>));
+      CHK(lds_limit((const void*)k_xcontract_split<N, 2, 0>));
+      CHK(lds_limit((const void*)k_xcontract_split<N, 3, 0>));
+      CHK(lds_limit((const void*)k_xcontract_split<N, 4, 0>));
+      CHK(lds_limit((const void*)k_xcontract_split<N, 1, 1>));
+      CHK(lds_limit((const void*)k_xcontract_split<N, 2, 1>));
+      CHK(lds_limit((const void*)k_xcontract_split<N, 3, 1>));
+      CHK(lds_limit((const void*)k_xcontract_split<N, 4, 1>));
+      CHK(lds_limit((const void*)k_xupdate_split<N, 0>));
+      CHK(lds_limit((const void*)k_xupdate_split<N, 1>));
+      CHK(lds_limit((const void*)k_sub_mid<N, 1>));
+      CHK(lds_limit((const void*)k_sub_mid<N, 2>));
+      CHK(lds_limit((const void*)k_sub_mid<N, 3>));
+      CHK(lds_limit((const void*)k_sub_mid<N, 4>));
+      CHK(lds_limit((const void*)k_sub_sq<N>));
+      CHK(lds_limit((const void*)k_sub_fwd<N, 0>));
+      CHK(lds_limit((const void*)k_sub_fwd<N, 1>));
+    }
     if constexpr (plane_size<N>()) {
       CHK(lds_limit((const void*)k_plane_sq<N>));
       CHK(lds_limit((const void*)k_plane_sq0<N>));
@@ -190,7 +209,80 @@ struct Impl {
     return GSK_OK;
   }
 
+  // ---- split-y schedule (n = 256): sub-plane launches have grid (n * P sub-planes, fields)
+  static constexpr size_t SPLIT_XLDS = (size_t)(N + N * kSplitW) * sizeof(float2);
+  static int split_fwd(gsk_sim* s, bool square) {
+    if constexpr (split_size<N>()) {
+      const dim3 grid(N * kSplitP, s->F);
+      if (square)
+        LAUNCH((k_sub_fwd<N, 1>), grid, kSubT, sub_lds<N>(), s, s->fld, s->kbuf, s->sg, s->twN, s->twM, s->rtn);
+      else
+        LAUNCH((k_sub_fwd<N, 0>), grid, kSubT, sub_lds<N>(), s, s->fld, s->kbuf, s->sg, s->twN, s->twM, s->rtn);
+      return GSK_OK;
+    } else {
+      (void)s; (void)square;
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+    }
+  }
+  template <int MODE>
+  static int split_xcontract(gsk_sim* s, const MartK& mk) {
+    if constexpr (split_size<N>()) {
+      const int gx = s->sg.ntile;
+      if (MODE == 1 && gx > s->rblocks) return fail(GSK_ERR, "reduction scratch too small");
+      constexpr int T = split_xthreads<N>();
+      switch (s->F) {
+        case 1: LAUNCH((k_xcontract_split<N, 1, MODE>), gx, T, SPLIT_XLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part); break;
+        case 2: LAUNCH((k_xcontract_split<N, 2, MODE>), gx, T, SPLIT_XLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part); break;
+        case 3: LAUNCH((k_xcontract_split<N, 3, MODE>), gx, T, SPLIT_XLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part); break;
+        default: LAUNCH((k_xcontract_split<N, 4, MODE>), gx, T, SPLIT_XLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part);
+      }
+      return GSK_OK;
+    } else {
+      (void)s; (void)mk;
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+    }
+  }
+  template <int MODE>
+  static int split_xupdate(gsk_sim* s, const MartK& mk) {
+    if constexpr (split_size<N>()) {
+      LAUNCH((k_xupdate_split<N, MODE>), dim3(s->sg.ntile, s->F), split_xthreads<N>(), SPLIT_XLDS, s, s->kbuf,
+             s->state, s->sg, mk, s->twN);
+      return GSK_OK;
+    } else {
+      (void)s; (void)mk;
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+    }
+  }
+  static int step_mart_split(gsk_sim* s) {
+    if constexpr (split_size<N>()) {
+      const MartK mk = mart_consts(s);
+      const dim3 grid(N * kSplitP, s->F);
+      const int gm = N * kSplitP * s->F;  // k_sub_mid: one dimension, see its id mapping
+      static_assert((N * kSplitP) % 8 == 0, "sub-planes in groups of 8");
+      if (!s->sq_ready) CHK(split_fwd(s, true));
+      CHK(split_xcontract<0>(s, mk));
+      switch (s->F) {
+        case 1: LAUNCH((k_sub_mid<N, 1>), gm, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn); break;
+        case 2: LAUNCH((k_sub_mid<N, 2>), gm, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn); break;
+        case 3: LAUNCH((k_sub_mid<N, 3>), gm, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn); break;
+        default: LAUNCH((k_sub_mid<N, 4>), gm, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn);
+      }
+      CHK(split_xupdate<0>(s, mk));
+      LAUNCH((k_sub_sq<N>), grid, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, s->twN, s->twM, s->rtn, s->dstep);
+      s->sq_ready = true;
+      return GSK_OK;
+    } else {
+      (void)s;
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+    }
+  }
+
   static int init_state(gsk_sim* s) {
+    if (s->split) {  // E-form transform of eta, then the state in the split tile order
+      CHK(split_fwd(s, false));
+      MartK mk = mart_consts(s);
+      return split_xupdate<1>(s, mk);
+    }
     if (s->cfg.physics == GSK_PHYS_MART) {
       CHK(zfwd(s, Z_COPY, s->fld, s->F));
       CHK(ypass(s, true, s->F));
@@ -209,7 +301,9 @@ struct Impl {
   // plane_on): the first kernel of a step, unless the previous step's fused last pass has left
   // it there
   static int square_fwd(gsk_sim* s) {
-    if (plane_on(s)) {
+    if (s->split) {
+      CHK(split_fwd(s, true));
+    } else if (plane_on(s)) {
       if constexpr (plane_size<N>()) {
         LAUNCH((k_plane_sq0<N>), dim3(N, s->F), plane_threads<N>(), plane_lds<N>(), s, s->fld, s->kbuf, geo_next(s),
                s->twN, s->twM, s->rtn);
@@ -222,6 +316,7 @@ struct Impl {
   }
 
   static int step_mart_fused(gsk_sim* s) {
+    if (s->split) return step_mart_split(s);
     const int gx = s->g.ntile;
     const int F = s->F;
     MartK mk = mart_consts(s);
@@ -525,6 +620,16 @@ struct Impl {
   }
 
   static int energy_mart_el_fused(gsk_sim* s, double* el) {
+    if (s->split) {  // E-form S_hat; the split contraction kernel sums the energy per tile
+      // and only reads the buffer, so the carried S_hat stays what the next step starts from
+      if (!s->sq_ready) CHK(square_fwd(s));
+      MartK mk = mart_consts(s);
+      CHK(split_xcontract<1>(s, mk));
+      LAUNCH(r_finish<>, 1, 1024, 0, s, s->part, s->sg.ntile, 1, s->rout);
+      HIPCHK(hipMemcpyAsync(el, s->rout, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+      HIPCHK(hipStreamSynchronize(s->stream));
+      return GSK_OK;
+    }
     // S_hat = FFT(eta^2) through z and y; the energy kernel applies x and contracts.
     if (plane_on(s)) {
       if (!s->sq_ready) CHK(square_fwd(s));  // y,z-transformed already

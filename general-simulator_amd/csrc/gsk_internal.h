@@ -23,6 +23,7 @@
 #include "kernels_adjoint.h"
 #include "kernels_aux.h"
 #include "kernels_plane.h"
+#include "kernels_split.h"
 #include "kernels_ref.h"
 
 using namespace gsk;
@@ -180,6 +181,10 @@ struct gsk_sim {
   // fields (left there by the last step's fused inverse/square pass); cleared by anything that
   // writes the fields or uses the buffer otherwise
   bool sq_ready = false;
+  // split-y schedule (n = 256, spectral martensite): its geometry; the B tensor and the k-space
+  // state are then in its tile order, and the carried S_hat is in E-form
+  bool split = false;
+  SGeo sg{};
   int order = 0;  // visiting order of the last per-field launch
   long long step = 0;
   // profiling
@@ -211,6 +216,17 @@ inline bool plane_on(const gsk_sim* s) {
   }();
   return !off && (s->n == 32 || s->n == 64 || s->n == 128) && s->F >= 1 && s->F <= 4 &&
          s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL;
+}
+
+// Split-y schedule (kernels_split.h): the spectral martensite step at n = 256 with 1..4 variants,
+// when GSK_SPLIT=1 is set in the environment (otherwise the chain of y and z kernels runs).
+inline bool split_wanted(const gsk_sim* s) {
+  static const bool on = [] {
+    const char* e = std::getenv("GSK_SPLIT");
+    return e && e[0] == '1';
+  }();
+  return on && s->n == 256 && s->F >= 1 && s->F <= 4 && s->cfg.path == GSK_PATH_FUSED32 &&
+         s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL;
 }
 
 inline size_t nvox(const gsk_sim* s) { return (size_t)s->n * s->n * s->n; }
@@ -448,6 +464,15 @@ inline void choose_geometry(gsk_sim* s, int Fz, int Fx_resident) {
   s->lds_y = (size_t)(n + n * W) * sizeof(float2);
   s->lds_xc = (size_t)(n + (size_t)Fx_resident * n * W) * sizeof(float2);
   s->lds_z = (2 * M + (size_t)Fz * L * zpitch(M)) * sizeof(float2);
+  s->split = split_wanted(s);
+  if (s->split) {
+    SGeo& q = s->sg;
+    q.n = n; q.M = M; q.m = n / kSplitP;
+    q.nnyq = q.m / kSplitWc;
+    q.nchm = M / kSplitWc;
+    q.ntile = q.nnyq + q.m * q.nchm;
+    q.kfs = g.kfs; q.nq = g.nq; q.nvox = g.nvox;
+  }
 }
 
 inline void choose_plain(gsk_sim* s) {
@@ -511,6 +536,13 @@ inline void build_slots(gsk_sim* s) {
   }
   if (s->cfg.physics == GSK_PHYS_MART) {
     const int npq = F * (F + 1) / 2;
+    if (s->split) {  // four launches: the split-y schedule
+      s->slots.push_back({"x_contract", F * 2 * hs + npq * hs / 2});
+      s->slots.push_back({"sub_mid", F * (2 * hs + F * rv)});  // R and F spectra, eta of every variant
+      s->slots.push_back({"x_update_g0", F * 4 * hs});
+      s->slots.push_back({"sub_sq", F * (2 * hs + rv)});  // eta and S spectra, eta written
+      return;
+    }
     const bool pl = plane_on(s);
     if (!pl) s->slots.push_back({"y_fwd_S", F * 2 * hs});
     s->slots.push_back({"x_contract", F * 2 * hs + npq * hs / 2});

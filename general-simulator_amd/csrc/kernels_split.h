@@ -1,0 +1,589 @@
+// Split-y schedule of the FP32 fused martensite step (n = 256): four launches per step.
+//
+// The y transform of an x-plane (length n) is split as n = P * m (P = 4, m = 64):
+//   inverse  x[P t + j] = sum_k1 w_m^(k1 t) A_j[k1],
+//            A_j[k1]    = w_n^(k1 j) sum_k2 X[k1 + m k2] w_P^(k2 j)
+//   forward  E_j[k1]    = sum_t x[P t + j] w_m^(-k1 t),
+//            X[k1 + m k2] = sum_j w_P^(-k2 j) (w_n^(-k1 j) E_j[k1])          (w_L = exp(2 pi i / L))
+// The x kernels apply the P-point stage and its twiddles in registers across lanes (the P values
+// of one (x, k1, kz) sit in lanes c, c ^ Wc, c ^ 2 Wc of a wave), between their x transforms and
+// the pointwise k-space work. Sub-plane j of an x-plane (the m rows y = P t + j) then needs only
+// the m-point y transforms, so a work group keeps it in LDS through y inverse, the z pass and y
+// forward: 64 rows of 129 values at pitch 136, 72 KB (two work groups per CU).
+//
+// Sub-plane j of x-plane x is stored in the rows x*n + j*m + k1 (k1 < m) of the half-spectrum
+// layout (kz < n/2 in rows of n/2 values, the Nyquist value kz = n/2 in the Nyquist plane):
+// A_j (inverse side, written by the x kernels) or E_j (forward side, written by the sub-plane
+// kernels).
+//
+// Step: k_xcontract_split (E-form S_hat -> A-form R_hat), k_sub_mid (A-form R_hat -> E-form F_hat,
+// the force on the sub-plane's rows), k_xupdate_split (E-form F_hat, k-space state -> A-form
+// eta_hat), k_sub_sq (A-form eta_hat -> eta written, E-form S_hat of the next step).
+// HBM bytes per voxel and variant: 8 + 2 + 8 + 4 V (eta of every variant, read through L2 by the
+// V groups of a sub-plane) + 16 + 12, against 28 + 28 for the two y -> z -> y chains they replace.
+#pragma once
+#include "kernels_plane.h"
+
+namespace gsk {
+
+// Columns per sub-plane group of an x tile: 8 (a 64-byte row segment, 512-thread x work groups)
+// or 4 (32 bytes, 256 threads); the x tiles that share a 128-byte line run on one XCD.
+#ifndef GSK_SPLIT_WC
+#define GSK_SPLIT_WC 8
+#endif
+// Waves per SIMD the sub-plane kernels are compiled for (0: the compiler's choice).
+#ifndef GSK_SUB_WPE
+#define GSK_SUB_WPE 0
+#endif
+#if GSK_SUB_WPE > 0
+#define GSK_SUB_ATTR __attribute__((amdgpu_waves_per_eu(GSK_SUB_WPE)))
+#else
+#define GSK_SUB_ATTR
+#endif
+
+// Threads of a sub-plane work group: 512 (at most 128 VGPRs, so two groups, 2 x 72 KB of LDS,
+// share a CU and one group's HBM phases overlap the other's LDS passes) or 1024.
+#ifndef GSK_SUB_T
+#define GSK_SUB_T 512
+#endif
+
+constexpr int kSplitP = 4;  // sub-planes per x-plane
+constexpr int kSubT = GSK_SUB_T;
+constexpr int kSplitWc = GSK_SPLIT_WC;
+constexpr int kSplitW = kSplitP * kSplitWc;
+constexpr int kSplitG = 16 / kSplitWc;  // x tiles per 128-byte line
+static_assert(kSplitWc == 4 || kSplitWc == 8 || kSplitWc == 16, "sub-plane group width 4, 8 or 16");
+
+template <int N> constexpr bool split_size() { return N == 256; }
+
+// Geometry of the split schedule.
+struct SGeo {
+  int n, M, m;        // edge, n/2, rows per sub-plane (n / P)
+  int nnyq, nchm;     // Nyquist tiles (m / Wc), kz chunks per k1 (M / Wc)
+  int ntile;          // nnyq + m * nchm
+  long long kfs, nq, nvox;
+};
+
+__host__ __device__ constexpr int bitrev4(int j) { return ((j & 1) << 1) | (j >> 1); }
+
+// x tile of the split schedule: W = P * Wc columns, column c = j * Wc + i (sub-plane j). Main
+// tiles cover one k1 and the kz chunk [kz0, kz0 + Wc); Nyquist tiles the kz = n/2 value of the Wc
+// consecutive k1 = k1_0 + i. Element (c, x) of a field: base + j * gs + i + x * ps (E / A form).
+// After the lane stage, column c carries ky = k1 + m * bitrev(j) (and kz).
+struct XSTile {
+  unsigned base, gs, ps;
+  int k1, kz0;
+  bool nyq;
+  __device__ __forceinline__ unsigned coff(int c) const {
+    return base + (unsigned)(c / kSplitWc) * gs + (unsigned)(c % kSplitWc);
+  }
+  __device__ __forceinline__ int ky(int c, int m) const {
+    return (nyq ? k1 + c % kSplitWc : k1) + m * bitrev4(c / kSplitWc);
+  }
+  __device__ __forceinline__ int kz(int c, int M) const { return nyq ? M : kz0 + c % kSplitWc; }
+  __device__ __forceinline__ int k1c(int c) const { return nyq ? k1 + c % kSplitWc : k1; }
+};
+
+// Work group -> tile. Work groups are dealt to the 8 XCDs round robin (XCD = id % 8); the G
+// tiles Gu .. Gu + G - 1 share every 128-byte line of their row segments, so the ids
+// 8Gq + k + 8i (i < G: same XCD k, dispatched together) take them and each line is fetched into
+// one L2.
+__device__ __forceinline__ int xs_swizzle(int b, int ntile) {
+  constexpr int G = kSplitG, B = 8 * G;
+  if (G == 1 || b >= ntile / B * B) return b;
+  const int r = b % B;
+  return b - r + G * (r % 8) + r / 8;
+}
+
+__device__ __forceinline__ XSTile xs_tile(int t, const SGeo& g) {
+  if (t < g.nnyq) return XSTile{(unsigned)(g.nq + (long long)t * kSplitWc), (unsigned)g.m, (unsigned)g.n, t * kSplitWc, 0, true};
+  const int u = t - g.nnyq, k1 = u / g.nchm, ch = u % g.nchm;
+  return XSTile{(unsigned)(k1 * g.M + ch * kSplitWc), (unsigned)(g.m * g.M), (unsigned)(g.n * g.M), k1,
+                ch * kSplitWc, false};
+}
+
+// The P = 4 point stage across the lanes c, c ^ Wc, c ^ 2 Wc, c ^ 3 Wc (j = lane / Wc % 4).
+// Forward: lane j holds w_n^(-k1 j) E_j on entry and X[k1 + m * bitrev(j)] on exit.
+// Inverse: lane j holds X[k1 + m * bitrev(j)] on entry and sum_k2 X[k1 + m k2] w_4^(k2 j) on exit.
+// Butterfly of the lane pairs (l, l ^ MASK) of a wave: lower lane lo + hi, upper lane lo - hi.
+// The partner value comes over the VALU: DPP row rotation by 8 (MASK 8, = lane ^ 8 inside a row
+// of 16), gfx950's v_permlane16_swap / v_permlane32_swap (MASK 16 / 32; swapping a register with
+// itself leaves the lower row's value in the first result and the upper row's in the second, in
+// both lanes of a pair). Other masks: ds_bpermute through the LDS crossbar.
+template <int MASK> __device__ __forceinline__ float lane_bfly(float v, bool up) {
+  if constexpr (MASK == 16) {
+    const unsigned u = __float_as_uint(v);
+    const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    const float lo = __uint_as_float(r[0]), hi = __uint_as_float(r[1]);
+    return up ? lo - hi : lo + hi;
+  } else if constexpr (MASK == 32) {
+    const unsigned u = __float_as_uint(v);
+    const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    const float lo = __uint_as_float(r[0]), hi = __uint_as_float(r[1]);
+    return up ? lo - hi : lo + hi;
+  } else if constexpr (MASK == 8) {
+    const float p = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128 /* row_ror:8 */, 0xF, 0xF, false));
+    return up ? p - v : v + p;
+  } else {
+    const float p = __shfl_xor(v, MASK, 64);
+    return up ? p - v : v + p;
+  }
+}
+template <int MASK> __device__ __forceinline__ float2 lane_bfly2(float2 v, bool up) {
+  return make_float2(lane_bfly<MASK>(v.x, up), lane_bfly<MASK>(v.y, up));
+}
+// (every lane of the wave active; j = lane / Wc % 4, so j & 1 is lane & Wc and j & 2 lane & 2 Wc)
+template <bool FWD> __device__ __forceinline__ float2 lane_dft4(float2 v, int j) {
+  if (FWD) {
+    v = lane_bfly2<2 * kSplitWc>(v, j & 2);
+    if (j == 3) v = rot<true>(v);
+    v = lane_bfly2<kSplitWc>(v, j & 1);
+  } else {
+    v = lane_bfly2<kSplitWc>(v, j & 1);
+    if (j == 3) v = rot<false>(v);
+    v = lane_bfly2<2 * kSplitWc>(v, j & 2);
+  }
+  return v;
+}
+
+template <int N> constexpr int split_xthreads() { return kSplitW * (N / edge_radix<N>()); }
+
+// B tensor in the split tile order, packed p <= q: ((t * npq + pq) * n + kx) * W + c.
+template <int = 0>
+__global__ void k_btensor_split(float* __restrict__ Bt, SGeo g, BArgs a) {
+  const int n = g.n, T = n * kSplitW;
+  const int V = a.V1, npq = V * (V + 1) / 2;
+  const long total = (long)g.ntile * T;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int e = (int)(i % T);
+    const long t = i / T;
+    const XSTile s = xs_tile((int)t, g);
+    const int c = e % kSplitW, kx = e / kSplitW;
+    double out[64];
+    btensor_half_spec(kx, s.ky(c, g.m), s.kz(c, g.M), n, a, out);
+    for (int p = 0; p < V; ++p)
+      for (int q = p; q < V; ++q) {
+        const int pq = p * V - (p * (p - 1)) / 2 + (q - p);
+        Bt[(t * npq + pq) * T + e] = (float)out[p * V + q];
+      }
+  }
+}
+
+// Contraction: E-form S_hat -> x FFT -> lane stage -> R_hat_p = sum_q B_pq S_hat_q / n^3 -> inverse
+// lane stage -> inverse x FFT -> A-form R_hat (MODE 0). MODE 1: the elastic energy sum
+// sum_k w(kz) sum_pq B_pq Re(S_p* S_q) of the tile into part[tile] instead (w = 1 for kz in
+// {0, n/2}, 2 otherwise).
+template <int N, int V, int MODE>
+__global__ void __launch_bounds__(split_xthreads<N>()) k_xcontract_split(float2* __restrict__ kbuf,
+                                                                         const float* __restrict__ Bt, SGeo g, MartK mk,
+                                                                         const float2* __restrict__ twN,
+                                                                         double* __restrict__ part) {
+  constexpr int RL = edge_radix<N>(), W = kSplitW, T = N * W;
+  constexpr int NPQ = V * (V + 1) / 2;
+  extern __shared__ float2 smem[];
+  __shared__ double sh[16];
+  float2* tw = smem;
+  float2* tile = smem + N;
+  const int t = xs_swizzle(blockIdx.x, g.ntile);
+  const XSTile s = xs_tile(t, g);
+  const int b = threadIdx.x, c = b % W, jl = c / kSplitWc;
+  load_tw<N>(tw, twN);
+  __syncthreads();
+  const Lines Ls{W, W, 1, 0, W};
+  float2 S[V][RL];
+#pragma unroll
+  for (int f = 0; f < V; ++f) {
+    const float2* fb = kbuf + f * g.kfs;
+    fft_g2r<float2, N>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, S[f]);
+  }
+  // w_n^(-k1 j): the lane stage's twiddle of this column
+  const float2 wt = tw[(s.k1c(c) * jl) % N];
+#pragma unroll
+  for (int f = 0; f < V; ++f)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) S[f][r] = lane_dft4<true>(cmul(S[f][r], wt), jl);
+  // keep the B loads below the lane stage: hoisted above it they hold 16 * NPQ registers at once
+  asm volatile("" ::: "memory");
+  const float* Bb = Bt + (long)t * NPQ * T;
+  if constexpr (MODE == 0) {
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int e = r * (N / RL) * W;
+      float bv[NPQ];
+#pragma unroll
+      for (int pq = 0; pq < NPQ; ++pq) bv[pq] = at32(Bb, (unsigned)(pq * T + b + e));
+      float2 R[V];
+#pragma unroll
+      for (int p = 0; p < V; ++p) {
+        float ax = 0.f, ay = 0.f;
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          const int lo = p < q ? p : q, hi = p < q ? q : p;
+          const int pq = lo * V - (lo * (lo - 1)) / 2 + (hi - lo);
+          ax += bv[pq] * S[q][r].x;
+          ay += bv[pq] * S[q][r].y;
+        }
+        R[p] = make_float2(ax * mk.inv_n3, ay * mk.inv_n3);
+      }
+#pragma unroll
+      for (int p = 0; p < V; ++p) S[p][r] = R[p];
+    }
+    const float2 wi = make_float2(wt.x, -wt.y);
+#pragma unroll
+    for (int f = 0; f < V; ++f)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) S[f][r] = cmul(lane_dft4<false>(S[f][r], jl), wi);
+#pragma unroll
+    for (int f = 0; f < V; ++f) {
+      float2* fb = kbuf + f * g.kfs;
+      fft_r2g<float2, N>(tile, Ls, tw, S[f],
+                         [&](int cc, int kx, float2 v) { at32(fb, s.coff(cc) + (unsigned)kx * s.ps) = v; });
+    }
+  } else {
+    const int kz = s.kz(c, g.M);
+    const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const int e = r * (N / RL) * W;
+      double sum = 0.0;
+#pragma unroll
+      for (int p = 0; p < V; ++p)
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+          const int lo = p < q ? p : q, hi = p < q ? q : p;
+          const int pq = lo * V - (lo * (lo - 1)) / 2 + (hi - lo);
+          sum += (double)at32(Bb, (unsigned)(pq * T + b + e)) *
+                 ((double)S[p][r].x * S[q][r].x + (double)S[p][r].y * S[q][r].y);
+        }
+      acc += w * sum;
+    }
+    const double rr = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = rr;
+  }
+}
+
+// Semi-implicit update: E-form F_hat -> x FFT -> lane stage -> eta_hat = (eta_hat - dt L F_hat) /
+// (1 + dt L kappa k^2) on the tile-ordered k-space state -> inverse lane stage -> inverse x FFT ->
+// A-form eta_hat / n^3 (MODE 0). MODE 1: the state from the E-form transform of eta, nothing
+// written to the k-space buffer. One field per work group (grid = tiles x fields).
+template <int N, int MODE>
+__global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* __restrict__ kbuf,
+                                                                       float2* __restrict__ state, SGeo g, MartK mk,
+                                                                       const float2* __restrict__ twN) {
+  constexpr int RL = edge_radix<N>(), W = kSplitW, T = N * W;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  float2* tile = smem + N;
+  const int t = xs_swizzle(blockIdx.x, g.ntile), f = blockIdx.y, F = gridDim.y;
+  const XSTile s = xs_tile(t, g);
+  const int b = threadIdx.x, c = b % W, j = b / W, jl = c / kSplitWc;
+  float2* st = state + ((long)t * F + f) * T;
+  float2 sv[RL];
+  if (MODE == 0) {
+#pragma unroll
+    for (int r = 0; r < RL; ++r) sv[r] = at32(st, (unsigned)(b + r * (N / RL) * W));
+  }
+  load_tw<N>(tw, twN);
+  __syncthreads();
+  const Lines Ls{W, W, 1, 0, W};
+  float2* fb = kbuf + f * g.kfs;
+  float2 v[RL];
+  fft_g2r<float2, N>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, v);
+  const float2 wt = tw[(s.k1c(c) * jl) % N];
+#pragma unroll
+  for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true>(cmul(v[r], wt), jl);
+  if (MODE == 1) {
+#pragma unroll
+    for (int r = 0; r < RL; ++r) at32(st, (unsigned)(b + r * (N / RL) * W)) = v[r];
+    return;
+  }
+  const float kyv = kcomp(s.ky(c, g.m), N, mk.kfac);
+  const float kzv = mk.kfac * (float)s.kz(c, g.M);
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    const float kxv = kcomp(j + r * (N / RL), N, mk.kfac);
+    const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+    const float den = 1.f / (1.f + mk.dtLk * k2);
+    const float2 nh = make_float2((sv[r].x - mk.dtL * v[r].x) * den, (sv[r].y - mk.dtL * v[r].y) * den);
+    at32(st, (unsigned)(b + r * (N / RL) * W)) = nh;
+    v[r] = make_float2(nh.x * mk.inv_n3, nh.y * mk.inv_n3);
+  }
+  const float2 wi = make_float2(wt.x, -wt.y);
+#pragma unroll
+  for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false>(v[r], jl), wi);
+  fft_r2g<float2, N>(tile, Ls, tw, v, [&](int cc, int kx, float2 val) { at32(fb, s.coff(cc) + (unsigned)kx * s.ps) = val; });
+}
+
+// Gradient energy from the split-order state: sum_f sum_k w(kz) k^2 |eta_hat_f|^2 per field.
+template <int = 0>
+__global__ void r_state_grad_split(const float2* __restrict__ state, SGeo g, int F, float kfac,
+                                   double* __restrict__ part) {
+  __shared__ double sh[16];
+  const int n = g.n, T = n * kSplitW;
+  const long total = (long)g.ntile * F * T;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int e = (int)(i % T);
+    const long tf = i / T;
+    const int f = (int)(tf % F);
+    const XSTile s = xs_tile((int)(tf / F), g);
+    const int c = e % kSplitW, kx = e / kSplitW;
+    const int ky = s.ky(c, g.m), kz = s.kz(c, g.M);
+    const double kxv = (double)kfac * ((kx < n / 2) ? kx : kx - n), kyv = (double)kfac * ((ky < n / 2) ? ky : ky - n),
+                 kzv = (double)kfac * kz;
+    const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
+    const float2 v = state[i];
+    acc[f] += w * (kxv * kxv + kyv * kyv + kzv * kzv) * ((double)v.x * v.x + (double)v.y * v.y);
+  }
+  for (int f = 0; f < F; ++f) {
+    const double r = block_sum(acc[f], sh);
+    if (threadIdx.x == 0) part[blockIdx.x * F + f] = r;
+  }
+}
+
+// ---------------------------------------------------------------- sub-plane kernels
+// LDS of a sub-plane work group: the m-point (y) and M-point (z) twiddles, the split factors and
+// the sub-plane: m rows of n/2 + 1 values at pitch n/2 + 8 (four consecutive rows then cover the
+// 64 LDS banks once in the radix-16 row pass).
+//
+// Each wave owns CW columns and RW rows of the sub-plane. A y transform (along a column) or a row
+// operation (C2R / R2C split, z transform, pointwise work) only touches the calling wave's
+// columns or rows, so it needs no work-group barrier: a step has four (after the load, after y
+// inverse, after the row phase, after y forward) instead of two per LDS pass.
+// The value v, which the compiler cannot see through: index arithmetic built on it in two places
+// is computed twice instead of being kept live from the first to the second.
+__device__ __forceinline__ int opaque(int v) {
+  int r;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+}
+
+template <int N> struct SubLds {
+  static constexpr int M = N / 2, Y = N / kSplitP, PP = M + 8, Q = M / 2, T = kSubT;
+  static constexpr int NW = T / 64, RW = Y / NW, CW = M / NW;
+  static_assert(Y % NW == 0 && M % NW == 0, "rows and columns in equal shares per wave");
+  float2 *twy, *twz, *rt, *pl;
+  int w, lane;
+  __device__ SubLds(float2* smem)
+      : twy(smem), twz(smem + Y), rt(smem + Y + M), pl(smem + Y + 2 * M), w((int)threadIdx.x / 64),
+        lane((int)threadIdx.x % 64) {}
+  __device__ void tables(const float2* __restrict__ twN, const float2* __restrict__ twM,
+                         const float2* __restrict__ rtn) const {
+    for (int i = threadIdx.x; i < Y; i += T) twy[i] = twN[i * kSplitP];  // exp(-2 pi i k / m)
+    for (int i = threadIdx.x; i < M; i += T) {
+      twz[i] = twM[i];
+      rt[i] = rtn[i];
+    }
+  }
+  // rows rb .. rb + m - 1 of one field's half spectrum <-> the sub-plane (whole work group)
+  __device__ void load(const float2* __restrict__ field, const SGeo& g, unsigned rb) const {
+    for (int e = threadIdx.x; e < Y * M; e += T) pl[(e / M) * PP + e % M] = at32(field, rb * M + (unsigned)e);
+    for (int y = threadIdx.x; y < Y; y += T) pl[y * PP + M] = at32(field, (unsigned)g.nq + rb + y);
+  }
+  __device__ void store(float2* __restrict__ field, const SGeo& g, unsigned rb) const {
+    for (int e = threadIdx.x; e < Y * M; e += T) at32(field, rb * M + (unsigned)e) = pl[(e / M) * PP + e % M];
+    for (int y = threadIdx.x; y < Y; y += T) at32(field, (unsigned)g.nq + rb + y) = pl[y * PP + M];
+  }
+  // m-point transforms of the wave's CW main columns (wave 0: also of the Nyquist column)
+  template <bool FWD> __device__ void ypass() const {
+    using Plan = typename PlanOf<Y>::type;
+    const int ln = opaque(lane);
+    fft_wlines<float2, Y, FWD, Plan, CW, false>(pl + w * CW, Lines{CW, CW, 1, 0, PP}, twy, ln);
+    if (w == 0) fft_wlines<float2, Y, FWD, Plan, 1, false>(pl + M, Lines{1, 1, 0, 0, PP}, twy, ln);
+  }
+  // the wave's first row
+  __device__ float2* rows() const { return pl + w * RW * PP; }
+  // C2R split of the wave's rows into the packed M-point spectrum, then the M-point inverse
+  __device__ void c2r() const {
+    float2* R0 = rows();
+    const int ln = opaque(lane);
+    for (int i = 0; i < RW; ++i) {
+      float2* z = R0 + i * PP;
+      for (int k = ln; k < Q; k += 64) {
+        if (k == 0) {
+          const float2 a = z[0], b = z[M], c = z[Q];
+          float2 zh, unused;
+          c2r_pair(c, c, rt[Q], zh, unused);
+          z[0] = make_float2(a.x + b.x, a.x - b.x);
+          z[Q] = zh;
+        } else {
+          float2 zk, zm;
+          c2r_pair(z[k], z[M - k], rt[k], zk, zm);
+          z[k] = zk;
+          z[M - k] = zm;
+        }
+      }
+    }
+    wave_lds_sync();
+    fft_wlines<float2, M, false, typename PlanOf<M>::type, RW, true>(R0, Lines{RW, RW, PP, 0, 1}, twz, ln);
+  }
+  // M-point forward along the wave's rows, then the split back to n/2 + 1 values per row
+  __device__ void r2c() const {
+    float2* R0 = rows();
+    const int ln = opaque(lane);
+    fft_wlines<float2, M, true, typename PlanOf<M>::type, RW, true>(R0, Lines{RW, RW, PP, 0, 1}, twz, ln);
+    for (int i = 0; i < RW; ++i) {
+      float2* z = R0 + i * PP;
+      for (int k = ln; k < Q; k += 64) {
+        if (k == 0) {
+          const float2 z0 = z[0], zh = z[Q];
+          z[0] = make_float2(z0.x + z0.y, 0.f);
+          z[M] = make_float2(z0.x - z0.y, 0.f);
+          z[Q] = make_float2(zh.x, -zh.y);
+        } else {
+          float2 xk, xm;
+          r2c_pair(z[k], z[M - k], rt[k], xk, xm);
+          z[k] = xk;
+          z[M - k] = xm;
+        }
+      }
+    }
+  }
+};
+
+template <int N> constexpr size_t sub_lds() {
+  using S = SubLds<N>;
+  return (size_t)(S::Y + 2 * S::M + S::Y * S::PP) * sizeof(float2);
+}
+
+// Middle of a step, per (sub-plane, variant p): A-form R_hat_p -> y inverse -> C2R -> G on the
+// sub-plane's rows y = P t + j: F_p = A eta_p - B eta_p^2 + C eta_p sum_q eta_q^2 + 2 w eta_p R_p
+// + a xi_p (eta of every variant as 16-byte quads, xi from the Philox blocks of the voxel quads)
+// -> R2C -> y forward -> E-form F_hat_p. Grid: n * P * V work groups; id 8Vq + 8p + k takes
+// variant p of sub-plane 8q + k, so the V work groups of a sub-plane, which read the same eta,
+// run on one XCD (id % 8 = k) within 8V ids of each other and share its L2.
+template <int N, int VT>
+__global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_mid(float2* __restrict__ kbuf, const float* __restrict__ eta, SGeo g,
+                                                  MartK mk, const float2* __restrict__ twN,
+                                                  const float2* __restrict__ twM, const float2* __restrict__ rtn) {
+  using SL = SubLds<N>;
+  constexpr int Q = SL::Q, PP = SL::PP, Y = SL::Y, RW = SL::RW;
+  extern __shared__ float2 smem[];
+  const SL s(smem);
+  const int grp = 8 * VT, rem = (int)blockIdx.x % grp, p = rem / 8;
+  const int sp = (int)blockIdx.x / grp * 8 + rem % 8;  // sub-plane
+  const int x = sp / kSplitP, j = sp % kSplitP;
+  const unsigned rb = (unsigned)(x * N + j * Y);
+  float2* field = kbuf + p * g.kfs;
+  s.load(field, g, rb);
+  s.tables(twN, twM, rtn);
+  __syncthreads();
+  s.template ypass<false>();
+  __syncthreads();
+  s.c2r();
+  const uint32_t step = step_of(mk.step, mk.stepp);
+  const float* eb = eta + (size_t)x * N * N;  // the x-plane's voxels
+  for (int i = 0; i < RW; ++i) {
+    const int t = s.w * RW + i;
+    float2* z = s.rows() + i * PP;
+    for (int q = s.lane; q < Q; q += 64) {
+      const unsigned vo = (unsigned)((kSplitP * t + j) * N + 4 * q);  // voxel offset inside the x-plane
+      float4 ep = make_float4(0.f, 0.f, 0.f, 0.f), s2 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int f = 0; f < VT; ++f) {
+        const float4 tq = at32(reinterpret_cast<const float4*>(eb + f * g.nvox), vo / 4);
+        s2.x += tq.x * tq.x; s2.y += tq.y * tq.y; s2.z += tq.z * tq.z; s2.w += tq.w * tq.w;
+        if (f == p) ep = tq;
+      }
+      float xi[4] = {0.f, 0.f, 0.f, 0.f};
+      if (mk.noise != 0.f) normal4(mk.seed, (unsigned)((x * N * N + vo) >> 2), (uint32_t)p, step, xi);
+      const float2 ra = z[2 * q], rbv = z[2 * q + 1];
+      const float e4[4] = {ep.x, ep.y, ep.z, ep.w}, s4[4] = {s2.x, s2.y, s2.z, s2.w}, r4[4] = {ra.x, ra.y, rbv.x, rbv.y};
+      float o[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float et = e4[u];
+        o[u] = mk.A * et - mk.B * et * et + mk.C * et * s4[u] + mk.two_wel * et * r4[u] + mk.noise * xi[u];
+      }
+      z[2 * q] = make_float2(o[0], o[1]);
+      z[2 * q + 1] = make_float2(o[2], o[3]);
+    }
+  }
+  wave_lds_sync();
+  s.r2c();
+  __syncthreads();
+  s.template ypass<true>();
+  __syncthreads();
+  s.store(field, g, rb);
+}
+
+// End of a step, per (sub-plane, field): A-form eta_hat -> y inverse -> C2R -> eta written to the
+// real field and squared -> R2C -> y forward -> E-form S_hat of the next step. adv (may be null):
+// the step counter the launch advances once. Grid: (n * P, F).
+template <int N>
+__global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_sq(float2* __restrict__ kbuf, float* __restrict__ out, SGeo g,
+                                                 const float2* __restrict__ twN, const float2* __restrict__ twM,
+                                                 const float2* __restrict__ rtn, uint32_t* adv) {
+  if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *adv += 1u;
+  using SL = SubLds<N>;
+  constexpr int Q = SL::Q, PP = SL::PP, Y = SL::Y, RW = SL::RW;
+  extern __shared__ float2 smem[];
+  const SL s(smem);
+  const int x = blockIdx.x / kSplitP, j = blockIdx.x % kSplitP, f = blockIdx.y;
+  const unsigned rb = (unsigned)(x * N + j * Y);
+  float2* field = kbuf + f * g.kfs;
+  s.load(field, g, rb);
+  s.tables(twN, twM, rtn);
+  __syncthreads();
+  s.template ypass<false>();
+  __syncthreads();
+  s.c2r();
+  float* dst = out + f * g.nvox + (size_t)x * N * N;
+  for (int i = 0; i < RW; ++i) {
+    const int t = s.w * RW + i;
+    float2* z = s.rows() + i * PP;
+    for (int q = s.lane; q < Q; q += 64) {
+      const float2 a = z[2 * q], b = z[2 * q + 1];
+      at32(reinterpret_cast<float4*>(dst), (unsigned)((kSplitP * t + j) * N + 4 * q) / 4) = make_float4(a.x, a.y, b.x, b.y);
+      // each lane squares the very elements it has just read
+      z[2 * q] = make_float2(a.x * a.x, a.y * a.y);
+      z[2 * q + 1] = make_float2(b.x * b.x, b.y * b.y);
+    }
+  }
+  wave_lds_sync();
+  s.r2c();
+  __syncthreads();
+  s.template ypass<true>();
+  __syncthreads();
+  s.store(field, g, rb);
+}
+
+// Real fields `in` (OP 1: squared) -> R2C along z -> y forward -> E-form spectra in kbuf, per
+// (sub-plane, field): the carried S_hat of a fresh start, bit-identical to what k_sub_sq leaves
+// (same FP32 values, same code), and (OP 0) the transform of eta that initialises the state.
+template <int N, int OP>
+__global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_fwd(const float* __restrict__ in, float2* __restrict__ kbuf, SGeo g,
+                                                  const float2* __restrict__ twN, const float2* __restrict__ twM,
+                                                  const float2* __restrict__ rtn) {
+  using SL = SubLds<N>;
+  constexpr int Q = SL::Q, PP = SL::PP, Y = SL::Y, RW = SL::RW;
+  extern __shared__ float2 smem[];
+  const SL s(smem);
+  const int x = blockIdx.x / kSplitP, j = blockIdx.x % kSplitP, f = blockIdx.y;
+  const unsigned rb = (unsigned)(x * N + j * Y);
+  const float* src = in + f * g.nvox + (size_t)x * N * N;
+  s.tables(twN, twM, rtn);
+  for (int i = 0; i < RW; ++i) {  // the wave's rows
+    const int t = s.w * RW + i;
+    float2* z = s.rows() + i * PP;
+    for (int q = s.lane; q < Q; q += 64) {
+      const float4 v = at32(reinterpret_cast<const float4*>(src), (unsigned)((kSplitP * t + j) * N + 4 * q) / 4);
+      if (OP == 1) {
+        z[2 * q] = make_float2(v.x * v.x, v.y * v.y);
+        z[2 * q + 1] = make_float2(v.z * v.z, v.w * v.w);
+      } else {
+        z[2 * q] = make_float2(v.x, v.y);
+        z[2 * q + 1] = make_float2(v.z, v.w);
+      }
+    }
+  }
+  __syncthreads();  // the tables
+  s.r2c();
+  __syncthreads();
+  s.template ypass<true>();
+  __syncthreads();
+  s.store(kbuf + f * g.kfs, g, rb);
+}
+
+}  // namespace gsk

@@ -256,12 +256,14 @@ def test_mart_fused_vs_plain_at_512(gpu):
     assert float(np.abs(got - want).max()) < 2e-6 * float(np.abs(want).max())
 
 
-@pytest.mark.parametrize("n", [64, 96])
+@pytest.mark.parametrize("n", [64, 96, 256])
 def test_mart_square_carry_bit_identical(gpu, n):
     """A step leaves the transformed eta^2 of its result for the next step (along z and y at
-    n = 64, where the plane-resident passes run; along z at n = 96). Steps that run back to
-    back must give bit for bit the trajectory of steps separated by energy reductions (which
-    use the k-space buffer, so the next step transforms eta^2 afresh)."""
+    n = 64, where the plane-resident passes run; along z at n = 96 and 256, or along z and y at
+    256 in the sub-plane kernels of the split-y schedule, GSK_SPLIT=1). Steps that run back to back must
+    give bit for bit the trajectory of steps separated by energy reductions and a set_step to
+    the current index (which drops the carried transform, so the next step transforms eta^2
+    afresh)."""
     say(f"mart eta^2 carry-over n={n}")
     kw = dict(A=-0.5, w_el=20.0, noise=0.002, seed=5)
     a = mart_sim(n, 3, **kw)
@@ -272,6 +274,7 @@ def test_mart_square_carry_bit_identical(gpu, n):
     for k in (2, 1, 3):
         b.step(k)
         b.energy()
+        b.set_step(b.step_index)
     ea, eb = a.get_field("eta"), b.get_field("eta")
     a.close()
     b.close()
