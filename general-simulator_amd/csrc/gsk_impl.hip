@@ -62,6 +62,19 @@ struct Impl {
     CHK(lds_limit((const void*)p_zrows<N, false>));
     CHK(lds_limit((const void*)p_strided<N, true>));
     CHK(lds_limit((const void*)p_strided<N, false>));
+    // the batched ensemble instantiations (member table)
+    CHK(lds_limit((const void*)k_ypass<N, true, true>));
+    CHK(lds_limit((const void*)k_ypass<N, false, true>));
+    CHK(lds_limit((const void*)k_xcontract_r<N, 1, true>));
+    CHK(lds_limit((const void*)k_xcontract_r<N, 2, true>));
+    CHK(lds_limit((const void*)k_xcontract_r<N, 3, true>));
+    CHK(lds_limit((const void*)k_xcontract_r<N, 4, true>));
+    CHK(lds_limit((const void*)k_xupdate_mart<N, 1, 0, true>));
+    CHK(lds_limit((const void*)k_zmid_mart<M, 1, true>));
+    CHK(lds_limit((const void*)k_zmid_mart<M, 2, true>));
+    CHK(lds_limit((const void*)k_zmid_mart<M, 3, true>));
+    CHK(lds_limit((const void*)k_zmid_mart<M, 4, true>));
+    CHK(lds_limit((const void*)k_zinv_sq<M, true>));
     if constexpr (split_size<N>()) {
       CHK(lds_limit((const void*)k_xcontract_split<N, 1, 0>));
       CHK(lds_limit((const void*)k_xcontract_split<N, 2, 0>));
@@ -82,6 +95,11 @@ struct Impl {
       CHK(lds_limit((const void*)k_sub_fwd<N, 1>));
     }
     if constexpr (plane_size<N>()) {
+      CHK(lds_limit((const void*)k_plane_sq<N, true>));
+      CHK(lds_limit((const void*)k_plane_mid<N, 1, true>));
+      CHK(lds_limit((const void*)k_plane_mid<N, 2, true>));
+      CHK(lds_limit((const void*)k_plane_mid<N, 3, true>));
+      CHK(lds_limit((const void*)k_plane_mid<N, 4, true>));
       CHK(lds_limit((const void*)k_plane_sq<N>));
       CHK(lds_limit((const void*)k_plane_sq0<N>));
       CHK(lds_limit((const void*)k_plane_mid<N, 1>));
@@ -256,9 +274,12 @@ struct Impl {
   static int step_mart_split(gsk_sim* s) {
     if constexpr (split_size<N>()) {
       const MartK mk = mart_consts(s);
-      const dim3 grid(N * kSplitP, s->F);
-      const int gm = N * kSplitP * s->F;  // k_sub_mid: one dimension, see its id mapping
+      // resident sub-plane work groups: two per CU, k_sub_mid's count a multiple of 8 F
       static_assert((N * kSplitP) % 8 == 0, "sub-planes in groups of 8");
+      if (!s->cus) HIPCHK(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->dev));
+      const int nitems = N * kSplitP * s->F, g8 = 8 * s->F;
+      const int gm = std::min(nitems, std::max(g8, 2 * s->cus / g8 * g8));
+      const int gq = std::min(nitems, std::max(8, 2 * s->cus / 8 * 8));
       if (!s->sq_ready) CHK(split_fwd(s, true));
       CHK(split_xcontract<0>(s, mk));
       switch (s->F) {
@@ -268,7 +289,8 @@ struct Impl {
         default: LAUNCH((k_sub_mid<N, 4>), gm, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn);
       }
       CHK(split_xupdate<0>(s, mk));
-      LAUNCH((k_sub_sq<N>), grid, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, s->twN, s->twM, s->rtn, s->dstep);
+      LAUNCH((k_sub_sq<N>), gq, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, s->twN, s->twM, s->rtn, s->dstep,
+             nitems);
       s->sq_ready = true;
       return GSK_OK;
     } else {
@@ -372,10 +394,10 @@ struct Impl {
     float2* nk = nullptr;
     auto y = [&](bool fwd) -> int {
       if (fwd)
-        hipLaunchKernelGGL((k_ypass<N, true>), dim3(gx, F, cnt), dim3(s0->xthreads), s0->lds_y, st, nk, geo_next(s0),
+        hipLaunchKernelGGL((k_ypass<N, true, true>), dim3(gx, F, cnt), dim3(s0->xthreads), s0->lds_y, st, nk, geo_next(s0),
                            F, s0->twN, tab);
       else
-        hipLaunchKernelGGL((k_ypass<N, false>), dim3(gx, F, cnt), dim3(s0->xthreads), s0->lds_y, st, nk, geo_next(s0),
+        hipLaunchKernelGGL((k_ypass<N, false, true>), dim3(gx, F, cnt), dim3(s0->xthreads), s0->lds_y, st, nk, geo_next(s0),
                            F, s0->twN, tab);
       HIPCHK(hipGetLastError());
       return GSK_OK;
@@ -385,7 +407,7 @@ struct Impl {
     switch (F) {
 #define GSK_XC(V)                                                                                                  \
   case V:                                                                                                          \
-    hipLaunchKernelGGL((k_xcontract_r<N, V>), dim3(gx, 1, cnt), dim3(s0->xthreads), s0->lds_y, st, nk,            \
+    hipLaunchKernelGGL((k_xcontract_r<N, V, true>), dim3(gx, 1, cnt), dim3(s0->xthreads), s0->lds_y, st, nk,            \
                        (const float*)nullptr, s0->g, mk, s0->twN, tab);                                            \
     break;
       GSK_XC(1) GSK_XC(2) GSK_XC(3) GSK_XC(4)
@@ -398,7 +420,7 @@ struct Impl {
         switch (F) {
 #define GSK_PM(V)                                                                                                  \
   case V:                                                                                                          \
-    hipLaunchKernelGGL((k_plane_mid<N, V>), dim3(N, F, cnt), dim3(plane_threads<N>()), plane_lds<N>(), st, nk,    \
+    hipLaunchKernelGGL((k_plane_mid<N, V, true>), dim3(N, F, cnt), dim3(plane_threads<N>()), plane_lds<N>(), st, nk,    \
                        (const float*)nullptr, geo_next(s0), mk, s0->twN, s0->twM, s0->rtn, tab);                   \
     break;
           GSK_PM(1) GSK_PM(2) GSK_PM(3) GSK_PM(4)
@@ -412,7 +434,7 @@ struct Impl {
       switch (F) {
 #define GSK_ZM(V)                                                                                                  \
   case V:                                                                                                          \
-    hipLaunchKernelGGL((k_zmid_mart<M, V>), dim3(s0->n * s0->n / LZ, 1, cnt), dim3(ZMT), ZM_LDS, st, nk,         \
+    hipLaunchKernelGGL((k_zmid_mart<M, V, true>), dim3(s0->n * s0->n / LZ, 1, cnt), dim3(ZMT), ZM_LDS, st, nk,         \
                        (const float*)nullptr, s0->g, mk, s0->twM, s0->rtn, tab);                                   \
     break;
         GSK_ZM(1) GSK_ZM(2) GSK_ZM(3) GSK_ZM(4)
@@ -422,17 +444,17 @@ struct Impl {
       HIPCHK(hipGetLastError());
       CHK(y(true));
     }
-    hipLaunchKernelGGL((k_xupdate_mart<N, 1, 0>), dim3(gx, F, cnt), dim3(s0->xthreads), s0->lds_y, st, nk,
+    hipLaunchKernelGGL((k_xupdate_mart<N, 1, 0, true>), dim3(gx, F, cnt), dim3(s0->xthreads), s0->lds_y, st, nk,
                        (float2*)nullptr, geo_next(s0), mk, s0->twN, 0, tab);
     HIPCHK(hipGetLastError());
     if (pl) {
       if constexpr (plane_size<N>()) {
-        hipLaunchKernelGGL((k_plane_sq<N>), dim3(N, F, cnt), dim3(plane_threads<N>()), plane_lds<N>(), st, nk,
+        hipLaunchKernelGGL((k_plane_sq<N, true>), dim3(N, F, cnt), dim3(plane_threads<N>()), plane_lds<N>(), st, nk,
                            (float*)nullptr, geo_next(s0), s0->twN, s0->twM, s0->rtn, (uint32_t*)nullptr, tab);
       }
     } else {
       CHK(y(false));
-      hipLaunchKernelGGL((k_zinv_sq<M>), dim3(rows, F, cnt), dim3(s0->zthreads), lz, st, nk, (float*)nullptr,
+      hipLaunchKernelGGL((k_zinv_sq<M, true>), dim3(rows, F, cnt), dim3(s0->zthreads), lz, st, nk, (float*)nullptr,
                          geo_next(s0), s0->twM, s0->rtn, (uint32_t*)nullptr, tab);
     }
     HIPCHK(hipGetLastError());

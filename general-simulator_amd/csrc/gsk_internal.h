@@ -139,6 +139,7 @@ struct gsk_sim {
   int n = 0, V = 0, F = 0, dev = 0;
   Geo g{};
   int zthreads = 0, xthreads = 0;
+  int cus = 0;  // compute units of the device (0 until first needed)
   size_t lds_z = 0, lds_y = 0, lds_xc = 0;
   hipStream_t stream = nullptr;
   // twiddles

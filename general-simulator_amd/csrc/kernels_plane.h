@@ -171,14 +171,14 @@ template <int N> struct PlaneLds {
 
 // Grid: (n planes, V variants[, members]) through geo_tile / geo_field. VT: the variant count
 // (1..4). mt (may be null): batched ensemble launch, member blockIdx.z's buffers and constants.
-template <int N, int VT>
+template <int N, int VT, bool BT = false>
 __global__ void __launch_bounds__(plane_threads<N>()) k_plane_mid(float2* __restrict__ kbuf, const float* __restrict__ eta,
                                                               Geo g, MartK mk, const float2* __restrict__ twN,
                                                               const float2* __restrict__ twM,
                                                               const float2* __restrict__ rtn,
                                                               const MemberPtrs* __restrict__ mt) {
   static_assert(plane_size<N>() && VT >= 1 && VT <= 4, "plane pass: n = 32, 64, 128 and 1..4 variants");
-  if (mt) {
+  if (BT && mt) {
     kbuf = mt[blockIdx.z].kbuf;
     eta = mt[blockIdx.z].fld;
     mk = mt[blockIdx.z].mk;
@@ -231,14 +231,14 @@ __global__ void __launch_bounds__(plane_threads<N>()) k_plane_mid(float2* __rest
 
 // Grid: (n planes, F fields[, members]). adv (may be null): step counter advanced once by the
 // launch (per member in a batched launch, whose table also supplies kbuf and the real fields).
-template <int N>
+template <int N, bool BT = false>
 __global__ void __launch_bounds__(plane_threads<N>()) k_plane_sq(float2* __restrict__ kbuf, float* __restrict__ out, Geo g,
                                                              const float2* __restrict__ twN,
                                                              const float2* __restrict__ twM,
                                                              const float2* __restrict__ rtn, uint32_t* adv,
                                                              const MemberPtrs* __restrict__ mt) {
   static_assert(plane_size<N>(), "plane pass: n = 32, 64, 128");
-  if (mt) {
+  if (BT && mt) {
     kbuf = mt[blockIdx.z].kbuf;
     out = mt[blockIdx.z].fld;
     adv = mt[blockIdx.z].dstep;

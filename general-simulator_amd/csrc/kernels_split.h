@@ -66,6 +66,14 @@ struct SGeo {
 
 __host__ __device__ constexpr int bitrev4(int j) { return ((j & 1) << 1) | (j >> 1); }
 
+// The value v, which the compiler cannot see through: index arithmetic built on it in two places
+// is computed twice instead of being kept live from the first to the second.
+__device__ __forceinline__ int opaque(int v) {
+  int r;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+}
+
 // x tile of the split schedule: W = P * Wc columns, column c = j * Wc + i (sub-plane j). Main
 // tiles cover one k1 and the kz chunk [kz0, kz0 + Wc); Nyquist tiles the kz = n/2 value of the Wc
 // consecutive k1 = k1_0 + i. Element (c, x) of a field: base + j * gs + i + x * ps (E / A form).
@@ -236,8 +244,9 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xcontract_split(float2*
 #pragma unroll
     for (int f = 0; f < V; ++f) {
       float2* fb = kbuf + f * g.kfs;
-      fft_r2g<float2, N>(tile, Ls, tw, S[f],
-                         [&](int cc, int kx, float2 v) { at32(fb, s.coff(cc) + (unsigned)kx * s.ps) = v; });
+      fft_r2g<float2, N>(tile, Ls, tw, S[f], [&](int cc, int kx, float2 v) {
+        at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = v;
+      });
     }
   } else {
     const int kz = s.kz(c, g.M);
@@ -295,24 +304,26 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true>(cmul(v[r], wt), jl);
   if (MODE == 1) {
 #pragma unroll
-    for (int r = 0; r < RL; ++r) at32(st, (unsigned)(b + r * (N / RL) * W)) = v[r];
+    for (int r = 0; r < RL; ++r) at32(st, (unsigned)(opaque(b) + r * (N / RL) * W)) = v[r];
     return;
   }
   const float kyv = kcomp(s.ky(c, g.m), N, mk.kfac);
   const float kzv = mk.kfac * (float)s.kz(c, g.M);
+  const int bs = opaque(b);  // the state stores recompute what the loads above computed
 #pragma unroll
   for (int r = 0; r < RL; ++r) {
     const float kxv = kcomp(j + r * (N / RL), N, mk.kfac);
     const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
     const float den = 1.f / (1.f + mk.dtLk * k2);
     const float2 nh = make_float2((sv[r].x - mk.dtL * v[r].x) * den, (sv[r].y - mk.dtL * v[r].y) * den);
-    at32(st, (unsigned)(b + r * (N / RL) * W)) = nh;
+    at32(st, (unsigned)(bs + r * (N / RL) * W)) = nh;
     v[r] = make_float2(nh.x * mk.inv_n3, nh.y * mk.inv_n3);
   }
   const float2 wi = make_float2(wt.x, -wt.y);
 #pragma unroll
   for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false>(v[r], jl), wi);
-  fft_r2g<float2, N>(tile, Ls, tw, v, [&](int cc, int kx, float2 val) { at32(fb, s.coff(cc) + (unsigned)kx * s.ps) = val; });
+  fft_r2g<float2, N>(tile, Ls, tw, v,
+                     [&](int cc, int kx, float2 val) { at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = val; });
 }
 
 // Gradient energy from the split-order state: sum_f sum_k w(kz) k^2 |eta_hat_f|^2 per field.
@@ -351,14 +362,6 @@ __global__ void r_state_grad_split(const float2* __restrict__ state, SGeo g, int
 // operation (C2R / R2C split, z transform, pointwise work) only touches the calling wave's
 // columns or rows, so it needs no work-group barrier: a step has four (after the load, after y
 // inverse, after the row phase, after y forward) instead of two per LDS pass.
-// The value v, which the compiler cannot see through: index arithmetic built on it in two places
-// is computed twice instead of being kept live from the first to the second.
-__device__ __forceinline__ int opaque(int v) {
-  int r;
-  asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
-  return r;
-}
-
 template <int N> struct SubLds {
   static constexpr int M = N / 2, Y = N / kSplitP, PP = M + 8, Q = M / 2, T = kSubT;
   static constexpr int NW = T / 64, RW = Y / NW, CW = M / NW;
@@ -376,10 +379,26 @@ template <int N> struct SubLds {
       rt[i] = rtn[i];
     }
   }
-  // rows rb .. rb + m - 1 of one field's half spectrum <-> the sub-plane (whole work group)
-  __device__ void load(const float2* __restrict__ field, const SGeo& g, unsigned rb) const {
-    for (int e = threadIdx.x; e < Y * M; e += T) pl[(e / M) * PP + e % M] = at32(field, rb * M + (unsigned)e);
-    for (int y = threadIdx.x; y < Y; y += T) pl[y * PP + M] = at32(field, (unsigned)g.nq + rb + y);
+  // rows rb .. rb + m - 1 of one field's half spectrum -> the sub-plane, in two halves: fetch()
+  // into registers (the loads stay in flight while the work group transforms the previous
+  // sub-plane), put() into LDS; store() writes the sub-plane back (whole work group)
+  static constexpr int LE = Y * M / T;
+  static_assert(Y * M % T == 0 && Y <= T, "whole rows per thread share");
+  struct Pre {
+    float2 v[LE], nq;
+  };
+  __device__ void fetch(const float2* __restrict__ field, const SGeo& g, unsigned rb, Pre& p) const {
+#pragma unroll
+    for (int u = 0; u < LE; ++u) p.v[u] = at32(field, rb * M + (unsigned)(threadIdx.x + u * T));
+    if ((int)threadIdx.x < Y) p.nq = at32(field, (unsigned)g.nq + rb + threadIdx.x);
+  }
+  __device__ void put(const Pre& p) const {
+#pragma unroll
+    for (int u = 0; u < LE; ++u) {
+      const int e = threadIdx.x + u * T;
+      pl[(e / M) * PP + e % M] = p.v[u];
+    }
+    if ((int)threadIdx.x < Y) pl[threadIdx.x * PP + M] = p.nq;
   }
   __device__ void store(float2* __restrict__ field, const SGeo& g, unsigned rb) const {
     for (int e = threadIdx.x; e < Y * M; e += T) at32(field, rb * M + (unsigned)e) = pl[(e / M) * PP + e % M];
@@ -450,103 +469,138 @@ template <int N> constexpr size_t sub_lds() {
 // Middle of a step, per (sub-plane, variant p): A-form R_hat_p -> y inverse -> C2R -> G on the
 // sub-plane's rows y = P t + j: F_p = A eta_p - B eta_p^2 + C eta_p sum_q eta_q^2 + 2 w eta_p R_p
 // + a xi_p (eta of every variant as 16-byte quads, xi from the Philox blocks of the voxel quads)
-// -> R2C -> y forward -> E-form F_hat_p. Grid: n * P * V work groups; id 8Vq + 8p + k takes
-// variant p of sub-plane 8q + k, so the V work groups of a sub-plane, which read the same eta,
-// run on one XCD (id % 8 = k) within 8V ids of each other and share its L2.
+// -> R2C -> y forward -> E-form F_hat_p.
+// Resident work groups (two per CU, grid G a multiple of 8V): group b takes the items b, b + G,
+// ... and loads each next item's spectrum while it transforms the current one. Item 8Vq + 8p + k
+// is variant p of sub-plane 8q + k, so the V items of a sub-plane, which read the same eta, run at
+// the same time on one XCD (item % 8 = b % 8 = k) and share its L2.
 template <int N, int VT>
 __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_mid(float2* __restrict__ kbuf, const float* __restrict__ eta, SGeo g,
                                                   MartK mk, const float2* __restrict__ twN,
                                                   const float2* __restrict__ twM, const float2* __restrict__ rtn) {
   using SL = SubLds<N>;
-  constexpr int Q = SL::Q, PP = SL::PP, Y = SL::Y, RW = SL::RW;
+  constexpr int Q = SL::Q, PP = SL::PP, Y = SL::Y, RW = SL::RW, NI = N * kSplitP * VT;
   extern __shared__ float2 smem[];
   const SL s(smem);
-  const int grp = 8 * VT, rem = (int)blockIdx.x % grp, p = rem / 8;
-  const int sp = (int)blockIdx.x / grp * 8 + rem % 8;  // sub-plane
-  const int x = sp / kSplitP, j = sp % kSplitP;
-  const unsigned rb = (unsigned)(x * N + j * Y);
-  float2* field = kbuf + p * g.kfs;
-  s.load(field, g, rb);
-  s.tables(twN, twM, rtn);
-  __syncthreads();
-  s.template ypass<false>();
-  __syncthreads();
-  s.c2r();
+  auto item = [&](int it, int& x, int& j, int& p) {
+    const int grp = 8 * VT, rem = it % grp, sp = it / grp * 8 + rem % 8;
+    p = rem / 8;
+    x = sp / kSplitP;
+    j = sp % kSplitP;
+  };
   const uint32_t step = step_of(mk.step, mk.stepp);
-  const float* eb = eta + (size_t)x * N * N;  // the x-plane's voxels
-  for (int i = 0; i < RW; ++i) {
-    const int t = s.w * RW + i;
-    float2* z = s.rows() + i * PP;
-    for (int q = s.lane; q < Q; q += 64) {
-      const unsigned vo = (unsigned)((kSplitP * t + j) * N + 4 * q);  // voxel offset inside the x-plane
-      float4 ep = make_float4(0.f, 0.f, 0.f, 0.f), s2 = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-      for (int f = 0; f < VT; ++f) {
-        const float4 tq = at32(reinterpret_cast<const float4*>(eb + f * g.nvox), vo / 4);
-        s2.x += tq.x * tq.x; s2.y += tq.y * tq.y; s2.z += tq.z * tq.z; s2.w += tq.w * tq.w;
-        if (f == p) ep = tq;
-      }
-      float xi[4] = {0.f, 0.f, 0.f, 0.f};
-      if (mk.noise != 0.f) normal4(mk.seed, (unsigned)((x * N * N + vo) >> 2), (uint32_t)p, step, xi);
-      const float2 ra = z[2 * q], rbv = z[2 * q + 1];
-      const float e4[4] = {ep.x, ep.y, ep.z, ep.w}, s4[4] = {s2.x, s2.y, s2.z, s2.w}, r4[4] = {ra.x, ra.y, rbv.x, rbv.y};
-      float o[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float et = e4[u];
-        o[u] = mk.A * et - mk.B * et * et + mk.C * et * s4[u] + mk.two_wel * et * r4[u] + mk.noise * xi[u];
-      }
-      z[2 * q] = make_float2(o[0], o[1]);
-      z[2 * q + 1] = make_float2(o[2], o[3]);
-    }
+  typename SL::Pre pre;
+  {
+    int x, j, p;
+    item(blockIdx.x, x, j, p);
+    s.fetch(kbuf + p * g.kfs, g, (unsigned)(x * N + j * Y), pre);
   }
-  wave_lds_sync();
-  s.r2c();
-  __syncthreads();
-  s.template ypass<true>();
-  __syncthreads();
-  s.store(field, g, rb);
+  s.tables(twN, twM, rtn);
+  for (int it = blockIdx.x; it < NI; it += gridDim.x) {
+    int x, j, p;
+    item(it, x, j, p);
+    const unsigned rb = (unsigned)(x * N + j * Y);
+    float2* field = kbuf + p * g.kfs;
+    s.put(pre);
+    __syncthreads();
+    if (it + (int)gridDim.x < NI) {
+      int x2, j2, p2;
+      item(it + gridDim.x, x2, j2, p2);
+      s.fetch(kbuf + p2 * g.kfs, g, (unsigned)(x2 * N + j2 * Y), pre);
+    }
+    s.template ypass<false>();
+    __syncthreads();
+    s.c2r();
+    const float* eb = eta + (size_t)x * N * N;  // the x-plane's voxels
+    for (int i = 0; i < RW; ++i) {
+      const int t = s.w * RW + i;
+      float2* z = s.rows() + i * PP;
+      for (int q = s.lane; q < Q; q += 64) {
+        const unsigned vo = (unsigned)((kSplitP * t + j) * N + 4 * q);  // voxel offset inside the x-plane
+        float4 ep = make_float4(0.f, 0.f, 0.f, 0.f), s2 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int f = 0; f < VT; ++f) {
+          const float4 tq = at32(reinterpret_cast<const float4*>(eb + f * g.nvox), vo / 4);
+          s2.x += tq.x * tq.x; s2.y += tq.y * tq.y; s2.z += tq.z * tq.z; s2.w += tq.w * tq.w;
+          if (f == p) ep = tq;
+        }
+        float xi[4] = {0.f, 0.f, 0.f, 0.f};
+        if (mk.noise != 0.f) normal4(mk.seed, (unsigned)((x * N * N + vo) >> 2), (uint32_t)p, step, xi);
+        const float2 ra = z[2 * q], rbv = z[2 * q + 1];
+        const float e4[4] = {ep.x, ep.y, ep.z, ep.w}, s4[4] = {s2.x, s2.y, s2.z, s2.w}, r4[4] = {ra.x, ra.y, rbv.x, rbv.y};
+        float o[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float et = e4[u];
+          o[u] = mk.A * et - mk.B * et * et + mk.C * et * s4[u] + mk.two_wel * et * r4[u] + mk.noise * xi[u];
+        }
+        z[2 * q] = make_float2(o[0], o[1]);
+        z[2 * q + 1] = make_float2(o[2], o[3]);
+      }
+    }
+    wave_lds_sync();
+    s.r2c();
+    __syncthreads();
+    s.template ypass<true>();
+    __syncthreads();
+    s.store(field, g, rb);
+    __syncthreads();  // the next put overwrites what store reads
+  }
 }
 
 // End of a step, per (sub-plane, field): A-form eta_hat -> y inverse -> C2R -> eta written to the
 // real field and squared -> R2C -> y forward -> E-form S_hat of the next step. adv (may be null):
-// the step counter the launch advances once. Grid: (n * P, F).
+// the step counter the launch advances once. Resident work groups as k_sub_mid; item
+// f * n * P + (sub-plane) is field f of that sub-plane.
 template <int N>
 __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_sq(float2* __restrict__ kbuf, float* __restrict__ out, SGeo g,
                                                  const float2* __restrict__ twN, const float2* __restrict__ twM,
-                                                 const float2* __restrict__ rtn, uint32_t* adv) {
-  if (adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *adv += 1u;
+                                                 const float2* __restrict__ rtn, uint32_t* adv, int nitems) {
+  if (adv && blockIdx.x == 0 && threadIdx.x == 0) *adv += 1u;
   using SL = SubLds<N>;
-  constexpr int Q = SL::Q, PP = SL::PP, Y = SL::Y, RW = SL::RW;
+  constexpr int Q = SL::Q, PP = SL::PP, Y = SL::Y, RW = SL::RW, NSP = N * kSplitP;
   extern __shared__ float2 smem[];
   const SL s(smem);
-  const int x = blockIdx.x / kSplitP, j = blockIdx.x % kSplitP, f = blockIdx.y;
-  const unsigned rb = (unsigned)(x * N + j * Y);
-  float2* field = kbuf + f * g.kfs;
-  s.load(field, g, rb);
-  s.tables(twN, twM, rtn);
-  __syncthreads();
-  s.template ypass<false>();
-  __syncthreads();
-  s.c2r();
-  float* dst = out + f * g.nvox + (size_t)x * N * N;
-  for (int i = 0; i < RW; ++i) {
-    const int t = s.w * RW + i;
-    float2* z = s.rows() + i * PP;
-    for (int q = s.lane; q < Q; q += 64) {
-      const float2 a = z[2 * q], b = z[2 * q + 1];
-      at32(reinterpret_cast<float4*>(dst), (unsigned)((kSplitP * t + j) * N + 4 * q) / 4) = make_float4(a.x, a.y, b.x, b.y);
-      // each lane squares the very elements it has just read
-      z[2 * q] = make_float2(a.x * a.x, a.y * a.y);
-      z[2 * q + 1] = make_float2(b.x * b.x, b.y * b.y);
-    }
+  typename SL::Pre pre;
+  {
+    const int it = blockIdx.x, sp = it % NSP;
+    s.fetch(kbuf + (it / NSP) * g.kfs, g, (unsigned)((sp / kSplitP) * N + (sp % kSplitP) * Y), pre);
   }
-  wave_lds_sync();
-  s.r2c();
-  __syncthreads();
-  s.template ypass<true>();
-  __syncthreads();
-  s.store(field, g, rb);
+  s.tables(twN, twM, rtn);
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int f = it / NSP, sp = it % NSP, x = sp / kSplitP, j = sp % kSplitP;
+    const unsigned rb = (unsigned)(x * N + j * Y);
+    float2* field = kbuf + f * g.kfs;
+    s.put(pre);
+    __syncthreads();
+    const int nx = it + (int)gridDim.x;
+    if (nx < nitems) {
+      const int sp2 = nx % NSP;
+      s.fetch(kbuf + (nx / NSP) * g.kfs, g, (unsigned)((sp2 / kSplitP) * N + (sp2 % kSplitP) * Y), pre);
+    }
+    s.template ypass<false>();
+    __syncthreads();
+    s.c2r();
+    float* dst = out + f * g.nvox + (size_t)x * N * N;
+    for (int i = 0; i < RW; ++i) {
+      const int t = s.w * RW + i;
+      float2* z = s.rows() + i * PP;
+      for (int q = s.lane; q < Q; q += 64) {
+        const float2 a = z[2 * q], b = z[2 * q + 1];
+        at32(reinterpret_cast<float4*>(dst), (unsigned)((kSplitP * t + j) * N + 4 * q) / 4) = make_float4(a.x, a.y, b.x, b.y);
+        // each lane squares the very elements it has just read
+        z[2 * q] = make_float2(a.x * a.x, a.y * a.y);
+        z[2 * q + 1] = make_float2(b.x * b.x, b.y * b.y);
+      }
+    }
+    wave_lds_sync();
+    s.r2c();
+    __syncthreads();
+    s.template ypass<true>();
+    __syncthreads();
+    s.store(field, g, rb);
+    __syncthreads();  // the next put overwrites what store reads
+  }
 }
 
 // Real fields `in` (OP 1: squared) -> R2C along z -> y forward -> E-form spectra in kbuf, per

@@ -77,10 +77,10 @@ __device__ __forceinline__ TileIO tio(float2* field, const STile& s) {
 }
 
 // ---------------------------------------------------------------- y pass (C2C along y, in place)
-template <int N, bool FWD>
+template <int N, bool FWD, bool BT = false>
 __global__ void __launch_bounds__(xy_threads<N>()) k_ypass(float2* __restrict__ kbuf, Geo g, int F,
                                                const float2* __restrict__ twN, const MemberPtrs* __restrict__ mt) {
-  if (mt) kbuf = mt[blockIdx.z].kbuf;  // batched ensemble launch
+  if (BT && mt) kbuf = mt[blockIdx.z].kbuf;  // batched ensemble launch
   extern __shared__ float2 smem[];
   float2* tw = smem;
   float2* tile = smem + N;
@@ -120,12 +120,12 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xpass(float2* __restrict__ 
 
 // ---------------------------------------------------------------- martensite contraction (V <= 4)
 // S_hat -> x FFT (to registers) -> R_hat_p = sum_q B_pq S_hat_q / n^3 -> inverse x FFT.
-template <int N, int V>
+template <int N, int V, bool BT = false>
 __global__ void __launch_bounds__(xy_threads<N>()) __attribute__((amdgpu_waves_per_eu(xc_wpe<N, V>())))
 k_xcontract_r(float2* __restrict__ kbuf, const float* __restrict__ Bt, Geo g,
                                                      MartK mk, const float2* __restrict__ twN,
                                                      const MemberPtrs* __restrict__ mt) {
-  if (mt) {  // batched ensemble launch
+  if (BT && mt) {  // batched ensemble launch
     kbuf = mt[blockIdx.z].kbuf;
     Bt = mt[blockIdx.z].Bt;
     mk = mt[blockIdx.z].mk;
@@ -243,11 +243,11 @@ __global__ void __launch_bounds__(xy_threads<N>()) k_xcontract(float2* __restric
 // Martensite: F_hat -> x FFT -> eta_hat = (eta_hat - dt L F_hat)/(1 + dt L kappa k^2) -> inverse x FFT,
 // for the G fields f0 .. f0+G-1.
 // MODE 0: step update; MODE 1: initialise the state from the forward transform of eta.
-template <int N, int G, int MODE>
+template <int N, int G, int MODE, bool BT = false>
 __global__ void __launch_bounds__(xy_threads<N>()) k_xupdate_mart(float2* __restrict__ kbuf, float2* __restrict__ state, Geo g,
                                                       MartK mk, const float2* __restrict__ twN, int f0,
                                                       const MemberPtrs* __restrict__ mt) {
-  if (mt) {  // batched ensemble launch
+  if (BT && mt) {  // batched ensemble launch
     kbuf = mt[blockIdx.z].kbuf;
     state = mt[blockIdx.z].state;
     mk = mt[blockIdx.z].mk;

@@ -361,12 +361,12 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zinv(const float2* __restrict
 // are taken of the FP32 values written to the real field, so S_f is bit-identical to a separate
 // forward pass over that field.
 // adv (may be null): step counter advanced once by the launch.
-template <int M>
+template <int M, bool BT = false>
 __global__ void __launch_bounds__(zthreads<M>()) k_zinv_sq(float2* __restrict__ kbuf, float* __restrict__ out, Geo g,
                                                           const float2* __restrict__ twM,
                                                           const float2* __restrict__ rtn, uint32_t* adv,
                                                           const MemberPtrs* __restrict__ mt) {
-  if (mt) {  // batched ensemble launch
+  if (BT && mt) {  // batched ensemble launch
     kbuf = mt[blockIdx.z].kbuf;
     out = mt[blockIdx.z].fld;
     adv = mt[blockIdx.z].dstep;
@@ -415,11 +415,11 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zinv_sq(float2* __restrict__ 
 // formed on those registers and the first forward pass starts from them. VT > 0: the variant count
 // is a compile-time constant and every eta_q the thread needs is prefetched into registers;
 // VT = 0: any variant count, eta read when needed.
-template <int M, int VT>
+template <int M, int VT, bool BT = false>
 __global__ void __launch_bounds__(zmid_threads<M>()) __attribute__((amdgpu_waves_per_eu(zmid_wpe<M, VT>())))
 k_zmid_mart(float2* __restrict__ kbuf, const float* __restrict__ eta, Geo g, MartK mk, const float2* __restrict__ twM,
             const float2* __restrict__ rtn, const MemberPtrs* __restrict__ mt) {
-  if (mt) {  // batched ensemble launch
+  if (BT && mt) {  // batched ensemble launch
     kbuf = mt[blockIdx.z].kbuf;
     eta = mt[blockIdx.z].fld;
     mk = mt[blockIdx.z].mk;
