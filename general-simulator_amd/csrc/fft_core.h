@@ -444,8 +444,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 // the butterflies l, l + 64, ... (at most K), numbered line-fastest (consecutive lanes on
 // consecutive lines: lines that lie side by side) or, JF, butterfly-fastest (consecutive lanes
 // along one line: lines whose elements lie side by side). No work-group barrier.
-template <typename T2, int N, int R, bool FWD, int Ns, int K, bool JF>
+template <typename T2, int N, int R, bool FWD, int Ns, int K, bool JF, bool SK>
 __device__ __forceinline__ void fft_wpass(T2* lds, const Lines& L, const T2* tw, int lane) {
+  // SK: element p of a line sits at p + p / 16, and twiddle t at t + t / 16 (a pass that writes
+  // or reads with a stride then spreads 16 lanes over 16 bank pairs)
+  auto at = [&](int p) -> int { return (SK ? p + (p >> 4) : p) * L.ps; };
   constexpr int NB = N / R;  // butterflies per line
   const int nb = L.nl * NB;
   T2 v[K][R];
@@ -459,7 +462,7 @@ __device__ __forceinline__ void fft_wpass(T2* lds, const Lines& L, const T2* tw,
       j[q] = JF ? b % NB : b / L.nl;
       base[q] = L.base(JF ? b / NB : b % L.nl);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = lds[base[q] + (j[q] + r * (N / R)) * L.ps];
+      for (int r = 0; r < R; ++r) v[q][r] = lds[base[q] + at(j[q] + r * (N / R))];
     }
   }
   wave_lds_sync();
@@ -471,7 +474,7 @@ __device__ __forceinline__ void fft_wpass(T2* lds, const Lines& L, const T2* tw,
       const int st = k * (N / (Ns * R));
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        T2 w = tw[r * st];
+        T2 w = tw[SK ? r * st + ((r * st) >> 4) : r * st];  // SK: the twiddle table is skewed too
         if (!FWD) w.y = -w.y;
         v[q][r] = cmul(v[q][r], w);
       }
@@ -479,25 +482,25 @@ __device__ __forceinline__ void fft_wpass(T2* lds, const Lines& L, const T2* tw,
     dftR<R, FWD>(v[q]);
     const int d = (j[q] / Ns) * Ns * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) lds[base[q] + (d + r * Ns) * L.ps] = v[q][r];
+    for (int r = 0; r < R; ++r) lds[base[q] + at(d + r * Ns)] = v[q][r];
   }
   wave_lds_sync();
 }
-template <typename T2, int N, bool FWD, int Ns, int NL, bool JF, int R0, int... Rs>
+template <typename T2, int N, bool FWD, int Ns, int NL, bool JF, bool SK, int R0, int... Rs>
 __device__ __forceinline__ void fft_wchain(T2* lds, const Lines& L, const T2* tw, int lane) {
-  fft_wpass<T2, N, R0, FWD, Ns, (NL * (N / R0) + 63) / 64, JF>(lds, L, tw, lane);
-  if constexpr (sizeof...(Rs) > 0) fft_wchain<T2, N, FWD, Ns * R0, NL, JF, Rs...>(lds, L, tw, lane);
+  fft_wpass<T2, N, R0, FWD, Ns, (NL * (N / R0) + 63) / 64, JF, SK>(lds, L, tw, lane);
+  if constexpr (sizeof...(Rs) > 0) fft_wchain<T2, N, FWD, Ns * R0, NL, JF, SK, Rs...>(lds, L, tw, lane);
 }
-template <typename T2, int N, bool FWD, int NL, bool JF, int... Rs>
+template <typename T2, int N, bool FWD, int NL, bool JF, bool SK, int... Rs>
 __device__ __forceinline__ void fft_wlines_(T2* lds, const Lines& L, const T2* tw, int lane, Radices<Rs...>) {
-  fft_wchain<T2, N, FWD, 1, NL, JF, Rs...>(lds, L, tw, lane);
+  fft_wchain<T2, N, FWD, 1, NL, JF, SK, Rs...>(lds, L, tw, lane);
 }
 // All lines of L (at most NL of them, touched by no other wave) with an explicit plan (forward in
-// order, inverse reversed), by the calling wave; lane = its lane id; JF as for fft_wpass.
-template <typename T2, int N, bool FWD, class Plan, int NL, bool JF>
+// order, inverse reversed), by the calling wave; lane = its lane id; JF and SK as for fft_wpass.
+template <typename T2, int N, bool FWD, class Plan, int NL, bool JF, bool SK = false>
 __device__ __forceinline__ void fft_wlines(T2* lds, const Lines& L, const T2* tw, int lane) {
-  if constexpr (FWD) fft_wlines_<T2, N, true, NL, JF>(lds, L, tw, lane, Plan{});
-  else fft_wlines_<T2, N, false, NL, JF>(lds, L, tw, lane, typename RevImpl<Plan, Radices<>>::type{});
+  if constexpr (FWD) fft_wlines_<T2, N, true, NL, JF, SK>(lds, L, tw, lane, Plan{});
+  else fft_wlines_<T2, N, false, NL, JF, SK>(lds, L, tw, lane, typename RevImpl<Plan, Radices<>>::type{});
 }
 
 // All lines of L (at most NL of them) with an explicit plan (forward in order, inverse reversed)

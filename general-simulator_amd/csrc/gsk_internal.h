@@ -219,13 +219,12 @@ inline bool plane_on(const gsk_sim* s) {
          s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL;
 }
 
-// Split-y schedule (kernels_split.h): the spectral martensite step at n = 256 with 1..4 variants,
-// when GSK_SPLIT=1 is set in the environment (otherwise the chain of y and z kernels runs).
+// Split-y schedule (kernels_split.h): the spectral martensite step at n = 256 with 1..4 variants.
+// GSK_SPLIT=0 in the environment when a handle is created keeps that handle on the chain of y and
+// z kernels.
 inline bool split_wanted(const gsk_sim* s) {
-  static const bool on = [] {
-    const char* e = std::getenv("GSK_SPLIT");
-    return e && e[0] == '1';
-  }();
+  const char* e = std::getenv("GSK_SPLIT");
+  const bool on = !(e && e[0] == '0');
   return on && s->n == 256 && s->F >= 1 && s->F <= 4 && s->cfg.path == GSK_PATH_FUSED32 &&
          s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL;
 }

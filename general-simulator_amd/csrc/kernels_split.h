@@ -47,6 +47,11 @@ namespace gsk {
 #define GSK_SUB_T 512
 #endif
 
+// Row pitch of the LDS sub-plane in complex values (0: the skewed row length plus one).
+#ifndef GSK_SUB_PP
+#define GSK_SUB_PP 0
+#endif
+
 constexpr int kSplitP = 4;  // sub-planes per x-plane
 constexpr int kSubT = GSK_SUB_T;
 constexpr int kSplitWc = GSK_SPLIT_WC;
@@ -355,27 +360,34 @@ __global__ void r_state_grad_split(const float2* __restrict__ state, SGeo g, int
 
 // ---------------------------------------------------------------- sub-plane kernels
 // LDS of a sub-plane work group: the m-point (y) and M-point (z) twiddles, the split factors and
-// the sub-plane: m rows of n/2 + 1 values at pitch n/2 + 8 (four consecutive rows then cover the
-// 64 LDS banks once in the radix-16 row pass).
+// the sub-plane: m rows of n/2 + 1 values. Value p of a row sits at sk(p) = p + p / 16 (pitch
+// sk(n/2) + 1): the first pass of a row transform writes with stride R (8 or 16), which without
+// the skew puts 16 lanes on two of the 16 LDS bank pairs; contiguous runs inside a 16-block keep
+// one skew and stay conflict-free.
 //
 // Each wave owns CW columns and RW rows of the sub-plane. A y transform (along a column) or a row
 // operation (C2R / R2C split, z transform, pointwise work) only touches the calling wave's
 // columns or rows, so it needs no work-group barrier: a step has four (after the load, after y
 // inverse, after the row phase, after y forward) instead of two per LDS pass.
 template <int N> struct SubLds {
-  static constexpr int M = N / 2, Y = N / kSplitP, PP = M + 8, Q = M / 2, T = kSubT;
+  __host__ __device__ static constexpr int sk(int p) { return p + (p >> 4); }
+  static constexpr int M = N / 2, Y = N / kSplitP, PP = GSK_SUB_PP > 0 ? GSK_SUB_PP : sk(N / 2) + 1, Q = M / 2,
+                       T = kSubT;
+  static_assert(PP > sk(N / 2), "pitch holds a skewed row");
   static constexpr int NW = T / 64, RW = Y / NW, CW = M / NW;
   static_assert(Y % NW == 0 && M % NW == 0, "rows and columns in equal shares per wave");
+  static_assert(CW <= 16 && 16 % CW == 0, "a wave's columns inside one 16-block (one skew)");
   float2 *twy, *twz, *rt, *pl;
   int w, lane;
+  static constexpr int TZ = sk(M - 1) + 1;  // the skewed z twiddle table
   __device__ SubLds(float2* smem)
-      : twy(smem), twz(smem + Y), rt(smem + Y + M), pl(smem + Y + 2 * M), w((int)threadIdx.x / 64),
+      : twy(smem), twz(smem + Y), rt(smem + Y + TZ), pl(smem + Y + TZ + M), w((int)threadIdx.x / 64),
         lane((int)threadIdx.x % 64) {}
   __device__ void tables(const float2* __restrict__ twN, const float2* __restrict__ twM,
                          const float2* __restrict__ rtn) const {
     for (int i = threadIdx.x; i < Y; i += T) twy[i] = twN[i * kSplitP];  // exp(-2 pi i k / m)
     for (int i = threadIdx.x; i < M; i += T) {
-      twz[i] = twM[i];
+      twz[sk(i)] = twM[i];
       rt[i] = rtn[i];
     }
   }
@@ -396,20 +408,20 @@ template <int N> struct SubLds {
 #pragma unroll
     for (int u = 0; u < LE; ++u) {
       const int e = threadIdx.x + u * T;
-      pl[(e / M) * PP + e % M] = p.v[u];
+      pl[(e / M) * PP + sk(e % M)] = p.v[u];
     }
-    if ((int)threadIdx.x < Y) pl[threadIdx.x * PP + M] = p.nq;
+    if ((int)threadIdx.x < Y) pl[threadIdx.x * PP + sk(M)] = p.nq;
   }
   __device__ void store(float2* __restrict__ field, const SGeo& g, unsigned rb) const {
-    for (int e = threadIdx.x; e < Y * M; e += T) at32(field, rb * M + (unsigned)e) = pl[(e / M) * PP + e % M];
-    for (int y = threadIdx.x; y < Y; y += T) at32(field, (unsigned)g.nq + rb + y) = pl[y * PP + M];
+    for (int e = threadIdx.x; e < Y * M; e += T) at32(field, rb * M + (unsigned)e) = pl[(e / M) * PP + sk(e % M)];
+    for (int y = threadIdx.x; y < Y; y += T) at32(field, (unsigned)g.nq + rb + y) = pl[y * PP + sk(M)];
   }
   // m-point transforms of the wave's CW main columns (wave 0: also of the Nyquist column)
   template <bool FWD> __device__ void ypass() const {
     using Plan = typename PlanOf<Y>::type;
     const int ln = opaque(lane);
-    fft_wlines<float2, Y, FWD, Plan, CW, false>(pl + w * CW, Lines{CW, CW, 1, 0, PP}, twy, ln);
-    if (w == 0) fft_wlines<float2, Y, FWD, Plan, 1, false>(pl + M, Lines{1, 1, 0, 0, PP}, twy, ln);
+    fft_wlines<float2, Y, FWD, Plan, CW, false>(pl + sk(w * CW), Lines{CW, CW, 1, 0, PP}, twy, ln);
+    if (w == 0) fft_wlines<float2, Y, FWD, Plan, 1, false>(pl + sk(M), Lines{1, 1, 0, 0, PP}, twy, ln);
   }
   // the wave's first row
   __device__ float2* rows() const { return pl + w * RW * PP; }
@@ -421,40 +433,40 @@ template <int N> struct SubLds {
       float2* z = R0 + i * PP;
       for (int k = ln; k < Q; k += 64) {
         if (k == 0) {
-          const float2 a = z[0], b = z[M], c = z[Q];
+          const float2 a = z[0], b = z[sk(M)], c = z[sk(Q)];
           float2 zh, unused;
           c2r_pair(c, c, rt[Q], zh, unused);
           z[0] = make_float2(a.x + b.x, a.x - b.x);
-          z[Q] = zh;
+          z[sk(Q)] = zh;
         } else {
           float2 zk, zm;
-          c2r_pair(z[k], z[M - k], rt[k], zk, zm);
-          z[k] = zk;
-          z[M - k] = zm;
+          c2r_pair(z[sk(k)], z[sk(M - k)], rt[k], zk, zm);
+          z[sk(k)] = zk;
+          z[sk(M - k)] = zm;
         }
       }
     }
     wave_lds_sync();
-    fft_wlines<float2, M, false, typename PlanOf<M>::type, RW, true>(R0, Lines{RW, RW, PP, 0, 1}, twz, ln);
+    fft_wlines<float2, M, false, typename PlanOf<M>::type, RW, true, true>(R0, Lines{RW, RW, PP, 0, 1}, twz, ln);
   }
   // M-point forward along the wave's rows, then the split back to n/2 + 1 values per row
   __device__ void r2c() const {
     float2* R0 = rows();
     const int ln = opaque(lane);
-    fft_wlines<float2, M, true, typename PlanOf<M>::type, RW, true>(R0, Lines{RW, RW, PP, 0, 1}, twz, ln);
+    fft_wlines<float2, M, true, typename PlanOf<M>::type, RW, true, true>(R0, Lines{RW, RW, PP, 0, 1}, twz, ln);
     for (int i = 0; i < RW; ++i) {
       float2* z = R0 + i * PP;
       for (int k = ln; k < Q; k += 64) {
         if (k == 0) {
-          const float2 z0 = z[0], zh = z[Q];
+          const float2 z0 = z[0], zh = z[sk(Q)];
           z[0] = make_float2(z0.x + z0.y, 0.f);
-          z[M] = make_float2(z0.x - z0.y, 0.f);
-          z[Q] = make_float2(zh.x, -zh.y);
+          z[sk(M)] = make_float2(z0.x - z0.y, 0.f);
+          z[sk(Q)] = make_float2(zh.x, -zh.y);
         } else {
           float2 xk, xm;
-          r2c_pair(z[k], z[M - k], rt[k], xk, xm);
-          z[k] = xk;
-          z[M - k] = xm;
+          r2c_pair(z[sk(k)], z[sk(M - k)], rt[k], xk, xm);
+          z[sk(k)] = xk;
+          z[sk(M - k)] = xm;
         }
       }
     }
@@ -463,7 +475,7 @@ template <int N> struct SubLds {
 
 template <int N> constexpr size_t sub_lds() {
   using S = SubLds<N>;
-  return (size_t)(S::Y + 2 * S::M + S::Y * S::PP) * sizeof(float2);
+  return (size_t)(S::Y + S::TZ + S::M + S::Y * S::PP) * sizeof(float2);
 }
 
 // Middle of a step, per (sub-plane, variant p): A-form R_hat_p -> y inverse -> C2R -> G on the
@@ -526,7 +538,7 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_mid(float2* __restri
         }
         float xi[4] = {0.f, 0.f, 0.f, 0.f};
         if (mk.noise != 0.f) normal4(mk.seed, (unsigned)((x * N * N + vo) >> 2), (uint32_t)p, step, xi);
-        const float2 ra = z[2 * q], rbv = z[2 * q + 1];
+        const float2 ra = z[SL::sk(2 * q)], rbv = z[SL::sk(2 * q + 1)];
         const float e4[4] = {ep.x, ep.y, ep.z, ep.w}, s4[4] = {s2.x, s2.y, s2.z, s2.w}, r4[4] = {ra.x, ra.y, rbv.x, rbv.y};
         float o[4];
 #pragma unroll
@@ -534,8 +546,8 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_mid(float2* __restri
           const float et = e4[u];
           o[u] = mk.A * et - mk.B * et * et + mk.C * et * s4[u] + mk.two_wel * et * r4[u] + mk.noise * xi[u];
         }
-        z[2 * q] = make_float2(o[0], o[1]);
-        z[2 * q + 1] = make_float2(o[2], o[3]);
+        z[SL::sk(2 * q)] = make_float2(o[0], o[1]);
+        z[SL::sk(2 * q + 1)] = make_float2(o[2], o[3]);
       }
     }
     wave_lds_sync();
@@ -586,11 +598,11 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_sq(float2* __restric
       const int t = s.w * RW + i;
       float2* z = s.rows() + i * PP;
       for (int q = s.lane; q < Q; q += 64) {
-        const float2 a = z[2 * q], b = z[2 * q + 1];
+        const float2 a = z[SL::sk(2 * q)], b = z[SL::sk(2 * q + 1)];
         at32(reinterpret_cast<float4*>(dst), (unsigned)((kSplitP * t + j) * N + 4 * q) / 4) = make_float4(a.x, a.y, b.x, b.y);
         // each lane squares the very elements it has just read
-        z[2 * q] = make_float2(a.x * a.x, a.y * a.y);
-        z[2 * q + 1] = make_float2(b.x * b.x, b.y * b.y);
+        z[SL::sk(2 * q)] = make_float2(a.x * a.x, a.y * a.y);
+        z[SL::sk(2 * q + 1)] = make_float2(b.x * b.x, b.y * b.y);
       }
     }
     wave_lds_sync();
@@ -624,11 +636,11 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_fwd(const float* __r
     for (int q = s.lane; q < Q; q += 64) {
       const float4 v = at32(reinterpret_cast<const float4*>(src), (unsigned)((kSplitP * t + j) * N + 4 * q) / 4);
       if (OP == 1) {
-        z[2 * q] = make_float2(v.x * v.x, v.y * v.y);
-        z[2 * q + 1] = make_float2(v.z * v.z, v.w * v.w);
+        z[SL::sk(2 * q)] = make_float2(v.x * v.x, v.y * v.y);
+        z[SL::sk(2 * q + 1)] = make_float2(v.z * v.z, v.w * v.w);
       } else {
-        z[2 * q] = make_float2(v.x, v.y);
-        z[2 * q + 1] = make_float2(v.z, v.w);
+        z[SL::sk(2 * q)] = make_float2(v.x, v.y);
+        z[SL::sk(2 * q + 1)] = make_float2(v.z, v.w);
       }
     }
   }

@@ -234,6 +234,31 @@ def test_mart_translation_equivariance_at_256(gpu):
     assert np.abs(np.roll(a, sh, axis=(1, 2, 3)) - b).max() < 1e-5 * np.abs(a).max()
 
 
+def test_split_schedule_matches_chain_at_256(gpu, monkeypatch):
+    """At 256^3 the fused martensite step runs the split-y schedule (four launches; sub-plane
+    kernels keep 64 y-rows in LDS through y, z and y) unless GSK_SPLIT=0 is set when the handle
+    is created, which keeps the chain of y and z kernels (eight launches). The two FP32
+    schedules order their arithmetic differently but follow one trajectory: 20 steps of strong
+    dynamics with noise agree to FP32 rounding, and so do the energies."""
+    say("split-y schedule vs y/z chain n=256")
+    kw = dict(A=-0.5, w_el=20.0, noise=0.002, seed=9)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("GSK_SPLIT", mode)
+        s = mart_sim(256, 3, **kw)
+        s.init_uniform("eta", -0.05, 0.05, 6)
+        s.step(20)
+        out[mode] = (s.get_field("eta", np.float64), s.energy())
+        names, _ = s.kernel_times(1)  # which schedule ran
+        s.close()
+        assert ("sub_mid" in names) == (mode == "1"), names
+    (a, Ea), (b, Eb) = out["1"], out["0"]
+    assert np.isfinite(a).all()
+    assert float(np.abs(a - b).max()) < 2e-6 * float(np.abs(b).max())
+    assert rel_l2(a, b) < 1e-6
+    np.testing.assert_allclose(Ea, Eb, rtol=1e-5, atol=1e-9 * abs(Eb[3]))
+
+
 # ------------------------------------------------------------------ precipitate
 def test_mart_fused_vs_plain_at_512(gpu):
     """512^3 with the 512^3 configuration's noise (0.002), strong dynamics: the FP32 fused step
