@@ -118,43 +118,54 @@ __device__ __forceinline__ XSTile xs_tile(int t, const SGeo& g) {
 // The P = 4 point stage across the lanes c, c ^ Wc, c ^ 2 Wc, c ^ 3 Wc (j = lane / Wc % 4).
 // Forward: lane j holds w_n^(-k1 j) E_j on entry and X[k1 + m * bitrev(j)] on exit.
 // Inverse: lane j holds X[k1 + m * bitrev(j)] on entry and sum_k2 X[k1 + m k2] w_4^(k2 j) on exit.
-// Butterfly of the lane pairs (l, l ^ MASK) of a wave: lower lane lo + hi, upper lane lo - hi.
-// The partner value comes over the VALU: DPP row rotation by 8 (MASK 8, = lane ^ 8 inside a row
-// of 16), gfx950's v_permlane16_swap / v_permlane32_swap (MASK 16 / 32; swapping a register with
-// itself leaves the lower row's value in the first result and the upper row's in the second, in
-// both lanes of a pair). Other masks: ds_bpermute through the LDS crossbar.
-template <int MASK> __device__ __forceinline__ float lane_bfly(float v, bool up) {
-  if constexpr (MASK == 16) {
+// Butterfly of the lane pairs (l, l ^ MASK) of a wave: lower lane lo + hi, upper lane lo - hi,
+// as fma(sg, hi, lo) with sg = +1 on the lower and -1 on the upper lane (the product is exact, so
+// this rounds as the sum or difference does). The partner value comes over the VALU: DPP row
+// rotation by 8 (MASK 8, = lane ^ 8 inside a row of 16), gfx950's v_permlane16_swap /
+// v_permlane32_swap (MASK 16 / 32; swapping a register with itself leaves the lower row's value
+// in the first result and the upper row's in the second, in both lanes of a pair). Other masks:
+// ds_bpermute through the LDS crossbar.
+template <int MASK> __device__ __forceinline__ float lane_bfly(float v, float sg) {
+  if constexpr (MASK == 16 || MASK == 32) {
     const unsigned u = __float_as_uint(v);
-    const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
-    const float lo = __uint_as_float(r[0]), hi = __uint_as_float(r[1]);
-    return up ? lo - hi : lo + hi;
-  } else if constexpr (MASK == 32) {
-    const unsigned u = __float_as_uint(v);
-    const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
-    const float lo = __uint_as_float(r[0]), hi = __uint_as_float(r[1]);
-    return up ? lo - hi : lo + hi;
-  } else if constexpr (MASK == 8) {
-    const float p = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128 /* row_ror:8 */, 0xF, 0xF, false));
-    return up ? p - v : v + p;
+    unsigned lo, hi;
+    if constexpr (MASK == 16) {
+      const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+      lo = r[0];
+      hi = r[1];
+    } else {
+      const auto r = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+      lo = r[0];
+      hi = r[1];
+    }
+    return __builtin_fmaf(sg, __uint_as_float(hi), __uint_as_float(lo));
   } else {
-    const float p = __shfl_xor(v, MASK, 64);
-    return up ? p - v : v + p;
+    // partner p: lower lane v + p, upper lane p - v
+    const float p = MASK == 8
+                        ? __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x128 /* row_ror:8 */, 0xF, 0xF, true))
+                        : __shfl_xor(v, MASK, 64);
+    return __builtin_fmaf(sg, v, p);
   }
 }
-template <int MASK> __device__ __forceinline__ float2 lane_bfly2(float2 v, bool up) {
-  return make_float2(lane_bfly<MASK>(v.x, up), lane_bfly<MASK>(v.y, up));
+template <int MASK> __device__ __forceinline__ float2 lane_bfly2(float2 v, float sg) {
+  return make_float2(lane_bfly<MASK>(v.x, sg), lane_bfly<MASK>(v.y, sg));
 }
-// (every lane of the wave active; j = lane / Wc % 4, so j & 1 is lane & Wc and j & 2 lane & 2 Wc)
-template <bool FWD> __device__ __forceinline__ float2 lane_dft4(float2 v, int j) {
+// Per-lane constants of the stage (every lane of the wave active; j = lane / Wc % 4, so j & 1 is
+// lane & Wc and j & 2 lane & 2 Wc).
+struct LaneStage {
+  float s1, s2;  // butterfly signs of the Wc and 2 Wc steps
+  bool j3;       // the lane whose value turns by -+i between the steps
+  __device__ explicit LaneStage(int j) : s1((j & 1) ? -1.f : 1.f), s2((j & 2) ? -1.f : 1.f), j3(j == 3) {}
+};
+template <bool FWD> __device__ __forceinline__ float2 lane_dft4(float2 v, const LaneStage& ls) {
   if (FWD) {
-    v = lane_bfly2<2 * kSplitWc>(v, j & 2);
-    if (j == 3) v = rot<true>(v);
-    v = lane_bfly2<kSplitWc>(v, j & 1);
+    v = lane_bfly2<2 * kSplitWc>(v, ls.s2);
+    v = make_float2(ls.j3 ? v.y : v.x, ls.j3 ? -v.x : v.y);  // rot<true> on lane 3
+    v = lane_bfly2<kSplitWc>(v, ls.s1);
   } else {
-    v = lane_bfly2<kSplitWc>(v, j & 1);
-    if (j == 3) v = rot<false>(v);
-    v = lane_bfly2<2 * kSplitWc>(v, j & 2);
+    v = lane_bfly2<kSplitWc>(v, ls.s1);
+    v = make_float2(ls.j3 ? -v.y : v.x, ls.j3 ? v.x : v.y);  // rot<false> on lane 3
+    v = lane_bfly2<2 * kSplitWc>(v, ls.s2);
   }
   return v;
 }
@@ -200,6 +211,7 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xcontract_split(float2*
   const int t = xs_swizzle(blockIdx.x, g.ntile);
   const XSTile s = xs_tile(t, g);
   const int b = threadIdx.x, c = b % W, jl = c / kSplitWc;
+  const LaneStage lst(jl);
   load_tw<N>(tw, twN);
   __syncthreads();
   const Lines Ls{W, W, 1, 0, W};
@@ -214,7 +226,7 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xcontract_split(float2*
 #pragma unroll
   for (int f = 0; f < V; ++f)
 #pragma unroll
-    for (int r = 0; r < RL; ++r) S[f][r] = lane_dft4<true>(cmul(S[f][r], wt), jl);
+    for (int r = 0; r < RL; ++r) S[f][r] = lane_dft4<true>(cmul(S[f][r], wt), lst);
   // keep the B loads below the lane stage: hoisted above it they hold 16 * NPQ registers at once
   asm volatile("" ::: "memory");
   const float* Bb = Bt + (long)t * NPQ * T;
@@ -245,7 +257,7 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xcontract_split(float2*
 #pragma unroll
     for (int f = 0; f < V; ++f)
 #pragma unroll
-      for (int r = 0; r < RL; ++r) S[f][r] = cmul(lane_dft4<false>(S[f][r], jl), wi);
+      for (int r = 0; r < RL; ++r) S[f][r] = cmul(lane_dft4<false>(S[f][r], lst), wi);
 #pragma unroll
     for (int f = 0; f < V; ++f) {
       float2* fb = kbuf + f * g.kfs;
@@ -292,6 +304,7 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   const int t = xs_swizzle(blockIdx.x, g.ntile), f = blockIdx.y, F = gridDim.y;
   const XSTile s = xs_tile(t, g);
   const int b = threadIdx.x, c = b % W, j = b / W, jl = c / kSplitWc;
+  const LaneStage lst(jl);
   float2* st = state + ((long)t * F + f) * T;
   float2 sv[RL];
   if (MODE == 0) {
@@ -306,7 +319,7 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   fft_g2r<float2, N>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, v);
   const float2 wt = tw[(s.k1c(c) * jl) % N];
 #pragma unroll
-  for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true>(cmul(v[r], wt), jl);
+  for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true>(cmul(v[r], wt), lst);
   if (MODE == 1) {
 #pragma unroll
     for (int r = 0; r < RL; ++r) at32(st, (unsigned)(opaque(b) + r * (N / RL) * W)) = v[r];
@@ -326,7 +339,7 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   }
   const float2 wi = make_float2(wt.x, -wt.y);
 #pragma unroll
-  for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false>(v[r], jl), wi);
+  for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false>(v[r], lst), wi);
   fft_r2g<float2, N>(tile, Ls, tw, v,
                      [&](int cc, int kx, float2 val) { at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = val; });
 }
