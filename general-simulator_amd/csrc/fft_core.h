@@ -358,6 +358,21 @@ __device__ __forceinline__ void fft_r2g(T2* lds, const Lines& L, const T2* tw, T
   gchain<T2, N, false, 2, 1, 1>(lds, L, tw, ld, st, vreg, InvPlan<N>{});
 }
 
+// The same pair with an explicit radix plan (forward in order, inverse reversed): vreg holds the
+// last radix's positions j + r*N/Rl; the work group provides nl*N/rmin threads (passes of
+// larger radix leave the surplus idle).
+template <typename T2, int N, class Plan, class LD>
+__device__ __forceinline__ void fft_g2r_p(T2* lds, const Lines& L, const T2* tw, LD ld, T2* vreg) {
+  auto st = [](int, int, T2) {};
+  gchain<T2, N, true, 1, 2, 1>(lds, L, tw, ld, st, vreg, Plan{});
+}
+template <typename T2, int N, class Plan, class ST>
+__device__ __forceinline__ void fft_r2g_p(T2* lds, const Lines& L, const T2* tw, T2* vreg, ST st) {
+  auto ld = [](int, int) { return T2{}; };
+  __syncthreads();
+  gchain<T2, N, false, 2, 1, 1>(lds, L, tw, ld, st, vreg, typename RevImpl<Plan, Radices<>>::type{});
+}
+
 // All-LDS transform with the pass stride Ns known at compile time (constant index math);
 // forward uses the plan in order, inverse the reversed plan.
 template <typename T2, int N, bool FWD, int Ns, int R0, int... Rs>

@@ -217,8 +217,8 @@ int gsk_build_btensor(gsk_sim* s, const double* C81, const double* eps, int neps
   for (int i = 0; i < neps * 9; ++i) s->eps[i] = eps[i];
   s->haveEps = true;
   if (s->cfg.path == GSK_PATH_FUSED32 && s->split) {
-    const long total = (long)s->sg.ntile * s->n * kSplitW;
-    hipLaunchKernelGGL(k_btensor_split<>, dim3(grid_1d(total)), dim3(256), 0, s->stream, s->Bt, s->sg, a);
+    const long total = (long)s->sg.ntile_c * s->n * split_w<kSplitWcC>();
+    hipLaunchKernelGGL(k_btensor_split<kSplitWcC>, dim3(grid_1d(total)), dim3(256), 0, s->stream, s->Bt, s->sg, a);
   } else if (s->cfg.path == GSK_PATH_FUSED32) {
     const long total = (long)(long)s->g.ntile * s->n * s->g.W;
     hipLaunchKernelGGL(k_btensor_tiles<>, dim3(grid_1d(total)), dim3(256), 0, s->stream, s->Bt, s->g, a);

@@ -228,7 +228,8 @@ struct Impl {
   }
 
   // ---- split-y schedule (n = 256): sub-plane launches have grid (n * P sub-planes, fields)
-  static constexpr size_t SPLIT_XLDS = (size_t)(N + N * kSplitW) * sizeof(float2);
+  static constexpr size_t SPLIT_XLDS = (size_t)(N + N * split_w<kSplitWc>()) * sizeof(float2);   // x update
+  static constexpr size_t SPLIT_CLDS = (size_t)(N + N * split_w<kSplitWcC>()) * sizeof(float2);  // contraction
   static int split_fwd(gsk_sim* s, bool square) {
     if constexpr (split_size<N>()) {
       const dim3 grid(N * kSplitP, s->F);
@@ -245,14 +246,14 @@ struct Impl {
   template <int MODE>
   static int split_xcontract(gsk_sim* s, const MartK& mk) {
     if constexpr (split_size<N>()) {
-      const int gx = s->sg.ntile;
+      const int gx = s->sg.ntile_c;
       if (MODE == 1 && gx > s->rblocks) return fail(GSK_ERR, "reduction scratch too small");
-      constexpr int T = split_xthreads<N>();
+      constexpr int T = split_cthreads<N>();
       switch (s->F) {
-        case 1: LAUNCH((k_xcontract_split<N, 1, MODE>), gx, T, SPLIT_XLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part); break;
-        case 2: LAUNCH((k_xcontract_split<N, 2, MODE>), gx, T, SPLIT_XLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part); break;
-        case 3: LAUNCH((k_xcontract_split<N, 3, MODE>), gx, T, SPLIT_XLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part); break;
-        default: LAUNCH((k_xcontract_split<N, 4, MODE>), gx, T, SPLIT_XLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part);
+        case 1: LAUNCH((k_xcontract_split<N, 1, MODE>), gx, T, SPLIT_CLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part); break;
+        case 2: LAUNCH((k_xcontract_split<N, 2, MODE>), gx, T, SPLIT_CLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part); break;
+        case 3: LAUNCH((k_xcontract_split<N, 3, MODE>), gx, T, SPLIT_CLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part); break;
+        default: LAUNCH((k_xcontract_split<N, 4, MODE>), gx, T, SPLIT_CLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part);
       }
       return GSK_OK;
     } else {
@@ -647,7 +648,7 @@ struct Impl {
       if (!s->sq_ready) CHK(square_fwd(s));
       MartK mk = mart_consts(s);
       CHK(split_xcontract<1>(s, mk));
-      LAUNCH(r_finish<>, 1, 1024, 0, s, s->part, s->sg.ntile, 1, s->rout);
+      LAUNCH(r_finish<>, 1, 1024, 0, s, s->part, s->sg.ntile_c, 1, s->rout);
       HIPCHK(hipMemcpyAsync(el, s->rout, sizeof(double), hipMemcpyDeviceToHost, s->stream));
       HIPCHK(hipStreamSynchronize(s->stream));
       return GSK_OK;

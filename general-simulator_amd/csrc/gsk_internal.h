@@ -468,9 +468,8 @@ inline void choose_geometry(gsk_sim* s, int Fz, int Fx_resident) {
   if (s->split) {
     SGeo& q = s->sg;
     q.n = n; q.M = M; q.m = n / kSplitP;
-    q.nnyq = q.m / kSplitWc;
-    q.nchm = M / kSplitWc;
-    q.ntile = q.nnyq + q.m * q.nchm;
+    q.ntile = split_ntile<kSplitWc>(q.m, M);
+    q.ntile_c = split_ntile<kSplitWcC>(q.m, M);
     q.kfs = g.kfs; q.nq = g.nq; q.nvox = g.nvox;
   }
 }

@@ -54,20 +54,28 @@ namespace gsk {
 
 constexpr int kSplitP = 4;  // sub-planes per x-plane
 constexpr int kSubT = GSK_SUB_T;
-constexpr int kSplitWc = GSK_SPLIT_WC;
-constexpr int kSplitW = kSplitP * kSplitWc;
-constexpr int kSplitG = 16 / kSplitWc;  // x tiles per 128-byte line
+constexpr int kSplitWc = GSK_SPLIT_WC;  // x update (and the k-space state's tile order)
+// Contraction's x transform and tile: GSK_XC_PLAN 0 (the edge plan 16 * 16 and kSplitWc columns
+// per sub-plane group: 16 values of each field per thread) or 1 (8 * 8 * 4 and 4 columns: 4
+// values per thread, 84 instead of 246 VGPRs at V = 3, but 232 instead of 203 us at 256^3).
+#ifndef GSK_XC_PLAN
+#define GSK_XC_PLAN 0
+#endif
+constexpr int kSplitWcC = GSK_XC_PLAN == 1 ? 4 : kSplitWc;
+template <int WC> constexpr int split_w() { return kSplitP * WC; }
 static_assert(kSplitWc == 4 || kSplitWc == 8 || kSplitWc == 16, "sub-plane group width 4, 8 or 16");
 
 template <int N> constexpr bool split_size() { return N == 256; }
 
 // Geometry of the split schedule.
 struct SGeo {
-  int n, M, m;        // edge, n/2, rows per sub-plane (n / P)
-  int nnyq, nchm;     // Nyquist tiles (m / Wc), kz chunks per k1 (M / Wc)
-  int ntile;          // nnyq + m * nchm
+  int n, M, m;   // edge, n/2, rows per sub-plane (n / P)
+  int ntile;     // x tiles of the update (kSplitWc columns per sub-plane group)
+  int ntile_c;   // x tiles of the contraction (kSplitWcC)
   long long kfs, nq, nvox;
 };
+// x tiles with WC columns per sub-plane group: m / WC Nyquist tiles, then m * (M / WC) main tiles
+template <int WC> __host__ __device__ constexpr int split_ntile(int m, int M) { return m / WC + m * (M / WC); }
 
 __host__ __device__ constexpr int bitrev4(int j) { return ((j & 1) << 1) | (j >> 1); }
 
@@ -79,40 +87,36 @@ __device__ __forceinline__ int opaque(int v) {
   return r;
 }
 
-// x tile of the split schedule: W = P * Wc columns, column c = j * Wc + i (sub-plane j). Main
-// tiles cover one k1 and the kz chunk [kz0, kz0 + Wc); Nyquist tiles the kz = n/2 value of the Wc
+// x tile of the split schedule: W = P * WC columns, column c = j * WC + i (sub-plane j). Main
+// tiles cover one k1 and the kz chunk [kz0, kz0 + WC); Nyquist tiles the kz = n/2 value of the WC
 // consecutive k1 = k1_0 + i. Element (c, x) of a field: base + j * gs + i + x * ps (E / A form).
 // After the lane stage, column c carries ky = k1 + m * bitrev(j) (and kz).
-struct XSTile {
+template <int WC> struct XSTile {
   unsigned base, gs, ps;
   int k1, kz0;
   bool nyq;
-  __device__ __forceinline__ unsigned coff(int c) const {
-    return base + (unsigned)(c / kSplitWc) * gs + (unsigned)(c % kSplitWc);
-  }
-  __device__ __forceinline__ int ky(int c, int m) const {
-    return (nyq ? k1 + c % kSplitWc : k1) + m * bitrev4(c / kSplitWc);
-  }
-  __device__ __forceinline__ int kz(int c, int M) const { return nyq ? M : kz0 + c % kSplitWc; }
-  __device__ __forceinline__ int k1c(int c) const { return nyq ? k1 + c % kSplitWc : k1; }
+  __device__ __forceinline__ unsigned coff(int c) const { return base + (unsigned)(c / WC) * gs + (unsigned)(c % WC); }
+  __device__ __forceinline__ int ky(int c, int m) const { return (nyq ? k1 + c % WC : k1) + m * bitrev4(c / WC); }
+  __device__ __forceinline__ int kz(int c, int M) const { return nyq ? M : kz0 + c % WC; }
+  __device__ __forceinline__ int k1c(int c) const { return nyq ? k1 + c % WC : k1; }
 };
 
 // Work group -> tile. Work groups are dealt to the 8 XCDs round robin (XCD = id % 8); the G
 // tiles Gu .. Gu + G - 1 share every 128-byte line of their row segments, so the ids
 // 8Gq + k + 8i (i < G: same XCD k, dispatched together) take them and each line is fetched into
 // one L2.
-__device__ __forceinline__ int xs_swizzle(int b, int ntile) {
-  constexpr int G = kSplitG, B = 8 * G;
+template <int WC> __device__ __forceinline__ int xs_swizzle(int b, int ntile) {
+  constexpr int G = 16 / WC, B = 8 * G;
   if (G == 1 || b >= ntile / B * B) return b;
   const int r = b % B;
   return b - r + G * (r % 8) + r / 8;
 }
 
-__device__ __forceinline__ XSTile xs_tile(int t, const SGeo& g) {
-  if (t < g.nnyq) return XSTile{(unsigned)(g.nq + (long long)t * kSplitWc), (unsigned)g.m, (unsigned)g.n, t * kSplitWc, 0, true};
-  const int u = t - g.nnyq, k1 = u / g.nchm, ch = u % g.nchm;
-  return XSTile{(unsigned)(k1 * g.M + ch * kSplitWc), (unsigned)(g.m * g.M), (unsigned)(g.n * g.M), k1,
-                ch * kSplitWc, false};
+template <int WC> __device__ __forceinline__ XSTile<WC> xs_tile(int t, const SGeo& g) {
+  const int nnyq = g.m / WC, nchm = g.M / WC;
+  if (t < nnyq) return XSTile<WC>{(unsigned)(g.nq + (long long)t * WC), (unsigned)g.m, (unsigned)g.n, t * WC, 0, true};
+  const int u = t - nnyq, k1 = u / nchm, ch = u % nchm;
+  return XSTile<WC>{(unsigned)(k1 * g.M + ch * WC), (unsigned)(g.m * g.M), (unsigned)(g.n * g.M), k1, ch * WC, false};
 }
 
 // The P = 4 point stage across the lanes c, c ^ Wc, c ^ 2 Wc, c ^ 3 Wc (j = lane / Wc % 4).
@@ -157,32 +161,46 @@ struct LaneStage {
   bool j3;       // the lane whose value turns by -+i between the steps
   __device__ explicit LaneStage(int j) : s1((j & 1) ? -1.f : 1.f), s2((j & 2) ? -1.f : 1.f), j3(j == 3) {}
 };
-template <bool FWD> __device__ __forceinline__ float2 lane_dft4(float2 v, const LaneStage& ls) {
+template <bool FWD, int WC> __device__ __forceinline__ float2 lane_dft4(float2 v, const LaneStage& ls) {
   if (FWD) {
-    v = lane_bfly2<2 * kSplitWc>(v, ls.s2);
+    v = lane_bfly2<2 * WC>(v, ls.s2);
     v = make_float2(ls.j3 ? v.y : v.x, ls.j3 ? -v.x : v.y);  // rot<true> on lane 3
-    v = lane_bfly2<kSplitWc>(v, ls.s1);
+    v = lane_bfly2<WC>(v, ls.s1);
   } else {
-    v = lane_bfly2<kSplitWc>(v, ls.s1);
+    v = lane_bfly2<WC>(v, ls.s1);
     v = make_float2(ls.j3 ? -v.y : v.x, ls.j3 ? v.x : v.y);  // rot<false> on lane 3
-    v = lane_bfly2<2 * kSplitWc>(v, ls.s2);
+    v = lane_bfly2<2 * WC>(v, ls.s2);
   }
   return v;
 }
 
-template <int N> constexpr int split_xthreads() { return kSplitW * (N / edge_radix<N>()); }
+// x update: the edge plan (16 * 16 at n = 256), kSplitWc columns per sub-plane group
+template <int N> constexpr int split_xthreads() { return split_w<kSplitWc>() * (N / edge_radix<N>()); }
+// contraction: the edge plan, or 8 * 8 * 4 (last radix 4: 4 values of each field per thread)
+template <int N> struct XcPlan {
+  using type = FwdPlan<N>;
+  static constexpr int rmin = PlanOf<N>::rmin, rlast = edge_radix<N>();
+};
+#if GSK_XC_PLAN == 1
+template <> struct XcPlan<256> {
+  using type = Radices<8, 8, 4>;
+  static constexpr int rmin = 4, rlast = 4;
+};
+#endif
+template <int N> constexpr int split_cthreads() { return split_w<kSplitWcC>() * (N / XcPlan<N>::rmin); }
 
-// B tensor in the split tile order, packed p <= q: ((t * npq + pq) * n + kx) * W + c.
-template <int = 0>
+// B tensor in the contraction's tile order, packed p <= q: ((t * npq + pq) * n + kx) * W + c.
+template <int WC>
 __global__ void k_btensor_split(float* __restrict__ Bt, SGeo g, BArgs a) {
-  const int n = g.n, T = n * kSplitW;
+  constexpr int W = split_w<WC>();
+  const int n = g.n, T = n * W;
   const int V = a.V1, npq = V * (V + 1) / 2;
-  const long total = (long)g.ntile * T;
+  const long total = (long)split_ntile<WC>(g.m, g.M) * T;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     const int e = (int)(i % T);
     const long t = i / T;
-    const XSTile s = xs_tile((int)t, g);
-    const int c = e % kSplitW, kx = e / kSplitW;
+    const XSTile<WC> s = xs_tile<WC>((int)t, g);
+    const int c = e % W, kx = e / W;
     double out[64];
     btensor_half_spec(kx, s.ky(c, g.m), s.kz(c, g.M), n, a, out);
     for (int p = 0; p < V; ++p)
@@ -198,19 +216,20 @@ __global__ void k_btensor_split(float* __restrict__ Bt, SGeo g, BArgs a) {
 // sum_k w(kz) sum_pq B_pq Re(S_p* S_q) of the tile into part[tile] instead (w = 1 for kz in
 // {0, n/2}, 2 otherwise).
 template <int N, int V, int MODE>
-__global__ void __launch_bounds__(split_xthreads<N>()) k_xcontract_split(float2* __restrict__ kbuf,
+__global__ void __launch_bounds__(split_cthreads<N>()) k_xcontract_split(float2* __restrict__ kbuf,
                                                                          const float* __restrict__ Bt, SGeo g, MartK mk,
                                                                          const float2* __restrict__ twN,
                                                                          double* __restrict__ part) {
-  constexpr int RL = edge_radix<N>(), W = kSplitW, T = N * W;
+  constexpr int WC = kSplitWcC, RL = XcPlan<N>::rlast, W = split_w<WC>(), T = N * W;
+  using Plan = typename XcPlan<N>::type;
   constexpr int NPQ = V * (V + 1) / 2;
   extern __shared__ float2 smem[];
   __shared__ double sh[16];
   float2* tw = smem;
   float2* tile = smem + N;
-  const int t = xs_swizzle(blockIdx.x, g.ntile);
-  const XSTile s = xs_tile(t, g);
-  const int b = threadIdx.x, c = b % W, jl = c / kSplitWc;
+  const int t = xs_swizzle<WC>(blockIdx.x, g.ntile_c);
+  const XSTile<WC> s = xs_tile<WC>(t, g);
+  const int b = threadIdx.x, c = b % W, jl = c / WC;
   const LaneStage lst(jl);
   load_tw<N>(tw, twN);
   __syncthreads();
@@ -219,14 +238,14 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xcontract_split(float2*
 #pragma unroll
   for (int f = 0; f < V; ++f) {
     const float2* fb = kbuf + f * g.kfs;
-    fft_g2r<float2, N>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, S[f]);
+    fft_g2r_p<float2, N, Plan>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, S[f]);
   }
   // w_n^(-k1 j): the lane stage's twiddle of this column
   const float2 wt = tw[(s.k1c(c) * jl) % N];
 #pragma unroll
   for (int f = 0; f < V; ++f)
 #pragma unroll
-    for (int r = 0; r < RL; ++r) S[f][r] = lane_dft4<true>(cmul(S[f][r], wt), lst);
+    for (int r = 0; r < RL; ++r) S[f][r] = lane_dft4<true, WC>(cmul(S[f][r], wt), lst);
   // keep the B loads below the lane stage: hoisted above it they hold 16 * NPQ registers at once
   asm volatile("" ::: "memory");
   const float* Bb = Bt + (long)t * NPQ * T;
@@ -257,11 +276,11 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xcontract_split(float2*
 #pragma unroll
     for (int f = 0; f < V; ++f)
 #pragma unroll
-      for (int r = 0; r < RL; ++r) S[f][r] = cmul(lane_dft4<false>(S[f][r], lst), wi);
+      for (int r = 0; r < RL; ++r) S[f][r] = cmul(lane_dft4<false, WC>(S[f][r], lst), wi);
 #pragma unroll
     for (int f = 0; f < V; ++f) {
       float2* fb = kbuf + f * g.kfs;
-      fft_r2g<float2, N>(tile, Ls, tw, S[f], [&](int cc, int kx, float2 v) {
+      fft_r2g_p<float2, N, Plan>(tile, Ls, tw, S[f], [&](int cc, int kx, float2 v) {
         at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = v;
       });
     }
@@ -297,13 +316,13 @@ template <int N, int MODE>
 __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* __restrict__ kbuf,
                                                                        float2* __restrict__ state, SGeo g, MartK mk,
                                                                        const float2* __restrict__ twN) {
-  constexpr int RL = edge_radix<N>(), W = kSplitW, T = N * W;
+  constexpr int WC = kSplitWc, RL = edge_radix<N>(), W = split_w<WC>(), T = N * W;
   extern __shared__ float2 smem[];
   float2* tw = smem;
   float2* tile = smem + N;
-  const int t = xs_swizzle(blockIdx.x, g.ntile), f = blockIdx.y, F = gridDim.y;
-  const XSTile s = xs_tile(t, g);
-  const int b = threadIdx.x, c = b % W, j = b / W, jl = c / kSplitWc;
+  const int t = xs_swizzle<WC>(blockIdx.x, g.ntile), f = blockIdx.y, F = gridDim.y;
+  const XSTile<WC> s = xs_tile<WC>(t, g);
+  const int b = threadIdx.x, c = b % W, j = b / W, jl = c / WC;
   const LaneStage lst(jl);
   float2* st = state + ((long)t * F + f) * T;
   float2 sv[RL];
@@ -319,7 +338,7 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   fft_g2r<float2, N>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, v);
   const float2 wt = tw[(s.k1c(c) * jl) % N];
 #pragma unroll
-  for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true>(cmul(v[r], wt), lst);
+  for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true, WC>(cmul(v[r], wt), lst);
   if (MODE == 1) {
 #pragma unroll
     for (int r = 0; r < RL; ++r) at32(st, (unsigned)(opaque(b) + r * (N / RL) * W)) = v[r];
@@ -339,7 +358,7 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   }
   const float2 wi = make_float2(wt.x, -wt.y);
 #pragma unroll
-  for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false>(v[r], lst), wi);
+  for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false, WC>(v[r], lst), wi);
   fft_r2g<float2, N>(tile, Ls, tw, v,
                      [&](int cc, int kx, float2 val) { at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = val; });
 }
@@ -349,15 +368,16 @@ template <int = 0>
 __global__ void r_state_grad_split(const float2* __restrict__ state, SGeo g, int F, float kfac,
                                    double* __restrict__ part) {
   __shared__ double sh[16];
-  const int n = g.n, T = n * kSplitW;
+  constexpr int W = split_w<kSplitWc>();
+  const int n = g.n, T = n * W;
   const long total = (long)g.ntile * F * T;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     const int e = (int)(i % T);
     const long tf = i / T;
     const int f = (int)(tf % F);
-    const XSTile s = xs_tile((int)(tf / F), g);
-    const int c = e % kSplitW, kx = e / kSplitW;
+    const XSTile<kSplitWc> s = xs_tile<kSplitWc>((int)(tf / F), g);
+    const int c = e % W, kx = e / W;
     const int ky = s.ky(c, g.m), kz = s.kz(c, g.M);
     const double kxv = (double)kfac * ((kx < n / 2) ? kx : kx - n), kyv = (double)kfac * ((ky < n / 2) ? ky : ky - n),
                  kzv = (double)kfac * kz;
