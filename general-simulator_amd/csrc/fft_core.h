@@ -344,33 +344,55 @@ __device__ __forceinline__ void fft_g2g(T2* lds, const Lines& L, const T2* tw, L
 
 // global -> registers forward transform; vreg[edge_radix<N>()] receives positions j + r*N/Rl
 template <typename T2, int N, class LD>
-__device__ __forceinline__ void fft_g2r(T2* lds, const Lines& L, const T2* tw, LD ld, T2* vreg) {
+__device__ __forceinline__ void fft_g2r(T2* lds, const Lines& L, const T2* tw, LD ld, T2* vreg, int b = threadIdx.x) {
   auto st = [](int, int, T2) {};
-  gchain<T2, N, true, 1, 2, 1>(lds, L, tw, ld, st, vreg, FwdPlan<N>{});
+  gchain<T2, N, true, 1, 2, 1>(lds, L, tw, ld, st, vreg, FwdPlan<N>{}, b);
 }
 
 // registers -> global inverse transform starting from the positions left by fft_g2r
 template <typename T2, int N, class ST>
-__device__ __forceinline__ void fft_r2g(T2* lds, const Lines& L, const T2* tw, T2* vreg, ST st) {
+__device__ __forceinline__ void fft_r2g(T2* lds, const Lines& L, const T2* tw, T2* vreg, ST st, int b = threadIdx.x) {
   auto ld = [](int, int) { return T2{}; };
   // in-place use: every thread has read its operands (fft_g2r) before anyone writes
   __syncthreads();
-  gchain<T2, N, false, 2, 1, 1>(lds, L, tw, ld, st, vreg, InvPlan<N>{});
+  gchain<T2, N, false, 2, 1, 1>(lds, L, tw, ld, st, vreg, InvPlan<N>{}, b);
 }
 
+// The optional b of these calls is the calling thread's index (threadIdx.x by default). A caller
+// whose work group is a power of two can pass threadIdx.x & (T - 1) so that the compiler knows every
+// butterfly slot is taken (no branch around a pass), and an opaque copy of it per call so that the
+// pass's index arithmetic is recomputed rather than kept live from one call to the next.
+//
 // The same pair with an explicit radix plan (forward in order, inverse reversed): vreg holds the
 // last radix's positions j + r*N/Rl; the work group provides nl*N/rmin threads (passes of
 // larger radix leave the surplus idle).
 template <typename T2, int N, class Plan, class LD>
-__device__ __forceinline__ void fft_g2r_p(T2* lds, const Lines& L, const T2* tw, LD ld, T2* vreg) {
+__device__ __forceinline__ void fft_g2r_p(T2* lds, const Lines& L, const T2* tw, LD ld, T2* vreg, int b = threadIdx.x) {
   auto st = [](int, int, T2) {};
-  gchain<T2, N, true, 1, 2, 1>(lds, L, tw, ld, st, vreg, Plan{});
+  gchain<T2, N, true, 1, 2, 1>(lds, L, tw, ld, st, vreg, Plan{}, b);
 }
 template <typename T2, int N, class Plan, class ST>
-__device__ __forceinline__ void fft_r2g_p(T2* lds, const Lines& L, const T2* tw, T2* vreg, ST st) {
+__device__ __forceinline__ void fft_r2g_p(T2* lds, const Lines& L, const T2* tw, T2* vreg, ST st, int b = threadIdx.x) {
   auto ld = [](int, int) { return T2{}; };
   __syncthreads();
-  gchain<T2, N, false, 2, 1, 1>(lds, L, tw, ld, st, vreg, typename RevImpl<Plan, Radices<>>::type{});
+  gchain<T2, N, false, 2, 1, 1>(lds, L, tw, ld, st, vreg, typename RevImpl<Plan, Radices<>>::type{}, b);
+}
+
+// The same pair ending (forward) or starting (inverse) in the LDS tile instead of registers: the
+// forward result sits at the positions the last pass's thread holds, (l, j + r*N/Rl), which the
+// first inverse pass reads back, so a thread may work on its own positions in between without a
+// barrier.
+template <typename T2, int N, class Plan, class LD>
+__device__ __forceinline__ void fft_g2l_p(T2* lds, const Lines& L, const T2* tw, LD ld, int b = threadIdx.x) {
+  auto st = [](int, int, T2) {};
+  T2* none = nullptr;
+  gchain<T2, N, true, 1, 0, 1>(lds, L, tw, ld, st, none, Plan{}, b);
+}
+template <typename T2, int N, class Plan, class ST>
+__device__ __forceinline__ void fft_l2g_p(T2* lds, const Lines& L, const T2* tw, ST st, int b = threadIdx.x) {
+  auto ld = [](int, int) { return T2{}; };
+  T2* none = nullptr;
+  gchain<T2, N, false, 0, 1, 1>(lds, L, tw, ld, st, none, typename RevImpl<Plan, Radices<>>::type{}, b);
 }
 
 // All-LDS transform with the pass stride Ns known at compile time (constant index math);

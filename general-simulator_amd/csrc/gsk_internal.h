@@ -537,7 +537,7 @@ inline void build_slots(gsk_sim* s) {
     const int npq = F * (F + 1) / 2;
     if (s->split) {  // four launches: the split-y schedule
       s->slots.push_back({"x_contract", F * 2 * hs + npq * hs / 2});
-      s->slots.push_back({"sub_mid", F * (2 * hs + F * rv)});  // R and F spectra, eta of every variant
+      s->slots.push_back({"sub_mid", F * (2 * hs + rv)});  // R and F spectra, each eta once
       s->slots.push_back({"x_update_g0", F * 4 * hs});
       s->slots.push_back({"sub_sq", F * (2 * hs + rv)});  // eta and S spectra, eta written
       return;
@@ -545,8 +545,8 @@ inline void build_slots(gsk_sim* s) {
     const bool pl = plane_on(s);
     if (!pl) s->slots.push_back({"y_fwd_S", F * 2 * hs});
     s->slots.push_back({"x_contract", F * 2 * hs + npq * hs / 2});
-    if (pl) {  // R and F spectra, eta of every variant per variant
-      s->slots.push_back({"plane_mid", F * (2 * hs + F * rv)});
+    if (pl) {  // R and F spectra, each eta once (the variant groups share them through L2)
+      s->slots.push_back({"plane_mid", F * (2 * hs + rv)});
     } else {
       s->slots.push_back({"y_inv_R", F * 2 * hs});
       s->slots.push_back({"zmid_force", F * (2 * hs + rv)});

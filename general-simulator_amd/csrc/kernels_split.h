@@ -215,12 +215,22 @@ __global__ void k_btensor_split(float* __restrict__ Bt, SGeo g, BArgs a) {
 // lane stage -> inverse x FFT -> A-form R_hat (MODE 0). MODE 1: the elastic energy sum
 // sum_k w(kz) sum_pq B_pq Re(S_p* S_q) of the tile into part[tile] instead (w = 1 for kz in
 // {0, n/2}, 2 otherwise).
+//
+// MODE 0 holds V - 1 fields' spectra in registers (16 values each at n = 256) and parks the last
+// field's in the LDS tile, at the positions the thread itself holds, so that at V <= 3 the kernel
+// fits in 128 VGPRs: two 512-thread work groups (2 x 66 KB of LDS) per CU, 4 waves per SIMD.
+// Every engine call takes the thread index through opaque(): the index arithmetic of a transform
+// (16 global offsets, 16 twiddle addresses) is then recomputed per field instead of being kept live
+// across all of them, and the power-of-two mask tells the compiler that every pass has all threads
+// active (one basic block, so global accesses use a scalar base and one 32-bit offset register).
+template <int V> constexpr int xc_waves() { return V <= 3 ? 4 : 2; }
 template <int N, int V, int MODE>
-__global__ void __launch_bounds__(split_cthreads<N>()) k_xcontract_split(float2* __restrict__ kbuf,
-                                                                         const float* __restrict__ Bt, SGeo g, MartK mk,
-                                                                         const float2* __restrict__ twN,
-                                                                         double* __restrict__ part) {
+__global__ void __launch_bounds__(split_cthreads<N>()) __attribute__((amdgpu_waves_per_eu(xc_waves<V>())))
+k_xcontract_split(float2* __restrict__ kbuf, const float* __restrict__ Bt, SGeo g, MartK mk,
+                  const float2* __restrict__ twN, double* __restrict__ part) {
   constexpr int WC = kSplitWcC, RL = XcPlan<N>::rlast, W = split_w<WC>(), T = N * W;
+  constexpr int TB = split_cthreads<N>();
+  static_assert((TB & (TB - 1)) == 0, "power-of-two work group");
   using Plan = typename XcPlan<N>::type;
   constexpr int NPQ = V * (V + 1) / 2;
   extern __shared__ float2 smem[];
@@ -229,34 +239,41 @@ __global__ void __launch_bounds__(split_cthreads<N>()) k_xcontract_split(float2*
   float2* tile = smem + N;
   const int t = xs_swizzle<WC>(blockIdx.x, g.ntile_c);
   const XSTile<WC> s = xs_tile<WC>(t, g);
-  const int b = threadIdx.x, c = b % W, jl = c / WC;
+  const int b = (int)threadIdx.x & (TB - 1), c = b % W, jl = c / WC;
+  auto bo = [&]() { return opaque(b) & (TB - 1); };
   const LaneStage lst(jl);
   load_tw<N>(tw, twN);
   __syncthreads();
   const Lines Ls{W, W, 1, 0, W};
-  float2 S[V][RL];
-#pragma unroll
-  for (int f = 0; f < V; ++f) {
-    const float2* fb = kbuf + f * g.kfs;
-    fft_g2r_p<float2, N, Plan>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, S[f]);
-  }
   // w_n^(-k1 j): the lane stage's twiddle of this column
   const float2 wt = tw[(s.k1c(c) * jl) % N];
-#pragma unroll
-  for (int f = 0; f < V; ++f)
-#pragma unroll
-    for (int r = 0; r < RL; ++r) S[f][r] = lane_dft4<true, WC>(cmul(S[f][r], wt), lst);
-  // keep the B loads below the lane stage: hoisted above it they hold 16 * NPQ registers at once
-  asm volatile("" ::: "memory");
   const float* Bb = Bt + (long)t * NPQ * T;
   if constexpr (MODE == 0) {
+    constexpr int PV = V - 1;  // fields whose spectra stay in registers
+    float2 S[PV > 0 ? PV : 1][RL];
+#pragma unroll
+    for (int f = 0; f < PV; ++f) {
+      const float2* fb = kbuf + f * g.kfs;
+      fft_g2r_p<float2, N, Plan>(
+          tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, S[f], bo());
+    }
+    {
+      const float2* fb = kbuf + PV * g.kfs;
+      fft_g2l_p<float2, N, Plan>(
+          tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, bo());
+    }
+    // position r of this thread's column segment: S[f][r] in registers, tile[b + e] for field PV
+    const float2 wi = make_float2(wt.x, -wt.y);
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const int e = r * (N / RL) * W;
+      float2 x[V];
+#pragma unroll
+      for (int f = 0; f < PV; ++f) x[f] = lane_dft4<true, WC>(cmul(S[f][r], wt), lst);
+      x[PV] = lane_dft4<true, WC>(cmul(tile[b + e], wt), lst);
       float bv[NPQ];
 #pragma unroll
       for (int pq = 0; pq < NPQ; ++pq) bv[pq] = at32(Bb, (unsigned)(pq * T + b + e));
-      float2 R[V];
 #pragma unroll
       for (int p = 0; p < V; ++p) {
         float ax = 0.f, ay = 0.f;
@@ -264,27 +281,39 @@ __global__ void __launch_bounds__(split_cthreads<N>()) k_xcontract_split(float2*
         for (int q = 0; q < V; ++q) {
           const int lo = p < q ? p : q, hi = p < q ? q : p;
           const int pq = lo * V - (lo * (lo - 1)) / 2 + (hi - lo);
-          ax += bv[pq] * S[q][r].x;
-          ay += bv[pq] * S[q][r].y;
+          ax += bv[pq] * x[q].x;
+          ay += bv[pq] * x[q].y;
         }
-        R[p] = make_float2(ax * mk.inv_n3, ay * mk.inv_n3);
+        const float2 R = cmul(lane_dft4<false, WC>(make_float2(ax * mk.inv_n3, ay * mk.inv_n3), lst), wi);
+        if (p < PV) S[p < PV ? p : 0][r] = R;
+        else tile[b + e] = R;
       }
-#pragma unroll
-      for (int p = 0; p < V; ++p) S[p][r] = R[p];
     }
-    const float2 wi = make_float2(wt.x, -wt.y);
+    {
+      float2* fb = kbuf + PV * g.kfs;
+      fft_l2g_p<float2, N, Plan>(
+          tile, Ls, tw, [&](int cc, int kx, float2 v) { at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = v; },
+          bo());
+    }
+#pragma unroll
+    for (int f = 0; f < PV; ++f) {
+      float2* fb = kbuf + f * g.kfs;
+      fft_r2g_p<float2, N, Plan>(
+          tile, Ls, tw, S[f], [&](int cc, int kx, float2 v) { at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = v; },
+          bo());
+    }
+  } else {
+    float2 S[V][RL];
+#pragma unroll
+    for (int f = 0; f < V; ++f) {
+      const float2* fb = kbuf + f * g.kfs;
+      fft_g2r_p<float2, N, Plan>(
+          tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, S[f], bo());
+    }
 #pragma unroll
     for (int f = 0; f < V; ++f)
 #pragma unroll
-      for (int r = 0; r < RL; ++r) S[f][r] = cmul(lane_dft4<false, WC>(S[f][r], lst), wi);
-#pragma unroll
-    for (int f = 0; f < V; ++f) {
-      float2* fb = kbuf + f * g.kfs;
-      fft_r2g_p<float2, N, Plan>(tile, Ls, tw, S[f], [&](int cc, int kx, float2 v) {
-        at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = v;
-      });
-    }
-  } else {
+      for (int r = 0; r < RL; ++r) S[f][r] = lane_dft4<true, WC>(cmul(S[f][r], wt), lst);
     const int kz = s.kz(c, g.M);
     const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
     double acc = 0.0;
@@ -320,9 +349,11 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   extern __shared__ float2 smem[];
   float2* tw = smem;
   float2* tile = smem + N;
+  constexpr int TB = split_xthreads<N>();
+  static_assert((TB & (TB - 1)) == 0, "power-of-two work group");
   const int t = xs_swizzle<WC>(blockIdx.x, g.ntile), f = blockIdx.y, F = gridDim.y;
   const XSTile<WC> s = xs_tile<WC>(t, g);
-  const int b = threadIdx.x, c = b % W, j = b / W, jl = c / WC;
+  const int b = (int)threadIdx.x & (TB - 1), c = b % W, j = b / W, jl = c / WC;
   const LaneStage lst(jl);
   float2* st = state + ((long)t * F + f) * T;
   float2 sv[RL];
@@ -335,7 +366,7 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   const Lines Ls{W, W, 1, 0, W};
   float2* fb = kbuf + f * g.kfs;
   float2 v[RL];
-  fft_g2r<float2, N>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, v);
+  fft_g2r<float2, N>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, v, b);
   const float2 wt = tw[(s.k1c(c) * jl) % N];
 #pragma unroll
   for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true, WC>(cmul(v[r], wt), lst);
@@ -359,8 +390,9 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   const float2 wi = make_float2(wt.x, -wt.y);
 #pragma unroll
   for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false, WC>(v[r], lst), wi);
-  fft_r2g<float2, N>(tile, Ls, tw, v,
-                     [&](int cc, int kx, float2 val) { at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = val; });
+  fft_r2g<float2, N>(
+      tile, Ls, tw, v, [&](int cc, int kx, float2 val) { at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = val; },
+      opaque(b) & (TB - 1));
 }
 
 // Gradient energy from the split-order state: sum_f sum_k w(kz) k^2 |eta_hat_f|^2 per field.
