@@ -80,8 +80,8 @@ extern "C" {
 
 int gsk_adjoint_forward(gsk_sim* s, int nsteps, int K) {
   if (!s || nsteps < 1 || K < 1) return fail(GSK_ERR_ARG, "bad argument");
-  if (s->cfg.path != GSK_PATH_FUSED32 || s->cfg.physics != GSK_PHYS_MART || s->cfg.rule != GSK_RULE_SPECTRAL)
-    return fail(GSK_ERR_UNSUPPORTED, "the adjoint covers the FP32 fused spectral martensite step");
+  if (s->cfg.path != GSK_PATH_FUSED32 || s->cfg.physics != GSK_PHYS_MART || s->cfg.rule != GSK_RULE_SPECTRAL || s->bx)
+    return fail(GSK_ERR_UNSUPPORTED, "the adjoint covers the FP32 fused spectral martensite step on cubes");
   if (s->V > 4) return fail(GSK_ERR_UNSUPPORTED, "the adjoint supports up to 4 variants");
   CHK(ensure_device(s->dev));
   CHK(prepare(s));

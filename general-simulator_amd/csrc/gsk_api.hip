@@ -70,6 +70,10 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
   if (!cfg || !out) return fail(GSK_ERR_ARG, "null argument");
   *out = nullptr;
   if (!supported_n(cfg->n)) return fail(GSK_ERR_UNSUPPORTED, "unsupported grid size " + std::to_string(cfg->n));
+  for (int e : {cfg->ny, cfg->nz})
+    if (e != 0 && !supported_n(e)) return fail(GSK_ERR_UNSUPPORTED, "unsupported grid size " + std::to_string(e));
+  if (cfg->ny < 0 || cfg->nz < 0 || cfg->dy < 0 || cfg->dz < 0) return fail(GSK_ERR_ARG, "negative grid argument");
+  const bool box = box_grid(*cfg);
   if (cfg->physics != GSK_PHYS_MART && cfg->physics != GSK_PHYS_PREC) return fail(GSK_ERR_ARG, "unknown physics");
   if (cfg->path != GSK_PATH_FUSED32 && cfg->path != GSK_PATH_PLAIN64) return fail(GSK_ERR_ARG, "unknown path");
   if (cfg->rule != GSK_RULE_SPECTRAL && cfg->rule != GSK_RULE_REFERENCE) return fail(GSK_ERR_ARG, "unknown rule");
@@ -82,6 +86,9 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
   gsk_sim* s = new gsk_sim();
   s->cfg = *cfg;
   s->n = cfg->n;
+  s->nx = cfg->n;
+  s->ny = cfg->ny > 0 ? cfg->ny : cfg->n;
+  s->nz = cfg->nz > 0 ? cfg->nz : cfg->n;
   s->V = cfg->nvar;
   s->F = cfg->physics == GSK_PHYS_MART ? s->V : s->V + 1;
   s->dev = cfg->device;
@@ -94,6 +101,7 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
   const size_t N3 = nvox(s);
   choose_geometry(s, s->F, (s->cfg.physics == GSK_PHYS_MART && s->F > 4) ? s->F : 1);
   choose_plain(s);
+  if (box) s->split = false;
   auto up = [&](auto*& dst, const auto& host) -> int {
     using T = std::remove_reference_t<decltype(*dst)>;
     if (hipMalloc((void**)&dst, host.size() * sizeof(T)) != hipSuccess) return fail(GSK_ERR_HIP, "hipMalloc failed");
@@ -117,7 +125,9 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
     return GSK_OK;
   };
   const size_t T = (size_t)n * s->g.W, ntile = (size_t)s->g.ntile;
-  if (cfg->path == GSK_PATH_FUSED32) {
+  if (box) {
+    if ((rc = box_create(s)) != GSK_OK) return bail(rc);
+  } else if (cfg->path == GSK_PATH_FUSED32) {
     if ((rc = alloc(s->fld, (size_t)s->F * N3)) != GSK_OK) return bail(rc);
     if ((rc = alloc(s->kbuf, (size_t)s->F * s->g.kfs)) != GSK_OK) return bail(rc);
     if ((rc = alloc(s->state, ntile * s->F * T)) != GSK_OK) return bail(rc);
@@ -146,7 +156,7 @@ int gsk_create(const gsk_config* cfg, gsk_sim** out) {
   s->rblocks = std::max<int>(1024, (int)ntile);
   if ((rc = alloc(s->part, (size_t)s->rblocks * 16)) != GSK_OK) return bail(rc);
   if ((rc = alloc(s->rout, 16)) != GSK_OK) return bail(rc);
-  if (cfg->path == GSK_PATH_FUSED32 && (rc = alloc(s->dstep, 1)) != GSK_OK) return bail(rc);
+  if (cfg->path == GSK_PATH_FUSED32 && !box && (rc = alloc(s->dstep, 1)) != GSK_OK) return bail(rc);
   if (hipStreamSynchronize(s->stream) != hipSuccess) return bail(fail(GSK_ERR_HIP, "stream synchronisation failed"));
   build_slots(s);
   *out = s;
@@ -159,6 +169,8 @@ int gsk_destroy(gsk_sim* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   adj_free(s->adj);
   s->adj = nullptr;
+  box_free(s->bx);
+  s->bx = nullptr;
   free_dev(s->twN); free_dev(s->twM); free_dev(s->rtn); free_dev(s->twN64);
   if (s->sharedB) s->Bt = nullptr;
   free_dev(s->fld); free_dev(s->kbuf); free_dev(s->state); free_dev(s->Bt); free_dev(s->dstep);
@@ -216,7 +228,9 @@ int gsk_build_btensor(gsk_sim* s, const double* C81, const double* eps, int neps
   s->haveBargs = true;
   for (int i = 0; i < neps * 9; ++i) s->eps[i] = eps[i];
   s->haveEps = true;
-  if (s->cfg.path == GSK_PATH_FUSED32 && s->split) {
+  if (s->bx) {
+    CHK(box_build_btensor(s, a));
+  } else if (s->cfg.path == GSK_PATH_FUSED32 && s->split) {
     const long total = (long)s->sg.ntile_c * s->n * split_w<kSplitWcC>();
     hipLaunchKernelGGL(k_btensor_split<kSplitWcC>, dim3(grid_1d(total)), dim3(256), 0, s->stream, s->Bt, s->sg, a);
   } else if (s->cfg.path == GSK_PATH_FUSED32) {
@@ -291,6 +305,12 @@ int gsk_set_field(gsk_sim* s, const char* name, const void* host, int dtype) {
   CHK(field_info(s, name, &off, &cnt));
   CHK(ensure_device(s->dev));
   const size_t N3 = nvox(s), count = (size_t)cnt * N3;
+  if (s->bx) {
+    std::vector<double> h(count);
+    for (size_t i = 0; i < count; ++i)
+      h[i] = dtype == GSK_F32 ? (double)((const float*)host)[i] : ((const double*)host)[i];
+    return box_set_field(s, off, count, h.data());
+  }
   if (s->cfg.path == GSK_PATH_PLAIN64) {
     std::vector<double> h(count);
     for (size_t i = 0; i < count; ++i)
@@ -343,6 +363,15 @@ int gsk_get_field(gsk_sim* s, const char* name, void* host, int dtype) {
   CHK(ensure_device(s->dev));
   CHK(sync(s));
   const size_t N3 = nvox(s), count = (size_t)cnt * N3;
+  if (s->bx) {
+    std::vector<double> h(count);
+    CHK(box_get_field(s, off, count, h.data()));
+    for (size_t i = 0; i < count; ++i) {
+      if (dtype == GSK_F32) ((float*)host)[i] = (float)h[i];
+      else ((double*)host)[i] = h[i];
+    }
+    return GSK_OK;
+  }
   if (s->cfg.path == GSK_PATH_PLAIN64) {
     std::vector<double> h(count);
     HIPCHK(hipMemcpy(h.data(), s->eta64 + (size_t)off * N3, count * sizeof(double), hipMemcpyDeviceToHost));
@@ -390,7 +419,7 @@ int gsk_step(gsk_sim* s, int nsteps) {
   CHK(ensure_device(s->dev));
   CHK(prepare(s));
   for (int i = 0; i < nsteps; ++i) {
-    CHK(dispatch_n<DStep>(s->n, s));
+    CHK(step_once(s));
     ++s->step;
   }
   return sync(s);
@@ -438,17 +467,20 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
   const int nb = 512;
   const gsk_config& c = s->cfg;
   double h[16];
+  // the real fields in FP64 (plain path, FP64 box) or FP32 (fused path, FP32 box)
+  const double* f64 = s->bx ? (s->bx->f64 ? (const double*)s->bx->fld : nullptr) : s->eta64;
+  const float* f32 = s->bx ? (s->bx->f64 ? nullptr : (const float*)s->bx->fld) : s->fld;
+  const bool plain = f64 != nullptr;
   auto real_pass = [&]() -> int {
-    const bool plain = s->cfg.path == GSK_PATH_PLAIN64;
     if (c.physics == GSK_PHYS_MART) {
-      if (plain) LAUNCH(r_mart_real<double>, nb, 256, 0, s, s->eta64, N3, s->V, c.A, c.B, c.C, s->part);
-      else LAUNCH(r_mart_real<float>, nb, 256, 0, s, s->fld, N3, s->V, c.A, c.B, c.C, s->part);
+      if (plain) LAUNCH(r_mart_real<double>, nb, 256, 0, s, f64, N3, s->V, c.A, c.B, c.C, s->part);
+      else LAUNCH(r_mart_real<float>, nb, 256, 0, s, f32, N3, s->V, c.A, c.B, c.C, s->part);
     } else {
       PrecE pe{};
       for (int i = 0; i < 8; ++i) pe.A[i] = c.a[i];
       pe.c1 = c.c1; pe.c2 = c.c2; pe.cbar = plain ? 0.0 : s->cbar; pe.V = s->V;
-      if (plain) LAUNCH(r_prec_real<double>, nb, 256, 0, s, s->eta64, N3, pe, s->part);
-      else LAUNCH(r_prec_real<float>, nb, 256, 0, s, s->fld, N3, pe, s->part);
+      if (plain) LAUNCH(r_prec_real<double>, nb, 256, 0, s, f64, N3, pe, s->part);
+      else LAUNCH(r_prec_real<float>, nb, 256, 0, s, f32, N3, pe, s->part);
     }
     LAUNCH(r_finish<>, 1, 1024, 0, s, s->part, nb, 2 + s->V, s->rout);
     HIPCHK(hipMemcpyAsync(h, s->rout, (2 + s->V) * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -477,9 +509,9 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
   if (what == GSK_RED_MASS) {
     if (c.physics != GSK_PHYS_PREC) return fail(GSK_ERR_UNSUPPORTED, "mass is a precipitate reduction");
     CHK(prepare(s));
-    if (s->cfg.path == GSK_PATH_PLAIN64) {  // the FP64 sum of the FP64 field, twice
+    if (s->bx || s->cfg.path == GSK_PATH_PLAIN64) {  // the FP64 sum of the stored field (+ N cbar), twice
       CHK(real_pass());
-      out[0] = out[1] = h[1];
+      out[0] = out[1] = (s->bx ? s->cbar * (double)N3 : 0.0) + h[1];
       return GSK_OK;
     }
     // out[0]: N * mean + Re of the k = 0 mode of the fluctuation state (the conserved quantity)
@@ -498,7 +530,9 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
     CHK(real_pass());
     const double chem = h[0];
     double gr = 0.0, el = 0.0;
-    if (s->cfg.path == GSK_PATH_PLAIN64 && c.physics == GSK_PHYS_PREC) {
+    if (s->bx) {
+      CHK(box_energy(s, &gr, &el));
+    } else if (s->cfg.path == GSK_PATH_PLAIN64 && c.physics == GSK_PHYS_PREC) {
       double ks[3];
       CHK(dispatch_n<DEpP>(s->n, s, ks));
       gr = 0.5 * (c.kappa_c * ks[0] + c.kappa * ks[1]) / (double)N3;
@@ -565,7 +599,7 @@ int gsk_time_steps(gsk_sim* s, int nsteps, double* ms) {
   HIPCHK(hipEventCreate(&b));
   HIPCHK(hipEventRecord(a, s->stream));
   for (int i = 0; i < nsteps; ++i) {
-    CHK(dispatch_n<DStep>(s->n, s));
+    CHK(step_once(s));
     ++s->step;
   }
   HIPCHK(hipEventRecord(b, s->stream));
@@ -587,6 +621,7 @@ int gsk_kernel_times(gsk_sim* s, int nsteps, double* ms, int maxk, int* nk, char
     HIPCHK(hipEventCreate(&e));
     s->ev.push_back(e);
   }
+  if (s->bx) return fail(GSK_ERR_UNSUPPORTED, "per-kernel timing covers the cube schedules");
   s->prof = true;
   s->evi = 0;
   for (int i = 0; i < nsteps; ++i) {

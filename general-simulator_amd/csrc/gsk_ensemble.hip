@@ -313,6 +313,8 @@ int gsk_ensemble_create(const gsk_config* cfgs, int count, gsk_ensemble** out) {
   *out = nullptr;
   for (int i = 1; i < count; ++i)
     if (cfgs[i].device != cfgs[0].device) return fail(GSK_ERR_ARG, "ensemble members must share one device");
+  for (int i = 0; i < count; ++i)
+    if (box_grid(cfgs[i])) return fail(GSK_ERR_UNSUPPORTED, "ensemble members are cubes");
   CHK(ensure_device(cfgs[0].device));
   gsk_ensemble* e = new gsk_ensemble();
   e->dev = cfgs[0].device;

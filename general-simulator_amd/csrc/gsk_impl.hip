@@ -58,10 +58,14 @@ struct Impl {
     CHK(lds_limit((const void*)r_xenergy_mart_r<N, 2>));
     CHK(lds_limit((const void*)r_xenergy_mart_r<N, 3>));
     CHK(lds_limit((const void*)r_xenergy_mart_r<N, 4>));
-    CHK(lds_limit((const void*)p_zrows<N, true>));
-    CHK(lds_limit((const void*)p_zrows<N, false>));
-    CHK(lds_limit((const void*)p_strided<N, true>));
-    CHK(lds_limit((const void*)p_strided<N, false>));
+    CHK(lds_limit((const void*)p_zrows<double2, N, true>));
+    CHK(lds_limit((const void*)p_zrows<double2, N, false>));
+    CHK(lds_limit((const void*)p_strided<double2, N, true>));
+    CHK(lds_limit((const void*)p_strided<double2, N, false>));
+    CHK(lds_limit((const void*)p_zrows<float2, N, true>));
+    CHK(lds_limit((const void*)p_zrows<float2, N, false>));
+    CHK(lds_limit((const void*)p_strided<float2, N, true>));
+    CHK(lds_limit((const void*)p_strided<float2, N, false>));
     // the batched ensemble instantiations (member table)
     CHK(lds_limit((const void*)k_ypass<N, true, true>));
     CHK(lds_limit((const void*)k_ypass<N, false, true>));
@@ -532,7 +536,7 @@ struct Impl {
     k.sc = (float)std::sqrt(std::fabs((0.02 * (c.temperature - c.T0)) / (c.B - c.C)));
     k.wG = (float)c.w_grad; k.wC = (float)c.w_chem; k.wE = (float)c.w_el; k.wD = (float)c.w_dis;
     k.wN = (float)c.noise; k.dt = (float)c.dt;
-    k.g = (float)(1.0 / (2.0 * c.dx) / 3.0);
+    k.gx = k.gy = k.gz = (float)(1.0 / (2.0 * c.dx) / 3.0);
     k.V = s->V;
     k.chem_form = c.chem_form;
     k.step = (uint32_t)s->step;
@@ -540,8 +544,8 @@ struct Impl {
     k.seed = c.seed;
     const long N3 = (long)s->n * s->n * s->n;
     // at most kRefBlocks blocks (grid-stride), so the per-block force sums fit s->part
-    LAUNCH(k_ref_mart<>, std::min(grid_1d(N3 / 4), kRefBlocks), 256, 0, s, s->fld, s->rbuf, s->defect, s->Kd, s->n, k,
-           s->fstat);
+    LAUNCH(k_ref_mart<>, std::min(grid_1d(N3 / 4), kRefBlocks), 256, 0, s, s->fld, s->rbuf, s->defect, s->Kd, s->n, s->n,
+           s->n, k, s->fstat);
     LAUNCH(k_copy4<>, grid_1d((long)s->V * N3 / 4), 256, 0, s, s->fld, s->rbuf, (long)s->V * N3);
     LAUNCH(k_advance<>, 1, 64, 0, s, s->dstep);
     return GSK_OK;
@@ -583,15 +587,15 @@ struct Impl {
     for (int f = 0; f < nf; ++f) {
       double2* b = buf + f * N3;
       const int gz = (int)(n * n / s->pL);
-      if (fwd) LAUNCH((p_zrows<N, true>), gz, s->pzthreads, s->lds_pz, s, b, s->pL, s->twN64);
-      else LAUNCH((p_zrows<N, false>), gz, s->pzthreads, s->lds_pz, s, b, s->pL, s->twN64);
-      const int gs = (int)(n * (n / s->pW));
+      if (fwd) LAUNCH((p_zrows<double2, N, true>), gz, s->pzthreads, s->lds_pz, s, b, s->pL, s->twN64);
+      else LAUNCH((p_zrows<double2, N, false>), gz, s->pzthreads, s->lds_pz, s, b, s->pL, s->twN64);
+      const int nch = (int)(n / s->pW), gs = (int)(n * nch);
       // y: outer x (stride n^2), positions y (stride n)
-      if (fwd) LAUNCH((p_strided<N, true>), gs, s->pxthreads, s->lds_px, s, b, s->pW, n * n, n, s->twN64);
-      else LAUNCH((p_strided<N, false>), gs, s->pxthreads, s->lds_px, s, b, s->pW, n * n, n, s->twN64);
+      if (fwd) LAUNCH((p_strided<double2, N, true>), gs, s->pxthreads, s->lds_px, s, b, s->pW, nch, n * n, n, s->twN64);
+      else LAUNCH((p_strided<double2, N, false>), gs, s->pxthreads, s->lds_px, s, b, s->pW, nch, n * n, n, s->twN64);
       // x: outer y (stride n), positions x (stride n^2)
-      if (fwd) LAUNCH((p_strided<N, true>), gs, s->pxthreads, s->lds_px, s, b, s->pW, n, n * n, s->twN64);
-      else LAUNCH((p_strided<N, false>), gs, s->pxthreads, s->lds_px, s, b, s->pW, n, n * n, s->twN64);
+      if (fwd) LAUNCH((p_strided<double2, N, true>), gs, s->pxthreads, s->lds_px, s, b, s->pW, nch, n, n * n, s->twN64);
+      else LAUNCH((p_strided<double2, N, false>), gs, s->pxthreads, s->lds_px, s, b, s->pW, nch, n, n * n, s->twN64);
     }
     return GSK_OK;
   }
@@ -599,17 +603,17 @@ struct Impl {
   static int step_mart_plain(gsk_sim* s) {
     const long N3 = (long)nvox(s), VN = s->V * N3;
     PlainK k = plain_consts(s);
-    LAUNCH(p_square<>, grid_1d(VN), 256, 0, s, s->eta64, s->S64, VN);
+    LAUNCH(p_square<double>, grid_1d(VN), 256, 0, s, s->eta64, s->S64, VN);
     CHK(c2c(s, s->S64, true, s->V));
-    LAUNCH(p_contract<>, grid_1d(N3), 256, 0, s, s->S64, s->R64, s->B64, N3, k);
+    LAUNCH(p_contract<double>, grid_1d(N3), 256, 0, s, s->S64, s->R64, s->B64, N3, k);
     CHK(c2c(s, s->R64, false, s->V));
-    LAUNCH(p_force<>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->R64, s->F64, N3, k);
-    LAUNCH(p_load<>, grid_1d(VN), 256, 0, s, s->eta64, s->E64, VN);
+    LAUNCH(p_force<double>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->R64, s->F64, N3, k);
+    LAUNCH(p_load<double>, grid_1d(VN), 256, 0, s, s->eta64, s->E64, VN);
     CHK(c2c(s, s->F64, true, s->V));
     CHK(c2c(s, s->E64, true, s->V));
-    LAUNCH(p_update<>, grid_1d(VN), 256, 0, s, s->E64, s->F64, s->n, k);
+    LAUNCH(p_update<double>, grid_1d(VN), 256, 0, s, s->E64, s->F64, k);
     CHK(c2c(s, s->E64, false, s->V));
-    LAUNCH(p_extract<>, grid_1d(VN), 256, 0, s, s->E64, s->eta64, VN);
+    LAUNCH(p_extract<double>, grid_1d(VN), 256, 0, s, s->E64, s->eta64, VN);
     return GSK_OK;
   }
 
@@ -619,12 +623,12 @@ struct Impl {
     const int F = s->F;
     const long N3 = (long)nvox(s), FN = F * N3;
     const PlainPK k = plain_prec_consts(s);
-    LAUNCH(p_prec_local<>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->F64, s->E64, N3, k);
+    LAUNCH(p_prec_local<double>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->F64, s->E64, N3, k, nullptr);
     CHK(c2c(s, s->F64, true, F));
     CHK(c2c(s, s->E64, true, F));
-    LAUNCH(p_prec_update<>, grid_1d(FN), 256, 0, s, s->E64, s->F64, s->B64, s->n, k);
+    LAUNCH(p_prec_update<double>, grid_1d(FN), 256, 0, s, s->E64, s->F64, s->B64, k);
     CHK(c2c(s, s->E64, false, F));
-    LAUNCH(p_extract<>, grid_1d(FN), 256, 0, s, s->E64, s->eta64, FN);
+    LAUNCH(p_extract<double>, grid_1d(FN), 256, 0, s, s->E64, s->eta64, FN);
     return GSK_OK;
   }
 
@@ -632,10 +636,10 @@ struct Impl {
   static int energy_prec_plain(gsk_sim* s, double* out) {
     const long N3 = (long)nvox(s), FN = s->F * N3;
     const PlainPK k = plain_prec_consts(s);
-    LAUNCH(p_load<>, grid_1d(FN), 256, 0, s, s->eta64, s->E64, FN);
+    LAUNCH(p_load<double>, grid_1d(FN), 256, 0, s, s->eta64, s->E64, FN);
     CHK(c2c(s, s->E64, true, s->F));
     const int nb = 256;
-    LAUNCH(r_plain_kspace_prec<>, nb, 256, 0, s, s->E64, s->B64, s->n, s->V, k.kfac, s->part);
+    LAUNCH(r_plain_kspace_prec<double>, nb, 256, 0, s, s->E64, s->B64, k, s->part);
     LAUNCH(r_finish<>, 1, 1024, 0, s, s->part, nb, 3, s->rout);
     HIPCHK(hipMemcpyAsync(out, s->rout, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -683,12 +687,12 @@ struct Impl {
   static int energy_mart_el_plain(gsk_sim* s, double* gr, double* el) {
     const long N3 = (long)nvox(s), VN = s->V * N3;
     PlainK k = plain_consts(s);
-    LAUNCH(p_square<>, grid_1d(VN), 256, 0, s, s->eta64, s->S64, VN);
+    LAUNCH(p_square<double>, grid_1d(VN), 256, 0, s, s->eta64, s->S64, VN);
     CHK(c2c(s, s->S64, true, s->V));
-    LAUNCH(p_load<>, grid_1d(VN), 256, 0, s, s->eta64, s->E64, VN);
+    LAUNCH(p_load<double>, grid_1d(VN), 256, 0, s, s->eta64, s->E64, VN);
     CHK(c2c(s, s->E64, true, s->V));
     const int nb = 256;
-    LAUNCH(r_plain_kspace<>, nb, 256, 0, s, s->E64, s->S64, s->B64, s->n, s->V, k.kfac, s->part);
+    LAUNCH(r_plain_kspace<double>, nb, 256, 0, s, s->E64, s->S64, s->B64, k, s->part);
     LAUNCH(r_finish<>, 1, 1024, 0, s, s->part, nb, 2, s->rout);
     double h[2];
     HIPCHK(hipMemcpyAsync(h, s->rout, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));

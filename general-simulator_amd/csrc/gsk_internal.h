@@ -134,9 +134,27 @@ struct Slot {
 
 struct AdjWS;  // adjoint workspace (checkpoints and buffers), defined with the adjoint entry points
 
+// Box grids (edges nx, ny, nz or spacings dx, dy, dz not all equal): the plain-path kernels on
+// per-axis transforms (gsk_box.hip). Buffers hold T = float (GSK_PATH_FUSED32) or double
+// (GSK_PATH_PLAIN64) values.
+struct BoxWS {
+  bool f64 = false;
+  double dx = 1, dy = 1, dz = 1;
+  void* fld = nullptr;                // F real fields [f][x][y][z]: the state
+  void *S = nullptr, *R = nullptr;    // V complex fields: FFT(eta^2), the contraction R
+  void *E = nullptr, *Fh = nullptr;   // F complex fields: the transformed fields and forces
+  void* B = nullptr;                  // [p][q][voxel] (V x V) or [voxel] (the precipitate's c source)
+  void* tw[3] = {nullptr, nullptr, nullptr};  // twiddles of the x, y and z transform lengths
+  int L = 1, W[2] = {1, 1};           // z rows per tile; column chunk of the x and y passes
+  int tz = 64, tx = 64, ty = 64;      // threads of the z, x and y passes
+  size_t ldz = 0, ldx = 0, ldy = 0;   // their LDS bytes
+};
+
 struct gsk_sim {
   gsk_config cfg{};
   int n = 0, V = 0, F = 0, dev = 0;
+  int nx = 0, ny = 0, nz = 0;  // grid edges (all n for a cube)
+  BoxWS* bx = nullptr;         // box grid: its buffers (null for a cube)
   Geo g{};
   int zthreads = 0, xthreads = 0;
   int cus = 0;  // compute units of the device (0 until first needed)
@@ -229,7 +247,7 @@ inline bool split_wanted(const gsk_sim* s) {
          s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL;
 }
 
-inline size_t nvox(const gsk_sim* s) { return (size_t)s->n * s->n * s->n; }
+inline size_t nvox(const gsk_sim* s) { return (size_t)s->nx * s->ny * s->nz; }
 inline size_t hs_count(const gsk_sim* s) { return (size_t)s->n * s->n * (s->n / 2 + 1); }
 // State element of field 0 at k = 0: first main x tile (after the n/W Nyquist tiles), kx = 0, column 0.
 inline size_t k0_index(const gsk_sim* s) { return (size_t)(s->n / s->g.W) * s->F * s->n * s->g.W; }
@@ -300,7 +318,8 @@ inline PlainK plain_consts(const gsk_sim* s) {
   k.noise = c.noise;
   k.dtL = c.dt * c.L;
   k.dtLk = c.dt * c.L * c.kappa;
-  k.kfac = (c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx));
+  k.kfx = k.kfy = k.kfz = (c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx));
+  k.d = BoxDim{s->n, s->n, s->n};
   k.inv_n3 = 1.0 / ((double)s->n * s->n * s->n);
   k.V = s->V;
   k.step = (uint32_t)s->step;
@@ -320,7 +339,9 @@ inline PlainPK plain_prec_consts(const gsk_sim* s) {
   k.xi = c.xi;
   k.S = c.S;
   k.noise = c.noise;
-  k.kfac = (c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx));
+  k.kfx = k.kfy = k.kfz = (c.kconv == GSK_K_REFERENCE ? 2.0 * 3.14 * c.dx / s->n : 2.0 * kPi / (s->n * c.dx));
+  k.d = BoxDim{s->n, s->n, s->n};
+  k.celastic = 1;
   k.inv_n3 = 1.0 / ((double)s->n * s->n * s->n);
   k.V = s->V;
   k.step = (uint32_t)s->step;
@@ -380,6 +401,16 @@ template <int N> int impl_r2c(gsk_sim* s, const float* in, float2* kb, int F);
 template <int N> int impl_c2r(gsk_sim* s, float2* kb, float* out, int F);
 template <int N> int impl_zinv(gsk_sim* s, float* out, int F);
 
+// Box grids (gsk_box.hip)
+bool box_grid(const gsk_config& c);  // edges or spacings not all equal
+int box_create(gsk_sim* s);
+void box_free(BoxWS* w);
+int box_build_btensor(gsk_sim* s, const BArgs& a);
+int box_set_field(gsk_sim* s, int off_fields, size_t count, const double* host);
+int box_get_field(gsk_sim* s, int off_fields, size_t count, double* host);
+int box_step(gsk_sim* s);
+int box_energy(gsk_sim* s, double* gr, double* el);
+
 template <template <int> class Fn, typename... Args>
 inline int dispatch_n(int n, Args&&... args) {
   switch (n) {
@@ -427,6 +458,9 @@ template <int N> struct DC2R {
   static int run(gsk_sim* s, float2* kb, float* out, int F) { return impl_c2r<N>(s, kb, out, F); }
 };
 template <int N> struct DZinv { static int run(gsk_sim* s, float* out, int F) { return impl_zinv<N>(s, out, F); } };
+
+// one step of a handle: the per-size schedule of a cube, or the box path
+inline int step_once(gsk_sim* s) { return s->bx ? box_step(s) : dispatch_n<DStep>(s->n, s); }
 
 inline int ensure_device(int dev) {
   int cnt = 0;
@@ -591,6 +625,7 @@ inline int sync(gsk_sim* s) {
 
 inline int prepare(gsk_sim* s) {
   if (!s->haveB) return fail(GSK_ERR_STATE, "B tensor not built (call gsk_build_btensor first)");
+  if (s->bx) return GSK_OK;  // box grids carry no k-space state
   const bool stateless = s->cfg.rule == GSK_RULE_REFERENCE && s->cfg.physics == GSK_PHYS_MART;
   if (s->dirty && s->cfg.path == GSK_PATH_FUSED32 && !stateless) {
     s->sq_ready = false;  // the initialisation uses the k-space buffer

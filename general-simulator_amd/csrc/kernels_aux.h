@@ -17,9 +17,8 @@ struct BArgs {
   double kfac;
 };
 
-// B at the integer wave vector (mx, my, mz)
-__device__ __forceinline__ void btensor_m(int mx, int my, int mz, const BArgs& a, double* out) {
-  const double g0 = a.kfac * mx, g1 = a.kfac * my, g2 = a.kfac * mz;
+// B at the wave vector (g0, g1, g2)
+__device__ __forceinline__ void btensor_k(double g0, double g1, double g2, const BArgs& a, double* out) {
   const double mo = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
   double u[3] = {0.0, 0.0, 0.0};
   if (mo > 5e-7) { u[0] = g0 / mo; u[1] = g1 / mo; u[2] = g2 / mo; }
@@ -70,6 +69,11 @@ __device__ __forceinline__ void btensor_m(int mx, int my, int mz, const BArgs& a
   }
 }
 
+// B at the integer wave vector (mx, my, mz) of a cube: k = kfac m
+__device__ __forceinline__ void btensor_m(int mx, int my, int mz, const BArgs& a, double* out) {
+  btensor_k(a.kfac * mx, a.kfac * my, a.kfac * mz, a, out);
+}
+
 // B at FFT index (kx, ky, kz) with the reference's mapping m -> m - n for m >= n/2
 // (src/gtensorb.cu:9-12): the Nyquist index n/2 is taken as -n/2.
 __device__ __forceinline__ void btensor_at(int kx, int ky, int kz, int n, const BArgs& a, double* out) {
@@ -105,6 +109,22 @@ __global__ void k_btensor_full(double* __restrict__ B, int n, BArgs a) {
   }
 }
 
+// Box grid nx x ny x nz: B[pq][x][y][z] in T (FP64 evaluation), per-axis wave numbers
+// k_d = kf_d * (m_d, or m_d - n_d for m_d >= n_d / 2) (src/gtensorb.cu:14-24). npq = V1 * V2 entries,
+// or 1 (only B_00: the scalar tensor of the precipitate's c source).
+template <typename T>
+__global__ void k_btensor_box(T* __restrict__ B, int nx, int ny, int nz, double kfx, double kfy, double kfz, BArgs a,
+                              int npq) {
+  const long N3 = (long)nx * ny * nz;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(i / ((long)ny * nz)), y = (int)((i / nz) % ny), z = (int)(i % nz);
+    double out[64];
+    btensor_k(kfx * ((x < nx / 2) ? x : x - nx), kfy * ((y < ny / 2) ? y : y - ny), kfz * ((z < nz / 2) ? z : z - nz),
+              a, out);
+    for (int pq = 0; pq < npq; ++pq) B[pq * N3 + i] = (T)out[pq];
+  }
+}
+
 // Tile-ordered FP32 layout for the fused x pass, packed p <= q (V1 == V2).
 template <int = 0>
 __global__ void k_btensor_tiles(float* __restrict__ Bt, Geo g, BArgs a) {
@@ -127,21 +147,55 @@ __global__ void k_btensor_tiles(float* __restrict__ Bt, Geo g, BArgs a) {
   }
 }
 
-// ====================================================================== FP64 plain path
-// One kernel per reference operation; 3D C2C transforms as three axis passes.
-struct PlainK {
-  double A, B, C, two_wel, noise, dtL, dtLk, kfac, inv_n3;
+// ====================================================================== plain path
+// One kernel per reference operation on full complex (C2C) spectra, in T = double (the FP64 plain
+// path of cubes and boxes) or T = float (the FP32 path of box grids). Grids are nx x ny x nz with
+// fields stored [f][x][y][z]; 3D transforms are three axis passes. Wave numbers per axis:
+// k_d = kf_d * (m_d, or m_d - n_d for m_d >= n_d / 2).
+template <typename T> struct Cx;
+template <> struct Cx<float> { using T2 = float2; };
+template <> struct Cx<double> { using T2 = double2; };
+template <typename T> __device__ __forceinline__ typename Cx<T>::T2 cx(T a, T b) {
+  typename Cx<T>::T2 r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+
+struct BoxDim {
+  int nx, ny, nz;
+  __device__ __forceinline__ void coords(long v, int& x, int& y, int& z) const {
+    x = (int)(v / ((long)ny * nz));
+    y = (int)((v / nz) % ny);
+    z = (int)(v % nz);
+  }
+};
+
+template <typename T> struct PlainKT {
+  T A, B, C, two_wel, noise, dtL, dtLk, inv_n3;
+  T kfx, kfy, kfz;
+  BoxDim d;
   int V;
   uint32_t step;
   unsigned long long seed;
 };
+using PlainK = PlainKT<double>;
 
-// contiguous rows (z axis), L rows of N per tile
-template <int N, bool FWD>
-__global__ void __launch_bounds__(1024) p_zrows(double2* __restrict__ buf, int L, const double2* __restrict__ twN) {
-  extern __shared__ double2 psm[];
-  double2* tw = psm;
-  double2* tile = psm + N;
+// |k|^2 at voxel index v of the spectrum
+template <typename T, class K> __device__ __forceinline__ T ksq_at(const K& k, long v) {
+  int x, y, z;
+  k.d.coords(v, x, y, z);
+  const T kx = k.kfx * (T)((x < k.d.nx / 2) ? x : x - k.d.nx), ky = k.kfy * (T)((y < k.d.ny / 2) ? y : y - k.d.ny),
+          kz = k.kfz * (T)((z < k.d.nz / 2) ? z : z - k.d.nz);
+  return kx * kx + ky * ky + kz * kz;
+}
+
+// contiguous rows (z axis): L rows of N per tile, rows r0 .. r0 + L - 1 of the buffer
+template <typename T2, int N, bool FWD>
+__global__ void __launch_bounds__(1024) p_zrows(T2* __restrict__ buf, int L, const T2* __restrict__ twN) {
+  extern __shared__ __align__(16) unsigned char psm_raw[];
+  T2* tw = reinterpret_cast<T2*>(psm_raw);
+  T2* tile = tw + N;
   const long r0 = (long)blockIdx.x * L;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twN[i];
   for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
@@ -150,184 +204,193 @@ __global__ void __launch_bounds__(1024) p_zrows(double2* __restrict__ buf, int L
   }
   __syncthreads();
   Lines Ls{L, L, N + 1, 0, 1};
-  fft_lines<double2, N, FWD>(tile, Ls, tw);
+  fft_lines<T2, N, FWD>(tile, Ls, tw);
   for (int e = threadIdx.x; e < L * N; e += blockDim.x) {
     const int l = e / N, m = e % N;
     buf[(r0 + l) * N + m] = tile[l * (N + 1) + m];
   }
 }
 
-// strided lines: block -> (outer, chunk); element (pos, c) at outer*os + pos*ps + chunk*W + c
-template <int N, bool FWD>
-__global__ void __launch_bounds__(1024) p_strided(double2* __restrict__ buf, int W, long os, long ps,
-                                                 const double2* __restrict__ twN) {
-  extern __shared__ double2 psm[];
-  double2* tw = psm;
-  double2* tile = psm + N;
-  const int nch = N / W;
+// strided lines: block -> (outer, chunk of W columns out of nch chunks); element (pos, c) at
+// outer*os + pos*ps + chunk*W + c
+template <typename T2, int N, bool FWD>
+__global__ void __launch_bounds__(1024) p_strided(T2* __restrict__ buf, int W, int nch, long os, long ps,
+                                                 const T2* __restrict__ twN) {
+  extern __shared__ __align__(16) unsigned char psm_raw[];
+  T2* tw = reinterpret_cast<T2*>(psm_raw);
+  T2* tile = tw + N;
   const long o = blockIdx.x / nch;
   const int ch = blockIdx.x % nch;
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twN[i];
-  double2* base = buf + o * os + (long)ch * W;
+  T2* base = buf + o * os + (long)ch * W;
   for (int e = threadIdx.x; e < N * W; e += blockDim.x) tile[e] = base[(e / W) * ps + e % W];
   __syncthreads();
   Lines Ls{W, W, 1, 0, W};
-  fft_lines<double2, N, FWD>(tile, Ls, tw);
+  fft_lines<T2, N, FWD>(tile, Ls, tw);
   for (int e = threadIdx.x; e < N * W; e += blockDim.x) base[(e / W) * ps + e % W] = tile[e];
 }
 
-template <int = 0>
-__global__ void p_square(const double* __restrict__ eta, double2* __restrict__ S, long n) {
+template <typename T>
+__global__ void p_square(const T* __restrict__ eta, typename Cx<T>::T2* __restrict__ S, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const double e = eta[i];
-    S[i] = make_double2(e * e, 0.0);
+    const T e = eta[i];
+    S[i] = cx<T>(e * e, T(0));
   }
 }
 
-template <int = 0>
-__global__ void p_load(const double* __restrict__ eta, double2* __restrict__ E, long n) {
+template <typename T>
+__global__ void p_load(const T* __restrict__ eta, typename Cx<T>::T2* __restrict__ E, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    E[i] = make_double2(eta[i], 0.0);
+    E[i] = cx<T>(eta[i], T(0));
 }
 
-template <int = 0>
-__global__ void p_contract(const double2* __restrict__ S, double2* __restrict__ R, const double* __restrict__ B,
-                           long N3, PlainK k) {
+// R_p = sum_q B_pq S_q / N (B: [p][q][voxel])
+template <typename T>
+__global__ void p_contract(const typename Cx<T>::T2* __restrict__ S, typename Cx<T>::T2* __restrict__ R,
+                           const T* __restrict__ B, long N3, PlainKT<T> k) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
     for (int p = 0; p < k.V; ++p) {
-      double2 acc = make_double2(0.0, 0.0);
+      T ax = 0, ay = 0;
       for (int q = 0; q < k.V; ++q) {
-        const double b = B[(long)(p * k.V + q) * N3 + i];
-        const double2 s = S[q * N3 + i];
-        acc.x += b * s.x;
-        acc.y += b * s.y;
+        const T b = B[(long)(p * k.V + q) * N3 + i];
+        const typename Cx<T>::T2 s = S[q * N3 + i];
+        ax += b * s.x;
+        ay += b * s.y;
       }
-      R[p * N3 + i] = make_double2(acc.x * k.inv_n3, acc.y * k.inv_n3);
+      R[p * N3 + i] = cx<T>(ax * k.inv_n3, ay * k.inv_n3);
     }
   }
 }
 
-template <int = 0>
-__global__ void p_force(const double* __restrict__ eta, const double2* __restrict__ R, double2* __restrict__ Fh,
-                        long N3, PlainK k) {
+template <typename T>
+__global__ void p_force(const T* __restrict__ eta, const typename Cx<T>::T2* __restrict__ R,
+                        typename Cx<T>::T2* __restrict__ Fh, long N3, PlainKT<T> k) {
   for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
     for (int i = 0; i < 4; ++i) {
       const long v = g4 * 4 + i;
-      double s2 = 0.0;
+      T s2 = 0;
       for (int p = 0; p < k.V; ++p) s2 += eta[p * N3 + v] * eta[p * N3 + v];
       for (int p = 0; p < k.V; ++p) {
-        const double e = eta[p * N3 + v];
-        double f = k.A * e - k.B * e * e + k.C * e * s2 + k.two_wel * e * R[p * N3 + v].x;
-        Fh[p * N3 + v] = make_double2(f, 0.0);
+        const T e = eta[p * N3 + v];
+        T f = k.A * e - k.B * e * e + k.C * e * s2 + k.two_wel * e * R[p * N3 + v].x;
+        Fh[p * N3 + v] = cx<T>(f, T(0));
       }
     }
-    if (k.noise != 0.0) {
+    if (k.noise != T(0)) {
       for (int p = 0; p < k.V; ++p) {
         float xi[4];
         normal4(k.seed, (uint32_t)g4, (uint32_t)p, k.step, xi);
-        for (int i = 0; i < 4; ++i) Fh[p * N3 + g4 * 4 + i].x += k.noise * (double)xi[i];
+        for (int i = 0; i < 4; ++i) Fh[p * N3 + g4 * 4 + i].x += k.noise * (T)xi[i];
       }
     }
   }
 }
 
-template <int = 0>
-__global__ void p_update(double2* __restrict__ E, const double2* __restrict__ Fh, int n, PlainK k) {
-  const long N3 = (long)n * n * n;
+template <typename T>
+__global__ void p_update(typename Cx<T>::T2* __restrict__ E, const typename Cx<T>::T2* __restrict__ Fh, PlainKT<T> k) {
+  const long N3 = (long)k.d.nx * k.d.ny * k.d.nz;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < (long)k.V * N3; i += (long)gridDim.x * blockDim.x) {
-    const long v = i % N3;
-    const int x = (int)(v / ((long)n * n)), y = (int)((v / n) % n), z = (int)(v % n);
-    const double kx = k.kfac * ((x < n / 2) ? x : x - n), ky = k.kfac * ((y < n / 2) ? y : y - n),
-                 kz = k.kfac * ((z < n / 2) ? z : z - n);
-    const double den = 1.0 / (1.0 + k.dtLk * (kx * kx + ky * ky + kz * kz)) * k.inv_n3;
-    const double2 e = E[i], f = Fh[i];
-    E[i] = make_double2((e.x - k.dtL * f.x) * den, (e.y - k.dtL * f.y) * den);
+    const T den = T(1) / (T(1) + k.dtLk * ksq_at<T>(k, i % N3)) * k.inv_n3;
+    const typename Cx<T>::T2 e = E[i], f = Fh[i];
+    E[i] = cx<T>((e.x - k.dtL * f.x) * den, (e.y - k.dtL * f.y) * den);
   }
 }
 
-template <int = 0>
-__global__ void p_extract(const double2* __restrict__ E, double* __restrict__ eta, long n) {
+template <typename T>
+__global__ void p_extract(const typename Cx<T>::T2* __restrict__ E, T* __restrict__ eta, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) eta[i] = E[i].x;
 }
 
-// ---- precipitate, FP64 plain path: the spectral rule of the precipitate plugin
-// (src/dynamics_precipitate.cu:118-275 with the c-sourced elasticity and the stabilisation S),
-// one kernel per operation. Fields: [c, eta_1..eta_V], F = V + 1.
-struct PlainPK {
-  double A[8];
-  double c1, c2, dtM, dtMkc, dtL, dtLke, xi, S, noise, kfac, inv_n3;
+// ---- precipitate on the plain path (src/dynamics_precipitate.cu:118-275), one kernel per
+// operation. Fields: [c, eta_1..eta_V], F = V + 1. The spectral rule has the c-sourced elasticity
+// and the stabilisation S; the reference rule (Rel != null) adds xi 2 R_v eta_v to F_v
+// (src/dynamics_precipitate.cu:170-195, :249) and noise on c (Philox field 0x8000).
+template <typename T> struct PlainPKT {
+  T A[8];
+  T c1, c2, dtM, dtMkc, dtL, dtLke, xi, S, noise, inv_n3;
+  T cbar;  // c = cbar + the stored field 0 (the FP32 box path stores the fluctuation; 0 otherwise)
+  T kfx, kfy, kfz;
+  BoxDim d;
   int V;
+  int celastic;  // 1: the spectral rule's xi B(k) c_hat term in the concentration update
   uint32_t step;
   unsigned long long seed;
 };
+using PlainPK = PlainPKT<double>;
 
 // mu_c = A2/2 sum eta^2 + A1 (c - c1) (:118-127) and F_v (:136-162) + noise a xi_v (Philox field v)
 // of every voxel into Fh; the fields themselves into E (both as complex with zero imaginary part).
-template <int = 0>
-__global__ void p_prec_local(const double* __restrict__ fld, double2* __restrict__ Fh, double2* __restrict__ E,
-                             long N3, PlainPK k) {
+// Rel (reference rule, may be null): the elastic R_v of the eta^2 source as complex spectra whose
+// real parts hold R_v.
+template <typename T>
+__global__ void p_prec_local(const T* __restrict__ fld, typename Cx<T>::T2* __restrict__ Fh,
+                             typename Cx<T>::T2* __restrict__ E, long N3, PlainPKT<T> k,
+                             const typename Cx<T>::T2* __restrict__ Rel) {
   const int V = k.V;
   for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
     for (int i = 0; i < 4; ++i) {
       const long v = g4 * 4 + i;
-      const double c = fld[v];
-      double e[8], s2 = 0.0;
+      const T c = fld[v] + k.cbar;
+      T e[8], s2 = 0;
       for (int a = 0; a < V; ++a) {
         e[a] = fld[(1 + a) * N3 + v];
         s2 += e[a] * e[a];
       }
-      Fh[v] = make_double2(0.5 * k.A[2] * s2 + k.A[1] * (c - k.c1), 0.0);
-      E[v] = make_double2(c, 0.0);
+      Fh[v] = cx<T>(T(0.5) * k.A[2] * s2 + k.A[1] * (c - k.c1), T(0));
+      E[v] = cx<T>(fld[v], T(0));
       for (int a = 0; a < V; ++a) {
-        double t2 = 0.0, t4 = 0.0;
+        T t2 = 0, t4 = 0;
         for (int b = 0; b < V; ++b)
           if (b != a) {
-            const double q = e[b] * e[b];
+            const T q = e[b] * e[b];
             t2 += q;
             t4 += q * q;
           }
-        const double et = e[a], e2 = et * et;
+        const T et = e[a], e2 = et * et;
         // ordered pairs i != j, both != a: (sum q)^2 - sum q^2
-        const double f = (2.0 * k.A[5] * et + 4.0 * k.A[6] * e2 * et) * t2 + 2.0 * k.A[6] * et * t4 +
-                         2.0 * k.A[7] * et * (t2 * t2 - t4) + k.A[2] * et * (c - k.c2) - k.A[3] * e2 * et +
-                         k.A[4] * e2 * e2 * et;
-        Fh[(1 + a) * N3 + v] = make_double2(f, 0.0);
-        E[(1 + a) * N3 + v] = make_double2(et, 0.0);
+        T f = (T(2) * k.A[5] * et + T(4) * k.A[6] * e2 * et) * t2 + T(2) * k.A[6] * et * t4 +
+              T(2) * k.A[7] * et * (t2 * t2 - t4) + k.A[2] * et * (c - k.c2) - k.A[3] * e2 * et + k.A[4] * e2 * e2 * et;
+        if (Rel) f += k.xi * T(2) * Rel[a * N3 + v].x * et;
+        Fh[(1 + a) * N3 + v] = cx<T>(f, T(0));
+        E[(1 + a) * N3 + v] = cx<T>(et, T(0));
       }
     }
-    if (k.noise != 0.0) {
+    if (k.noise != T(0)) {
       for (int a = 0; a < V; ++a) {
         float xi[4];
         normal4(k.seed, (uint32_t)g4, (uint32_t)a, k.step, xi);
-        for (int i = 0; i < 4; ++i) Fh[(1 + a) * N3 + g4 * 4 + i].x += k.noise * (double)xi[i];
+        for (int i = 0; i < 4; ++i) Fh[(1 + a) * N3 + g4 * 4 + i].x += k.noise * (T)xi[i];
+      }
+      if (Rel) {
+        float xi[4];
+        normal4(k.seed, (uint32_t)g4, 0x8000u, k.step, xi);
+        for (int i = 0; i < 4; ++i) Fh[g4 * 4 + i].x += k.noise * (T)xi[i];
       }
     }
   }
 }
 
-// Semi-implicit updates of the transformed fields (with the 1/n^3 of the inverse transform):
+// Semi-implicit updates of the transformed fields (with the 1/N of the inverse transform):
 //   c_hat   <- (c_hat - dt M k^2 (mu_hat + (xi B(k) - S) c_hat)) / (1 + dt M (S k^2 + kappa_c k^4))
 //   eta_hat <- (eta_hat - dt L F_hat) / (1 + dt L kappa_eta k^2)
-template <int = 0>
-__global__ void p_prec_update(double2* __restrict__ E, const double2* __restrict__ Fh, const double* __restrict__ Bc,
-                              int n, PlainPK k) {
-  const long N3 = (long)n * n * n;
+// (celastic = 0: no xi B(k) term; the reference rule also has S = 0)
+template <typename T>
+__global__ void p_prec_update(typename Cx<T>::T2* __restrict__ E, const typename Cx<T>::T2* __restrict__ Fh,
+                              const T* __restrict__ Bc, PlainPKT<T> k) {
+  const long N3 = (long)k.d.nx * k.d.ny * k.d.nz;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < (long)(k.V + 1) * N3;
        i += (long)gridDim.x * blockDim.x) {
     const long v = i % N3;
-    const int x = (int)(v / ((long)n * n)), y = (int)((v / n) % n), z = (int)(v % n);
-    const double kx = k.kfac * ((x < n / 2) ? x : x - n), ky = k.kfac * ((y < n / 2) ? y : y - n),
-                 kz = k.kfac * ((z < n / 2) ? z : z - n);
-    const double k2 = kx * kx + ky * ky + kz * kz;
-    const double2 e = E[i], f = Fh[i];
+    const T k2 = ksq_at<T>(k, v);
+    const typename Cx<T>::T2 e = E[i], f = Fh[i];
     if (i < N3) {
-      const double a = k.xi * Bc[v] - k.S;
-      const double mx = f.x + a * e.x, my = f.y + a * e.y;
-      const double den = k.inv_n3 / (1.0 + k.dtM * k.S * k2 + k.dtMkc * k2 * k2);
-      E[i] = make_double2((e.x - k.dtM * k2 * mx) * den, (e.y - k.dtM * k2 * my) * den);
+      const T a = (k.celastic ? k.xi * Bc[v] : T(0)) - k.S;
+      const T mx = f.x + a * e.x, my = f.y + a * e.y;
+      const T den = k.inv_n3 / (T(1) + k.dtM * k.S * k2 + k.dtMkc * k2 * k2);
+      E[i] = cx<T>((e.x - k.dtM * k2 * mx) * den, (e.y - k.dtM * k2 * my) * den);
     } else {
-      const double den = k.inv_n3 / (1.0 + k.dtLke * k2);
-      E[i] = make_double2((e.x - k.dtL * f.x) * den, (e.y - k.dtL * f.y) * den);
+      const T den = k.inv_n3 / (T(1) + k.dtLke * k2);
+      E[i] = cx<T>((e.x - k.dtL * f.x) * den, (e.y - k.dtL * f.y) * den);
     }
   }
 }
@@ -568,24 +631,23 @@ __global__ void __launch_bounds__(xy_threads<N>()) r_xenergy_mart_r(float2* __re
   if (threadIdx.x == 0) part[blockIdx.x] = rr;
 }
 
-// Plain-path energy pieces on full complex spectra (FP64): gradient sum k^2 |E|^2, elastic B_pq Re(S_p* S_q).
-template <int = 0>
-__global__ void r_plain_kspace(const double2* __restrict__ E, const double2* __restrict__ S,
-                               const double* __restrict__ B, int n, int V, double kfac, double* __restrict__ part) {
+// Plain-path energy pieces on full complex spectra: gradient sum k^2 |E|^2, elastic B_pq Re(S_p* S_q)
+// (FP64 sums of T values).
+template <typename T>
+__global__ void r_plain_kspace(const typename Cx<T>::T2* __restrict__ E, const typename Cx<T>::T2* __restrict__ S,
+                               const T* __restrict__ B, PlainKT<T> k, double* __restrict__ part) {
   __shared__ double sh[16];
-  const long N3 = (long)n * n * n;
+  const long N3 = (long)k.d.nx * k.d.ny * k.d.nz;
+  const int V = k.V;
   double gr = 0.0, el = 0.0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
-    const int x = (int)(i / ((long)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
-    const double kx = kfac * ((x < n / 2) ? x : x - n), ky = kfac * ((y < n / 2) ? y : y - n),
-                 kz = kfac * ((z < n / 2) ? z : z - n);
-    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double k2 = (double)ksq_at<T>(k, i);
     for (int p = 0; p < V; ++p) {
-      const double2 e = E[p * N3 + i];
-      gr += k2 * (e.x * e.x + e.y * e.y);
+      const typename Cx<T>::T2 e = E[p * N3 + i];
+      gr += k2 * ((double)e.x * e.x + (double)e.y * e.y);
       for (int q = 0; q < V; ++q) {
-        const double2 sp = S[p * N3 + i], sq = S[q * N3 + i];
-        el += B[(long)(p * V + q) * N3 + i] * (sp.x * sq.x + sp.y * sq.y);
+        const typename Cx<T>::T2 sp = S[p * N3 + i], sq = S[q * N3 + i];
+        el += (double)B[(long)(p * V + q) * N3 + i] * ((double)sp.x * sq.x + (double)sp.y * sq.y);
       }
     }
   }
@@ -595,26 +657,23 @@ __global__ void r_plain_kspace(const double2* __restrict__ E, const double2* __r
   if (threadIdx.x == 0) part[blockIdx.x * 2 + 1] = r;
 }
 
-// Precipitate, FP64 plain path: sum_k k^2 |c_hat|^2, sum_k k^2 sum_v |eta_hat_v|^2 and
+// Precipitate, plain path: sum_k k^2 |c_hat|^2, sum_k k^2 sum_v |eta_hat_v|^2 and
 // sum_k B(k) |c_hat|^2 over the full transformed fields E = [c, eta_1..eta_V].
-template <int = 0>
-__global__ void r_plain_kspace_prec(const double2* __restrict__ E, const double* __restrict__ Bc, int n, int V,
-                                    double kfac, double* __restrict__ part) {
+template <typename T>
+__global__ void r_plain_kspace_prec(const typename Cx<T>::T2* __restrict__ E, const T* __restrict__ Bc,
+                                    PlainPKT<T> k, double* __restrict__ part) {
   __shared__ double sh[16];
-  const long N3 = (long)n * n * n;
+  const long N3 = (long)k.d.nx * k.d.ny * k.d.nz;
   double gc = 0.0, ge = 0.0, el = 0.0;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
-    const int x = (int)(i / ((long)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
-    const double kx = kfac * ((x < n / 2) ? x : x - n), ky = kfac * ((y < n / 2) ? y : y - n),
-                 kz = kfac * ((z < n / 2) ? z : z - n);
-    const double k2 = kx * kx + ky * ky + kz * kz;
-    const double2 c = E[i];
-    const double a2 = c.x * c.x + c.y * c.y;
+    const double k2 = (double)ksq_at<T>(k, i);
+    const typename Cx<T>::T2 c = E[i];
+    const double a2 = (double)c.x * c.x + (double)c.y * c.y;
     gc += k2 * a2;
-    el += Bc[i] * a2;
-    for (int v = 1; v <= V; ++v) {
-      const double2 e = E[v * N3 + i];
-      ge += k2 * (e.x * e.x + e.y * e.y);
+    if (Bc) el += (double)Bc[i] * a2;
+    for (int v = 1; v <= k.V; ++v) {
+      const typename Cx<T>::T2 e = E[v * N3 + i];
+      ge += k2 * ((double)e.x * e.x + (double)e.y * e.y);
     }
   }
   double r = block_sum(gc, sh);

@@ -1,6 +1,6 @@
 // Reference rule of the martensite plugin (src/dynamics_mart.cu:181-417), FP32 like the reference:
 //   eta += dt (w_grad Grad + Chem + El + Dis + w_noise xi), then eta *= (1 - defect)
-//   Grad = sum_d (eta(x+e_d) + eta(x-e_d) - 2 eta(x)) / (2 dx) / 3          (:181-188, periodic)
+//   Grad = sum_d (eta(x+e_d) + eta(x-e_d) - 2 eta(x)) / (2 Delta_d) / 3     (:181-188, periodic)
 //   Chem = HEAD side: s = sqrt|a1/(a2-a3)|; eta >= 0: -w_chem (s eta)(a1 - a2 (s eta)^2 + a3 sum_i (s eta_i)^2),
 //          eta < 0: 1000 w_chem a2 eta^2;  master side: -w_chem eta (a1 - a2 eta + a3 sum_i eta_i)   (:222-250)
 //   El   = -w_el R_p eta_p with R_p = IFFT[sum_q B_pq FFT(eta_q^2)]/N over all q (HEAD, :270-289)
@@ -13,30 +13,33 @@ namespace gsk {
 
 struct RefMartK {
   float a1, a2, a3, sc;  // sc = sqrt|a1/(a2-a3)|
-  float wG, wC, wE, wD, wN, dt, g;  // g = 1/(2 dx)/3
+  float wG, wC, wE, wD, wN, dt;
+  float gx, gy, gz;  // 1/(2 Delta_d)/3 per axis
   int V, chem_form;
   uint32_t step;
   const uint32_t* stepp;
   unsigned long long seed;
 };
 
-// One thread per 4 consecutive z voxels; eta: V fields, RE: V fields holding R on entry and
+// One thread per 4 consecutive z voxels of an nx x ny x nz grid (nz a multiple of 4); eta: V
+// fields, RE: V fields holding R on entry and
 // the new eta on exit (each element is read before it is overwritten by the same thread).
 // fpart (may be null): per-block FP64 sums of the four force terms of this step, w_grad Grad,
 // Chem, El, Dis, over every voxel and variant (the reference's GradientForce, ChemicalForce,
 // ElasticForce, DislocationForce arrays that Get averages, src/dynamics_mart.cu:443-456).
 template <int = 0>
 __global__ void k_ref_mart(const float* __restrict__ eta, float* __restrict__ RE, const float* __restrict__ defect,
-                           const float* __restrict__ K, int n, RefMartK k, double* __restrict__ fpart) {
+                           const float* __restrict__ K, int nx, int ny, int nz, RefMartK k,
+                           double* __restrict__ fpart) {
   double fs[4] = {0.0, 0.0, 0.0, 0.0};
-  const long N = (long)n * n * n, nq = N / 4;
-  const long nn = (long)n * n;
+  const long N = (long)nx * ny * nz, nq = N / 4;
+  const long nn = (long)ny * nz;
   const uint32_t step = step_of(k.step, k.stepp);
   for (long qd = blockIdx.x * (long)blockDim.x + threadIdx.x; qd < nq; qd += (long)gridDim.x * blockDim.x) {
     const long i0 = qd * 4;
-    const int x = (int)(i0 / nn), y = (int)((i0 / n) % n), z0 = (int)(i0 % n);
-    const long dxp = ((x + 1) % n - x) * nn, dxm = ((x + n - 1) % n - x) * nn;
-    const long dyp = (long)((y + 1) % n - y) * n, dym = (long)((y + n - 1) % n - y) * n;
+    const int x = (int)(i0 / nn), y = (int)((i0 / nz) % ny), z0 = (int)(i0 % nz);
+    const long dxp = ((x + 1) % nx - x) * nn, dxm = ((x + nx - 1) % nx - x) * nn;
+    const long dyp = (long)((y + 1) % ny - y) * nz, dym = (long)((y + ny - 1) % ny - y) * nz;
     float S2[4] = {0.f, 0.f, 0.f, 0.f}, S1[4] = {0.f, 0.f, 0.f, 0.f};
     for (int u = 0; u < k.V; ++u)
       for (int j = 0; j < 4; ++j) {
@@ -51,10 +54,10 @@ __global__ void k_ref_mart(const float* __restrict__ eta, float* __restrict__ RE
       for (int j = 0; j < 4; ++j) {
         const long i = i0 + j;
         const int z = z0 + j;
-        const long zp = i + ((z + 1) % n - z), zm = i + ((z + n - 1) % n - z);
+        const long zp = i + ((z + 1) % nz - z), zm = i + ((z + nz - 1) % nz - z);
         const float e = ev[i];
-        const float grad = (ev[i + dxp] + ev[i + dxm] - 2.f * e) * k.g + (ev[i + dyp] + ev[i + dym] - 2.f * e) * k.g +
-                           (ev[zp] + ev[zm] - 2.f * e) * k.g;
+        const float grad = (ev[i + dxp] + ev[i + dxm] - 2.f * e) * k.gx + (ev[i + dyp] + ev[i + dym] - 2.f * e) * k.gy +
+                           (ev[zp] + ev[zm] - 2.f * e) * k.gz;
         float chem;
         if (k.chem_form == 0)
           chem = e >= 0.f ? -k.wC * (k.sc * e) * (k.a1 - k.a2 * (k.sc * e) * (k.sc * e) + k.a3 * S2[j])

@@ -49,6 +49,7 @@ class Config(ctypes.Structure):
         ("kappa_c", ctypes.c_double), ("xi", ctypes.c_double), ("S", ctypes.c_double),
         ("rule", ctypes.c_int), ("chem_form", ctypes.c_int), ("w_grad", ctypes.c_double), ("w_chem", ctypes.c_double),
         ("w_dis", ctypes.c_double), ("temperature", ctypes.c_double), ("T0", ctypes.c_double),
+        ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
     ]
 
 
@@ -180,6 +181,8 @@ class Sim:
             h = _borrowed
         self._h = h
         self.n, self.nvar, self.physics = cfg.n, cfg.nvar, cfg.physics
+        # grid edges (nx, ny, nz): the box grid's, or n, n, n
+        self.dims = (cfg.n, cfg.ny or cfg.n, cfg.nz or cfg.n)
 
     def close(self) -> None:
         if self._h and self._own:
@@ -205,12 +208,12 @@ class Sim:
         _check(lib().gsk_build_btensor(self._h, _ptr(C), _ptr(eps), eps.shape[0]))
 
     def _shape(self, name: str):
-        n = self.n
+        d = self.dims
         if name == "eta":
-            return (self.nvar, n, n, n)
+            return (self.nvar,) + d
         if name == "dislocationstress":
-            return (6, n, n, n)
-        return (n, n, n)
+            return (6,) + d
+        return d
 
     def set_field(self, name: str, a: np.ndarray) -> None:
         a = np.asarray(a)

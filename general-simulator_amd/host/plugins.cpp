@@ -129,20 +129,23 @@ class GpuDynamics : public Dynamics {
       nx = std::stoi(w[0]); ny = std::stoi(w[1]); nz = std::stoi(w[2]);
       if (w.size() >= 6) { dx = std::stod(w[3]); dy = std::stod(w[4]); dz = std::stod(w[5]); }
     }
-    if (nx != ny || ny != nz || nx <= 0) {
-      log_line("Error: gridsize must describe a periodic n x n x n grid");
-      return Code_ERR;
-    }
-    if (dx != dy || dy != dz) {
-      log_line("Error: gridsize needs equal spacings dx = dy = dz");
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !(dx > 0) || !(dy > 0) || !(dz > 0)) {
+      log_line("Error: gridsize needs positive edges and spacings");
       return Code_ERR;
     }
     return Code_NORMAL;
   }
 
+  bool cube() const { return nx == ny && ny == nz && dx == dy && dy == dz; }
+
   void read_common() {
+    // a box grid (edges or spacings not all equal) runs the library's box path
     cfg_.n = nx;
+    cfg_.ny = ny;
+    cfg_.nz = nz;
     cfg_.dx = dx;
+    cfg_.dy = dy;
+    cfg_.dz = dz;
     double v;
     if (var_get(*Vars, "noiseamp", &v, 1)) cfg_.noise = v;
     unsigned long long seed;
@@ -443,6 +446,10 @@ class Dynamics_stress : public GpuDynamics {
  public:
   int Initialize() override {
     if (read_grid()) return Code_ERR;
+    if (!cube()) {
+      log_line("Error: the stress plugin takes a periodic n x n x n grid with one spacing");
+      return Code_ERR;
+    }
     gsk_config_default(&cfg_, GSK_PHYS_MART);
     read_common();
     var_get(*Vars, "xi", &xi_, 1);  // default 4000 (:25)

@@ -64,7 +64,7 @@ extern "C" {
 typedef struct gsk_sim gsk_sim;
 
 typedef struct gsk_config {
-  int n;                 /* grid edge (periodic n^3): 16, 32, 64, 96, 128, 192, 256 or 512  */
+  int n;                 /* grid edge nx (periodic): 16, 32, 64, 96, 128, 192, 256 or 512 */
   int nvar;              /* number of order parameters eta_v                              */
   int physics;           /* GSK_PHYS_*                                                    */
   int path;              /* GSK_PATH_*                                                    */
@@ -96,6 +96,15 @@ typedef struct gsk_config {
   int chem_form;         /* martensite reference chemical force: 0 HEAD side, 1 master side */
   double w_grad, w_chem, w_dis; /* martensite reference weights                           */
   double temperature, T0;       /* martensite reference: a1 = 0.02 (temperature - T0)       */
+  /* Box grids: the reference's `gridsize nx ny nz dx dy dz` (src/dynamics_mart.cu:18) with
+   * nx = n and dx as above. 0 means "as n" (ny, nz) or "as dx" (dy, dz). A grid whose edges or
+   * spacings are not all equal runs the box path: one kernel per operation on full complex
+   * spectra, each axis transformed at its own length (any of the supported edges), FP32 on
+   * GSK_PATH_FUSED32 (both rules) and FP64 on GSK_PATH_PLAIN64 (spectral rules); wave numbers
+   * per axis (src/gtensorb.cu:14-24), difference quotients per axis (src/dynamics_mart.cu:181-188).
+   * Ensembles, the adjoint, the stress transform and per-kernel timing take cubes only. */
+  int ny, nz;
+  double dy, dz;
 } gsk_config;
 
 #define GSK_RULE_SPECTRAL 0

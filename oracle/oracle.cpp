@@ -128,51 +128,76 @@ extern "C" void or_fft1d(int n, double* data, int sign) {
   pl.run(reinterpret_cast<cd*>(data), s.data(), 1);
 }
 
-static void fft3d(int n, cd* a, int sign) {
-  const FftPlan pl(n, sign);
-  const int64_t n2 = (int64_t)n * n;
-  const int B = n < 16 ? n : 16;  // lines per batch (n is a multiple of 16 or smaller than 16 here)
-  const int nb = (n + B - 1) / B;
+// A periodic grid of nx x ny x nz voxels with spacings dx, dy, dz, fields stored [x][y][z] (z
+// fastest, include/datamain.th:70-76); the reference reads it from `gridsize nx ny nz dx dy dz`
+// (src/dynamics_mart.cu:18). The cube entry points use nx = ny = nz = n and one spacing.
+struct Grid {
+  int nx, ny, nz;
+  double dx, dy, dz;
+  int64_t N;
+  void coords(int64_t i, int& x, int& y, int& z) const {
+    x = (int)(i / ((int64_t)ny * nz));
+    y = (int)((i / nz) % ny);
+    z = (int)(i % nz);
+  }
+};
+static Grid cube_grid(int n, double dx) { return Grid{n, n, n, dx, dx, dx, (int64_t)n * n * n}; }
+static Grid box_grid(const or_grid* g) {
+  return Grid{g->nx, g->ny, g->nz, g->dx, g->dy, g->dz, (int64_t)g->nx * g->ny * g->nz};
+}
+
+// 3D transform as 1D transforms along z, y and x, each on batches of up to 16 lines (the result
+// does not depend on the batching: every line is transformed on its own).
+static void fft3d(const Grid& g, cd* a, int sign) {
+  const FftPlan px(g.nx, sign), py(g.ny, sign), pz(g.nz, sign);
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int64_t sx = (int64_t)ny * nz, sy = nz;
+  const int B = 16;
+  const int nby = (ny + B - 1) / B, nbz = (nz + B - 1) / B;
+  const int L = std::max(nx, std::max(ny, nz));
 #pragma omp parallel
   {
-    std::vector<cd> buf((size_t)n * B), s((size_t)n * B);
+    std::vector<cd> buf((size_t)L * B), s((size_t)L * B);
     // z lines (contiguous): B lines of one (x, y-block) gathered transposed
 #pragma omp for schedule(static)
-    for (int64_t t = 0; t < (int64_t)n * nb; ++t) {
-      const int64_t x = t / nb, y0 = (t % nb) * B;
-      const int Bt = (int)std::min<int64_t>(B, n - y0);
+    for (int64_t t = 0; t < (int64_t)nx * nby; ++t) {
+      const int64_t x = t / nby, y0 = (t % nby) * B;
+      const int Bt = (int)std::min<int64_t>(B, ny - y0);
       for (int b = 0; b < Bt; ++b)
-        for (int z = 0; z < n; ++z) buf[(size_t)z * Bt + b] = a[x * n2 + (y0 + b) * n + z];
-      pl.run(buf.data(), s.data(), Bt);
+        for (int z = 0; z < nz; ++z) buf[(size_t)z * Bt + b] = a[x * sx + (y0 + b) * sy + z];
+      pz.run(buf.data(), s.data(), Bt);
       for (int b = 0; b < Bt; ++b)
-        for (int z = 0; z < n; ++z) a[x * n2 + (y0 + b) * n + z] = buf[(size_t)z * Bt + b];
+        for (int z = 0; z < nz; ++z) a[x * sx + (y0 + b) * sy + z] = buf[(size_t)z * Bt + b];
     }
     // y lines: for each x and block of B z values, element (y, b) = a[x][y][z0 + b]
 #pragma omp for schedule(static)
-    for (int64_t t = 0; t < (int64_t)n * nb; ++t) {
-      const int64_t x = t / nb, z0 = (t % nb) * B;
-      const int Bt = (int)std::min<int64_t>(B, n - z0);
-      for (int y = 0; y < n; ++y)
-        for (int b = 0; b < Bt; ++b) buf[(size_t)y * Bt + b] = a[x * n2 + (int64_t)y * n + z0 + b];
-      pl.run(buf.data(), s.data(), Bt);
-      for (int y = 0; y < n; ++y)
-        for (int b = 0; b < Bt; ++b) a[x * n2 + (int64_t)y * n + z0 + b] = buf[(size_t)y * Bt + b];
+    for (int64_t t = 0; t < (int64_t)nx * nbz; ++t) {
+      const int64_t x = t / nbz, z0 = (t % nbz) * B;
+      const int Bt = (int)std::min<int64_t>(B, nz - z0);
+      for (int y = 0; y < ny; ++y)
+        for (int b = 0; b < Bt; ++b) buf[(size_t)y * Bt + b] = a[x * sx + (int64_t)y * sy + z0 + b];
+      py.run(buf.data(), s.data(), Bt);
+      for (int y = 0; y < ny; ++y)
+        for (int b = 0; b < Bt; ++b) a[x * sx + (int64_t)y * sy + z0 + b] = buf[(size_t)y * Bt + b];
     }
     // x lines: for each y and block of B z values, element (x, b) = a[x][y][z0 + b]
 #pragma omp for schedule(static)
-    for (int64_t t = 0; t < (int64_t)n * nb; ++t) {
-      const int64_t y = t / nb, z0 = (t % nb) * B;
-      const int Bt = (int)std::min<int64_t>(B, n - z0);
-      for (int x = 0; x < n; ++x)
-        for (int b = 0; b < Bt; ++b) buf[(size_t)x * Bt + b] = a[(int64_t)x * n2 + y * n + z0 + b];
-      pl.run(buf.data(), s.data(), Bt);
-      for (int x = 0; x < n; ++x)
-        for (int b = 0; b < Bt; ++b) a[(int64_t)x * n2 + y * n + z0 + b] = buf[(size_t)x * Bt + b];
+    for (int64_t t = 0; t < (int64_t)ny * nbz; ++t) {
+      const int64_t y = t / nbz, z0 = (t % nbz) * B;
+      const int Bt = (int)std::min<int64_t>(B, nz - z0);
+      for (int x = 0; x < nx; ++x)
+        for (int b = 0; b < Bt; ++b) buf[(size_t)x * Bt + b] = a[(int64_t)x * sx + y * sy + z0 + b];
+      px.run(buf.data(), s.data(), Bt);
+      for (int x = 0; x < nx; ++x)
+        for (int b = 0; b < Bt; ++b) a[(int64_t)x * sx + y * sy + z0 + b] = buf[(size_t)x * Bt + b];
     }
   }
 }
 
-extern "C" void or_fft3d(int n, double* data, int sign) { fft3d(n, reinterpret_cast<cd*>(data), sign); }
+extern "C" void or_fft3d(int n, double* data, int sign) { fft3d(cube_grid(n, 1.0), reinterpret_cast<cd*>(data), sign); }
+extern "C" void or_fft3d_box(const or_grid* g, double* data, int sign) {
+  fft3d(box_grid(g), reinterpret_cast<cd*>(data), sign);
+}
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of
@@ -339,9 +364,11 @@ static void inv3(const double a[9], double o[9]) {
   o[8] = (a[0] * a[4] - a[1] * a[3]) / det;
 }
 
-extern "C" void or_btensor(int n, double dx, int kconv, const double C[81], const double* eps1, int V1,
-                           const double* eps2, int V2, double* B) {
-  const int64_t N = (int64_t)n * n * n;
+// B over a grid: the per-axis wave numbers of GTensorB_VF with each axis's own n and spacing
+// (src/gtensorb.cu:14-24), the unit vector and the tensor of src/gtensorb.cu:110-182.
+static void btensor(const Grid& g, int kconv, const double C[81], const double* eps1, int V1, const double* eps2,
+                    int V2, double* B) {
+  const int64_t N = g.N;
   // sigma = C : eps  (src/gtensorb.cu:72-84)
   std::vector<double> s1(V1 * 9, 0.0), s2(V2 * 9, 0.0), cee(V1 * V2, 0.0);
   for (int p = 0; p < V1; ++p)
@@ -363,17 +390,18 @@ extern "C" void or_btensor(int n, double dx, int kconv, const double C[81], cons
     }
 #pragma omp parallel for schedule(static)
   for (int64_t idx = 0; idx < N; ++idx) {
-    int x = (int)(idx / ((int64_t)n * n)), y = (int)((idx / n) % n), z = (int)(idx % n);
-    double g[3] = {or_kcomp(x, n, dx, kconv), or_kcomp(y, n, dx, kconv), or_kcomp(z, n, dx, kconv)};
-    double mo = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    int x, y, z;
+    g.coords(idx, x, y, z);
+    double k[3] = {or_kcomp(x, g.nx, g.dx, kconv), or_kcomp(y, g.ny, g.dy, kconv), or_kcomp(z, g.nz, g.dz, kconv)};
+    double mo = std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]);
     double u[3] = {0, 0, 0};
-    if (mo > 5e-7) { u[0] = g[0] / mo; u[1] = g[1] / mo; u[2] = g[2] / mo; }  // src/gtensorb.cu:25-33
+    if (mo > 5e-7) { u[0] = k[0] / mo; u[1] = k[1] / mo; u[2] = k[2] / mo; }  // src/gtensorb.cu:25-33
     double om_inv[9], om[9];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {
         double acc = 0;
-        for (int k = 0; k < 3; ++k)
-          for (int l = 0; l < 3; ++l) acc += C[((i * 3 + k) * 3 + l) * 3 + j] * u[k] * u[l];  // :127-130
+        for (int kk = 0; kk < 3; ++kk)
+          for (int l = 0; l < 3; ++l) acc += C[((i * 3 + kk) * 3 + l) * 3 + j] * u[kk] * u[l];  // :127-130
         om_inv[i * 3 + j] = acc;
       }
     inv3(om_inv, om);
@@ -382,14 +410,24 @@ extern "C" void or_btensor(int n, double dx, int kconv, const double C[81], cons
       for (int j = 0; j < 3; ++j) a[j] = u[0] * s1[p * 9 + j] + u[1] * s1[p * 9 + 3 + j] + u[2] * s1[p * 9 + 6 + j];
       for (int q = 0; q < V2; ++q) {
         double b[3];  // b_k = sigma^q_kl n_l
-        for (int k = 0; k < 3; ++k) b[k] = s2[q * 9 + k * 3] * u[0] + s2[q * 9 + k * 3 + 1] * u[1] + s2[q * 9 + k * 3 + 2] * u[2];
+        for (int kk = 0; kk < 3; ++kk)
+          b[kk] = s2[q * 9 + kk * 3] * u[0] + s2[q * 9 + kk * 3 + 1] * u[1] + s2[q * 9 + kk * 3 + 2] * u[2];
         double t = 0;
         for (int j = 0; j < 3; ++j)
-          for (int k = 0; k < 3; ++k) t += a[j] * om[j * 3 + k] * b[k];
+          for (int kk = 0; kk < 3; ++kk) t += a[j] * om[j * 3 + kk] * b[kk];
         B[(int64_t)(p * V2 + q) * N + idx] = cee[p * V2 + q] - t;  // :168-178
       }
     }
   }
+}
+
+extern "C" void or_btensor(int n, double dx, int kconv, const double C[81], const double* eps1, int V1,
+                           const double* eps2, int V2, double* B) {
+  btensor(cube_grid(n, dx), kconv, C, eps1, V1, eps2, V2, B);
+}
+extern "C" void or_btensor_box(const or_grid* g, int kconv, const double C[81], const double* eps1, int V1,
+                               const double* eps2, int V2, double* B) {
+  btensor(box_grid(g), kconv, C, eps1, V1, eps2, V2, B);
 }
 
 // ---------------------------------------------------------------------------
@@ -400,15 +438,28 @@ static std::vector<double> ksq_table(int n, double dx, int kconv) {
   for (int m = 0; m < n; ++m) { double v = or_kcomp(m, n, dx, kconv); k[m] = v * v; }
   return k;
 }
+// |k|^2 of every voxel index of a grid (each axis with its own n and spacing)
+struct KSq {
+  std::vector<double> x, y, z;
+  const Grid* g;
+  KSq(const Grid& gr, int kconv)
+      : x(ksq_table(gr.nx, gr.dx, kconv)), y(ksq_table(gr.ny, gr.dy, kconv)), z(ksq_table(gr.nz, gr.dz, kconv)),
+        g(&gr) {}
+  double operator()(int64_t i) const {
+    int a, b, c;
+    g->coords(i, a, b, c);
+    return x[a] + y[b] + z[c];
+  }
+};
 
 // R_p = IFFT[ sum_q B_pq FFT(eta_q^2) ] / N   (src/dynamics_mart.cu:283-289, HEAD: all q)
-static void elastic_R(int n, int V, const double* B, const double* eta, std::vector<cd>& R) {
-  const int64_t N = (int64_t)n * n * n;
+static void elastic_R(const Grid& g, int V, const double* B, const double* eta, std::vector<cd>& R) {
+  const int64_t N = g.N;
   std::vector<cd> S((size_t)V * N);
   for (int q = 0; q < V; ++q) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) { double e = eta[q * N + i]; S[q * N + i] = cd(e * e, 0.0); }
-    fft3d(n, S.data() + q * N, -1);
+    fft3d(g, S.data() + q * N, -1);
   }
   R.assign((size_t)V * N, cd(0, 0));
   for (int p = 0; p < V; ++p) {
@@ -418,14 +469,14 @@ static void elastic_R(int n, int V, const double* B, const double* eta, std::vec
       for (int q = 0; q < V; ++q) acc += B[(int64_t)(p * V + q) * N + i] * S[q * N + i];
       R[p * N + i] = acc / (double)N;
     }
-    fft3d(n, R.data() + p * N, +1);
+    fft3d(g, R.data() + p * N, +1);
   }
 }
 
-extern "C" void or_mart_force(int n, int V, const or_mart_params* p, const double* B, const double* eta, double* F) {
-  const int64_t N = (int64_t)n * n * n;
+static void mart_force(const Grid& g, int V, const or_mart_params* p, const double* B, const double* eta, double* F) {
+  const int64_t N = g.N;
   std::vector<cd> R;
-  elastic_R(n, V, B, eta, R);
+  elastic_R(g, V, B, eta, R);
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < N; ++i) {
     double s2 = 0;
@@ -436,15 +487,19 @@ extern "C" void or_mart_force(int n, int V, const or_mart_params* p, const doubl
     }
   }
 }
+extern "C" void or_mart_force(int n, int V, const or_mart_params* p, const double* B, const double* eta, double* F) {
+  mart_force(cube_grid(n, p->dx), V, p, B, eta, F);
+}
 
-extern "C" void or_mart_run(int n, int V, const or_mart_params* p, const double* B, double* eta, int step0,
-                            int nsteps) {
-  const int64_t N = (int64_t)n * n * n;
-  std::vector<double> F((size_t)V * N), k2 = ksq_table(n, p->dx, p->kconv);
+static void mart_run(const Grid& g, int V, const or_mart_params* p, const double* B, double* eta, int step0,
+                     int nsteps) {
+  const int64_t N = g.N;
+  std::vector<double> F((size_t)V * N);
+  const KSq kk(g, p->kconv);
   std::vector<float> xi(N);
   std::vector<cd> eh(N), fh(N);
   for (int s = 0; s < nsteps; ++s) {
-    or_mart_force(n, V, p, B, eta, F.data());
+    mart_force(g, V, p, B, eta, F.data());
     for (int v = 0; v < V; ++v) {
       if (p->noise != 0.0) {
         or_noise_field(p->seed, (uint32_t)(step0 + s), (uint32_t)v, N, xi.data());
@@ -453,19 +508,24 @@ extern "C" void or_mart_run(int n, int V, const or_mart_params* p, const double*
       }
 #pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < N; ++i) { eh[i] = cd(eta[v * N + i], 0); fh[i] = cd(F[v * N + i], 0); }
-      fft3d(n, eh.data(), -1);
-      fft3d(n, fh.data(), -1);
+      fft3d(g, eh.data(), -1);
+      fft3d(g, fh.data(), -1);
 #pragma omp parallel for schedule(static)
-      for (int64_t i = 0; i < N; ++i) {
-        int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
-        double kk = k2[x] + k2[y] + k2[z];
-        eh[i] = (eh[i] - p->dt * p->L * fh[i]) / (1.0 + p->dt * p->L * p->kappa * kk) / (double)N;
-      }
-      fft3d(n, eh.data(), +1);
+      for (int64_t i = 0; i < N; ++i)
+        eh[i] = (eh[i] - p->dt * p->L * fh[i]) / (1.0 + p->dt * p->L * p->kappa * kk(i)) / (double)N;
+      fft3d(g, eh.data(), +1);
 #pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < N; ++i) eta[v * N + i] = eh[i].real();
     }
   }
+}
+extern "C" void or_mart_run(int n, int V, const or_mart_params* p, const double* B, double* eta, int step0,
+                            int nsteps) {
+  mart_run(cube_grid(n, p->dx), V, p, B, eta, step0, nsteps);
+}
+extern "C" void or_mart_run_box(const or_grid* g, int V, const or_mart_params* p, const double* B, double* eta,
+                                int step0, int nsteps) {
+  mart_run(box_grid(g), V, p, B, eta, step0, nsteps);
 }
 
 // Store-all adjoint of or_mart_run (FP64 reference gradient for the GPU adjoint; the
@@ -481,12 +541,13 @@ extern "C" void or_mart_run(int n, int V, const or_mart_params* p, const double*
 // g_par = {ds_1..ds_V, dL, dkappa}.
 extern "C" void or_mart_adjoint(int n, int V, const or_mart_params* p, const double* B, const double* eta0, int step0,
                                 int nsteps, const double* gT, double* g_eta0, double* g_par) {
-  const int64_t N = (int64_t)n * n * n, VN = (int64_t)V * N;
+  const Grid g = cube_grid(n, p->dx);
+  const int64_t N = g.N, VN = (int64_t)V * N;
   std::vector<double> traj((size_t)(nsteps + 1) * VN);
   std::memcpy(traj.data(), eta0, VN * sizeof(double));
   for (int s = 0; s < nsteps; ++s) {
     std::memcpy(traj.data() + (s + 1) * VN, traj.data() + s * VN, VN * sizeof(double));
-    or_mart_run(n, V, p, B, traj.data() + (s + 1) * VN, step0 + s, 1);
+    mart_run(g, V, p, B, traj.data() + (s + 1) * VN, step0 + s, 1);
   }
   const std::vector<double> k2 = ksq_table(n, p->dx, p->kconv);
   auto kk_of = [&](int64_t i) {
@@ -504,8 +565,8 @@ extern "C" void or_mart_adjoint(int n, int V, const or_mart_params* p, const dou
     double X = 0.0;  // (1/N) sum_k Re(conj(lam_hat) eta1_hat) k^2 D
     for (int v = 0; v < V; ++v) {
       for (int64_t i = 0; i < N; ++i) { a[i] = cd(lam[v * N + i], 0); b[i] = cd(e1[v * N + i], 0); }
-      fft3d(n, a.data(), -1);
-      fft3d(n, b.data(), -1);
+      fft3d(g, a.data(), -1);
+      fft3d(g, b.data(), -1);
       double x = 0.0;
 #pragma omp parallel for reduction(+ : x) schedule(static)
       for (int64_t i = 0; i < N; ++i) {
@@ -514,12 +575,12 @@ extern "C" void or_mart_adjoint(int n, int V, const or_mart_params* p, const dou
         a[i] *= D / (double)N;
       }
       X += x / (double)N;
-      fft3d(n, a.data(), +1);
+      fft3d(g, a.data(), +1);
       for (int64_t i = 0; i < N; ++i) mu[v * N + i] = a[i].real();
     }
     gL += -p->dt * p->kappa * X;
     gk += -dtL * X;
-    elastic_R(n, V, B, e, R);
+    elastic_R(g, V, B, e, R);
     // spectra of eta_q^2 and eta_r mu_r
     for (int q = 0; q < V; ++q) {
       for (int64_t i = 0; i < N; ++i) {
@@ -527,8 +588,8 @@ extern "C" void or_mart_adjoint(int n, int V, const or_mart_params* p, const dou
         Sh[q * N + i] = cd(ev * ev, 0);
         Ph[q * N + i] = cd(ev * mu[q * N + i], 0);
       }
-      fft3d(n, Sh.data() + q * N, -1);
-      fft3d(n, Ph.data() + q * N, -1);
+      fft3d(g, Sh.data() + q * N, -1);
+      fft3d(g, Ph.data() + q * N, -1);
     }
     for (int r = 0; r < V; ++r)
       for (int q = 0; q < V; ++q) {
@@ -546,7 +607,7 @@ extern "C" void or_mart_adjoint(int n, int V, const or_mart_params* p, const dou
         for (int r = 0; r < V; ++r) acc += B[(int64_t)(pp * V + r) * N + i] * Ph[r * N + i];
         Q[pp * N + i] = acc / (double)N;
       }
-      fft3d(n, Q.data() + pp * N, +1);
+      fft3d(g, Q.data() + pp * N, +1);
     }
     double muF = 0.0;
     for (int v = 0; v < V; ++v) {
@@ -580,12 +641,12 @@ extern "C" void or_mart_adjoint(int n, int V, const or_mart_params* p, const dou
   g_par[V + 1] = gk;
 }
 
-extern "C" void or_mart_energy(int n, int V, const or_mart_params* p, const double* B, const double* eta,
-                               double out[4]) {
-  const int64_t N = (int64_t)n * n * n;
+static void mart_energy(const Grid& g, int V, const or_mart_params* p, const double* B, const double* eta,
+                        double out[4]) {
+  const int64_t N = g.N;
   double ech = 0, egr = 0, eel = 0;
   std::vector<cd> R;
-  elastic_R(n, V, B, eta, R);
+  elastic_R(g, V, B, eta, R);
 #pragma omp parallel for reduction(+ : ech, eel) schedule(static)
   for (int64_t i = 0; i < N; ++i) {
     double s2 = 0, s3 = 0, el = 0;
@@ -597,20 +658,25 @@ extern "C" void or_mart_energy(int n, int V, const or_mart_params* p, const doub
     ech += 0.5 * p->A * s2 - p->B / 3.0 * s3 + 0.25 * p->C * s2 * s2;
     eel += 0.5 * p->w_el * el;
   }
-  std::vector<double> k2 = ksq_table(n, p->dx, p->kconv);
+  const KSq kk(g, p->kconv);
   std::vector<cd> eh(N);
   for (int v = 0; v < V; ++v) {
     for (int64_t i = 0; i < N; ++i) eh[i] = cd(eta[v * N + i], 0);
-    fft3d(n, eh.data(), -1);
+    fft3d(g, eh.data(), -1);
     double acc = 0;
 #pragma omp parallel for reduction(+ : acc) schedule(static)
-    for (int64_t i = 0; i < N; ++i) {
-      int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
-      acc += (k2[x] + k2[y] + k2[z]) * std::norm(eh[i]);
-    }
+    for (int64_t i = 0; i < N; ++i) acc += kk(i) * std::norm(eh[i]);
     egr += 0.5 * p->kappa * acc / (double)N;
   }
   out[0] = ech; out[1] = egr; out[2] = eel; out[3] = ech + egr + eel;
+}
+extern "C" void or_mart_energy(int n, int V, const or_mart_params* p, const double* B, const double* eta,
+                               double out[4]) {
+  mart_energy(cube_grid(n, p->dx), V, p, B, eta, out);
+}
+extern "C" void or_mart_energy_box(const or_grid* g, int V, const or_mart_params* p, const double* B,
+                                   const double* eta, double out[4]) {
+  mart_energy(box_grid(g), V, p, B, eta, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -652,10 +718,11 @@ extern "C" void or_prec_local(int V, const or_prec_params* p, double c, const do
   }
 }
 
-extern "C" void or_prec_run(int n, int V, const or_prec_params* p, const double* Bc, double* c, double* eta,
-                            int step0, int nsteps) {
-  const int64_t N = (int64_t)n * n * n;
-  std::vector<double> k2 = ksq_table(n, p->dx, p->kconv), mu(N), F((size_t)V * N);
+static void prec_run(const Grid& g, int V, const or_prec_params* p, const double* Bc, double* c, double* eta,
+                     int step0, int nsteps) {
+  const int64_t N = g.N;
+  const KSq kk(g, p->kconv);
+  std::vector<double> mu(N), F((size_t)V * N);
   std::vector<float> xi(N);
   std::vector<cd> ch(N), mh(N), eh(N), fh(N);
   for (int s = 0; s < nsteps; ++s) {
@@ -669,17 +736,16 @@ extern "C" void or_prec_run(int n, int V, const or_prec_params* p, const double*
     // concentration
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) { ch[i] = cd(c[i], 0); mh[i] = cd(mu[i], 0); }
-    fft3d(n, ch.data(), -1);
-    fft3d(n, mh.data(), -1);
+    fft3d(g, ch.data(), -1);
+    fft3d(g, mh.data(), -1);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) {
-      int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
-      double kk = k2[x] + k2[y] + k2[z];
+      const double k2 = kk(i);
       cd mt = mh[i] + (p->xi * Bc[i] - p->S) * ch[i];
-      ch[i] = (ch[i] - p->dt * p->M * kk * mt) / (1.0 + p->dt * p->M * (p->S * kk + p->kappa_c * kk * kk)) /
+      ch[i] = (ch[i] - p->dt * p->M * k2 * mt) / (1.0 + p->dt * p->M * (p->S * k2 + p->kappa_c * k2 * k2)) /
               (double)N;
     }
-    fft3d(n, ch.data(), +1);
+    fft3d(g, ch.data(), +1);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) c[i] = ch[i].real();
     // order parameters
@@ -691,19 +757,24 @@ extern "C" void or_prec_run(int n, int V, const or_prec_params* p, const double*
       }
 #pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < N; ++i) { eh[i] = cd(eta[v * N + i], 0); fh[i] = cd(F[v * N + i], 0); }
-      fft3d(n, eh.data(), -1);
-      fft3d(n, fh.data(), -1);
+      fft3d(g, eh.data(), -1);
+      fft3d(g, fh.data(), -1);
 #pragma omp parallel for schedule(static)
-      for (int64_t i = 0; i < N; ++i) {
-        int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
-        double kk = k2[x] + k2[y] + k2[z];
-        eh[i] = (eh[i] - p->dt * p->L * fh[i]) / (1.0 + p->dt * p->L * p->kappa_eta * kk) / (double)N;
-      }
-      fft3d(n, eh.data(), +1);
+      for (int64_t i = 0; i < N; ++i)
+        eh[i] = (eh[i] - p->dt * p->L * fh[i]) / (1.0 + p->dt * p->L * p->kappa_eta * kk(i)) / (double)N;
+      fft3d(g, eh.data(), +1);
 #pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < N; ++i) eta[v * N + i] = eh[i].real();
     }
   }
+}
+extern "C" void or_prec_run(int n, int V, const or_prec_params* p, const double* Bc, double* c, double* eta,
+                            int step0, int nsteps) {
+  prec_run(cube_grid(n, p->dx), V, p, Bc, c, eta, step0, nsteps);
+}
+extern "C" void or_prec_run_box(const or_grid* g, int V, const or_prec_params* p, const double* Bc, double* c,
+                                double* eta, int step0, int nsteps) {
+  prec_run(box_grid(g), V, p, Bc, c, eta, step0, nsteps);
 }
 
 // ---------------------------------------------------------------------------
@@ -723,25 +794,28 @@ extern "C" void or_prec_run(int n, int V, const or_prec_params* p, const double*
 //     p->noise on both fields (the reference's 1e-4 weightnoise, :203, :244): eta_v uses
 //     Philox field v, c uses field 0x8000.
 // ---------------------------------------------------------------------------
-extern "C" void or_mart_ref_run(int n, int V, const or_mart_ref_params* p, const double* B, double* eta,
-                                const double* defect, const double* K, int step0, int nsteps) {
-  const int64_t N = (int64_t)n * n * n, VN = (int64_t)V * N;
+static void mart_ref_run(const Grid& g, int V, const or_mart_ref_params* p, const double* B, double* eta,
+                         const double* defect, const double* K, int step0, int nsteps) {
+  const int64_t N = g.N, VN = (int64_t)V * N;
   std::vector<double> out((size_t)VN);
   std::vector<float> xi((size_t)VN, 0.0f);
   std::vector<cd> R;
-  const double g = 1.0 / (2.0 * p->dx) / 3.0;
+  // 1 / (2 Delta_d) / 3 per axis (src/dynamics_mart.cu:181-188)
+  const double gx = 1.0 / (2.0 * g.dx) / 3.0, gy = 1.0 / (2.0 * g.dy) / 3.0, gz = 1.0 / (2.0 * g.dz) / 3.0;
   const double sc = std::sqrt(std::fabs(p->a1 / (p->a2 - p->a3)));
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
   for (int s = 0; s < nsteps; ++s) {
-    elastic_R(n, V, B, eta, R);
+    elastic_R(g, V, B, eta, R);
     if (p->w_noise != 0.0)
       for (int v = 0; v < V; ++v) or_noise_field(p->seed, (uint32_t)(step0 + s), (uint32_t)v, N, xi.data() + v * N);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) {
-      const int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
-      const int64_t ox = (int64_t)n * n, oy = n;
-      const int64_t xp = i + ((x + 1) % n - x) * ox, xm = i + ((x + n - 1) % n - x) * ox;
-      const int64_t yp = i + ((y + 1) % n - y) * oy, ym = i + ((y + n - 1) % n - y) * oy;
-      const int64_t zp = i + ((z + 1) % n - z), zm = i + ((z + n - 1) % n - z);
+      int x, y, z;
+      g.coords(i, x, y, z);
+      const int64_t ox = (int64_t)ny * nz, oy = nz;
+      const int64_t xp = i + ((x + 1) % nx - x) * ox, xm = i + ((x + nx - 1) % nx - x) * ox;
+      const int64_t yp = i + ((y + 1) % ny - y) * oy, ym = i + ((y + ny - 1) % ny - y) * oy;
+      const int64_t zp = i + ((z + 1) % nz - z), zm = i + ((z + nz - 1) % nz - z);
       double S2 = 0.0, S1 = 0.0;
       for (int u = 0; u < V; ++u) {
         const double e = eta[u * N + i];
@@ -751,8 +825,8 @@ extern "C" void or_mart_ref_run(int n, int V, const or_mart_ref_params* p, const
       for (int v = 0; v < V; ++v) {
         const double* ev = eta + v * N;
         const double e = ev[i];
-        const double grad = (ev[xp] + ev[xm] - 2.0 * e) * g + (ev[yp] + ev[ym] - 2.0 * e) * g +
-                            (ev[zp] + ev[zm] - 2.0 * e) * g;
+        const double grad = (ev[xp] + ev[xm] - 2.0 * e) * gx + (ev[yp] + ev[ym] - 2.0 * e) * gy +
+                            (ev[zp] + ev[zm] - 2.0 * e) * gz;
         double chem;
         if (p->chem_form == 0)
           chem = e >= 0.0 ? -p->w_chem * (sc * e) * (p->a1 - p->a2 * (sc * e) * (sc * e) + p->a3 * S2)
@@ -769,15 +843,24 @@ extern "C" void or_mart_ref_run(int n, int V, const or_mart_ref_params* p, const
     std::memcpy(eta, out.data(), VN * sizeof(double));
   }
 }
+extern "C" void or_mart_ref_run(int n, int V, const or_mart_ref_params* p, const double* B, double* eta,
+                                const double* defect, const double* K, int step0, int nsteps) {
+  mart_ref_run(cube_grid(n, p->dx), V, p, B, eta, defect, K, step0, nsteps);
+}
+extern "C" void or_mart_ref_run_box(const or_grid* g, int V, const or_mart_ref_params* p, const double* B,
+                                    double* eta, const double* defect, const double* K, int step0, int nsteps) {
+  mart_ref_run(box_grid(g), V, p, B, eta, defect, K, step0, nsteps);
+}
 
-extern "C" void or_prec_ref_run(int n, int V, const or_prec_params* p, const double* B, double* c, double* eta,
-                                int step0, int nsteps) {
-  const int64_t N = (int64_t)n * n * n;
-  std::vector<double> k2 = ksq_table(n, p->dx, p->kconv), mu(N), F((size_t)V * N);
+static void prec_ref_run(const Grid& g, int V, const or_prec_params* p, const double* B, double* c, double* eta,
+                         int step0, int nsteps) {
+  const int64_t N = g.N;
+  const KSq kk(g, p->kconv);
+  std::vector<double> mu(N), F((size_t)V * N);
   std::vector<float> xi(N);
   std::vector<cd> ch(N), mh(N), eh(N), fh(N), R;
   for (int s = 0; s < nsteps; ++s) {
-    elastic_R(n, V, B, eta, R);
+    elastic_R(g, V, B, eta, R);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) {
       double e[16], Fl[16];
@@ -791,15 +874,14 @@ extern "C" void or_prec_ref_run(int n, int V, const or_prec_params* p, const dou
     }
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) { ch[i] = cd(c[i], 0); mh[i] = cd(mu[i], 0); }
-    fft3d(n, ch.data(), -1);
-    fft3d(n, mh.data(), -1);
+    fft3d(g, ch.data(), -1);
+    fft3d(g, mh.data(), -1);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) {
-      int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
-      double kk = k2[x] + k2[y] + k2[z];
-      ch[i] = (ch[i] - p->dt * p->M * kk * mh[i]) / (1.0 + p->dt * p->M * p->kappa_c * kk * kk) / (double)N;
+      const double k2 = kk(i);
+      ch[i] = (ch[i] - p->dt * p->M * k2 * mh[i]) / (1.0 + p->dt * p->M * p->kappa_c * k2 * k2) / (double)N;
     }
-    fft3d(n, ch.data(), +1);
+    fft3d(g, ch.data(), +1);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < N; ++i) c[i] = ch[i].real();
     for (int v = 0; v < V; ++v) {
@@ -810,33 +892,39 @@ extern "C" void or_prec_ref_run(int n, int V, const or_prec_params* p, const dou
       }
 #pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < N; ++i) { eh[i] = cd(eta[v * N + i], 0); fh[i] = cd(F[v * N + i], 0); }
-      fft3d(n, eh.data(), -1);
-      fft3d(n, fh.data(), -1);
+      fft3d(g, eh.data(), -1);
+      fft3d(g, fh.data(), -1);
 #pragma omp parallel for schedule(static)
-      for (int64_t i = 0; i < N; ++i) {
-        int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
-        double kk = k2[x] + k2[y] + k2[z];
-        eh[i] = (eh[i] - p->dt * p->L * fh[i]) / (1.0 + p->dt * p->L * p->kappa_eta * kk) / (double)N;
-      }
-      fft3d(n, eh.data(), +1);
+      for (int64_t i = 0; i < N; ++i)
+        eh[i] = (eh[i] - p->dt * p->L * fh[i]) / (1.0 + p->dt * p->L * p->kappa_eta * kk(i)) / (double)N;
+      fft3d(g, eh.data(), +1);
 #pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < N; ++i) eta[v * N + i] = eh[i].real();
     }
   }
+}
+extern "C" void or_prec_ref_run(int n, int V, const or_prec_params* p, const double* B, double* c, double* eta,
+                                int step0, int nsteps) {
+  prec_ref_run(cube_grid(n, p->dx), V, p, B, c, eta, step0, nsteps);
+}
+extern "C" void or_prec_ref_run_box(const or_grid* g, int V, const or_prec_params* p, const double* B, double* c,
+                                    double* eta, int step0, int nsteps) {
+  prec_ref_run(box_grid(g), V, p, B, c, eta, step0, nsteps);
 }
 
 // Stress plugin (src/dynamics_stress.cu:126-185): stress_b = -xi IFFT[sum_v B_bv FFT(s_v)]/N for
 // the 6 base strains b, s_v = eta_v^2 (source 2) or eta_v (source 1); defect = any |eta_v| > 0.9.
 extern "C" void or_stress(int n, int V, const double* B, double xi, int source, const double* eta, double* stress,
                           double* defect) {
-  const int64_t N = (int64_t)n * n * n;
+  const Grid g = cube_grid(n, 1.0);
+  const int64_t N = g.N;
   std::vector<cd> S((size_t)V * N), R(N);
   for (int v = 0; v < V; ++v) {
     for (int64_t i = 0; i < N; ++i) {
       const double e = eta[v * N + i];
       S[v * N + i] = cd(source == 2 ? e * e : e, 0.0);
     }
-    fft3d(n, S.data() + v * N, -1);
+    fft3d(g, S.data() + v * N, -1);
   }
   for (int b = 0; b < 6; ++b) {
 #pragma omp parallel for schedule(static)
@@ -845,7 +933,7 @@ extern "C" void or_stress(int n, int V, const double* B, double xi, int source, 
       for (int v = 0; v < V; ++v) acc += B[(int64_t)(b * V + v) * N + i] * S[v * N + i];
       R[i] = acc / (double)N;
     }
-    fft3d(n, R.data(), +1);
+    fft3d(g, R.data(), +1);
     for (int64_t i = 0; i < N; ++i) stress[b * N + i] = -xi * R[i].real();
   }
   if (defect)
@@ -857,9 +945,9 @@ extern "C" void or_stress(int n, int V, const double* B, double xi, int source, 
     }
 }
 
-extern "C" void or_prec_energy(int n, int V, const or_prec_params* p, const double* Bc, const double* c,
-                               const double* eta, double out[4]) {
-  const int64_t N = (int64_t)n * n * n;
+static void prec_energy(const Grid& g, int V, const or_prec_params* p, const double* Bc, const double* c,
+                        const double* eta, double out[4]) {
+  const int64_t N = g.N;
   double ech = 0;
 #pragma omp parallel for reduction(+ : ech) schedule(static)
   for (int64_t i = 0; i < N; ++i) {
@@ -868,25 +956,32 @@ extern "C" void or_prec_energy(int n, int V, const or_prec_params* p, const doub
     or_prec_local(V, p, c[i], e, nullptr, nullptr, &f);
     ech += f;
   }
-  std::vector<double> k2 = ksq_table(n, p->dx, p->kconv);
+  const KSq kk(g, p->kconv);
   std::vector<cd> h(N);
   double egr = 0, eel = 0;
   for (int v = -1; v < V; ++v) {
     const double* src = (v < 0) ? c : eta + (int64_t)v * N;
     for (int64_t i = 0; i < N; ++i) h[i] = cd(src[i], 0);
-    fft3d(n, h.data(), -1);
+    fft3d(g, h.data(), -1);
     double acc = 0, accel = 0;
 #pragma omp parallel for reduction(+ : acc, accel) schedule(static)
     for (int64_t i = 0; i < N; ++i) {
-      int x = (int)(i / ((int64_t)n * n)), y = (int)((i / n) % n), z = (int)(i % n);
-      double kk = k2[x] + k2[y] + k2[z], a2 = std::norm(h[i]);
-      acc += kk * a2;
+      const double a2 = std::norm(h[i]);
+      acc += kk(i) * a2;
       if (v < 0) accel += Bc[i] * a2;
     }
     egr += 0.5 * (v < 0 ? p->kappa_c : p->kappa_eta) * acc / (double)N;
     if (v < 0) eel = 0.5 * p->xi * accel / (double)N;
   }
   out[0] = ech; out[1] = egr; out[2] = eel; out[3] = ech + egr + eel;
+}
+extern "C" void or_prec_energy(int n, int V, const or_prec_params* p, const double* Bc, const double* c,
+                               const double* eta, double out[4]) {
+  prec_energy(cube_grid(n, p->dx), V, p, Bc, c, eta, out);
+}
+extern "C" void or_prec_energy_box(const or_grid* g, int V, const or_prec_params* p, const double* Bc,
+                                   const double* c, const double* eta, double out[4]) {
+  prec_energy(box_grid(g), V, p, Bc, c, eta, out);
 }
 
 extern "C" double or_sum(const double* a, int64_t n) {

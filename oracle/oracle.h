@@ -73,6 +73,30 @@ void or_prec_ref_run(int n, int V, const or_prec_params* p, const double* B, dou
 void or_stress(int n, int V, const double* B, double xi, int source, const double* eta, double* stress,
                double* defect);
 
+/* Box grids: nx x ny x nz voxels with spacings dx, dy, dz (the reference's `gridsize nx ny nz dx dy
+ * dz`, src/dynamics_mart.cu:18; per-axis wave numbers src/gtensorb.cu:14-24, per-axis difference
+ * quotients src/dynamics_mart.cu:181-188). The _box entry points take the grid in place of n and
+ * use its spacings in place of the params' dx. */
+typedef struct or_grid {
+  int nx, ny, nz;
+  double dx, dy, dz;
+} or_grid;
+void or_fft3d_box(const or_grid* g, double* data, int sign);
+void or_btensor_box(const or_grid* g, int kconv, const double C[81], const double* eps1, int V1,
+                    const double* eps2, int V2, double* B);
+void or_mart_run_box(const or_grid* g, int V, const or_mart_params* p, const double* B, double* eta, int step0,
+                     int nsteps);
+void or_mart_energy_box(const or_grid* g, int V, const or_mart_params* p, const double* B, const double* eta,
+                        double out[4]);
+void or_mart_ref_run_box(const or_grid* g, int V, const or_mart_ref_params* p, const double* B, double* eta,
+                         const double* defect, const double* K, int step0, int nsteps);
+void or_prec_run_box(const or_grid* g, int V, const or_prec_params* p, const double* Bc, double* c, double* eta,
+                     int step0, int nsteps);
+void or_prec_energy_box(const or_grid* g, int V, const or_prec_params* p, const double* Bc, const double* c,
+                        const double* eta, double out[4]);
+void or_prec_ref_run_box(const or_grid* g, int V, const or_prec_params* p, const double* B, double* c,
+                         double* eta, int step0, int nsteps);
+
 void or_set_threads(int nt);
 int or_get_threads(void);
 

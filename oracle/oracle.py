@@ -39,6 +39,16 @@ class PrecParams(ctypes.Structure):
                 ("seed", ctypes.c_uint64)]
 
 
+class Grid(ctypes.Structure):
+    """A box grid nx x ny x nz with spacings dx, dy, dz (or_grid)."""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double)]
+
+
+def grid(nx: int, ny: int, nz: int, dx: float = 1.0, dy: float = 1.0, dz: float = 1.0) -> Grid:
+    return Grid(nx, ny, nz, dx, dy, dz)
+
+
 _lib = None
 
 
@@ -79,6 +89,15 @@ def lib():
         L.or_prec_run.argtypes = [I, I, ctypes.POINTER(PrecParams), P, P, P, I, I]
         L.or_prec_energy.argtypes = [I, I, ctypes.POINTER(PrecParams), P, P, P, P]
         L.or_prec_local.argtypes = [I, ctypes.POINTER(PrecParams), D, P, P, P, P]
+        G = ctypes.POINTER(Grid)
+        L.or_fft3d_box.argtypes = [G, P, I]
+        L.or_btensor_box.argtypes = [G, I, P, P, I, P, I, P]
+        L.or_mart_run_box.argtypes = [G, I, ctypes.POINTER(MartParams), P, P, I, I]
+        L.or_mart_energy_box.argtypes = [G, I, ctypes.POINTER(MartParams), P, P, P]
+        L.or_mart_ref_run_box.argtypes = [G, I, ctypes.POINTER(MartRefParams), P, P, P, P, I, I]
+        L.or_prec_run_box.argtypes = [G, I, ctypes.POINTER(PrecParams), P, P, P, I, I]
+        L.or_prec_energy_box.argtypes = [G, I, ctypes.POINTER(PrecParams), P, P, P, P]
+        L.or_prec_ref_run_box.argtypes = [G, I, ctypes.POINTER(PrecParams), P, P, P, I, I]
         L.or_sum.argtypes = [P, I64]
         L.or_sum.restype = D
         _lib = L
@@ -286,3 +305,76 @@ def prec_local(c: float, eta: np.ndarray, p: PrecParams):
     f = ctypes.c_double()
     lib().or_prec_local(e.size, ctypes.byref(p), c, _p(e), ctypes.byref(mu), _p(F), ctypes.byref(f))
     return mu.value, F, f.value
+
+
+# ----------------------------------------------------------------------------- box grids
+# Fields on a box grid g have shape (..., g.nx, g.ny, g.nz); the params' dx is not used (the
+# grid carries dx, dy, dz).
+def _dims(g: Grid):
+    return (g.nx, g.ny, g.nz)
+
+
+def fft3d_box(x: np.ndarray, g: Grid, sign: int = -1) -> np.ndarray:
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    assert a.shape == _dims(g)
+    lib().or_fft3d_box(ctypes.byref(g), _p(a), sign)
+    return a
+
+
+def btensor_box(g: Grid, C: np.ndarray, eps1: np.ndarray, eps2=None, kconv: int = 0) -> np.ndarray:
+    C = np.ascontiguousarray(C, dtype=np.float64).reshape(81)
+    e1 = np.ascontiguousarray(eps1, dtype=np.float64).reshape(-1, 9)
+    e2 = e1 if eps2 is None else np.ascontiguousarray(eps2, dtype=np.float64).reshape(-1, 9)
+    B = np.empty((e1.shape[0], e2.shape[0]) + _dims(g))
+    lib().or_btensor_box(ctypes.byref(g), kconv, _p(C), _p(e1), e1.shape[0], _p(e2), e2.shape[0], _p(B))
+    return B
+
+
+def mart_run_box(eta, B, p: MartParams, g: Grid, nsteps: int, step0: int = 0) -> np.ndarray:
+    e = np.ascontiguousarray(eta, dtype=np.float64).copy()
+    assert e.shape[1:] == _dims(g)
+    lib().or_mart_run_box(ctypes.byref(g), e.shape[0], ctypes.byref(p), _p(np.ascontiguousarray(B, dtype=np.float64)),
+                          _p(e), step0, nsteps)
+    return e
+
+
+def mart_energy_box(eta, B, p: MartParams, g: Grid) -> np.ndarray:
+    e = np.ascontiguousarray(eta, dtype=np.float64)
+    out = np.zeros(4)
+    lib().or_mart_energy_box(ctypes.byref(g), e.shape[0], ctypes.byref(p), _p(np.ascontiguousarray(B, dtype=np.float64)),
+                             _p(e), _p(out))
+    return out
+
+
+def mart_ref_run_box(eta, B, p: MartRefParams, g: Grid, nsteps: int, step0: int = 0, defect=None, K=None):
+    e = np.ascontiguousarray(eta, dtype=np.float64).copy()
+    d = None if defect is None else np.ascontiguousarray(defect, dtype=np.float64)
+    k = None if K is None else np.ascontiguousarray(K, dtype=np.float64)
+    lib().or_mart_ref_run_box(ctypes.byref(g), e.shape[0], ctypes.byref(p), _p(np.ascontiguousarray(B, dtype=np.float64)),
+                              _p(e), None if d is None else _p(d), None if k is None else _p(k), step0, nsteps)
+    return e
+
+
+def prec_run_box(c, eta, Bc, p: PrecParams, g: Grid, nsteps: int, step0: int = 0):
+    cc = np.ascontiguousarray(c, dtype=np.float64).copy()
+    e = np.ascontiguousarray(eta, dtype=np.float64).copy()
+    lib().or_prec_run_box(ctypes.byref(g), e.shape[0], ctypes.byref(p), _p(np.ascontiguousarray(Bc, dtype=np.float64)),
+                          _p(cc), _p(e), step0, nsteps)
+    return cc, e
+
+
+def prec_energy_box(c, eta, Bc, p: PrecParams, g: Grid) -> np.ndarray:
+    cc = np.ascontiguousarray(c, dtype=np.float64)
+    e = np.ascontiguousarray(eta, dtype=np.float64)
+    out = np.zeros(4)
+    lib().or_prec_energy_box(ctypes.byref(g), e.shape[0], ctypes.byref(p), _p(np.ascontiguousarray(Bc, dtype=np.float64)),
+                             _p(cc), _p(e), _p(out))
+    return out
+
+
+def prec_ref_run_box(c, eta, B, p: PrecParams, g: Grid, nsteps: int, step0: int = 0):
+    cc = np.ascontiguousarray(c, dtype=np.float64).copy()
+    e = np.ascontiguousarray(eta, dtype=np.float64).copy()
+    lib().or_prec_ref_run_box(ctypes.byref(g), e.shape[0], ctypes.byref(p), _p(np.ascontiguousarray(B, dtype=np.float64)),
+                              _p(cc), _p(e), step0, nsteps)
+    return cc, e

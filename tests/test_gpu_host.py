@@ -216,3 +216,27 @@ def test_gsi_reference_mart_info_force_columns(gpu, tmp_path):
             s.step(1)
         want = s.mart_forces()
     np.testing.assert_allclose(vals[2:], want, rtol=2e-5, atol=1e-12)
+
+
+def test_gsi_box_grid_reference_mart_equals_library(gpu, tmp_path):
+    """`set gridsize nx ny nz dx dy dz` with unequal edges and spacings (src/dynamics_mart.cu:18):
+    gsi hands the box to the library's box path; the dump has the box's shape and equals the
+    library's own run bit for bit."""
+    say("gsi box grid reference martensite vs library")
+    dims, sp, V, steps = (32, 16, 64), (1.0, 0.5, 2.0), 3, 8
+    eps = gsk.mart_eigenstrains(V, base=0.0, axis=0.01)
+    script = (f"sys mart\nset gridsize {dims[0]} {dims[1]} {dims[2]} {sp[0]} {sp[1]} {sp[2]}\nset deltatime 0.005\n"
+              f"set coefficient 0 32.05 37.5\nset weightelastic 1000\nset weightnoise 0.0001\nset seed 6\n"
+              f"readhere varianttensor\n{tensor_lines(eps)}set initeta uniform 0 0.2 3\n"
+              f"set binary on\ndump out 1 eta\nrun {steps}\n")
+    run_gsi(tmp_path, script)
+    got = read_binary(tmp_path / "out" / f"eta.{steps}.data")
+    assert got.shape == (V,) + dims
+    cfg = gsk.reference_config(gsk.PHYS_MART, n=dims[0], ny=dims[1], nz=dims[2], dx=sp[0], dy=sp[1], dz=sp[2],
+                               nvar=V, dt=0.005, B=32.05, C=37.5, w_el=1000.0, noise=1e-4, seed=6)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(gsk.cubic_modulus(3.5, 1.5, 1.0), eps)
+        s.init_uniform("eta", 0.0, 0.2, 3)
+        s.step(steps)
+        want = s.get_field("eta")
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))

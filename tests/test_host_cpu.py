@@ -82,10 +82,19 @@ def test_missing_varianttensor_aborts_run(tmp_path):
     assert rc == 1 and "strain tensor does not set" in out
 
 
-def test_non_cubic_grid_rejected(tmp_path):
-    rc, out = run(tmp_path, "sys mart\nset gridsize 16 16 32 1 1 1\nreadhere varianttensor\n3 1 3 3\n"
-                            "1 0 0 0 1 0 0 0 1\nrun 1\n")
-    assert rc == 1 and "periodic n x n x n" in out
+def test_grid_validation(tmp_path):
+    """Box grids (`gridsize nx ny nz dx dy dz` with unequal edges or spacings, src/dynamics_mart.cu:18)
+    reach the library's box path; edges and spacings must be positive, and the stress plugin takes
+    cubes only (both checked by the host before any GPU work)."""
+    vt = "readhere varianttensor\n3 1 3 3\n1 0 0 0 1 0 0 0 1\n"
+    rc, out = run(tmp_path, "sys mart\nset gridsize 16 0 32 1 1 1\n" + vt + "run 1\n")
+    assert rc == 1 and "positive edges and spacings" in out
+    rc, out = run(tmp_path, "sys mart\nset gridsize 16 16 32 1 -1 1\n" + vt + "run 1\n")
+    assert rc == 1 and "positive edges and spacings" in out
+    rc, out = run(tmp_path, "sys stress\nset gridsize 16 16 32 1 1 1\n" + vt + "run\n")
+    assert rc == 1 and "stress plugin takes a periodic n x n x n grid" in out
+    rc, out = run(tmp_path, "sys stress\nset gridsize 16 16 16 1 1 0.5\n" + vt + "run\n")
+    assert rc == 1 and "stress plugin takes a periodic n x n x n grid" in out
 
 
 def test_control_flow_loop_while_if(tmp_path):
