@@ -222,6 +222,19 @@ def run_ensemble(a):
     seq = vox * a.ens_seq_steps / ts / 1e6
     finite = all(bool(__import__("numpy").isfinite(m.get_field("eta")).all()) for m in e.members)
     e.close()
+    # the same members one by one on the FP64 plain path (one kernel per operation, C2C transforms):
+    # each member's own handle, a warm-up step, then ens_plain_steps steps timed with HIP events
+    plain_ms = 0.0
+    if a.ens_plain_steps > 0:
+        C, eps = gsk.cubic_modulus(0.7, 0.3, 0.2), gsk.mart_eigenstrains(3)
+        for c in cfgs:
+            c.path = gsk.PATH_PLAIN64
+            with gsk.Sim(c) as m:
+                m.build_btensor(C, eps)
+                m.init_uniform("eta", -0.01, 0.01, c.seed)
+                m.step(1)
+                plain_ms += m.time_steps(a.ens_plain_steps) / a.ens_plain_steps
+        log(f"plain path one by one: {plain_ms:.2f} ms per ensemble step")
     names = {0: "sequential", 1: "streams", 2: "graph", 3: "batched", 4: "batched_graph"}
     ms = t / a.steps * 1e3
     result = {
@@ -236,6 +249,10 @@ def run_ensemble(a):
         "schedule": {"mode": names[best[0]], "branches": best[1], "steps_per_graph": best[2]},
         "sequential": {"value": round(seq, 2), "steps": a.ens_seq_steps},
         "speedup_vs_sequential": round(rate / seq, 3),
+        "plain_one_by_one": ({"value": round(vox / (plain_ms / 1e3) / 1e6, 2), "ms_per_step": round(plain_ms, 3),
+                              "steps": a.ens_plain_steps, "path": "FP64 plain, members one after another"}
+                             if plain_ms > 0 else None),
+        "speedup_vs_plain_one_by_one": round(plain_ms / ms, 3) if plain_ms > 0 else None,
         "autotune": [{"mode": names[int(r[0])], "branches": int(r[1]), "steps_per_graph": int(r[2]),
                       "ms_per_step": round(r[3], 4)} for r in table],
         "finite": finite,
@@ -294,6 +311,41 @@ def run_prec(a):
     print(json.dumps(result), flush=True)
 
 
+def run_box(a):
+    """A box grid (nx x ny x nz, spacings 1, 0.5, 2): the 3-variant spectral martensite step on the box
+    path, FP32 and FP64, inputs resident; one kernel per operation with C2C transforms per axis."""
+    import torch
+
+    import gsk
+
+    dims = tuple(a.box)
+    vox = dims[0] * dims[1] * dims[2]
+    out = {}
+    for path, name in ((gsk.PATH_FUSED32, "f32"), (gsk.PATH_PLAIN64, "f64")):
+        cfg = gsk.default_config(gsk.PHYS_MART, n=dims[0], ny=dims[1], nz=dims[2], dx=1.0, dy=0.5, dz=2.0, nvar=3,
+                                 path=path, device=0)
+        with gsk.Sim(cfg) as s:
+            s.build_btensor(gsk.cubic_modulus(0.7, 0.3, 0.2), gsk.mart_eigenstrains(3))
+            s.init_uniform("eta", -0.01, 0.01, 0)
+            s.step(2)
+            steps = max(2, a.steps // (4 if name == "f32" else 10))
+            wall, _, walls = timed_repeats(s, steps, a.repeats, torch)
+        ms = wall / steps
+        out[name] = {"value": round(vox / (ms / 1e3) / 1e6, 2), "ms_per_step": round(ms, 4), "steps": steps,
+                     "repeats": repeat_stats(walls, steps)}
+        log(f"box {dims} {name}: {ms:.3f} ms/step")
+    result = {
+        "metric": f"Mvoxel-steps/s, {dims[0]}x{dims[1]}x{dims[2]} box, 3-variant AC+elasticity, box path",
+        "value": out["f32"]["value"], "unit": "Mvoxel-steps/s", "n_gpus": 1, "steps": out["f32"]["steps"],
+        "warmup": 2, "ms_per_step": out["f32"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (Philox uniform initial fields, seed 0)",
+        "config": {"workload": f"martensite box {dims[0]}x{dims[1]}x{dims[2]}, spacings 1/0.5/2, 3 variants, "
+                               "box path (C2C per axis, one kernel per operation)", "parallelism": "single GPU"},
+        "box_f32": out["f32"], "box_f64": out["f64"],
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,12 +363,15 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs for roofline.traffic")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size once (counter calibration)")
-    ap.add_argument("--workload", choices=["mart256", "ensemble", "prec256"], default="mart256",
+    ap.add_argument("--workload", choices=["mart256", "ensemble", "prec256", "box"], default="mart256",
                     help="mart256: the headline 256^3 step; ensemble: 96 mixed-size instances; "
-                         "prec256: the precipitate step at 256^3")
+                         "prec256: the precipitate step at 256^3; box: a box grid on the box path")
+    ap.add_argument("--box", type=int, nargs=3, default=[256, 256, 128], help="box workload: nx ny nz")
     ap.add_argument("--ens-per-size", type=int, default=32)
     ap.add_argument("--ens-trial", type=int, default=5)
     ap.add_argument("--ens-seq-steps", type=int, default=20)
+    ap.add_argument("--ens-plain-steps", type=int, default=3,
+                    help="ensemble: steps per member timed on the FP64 plain path, one member after another (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: the library runs on one GPU "
@@ -328,6 +383,9 @@ def main():
         return
     if a.workload == "prec256":
         run_prec(a)
+        return
+    if a.workload == "box":
+        run_box(a)
         return
 
     import torch
