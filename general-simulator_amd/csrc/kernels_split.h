@@ -292,14 +292,14 @@ k_xcontract_split(float2* __restrict__ kbuf, const float* __restrict__ Bt, SGeo 
     {
       float2* fb = kbuf + PV * g.kfs;
       fft_l2g_p<float2, N, Plan>(
-          tile, Ls, tw, [&](int cc, int kx, float2 v) { at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = v; },
+          tile, Ls, tw, [&](int cc, int kx, float2 v) { at32(fb, s.coff(opaque(cc) & (W - 1)) + (unsigned)kx * s.ps) = v; },
           bo());
     }
 #pragma unroll
     for (int f = 0; f < PV; ++f) {
       float2* fb = kbuf + f * g.kfs;
       fft_r2g_p<float2, N, Plan>(
-          tile, Ls, tw, S[f], [&](int cc, int kx, float2 v) { at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = v; },
+          tile, Ls, tw, S[f], [&](int cc, int kx, float2 v) { at32(fb, s.coff(opaque(cc) & (W - 1)) + (unsigned)kx * s.ps) = v; },
           bo());
     }
   } else {
@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
 #pragma unroll
   for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false, WC>(v[r], lst), wi);
   fft_r2g<float2, N>(
-      tile, Ls, tw, v, [&](int cc, int kx, float2 val) { at32(fb, s.coff(opaque(cc)) + (unsigned)kx * s.ps) = val; },
+      tile, Ls, tw, v, [&](int cc, int kx, float2 val) { at32(fb, s.coff(opaque(cc) & (W - 1)) + (unsigned)kx * s.ps) = val; },
       opaque(b) & (TB - 1));
 }
 
@@ -484,7 +484,7 @@ template <int N> struct SubLds {
   // m-point transforms of the wave's CW main columns (wave 0: also of the Nyquist column)
   template <bool FWD> __device__ void ypass() const {
     using Plan = typename PlanOf<Y>::type;
-    const int ln = opaque(lane);
+    const int ln = opaque(lane) & 63;  // the mask keeps the lane index known non-negative
     fft_wlines<float2, Y, FWD, Plan, CW, false>(pl + sk(w * CW), Lines{CW, CW, 1, 0, PP}, twy, ln);
     if (w == 0) fft_wlines<float2, Y, FWD, Plan, 1, false>(pl + sk(M), Lines{1, 1, 0, 0, PP}, twy, ln);
   }
@@ -493,7 +493,7 @@ template <int N> struct SubLds {
   // C2R split of the wave's rows into the packed M-point spectrum, then the M-point inverse
   __device__ void c2r() const {
     float2* R0 = rows();
-    const int ln = opaque(lane);
+    const int ln = opaque(lane) & 63;  // the mask keeps the lane index known non-negative
     for (int i = 0; i < RW; ++i) {
       float2* z = R0 + i * PP;
       for (int k = ln; k < Q; k += 64) {
@@ -517,7 +517,7 @@ template <int N> struct SubLds {
   // M-point forward along the wave's rows, then the split back to n/2 + 1 values per row
   __device__ void r2c() const {
     float2* R0 = rows();
-    const int ln = opaque(lane);
+    const int ln = opaque(lane) & 63;  // the mask keeps the lane index known non-negative
     fft_wlines<float2, M, true, typename PlanOf<M>::type, RW, true, true>(R0, Lines{RW, RW, PP, 0, 1}, twz, ln);
     for (int i = 0; i < RW; ++i) {
       float2* z = R0 + i * PP;
