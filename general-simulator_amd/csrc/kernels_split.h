@@ -589,18 +589,28 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_mid(float2* __restri
     __syncthreads();
     s.c2r();
     const float* eb = eta + (size_t)x * N * N;  // the x-plane's voxels
+    // One voxel quad per lane and row (Q = 64). The quads of all variants of a row are requested
+    // before any is used, and the variant p's quad is picked by selects, not a branch: without
+    // that, each load waited for the one before it (three round trips per row).
+    static_assert(Q == 64, "one voxel quad per lane and row");
+    const int q = s.lane;
     for (int i = 0; i < RW; ++i) {
       const int t = s.w * RW + i;
       float2* z = s.rows() + i * PP;
-      for (int q = s.lane; q < Q; q += 64) {
-        const unsigned vo = (unsigned)((kSplitP * t + j) * N + 4 * q);  // voxel offset inside the x-plane
-        float4 ep = make_float4(0.f, 0.f, 0.f, 0.f), s2 = make_float4(0.f, 0.f, 0.f, 0.f);
+      const unsigned vo = (unsigned)((kSplitP * t + j) * N + 4 * q);  // voxel offset inside the x-plane
+      float4 ep = make_float4(0.f, 0.f, 0.f, 0.f), s2 = make_float4(0.f, 0.f, 0.f, 0.f);
+      {
+        float4 tq[VT];
+#pragma unroll
+        for (int f = 0; f < VT; ++f) tq[f] = at32(reinterpret_cast<const float4*>(eb + (size_t)f * g.nvox), vo / 4);
 #pragma unroll
         for (int f = 0; f < VT; ++f) {
-          const float4 tq = at32(reinterpret_cast<const float4*>(eb + f * g.nvox), vo / 4);
-          s2.x += tq.x * tq.x; s2.y += tq.y * tq.y; s2.z += tq.z * tq.z; s2.w += tq.w * tq.w;
-          if (f == p) ep = tq;
+          s2.x += tq[f].x * tq[f].x; s2.y += tq[f].y * tq[f].y; s2.z += tq[f].z * tq[f].z; s2.w += tq[f].w * tq[f].w;
+          const bool sel = f == p;
+          ep = make_float4(sel ? tq[f].x : ep.x, sel ? tq[f].y : ep.y, sel ? tq[f].z : ep.z, sel ? tq[f].w : ep.w);
         }
+      }
+      {
         float xi[4] = {0.f, 0.f, 0.f, 0.f};
         if (mk.noise != 0.f) normal4(mk.seed, (unsigned)((x * N * N + vo) >> 2), (uint32_t)p, step, xi);
         const float2 ra = z[SL::sk(2 * q)], rbv = z[SL::sk(2 * q + 1)];
