@@ -490,50 +490,63 @@ template <int N> struct SubLds {
   }
   // the wave's first row
   __device__ float2* rows() const { return pl + w * RW * PP; }
-  // C2R split of the wave's rows into the packed M-point spectrum, then the M-point inverse
+  // C2R split of the wave's rows into the packed M-point spectrum, then the M-point inverse.
+  // Lane k (one per row and value, Q = 64 = the lane count) splits the pair (k, M - k); lane 0's
+  // partner is the Nyquist value in slot M, and its split, with the imaginary parts of X_0 and X_M
+  // taken as zero, is (X_0 + X_M, X_0 - X_M): the general formula at w = 1, so no lane branches.
+  // Lane 0 also writes its Z_{M-k} into slot M, which the M-point transform does not read. The
+  // self-paired value k = Q of the wave's RW rows is split afterwards, one row per lane.
   __device__ void c2r() const {
+    static_assert(Q == 64 && RW <= 64, "one pair per lane and row");
     float2* R0 = rows();
     const int ln = opaque(lane) & 63;  // the mask keeps the lane index known non-negative
+    const float2 w = rt[ln];
+    const bool k0 = ln == 0;
     for (int i = 0; i < RW; ++i) {
       float2* z = R0 + i * PP;
-      for (int k = ln; k < Q; k += 64) {
-        if (k == 0) {
-          const float2 a = z[0], b = z[sk(M)], c = z[sk(Q)];
-          float2 zh, unused;
-          c2r_pair(c, c, rt[Q], zh, unused);
-          z[0] = make_float2(a.x + b.x, a.x - b.x);
-          z[sk(Q)] = zh;
-        } else {
-          float2 zk, zm;
-          c2r_pair(z[sk(k)], z[sk(M - k)], rt[k], zk, zm);
-          z[sk(k)] = zk;
-          z[sk(M - k)] = zm;
-        }
-      }
+      float2 xk = z[sk(ln)], xm = z[sk(M - ln)];
+      xk.y = k0 ? 0.f : xk.y;
+      xm.y = k0 ? 0.f : xm.y;
+      float2 zk, zm;
+      c2r_pair(xk, xm, w, zk, zm);
+      z[sk(ln)] = zk;
+      z[sk(M - ln)] = zm;
+    }
+    if (ln < RW) {
+      float2* z = R0 + ln * PP;
+      const float2 c = z[sk(Q)];
+      float2 zh, unused;
+      c2r_pair(c, c, rt[Q], zh, unused);
+      z[sk(Q)] = zh;
     }
     wave_lds_sync();
     fft_wlines<float2, M, false, typename PlanOf<M>::type, RW, true, true>(R0, Lines{RW, RW, PP, 0, 1}, twz, ln);
   }
-  // M-point forward along the wave's rows, then the split back to n/2 + 1 values per row
+  // M-point forward along the wave's rows, then the split back to n/2 + 1 values per row: lane k
+  // turns (Z_k, Z_{M-k}) into (X_k, X_{M-k}); lane 0's partner is Z_0 itself, which the general
+  // formula at w = 1 turns into X_0 = Z_0.x + Z_0.y and, in slot M, X_M = Z_0.x - Z_0.y (imaginary
+  // parts zero). X_Q = conj Z_Q afterwards, one row per lane.
   __device__ void r2c() const {
     float2* R0 = rows();
     const int ln = opaque(lane) & 63;  // the mask keeps the lane index known non-negative
     fft_wlines<float2, M, true, typename PlanOf<M>::type, RW, true, true>(R0, Lines{RW, RW, PP, 0, 1}, twz, ln);
+    const float2 w = rt[ln];
+    const bool k0 = ln == 0;
+    const int pm = (M - ln) & (M - 1);
     for (int i = 0; i < RW; ++i) {
       float2* z = R0 + i * PP;
-      for (int k = ln; k < Q; k += 64) {
-        if (k == 0) {
-          const float2 z0 = z[0], zh = z[sk(Q)];
-          z[0] = make_float2(z0.x + z0.y, 0.f);
-          z[sk(M)] = make_float2(z0.x - z0.y, 0.f);
-          z[sk(Q)] = make_float2(zh.x, -zh.y);
-        } else {
-          float2 xk, xm;
-          r2c_pair(z[sk(k)], z[sk(M - k)], rt[k], xk, xm);
-          z[sk(k)] = xk;
-          z[sk(M - k)] = xm;
-        }
-      }
+      const float2 zk = z[sk(ln)], zm = z[sk(pm)];
+      float2 xk, xm;
+      r2c_pair(zk, zm, w, xk, xm);
+      xk.y = k0 ? 0.f : xk.y;
+      xm.y = k0 ? 0.f : xm.y;
+      z[sk(ln)] = xk;
+      z[sk(M - ln)] = xm;
+    }
+    if (ln < RW) {
+      float2* z = R0 + ln * PP;
+      const float2 zh = z[sk(Q)];
+      z[sk(Q)] = make_float2(zh.x, -zh.y);
     }
   }
 };
