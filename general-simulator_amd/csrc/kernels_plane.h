@@ -119,44 +119,53 @@ template <int N> struct PlaneLds {
     }
   }
   // C2R: split of every row into the packed M-point spectrum (task (y, k) owns the pair k, M - k;
-  // k = 0 also owns M/2 and M), then the M-point inverse along the rows
+  // k = 0 pairs X_0 with the Nyquist value X_M in slot M and, with their imaginary parts taken as
+  // zero, the general formula at w = 1 gives Z_0 = (X_0 + X_M, X_0 - X_M), its second output going
+  // back to slot M, which the M-point transform does not read; the self-paired M/2 of row y is
+  // split by thread y), then the M-point inverse along the rows
   __device__ void c2r() const {
     for (int e = threadIdx.x; e < N * Q; e += T) {
       float2* z = pl + (e / Q) * P;
       const int k = e % Q;
-      if (k == 0) {
-        const float2 a = z[0], b = z[M], c = z[Q];
-        float2 zh, unused;
-        c2r_pair(c, c, rt[Q], zh, unused);
-        z[0] = make_float2(a.x + b.x, a.x - b.x);
-        z[Q] = zh;
-      } else {
-        float2 zk, zm;
-        c2r_pair(z[k], z[M - k], rt[k], zk, zm);
-        z[k] = zk;
-        z[M - k] = zm;
-      }
+      const bool k0 = k == 0;
+      float2 xk = z[k], xm = z[M - k];
+      xk.y = k0 ? 0.f : xk.y;
+      xm.y = k0 ? 0.f : xm.y;
+      float2 zk, zm;
+      c2r_pair(xk, xm, rt[k], zk, zm);
+      z[k] = zk;
+      z[M - k] = zm;
+    }
+    for (int y = threadIdx.x; y < N; y += T) {
+      float2* z = pl + y * P;
+      const float2 c = z[Q];
+      float2 zh, unused;
+      c2r_pair(c, c, rt[Q], zh, unused);
+      z[Q] = zh;
     }
     __syncthreads();
     fft_lines_p<float2, M, false, typename ZPlanOf<M>::type>(pl, Lines{N, N, P, 0, 1}, twz);
   }
-  // R2C: M-point forward along the rows, then the split back to n/2 + 1 values per row
+  // R2C: M-point forward along the rows, then the split back to n/2 + 1 values per row (k = 0:
+  // Z_0 with itself, X_0 = Z_0.x + Z_0.y and X_M = Z_0.x - Z_0.y into slot M; X_{M/2} = conj
+  // Z_{M/2} by thread y)
   __device__ void r2c() const {
     fft_lines_p<float2, M, true, typename ZPlanOf<M>::type>(pl, Lines{N, N, P, 0, 1}, twz);
     for (int e = threadIdx.x; e < N * Q; e += T) {
       float2* z = pl + (e / Q) * P;
       const int k = e % Q;
-      if (k == 0) {
-        const float2 z0 = z[0], zh = z[Q];
-        z[0] = make_float2(z0.x + z0.y, 0.f);
-        z[M] = make_float2(z0.x - z0.y, 0.f);
-        z[Q] = make_float2(zh.x, -zh.y);
-      } else {
-        float2 xk, xm;
-        r2c_pair(z[k], z[M - k], rt[k], xk, xm);
-        z[k] = xk;
-        z[M - k] = xm;
-      }
+      const bool k0 = k == 0;
+      float2 xk, xm;
+      r2c_pair(z[k], z[k0 ? 0 : M - k], rt[k], xk, xm);
+      xk.y = k0 ? 0.f : xk.y;
+      xm.y = k0 ? 0.f : xm.y;
+      z[k] = xk;
+      z[M - k] = xm;
+    }
+    for (int y = threadIdx.x; y < N; y += T) {
+      float2* z = pl + y * P;
+      const float2 zh = z[Q];
+      z[Q] = make_float2(zh.x, -zh.y);
     }
     __syncthreads();
   }

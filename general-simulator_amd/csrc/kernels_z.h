@@ -181,11 +181,14 @@ struct ZShape {
   __device__ static bool valid(int e) { return FULL || e < L * H2; }
 };
 
-// The half-spectrum values of one tile that a thread owns, held in registers.
+// The half-spectrum values of one tile that a thread owns, held in registers: the pair (k, M - k)
+// of each of its tasks (k = 0: the Nyquist value kz = M as the partner), and for threads tid < L
+// the self-paired value k = M/2 of tile row tid.
 template <int M, int TM = 1, int LR = zrows(M), int RM = ZPlanOf<M>::rmin>
 struct ZSpec {
   using S = ZShape<M, TM, LR, RM>;
-  float2 a[S::NT], b[S::NT], c[S::NT];
+  static_assert(S::T >= S::L, "one thread per row for the self-paired values");
+  float2 a[S::NT], b[S::NT], cq;
   // Rows r0.. of one field: a uniform field base and 32-bit element offsets (a field holds
   // fewer than 2^29 elements), so the loads use a scalar base and one offset register each.
   __device__ __forceinline__ void load(const float2* __restrict__ field, const Geo& g, long r0) {
@@ -197,36 +200,42 @@ struct ZSpec {
         const unsigned l = e / S::H2, k = e % S::H2, ro = rb + l * M;
         a[i] = at32(field, ro + k);
         b[i] = at32(field, k == 0 ? nb + l : ro + (M - k));  // k = 0: the kz = M value (Nyquist plane)
-        if (k == 0) c[i] = at32(field, ro + S::H2);
       }
     }
+    if (threadIdx.x < (unsigned)S::L) cq = at32(field, rb + threadIdx.x * M + S::H2);
   }
-  // C2R split into the packed spectra of the LDS tile (row pitch S::P)
+  // C2R split into the packed spectra of the LDS tile (row pitch S::P > M). The pair of k = 0,
+  // with the imaginary parts of X_0 and X_M taken as zero, is split by the general formula at
+  // w = 1 into Z_0 = (X_0 + X_M, X_0 - X_M); its second output lands in the row's padding slot M.
   __device__ __forceinline__ void split(float2* tile, const float2* rt, int tid = threadIdx.x) const {
 #pragma unroll
     for (int i = 0; i < S::NT; ++i) {
-      const int e = tid + i * S::T;
+      const unsigned e = (unsigned)tid + i * S::T;
       if (S::valid(e)) {
-        const int l = e / S::H2, k = e % S::H2;
+        const unsigned l = e / S::H2, k = e % S::H2;
         float2* z = tile + l * S::P;
-        if (k == 0) {
-          z[0] = make_float2(a[i].x + b[i].x, a[i].x - b[i].x);
-          float2 zh, unused;
-          c2r_pair(c[i], c[i], rt[S::H2], zh, unused);
-          z[S::H2] = zh;
-        } else {
-          float2 zk, zm;
-          c2r_pair(a[i], b[i], rt[k], zk, zm);
-          z[k] = zk;
-          z[M - k] = zm;
-        }
+        const bool k0 = k == 0;
+        float2 xk = a[i], xm = b[i];
+        xk.y = k0 ? 0.f : xk.y;
+        xm.y = k0 ? 0.f : xm.y;
+        float2 zk, zm;
+        c2r_pair(xk, xm, rt[k], zk, zm);
+        z[k] = zk;
+        z[M - k] = zm;
       }
+    }
+    if ((unsigned)tid < (unsigned)S::L) {
+      float2 zh, unused;
+      c2r_pair(cq, cq, rt[S::H2], zh, unused);
+      tile[tid * S::P + S::H2] = zh;
     }
   }
 };
 
 // packed spectra of the LDS tile -> half-spectrum rows r0.. of one field (R2C direction);
-// offsets as in ZSpec::load
+// offsets as in ZSpec::load. The pair of k = 0 is Z_0 with itself: the general formula at w = 1
+// gives X_0 = Z_0.x + Z_0.y and X_M = Z_0.x - Z_0.y (imaginary parts zero); X_{M/2} = conj Z_{M/2}
+// of row tid is written by thread tid < L.
 template <int M, int TM = 1, int LR = zrows(M), int RM = ZPlanOf<M>::rmin>
 __device__ __forceinline__ void z_store_spec(const float2* tile, float2* __restrict__ field, const Geo& g, long r0,
                                              const float2* rt, int tid = threadIdx.x) {
@@ -234,22 +243,22 @@ __device__ __forceinline__ void z_store_spec(const float2* tile, float2* __restr
   const unsigned rb = (unsigned)r0 * M, nb = (unsigned)(g.nq + r0);
 #pragma unroll
   for (int i = 0; i < S::NT; ++i) {
-    const unsigned e = tid + i * S::T;
+    const unsigned e = (unsigned)tid + i * S::T;
     if (S::valid(e)) {
       const unsigned l = e / S::H2, k = e % S::H2, ro = rb + l * M;
       const float2* z = tile + l * S::P;
+      const bool k0 = k == 0;
       float2 xk, xm;
-      if (k == 0) {
-        const float2 z0 = z[0], zh = z[S::H2];
-        xk = make_float2(z0.x + z0.y, 0.f);
-        xm = make_float2(z0.x - z0.y, 0.f);
-        at32(field, ro + S::H2) = make_float2(zh.x, -zh.y);
-      } else {
-        r2c_pair(z[k], z[M - k], rt[k], xk, xm);
-      }
+      r2c_pair(z[k], z[k0 ? 0 : M - k], rt[k], xk, xm);
+      xk.y = k0 ? 0.f : xk.y;
+      xm.y = k0 ? 0.f : xm.y;
       at32(field, ro + k) = xk;
-      at32(field, k == 0 ? nb + l : ro + (M - k)) = xm;
+      at32(field, k0 ? nb + l : ro + (M - k)) = xm;
     }
+  }
+  if ((unsigned)tid < (unsigned)S::L) {
+    const float2 zh = tile[tid * S::P + S::H2];
+    at32(field, rb + (unsigned)tid * M + S::H2) = make_float2(zh.x, -zh.y);
   }
 }
 
