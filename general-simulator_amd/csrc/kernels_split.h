@@ -56,12 +56,14 @@ constexpr int kSplitP = 4;  // sub-planes per x-plane
 constexpr int kSubT = GSK_SUB_T;
 constexpr int kSplitWc = GSK_SPLIT_WC;  // x update (and the k-space state's tile order)
 // Contraction's x transform and tile: GSK_XC_PLAN 0 (the edge plan 16 * 16 and kSplitWc columns
-// per sub-plane group: 16 values of each field per thread) or 1 (8 * 8 * 4 and 4 columns: 4
-// values per thread, 84 instead of 246 VGPRs at V = 3, but 232 instead of 203 us at 256^3).
+// per sub-plane group: 16 values of each field per thread), 1 (8 * 8 * 4 and 4 columns: 4 values
+// per thread, 84 instead of 246 VGPRs at V = 3 before the LDS-parked field, but 232 instead of
+// 203 us at 256^3) or 2 (the edge plan and 4 columns: 256-thread groups, four per CU, but 32-byte
+// row segments: 185 -> 283 us).
 #ifndef GSK_XC_PLAN
 #define GSK_XC_PLAN 0
 #endif
-constexpr int kSplitWcC = GSK_XC_PLAN == 1 ? 4 : kSplitWc;
+constexpr int kSplitWcC = GSK_XC_PLAN >= 1 ? 4 : kSplitWc;
 template <int WC> constexpr int split_w() { return kSplitP * WC; }
 static_assert(kSplitWc == 4 || kSplitWc == 8 || kSplitWc == 16, "sub-plane group width 4, 8 or 16");
 
