@@ -195,7 +195,10 @@ int step_mart_ref(gsk_sim* s) {
   k.seed = c.seed;
   LAUNCH(k_ref_mart<>, std::min(grid_1d(N3 / 4), kRefBlocks), 256, 0, s, fld, s->rbuf, s->defect, s->Kd, s->nx, s->ny,
          s->nz, k, s->fstat);
-  LAUNCH(k_copy4<>, grid_1d(VN / 4), 256, 0, s, fld, s->rbuf, VN);
+  // the new eta is in rbuf: the two buffers trade places instead of copying V fields back
+  float* next = s->rbuf;
+  s->rbuf = fld;
+  w.fld = next;
   return GSK_OK;
 }
 

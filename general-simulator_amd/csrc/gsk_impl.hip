@@ -546,8 +546,9 @@ struct Impl {
     // at most kRefBlocks blocks (grid-stride), so the per-block force sums fit s->part
     LAUNCH(k_ref_mart<>, std::min(grid_1d(N3 / 4), kRefBlocks), 256, 0, s, s->fld, s->rbuf, s->defect, s->Kd, s->n, s->n,
            s->n, k, s->fstat);
-    LAUNCH(k_copy4<>, grid_1d((long)s->V * N3 / 4), 256, 0, s, s->fld, s->rbuf, (long)s->V * N3);
     LAUNCH(k_advance<>, 1, 64, 0, s, s->dstep);
+    // the new eta is in rbuf: the two buffers trade places instead of copying V fields back
+    std::swap(s->fld, s->rbuf);
     return GSK_OK;
   }
 
