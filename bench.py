@@ -346,6 +346,41 @@ def run_box(a):
     print(json.dumps(result), flush=True)
 
 
+def run_ref(a):
+    """The martensite plugin's own rule (explicit Euler with the elastic R from the fused transforms,
+    src/dynamics_mart.cu:348-417) and the precipitate plugin's (src/dynamics_precipitate.cu:278-289)
+    at n^3: what an unmodified reference script runs."""
+    import torch
+
+    import gsk
+
+    n = a.n
+    out = {}
+    for phys, name in ((gsk.PHYS_MART, "mart_reference"), (gsk.PHYS_PREC, "prec_reference")):
+        V = 3
+        cfg = gsk.reference_config(phys, n=n, nvar=V, device=0, temperature=300.0, w_el=1e3, dt=0.002)
+        with gsk.Sim(cfg) as s:
+            s.build_btensor(gsk.cubic_modulus(3.5, 1.5, 1.0), gsk.mart_eigenstrains(V, base=0.0, axis=0.01))
+            if phys == gsk.PHYS_PREC:
+                s.init_uniform("concentration", 0.245, 0.255, 1)
+            s.init_uniform("eta", 0.0, 0.1, 2)
+            s.step(a.warmup)
+            wall, _, walls = timed_repeats(s, a.steps, a.repeats, torch)
+        ms = wall / a.steps
+        out[name] = {"value": round(n**3 / (ms / 1e3) / 1e6, 2), "ms_per_step": round(ms, 4),
+                     "repeats": repeat_stats(walls, a.steps)}
+        log(f"{name} n={n}: {ms:.4f} ms/step")
+    result = {
+        "metric": f"Mvoxel-steps/s, {n}^3 3-variant martensite, the plugin's own rule (FP32)",
+        "value": out["mart_reference"]["value"], "unit": "Mvoxel-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": out["mart_reference"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (Philox uniform fields)",
+        "config": {"workload": f"reference rules at {n}^3, 3 variants", "parallelism": "single GPU"},
+        "rules": out,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -363,9 +398,10 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs for roofline.traffic")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size once (counter calibration)")
-    ap.add_argument("--workload", choices=["mart256", "ensemble", "prec256", "box"], default="mart256",
+    ap.add_argument("--workload", choices=["mart256", "ensemble", "prec256", "box", "reference"], default="mart256",
                     help="mart256: the headline 256^3 step; ensemble: 96 mixed-size instances; "
-                         "prec256: the precipitate step at 256^3; box: a box grid on the box path")
+                         "prec256: the precipitate step at 256^3; box: a box grid on the box path; "
+                         "reference: the plugins' own rules at --n")
     ap.add_argument("--box", type=int, nargs=3, default=[256, 256, 128], help="box workload: nx ny nz")
     ap.add_argument("--ens-per-size", type=int, default=32)
     ap.add_argument("--ens-trial", type=int, default=5)
@@ -386,6 +422,9 @@ def main():
         return
     if a.workload == "box":
         run_box(a)
+        return
+    if a.workload == "reference":
+        run_ref(a)
         return
 
     import torch
