@@ -366,9 +366,13 @@ def run_ref(a):
             s.init_uniform("eta", 0.0, 0.1, 2)
             s.step(a.warmup)
             wall, _, walls = timed_repeats(s, a.steps, a.repeats, torch)
+            ksteps = max(1, a.profile_steps)
+            names, kms = s.kernel_times(ksteps)
+            kms = kms / ksteps  # per step
         ms = wall / a.steps
         out[name] = {"value": round(n**3 / (ms / 1e3) / 1e6, 2), "ms_per_step": round(ms, 4),
-                     "repeats": repeat_stats(walls, a.steps)}
+                     "repeats": repeat_stats(walls, a.steps),
+                     "kernels_us": {k: round(1e3 * float(t), 1) for k, t in zip(names, kms)}}
         log(f"{name} n={n}: {ms:.4f} ms/step")
     result = {
         "metric": f"Mvoxel-steps/s, {n}^3 3-variant martensite, the plugin's own rule (FP32)",

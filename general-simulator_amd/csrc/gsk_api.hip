@@ -291,6 +291,7 @@ static int set_ref_field(gsk_sim* s, const char* name, const void* host, int dty
   if (!*dst) HIPCHK(hipMalloc((void**)dst, h.size() * sizeof(float)));
   HIPCHK(hipMemcpy(*dst, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
   ++s->gen;
+  build_slots(s);  // the update now reads this field too
   return GSK_OK;
 }
 
@@ -499,8 +500,7 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
   if (what == GSK_RED_MART_FORCES) {
     if (c.physics != GSK_PHYS_MART || c.rule != GSK_RULE_REFERENCE || !s->fstat)
       return fail(GSK_ERR_UNSUPPORTED, "the force means belong to the martensite reference rule");
-    const int nbr = std::min(grid_1d(N3 / 4), kRefBlocks);
-    LAUNCH(r_finish<>, 1, 1024, 0, s, s->fstat, nbr, 4, s->rout);
+    LAUNCH(r_finish<>, 1, 1024, 0, s, s->fstat, s->fblocks, 4, s->rout);
     HIPCHK(hipMemcpyAsync(h, s->rout, 4 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     CHK(sync(s));
     for (int j = 0; j < 4; ++j) out[j] = h[j] / ((double)s->V * (double)N3);

@@ -193,8 +193,7 @@ int step_mart_ref(gsk_sim* s) {
   k.step = (uint32_t)s->step;
   k.stepp = nullptr;
   k.seed = c.seed;
-  LAUNCH(k_ref_mart<>, std::min(grid_1d(N3 / 4), kRefBlocks), 256, 0, s, fld, s->rbuf, s->defect, s->Kd, s->nx, s->ny,
-         s->nz, k, s->fstat);
+  CHK(launch_ref_mart(s, fld, s->rbuf, s->nx, s->ny, s->nz, k));
   // the new eta is in rbuf: the two buffers trade places instead of copying V fields back
   float* next = s->rbuf;
   s->rbuf = fld;

@@ -95,6 +95,7 @@ struct Impl {
       CHK(lds_limit((const void*)k_sub_mid<N, 3>));
       CHK(lds_limit((const void*)k_sub_mid<N, 4>));
       CHK(lds_limit((const void*)k_sub_sq<N>));
+      CHK(lds_limit((const void*)k_sub_sq<N, false>));
       CHK(lds_limit((const void*)k_sub_fwd<N, 0>));
       CHK(lds_limit((const void*)k_sub_fwd<N, 1>));
     }
@@ -512,6 +513,18 @@ struct Impl {
   static int elastic_R_real(gsk_sim* s, const float* eta, float* out) {
     const int V = s->V, gx = s->g.ntile;
     MartK mk = mart_consts(s);
+    if (s->split) {  // squared, R2C and y forward -> contraction -> y inverse and C2R (three launches)
+      if constexpr (split_size<N>()) {
+        if (eta != s->fld) return fail(GSK_ERR, "split elastic R takes the handle's fields");
+        CHK(split_fwd(s, true));
+        CHK(split_xcontract<0>(s, mk));
+        if (!s->cus) HIPCHK(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->dev));
+        const int nitems = N * kSplitP * s->F, gq = std::min(nitems, std::max(8, 2 * s->cus / 8 * 8));
+        LAUNCH((k_sub_sq<N, false>), gq, kSubT, sub_lds<N>(), s, s->kbuf, out, s->sg, s->twN, s->twM, s->rtn,
+               (uint32_t*)nullptr, nitems);
+        return GSK_OK;
+      }
+    }
     CHK(zfwd(s, Z_SQUARE, eta, V));
     CHK(ypass(s, true, V));
     switch (V) {
@@ -542,10 +555,7 @@ struct Impl {
     k.step = (uint32_t)s->step;
     k.stepp = s->dstep;
     k.seed = c.seed;
-    const long N3 = (long)s->n * s->n * s->n;
-    // at most kRefBlocks blocks (grid-stride), so the per-block force sums fit s->part
-    LAUNCH(k_ref_mart<>, std::min(grid_1d(N3 / 4), kRefBlocks), 256, 0, s, s->fld, s->rbuf, s->defect, s->Kd, s->n, s->n,
-           s->n, k, s->fstat);
+    CHK(launch_ref_mart(s, s->fld, s->rbuf, s->n, s->n, s->n, k));
     LAUNCH(k_advance<>, 1, 64, 0, s, s->dstep);
     // the new eta is in rbuf: the two buffers trade places instead of copying V fields back
     std::swap(s->fld, s->rbuf);

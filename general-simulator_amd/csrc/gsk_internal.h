@@ -187,6 +187,7 @@ struct gsk_sim {
   float* defect = nullptr;    // n^3 blocking field (martensite reference rule), optional
   float* Kd = nullptr;        // V*n^3 dislocation force constants, optional
   double* fstat = nullptr;    // per-block sums of the four force terms of the last reference step
+  int fblocks = 0;            // work groups (partial sums in fstat) of the last reference step
   double eps[72] = {};        // eigenstrains of the last B build
   bool haveEps = false;
   bool sharedB = false;       // Bt belongs to an ensemble, not to this handle
@@ -237,14 +238,14 @@ inline bool plane_on(const gsk_sim* s) {
          s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL;
 }
 
-// Split-y schedule (kernels_split.h): the spectral martensite step at n = 256 with 1..4 variants.
-// GSK_SPLIT=0 in the environment when a handle is created keeps that handle on the chain of y and
-// z kernels.
+// Split-y schedule (kernels_split.h): the martensite step at n = 256 with 1..4 variants (the
+// spectral rule's step; the reference rule's elastic R). GSK_SPLIT=0 in the environment when a
+// handle is created keeps that handle on the chain of y and z kernels.
 inline bool split_wanted(const gsk_sim* s) {
   const char* e = std::getenv("GSK_SPLIT");
   const bool on = !(e && e[0] == '0');
   return on && s->n == 256 && s->F >= 1 && s->F <= 4 && s->cfg.path == GSK_PATH_FUSED32 &&
-         s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL;
+         s->cfg.physics == GSK_PHYS_MART;
 }
 
 inline size_t nvox(const gsk_sim* s) { return (size_t)s->nx * s->ny * s->nz; }
@@ -357,6 +358,39 @@ inline int grid_1d(long work, int threads = 256) {
   if (b > 65536) b = 65536;
   if (b < 1) b = 1;
   return (int)b;
+}
+
+// The martensite reference-rule update of the V fields at eta on an nx x ny x nz grid, R in RE on
+// entry (the new eta on exit): the 16-byte XCD-ranged kernel for 1..4 variants, the general one
+// otherwise. Records the work-group count whose force sums fstat holds.
+inline int launch_ref_mart(gsk_sim* s, const float* eta, float* RE, int nx, int ny, int nz, const RefMartK& k) {
+  const long nq = (long)nx * ny * nz / 4;
+  const int V = s->V;
+  if (V < 1 || V > 4) {
+    s->fblocks = (int)std::min<long>(grid_1d(nq), kRefBlocks);
+    LAUNCH(k_ref_mart<>, s->fblocks, 256, 0, s, eta, RE, s->defect, s->Kd, nx, ny, nz, k, s->fstat);
+    return GSK_OK;
+  }
+  if (!s->cus) HIPCHK(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->dev));
+  const long chunks = (nq + 255) / 256;  // 256 quads per work group and chunk
+  const long g = std::min<long>(std::min<long>(kRefBlocks, 4L * s->cus), (chunks + 7) / 8 * 8);
+  const int G = (int)std::max<long>(8, g / 8 * 8);
+  const int L = nz / 4;
+  const bool zs = L <= 64 && 64 % L == 0 && nq % 256 == 0;
+  s->fblocks = G;
+#define GSK_REFQ(v, z) LAUNCH((k_ref_mart_q<v, z>), G, 256, 0, s, eta, RE, s->defect, s->Kd, nx, ny, nz, k, s->fstat)
+  switch (V * 2 + (zs ? 1 : 0)) {
+    case 2: GSK_REFQ(1, false); break;
+    case 3: GSK_REFQ(1, true); break;
+    case 4: GSK_REFQ(2, false); break;
+    case 5: GSK_REFQ(2, true); break;
+    case 6: GSK_REFQ(3, false); break;
+    case 7: GSK_REFQ(3, true); break;
+    case 8: GSK_REFQ(4, false); break;
+    default: GSK_REFQ(4, true);
+  }
+#undef GSK_REFQ
+  return GSK_OK;
 }
 
 // Allow a kernel the device's whole LDS for dynamic shared memory (minus its static part).
@@ -567,6 +601,25 @@ inline void build_slots(gsk_sim* s) {
     s->slots.push_back({"p_extract", V * (cf + r8)});
     return;
   }
+  if (s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_REFERENCE) {
+    // elastic R in real space, then the explicit update (eta and R read, the new eta written
+    // over R; the dislocation term K and the defect mask when set)
+    const int npq = F * (F + 1) / 2;
+    if (s->split) {  // three launches for R: sub-plane forward, contraction, sub-plane inverse
+      s->slots.push_back({"sub_fwd_sq", F * (rv + hs)});
+      s->slots.push_back({"x_contract", F * 2 * hs + npq * hs / 2});
+      s->slots.push_back({"sub_inv_R", F * (hs + rv)});
+    } else {
+      s->slots.push_back({"zfwd_sq", F * (rv + hs)});
+      s->slots.push_back({"y_fwd_S", F * 2 * hs});
+      s->slots.push_back({"x_contract", F * 2 * hs + npq * hs / 2});
+      s->slots.push_back({"y_inv_R", F * 2 * hs});
+      s->slots.push_back({"zinv_R", F * (hs + rv)});
+    }
+    s->slots.push_back({"ref_update", F * 3 * rv + (s->Kd ? F * rv : 0.0) + (s->defect ? rv : 0.0)});
+    s->slots.push_back({"advance", 0.0});
+    return;
+  }
   if (s->cfg.physics == GSK_PHYS_MART) {
     const int npq = F * (F + 1) / 2;
     if (s->split) {  // four launches: the split-y schedule
@@ -600,6 +653,14 @@ inline void build_slots(gsk_sim* s) {
       s->slots.push_back({"zinv_eta_sq", F * (2 * hs + rv)});  // C2R, write eta, R2C of eta^2
     }
   } else {
+    if (s->cfg.rule == GSK_RULE_REFERENCE) {  // the elastic R of the V order parameters first
+      const int V = s->V, npq = V * (V + 1) / 2;
+      s->slots.push_back({"zfwd_sq", V * (rv + hs)});
+      s->slots.push_back({"y_fwd_S", V * 2 * hs});
+      s->slots.push_back({"x_contract", V * 2 * hs + npq * hs / 2});
+      s->slots.push_back({"y_inv_R", V * 2 * hs});
+      s->slots.push_back({"zinv_R", V * (hs + rv)});
+    }
     const bool fuse = prec_carries(s);
     if (!fuse) s->slots.push_back({"zfwd_local", F * (rv + hs)});
     s->slots.push_back({"y_fwd", F * 2 * hs});

@@ -653,8 +653,9 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_mid(float2* __restri
 // End of a step, per (sub-plane, field): A-form eta_hat -> y inverse -> C2R -> eta written to the
 // real field and squared -> R2C -> y forward -> E-form S_hat of the next step. adv (may be null):
 // the step counter the launch advances once. Resident work groups as k_sub_mid; item
-// f * n * P + (sub-plane) is field f of that sub-plane.
-template <int N>
+// f * n * P + (sub-plane) is field f of that sub-plane. SQ = false: the inverse only (y inverse,
+// C2R, the real field written), for fields that end in real space (the reference rule's R).
+template <int N, bool SQ = true>
 __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_sq(float2* __restrict__ kbuf, float* __restrict__ out, SGeo g,
                                                  const float2* __restrict__ twN, const float2* __restrict__ twM,
                                                  const float2* __restrict__ rtn, uint32_t* adv, int nitems) {
@@ -690,18 +691,21 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_sq(float2* __restric
       for (int q = s.lane; q < Q; q += 64) {
         const float2 a = z[SL::sk(2 * q)], b = z[SL::sk(2 * q + 1)];
         at32(reinterpret_cast<float4*>(dst), (unsigned)((kSplitP * t + j) * N + 4 * q) / 4) = make_float4(a.x, a.y, b.x, b.y);
-        // each lane squares the very elements it has just read
-        z[SL::sk(2 * q)] = make_float2(a.x * a.x, a.y * a.y);
-        z[SL::sk(2 * q + 1)] = make_float2(b.x * b.x, b.y * b.y);
+        if (SQ) {  // each lane squares the very elements it has just read
+          z[SL::sk(2 * q)] = make_float2(a.x * a.x, a.y * a.y);
+          z[SL::sk(2 * q + 1)] = make_float2(b.x * b.x, b.y * b.y);
+        }
       }
     }
-    wave_lds_sync();
-    s.r2c();
-    __syncthreads();
-    s.template ypass<true>();
-    __syncthreads();
-    s.store(field, g, rb);
-    __syncthreads();  // the next put overwrites what store reads
+    if (SQ) {
+      wave_lds_sync();
+      s.r2c();
+      __syncthreads();
+      s.template ypass<true>();
+      __syncthreads();
+      s.store(field, g, rb);
+    }
+    __syncthreads();  // the next put overwrites what store (or the row phase) reads
   }
 }
 

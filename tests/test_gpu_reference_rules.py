@@ -59,6 +59,74 @@ def test_mart_reference_rule_matches_oracle(gpu, chem_form, noise):
     assert err < 2e-5
 
 
+def _mart_ref_256(noise):
+    C = gsk.cubic_modulus(3.5, 1.5, 1.0)
+    eps = gsk.mart_eigenstrains(3, base=0.0, axis=0.01)
+    cfg = gsk.reference_config(gsk.PHYS_MART, n=256, nvar=3, temperature=300.0, noise=noise, w_el=1e3, seed=6,
+                               dt=0.005)
+    return C, eps, cfg
+
+
+def test_mart_reference_rule_matches_oracle_at_256(gpu):
+    """n = 256, where the elastic R of the reference rule runs through the sub-plane kernels and
+    the split-order contraction (three launches) instead of the chain of z, y and x kernels:
+    three steps with noise, defects and dislocation stress against the FP64 oracle."""
+    say("mart reference rule n=256 (split R) vs oracle")
+    n, V, steps = 256, 3, 3
+    C, eps, cfg = _mart_ref_256(1e-4)
+    rng = np.random.default_rng(2)
+    defect = (rng.random((n, n, n)) < 0.01).astype(np.float32)
+    stress = rng.uniform(-0.01, 0.01, (6, n, n, n)).astype(np.float32)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(C, eps)
+        s.init_uniform("eta", 0.0, 0.2, 3)
+        s.set_field("defect", defect)
+        s.set_field("dislocationstress", stress)
+        eta0 = s.get_field("eta").astype(np.float64)
+        s.step(steps)
+        got = s.get_field("eta", np.float64)
+        names, _ = s.kernel_times(1)
+    assert "sub_inv_R" in names, names
+    a1 = 0.02 * (cfg.temperature - cfg.T0)
+    p = O.mart_ref_params(a1=a1, a2=cfg.B, a3=cfg.C, w_grad=cfg.w_grad, w_chem=cfg.w_chem, w_el=cfg.w_el,
+                          w_dis=cfg.w_dis, w_noise=cfg.noise, dt=cfg.dt, dx=cfg.dx, kconv=cfg.kconv,
+                          chem_form=cfg.chem_form, seed=cfg.seed)
+    comp = [0, 1, 2, 1, 3, 4, 2, 4, 5]
+    sig = stress.astype(np.float64)[comp].reshape(3, 3, n, n, n)
+    K = 25.0 * np.einsum("ijxyz,pij->pxyz", sig, eps)
+    del sig
+    want = O.mart_ref_run(eta0, O.btensor(n, C, eps, kconv=cfg.kconv), p, steps, defect=defect.astype(np.float64),
+                          K=K)
+    err = rel(got, want)
+    say(f"  rel L2 {err:.2e}")
+    assert np.isfinite(got).all()
+    assert np.all(got[:, defect > 0] == 0.0)
+    assert err < 1e-5
+
+
+def test_mart_reference_rule_split_matches_chain_at_256(gpu, monkeypatch):
+    """The reference rule's elastic R at n = 256 through the sub-plane kernels (default) and
+    through the chain of z, y and x kernels (GSK_SPLIT=0 when the handle is created): two FP32
+    orderings of one computation, so 10 steps with noise agree to FP32 rounding."""
+    say("mart reference rule n=256: split R vs chain")
+    C, eps, cfg = _mart_ref_256(1e-3)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("GSK_SPLIT", mode)
+        with gsk.Sim(cfg) as s:
+            s.build_btensor(C, eps)
+            s.init_uniform("eta", 0.0, 0.2, 8)
+            s.step(10)
+            out[mode] = s.get_field("eta", np.float64)
+            names, _ = s.kernel_times(1)
+        assert ("sub_inv_R" in names) == (mode == "1"), names
+        assert ("zinv_R" in names) == (mode == "0"), names
+    a, b = out["1"], out["0"]
+    assert np.isfinite(a).all()
+    assert float(np.abs(a - b).max()) < 2e-6 * float(np.abs(b).max())
+    assert rel(a, b) < 1e-6
+
+
 def test_prec_reference_rule_matches_oracle(gpu):
     say("precipitate reference rule")
     n, V, steps = 32, 3, 30
