@@ -230,7 +230,12 @@ int gsk_build_btensor(gsk_sim* s, const double* C81, const double* eps, int neps
   s->haveEps = true;
   if (s->bx) {
     CHK(box_build_btensor(s, a));
-  } else if (s->cfg.path == GSK_PATH_FUSED32 && s->split) {
+  } else if (s->cfg.path == GSK_PATH_FUSED32 && s->split && s->cfg.physics == GSK_PHYS_PREC &&
+             s->cfg.rule == GSK_RULE_SPECTRAL) {
+    // the concentration's scalar B(k), read by the precipitate's x update (its tile order)
+    const long total = (long)s->sg.ntile * s->n * split_w<kSplitWc>();
+    hipLaunchKernelGGL(k_btensor_split<kSplitWc>, dim3(grid_1d(total)), dim3(256), 0, s->stream, s->Bt, s->sg, a);
+  } else if (s->cfg.path == GSK_PATH_FUSED32 && s->split) {  // read by the contraction
     const long total = (long)s->sg.ntile_c * s->n * split_w<kSplitWcC>();
     hipLaunchKernelGGL(k_btensor_split<kSplitWcC>, dim3(grid_1d(total)), dim3(256), 0, s->stream, s->Bt, s->sg, a);
   } else if (s->cfg.path == GSK_PATH_FUSED32) {
@@ -562,7 +567,8 @@ int gsk_reduce(gsk_sim* s, int what, double* out) {
         double acc = 0;
         for (int f = 1; f < s->F; ++f) acc += gh[f];
         gr = 0.5 * c.kappa_c * gh[0] / (double)N3 + 0.5 * c.kappa * acc / (double)N3;
-        LAUNCH(r_state_el_prec<>, nb, 256, 0, s, s->state, s->Bt, s->g, s->F, s->part);
+        if (s->split) LAUNCH(r_state_el_prec_split<>, nb, 256, 0, s, s->state, s->Bt, s->sg, s->F, s->part);
+        else LAUNCH(r_state_el_prec<>, nb, 256, 0, s, s->state, s->Bt, s->g, s->F, s->part);
         LAUNCH(r_finish<>, 1, 1024, 0, s, s->part, nb, 1, s->rout);
         double eh;
         HIPCHK(hipMemcpyAsync(&eh, s->rout, sizeof(double), hipMemcpyDeviceToHost, s->stream));

@@ -98,6 +98,12 @@ struct Impl {
       CHK(lds_limit((const void*)k_sub_sq<N, false>));
       CHK(lds_limit((const void*)k_sub_fwd<N, 0>));
       CHK(lds_limit((const void*)k_sub_fwd<N, 1>));
+      CHK(lds_limit((const void*)k_xupdate_split_prec<N, 0>));
+      CHK(lds_limit((const void*)k_xupdate_split_prec<N, 1>));
+      CHK(lds_limit((const void*)k_sub_local<N, 2>));
+      CHK(lds_limit((const void*)k_sub_local<N, 3>));
+      CHK(lds_limit((const void*)k_sub_local<N, 4>));
+      CHK(lds_limit((const void*)k_sub_local<N, 5>));
     }
     if constexpr (plane_size<N>()) {
       CHK(lds_limit((const void*)k_plane_sq<N, true>));
@@ -235,26 +241,30 @@ struct Impl {
   // ---- split-y schedule (n = 256): sub-plane launches have grid (n * P sub-planes, fields)
   static constexpr size_t SPLIT_XLDS = (size_t)(N + N * split_w<kSplitWc>()) * sizeof(float2);   // x update
   static constexpr size_t SPLIT_CLDS = (size_t)(N + N * split_w<kSplitWcC>()) * sizeof(float2);  // contraction
-  static int split_fwd(gsk_sim* s, bool square) {
+  // E-form spectra of `count` real fields at `in` (square: of their squares) into the k-space
+  // buffer's first `count` fields; the handle's fields by default
+  static int split_fwd(gsk_sim* s, bool square, const float* in = nullptr, int count = 0) {
     if constexpr (split_size<N>()) {
-      const dim3 grid(N * kSplitP, s->F);
+      if (!in) in = s->fld;
+      const dim3 grid(N * kSplitP, count > 0 ? count : s->F);
       if (square)
-        LAUNCH((k_sub_fwd<N, 1>), grid, kSubT, sub_lds<N>(), s, s->fld, s->kbuf, s->sg, s->twN, s->twM, s->rtn);
+        LAUNCH((k_sub_fwd<N, 1>), grid, kSubT, sub_lds<N>(), s, in, s->kbuf, s->sg, s->twN, s->twM, s->rtn);
       else
-        LAUNCH((k_sub_fwd<N, 0>), grid, kSubT, sub_lds<N>(), s, s->fld, s->kbuf, s->sg, s->twN, s->twM, s->rtn);
+        LAUNCH((k_sub_fwd<N, 0>), grid, kSubT, sub_lds<N>(), s, in, s->kbuf, s->sg, s->twN, s->twM, s->rtn);
       return GSK_OK;
     } else {
-      (void)s; (void)square;
+      (void)s; (void)square; (void)in; (void)count;
       return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
     }
   }
+  // the contraction over the first `count` spectra of the k-space buffer (the handle's F by default)
   template <int MODE>
-  static int split_xcontract(gsk_sim* s, const MartK& mk) {
+  static int split_xcontract(gsk_sim* s, const MartK& mk, int count = 0) {
     if constexpr (split_size<N>()) {
       const int gx = s->sg.ntile_c;
       if (MODE == 1 && gx > s->rblocks) return fail(GSK_ERR, "reduction scratch too small");
       constexpr int T = split_cthreads<N>();
-      switch (s->F) {
+      switch (count > 0 ? count : s->F) {
         case 1: LAUNCH((k_xcontract_split<N, 1, MODE>), gx, T, SPLIT_CLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part); break;
         case 2: LAUNCH((k_xcontract_split<N, 2, MODE>), gx, T, SPLIT_CLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part); break;
         case 3: LAUNCH((k_xcontract_split<N, 3, MODE>), gx, T, SPLIT_CLDS, s, s->kbuf, s->Bt, s->sg, mk, s->twN, s->part); break;
@@ -262,7 +272,7 @@ struct Impl {
       }
       return GSK_OK;
     } else {
-      (void)s; (void)mk;
+      (void)s; (void)mk; (void)count;
       return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
     }
   }
@@ -305,7 +315,70 @@ struct Impl {
     }
   }
 
+  // precipitate, split-y schedule: the update of the F fields (MODE 1: the state from their E-form
+  // transforms), the local forces of the F fields into E-form spectra, the inverse into real fields
+  template <int MODE>
+  static int split_xupdate_prec(gsk_sim* s, const PrecK& pk) {
+    if constexpr (split_size<N>()) {
+      LAUNCH((k_xupdate_split_prec<N, MODE>), dim3(s->sg.ntile, s->F), split_xthreads<N>(), SPLIT_XLDS, s, s->kbuf,
+             s->state, s->Bt, s->sg, pk, s->twN);
+      return GSK_OK;
+    } else {
+      (void)s; (void)pk;
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+    }
+  }
+  static int split_local(gsk_sim* s, const PrecK& pk, const float* R, float el, float cn) {
+    if constexpr (split_size<N>()) {
+      const int items = N * kSplitP * s->F;
+      const SGeo& g = s->sg;
+      switch (s->F) {
+        case 2: LAUNCH((k_sub_local<N, 2>), items, kSubT, sub_lds<N>(), s, s->fld, s->kbuf, g, pk, R, el, cn, s->twN, s->twM, s->rtn); break;
+        case 3: LAUNCH((k_sub_local<N, 3>), items, kSubT, sub_lds<N>(), s, s->fld, s->kbuf, g, pk, R, el, cn, s->twN, s->twM, s->rtn); break;
+        case 4: LAUNCH((k_sub_local<N, 4>), items, kSubT, sub_lds<N>(), s, s->fld, s->kbuf, g, pk, R, el, cn, s->twN, s->twM, s->rtn); break;
+        default: LAUNCH((k_sub_local<N, 5>), items, kSubT, sub_lds<N>(), s, s->fld, s->kbuf, g, pk, R, el, cn, s->twN, s->twM, s->rtn);
+      }
+      return GSK_OK;
+    } else {
+      (void)s; (void)pk; (void)R; (void)el; (void)cn;
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+    }
+  }
+  // y inverse and C2R of the first `count` A-form spectra into real fields at out; adv (may be
+  // null): the step counter the launch advances
+  static int split_inv(gsk_sim* s, float* out, int count, uint32_t* adv) {
+    if constexpr (split_size<N>()) {
+      if (!s->cus) HIPCHK(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->dev));
+      const int nitems = N * kSplitP * count, gq = std::min(nitems, std::max(8, 2 * s->cus / 8 * 8));
+      LAUNCH((k_sub_sq<N, false>), gq, kSubT, sub_lds<N>(), s, s->kbuf, out, s->sg, s->twN, s->twM, s->rtn, adv,
+             nitems);
+      return GSK_OK;
+    } else {
+      (void)s; (void)out; (void)count; (void)adv;
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+    }
+  }
+  // Spectral precipitate step, split-y schedule (three launches): x update of the carried local
+  // forces (advancing the step counter), y inverse and C2R of the fields, the next step's local
+  // forces from the new fields.
+  static int step_prec_split(gsk_sim* s) {
+    PrecK pk = prec_consts(s);
+    if (!s->sq_ready) CHK(split_local(s, pk, nullptr, 0.f, 0.f));
+    pk.adv = s->dstep;
+    CHK(split_xupdate_prec<0>(s, pk));
+    pk.adv = nullptr;
+    CHK(split_inv(s, s->fld, s->F, nullptr));
+    CHK(split_local(s, pk, nullptr, 0.f, 0.f));
+    s->sq_ready = true;
+    return GSK_OK;
+  }
+
   static int init_state(gsk_sim* s) {
+    if (s->split && s->cfg.physics == GSK_PHYS_PREC) {  // E-form transforms of the fields, then the state
+      CHK(split_fwd(s, false));
+      PrecK pk = prec_consts(s);
+      return split_xupdate_prec<1>(s, pk);
+    }
     if (s->split) {  // E-form transform of eta, then the state in the split tile order
       CHK(split_fwd(s, false));
       MartK mk = mart_consts(s);
@@ -329,7 +402,9 @@ struct Impl {
   // plane_on): the first kernel of a step, unless the previous step's fused last pass has left
   // it there
   static int square_fwd(gsk_sim* s) {
-    if (s->split) {
+    if (s->split && s->cfg.physics == GSK_PHYS_PREC) {
+      CHK(split_local(s, prec_consts(s), nullptr, 0.f, 0.f));
+    } else if (s->split) {
       CHK(split_fwd(s, true));
     } else if (plane_on(s)) {
       if constexpr (plane_size<N>()) {
@@ -514,16 +589,9 @@ struct Impl {
     const int V = s->V, gx = s->g.ntile;
     MartK mk = mart_consts(s);
     if (s->split) {  // squared, R2C and y forward -> contraction -> y inverse and C2R (three launches)
-      if constexpr (split_size<N>()) {
-        if (eta != s->fld) return fail(GSK_ERR, "split elastic R takes the handle's fields");
-        CHK(split_fwd(s, true));
-        CHK(split_xcontract<0>(s, mk));
-        if (!s->cus) HIPCHK(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->dev));
-        const int nitems = N * kSplitP * s->F, gq = std::min(nitems, std::max(8, 2 * s->cus / 8 * 8));
-        LAUNCH((k_sub_sq<N, false>), gq, kSubT, sub_lds<N>(), s, s->kbuf, out, s->sg, s->twN, s->twM, s->rtn,
-               (uint32_t*)nullptr, nitems);
-        return GSK_OK;
-      }
+      CHK(split_fwd(s, true, eta, V));
+      CHK(split_xcontract<0>(s, mk, V));
+      return split_inv(s, out, V, nullptr);
     }
     CHK(zfwd(s, Z_SQUARE, eta, V));
     CHK(ypass(s, true, V));
@@ -564,12 +632,19 @@ struct Impl {
 
   static int step_prec_ref(gsk_sim* s) {
     CHK(elastic_R_real(s, s->fld + nvox(s), s->rbuf));
+    if (s->split) {  // local forces with R, x update, inverse (advancing the step counter)
+      const PrecK pk = prec_consts(s);
+      CHK(split_local(s, pk, s->rbuf, (float)(2.0 * s->cfg.xi), (float)s->cfg.noise));
+      CHK(split_xupdate_prec<0>(s, pk));
+      return split_inv(s, s->fld, s->F, s->dstep);
+    }
     return step_prec_fused(s);
   }
 
   // With 2..5 fields the last z pass is fused with the next step's first one (k_zinv_local_r)
   // and the x update advances the noise counter.
   static int step_prec_fused(gsk_sim* s) {
+    if (s->split) return step_prec_split(s);
     const int F = s->F;
     PrecK pk = prec_consts(s);
     const bool fuse = prec_carries(s);

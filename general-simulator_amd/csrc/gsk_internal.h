@@ -238,20 +238,26 @@ inline bool plane_on(const gsk_sim* s) {
          s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL;
 }
 
-// Split-y schedule (kernels_split.h): the martensite step at n = 256 with 1..4 variants (the
-// spectral rule's step; the reference rule's elastic R). GSK_SPLIT=0 in the environment when a
-// handle is created keeps that handle on the chain of y and z kernels.
+// Split-y schedule (kernels_split.h) at n = 256: the martensite step with 1..4 variants (the
+// spectral rule's step; the reference rule's elastic R) and the precipitate step with 2..5 fields
+// (both rules; the reference rule's elastic R of up to 4 order parameters). GSK_SPLIT=0 in the
+// environment when a handle is created keeps that handle on the chain of y and z kernels.
 inline bool split_wanted(const gsk_sim* s) {
   const char* e = std::getenv("GSK_SPLIT");
   const bool on = !(e && e[0] == '0');
-  return on && s->n == 256 && s->F >= 1 && s->F <= 4 && s->cfg.path == GSK_PATH_FUSED32 &&
-         s->cfg.physics == GSK_PHYS_MART;
+  if (!on || s->n != 256 || s->cfg.path != GSK_PATH_FUSED32) return false;
+  if (s->cfg.physics == GSK_PHYS_MART) return s->F >= 1 && s->F <= 4;
+  return s->F >= 2 && s->F <= 5;
 }
 
 inline size_t nvox(const gsk_sim* s) { return (size_t)s->nx * s->ny * s->nz; }
 inline size_t hs_count(const gsk_sim* s) { return (size_t)s->n * s->n * (s->n / 2 + 1); }
 // State element of field 0 at k = 0: first main x tile (after the n/W Nyquist tiles), kx = 0, column 0.
-inline size_t k0_index(const gsk_sim* s) { return (size_t)(s->n / s->g.W) * s->F * s->n * s->g.W; }
+inline size_t k0_index(const gsk_sim* s) {
+  if (s->split)  // the first main tile (k1 = 0, kz from 0) after the Nyquist tiles, column 0
+    return (size_t)(s->sg.m / kSplitWc) * s->F * s->n * split_w<kSplitWc>();
+  return (size_t)(s->n / s->g.W) * s->F * s->n * s->g.W;
+}
 
 inline int mark(gsk_sim* s) {
   if (s->prof) {
@@ -652,6 +658,18 @@ inline void build_slots(gsk_sim* s) {
       s->slots.push_back({"y_inv_eta", F * 2 * hs});
       s->slots.push_back({"zinv_eta_sq", F * (2 * hs + rv)});  // C2R, write eta, R2C of eta^2
     }
+  } else if (s->split) {  // precipitate, split-y schedule
+    const bool ref = s->cfg.rule == GSK_RULE_REFERENCE;
+    if (ref) {  // the elastic R of the V order parameters, then the local forces with it
+      const int V = s->V, npq = V * (V + 1) / 2;
+      s->slots.push_back({"sub_fwd_sq", V * (rv + hs)});
+      s->slots.push_back({"x_contract", V * 2 * hs + npq * hs / 2});
+      s->slots.push_back({"sub_inv_R", V * (hs + rv)});
+      s->slots.push_back({"sub_local", F * (rv + hs) + V * rv});
+    }
+    s->slots.push_back({"x_update", F * 4 * hs + (ref ? 0.0 : hs / 2)});
+    s->slots.push_back({"sub_inv", F * (hs + rv)});
+    if (!ref) s->slots.push_back({"sub_local", F * (rv + hs)});  // each field read once (L2-shared)
   } else {
     if (s->cfg.rule == GSK_RULE_REFERENCE) {  // the elastic R of the V order parameters first
       const int V = s->V, npq = V * (V + 1) / 2;

@@ -397,6 +397,109 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
       opaque(b) & (TB - 1));
 }
 
+// Precipitate update in the split order, one field per work group (grid = tiles x fields):
+// E-form F_hat_f -> x FFT -> lane stage -> on the tile-ordered k-space state
+//   f = 0: dc_hat <- (dc_hat - dt M k^2 (mu_hat + (xi B(k) - S) dc_hat)) / (1 + dt M (S k^2 + kappa_c k^4))
+//   f > 0: eta_hat <- (eta_hat - dt L F_hat) / (1 + dt L kappa_eta k^2)
+// -> inverse lane stage -> inverse x FFT -> A-form field spectrum / n^3 (MODE 0). B(k) (celastic):
+// the scalar tensor of the concentration in this kernel's tile order. MODE 1: the state from the
+// E-form transform of the fields, the k = 0 mode of dc set from its FP64 sum (pk.k0). pk.adv (MODE
+// 0, may be null): the step counter the launch advances once.
+template <int N, int MODE>
+__global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split_prec(float2* __restrict__ kbuf,
+                                                                            float2* __restrict__ state,
+                                                                            const float* __restrict__ Bt, SGeo g,
+                                                                            PrecK pk, const float2* __restrict__ twN) {
+  constexpr int WC = kSplitWc, RL = edge_radix<N>(), W = split_w<WC>(), T = N * W;
+  extern __shared__ float2 smem[];
+  float2* tw = smem;
+  float2* tile = smem + N;
+  constexpr int TB = split_xthreads<N>();
+  static_assert((TB & (TB - 1)) == 0, "power-of-two work group");
+  if (MODE == 0 && pk.adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *pk.adv += 1u;
+  const int t = xs_swizzle<WC>(blockIdx.x, g.ntile), f = blockIdx.y, F = gridDim.y;
+  const XSTile<WC> s = xs_tile<WC>(t, g);
+  const int b = (int)threadIdx.x & (TB - 1), c = b % W, j = b / W, jl = c / WC;
+  const LaneStage lst(jl);
+  float2* st = state + ((long)t * F + f) * T;
+  float2 sv[RL];
+  if (MODE == 0) {
+#pragma unroll
+    for (int r = 0; r < RL; ++r) sv[r] = at32(st, (unsigned)(b + r * (N / RL) * W));
+  }
+  load_tw<N>(tw, twN);
+  __syncthreads();
+  const Lines Ls{W, W, 1, 0, W};
+  float2* fb = kbuf + f * g.kfs;
+  float2 v[RL];
+  fft_g2r<float2, N>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, v, b);
+  const float2 wt = tw[(s.k1c(c) * jl) % N];
+#pragma unroll
+  for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true, WC>(cmul(v[r], wt), lst);
+  const int ky = s.ky(c, g.m), kz = s.kz(c, g.M);
+  const int bs = opaque(b) & (TB - 1);  // the state stores recompute what the loads above computed
+  if (MODE == 1) {
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      float2 z = v[r];
+      if (f == 0 && j + r * (N / RL) == 0 && ky == 0 && kz == 0) z = make_float2(pk.k0, 0.f);
+      at32(st, (unsigned)(bs + r * (N / RL) * W)) = z;
+    }
+    return;
+  }
+  const float kyv = kcomp(ky, N, pk.kfac);
+  const float kzv = pk.kfac * (float)kz;
+  const float* Bb = Bt + (long)t * T;
+#pragma unroll
+  for (int r = 0; r < RL; ++r) {
+    const float kxv = kcomp(j + r * (N / RL), N, pk.kfac);
+    const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+    const float2 sh = sv[r], fh = v[r];
+    float2 nh;
+    if (f == 0) {
+      const float bv = pk.celastic ? at32(Bb, (unsigned)(bs + r * (N / RL) * W)) : 0.f;
+      const float a = pk.xi * bv - pk.S;
+      const float mx = fh.x + a * sh.x, my = fh.y + a * sh.y;
+      const float den = 1.f / (1.f + pk.dtM * pk.S * k2 + pk.dtMkc * k2 * k2);
+      nh = make_float2((sh.x - pk.dtM * k2 * mx) * den, (sh.y - pk.dtM * k2 * my) * den);
+    } else {
+      const float den = 1.f / (1.f + pk.dtLke * k2);
+      nh = make_float2((sh.x - pk.dtL * fh.x) * den, (sh.y - pk.dtL * fh.y) * den);
+    }
+    at32(st, (unsigned)(bs + r * (N / RL) * W)) = nh;
+    v[r] = make_float2(nh.x * pk.inv_n3, nh.y * pk.inv_n3);
+  }
+  const float2 wi = make_float2(wt.x, -wt.y);
+#pragma unroll
+  for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false, WC>(v[r], lst), wi);
+  fft_r2g<float2, N>(
+      tile, Ls, tw, v, [&](int cc, int kx, float2 val) { at32(fb, s.coff(opaque(cc) & (W - 1)) + (unsigned)kx * s.ps) = val; },
+      opaque(b) & (TB - 1));
+}
+
+// Precipitate elastic energy sum_k w(kz) B(k) |dc_hat(k)|^2 from the split-order state (field 0;
+// B in the update's tile order).
+template <int = 0>
+__global__ void r_state_el_prec_split(const float2* __restrict__ state, const float* __restrict__ Bt, SGeo g, int F,
+                                      double* __restrict__ part) {
+  __shared__ double sh[16];
+  constexpr int W = split_w<kSplitWc>();
+  const int T = g.n * W;
+  const long total = (long)g.ntile * T;
+  double acc = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const int e = (int)(i % T);
+    const long t = i / T;
+    const XSTile<kSplitWc> s = xs_tile<kSplitWc>((int)t, g);
+    const int kz = s.kz(e % W, g.M);
+    const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
+    const float2 v = state[(t * F + 0) * T + e];
+    acc += w * (double)Bt[t * T + e] * ((double)v.x * v.x + (double)v.y * v.y);
+  }
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
 // Gradient energy from the split-order state: sum_f sum_k w(kz) k^2 |eta_hat_f|^2 per field.
 template <int = 0>
 __global__ void r_state_grad_split(const float2* __restrict__ state, SGeo g, int F, float kfac,
@@ -737,6 +840,65 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_fwd(const float* __r
         z[SL::sk(2 * q + 1)] = make_float2(v.z, v.w);
       }
     }
+  }
+  __syncthreads();  // the tables
+  s.r2c();
+  __syncthreads();
+  s.template ypass<true>();
+  __syncthreads();
+  s.store(kbuf + f * g.kfs, g, rb);
+}
+
+// Local precipitate forces, per (sub-plane, field f): the FT = V + 1 real fields of the
+// sub-plane's rows (concentration fluctuation, eta_1..V) as 16-byte voxel quads -> mu_c (f = 0)
+// or F_f (prec_local4, Philox normals of the voxel quads) -> R2C along z -> y forward -> E-form
+// F_hat_f in kbuf. R (may be null): the reference rule's real elastic R of the order parameters,
+// added to F_f as el R_f eta_f; cnoise: the reference rule's noise on mu_c. Item 8 FT q + 8 f + k
+// is field f of sub-plane 8q + k, so the FT items of a sub-plane, which read the same voxels, run
+// at about the same time on one XCD (work-group index mod 8) and share its L2.
+template <int N, int FT>
+__global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_local(const float* __restrict__ fld, float2* __restrict__ kbuf,
+                                                    SGeo g, PrecK pk, const float* __restrict__ R, float el,
+                                                    float cnoise, const float2* __restrict__ twN,
+                                                    const float2* __restrict__ twM, const float2* __restrict__ rtn) {
+  using SL = SubLds<N>;
+  constexpr int Q = SL::Q, PP = SL::PP, Y = SL::Y, RW = SL::RW;
+  static_assert(Q == 64, "one voxel quad per lane and row");
+  extern __shared__ float2 smem[];
+  const SL s(smem);
+  const int it = blockIdx.x, rem = it % (8 * FT), sp = it / (8 * FT) * 8 + rem % 8, f = rem / 8;
+  const int x = sp / kSplitP, j = sp % kSplitP;
+  const unsigned rb = (unsigned)(x * N + j * Y);
+  const float* xb = fld + (size_t)x * N * N;  // the x-plane's voxels
+  const uint32_t step = step_of(pk.step, pk.stepp);
+  s.tables(twN, twM, rtn);
+  const int q = s.lane;
+  for (int i = 0; i < RW; ++i) {  // the wave's rows
+    const int t = s.w * RW + i;
+    float2* z = s.rows() + i * PP;
+    const unsigned vo = (unsigned)((kSplitP * t + j) * N + 4 * q);  // voxel offset inside the x-plane
+    float e[FT][4];
+#pragma unroll
+    for (int u = 0; u < FT; ++u) {
+      const float4 v = at32(reinterpret_cast<const float4*>(xb + (size_t)u * g.nvox), vo / 4);
+      e[u][0] = v.x; e[u][1] = v.y; e[u][2] = v.z; e[u][3] = v.w;
+    }
+    const uint32_t gq = (uint32_t)(((size_t)x * N * N + vo) >> 2);
+    float xi[4] = {0.f, 0.f, 0.f, 0.f};
+    float rk[4] = {0.f, 0.f, 0.f, 0.f};
+    if (f == 0) {
+      if (cnoise != 0.f) normal4(pk.seed, gq, 0x8000u, step, xi);
+    } else {
+      if (pk.noise != 0.f) normal4(pk.seed, gq, (uint32_t)(f - 1), step, xi);
+      if (R) {
+        const float4 rq = at32(reinterpret_cast<const float4*>(R + (size_t)(f - 1) * g.nvox + (size_t)x * N * N), vo / 4);
+        rk[0] = rq.x; rk[1] = rq.y; rk[2] = rq.z; rk[3] = rq.w;
+      }
+    }
+    float o[4];
+    prec_local4<FT>(pk, f, e, xi, cnoise, R != nullptr, rk, el, o);
+    z[SL::sk(2 * q)] = make_float2(o[0], o[1]);
+    z[SL::sk(2 * q + 1)] = make_float2(o[2], o[3]);
   }
   __syncthreads();  // the tables
   s.r2c();

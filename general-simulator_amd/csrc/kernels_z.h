@@ -621,6 +621,51 @@ __global__ void __launch_bounds__(zthreads<M>()) k_zfwd_local(const float* __res
   for (int f = 0; f < F; ++f) z_store_spec<M>(tile + f * L * zpitch(M), kout + f * g.kfs, g, r0, rt);
 }
 
+// Local precipitate force of output field f at four voxels from the FT = V + 1 field values
+// e[u][k] (u = 0: the concentration fluctuation about cbar, u = 1..V: the order parameters):
+//   f = 0: mu_c = A2/2 sum_a eta_a^2 + A1 (c - c1) + cnoise xi
+//   f > 0: F_f = (2 A5 eta + 4 A6 eta^3) t2 + 2 A6 eta t4 + 2 A7 eta t22 + A2 eta (c - c2) - A3 eta^3
+//                + A4 eta^5 + noise xi [+ el R eta]
+// with t2, t4 the sums of eta_b^2, eta_b^4 over b != f and t22 = t2^2 - t4 (ordered pairs of other
+// variants). xi: the normal deviates of field f's noise block; rk (hasR): the elastic R of eta_f.
+template <int FT>
+__device__ __forceinline__ void prec_local4(const PrecK& pk, int f, const float (&e)[FT][4], const float (&xi)[4],
+                                            float cnoise, bool hasR, const float (&rk)[4], float el, float (&out)[4]) {
+  constexpr int V = FT - 1;
+  if (f == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float s2 = 0.f;
+#pragma unroll
+      for (int a = 1; a <= V; ++a) s2 += e[a][k] * e[a][k];
+      out[k] = 0.5f * pk.A2 * s2 + pk.A1 * (pk.cbar + e[0][k] - pk.c1) + cnoise * xi[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float et = 0.f, t2 = 0.f, t4 = 0.f;
+#pragma unroll
+      for (int b = 1; b <= V; ++b) {
+        const float x = e[b][k];
+        if (b == f) {
+          et = x;
+        } else {
+          const float qq = x * x;
+          t2 += qq;
+          t4 += qq * qq;
+        }
+      }
+      const float t22 = t2 * t2 - t4;
+      const float c = pk.cbar + e[0][k];
+      const float e2 = et * et;
+      float o = (2.f * pk.A5 * et + 4.f * pk.A6 * e2 * et) * t2 + 2.f * pk.A6 * et * t4 + 2.f * pk.A7 * et * t22 +
+                pk.A2 * et * (c - pk.c2) - pk.A3 * e2 * et + pk.A4 * e2 * e2 * et + pk.noise * xi[k];
+      if (hasR) o += el * rk[k] * et;
+      out[k] = o;
+    }
+  }
+}
+
 // Waves-per-SIMD floor of the register-resident precipitate z passes: 4 (at most 128 VGPRs)
 // for n >= 256.
 template <int M, int FT> constexpr int zlocal_wpe() { return M >= 128 ? 4 : 1; }
@@ -636,7 +681,7 @@ __device__ __forceinline__ void zlocal_forces_r(const float4 (&v)[ZShape<M>::NT]
                                                 const PrecK& pk, long r0, const float* __restrict__ R, float el,
                                                 float cnoise) {
   using S = ZShape<M>;
-  constexpr int n = 2 * M, V = FT - 1;
+  constexpr int n = 2 * M;
   const Lines Ls{S::L, S::L, zpitch(M), 0, 1};
   auto no_ld = [](int, int) { return float2{}; };
   auto no_st = [](int, int, float2) {};
@@ -659,46 +704,19 @@ __device__ __forceinline__ void zlocal_forces_r(const float4 (&v)[ZShape<M>::NT]
         e[u][0] = v[i][u].x; e[u][1] = v[i][u].y; e[u][2] = v[i][u].z; e[u][3] = v[i][u].w;
       }
       float out[4];
+      float xi[4] = {0.f, 0.f, 0.f, 0.f};
+      float rk[4] = {0.f, 0.f, 0.f, 0.f};
       if (f == 0) {
-        float xc[4] = {0.f, 0.f, 0.f, 0.f};
-        if (cnoise != 0.f) normal4(pk.seed, gq, 0x8000u, step, xc);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          float s2 = 0.f;
-#pragma unroll
-          for (int a = 1; a <= V; ++a) s2 += e[a][k] * e[a][k];
-          out[k] = 0.5f * pk.A2 * s2 + pk.A1 * (pk.cbar + e[0][k] - pk.c1) + cnoise * xc[k];
-        }
+        if (cnoise != 0.f) normal4(pk.seed, gq, 0x8000u, step, xi);
       } else {
-        float xi[4] = {0.f, 0.f, 0.f, 0.f};
         if (pk.noise != 0.f) normal4(pk.seed, gq, (uint32_t)(f - 1), step, xi);
-        float4 rq = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (R) rq = at32(reinterpret_cast<const float4*>(R + (f - 1) * g.nvox + r0 * n), (unsigned)(l * n + 4 * q) / 4);
-        const float rk[4] = {rq.x, rq.y, rq.z, rq.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          float et = 0.f, t2 = 0.f, t4 = 0.f;
-#pragma unroll
-          for (int b = 1; b <= V; ++b) {
-            const float x = e[b][k];
-            if (b == f) {
-              et = x;
-            } else {
-              const float qq = x * x;
-              t2 += qq;
-              t4 += qq * qq;
-            }
-          }
-          // ordered pairs i != j, both != f: (sum q)^2 - sum q^2
-          const float t22 = t2 * t2 - t4;
-          const float c = pk.cbar + e[0][k];
-          const float e2 = et * et;
-          float o = (2.f * pk.A5 * et + 4.f * pk.A6 * e2 * et) * t2 + 2.f * pk.A6 * et * t4 + 2.f * pk.A7 * et * t22 +
-                    pk.A2 * et * (c - pk.c2) - pk.A3 * e2 * et + pk.A4 * e2 * e2 * et + pk.noise * xi[k];
-          if (R) o += el * rk[k] * et;
-          out[k] = o;
+        if (R) {
+          const float4 rq =
+              at32(reinterpret_cast<const float4*>(R + (f - 1) * g.nvox + r0 * n), (unsigned)(l * n + 4 * q) / 4);
+          rk[0] = rq.x; rk[1] = rq.y; rk[2] = rq.z; rk[3] = rq.w;
         }
       }
+      prec_local4<FT>(pk, f, e, xi, cnoise, R != nullptr, rk, el, out);
       float2* row = tile + l * zpitch(M);
       row[2 * q] = make_float2(out[0], out[1]);
       row[2 * q + 1] = make_float2(out[2], out[3]);

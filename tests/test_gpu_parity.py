@@ -306,12 +306,14 @@ def test_mart_square_carry_bit_identical(gpu, n):
     assert np.array_equal(ea.view(np.uint32), eb.view(np.uint32))
 
 
-def test_prec_local_force_carry_bit_identical(gpu):
-    """The precipitate step's last z pass also forms the next step's local forces (with the
-    next step's noise). Steps back to back must give bit for bit the trajectory of steps
-    whose carried-over forces are dropped in between (a parameter set to its own value)."""
-    say("prec local-force carry-over n=64")
-    a, b = prec_sim(64), prec_sim(64)
+@pytest.mark.parametrize("n", [64, 256])
+def test_prec_local_force_carry_bit_identical(gpu, n):
+    """The precipitate step's last pass also forms the next step's local forces (with the next
+    step's noise): the last z pass at n = 64, the sub-plane local-force kernel of the split-y
+    schedule at 256. Steps back to back must give bit for bit the trajectory of steps whose
+    carried-over forces are dropped in between (a parameter set to its own value)."""
+    say(f"prec local-force carry-over n={n}")
+    a, b = prec_sim(n), prec_sim(n)
     for s in (a, b):
         s.init_uniform("concentration", 0.245, 0.255, 11)
         s.init_uniform("eta", 0.0, 0.01, 12)
@@ -379,7 +381,7 @@ def prec_init(s, seed=11):
     s.init_uniform("eta", 0.0, 0.01, seed + 1)
 
 
-@pytest.mark.parametrize("n,steps", [(16, 50), (32, 200), (96, 20)])
+@pytest.mark.parametrize("n,steps", [(16, 50), (32, 200), (96, 20), (256, 3)])
 def test_prec_fused32_matches_oracle(gpu, n, steps):
     say(f"prec fused n={n} steps={steps}")
     s = prec_sim(n)
@@ -407,6 +409,34 @@ def test_prec_fused32_matches_oracle(gpu, n, steps):
     assert abs(cw.sum() - c0.sum()) / c0.sum() < 1e-13
     Ew = O.prec_energy(cw, ew, Bc, p)
     np.testing.assert_allclose(E, Ew, rtol=1e-5, atol=1e-9 * abs(Ew[3]))
+
+
+def test_prec_split_schedule_matches_chain_at_256(gpu, monkeypatch):
+    """At 256^3 the precipitate step runs the split-y schedule (x update, sub-plane inverse,
+    sub-plane local forces: three launches) unless GSK_SPLIT=0 is set when the handle is
+    created (the chain of y and z kernels, four launches). Two FP32 orderings of one step: 20
+    steps with noise agree to FP32 rounding in c, eta and the energies; both conserve sum c to the
+    last bit."""
+    say("prec split-y schedule vs y/z chain n=256")
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("GSK_SPLIT", mode)
+        s = prec_sim(256, noise=0.01)
+        prec_init(s)
+        m0 = s.mass()
+        s.step(20)
+        out[mode] = (s.get_field("concentration", np.float64), s.get_field("eta", np.float64), s.energy(), m0,
+                     s.mass())
+        names, _ = s.kernel_times(1)
+        s.close()
+        assert ("sub_local" in names) == (mode == "1"), names
+    (ca, ea, Ea, m0a, m1a), (cb, eb, Eb, m0b, m1b) = out["1"], out["0"]
+    assert np.isfinite(ea).all() and np.isfinite(ca).all()
+    assert m1a == m0a and m1b == m0b and m0a == m0b
+    assert float(np.abs(ca - cb).max()) < 1e-6
+    assert float(np.abs(ea - eb).max()) < 2e-6 * float(np.abs(eb).max())
+    assert rel_l2(ea, eb) < 1e-6
+    np.testing.assert_allclose(Ea, Eb, rtol=1e-5, atol=1e-9 * abs(Eb[3]))
 
 
 def test_prec_mass_conservation_at_256(gpu):

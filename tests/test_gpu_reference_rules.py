@@ -155,6 +155,37 @@ def test_prec_reference_rule_matches_oracle(gpu):
     assert m1 == m0
 
 
+def test_prec_reference_rule_split_matches_chain_at_256(gpu, monkeypatch):
+    """The precipitate reference rule at n = 256 through the split-y kernels (R of the eta^2
+    source in three launches, local forces with it in the sub-plane kernel, x update, sub-plane
+    inverse) and through the chain of z, y and x kernels (GSK_SPLIT=0): 10 steps with noise agree
+    to FP32 rounding, and sum c is conserved by both."""
+    say("prec reference rule n=256: split vs chain")
+    C = gsk.cubic_modulus(3.5, 1.5, 1.0)
+    eps = gsk.mart_eigenstrains(3, base=0.0, axis=0.02)
+    cfg = gsk.reference_config(gsk.PHYS_PREC, n=256, nvar=3, M=1.0, L=1.0, kappa=1.5, kappa_c=1.5, dt=0.002,
+                               seed=8, xi=50.0, noise=0.01)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("GSK_SPLIT", mode)
+        with gsk.Sim(cfg) as s:
+            s.build_btensor(C, eps)
+            s.init_uniform("concentration", 0.245, 0.255, 11)
+            s.init_uniform("eta", 0.0, 0.1, 12)
+            m0 = s.mass()
+            s.step(10)
+            out[mode] = (s.get_field("concentration", np.float64), s.get_field("eta", np.float64), m0, s.mass())
+            names, _ = s.kernel_times(1)
+        assert ("sub_inv_R" in names) == (mode == "1"), names
+    (ca, ea, m0a, m1a), (cb, eb, m0b, m1b) = out["1"], out["0"]
+    say(f"  max|d eta| {np.abs(ea - eb).max():.2e}  max|d c| {np.abs(ca - cb).max():.2e}")
+    assert np.isfinite(ea).all() and np.isfinite(ca).all()
+    assert m1a == m0a and m1b == m0b
+    assert float(np.abs(ca - cb).max()) < 1e-6
+    assert float(np.abs(ea - eb).max()) < 2e-6 * float(np.abs(eb).max())
+    assert rel(ea, eb) < 1e-6
+
+
 def test_reference_rule_errors(gpu):
     cfg = gsk.reference_config(gsk.PHYS_MART, n=16, nvar=2)
     with gsk.Sim(cfg) as s:
