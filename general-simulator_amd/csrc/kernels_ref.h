@@ -98,11 +98,24 @@ __global__ void k_ref_mart(const float* __restrict__ eta, float* __restrict__ RE
   }
 }
 
+// 16-byte non-temporal load and store (data read or written once, kept out of the caches' way)
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_nt4(const float4* p) {
+  const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st_nt4(float4* p, float4 v) {
+  const nt_f4 w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<nt_f4*>(p));
+}
+
 // The same update for V = 1..4 variants with 16-byte loads and stores: a chunk is 256
 // consecutive quads (1024 voxels) per work group. The chunks are split into eight contiguous
 // ranges, one per XCD (work group index mod 8, the order in which work groups are dealt to the
 // XCDs), so the neighbouring rows and planes that a work group reads are the rows and planes
-// the other work groups of its XCD read at about the same time, out of that XCD's L2.
+// the other work groups of its XCD read at about the same time, out of that XCD's L2. R, K, the
+// defect mask and the new eta are read or written once: non-temporal accesses keep them from
+// evicting the eta planes that are read again as neighbours.
 // ZS: the z rows are L = nz/4 lanes of one wave (L divides 64) and every chunk is whole: the z
 // neighbours across a quad boundary come from the neighbouring lane (a lane shuffle, periodic in
 // the row) instead of a load. Work groups: a multiple of 8, each writes fpart[blockIdx.x*4 + j].
@@ -139,13 +152,15 @@ __global__ void __launch_bounds__(256) k_ref_mart_q(const float* __restrict__ et
         S1[j] += ev[j];
       }
     }
-    const float4 D = defect ? reinterpret_cast<const float4*>(defect)[qd] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 D =
+        defect ? ld_nt4(reinterpret_cast<const float4*>(defect) + qd) : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int v = 0; v < V; ++v) {
       const unsigned o = (unsigned)v * nq;
       const float4 xp = e4[o + qxp], xm = e4[o + qxm], yp = e4[o + qyp], ym = e4[o + qym];
-      const float4 R = r4[o + qd];
-      const float4 Kv = K ? reinterpret_cast<const float4*>(K)[o + qd] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 R = ld_nt4(r4 + o + qd);
+      const float4 Kv =
+          K ? ld_nt4(reinterpret_cast<const float4*>(K) + o + qd) : make_float4(0.f, 0.f, 0.f, 0.f);
       float zlo, zhi;  // eta at z0 - 1 and z0 + 4 (periodic)
       if (ZS) {
         zlo = __shfl(E[v].w, (int)(rowb + ((lr - 1u) & (L - 1u))), 64);
@@ -176,7 +191,7 @@ __global__ void __launch_bounds__(256) k_ref_mart_q(const float* __restrict__ et
         nv.z *= 1.f - D.z;
         nv.w *= 1.f - D.w;
       }
-      r4[o + qd] = nv;
+      st_nt4(r4 + o + qd, nv);
     }
   }
   if (fpart) {
