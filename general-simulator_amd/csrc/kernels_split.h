@@ -870,6 +870,7 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_local(const float* _
   const int x = sp / kSplitP, j = sp % kSplitP;
   const unsigned rb = (unsigned)(x * N + j * Y);
   const float* xb = fld + (size_t)x * N * N;  // the x-plane's voxels
+  const unsigned nv = (unsigned)g.nvox;
   const uint32_t step = step_of(pk.step, pk.stepp);
   s.tables(twN, twM, rtn);
   const int q = s.lane;
@@ -879,11 +880,11 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_local(const float* _
     const unsigned vo = (unsigned)((kSplitP * t + j) * N + 4 * q);  // voxel offset inside the x-plane
     float e[FT][4];
 #pragma unroll
-    for (int u = 0; u < FT; ++u) {
-      const float4 v = at32(reinterpret_cast<const float4*>(xb + (size_t)u * g.nvox), vo / 4);
+    for (int u = 0; u < FT; ++u) {  // 32-bit offsets: FT fields hold fewer than 2^31 values
+      const float4 v = at32(reinterpret_cast<const float4*>(xb), ((unsigned)u * nv + vo) / 4);
       e[u][0] = v.x; e[u][1] = v.y; e[u][2] = v.z; e[u][3] = v.w;
     }
-    const uint32_t gq = (uint32_t)(((size_t)x * N * N + vo) >> 2);
+    const uint32_t gq = (uint32_t)(((unsigned)x * N * N + vo) >> 2);
     float xi[4] = {0.f, 0.f, 0.f, 0.f};
     float rk[4] = {0.f, 0.f, 0.f, 0.f};
     if (f == 0) {
@@ -891,7 +892,7 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_local(const float* _
     } else {
       if (pk.noise != 0.f) normal4(pk.seed, gq, (uint32_t)(f - 1), step, xi);
       if (R) {
-        const float4 rq = at32(reinterpret_cast<const float4*>(R + (size_t)(f - 1) * g.nvox + (size_t)x * N * N), vo / 4);
+        const float4 rq = at32(reinterpret_cast<const float4*>(R + (size_t)x * N * N), ((unsigned)(f - 1) * nv + vo) / 4);
         rk[0] = rq.x; rk[1] = rq.y; rk[2] = rq.z; rk[3] = rq.w;
       }
     }
