@@ -59,6 +59,46 @@ def test_mart_reference_rule_matches_oracle(gpu, chem_form, noise):
     assert err < 2e-5
 
 
+@pytest.mark.parametrize("n,V", [(16, 6), (32, 2)])
+def test_mart_reference_rule_variant_counts(gpu, n, V):
+    """Variant counts beyond the 16-byte update kernel's 1..4 (V = 6: the general kernel,
+    `k_ref_mart`) and another one inside it (V = 2), with noise, defects and dislocation stress,
+    against the FP64 oracle. Eigenstrains of variants p and p + 3 share an axis."""
+    say(f"mart reference rule n={n} V={V}")
+    steps = 20
+    C = gsk.cubic_modulus(3.5, 1.5, 1.0)
+    rng = np.random.default_rng(3)
+    eps = gsk.mart_eigenstrains(V, base=0.0, axis=0.01) + 0.002 * rng.standard_normal((V, 3, 3))
+    eps = 0.5 * (eps + eps.transpose(0, 2, 1))
+    cfg = gsk.reference_config(gsk.PHYS_MART, n=n, nvar=V, temperature=300.0, noise=1e-4, w_el=1e3, seed=9,
+                               dt=0.005)
+    defect = (rng.random((n, n, n)) < 0.02).astype(np.float32)
+    stress = rng.uniform(-0.01, 0.01, (6, n, n, n)).astype(np.float32)
+    with gsk.Sim(cfg) as s:
+        s.build_btensor(C, eps)
+        s.init_uniform("eta", 0.0, 0.2, 5)
+        s.set_field("defect", defect)
+        s.set_field("dislocationstress", stress)
+        eta0 = s.get_field("eta").astype(np.float64)
+        s.step(steps)
+        got = s.get_field("eta", np.float64)
+        forces = s.mart_forces()
+    a1 = 0.02 * (cfg.temperature - cfg.T0)
+    p = O.mart_ref_params(a1=a1, a2=cfg.B, a3=cfg.C, w_grad=cfg.w_grad, w_chem=cfg.w_chem, w_el=cfg.w_el,
+                          w_dis=cfg.w_dis, w_noise=cfg.noise, dt=cfg.dt, dx=cfg.dx, kconv=cfg.kconv,
+                          chem_form=cfg.chem_form, seed=cfg.seed)
+    comp = [0, 1, 2, 1, 3, 4, 2, 4, 5]
+    sig = stress.astype(np.float64)[comp].reshape(3, 3, n, n, n)
+    K = 25.0 * np.einsum("ijxyz,pij->pxyz", sig, eps)
+    want = O.mart_ref_run(eta0, O.btensor(n, C, eps, kconv=cfg.kconv), p, steps, defect=defect.astype(np.float64),
+                          K=K)
+    err = rel(got, want)
+    say(f"  rel L2 {err:.2e}")
+    assert np.isfinite(got).all() and np.isfinite(forces).all()
+    assert np.all(got[:, defect > 0] == 0.0)
+    assert err < 2e-5
+
+
 def _mart_ref_256(noise):
     C = gsk.cubic_modulus(3.5, 1.5, 1.0)
     eps = gsk.mart_eigenstrains(3, base=0.0, axis=0.01)
