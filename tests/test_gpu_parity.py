@@ -234,18 +234,20 @@ def test_mart_translation_equivariance_at_256(gpu):
     assert np.abs(np.roll(a, sh, axis=(1, 2, 3)) - b).max() < 1e-5 * np.abs(a).max()
 
 
-def test_split_schedule_matches_chain_at_256(gpu, monkeypatch):
+@pytest.mark.parametrize("V", [3, 1, 2, 4])
+def test_split_schedule_matches_chain_at_256(gpu, monkeypatch, V):
     """At 256^3 the fused martensite step runs the split-y schedule (four launches; sub-plane
     kernels keep 64 y-rows in LDS through y, z and y) unless GSK_SPLIT=0 is set when the handle
     is created, which keeps the chain of y and z kernels (eight launches). The two FP32
     schedules order their arithmetic differently but follow one trajectory: 20 steps of strong
-    dynamics with noise agree to FP32 rounding, and so do the energies."""
-    say("split-y schedule vs y/z chain n=256")
+    dynamics with noise agree to FP32 rounding, and so do the energies. Every variant count of
+    the split kernels (1..4; the contraction keeps V - 1 fields in registers and one in LDS)."""
+    say(f"split-y schedule vs y/z chain n=256 V={V}")
     kw = dict(A=-0.5, w_el=20.0, noise=0.002, seed=9)
     out = {}
     for mode in ("1", "0"):
         monkeypatch.setenv("GSK_SPLIT", mode)
-        s = mart_sim(256, 3, **kw)
+        s = mart_sim(256, V, **kw)
         s.init_uniform("eta", -0.05, 0.05, 6)
         s.step(20)
         out[mode] = (s.get_field("eta", np.float64), s.energy())
@@ -411,17 +413,18 @@ def test_prec_fused32_matches_oracle(gpu, n, steps):
     np.testing.assert_allclose(E, Ew, rtol=1e-5, atol=1e-9 * abs(Ew[3]))
 
 
-def test_prec_split_schedule_matches_chain_at_256(gpu, monkeypatch):
+@pytest.mark.parametrize("V", [4, 1, 2])
+def test_prec_split_schedule_matches_chain_at_256(gpu, monkeypatch, V):
     """At 256^3 the precipitate step runs the split-y schedule (x update, sub-plane inverse,
     sub-plane local forces: three launches) unless GSK_SPLIT=0 is set when the handle is
     created (the chain of y and z kernels, four launches). Two FP32 orderings of one step: 20
     steps with noise agree to FP32 rounding in c, eta and the energies; both conserve sum c to the
-    last bit."""
-    say("prec split-y schedule vs y/z chain n=256")
+    last bit. V order parameters: 2..5 fields of the local-force kernel."""
+    say(f"prec split-y schedule vs y/z chain n=256 V={V}")
     out = {}
     for mode in ("1", "0"):
         monkeypatch.setenv("GSK_SPLIT", mode)
-        s = prec_sim(256, noise=0.01)
+        s = prec_sim(256, V=V, noise=0.01)
         prec_init(s)
         m0 = s.mass()
         s.step(20)
