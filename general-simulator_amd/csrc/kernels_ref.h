@@ -112,10 +112,11 @@ __device__ __forceinline__ void st_nt4(float4* p, float4 v) {
 // The same update for V = 1..4 variants with 16-byte loads and stores: a chunk is 256
 // consecutive quads (1024 voxels) per work group. The chunks are split into eight contiguous
 // ranges, one per XCD (work group index mod 8, the order in which work groups are dealt to the
-// XCDs), so the neighbouring rows and planes that a work group reads are the rows and planes
-// the other work groups of its XCD read at about the same time, out of that XCD's L2. R, K, the
-// defect mask and the new eta are read or written once: non-temporal accesses keep them from
-// evicting the eta planes that are read again as neighbours.
+// XCDs). Chunks are numbered in-plane position first, plane second (when a plane holds whole
+// chunks), so the work groups of an XCD at one time hold the same rows of consecutive planes: the
+// x neighbours of a chunk are read by the groups beside it, the y neighbours by the previous and
+// next sweep, out of that XCD's L2. R, K, the defect mask and the new eta are read or written
+// once: non-temporal accesses keep them from evicting the eta that is read again as neighbours.
 // ZS: the z rows are L = nz/4 lanes of one wave (L divides 64) and every chunk is whole: the z
 // neighbours across a quad boundary come from the neighbouring lane (a lane shuffle, periodic in
 // the row) instead of a load. Work groups: a multiple of 8, each writes fpart[blockIdx.x*4 + j].
@@ -132,8 +133,12 @@ __global__ void __launch_bounds__(256) k_ref_mart_q(const float* __restrict__ et
   const float4* e4 = reinterpret_cast<const float4*>(eta);
   float4* r4 = reinterpret_cast<float4*>(RE);
   const unsigned lane = threadIdx.x & 63u, lr = lane & (L - 1u), rowb = lane - lr;
+  // chunk order: the CP chunks of a plane (CP = qp / 256 when whole) are walked plane by plane for
+  // each in-plane chunk, so that consecutive chunks are x-neighbours
+  const unsigned CP = qp % 256u == 0u ? qp / 256u : 0u;
   for (unsigned c = (unsigned)((unsigned long long)C * xcd / 8u) + blockIdx.x / 8u; c < c1; c += G8) {
-    const unsigned qd = c * 256u + threadIdx.x;
+    const unsigned ch = CP ? (c % (unsigned)nx) * CP + c / (unsigned)nx : c;
+    const unsigned qd = ch * 256u + threadIdx.x;
     if (!ZS && qd >= nq) continue;
     const unsigned x = qd / qp, rem = qd - x * qp, y = rem / L, zq = rem - y * L;
     const unsigned qxp = x + 1u == (unsigned)nx ? qd - x * qp : qd + qp;
