@@ -157,14 +157,17 @@ def test_mart_reference_rule_split_matches_chain_at_256(gpu, monkeypatch):
             s.build_btensor(C, eps)
             s.init_uniform("eta", 0.0, 0.2, 8)
             s.step(10)
-            out[mode] = s.get_field("eta", np.float64)
+            out[mode] = (s.get_field("eta", np.float64), s.mart_forces())
             names, _ = s.kernel_times(1)
         assert ("sub_inv_R" in names) == (mode == "1"), names
         assert ("zinv_R" in names) == (mode == "0"), names
-    a, b = out["1"], out["0"]
+    (a, fa), (b, fb) = out["1"], out["0"]
     assert np.isfinite(a).all()
     assert float(np.abs(a - b).max()) < 2e-6 * float(np.abs(b).max())
     assert rel(a, b) < 1e-6
+    # the four force means of the last step (kernel_times ran one more step on each handle)
+    say(f"  force means split {fa}, chain {fb}")
+    assert np.all(np.abs(fa - fb) <= 1e-4 * np.abs(fb).max())
 
 
 def test_prec_reference_rule_matches_oracle(gpu):
