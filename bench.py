@@ -129,8 +129,8 @@ def repeat_stats(walls, steps):
 
 
 def cpu_threads():
-    """Threads the host offers this process: OMP_NUM_THREADS when set (the GPU box sets it to its
-    CPU share), else the CPUs in the process's affinity mask."""
+    """Threads the host offers this process: OMP_NUM_THREADS when set (a shared machine may set it
+    to the process's CPU share), else the CPUs in the process's affinity mask."""
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:
         return int(env)
