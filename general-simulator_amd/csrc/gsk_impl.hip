@@ -248,13 +248,13 @@ struct Impl {
       if (!in) in = s->fld;
       const dim3 grid(N * kSplitP, count > 0 ? count : s->F);
       if (square)
-        LAUNCH((k_sub_fwd<N, 1>), grid, kSubT, sub_lds<N>(), s, in, s->kbuf, s->sg, s->twN, s->twM, s->rtn);
+        LAUNCH((k_sub_fwd<N, 1>), grid, sub_threads<N>(), sub_lds<N>(), s, in, s->kbuf, s->sg, s->twN, s->twM, s->rtn);
       else
-        LAUNCH((k_sub_fwd<N, 0>), grid, kSubT, sub_lds<N>(), s, in, s->kbuf, s->sg, s->twN, s->twM, s->rtn);
+        LAUNCH((k_sub_fwd<N, 0>), grid, sub_threads<N>(), sub_lds<N>(), s, in, s->kbuf, s->sg, s->twN, s->twM, s->rtn);
       return GSK_OK;
     } else {
       (void)s; (void)square; (void)in; (void)count;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
     }
   }
   // the contraction over the first `count` spectra of the k-space buffer (the handle's F by default)
@@ -273,7 +273,7 @@ struct Impl {
       return GSK_OK;
     } else {
       (void)s; (void)mk; (void)count;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
     }
   }
   template <int MODE>
@@ -284,7 +284,7 @@ struct Impl {
       return GSK_OK;
     } else {
       (void)s; (void)mk;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
     }
   }
   static int step_mart_split(gsk_sim* s) {
@@ -299,19 +299,19 @@ struct Impl {
       if (!s->sq_ready) CHK(split_fwd(s, true));
       CHK(split_xcontract<0>(s, mk));
       switch (s->F) {
-        case 1: LAUNCH((k_sub_mid<N, 1>), gm, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn); break;
-        case 2: LAUNCH((k_sub_mid<N, 2>), gm, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn); break;
-        case 3: LAUNCH((k_sub_mid<N, 3>), gm, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn); break;
-        default: LAUNCH((k_sub_mid<N, 4>), gm, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn);
+        case 1: LAUNCH((k_sub_mid<N, 1>), gm, sub_threads<N>(), sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn); break;
+        case 2: LAUNCH((k_sub_mid<N, 2>), gm, sub_threads<N>(), sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn); break;
+        case 3: LAUNCH((k_sub_mid<N, 3>), gm, sub_threads<N>(), sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn); break;
+        default: LAUNCH((k_sub_mid<N, 4>), gm, sub_threads<N>(), sub_lds<N>(), s, s->kbuf, s->fld, s->sg, mk, s->twN, s->twM, s->rtn);
       }
       CHK(split_xupdate<0>(s, mk));
-      LAUNCH((k_sub_sq<N>), gq, kSubT, sub_lds<N>(), s, s->kbuf, s->fld, s->sg, s->twN, s->twM, s->rtn, s->dstep,
+      LAUNCH((k_sub_sq<N>), gq, sub_threads<N>(), sub_lds<N>(), s, s->kbuf, s->fld, s->sg, s->twN, s->twM, s->rtn, s->dstep,
              nitems);
       s->sq_ready = true;
       return GSK_OK;
     } else {
       (void)s;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
     }
   }
 
@@ -325,7 +325,7 @@ struct Impl {
       return GSK_OK;
     } else {
       (void)s; (void)pk;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
     }
   }
   static int split_local(gsk_sim* s, const PrecK& pk, const float* R, float el, float cn) {
@@ -333,15 +333,15 @@ struct Impl {
       const int items = N * kSplitP * s->F;
       const SGeo& g = s->sg;
       switch (s->F) {
-        case 2: LAUNCH((k_sub_local<N, 2>), items, kSubT, sub_lds<N>(), s, s->fld, s->kbuf, g, pk, R, el, cn, s->twN, s->twM, s->rtn); break;
-        case 3: LAUNCH((k_sub_local<N, 3>), items, kSubT, sub_lds<N>(), s, s->fld, s->kbuf, g, pk, R, el, cn, s->twN, s->twM, s->rtn); break;
-        case 4: LAUNCH((k_sub_local<N, 4>), items, kSubT, sub_lds<N>(), s, s->fld, s->kbuf, g, pk, R, el, cn, s->twN, s->twM, s->rtn); break;
-        default: LAUNCH((k_sub_local<N, 5>), items, kSubT, sub_lds<N>(), s, s->fld, s->kbuf, g, pk, R, el, cn, s->twN, s->twM, s->rtn);
+        case 2: LAUNCH((k_sub_local<N, 2>), items, sub_threads<N>(), sub_lds<N>(), s, s->fld, s->kbuf, g, pk, R, el, cn, s->twN, s->twM, s->rtn); break;
+        case 3: LAUNCH((k_sub_local<N, 3>), items, sub_threads<N>(), sub_lds<N>(), s, s->fld, s->kbuf, g, pk, R, el, cn, s->twN, s->twM, s->rtn); break;
+        case 4: LAUNCH((k_sub_local<N, 4>), items, sub_threads<N>(), sub_lds<N>(), s, s->fld, s->kbuf, g, pk, R, el, cn, s->twN, s->twM, s->rtn); break;
+        default: LAUNCH((k_sub_local<N, 5>), items, sub_threads<N>(), sub_lds<N>(), s, s->fld, s->kbuf, g, pk, R, el, cn, s->twN, s->twM, s->rtn);
       }
       return GSK_OK;
     } else {
       (void)s; (void)pk; (void)R; (void)el; (void)cn;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
     }
   }
   // y inverse and C2R of the first `count` A-form spectra into real fields at out; adv (may be
@@ -350,12 +350,12 @@ struct Impl {
     if constexpr (split_size<N>()) {
       if (!s->cus) HIPCHK(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->dev));
       const int nitems = N * kSplitP * count, gq = std::min(nitems, std::max(8, 2 * s->cus / 8 * 8));
-      LAUNCH((k_sub_sq<N, false>), gq, kSubT, sub_lds<N>(), s, s->kbuf, out, s->sg, s->twN, s->twM, s->rtn, adv,
+      LAUNCH((k_sub_sq<N, false>), gq, sub_threads<N>(), sub_lds<N>(), s, s->kbuf, out, s->sg, s->twN, s->twM, s->rtn, adv,
              nitems);
       return GSK_OK;
     } else {
       (void)s; (void)out; (void)count; (void)adv;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
     }
   }
   // Spectral precipitate step, split-y schedule (three launches): x update of the carried local

@@ -54,6 +54,9 @@ namespace gsk {
 
 constexpr int kSplitP = 4;  // sub-planes per x-plane
 constexpr int kSubT = GSK_SUB_T;
+// Threads of a sub-plane work group: kSubT at n = 256; 384 at n = 192 (six waves, so that each
+// wave owns 16 of the 96 main columns and 8 of the 48 rows).
+template <int N> constexpr int sub_threads() { return N == 192 ? 384 : kSubT; }
 constexpr int kSplitWc = GSK_SPLIT_WC;  // x update (and the k-space state's tile order)
 // Contraction's x transform and tile: GSK_XC_PLAN 0 (the edge plan 16 * 16 and kSplitWc columns
 // per sub-plane group: 16 values of each field per thread), 1 (8 * 8 * 4 and 4 columns: 4 values
@@ -67,7 +70,7 @@ constexpr int kSplitWcC = GSK_XC_PLAN >= 1 ? 4 : kSplitWc;
 template <int WC> constexpr int split_w() { return kSplitP * WC; }
 static_assert(kSplitWc == 4 || kSplitWc == 8 || kSplitWc == 16, "sub-plane group width 4, 8 or 16");
 
-template <int N> constexpr bool split_size() { return N == 256; }
+template <int N> constexpr bool split_size() { return N == 256 || N == 192; }
 
 // Geometry of the split schedule.
 struct SGeo {
@@ -176,8 +179,15 @@ template <bool FWD, int WC> __device__ __forceinline__ float2 lane_dft4(float2 v
   return v;
 }
 
-// x update: the edge plan (16 * 16 at n = 256), kSplitWc columns per sub-plane group
-template <int N> constexpr int split_xthreads() { return split_w<kSplitWc>() * (N / edge_radix<N>()); }
+// x update: the edge plan (16 * 16 at n = 256, 12 * 16 at 192), kSplitWc columns per sub-plane
+// group, one thread per butterfly of the plan's smallest radix. The last forward pass leaves its
+// RL = edge-radix values per thread in the first W * N / RL threads (all of them at 256; 384 of
+// 512 at 192, whole waves); split_active() tells a thread whether it holds them.
+template <int N> constexpr int split_xthreads() { return split_w<kSplitWc>() * (N / PlanOf<N>::rmin); }
+template <int N, int W, int RL, int TB> __device__ __forceinline__ bool split_active(int b) {
+  static_assert((W * (N / RL)) % 64 == 0, "the holding threads fill whole waves");
+  return W * (N / RL) == TB || b < W * (N / RL);
+}
 // contraction: the edge plan, or 8 * 8 * 4 (last radix 4: 4 values of each field per thread)
 template <int N> struct XcPlan {
   using type = FwdPlan<N>;
@@ -266,6 +276,7 @@ k_xcontract_split(float2* __restrict__ kbuf, const float* __restrict__ Bt, SGeo 
     }
     // position r of this thread's column segment: S[f][r] in registers, tile[b + e] for field PV
     const float2 wi = make_float2(wt.x, -wt.y);
+    if (split_active<N, W, RL, TB>(b)) {
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const int e = r * (N / RL) * W;
@@ -291,6 +302,7 @@ k_xcontract_split(float2* __restrict__ kbuf, const float* __restrict__ Bt, SGeo 
         else tile[b + e] = R;
       }
     }
+    }
     {
       float2* fb = kbuf + PV * g.kfs;
       fft_l2g_p<float2, N, Plan>(
@@ -312,13 +324,14 @@ k_xcontract_split(float2* __restrict__ kbuf, const float* __restrict__ Bt, SGeo 
       fft_g2r_p<float2, N, Plan>(
           tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, S[f], bo());
     }
+    const int kz = s.kz(c, g.M);
+    const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
+    double acc = 0.0;
+    if (split_active<N, W, RL, TB>(b)) {
 #pragma unroll
     for (int f = 0; f < V; ++f)
 #pragma unroll
       for (int r = 0; r < RL; ++r) S[f][r] = lane_dft4<true, WC>(cmul(S[f][r], wt), lst);
-    const int kz = s.kz(c, g.M);
-    const double w = (kz == 0 || kz == g.M) ? 1.0 : 2.0;
-    double acc = 0.0;
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
       const int e = r * (N / RL) * W;
@@ -333,6 +346,7 @@ k_xcontract_split(float2* __restrict__ kbuf, const float* __restrict__ Bt, SGeo 
                  ((double)S[p][r].x * S[q][r].x + (double)S[p][r].y * S[q][r].y);
         }
       acc += w * sum;
+    }
     }
     const double rr = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = rr;
@@ -356,10 +370,11 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   const int t = xs_swizzle<WC>(blockIdx.x, g.ntile), f = blockIdx.y, F = gridDim.y;
   const XSTile<WC> s = xs_tile<WC>(t, g);
   const int b = (int)threadIdx.x & (TB - 1), c = b % W, j = b / W, jl = c / WC;
+  const bool act = split_active<N, W, RL, TB>(b);  // holds RL values after the forward transform
   const LaneStage lst(jl);
   float2* st = state + ((long)t * F + f) * T;
   float2 sv[RL];
-  if (MODE == 0) {
+  if (MODE == 0 && act) {
 #pragma unroll
     for (int r = 0; r < RL; ++r) sv[r] = at32(st, (unsigned)(b + r * (N / RL) * W));
   }
@@ -370,28 +385,31 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   float2 v[RL];
   fft_g2r<float2, N>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, v, b);
   const float2 wt = tw[(s.k1c(c) * jl) % N];
+  if (act) {
 #pragma unroll
-  for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true, WC>(cmul(v[r], wt), lst);
-  if (MODE == 1) {
+    for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true, WC>(cmul(v[r], wt), lst);
+    if (MODE == 1) {
 #pragma unroll
-    for (int r = 0; r < RL; ++r) at32(st, (unsigned)(opaque(b) + r * (N / RL) * W)) = v[r];
-    return;
+      for (int r = 0; r < RL; ++r) at32(st, (unsigned)(opaque(b) + r * (N / RL) * W)) = v[r];
+    } else {
+      const float kyv = kcomp(s.ky(c, g.m), N, mk.kfac);
+      const float kzv = mk.kfac * (float)s.kz(c, g.M);
+      const int bs = opaque(b);  // the state stores recompute what the loads above computed
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const float kxv = kcomp(j + r * (N / RL), N, mk.kfac);
+        const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
+        const float den = 1.f / (1.f + mk.dtLk * k2);
+        const float2 nh = make_float2((sv[r].x - mk.dtL * v[r].x) * den, (sv[r].y - mk.dtL * v[r].y) * den);
+        at32(st, (unsigned)(bs + r * (N / RL) * W)) = nh;
+        v[r] = make_float2(nh.x * mk.inv_n3, nh.y * mk.inv_n3);
+      }
+      const float2 wi = make_float2(wt.x, -wt.y);
+#pragma unroll
+      for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false, WC>(v[r], lst), wi);
+    }
   }
-  const float kyv = kcomp(s.ky(c, g.m), N, mk.kfac);
-  const float kzv = mk.kfac * (float)s.kz(c, g.M);
-  const int bs = opaque(b);  // the state stores recompute what the loads above computed
-#pragma unroll
-  for (int r = 0; r < RL; ++r) {
-    const float kxv = kcomp(j + r * (N / RL), N, mk.kfac);
-    const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
-    const float den = 1.f / (1.f + mk.dtLk * k2);
-    const float2 nh = make_float2((sv[r].x - mk.dtL * v[r].x) * den, (sv[r].y - mk.dtL * v[r].y) * den);
-    at32(st, (unsigned)(bs + r * (N / RL) * W)) = nh;
-    v[r] = make_float2(nh.x * mk.inv_n3, nh.y * mk.inv_n3);
-  }
-  const float2 wi = make_float2(wt.x, -wt.y);
-#pragma unroll
-  for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false, WC>(v[r], lst), wi);
+  if (MODE == 1) return;
   fft_r2g<float2, N>(
       tile, Ls, tw, v, [&](int cc, int kx, float2 val) { at32(fb, s.coff(opaque(cc) & (W - 1)) + (unsigned)kx * s.ps) = val; },
       opaque(b) & (TB - 1));
@@ -420,10 +438,11 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split_prec(floa
   const int t = xs_swizzle<WC>(blockIdx.x, g.ntile), f = blockIdx.y, F = gridDim.y;
   const XSTile<WC> s = xs_tile<WC>(t, g);
   const int b = (int)threadIdx.x & (TB - 1), c = b % W, j = b / W, jl = c / WC;
+  const bool act = split_active<N, W, RL, TB>(b);  // holds RL values after the forward transform
   const LaneStage lst(jl);
   float2* st = state + ((long)t * F + f) * T;
   float2 sv[RL];
-  if (MODE == 0) {
+  if (MODE == 0 && act) {
 #pragma unroll
     for (int r = 0; r < RL; ++r) sv[r] = at32(st, (unsigned)(b + r * (N / RL) * W));
   }
@@ -434,10 +453,11 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split_prec(floa
   float2 v[RL];
   fft_g2r<float2, N>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, v, b);
   const float2 wt = tw[(s.k1c(c) * jl) % N];
-#pragma unroll
-  for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true, WC>(cmul(v[r], wt), lst);
   const int ky = s.ky(c, g.m), kz = s.kz(c, g.M);
   const int bs = opaque(b) & (TB - 1);  // the state stores recompute what the loads above computed
+  if (act) {
+#pragma unroll
+  for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true, WC>(cmul(v[r], wt), lst);
   if (MODE == 1) {
 #pragma unroll
     for (int r = 0; r < RL; ++r) {
@@ -445,8 +465,7 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split_prec(floa
       if (f == 0 && j + r * (N / RL) == 0 && ky == 0 && kz == 0) z = make_float2(pk.k0, 0.f);
       at32(st, (unsigned)(bs + r * (N / RL) * W)) = z;
     }
-    return;
-  }
+  } else {
   const float kyv = kcomp(ky, N, pk.kfac);
   const float kzv = pk.kfac * (float)kz;
   const float* Bb = Bt + (long)t * T;
@@ -472,6 +491,9 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split_prec(floa
   const float2 wi = make_float2(wt.x, -wt.y);
 #pragma unroll
   for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false, WC>(v[r], lst), wi);
+  }
+  }
+  if (MODE == 1) return;
   fft_r2g<float2, N>(
       tile, Ls, tw, v, [&](int cc, int kx, float2 val) { at32(fb, s.coff(opaque(cc) & (W - 1)) + (unsigned)kx * s.ps) = val; },
       opaque(b) & (TB - 1));
@@ -542,7 +564,7 @@ __global__ void r_state_grad_split(const float2* __restrict__ state, SGeo g, int
 template <int N> struct SubLds {
   __host__ __device__ static constexpr int sk(int p) { return p + (p >> 4); }
   static constexpr int M = N / 2, Y = N / kSplitP, PP = GSK_SUB_PP > 0 ? GSK_SUB_PP : sk(N / 2) + 1, Q = M / 2,
-                       T = kSubT;
+                       T = sub_threads<N>();
   static_assert(PP > sk(N / 2), "pitch holds a skewed row");
   static constexpr int NW = T / 64, RW = Y / NW, CW = M / NW;
   static_assert(Y % NW == 0 && M % NW == 0, "rows and columns in equal shares per wave");
@@ -602,11 +624,12 @@ template <int N> struct SubLds {
   // Lane 0 also writes its Z_{M-k} into slot M, which the M-point transform does not read. The
   // self-paired value k = Q of the wave's RW rows is split afterwards, one row per lane.
   __device__ void c2r() const {
-    static_assert(Q == 64 && RW <= 64, "one pair per lane and row");
+    static_assert(Q <= 64 && RW <= 64, "at most one pair per lane and row");
     float2* R0 = rows();
     const int ln = opaque(lane) & 63;  // the mask keeps the lane index known non-negative
     const float2 w = rt[ln];
     const bool k0 = ln == 0;
+    if (Q == 64 || ln < Q)  // lanes Q.. (n = 192) have no pair
     for (int i = 0; i < RW; ++i) {
       float2* z = R0 + i * PP;
       float2 xk = z[sk(ln)], xm = z[sk(M - ln)];
@@ -637,7 +660,10 @@ template <int N> struct SubLds {
     fft_wlines<float2, M, true, typename PlanOf<M>::type, RW, true, true>(R0, Lines{RW, RW, PP, 0, 1}, twz, ln);
     const float2 w = rt[ln];
     const bool k0 = ln == 0;
-    const int pm = (M - ln) & (M - 1);
+    int pm;  // the partner M - k, and 0 for k = 0
+    if constexpr ((M & (M - 1)) == 0) pm = (M - ln) & (M - 1);
+    else pm = k0 ? 0 : M - ln;
+    if (Q == 64 || ln < Q)  // lanes Q.. (n = 192) have no pair
     for (int i = 0; i < RW; ++i) {
       float2* z = R0 + i * PP;
       const float2 zk = z[sk(ln)], zm = z[sk(pm)];
@@ -670,7 +696,7 @@ template <int N> constexpr size_t sub_lds() {
 // is variant p of sub-plane 8q + k, so the V items of a sub-plane, which read the same eta, run at
 // the same time on one XCD (item % 8 = b % 8 = k) and share its L2.
 template <int N, int VT>
-__global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_mid(float2* __restrict__ kbuf, const float* __restrict__ eta, SGeo g,
+__global__ void __launch_bounds__(sub_threads<N>()) GSK_SUB_ATTR k_sub_mid(float2* __restrict__ kbuf, const float* __restrict__ eta, SGeo g,
                                                   MartK mk, const float2* __restrict__ twN,
                                                   const float2* __restrict__ twM, const float2* __restrict__ rtn) {
   using SL = SubLds<N>;
@@ -710,8 +736,9 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_mid(float2* __restri
     // One voxel quad per lane and row (Q = 64). The quads of all variants of a row are requested
     // before any is used, and the variant p's quad is picked by selects, not a branch: without
     // that, each load waited for the one before it (three round trips per row).
-    static_assert(Q == 64, "one voxel quad per lane and row");
+    static_assert(Q <= 64, "at most one voxel quad per lane and row");
     const int q = s.lane;
+    if (Q == 64 || q < Q)  // lanes Q.. (n = 192) hold no quad
     for (int i = 0; i < RW; ++i) {
       const int t = s.w * RW + i;
       float2* z = s.rows() + i * PP;
@@ -759,7 +786,7 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_mid(float2* __restri
 // f * n * P + (sub-plane) is field f of that sub-plane. SQ = false: the inverse only (y inverse,
 // C2R, the real field written), for fields that end in real space (the reference rule's R).
 template <int N, bool SQ = true>
-__global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_sq(float2* __restrict__ kbuf, float* __restrict__ out, SGeo g,
+__global__ void __launch_bounds__(sub_threads<N>()) GSK_SUB_ATTR k_sub_sq(float2* __restrict__ kbuf, float* __restrict__ out, SGeo g,
                                                  const float2* __restrict__ twN, const float2* __restrict__ twM,
                                                  const float2* __restrict__ rtn, uint32_t* adv, int nitems) {
   if (adv && blockIdx.x == 0 && threadIdx.x == 0) *adv += 1u;
@@ -816,7 +843,7 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_sq(float2* __restric
 // (sub-plane, field): the carried S_hat of a fresh start, bit-identical to what k_sub_sq leaves
 // (same FP32 values, same code), and (OP 0) the transform of eta that initialises the state.
 template <int N, int OP>
-__global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_fwd(const float* __restrict__ in, float2* __restrict__ kbuf, SGeo g,
+__global__ void __launch_bounds__(sub_threads<N>()) GSK_SUB_ATTR k_sub_fwd(const float* __restrict__ in, float2* __restrict__ kbuf, SGeo g,
                                                   const float2* __restrict__ twN, const float2* __restrict__ twM,
                                                   const float2* __restrict__ rtn) {
   using SL = SubLds<N>;
@@ -857,13 +884,13 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_fwd(const float* __r
 // is field f of sub-plane 8q + k, so the FT items of a sub-plane, which read the same voxels, run
 // at about the same time on one XCD (work-group index mod 8) and share its L2.
 template <int N, int FT>
-__global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_local(const float* __restrict__ fld, float2* __restrict__ kbuf,
+__global__ void __launch_bounds__(sub_threads<N>()) GSK_SUB_ATTR k_sub_local(const float* __restrict__ fld, float2* __restrict__ kbuf,
                                                     SGeo g, PrecK pk, const float* __restrict__ R, float el,
                                                     float cnoise, const float2* __restrict__ twN,
                                                     const float2* __restrict__ twM, const float2* __restrict__ rtn) {
   using SL = SubLds<N>;
   constexpr int Q = SL::Q, PP = SL::PP, Y = SL::Y, RW = SL::RW;
-  static_assert(Q == 64, "one voxel quad per lane and row");
+  static_assert(Q <= 64, "at most one voxel quad per lane and row");
   extern __shared__ float2 smem[];
   const SL s(smem);
   const int it = blockIdx.x, rem = it % (8 * FT), sp = it / (8 * FT) * 8 + rem % 8, f = rem / 8;
@@ -874,6 +901,7 @@ __global__ void __launch_bounds__(kSubT) GSK_SUB_ATTR k_sub_local(const float* _
   const uint32_t step = step_of(pk.step, pk.stepp);
   s.tables(twN, twM, rtn);
   const int q = s.lane;
+  if (Q == 64 || q < Q)  // lanes Q.. (n = 192) hold no quad
   for (int i = 0; i < RW; ++i) {  // the wave's rows
     const int t = s.w * RW + i;
     float2* z = s.rows() + i * PP;

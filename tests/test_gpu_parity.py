@@ -234,6 +234,31 @@ def test_mart_translation_equivariance_at_256(gpu):
     assert np.abs(np.roll(a, sh, axis=(1, 2, 3)) - b).max() < 1e-5 * np.abs(a).max()
 
 
+@pytest.mark.parametrize("n,V", [(192, 3), (192, 1), (192, 4), (192, 2)])
+def test_split_schedule_matches_chain_at_192(gpu, monkeypatch, n, V):
+    """The split-y schedule at n = 192 (sub-planes of 48 rows, 384-thread sub-plane groups, x
+    transforms 12 * 16 whose last pass leaves values in 384 of 512 threads) against the chain
+    of y and z kernels (GSK_SPLIT=0): 20 steps of strong dynamics with noise agree to FP32
+    rounding, and so do the energies."""
+    say(f"split-y schedule vs y/z chain n={n} V={V}")
+    kw = dict(A=-0.5, w_el=20.0, noise=0.002, seed=9)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("GSK_SPLIT", mode)
+        s = mart_sim(n, V, **kw)
+        s.init_uniform("eta", -0.05, 0.05, 6)
+        s.step(20)
+        out[mode] = (s.get_field("eta", np.float64), s.energy())
+        names, _ = s.kernel_times(1)
+        s.close()
+        assert ("sub_mid" in names) == (mode == "1"), names
+    (a, Ea), (b, Eb) = out["1"], out["0"]
+    assert np.isfinite(a).all()
+    assert float(np.abs(a - b).max()) < 2e-6 * float(np.abs(b).max())
+    assert rel_l2(a, b) < 1e-6
+    np.testing.assert_allclose(Ea, Eb, rtol=1e-5, atol=1e-9 * abs(Eb[3]))
+
+
 @pytest.mark.parametrize("V", [3, 1, 2, 4])
 def test_split_schedule_matches_chain_at_256(gpu, monkeypatch, V):
     """At 256^3 the fused martensite step runs the split-y schedule (four launches; sub-plane
@@ -283,7 +308,7 @@ def test_mart_fused_vs_plain_at_512(gpu):
     assert float(np.abs(got - want).max()) < 2e-6 * float(np.abs(want).max())
 
 
-@pytest.mark.parametrize("n", [64, 96, 256])
+@pytest.mark.parametrize("n", [64, 96, 192, 256])
 def test_mart_square_carry_bit_identical(gpu, n):
     """A step leaves the transformed eta^2 of its result for the next step (along z and y at
     n = 64, where the plane-resident passes run; along z at n = 96 and 256, or along z and y at
@@ -308,7 +333,7 @@ def test_mart_square_carry_bit_identical(gpu, n):
     assert np.array_equal(ea.view(np.uint32), eb.view(np.uint32))
 
 
-@pytest.mark.parametrize("n", [64, 256])
+@pytest.mark.parametrize("n", [64, 192, 256])
 def test_prec_local_force_carry_bit_identical(gpu, n):
     """The precipitate step's last pass also forms the next step's local forces (with the next
     step's noise): the last z pass at n = 64, the sub-plane local-force kernel of the split-y
@@ -413,18 +438,18 @@ def test_prec_fused32_matches_oracle(gpu, n, steps):
     np.testing.assert_allclose(E, Ew, rtol=1e-5, atol=1e-9 * abs(Ew[3]))
 
 
-@pytest.mark.parametrize("V", [4, 1, 2])
-def test_prec_split_schedule_matches_chain_at_256(gpu, monkeypatch, V):
-    """At 256^3 the precipitate step runs the split-y schedule (x update, sub-plane inverse,
+@pytest.mark.parametrize("n,V", [(256, 4), (256, 1), (256, 2), (192, 4), (192, 1)])
+def test_prec_split_schedule_matches_chain(gpu, monkeypatch, n, V):
+    """At 256^3 and 192^3 the precipitate step runs the split-y schedule (x update, sub-plane inverse,
     sub-plane local forces: three launches) unless GSK_SPLIT=0 is set when the handle is
     created (the chain of y and z kernels, four launches). Two FP32 orderings of one step: 20
     steps with noise agree to FP32 rounding in c, eta and the energies; both conserve sum c to the
     last bit. V order parameters: 2..5 fields of the local-force kernel."""
-    say(f"prec split-y schedule vs y/z chain n=256 V={V}")
+    say(f"prec split-y schedule vs y/z chain n={n} V={V}")
     out = {}
     for mode in ("1", "0"):
         monkeypatch.setenv("GSK_SPLIT", mode)
-        s = prec_sim(256, V=V, noise=0.01)
+        s = prec_sim(n, V=V, noise=0.01)
         prec_init(s)
         m0 = s.mass()
         s.step(20)
