@@ -1,6 +1,7 @@
-// Split-y schedule of the FP32 fused martensite step (n = 256): four launches per step.
+// Split-y schedule of the FP32 fused steps at n = 256 and 192: four launches per martensite step,
+// three per precipitate step, three for the reference rules' elastic R.
 //
-// The y transform of an x-plane (length n) is split as n = P * m (P = 4, m = 64):
+// The y transform of an x-plane (length n) is split as n = P * m (P = 4, m = 64 or 48):
 //   inverse  x[P t + j] = sum_k1 w_m^(k1 t) A_j[k1],
 //            A_j[k1]    = w_n^(k1 j) sum_k2 X[k1 + m k2] w_P^(k2 j)
 //   forward  E_j[k1]    = sum_t x[P t + j] w_m^(-k1 t),
@@ -9,7 +10,7 @@
 // of one (x, k1, kz) sit in lanes c, c ^ Wc, c ^ 2 Wc of a wave), between their x transforms and
 // the pointwise k-space work. Sub-plane j of an x-plane (the m rows y = P t + j) then needs only
 // the m-point y transforms, so a work group keeps it in LDS through y inverse, the z pass and y
-// forward: 64 rows of 129 values at pitch 136, 72 KB (two work groups per CU).
+// forward: at n = 256, 64 rows of 129 values at pitch 137, 72 KB (two work groups per CU).
 //
 // Sub-plane j of x-plane x is stored in the rows x*n + j*m + k1 (k1 < m) of the half-spectrum
 // layout (kz < n/2 in rows of n/2 values, the Nyquist value kz = n/2 in the Nyquist plane):
