@@ -99,10 +99,10 @@ def test_mart_reference_rule_variant_counts(gpu, n, V):
     assert err < 2e-5
 
 
-def _mart_ref_256(noise):
+def _mart_ref_256(noise, n=256):
     C = gsk.cubic_modulus(3.5, 1.5, 1.0)
     eps = gsk.mart_eigenstrains(3, base=0.0, axis=0.01)
-    cfg = gsk.reference_config(gsk.PHYS_MART, n=256, nvar=3, temperature=300.0, noise=noise, w_el=1e3, seed=6,
+    cfg = gsk.reference_config(gsk.PHYS_MART, n=n, nvar=3, temperature=300.0, noise=noise, w_el=1e3, seed=6,
                                dt=0.005)
     return C, eps, cfg
 
@@ -144,12 +144,13 @@ def test_mart_reference_rule_matches_oracle_at_256(gpu):
     assert err < 1e-5
 
 
-def test_mart_reference_rule_split_matches_chain_at_256(gpu, monkeypatch):
-    """The reference rule's elastic R at n = 256 through the sub-plane kernels (default) and
-    through the chain of z, y and x kernels (GSK_SPLIT=0 when the handle is created): two FP32
-    orderings of one computation, so 10 steps with noise agree to FP32 rounding."""
-    say("mart reference rule n=256: split R vs chain")
-    C, eps, cfg = _mart_ref_256(1e-3)
+@pytest.mark.parametrize("n", [256, 192])
+def test_mart_reference_rule_split_matches_chain_at_256(gpu, monkeypatch, n):
+    """The reference rule's elastic R at n = 256 and 192 through the sub-plane kernels (default)
+    and through the chain of z, y and x kernels (GSK_SPLIT=0 when the handle is created): two
+    FP32 orderings of one computation, so 10 steps with noise agree to FP32 rounding."""
+    say(f"mart reference rule n={n}: split R vs chain")
+    C, eps, cfg = _mart_ref_256(1e-3, n)
     out = {}
     for mode in ("1", "0"):
         monkeypatch.setenv("GSK_SPLIT", mode)
@@ -198,15 +199,16 @@ def test_prec_reference_rule_matches_oracle(gpu):
     assert m1 == m0
 
 
-def test_prec_reference_rule_split_matches_chain_at_256(gpu, monkeypatch):
+@pytest.mark.parametrize("n", [256, 192])
+def test_prec_reference_rule_split_matches_chain_at_256(gpu, monkeypatch, n):
     """The precipitate reference rule at n = 256 through the split-y kernels (R of the eta^2
     source in three launches, local forces with it in the sub-plane kernel, x update, sub-plane
     inverse) and through the chain of z, y and x kernels (GSK_SPLIT=0): 10 steps with noise agree
     to FP32 rounding, and sum c is conserved by both."""
-    say("prec reference rule n=256: split vs chain")
+    say(f"prec reference rule n={n}: split vs chain")
     C = gsk.cubic_modulus(3.5, 1.5, 1.0)
     eps = gsk.mart_eigenstrains(3, base=0.0, axis=0.02)
-    cfg = gsk.reference_config(gsk.PHYS_PREC, n=256, nvar=3, M=1.0, L=1.0, kappa=1.5, kappa_c=1.5, dt=0.002,
+    cfg = gsk.reference_config(gsk.PHYS_PREC, n=n, nvar=3, M=1.0, L=1.0, kappa=1.5, kappa_c=1.5, dt=0.002,
                                seed=8, xi=50.0, noise=0.01)
     out = {}
     for mode in ("1", "0"):
