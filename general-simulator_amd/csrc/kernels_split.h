@@ -354,14 +354,15 @@ k_xcontract_split(float2* __restrict__ kbuf, const float* __restrict__ Bt, SGeo 
   }
 }
 
-// Semi-implicit update: E-form F_hat -> x FFT -> lane stage -> eta_hat = (eta_hat - dt L F_hat) /
-// (1 + dt L kappa k^2) on the tile-ordered k-space state -> inverse lane stage -> inverse x FFT ->
-// A-form eta_hat / n^3 (MODE 0). MODE 1: the state from the E-form transform of eta, nothing
-// written to the k-space buffer. One field per work group (grid = tiles x fields).
-template <int N, int MODE>
-__global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* __restrict__ kbuf,
-                                                                       float2* __restrict__ state, SGeo g, MartK mk,
-                                                                       const float2* __restrict__ twN) {
+// The x update of one tile and field (grid = tiles x fields): E-form F_hat -> x FFT -> lane stage
+// -> on the tile-ordered k-space state, element by element, state <- upd(state, F_hat, k^2, e) (e:
+// the element's index in a tile-ordered array: the state's, the B tensor's) -> inverse lane stage ->
+// inverse x FFT -> A-form field spectrum / n^3 (MODE 0). MODE 1: the state <- init(E-form
+// transform, kx, ky, kz), nothing written to the k-space buffer.
+template <int N, int MODE, class Init, class Upd>
+__device__ __forceinline__ void xupdate_split_tile(float2* __restrict__ kbuf, float2* __restrict__ state,
+                                                   const SGeo& g, float kfac, float inv_n3,
+                                                   const float2* __restrict__ twN, Init init, Upd upd) {
   constexpr int WC = kSplitWc, RL = edge_radix<N>(), W = split_w<WC>(), T = N * W;
   extern __shared__ float2 smem[];
   float2* tw = smem;
@@ -375,7 +376,7 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   const LaneStage lst(jl);
   float2* st = state + ((long)t * F + f) * T;
   float2 sv[RL];
-  if (MODE == 0 && act) {
+  if (MODE == 0 && act) {  // the state does not depend on the transform: fetched first
 #pragma unroll
     for (int r = 0; r < RL; ++r) sv[r] = at32(st, (unsigned)(b + r * (N / RL) * W));
   }
@@ -386,24 +387,25 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
   float2 v[RL];
   fft_g2r<float2, N>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, v, b);
   const float2 wt = tw[(s.k1c(c) * jl) % N];
+  const int ky = s.ky(c, g.m), kz = s.kz(c, g.M);
   if (act) {
 #pragma unroll
     for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true, WC>(cmul(v[r], wt), lst);
+    const int bs = opaque(b) & (TB - 1);  // the state stores recompute what the loads above computed
     if (MODE == 1) {
 #pragma unroll
-      for (int r = 0; r < RL; ++r) at32(st, (unsigned)(opaque(b) + r * (N / RL) * W)) = v[r];
+      for (int r = 0; r < RL; ++r) at32(st, (unsigned)(bs + r * (N / RL) * W)) = init(v[r], j + r * (N / RL), ky, kz);
     } else {
-      const float kyv = kcomp(s.ky(c, g.m), N, mk.kfac);
-      const float kzv = mk.kfac * (float)s.kz(c, g.M);
-      const int bs = opaque(b);  // the state stores recompute what the loads above computed
+      const float kyv = kcomp(ky, N, kfac);
+      const float kzv = kfac * (float)kz;
 #pragma unroll
       for (int r = 0; r < RL; ++r) {
-        const float kxv = kcomp(j + r * (N / RL), N, mk.kfac);
+        const float kxv = kcomp(j + r * (N / RL), N, kfac);
         const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
-        const float den = 1.f / (1.f + mk.dtLk * k2);
-        const float2 nh = make_float2((sv[r].x - mk.dtL * v[r].x) * den, (sv[r].y - mk.dtL * v[r].y) * den);
-        at32(st, (unsigned)(bs + r * (N / RL) * W)) = nh;
-        v[r] = make_float2(nh.x * mk.inv_n3, nh.y * mk.inv_n3);
+        const unsigned e = (unsigned)(bs + r * (N / RL) * W);
+        const float2 nh = upd(sv[r], v[r], k2, e);
+        at32(st, e) = nh;
+        v[r] = make_float2(nh.x * inv_n3, nh.y * inv_n3);
       }
       const float2 wi = make_float2(wt.x, -wt.y);
 #pragma unroll
@@ -416,88 +418,50 @@ __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* _
       opaque(b) & (TB - 1));
 }
 
-// Precipitate update in the split order, one field per work group (grid = tiles x fields):
-// E-form F_hat_f -> x FFT -> lane stage -> on the tile-ordered k-space state
+// Martensite: eta_hat <- (eta_hat - dt L F_hat) / (1 + dt L kappa k^2). MODE 1: the state from the
+// E-form transform of eta.
+template <int N, int MODE>
+__global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split(float2* __restrict__ kbuf,
+                                                                       float2* __restrict__ state, SGeo g, MartK mk,
+                                                                       const float2* __restrict__ twN) {
+  xupdate_split_tile<N, MODE>(
+      kbuf, state, g, mk.kfac, mk.inv_n3, twN, [](float2 z, int, int, int) { return z; },
+      [&](float2 sh, float2 fh, float k2, unsigned) {
+        const float den = 1.f / (1.f + mk.dtLk * k2);
+        return make_float2((sh.x - mk.dtL * fh.x) * den, (sh.y - mk.dtL * fh.y) * den);
+      });
+}
+
+// Precipitate, field f of (dc, eta_1..V):
 //   f = 0: dc_hat <- (dc_hat - dt M k^2 (mu_hat + (xi B(k) - S) dc_hat)) / (1 + dt M (S k^2 + kappa_c k^4))
 //   f > 0: eta_hat <- (eta_hat - dt L F_hat) / (1 + dt L kappa_eta k^2)
-// -> inverse lane stage -> inverse x FFT -> A-form field spectrum / n^3 (MODE 0). B(k) (celastic):
-// the scalar tensor of the concentration in this kernel's tile order. MODE 1: the state from the
-// E-form transform of the fields, the k = 0 mode of dc set from its FP64 sum (pk.k0). pk.adv (MODE
-// 0, may be null): the step counter the launch advances once.
+// B(k) (celastic): the concentration's scalar tensor in this kernel's tile order. MODE 1: the
+// state from the E-form transform of the fields, the k = 0 mode of dc set from its FP64 sum
+// (pk.k0). pk.adv (MODE 0, may be null): the step counter the launch advances once.
 template <int N, int MODE>
 __global__ void __launch_bounds__(split_xthreads<N>()) k_xupdate_split_prec(float2* __restrict__ kbuf,
                                                                             float2* __restrict__ state,
                                                                             const float* __restrict__ Bt, SGeo g,
                                                                             PrecK pk, const float2* __restrict__ twN) {
-  constexpr int WC = kSplitWc, RL = edge_radix<N>(), W = split_w<WC>(), T = N * W;
-  extern __shared__ float2 smem[];
-  float2* tw = smem;
-  float2* tile = smem + N;
-  constexpr int TB = split_xthreads<N>();
-  static_assert((TB & (TB - 1)) == 0, "power-of-two work group");
   if (MODE == 0 && pk.adv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *pk.adv += 1u;
-  const int t = xs_swizzle<WC>(blockIdx.x, g.ntile), f = blockIdx.y, F = gridDim.y;
-  const XSTile<WC> s = xs_tile<WC>(t, g);
-  const int b = (int)threadIdx.x & (TB - 1), c = b % W, j = b / W, jl = c / WC;
-  const bool act = split_active<N, W, RL, TB>(b);  // holds RL values after the forward transform
-  const LaneStage lst(jl);
-  float2* st = state + ((long)t * F + f) * T;
-  float2 sv[RL];
-  if (MODE == 0 && act) {
-#pragma unroll
-    for (int r = 0; r < RL; ++r) sv[r] = at32(st, (unsigned)(b + r * (N / RL) * W));
-  }
-  load_tw<N>(tw, twN);
-  __syncthreads();
-  const Lines Ls{W, W, 1, 0, W};
-  float2* fb = kbuf + f * g.kfs;
-  float2 v[RL];
-  fft_g2r<float2, N>(tile, Ls, tw, [&](int cc, int kx) { return at32(fb, s.coff(cc) + (unsigned)kx * s.ps); }, v, b);
-  const float2 wt = tw[(s.k1c(c) * jl) % N];
-  const int ky = s.ky(c, g.m), kz = s.kz(c, g.M);
-  const int bs = opaque(b) & (TB - 1);  // the state stores recompute what the loads above computed
-  if (act) {
-#pragma unroll
-  for (int r = 0; r < RL; ++r) v[r] = lane_dft4<true, WC>(cmul(v[r], wt), lst);
-  if (MODE == 1) {
-#pragma unroll
-    for (int r = 0; r < RL; ++r) {
-      float2 z = v[r];
-      if (f == 0 && j + r * (N / RL) == 0 && ky == 0 && kz == 0) z = make_float2(pk.k0, 0.f);
-      at32(st, (unsigned)(bs + r * (N / RL) * W)) = z;
-    }
-  } else {
-  const float kyv = kcomp(ky, N, pk.kfac);
-  const float kzv = pk.kfac * (float)kz;
-  const float* Bb = Bt + (long)t * T;
-#pragma unroll
-  for (int r = 0; r < RL; ++r) {
-    const float kxv = kcomp(j + r * (N / RL), N, pk.kfac);
-    const float k2 = kxv * kxv + kyv * kyv + kzv * kzv;
-    const float2 sh = sv[r], fh = v[r];
-    float2 nh;
-    if (f == 0) {
-      const float bv = pk.celastic ? at32(Bb, (unsigned)(bs + r * (N / RL) * W)) : 0.f;
-      const float a = pk.xi * bv - pk.S;
-      const float mx = fh.x + a * sh.x, my = fh.y + a * sh.y;
-      const float den = 1.f / (1.f + pk.dtM * pk.S * k2 + pk.dtMkc * k2 * k2);
-      nh = make_float2((sh.x - pk.dtM * k2 * mx) * den, (sh.y - pk.dtM * k2 * my) * den);
-    } else {
-      const float den = 1.f / (1.f + pk.dtLke * k2);
-      nh = make_float2((sh.x - pk.dtL * fh.x) * den, (sh.y - pk.dtL * fh.y) * den);
-    }
-    at32(st, (unsigned)(bs + r * (N / RL) * W)) = nh;
-    v[r] = make_float2(nh.x * pk.inv_n3, nh.y * pk.inv_n3);
-  }
-  const float2 wi = make_float2(wt.x, -wt.y);
-#pragma unroll
-  for (int r = 0; r < RL; ++r) v[r] = cmul(lane_dft4<false, WC>(v[r], lst), wi);
-  }
-  }
-  if (MODE == 1) return;
-  fft_r2g<float2, N>(
-      tile, Ls, tw, v, [&](int cc, int kx, float2 val) { at32(fb, s.coff(opaque(cc) & (W - 1)) + (unsigned)kx * s.ps) = val; },
-      opaque(b) & (TB - 1));
+  const int f = blockIdx.y;
+  const float* Bb = Bt + (long)xs_swizzle<kSplitWc>(blockIdx.x, g.ntile) * (N * split_w<kSplitWc>());
+  xupdate_split_tile<N, MODE>(
+      kbuf, state, g, pk.kfac, pk.inv_n3, twN,
+      [&](float2 z, int kx, int ky, int kz) {
+        return (f == 0 && kx == 0 && ky == 0 && kz == 0) ? make_float2(pk.k0, 0.f) : z;
+      },
+      [&](float2 sh, float2 fh, float k2, unsigned e) {
+        if (f == 0) {
+          const float bv = pk.celastic ? at32(Bb, e) : 0.f;
+          const float a = pk.xi * bv - pk.S;
+          const float mx = fh.x + a * sh.x, my = fh.y + a * sh.y;
+          const float den = 1.f / (1.f + pk.dtM * pk.S * k2 + pk.dtMkc * k2 * k2);
+          return make_float2((sh.x - pk.dtM * k2 * mx) * den, (sh.y - pk.dtM * k2 * my) * den);
+        }
+        const float den = 1.f / (1.f + pk.dtLke * k2);
+        return make_float2((sh.x - pk.dtL * fh.x) * den, (sh.y - pk.dtL * fh.y) * den);
+      });
 }
 
 // Precipitate elastic energy sum_k w(kz) B(k) |dc_hat(k)|^2 from the split-order state (field 0;
