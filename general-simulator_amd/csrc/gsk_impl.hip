@@ -254,7 +254,7 @@ struct Impl {
       return GSK_OK;
     } else {
       (void)s; (void)square; (void)in; (void)count;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 128, 192 or 256");
     }
   }
   // the contraction over the first `count` spectra of the k-space buffer (the handle's F by default)
@@ -273,7 +273,7 @@ struct Impl {
       return GSK_OK;
     } else {
       (void)s; (void)mk; (void)count;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 128, 192 or 256");
     }
   }
   template <int MODE>
@@ -284,7 +284,7 @@ struct Impl {
       return GSK_OK;
     } else {
       (void)s; (void)mk;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 128, 192 or 256");
     }
   }
   static int step_mart_split(gsk_sim* s) {
@@ -294,8 +294,8 @@ struct Impl {
       static_assert((N * kSplitP) % 8 == 0, "sub-planes in groups of 8");
       if (!s->cus) HIPCHK(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->dev));
       const int nitems = N * kSplitP * s->F, g8 = 8 * s->F;
-      const int gm = std::min(nitems, std::max(g8, 2 * s->cus / g8 * g8));
-      const int gq = std::min(nitems, std::max(8, 2 * s->cus / 8 * 8));
+      const int gm = std::min(nitems, std::max(g8, sub_groups_per_cu<N>() * s->cus / g8 * g8));
+      const int gq = std::min(nitems, std::max(8, sub_groups_per_cu<N>() * s->cus / 8 * 8));
       if (!s->sq_ready) CHK(split_fwd(s, true));
       CHK(split_xcontract<0>(s, mk));
       switch (s->F) {
@@ -311,7 +311,7 @@ struct Impl {
       return GSK_OK;
     } else {
       (void)s;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 128, 192 or 256");
     }
   }
 
@@ -325,7 +325,7 @@ struct Impl {
       return GSK_OK;
     } else {
       (void)s; (void)pk;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 128, 192 or 256");
     }
   }
   static int split_local(gsk_sim* s, const PrecK& pk, const float* R, float el, float cn) {
@@ -341,7 +341,7 @@ struct Impl {
       return GSK_OK;
     } else {
       (void)s; (void)pk; (void)R; (void)el; (void)cn;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 128, 192 or 256");
     }
   }
   // y inverse and C2R of the first `count` A-form spectra into real fields at out; adv (may be
@@ -349,13 +349,13 @@ struct Impl {
   static int split_inv(gsk_sim* s, float* out, int count, uint32_t* adv) {
     if constexpr (split_size<N>()) {
       if (!s->cus) HIPCHK(hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->dev));
-      const int nitems = N * kSplitP * count, gq = std::min(nitems, std::max(8, 2 * s->cus / 8 * 8));
+      const int nitems = N * kSplitP * count, gq = std::min(nitems, std::max(8, sub_groups_per_cu<N>() * s->cus / 8 * 8));
       LAUNCH((k_sub_sq<N, false>), gq, sub_threads<N>(), sub_lds<N>(), s, s->kbuf, out, s->sg, s->twN, s->twM, s->rtn, adv,
              nitems);
       return GSK_OK;
     } else {
       (void)s; (void)out; (void)count; (void)adv;
-      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 256 or 192");
+      return fail(GSK_ERR_UNSUPPORTED, "the split-y schedule takes n = 128, 192 or 256");
     }
   }
   // Spectral precipitate step, split-y schedule (three launches): x update of the carried local

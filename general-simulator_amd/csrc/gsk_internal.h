@@ -238,14 +238,14 @@ inline bool plane_on(const gsk_sim* s) {
          s->cfg.path == GSK_PATH_FUSED32 && s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_SPECTRAL;
 }
 
-// Split-y schedule (kernels_split.h) at n = 256 and 192: the martensite step with 1..4 variants (the
+// Split-y schedule (kernels_split.h) at n = 128, 192, 256: the martensite step with 1..4 variants (the
 // spectral rule's step; the reference rule's elastic R) and the precipitate step with 2..5 fields
 // (both rules; the reference rule's elastic R of up to 4 order parameters). GSK_SPLIT=0 in the
 // environment when a handle is created keeps that handle on the chain of y and z kernels.
 inline bool split_wanted(const gsk_sim* s) {
   const char* e = std::getenv("GSK_SPLIT");
   const bool on = !(e && e[0] == '0');
-  if (!on || (s->n != 256 && s->n != 192) || s->cfg.path != GSK_PATH_FUSED32) return false;
+  if (!on || (s->n != 256 && s->n != 192 && s->n != 128) || s->cfg.path != GSK_PATH_FUSED32) return false;
   if (s->cfg.physics == GSK_PHYS_MART) return s->F >= 1 && s->F <= 4;
   return s->F >= 2 && s->F <= 5;
 }

@@ -234,12 +234,13 @@ def test_mart_translation_equivariance_at_256(gpu):
     assert np.abs(np.roll(a, sh, axis=(1, 2, 3)) - b).max() < 1e-5 * np.abs(a).max()
 
 
-@pytest.mark.parametrize("n,V", [(192, 3), (192, 1), (192, 4), (192, 2)])
+@pytest.mark.parametrize("n,V", [(192, 3), (192, 1), (192, 4), (192, 2), (128, 3), (128, 1), (128, 4)])
 def test_split_schedule_matches_chain_at_192(gpu, monkeypatch, n, V):
     """The split-y schedule at n = 192 (sub-planes of 48 rows, 384-thread sub-plane groups, x
-    transforms 12 * 16 whose last pass leaves values in 384 of 512 threads) against the chain
-    of y and z kernels (GSK_SPLIT=0): 20 steps of strong dynamics with noise agree to FP32
-    rounding, and so do the energies."""
+    transforms 12 * 16 whose last pass leaves values in 384 of 512 threads) and n = 128 (32 rows,
+    256-thread groups, rows of 32 voxel quads) against what GSK_SPLIT=0 runs (the chain of y and
+    z kernels; the plane-resident kernels at 128): 20 steps of strong dynamics with noise agree
+    to FP32 rounding, and so do the energies."""
     say(f"split-y schedule vs y/z chain n={n} V={V}")
     kw = dict(A=-0.5, w_el=20.0, noise=0.002, seed=9)
     out = {}
@@ -308,7 +309,7 @@ def test_mart_fused_vs_plain_at_512(gpu):
     assert float(np.abs(got - want).max()) < 2e-6 * float(np.abs(want).max())
 
 
-@pytest.mark.parametrize("n", [64, 96, 192, 256])
+@pytest.mark.parametrize("n", [64, 96, 128, 192, 256])
 def test_mart_square_carry_bit_identical(gpu, n):
     """A step leaves the transformed eta^2 of its result for the next step (along z and y at
     n = 64, where the plane-resident passes run; along z at n = 96 and 256, or along z and y at
@@ -333,7 +334,7 @@ def test_mart_square_carry_bit_identical(gpu, n):
     assert np.array_equal(ea.view(np.uint32), eb.view(np.uint32))
 
 
-@pytest.mark.parametrize("n", [64, 192, 256])
+@pytest.mark.parametrize("n", [64, 128, 192, 256])
 def test_prec_local_force_carry_bit_identical(gpu, n):
     """The precipitate step's last pass also forms the next step's local forces (with the next
     step's noise): the last z pass at n = 64, the sub-plane local-force kernel of the split-y
@@ -438,7 +439,7 @@ def test_prec_fused32_matches_oracle(gpu, n, steps):
     np.testing.assert_allclose(E, Ew, rtol=1e-5, atol=1e-9 * abs(Ew[3]))
 
 
-@pytest.mark.parametrize("n,V", [(256, 4), (256, 1), (256, 2), (192, 4), (192, 1)])
+@pytest.mark.parametrize("n,V", [(256, 4), (256, 1), (256, 2), (192, 4), (192, 1), (128, 4)])
 def test_prec_split_schedule_matches_chain(gpu, monkeypatch, n, V):
     """At 256^3 and 192^3 the precipitate step runs the split-y schedule (x update, sub-plane inverse,
     sub-plane local forces: three launches) unless GSK_SPLIT=0 is set when the handle is

@@ -144,7 +144,7 @@ def test_mart_reference_rule_matches_oracle_at_256(gpu):
     assert err < 1e-5
 
 
-@pytest.mark.parametrize("n", [256, 192])
+@pytest.mark.parametrize("n", [256, 192, 128])
 def test_mart_reference_rule_split_matches_chain_at_256(gpu, monkeypatch, n):
     """The reference rule's elastic R at n = 256 and 192 through the sub-plane kernels (default)
     and through the chain of z, y and x kernels (GSK_SPLIT=0 when the handle is created): two
@@ -199,7 +199,7 @@ def test_prec_reference_rule_matches_oracle(gpu):
     assert m1 == m0
 
 
-@pytest.mark.parametrize("n", [256, 192])
+@pytest.mark.parametrize("n", [256, 192, 128])
 def test_prec_reference_rule_split_matches_chain_at_256(gpu, monkeypatch, n):
     """The precipitate reference rule at n = 256 through the split-y kernels (R of the eta^2
     source in three launches, local forces with it in the sub-plane kernel, x update, sub-plane
