@@ -140,8 +140,7 @@ template <typename T> int step_mart(gsk_sim* s) {
   C2<T>* Fh = (C2<T>*)w.Fh;
   const PlainKT<T> k = mart_k<T>(s);
   CHK(elastic_R<T>(s, fld));
-  LAUNCH(p_force<T>, grid_1d(N3 / 4), 256, 0, s, fld, (const C2<T>*)w.R, Fh, N3, k);
-  LAUNCH(p_load<T>, grid_1d(VN), 256, 0, s, fld, E, VN);
+  LAUNCH(p_force<T>, grid_1d(N3 / 4), 256, 0, s, fld, (const C2<T>*)w.R, Fh, E, N3, k);
   CHK(c2c(s, Fh, true, s->V));
   CHK(c2c(s, E, true, s->V));
   LAUNCH(p_update<T>, grid_1d(VN), 256, 0, s, E, Fh, k);

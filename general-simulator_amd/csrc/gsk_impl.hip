@@ -693,8 +693,7 @@ struct Impl {
     CHK(c2c(s, s->S64, true, s->V));
     LAUNCH(p_contract<double>, grid_1d(N3), 256, 0, s, s->S64, s->R64, s->B64, N3, k);
     CHK(c2c(s, s->R64, false, s->V));
-    LAUNCH(p_force<double>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->R64, s->F64, N3, k);
-    LAUNCH(p_load<double>, grid_1d(VN), 256, 0, s, s->eta64, s->E64, VN);
+    LAUNCH(p_force<double>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->R64, s->F64, s->E64, N3, k);
     CHK(c2c(s, s->F64, true, s->V));
     CHK(c2c(s, s->E64, true, s->V));
     LAUNCH(p_update<double>, grid_1d(VN), 256, 0, s, s->E64, s->F64, k);

@@ -162,6 +162,35 @@ template <typename T> __device__ __forceinline__ typename Cx<T>::T2 cx(T a, T b)
   return r;
 }
 
+// Quads: four consecutive voxels of one field, starting at a multiple of 4 (every box edge is a
+// multiple of 16, so each field holds whole quads and a quad of T or of complex T is aligned to
+// its own size). Each thread moves its quad with 16-byte accesses.
+__device__ __forceinline__ void ld_quad(const float* p, float (&o)[4]) {
+  const float4 q = *reinterpret_cast<const float4*>(p);
+  o[0] = q.x, o[1] = q.y, o[2] = q.z, o[3] = q.w;
+}
+__device__ __forceinline__ void ld_quad(const double* p, double (&o)[4]) {
+  const double2 a = reinterpret_cast<const double2*>(p)[0], b = reinterpret_cast<const double2*>(p)[1];
+  o[0] = a.x, o[1] = a.y, o[2] = b.x, o[3] = b.y;
+}
+// real parts of a quad of complex values
+__device__ __forceinline__ void ld_quad_re(const float2* p, float (&o)[4]) {
+  const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
+  o[0] = a.x, o[1] = a.z, o[2] = b.x, o[3] = b.z;
+}
+__device__ __forceinline__ void ld_quad_re(const double2* p, double (&o)[4]) {
+  for (int i = 0; i < 4; ++i) o[i] = p[i].x;
+}
+// a quad of complex values with zero imaginary parts
+__device__ __forceinline__ void st_quad_re(float2* p, const float (&v)[4]) {
+  reinterpret_cast<float4*>(p)[0] = make_float4(v[0], 0.f, v[1], 0.f);
+  reinterpret_cast<float4*>(p)[1] = make_float4(v[2], 0.f, v[3], 0.f);
+}
+__device__ __forceinline__ void st_quad_re(double2* p, const double (&v)[4]) {
+  for (int i = 0; i < 4; ++i) p[i] = make_double2(v[i], 0.0);
+}
+constexpr int kPlainMaxV = 8;  // variants on the plain path (martensite 1..8, precipitate 1..7)
+
 struct BoxDim {
   int nx, ny, nz;
   __device__ __forceinline__ void coords(long v, int& x, int& y, int& z) const {
@@ -262,27 +291,39 @@ __global__ void p_contract(const typename Cx<T>::T2* __restrict__ S, typename Cx
   }
 }
 
+// F_p = A eta_p - B eta_p^2 + C eta_p sum_q eta_q^2 + 2 w_el eta_p R_p (+ noise xi_p, Philox field
+// p, keyed by the quad) into Fh, and (E != null) the fields themselves into E, both as complex with
+// zero imaginary parts. One thread per quad of voxels; the variants stay in registers.
 template <typename T>
 __global__ void p_force(const T* __restrict__ eta, const typename Cx<T>::T2* __restrict__ R,
-                        typename Cx<T>::T2* __restrict__ Fh, long N3, PlainKT<T> k) {
+                        typename Cx<T>::T2* __restrict__ Fh, typename Cx<T>::T2* __restrict__ E, long N3,
+                        PlainKT<T> k) {
   for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
-    for (int i = 0; i < 4; ++i) {
-      const long v = g4 * 4 + i;
-      T s2 = 0;
-      for (int p = 0; p < k.V; ++p) s2 += eta[p * N3 + v] * eta[p * N3 + v];
-      for (int p = 0; p < k.V; ++p) {
-        const T e = eta[p * N3 + v];
-        T f = k.A * e - k.B * e * e + k.C * e * s2 + k.two_wel * e * R[p * N3 + v].x;
-        Fh[p * N3 + v] = cx<T>(f, T(0));
+    const long v0 = g4 * 4;
+    T e[kPlainMaxV][4], s2[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int p = 0; p < kPlainMaxV; ++p)
+      if (p < k.V) {
+        ld_quad(eta + p * N3 + v0, e[p]);
+        for (int i = 0; i < 4; ++i) s2[i] += e[p][i] * e[p][i];
       }
-    }
-    if (k.noise != T(0)) {
-      for (int p = 0; p < k.V; ++p) {
-        float xi[4];
-        normal4(k.seed, (uint32_t)g4, (uint32_t)p, k.step, xi);
-        for (int i = 0; i < 4; ++i) Fh[p * N3 + g4 * 4 + i].x += k.noise * (T)xi[i];
+#pragma unroll
+    for (int p = 0; p < kPlainMaxV; ++p)
+      if (p < k.V) {
+        T r[4], f[4];
+        ld_quad_re(R + p * N3 + v0, r);
+        for (int i = 0; i < 4; ++i) {
+          const T ev = e[p][i];
+          f[i] = k.A * ev - k.B * ev * ev + k.C * ev * s2[i] + k.two_wel * ev * r[i];
+        }
+        if (k.noise != T(0)) {
+          float xi[4];
+          normal4(k.seed, (uint32_t)g4, (uint32_t)p, k.step, xi);
+          for (int i = 0; i < 4; ++i) f[i] += k.noise * (T)xi[i];
+        }
+        st_quad_re(Fh + p * N3 + v0, f);
+        if (E) st_quad_re(E + p * N3 + v0, e[p]);
       }
-    }
   }
 }
 
@@ -328,45 +369,52 @@ __global__ void p_prec_local(const T* __restrict__ fld, typename Cx<T>::T2* __re
                              const typename Cx<T>::T2* __restrict__ Rel) {
   const int V = k.V;
   for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
-    for (int i = 0; i < 4; ++i) {
-      const long v = g4 * 4 + i;
-      const T c = fld[v] + k.cbar;
-      T e[8], s2 = 0;
-      for (int a = 0; a < V; ++a) {
-        e[a] = fld[(1 + a) * N3 + v];
-        s2 += e[a] * e[a];
+    const long v0 = g4 * 4;
+    T c0[4], e[kPlainMaxV][4], s2[4] = {0, 0, 0, 0}, mu[4];
+    ld_quad(fld + v0, c0);
+#pragma unroll
+    for (int a = 0; a < kPlainMaxV; ++a)
+      if (a < V) {
+        ld_quad(fld + (1 + a) * N3 + v0, e[a]);
+        for (int i = 0; i < 4; ++i) s2[i] += e[a][i] * e[a][i];
       }
-      Fh[v] = cx<T>(T(0.5) * k.A[2] * s2 + k.A[1] * (c - k.c1), T(0));
-      E[v] = cx<T>(fld[v], T(0));
-      for (int a = 0; a < V; ++a) {
-        T t2 = 0, t4 = 0;
-        for (int b = 0; b < V; ++b)
-          if (b != a) {
-            const T q = e[b] * e[b];
-            t2 += q;
-            t4 += q * q;
-          }
-        const T et = e[a], e2 = et * et;
-        // ordered pairs i != j, both != a: (sum q)^2 - sum q^2
-        T f = (T(2) * k.A[5] * et + T(4) * k.A[6] * e2 * et) * t2 + T(2) * k.A[6] * et * t4 +
-              T(2) * k.A[7] * et * (t2 * t2 - t4) + k.A[2] * et * (c - k.c2) - k.A[3] * e2 * et + k.A[4] * e2 * e2 * et;
-        if (Rel) f += k.xi * T(2) * Rel[a * N3 + v].x * et;
-        Fh[(1 + a) * N3 + v] = cx<T>(f, T(0));
-        E[(1 + a) * N3 + v] = cx<T>(et, T(0));
-      }
+    for (int i = 0; i < 4; ++i) mu[i] = T(0.5) * k.A[2] * s2[i] + k.A[1] * (c0[i] + k.cbar - k.c1);
+    if (Rel && k.noise != T(0)) {
+      float xi[4];
+      normal4(k.seed, (uint32_t)g4, 0x8000u, k.step, xi);
+      for (int i = 0; i < 4; ++i) mu[i] += k.noise * (T)xi[i];
     }
-    if (k.noise != T(0)) {
-      for (int a = 0; a < V; ++a) {
-        float xi[4];
-        normal4(k.seed, (uint32_t)g4, (uint32_t)a, k.step, xi);
-        for (int i = 0; i < 4; ++i) Fh[(1 + a) * N3 + g4 * 4 + i].x += k.noise * (T)xi[i];
+    st_quad_re(Fh + v0, mu);
+    st_quad_re(E + v0, c0);
+#pragma unroll
+    for (int a = 0; a < kPlainMaxV; ++a)
+      if (a < V) {
+        T r[4], f[4];
+        if (Rel) ld_quad_re(Rel + a * N3 + v0, r);
+        for (int i = 0; i < 4; ++i) {
+          T t2 = 0, t4 = 0;
+#pragma unroll
+          for (int b = 0; b < kPlainMaxV; ++b)
+            if (b < V && b != a) {
+              const T q = e[b][i] * e[b][i];
+              t2 += q;
+              t4 += q * q;
+            }
+          const T c = c0[i] + k.cbar, et = e[a][i], e2 = et * et;
+          // ordered pairs i != j, both != a: (sum q)^2 - sum q^2
+          f[i] = (T(2) * k.A[5] * et + T(4) * k.A[6] * e2 * et) * t2 + T(2) * k.A[6] * et * t4 +
+                 T(2) * k.A[7] * et * (t2 * t2 - t4) + k.A[2] * et * (c - k.c2) - k.A[3] * e2 * et +
+                 k.A[4] * e2 * e2 * et;
+          if (Rel) f[i] += k.xi * T(2) * r[i] * et;
+        }
+        if (k.noise != T(0)) {
+          float xi[4];
+          normal4(k.seed, (uint32_t)g4, (uint32_t)a, k.step, xi);
+          for (int i = 0; i < 4; ++i) f[i] += k.noise * (T)xi[i];
+        }
+        st_quad_re(Fh + (1 + a) * N3 + v0, f);
+        st_quad_re(E + (1 + a) * N3 + v0, e[a]);
       }
-      if (Rel) {
-        float xi[4];
-        normal4(k.seed, (uint32_t)g4, 0x8000u, k.step, xi);
-        for (int i = 0; i < 4; ++i) Fh[g4 * 4 + i].x += k.noise * (T)xi[i];
-      }
-    }
   }
 }
 
