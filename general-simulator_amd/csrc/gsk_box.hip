@@ -118,34 +118,38 @@ template <typename T> PlainPKT<T> prec_k(const gsk_sim* s) {
   return k;
 }
 
-// R_p = IFFT[sum_q B_pq FFT(eta_q^2)] / N into R (complex, real part = R_p) from the V fields at eta
+// R_p = IFFT[sum_q B_pq FFT(eta_q^2)] / N from the V fields at eta into R as packed pairs
+// (R_2j + i R_2j+1 in complex field j). Every field on the way is real, so the squares and the
+// R_p share transforms two by two: ceil(V / 2) forward and ceil(V / 2) inverse transforms.
 template <typename T> int elastic_R(gsk_sim* s, const T* eta) {
   BoxWS& w = *s->bx;
-  const long N3 = (long)nvox(s), VN = (long)s->V * N3;
+  const long N3 = (long)nvox(s);
+  const int P = (s->V + 1) / 2;
   C2<T>* S = (C2<T>*)w.S;
   C2<T>* R = (C2<T>*)w.R;
   PlainKT<T> k = mart_k<T>(s);
-  LAUNCH(p_square<T>, grid_1d(VN), 256, 0, s, eta, S, VN);
-  CHK(c2c(s, S, true, s->V));
-  LAUNCH(p_contract<T>, grid_1d(N3), 256, 0, s, S, R, (const T*)w.B, N3, k);
-  return c2c(s, R, false, s->V);
+  LAUNCH(p_square_pair<T>, grid_1d(N3 / 4), 256, 0, s, eta, S, N3, s->V);
+  CHK(c2c(s, S, true, P));
+  LAUNCH(p_contract_pair<T>, grid_1d(N3), 256, 0, s, S, R, (const T*)w.B, N3, k);
+  return c2c(s, R, false, P);
 }
 
-// Martensite, spectral rule (the plain path's step on a box)
+// Martensite, spectral rule (the plain path's step on a box). Each force shares its transform with
+// its own field (F_p + i eta_p: V forward transforms) and the new fields share theirs in pairs
+// (ceil(V / 2) inverse transforms).
 template <typename T> int step_mart(gsk_sim* s) {
   BoxWS& w = *s->bx;
-  const long N3 = (long)nvox(s), VN = (long)s->V * N3;
+  const long N3 = (long)nvox(s);
   T* fld = (T*)w.fld;
   C2<T>* E = (C2<T>*)w.E;
   C2<T>* Fh = (C2<T>*)w.Fh;
   const PlainKT<T> k = mart_k<T>(s);
   CHK(elastic_R<T>(s, fld));
-  LAUNCH(p_force<T>, grid_1d(N3 / 4), 256, 0, s, fld, (const C2<T>*)w.R, Fh, E, N3, k);
+  LAUNCH(p_force_pair<T>, grid_1d(N3 / 4), 256, 0, s, fld, (const C2<T>*)w.R, Fh, N3, k);
   CHK(c2c(s, Fh, true, s->V));
-  CHK(c2c(s, E, true, s->V));
-  LAUNCH(p_update<T>, grid_1d(VN), 256, 0, s, E, Fh, k);
-  CHK(c2c(s, E, false, s->V));
-  LAUNCH(p_extract<T>, grid_1d(VN), 256, 0, s, E, fld, VN);
+  LAUNCH(p_update_pair<T>, grid_1d(N3), 256, 0, s, Fh, E, k);
+  CHK(c2c(s, E, false, (s->V + 1) / 2));
+  LAUNCH(p_extract_pair<T>, grid_1d(N3 / 4), 256, 0, s, E, fld, N3, s->V);
   return GSK_OK;
 }
 
@@ -153,19 +157,20 @@ template <typename T> int step_mart(gsk_sim* s) {
 // the V x V tensor, noise on c, no elasticity in the c update)
 template <typename T> int step_prec(gsk_sim* s) {
   BoxWS& w = *s->bx;
-  const long N3 = (long)nvox(s), FN = (long)s->F * N3;
+  const long N3 = (long)nvox(s);
   const bool ref = s->cfg.rule == GSK_RULE_REFERENCE;
   T* fld = (T*)w.fld;
   C2<T>* E = (C2<T>*)w.E;
   C2<T>* Fh = (C2<T>*)w.Fh;
   const PlainPKT<T> k = prec_k<T>(s);
   if (ref) CHK(elastic_R<T>(s, fld + N3));
-  LAUNCH(p_prec_local<T>, grid_1d(N3 / 4), 256, 0, s, fld, Fh, E, N3, k, ref ? (const C2<T>*)w.R : nullptr);
+  // forces and fields share transforms (F_f + i field_f), the new fields share theirs in pairs
+  LAUNCH((p_prec_local<T, true>), grid_1d(N3 / 4), 256, 0, s, fld, Fh, E, N3, k,
+         ref ? (const C2<T>*)w.R : nullptr);
   CHK(c2c(s, Fh, true, s->F));
-  CHK(c2c(s, E, true, s->F));
-  LAUNCH(p_prec_update<T>, grid_1d(FN), 256, 0, s, E, Fh, ref ? nullptr : (const T*)w.B, k);
-  CHK(c2c(s, E, false, s->F));
-  LAUNCH(p_extract<T>, grid_1d(FN), 256, 0, s, E, fld, FN);
+  LAUNCH(p_prec_update_pair<T>, grid_1d(N3), 256, 0, s, Fh, E, ref ? nullptr : (const T*)w.B, k);
+  CHK(c2c(s, E, false, (s->F + 1) / 2));
+  LAUNCH(p_extract_pair<T>, grid_1d(N3 / 4), 256, 0, s, E, fld, N3, s->F);
   return GSK_OK;
 }
 
@@ -173,10 +178,10 @@ template <typename T> int step_prec(gsk_sim* s) {
 int step_mart_ref(gsk_sim* s) {
   BoxWS& w = *s->bx;
   const gsk_config& c = s->cfg;
-  const long N3 = (long)nvox(s), VN = (long)s->V * N3;
+  const long N3 = (long)nvox(s);
   float* fld = (float*)w.fld;
   CHK(elastic_R<float>(s, fld));
-  LAUNCH(p_extract<float>, grid_1d(VN), 256, 0, s, (const float2*)w.R, s->rbuf, VN);
+  LAUNCH(p_extract_pair<float>, grid_1d(N3 / 4), 256, 0, s, (const float2*)w.R, s->rbuf, N3, s->V);
   RefMartK k{};
   k.a1 = (float)(0.02 * (c.temperature - c.T0));
   k.a2 = (float)c.B;

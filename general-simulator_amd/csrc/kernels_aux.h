@@ -189,6 +189,31 @@ __device__ __forceinline__ void st_quad_re(float2* p, const float (&v)[4]) {
 __device__ __forceinline__ void st_quad_re(double2* p, const double (&v)[4]) {
   for (int i = 0; i < 4; ++i) p[i] = make_double2(v[i], 0.0);
 }
+// a quad of complex values as real and imaginary parts
+__device__ __forceinline__ void ld_quad(const float2* p, float (&re)[4], float (&im)[4]) {
+  const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
+  re[0] = a.x, im[0] = a.y, re[1] = a.z, im[1] = a.w, re[2] = b.x, im[2] = b.y, re[3] = b.z, im[3] = b.w;
+}
+__device__ __forceinline__ void ld_quad(const double2* p, double (&re)[4], double (&im)[4]) {
+  for (int i = 0; i < 4; ++i) {
+    const double2 a = p[i];
+    re[i] = a.x, im[i] = a.y;
+  }
+}
+__device__ __forceinline__ void st_quad(float2* p, const float (&re)[4], const float (&im)[4]) {
+  reinterpret_cast<float4*>(p)[0] = make_float4(re[0], im[0], re[1], im[1]);
+  reinterpret_cast<float4*>(p)[1] = make_float4(re[2], im[2], re[3], im[3]);
+}
+__device__ __forceinline__ void st_quad(double2* p, const double (&re)[4], const double (&im)[4]) {
+  for (int i = 0; i < 4; ++i) p[i] = make_double2(re[i], im[i]);
+}
+__device__ __forceinline__ void st_quad(float* p, const float (&v)[4]) {
+  *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void st_quad(double* p, const double (&v)[4]) {
+  reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
+  reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
+}
 constexpr int kPlainMaxV = 8;  // variants on the plain path (martensite 1..8, precipitate 1..7)
 
 struct BoxDim {
@@ -210,13 +235,36 @@ template <typename T> struct PlainKT {
 };
 using PlainK = PlainKT<double>;
 
-// |k|^2 at voxel index v of the spectrum
-template <typename T, class K> __device__ __forceinline__ T ksq_at(const K& k, long v) {
-  int x, y, z;
-  k.d.coords(v, x, y, z);
+// |k|^2 at voxel (x, y, z) of the spectrum, and at voxel index v
+template <typename T, class K> __device__ __forceinline__ T ksq_xyz(const K& k, int x, int y, int z) {
   const T kx = k.kfx * (T)((x < k.d.nx / 2) ? x : x - k.d.nx), ky = k.kfy * (T)((y < k.d.ny / 2) ? y : y - k.d.ny),
           kz = k.kfz * (T)((z < k.d.nz / 2) ? z : z - k.d.nz);
   return kx * kx + ky * ky + kz * kz;
+}
+template <typename T, class K> __device__ __forceinline__ T ksq_at(const K& k, long v) {
+  int x, y, z;
+  k.d.coords(v, x, y, z);
+  return ksq_xyz<T>(k, x, y, z);
+}
+
+// Pairs: two real fields share one C2C transform as G = X + iY, and their transforms come back
+// at k from G(k) and G(-k): X(k) = (G(k) + conj G(-k)) / 2, Y(k) = (G(k) - conj G(-k)) / 2i; the
+// inverse of X + iY for two Hermitian spectra X, Y is x + iy. The box steps use this to halve
+// the transforms of real fields.
+// index of the voxel at -k (each coordinate m -> (n - m) mod n)
+__device__ __forceinline__ long neg_voxel(const BoxDim& d, int x, int y, int z) {
+  return ((long)(x ? d.nx - x : 0) * d.ny + (y ? d.ny - y : 0)) * d.nz + (z ? d.nz - z : 0);
+}
+template <typename T>
+__device__ __forceinline__ void unpack2(typename Cx<T>::T2 g, typename Cx<T>::T2 gm, typename Cx<T>::T2& X,
+                                        typename Cx<T>::T2& Y) {
+  X = cx<T>(T(0.5) * (g.x + gm.x), T(0.5) * (g.y - gm.y));
+  Y = cx<T>(T(0.5) * (g.y + gm.y), T(0.5) * (gm.x - g.x));
+}
+// X + iY
+template <typename T>
+__device__ __forceinline__ typename Cx<T>::T2 pack2(typename Cx<T>::T2 X, typename Cx<T>::T2 Y) {
+  return cx<T>(X.x - Y.y, X.y + Y.x);
 }
 
 // contiguous rows (z axis): L rows of N per tile, rows r0 .. r0 + L - 1 of the buffer
@@ -295,7 +343,7 @@ __global__ void p_contract(const typename Cx<T>::T2* __restrict__ S, typename Cx
 // p, keyed by the quad) into Fh, and (E != null) the fields themselves into E, both as complex with
 // zero imaginary parts. One thread per quad of voxels; the variants stay in registers.
 template <typename T>
-__global__ void p_force(const T* __restrict__ eta, const typename Cx<T>::T2* __restrict__ R,
+__global__ void __launch_bounds__(256) p_force(const T* __restrict__ eta, const typename Cx<T>::T2* __restrict__ R,
                         typename Cx<T>::T2* __restrict__ Fh, typename Cx<T>::T2* __restrict__ E, long N3,
                         PlainKT<T> k) {
   for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
@@ -342,6 +390,131 @@ __global__ void p_extract(const typename Cx<T>::T2* __restrict__ E, T* __restric
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) eta[i] = E[i].x;
 }
 
+// ---- the martensite step with paired transforms (box grids). P = ceil(V / 2) pairs; field 2j
+// of a pair is the real part, field 2j + 1 (when it exists) the imaginary part.
+// S_j = eta_2j^2 + i eta_2j+1^2
+template <typename T>
+__global__ void p_square_pair(const T* __restrict__ eta, typename Cx<T>::T2* __restrict__ S, long N3, int V) {
+  for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
+    const long v0 = g4 * 4;
+#pragma unroll
+    for (int j = 0; j < kPlainMaxV / 2; ++j)
+      if (2 * j < V) {
+        T a[4], b[4] = {0, 0, 0, 0};
+        ld_quad(eta + 2 * j * N3 + v0, a);
+        if (2 * j + 1 < V) ld_quad(eta + (2 * j + 1) * N3 + v0, b);
+        for (int i = 0; i < 4; ++i) a[i] *= a[i], b[i] *= b[i];
+        st_quad(S + j * N3 + v0, a, b);
+      }
+  }
+}
+
+// R_p = sum_q B_pq S_q / N with S_q unpacked from the transformed pairs; R packed as pairs
+template <typename T>
+__global__ void p_contract_pair(const typename Cx<T>::T2* __restrict__ S, typename Cx<T>::T2* __restrict__ R,
+                                const T* __restrict__ B, long N3, PlainKT<T> k) {
+  using T2 = typename Cx<T>::T2;
+  const int V = k.V;
+  for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < N3; v += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    k.d.coords(v, x, y, z);
+    const long nv = neg_voxel(k.d, x, y, z);
+    T2 sq[kPlainMaxV], r[kPlainMaxV];
+#pragma unroll
+    for (int j = 0; j < kPlainMaxV / 2; ++j)
+      if (2 * j < V) unpack2<T>(S[j * N3 + v], S[j * N3 + nv], sq[2 * j], sq[2 * j + 1]);
+#pragma unroll
+    for (int p = 0; p < kPlainMaxV; ++p) {
+      T ax = 0, ay = 0;
+#pragma unroll
+      for (int q = 0; q < kPlainMaxV; ++q)
+        if (p < V && q < V) {
+          const T b = B[(long)(p * V + q) * N3 + v];
+          ax += b * sq[q].x;
+          ay += b * sq[q].y;
+        }
+      r[p] = cx<T>(ax * k.inv_n3, ay * k.inv_n3);
+    }
+#pragma unroll
+    for (int j = 0; j < kPlainMaxV / 2; ++j)
+      if (2 * j < V) R[j * N3 + v] = pack2<T>(r[2 * j], 2 * j + 1 < V ? r[2 * j + 1] : cx<T>(T(0), T(0)));
+  }
+}
+
+// G_p = F_p + i eta_p with F_p as p_force (R_p from the packed pairs, noise keyed as p_force)
+template <typename T>
+__global__ void __launch_bounds__(256) p_force_pair(const T* __restrict__ eta, const typename Cx<T>::T2* __restrict__ R,
+                             typename Cx<T>::T2* __restrict__ G, long N3, PlainKT<T> k) {
+  for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
+    const long v0 = g4 * 4;
+    T e[kPlainMaxV][4], s2[4] = {0, 0, 0, 0}, r[2][4];
+#pragma unroll
+    for (int p = 0; p < kPlainMaxV; ++p)
+      if (p < k.V) {
+        ld_quad(eta + p * N3 + v0, e[p]);
+        for (int i = 0; i < 4; ++i) s2[i] += e[p][i] * e[p][i];
+      }
+#pragma unroll
+    for (int p = 0; p < kPlainMaxV; ++p)
+      if (p < k.V) {
+        if (!(p & 1)) ld_quad(R + (p >> 1) * N3 + v0, r[0], r[1]);
+        T f[4];
+        for (int i = 0; i < 4; ++i) {
+          const T ev = e[p][i];
+          f[i] = k.A * ev - k.B * ev * ev + k.C * ev * s2[i] + k.two_wel * ev * r[p & 1][i];
+        }
+        if (k.noise != T(0)) {
+          float xi[4];
+          normal4(k.seed, (uint32_t)g4, (uint32_t)p, k.step, xi);
+          for (int i = 0; i < 4; ++i) f[i] += k.noise * (T)xi[i];
+        }
+        st_quad(G + p * N3 + v0, f, e[p]);
+      }
+  }
+}
+
+// eta_hat_p <- (eta_hat_p - dt L F_hat_p) / (1 + dt L kappa k^2) / N with both unpacked from G_p;
+// the new spectra packed as pairs into H
+template <typename T>
+__global__ void p_update_pair(const typename Cx<T>::T2* __restrict__ G, typename Cx<T>::T2* __restrict__ H,
+                              PlainKT<T> k) {
+  using T2 = typename Cx<T>::T2;
+  const long N3 = (long)k.d.nx * k.d.ny * k.d.nz;
+  for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < N3; v += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    k.d.coords(v, x, y, z);
+    const long nv = neg_voxel(k.d, x, y, z);
+    const T den = T(1) / (T(1) + k.dtLk * ksq_xyz<T>(k, x, y, z)) * k.inv_n3;
+    T2 en[kPlainMaxV];
+#pragma unroll
+    for (int p = 0; p < kPlainMaxV; ++p)
+      if (p < k.V) {
+        T2 f, e;
+        unpack2<T>(G[p * N3 + v], G[p * N3 + nv], f, e);
+        en[p] = cx<T>((e.x - k.dtL * f.x) * den, (e.y - k.dtL * f.y) * den);
+      }
+#pragma unroll
+    for (int j = 0; j < kPlainMaxV / 2; ++j)
+      if (2 * j < k.V) H[j * N3 + v] = pack2<T>(en[2 * j], 2 * j + 1 < k.V ? en[2 * j + 1] : cx<T>(T(0), T(0)));
+  }
+}
+
+// field f of nf real fields from the real (f even) or imaginary (f odd) part of pair f / 2
+template <typename T>
+__global__ void p_extract_pair(const typename Cx<T>::T2* __restrict__ H, T* __restrict__ out, long N3, int nf) {
+  for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
+    const long v0 = g4 * 4;
+#pragma unroll
+    for (int j = 0; j < kPlainMaxV / 2; ++j)
+      if (2 * j < nf) {
+        T a[4], b[4];
+        ld_quad(H + j * N3 + v0, a, b);
+        st_quad(out + 2 * j * N3 + v0, a);
+        if (2 * j + 1 < nf) st_quad(out + (2 * j + 1) * N3 + v0, b);
+      }
+  }
+}
+
 // ---- precipitate on the plain path (src/dynamics_precipitate.cu:118-275), one kernel per
 // operation. Fields: [c, eta_1..eta_V], F = V + 1. The spectral rule has the c-sourced elasticity
 // and the stabilisation S; the reference rule (Rel != null) adds xi 2 R_v eta_v to F_v
@@ -362,9 +535,10 @@ using PlainPK = PlainPKT<double>;
 // mu_c = A2/2 sum eta^2 + A1 (c - c1) (:118-127) and F_v (:136-162) + noise a xi_v (Philox field v)
 // of every voxel into Fh; the fields themselves into E (both as complex with zero imaginary part).
 // Rel (reference rule, may be null): the elastic R_v of the eta^2 source as complex spectra whose
-// real parts hold R_v.
-template <typename T>
-__global__ void p_prec_local(const T* __restrict__ fld, typename Cx<T>::T2* __restrict__ Fh,
+// real parts hold R_v. PK (box grids): Fh_f = F_f + i field_f instead (E unused), and Rel holds R
+// as packed pairs.
+template <typename T, bool PK = false>
+__global__ void __launch_bounds__(256) p_prec_local(const T* __restrict__ fld, typename Cx<T>::T2* __restrict__ Fh,
                              typename Cx<T>::T2* __restrict__ E, long N3, PlainPKT<T> k,
                              const typename Cx<T>::T2* __restrict__ Rel) {
   const int V = k.V;
@@ -384,13 +558,25 @@ __global__ void p_prec_local(const T* __restrict__ fld, typename Cx<T>::T2* __re
       normal4(k.seed, (uint32_t)g4, 0x8000u, k.step, xi);
       for (int i = 0; i < 4; ++i) mu[i] += k.noise * (T)xi[i];
     }
-    st_quad_re(Fh + v0, mu);
-    st_quad_re(E + v0, c0);
+    if (PK) {
+      st_quad(Fh + v0, mu, c0);
+    } else {
+      st_quad_re(Fh + v0, mu);
+      st_quad_re(E + v0, c0);
+    }
+    T rp[2][4];
 #pragma unroll
     for (int a = 0; a < kPlainMaxV; ++a)
       if (a < V) {
         T r[4], f[4];
-        if (Rel) ld_quad_re(Rel + a * N3 + v0, r);
+        if (Rel) {
+          if (PK) {
+            if (!(a & 1)) ld_quad(Rel + (a >> 1) * N3 + v0, rp[0], rp[1]);
+            for (int i = 0; i < 4; ++i) r[i] = rp[a & 1][i];
+          } else {
+            ld_quad_re(Rel + a * N3 + v0, r);
+          }
+        }
         for (int i = 0; i < 4; ++i) {
           T t2 = 0, t4 = 0;
 #pragma unroll
@@ -412,8 +598,12 @@ __global__ void p_prec_local(const T* __restrict__ fld, typename Cx<T>::T2* __re
           normal4(k.seed, (uint32_t)g4, (uint32_t)a, k.step, xi);
           for (int i = 0; i < 4; ++i) f[i] += k.noise * (T)xi[i];
         }
-        st_quad_re(Fh + (1 + a) * N3 + v0, f);
-        st_quad_re(E + (1 + a) * N3 + v0, e[a]);
+        if (PK) {
+          st_quad(Fh + (1 + a) * N3 + v0, f, e[a]);
+        } else {
+          st_quad_re(Fh + (1 + a) * N3 + v0, f);
+          st_quad_re(E + (1 + a) * N3 + v0, e[a]);
+        }
       }
   }
 }
@@ -440,6 +630,42 @@ __global__ void p_prec_update(typename Cx<T>::T2* __restrict__ E, const typename
       const T den = k.inv_n3 / (T(1) + k.dtLke * k2);
       E[i] = cx<T>((e.x - k.dtL * f.x) * den, (e.y - k.dtL * f.y) * den);
     }
+  }
+}
+
+// p_prec_update on box grids: c_hat, mu_hat and eta_hat_v, F_hat_v unpacked from G_f = F_f + i
+// field_f (the output of p_prec_local<T, true> after its transform); the new spectra packed as
+// pairs into H
+template <typename T>
+__global__ void p_prec_update_pair(const typename Cx<T>::T2* __restrict__ G, typename Cx<T>::T2* __restrict__ H,
+                                   const T* __restrict__ Bc, PlainPKT<T> k) {
+  using T2 = typename Cx<T>::T2;
+  const long N3 = (long)k.d.nx * k.d.ny * k.d.nz;
+  const int F = k.V + 1;
+  for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < N3; v += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    k.d.coords(v, x, y, z);
+    const long nv = neg_voxel(k.d, x, y, z);
+    const T k2 = ksq_xyz<T>(k, x, y, z);
+    T2 en[kPlainMaxV];
+#pragma unroll
+    for (int f = 0; f < kPlainMaxV; ++f)
+      if (f < F) {
+        T2 fh, e;
+        unpack2<T>(G[f * N3 + v], G[f * N3 + nv], fh, e);
+        if (f == 0) {
+          const T a = (k.celastic ? k.xi * Bc[v] : T(0)) - k.S;
+          const T mx = fh.x + a * e.x, my = fh.y + a * e.y;
+          const T den = k.inv_n3 / (T(1) + k.dtM * k.S * k2 + k.dtMkc * k2 * k2);
+          en[f] = cx<T>((e.x - k.dtM * k2 * mx) * den, (e.y - k.dtM * k2 * my) * den);
+        } else {
+          const T den = k.inv_n3 / (T(1) + k.dtLke * k2);
+          en[f] = cx<T>((e.x - k.dtL * fh.x) * den, (e.y - k.dtL * fh.y) * den);
+        }
+      }
+#pragma unroll
+    for (int j = 0; j < kPlainMaxV / 2; ++j)
+      if (2 * j < F) H[j * N3 + v] = pack2<T>(en[2 * j], 2 * j + 1 < F ? en[2 * j + 1] : cx<T>(T(0), T(0)));
   }
 }
 

@@ -36,11 +36,13 @@ def grid(dims, sp):
     return O.grid(*dims, *sp)
 
 
-@pytest.mark.parametrize("dims,sp", BOXES)
+# the box steps share transforms between pairs of real fields: odd and even V, one variant
+# (a single pair with an empty partner) and the largest count
+@pytest.mark.parametrize("dims,sp,V", [(d, sp, 3) for d, sp in BOXES] + [(*BOXES[0], V) for V in (1, 2, 4, 8)])
 @pytest.mark.parametrize("path", [gsk.PATH_PLAIN64, gsk.PATH_FUSED32])
-def test_box_mart_spectral_matches_oracle(gpu, dims, sp, path):
-    say(f"box mart spectral dims={dims} sp={sp} path={path}")
-    V, steps = 3, 12
+def test_box_mart_spectral_matches_oracle(gpu, dims, sp, V, path):
+    say(f"box mart spectral dims={dims} sp={sp} V={V} path={path}")
+    steps = 12
     C = gsk.cubic_modulus(0.7, 0.3, 0.2)
     eps = gsk.mart_eigenstrains(V)
     cfg = box_cfg(gsk.PHYS_MART, dims, sp, nvar=V, path=path, A=-0.5, w_el=20.0, noise=0.002, seed=9)
@@ -68,11 +70,13 @@ def test_box_mart_spectral_matches_oracle(gpu, dims, sp, path):
     assert eE < (1e-11 if path == gsk.PATH_PLAIN64 else 1e-4)
 
 
-@pytest.mark.parametrize("dims,sp", BOXES[:2])
+# F = V + 1 fields: even (V = 3, 7) and odd (V = 2, 4) counts of fields to pair
+@pytest.mark.parametrize("dims,sp,V", [(d, sp, 3) for d, sp in BOXES[:2]] + [(*BOXES[0], 2), (*BOXES[2], 4),
+                                                                           (*BOXES[1], 7)])
 @pytest.mark.parametrize("path", [gsk.PATH_PLAIN64, gsk.PATH_FUSED32])
-def test_box_prec_spectral_matches_oracle(gpu, dims, sp, path):
-    say(f"box precipitate spectral dims={dims} sp={sp} path={path}")
-    V, steps = 3, 20
+def test_box_prec_spectral_matches_oracle(gpu, dims, sp, V, path):
+    say(f"box precipitate spectral dims={dims} sp={sp} V={V} path={path}")
+    steps = 20
     C = gsk.cubic_modulus(3.0, 1.0, 1.0)
     cfg = box_cfg(gsk.PHYS_PREC, dims, sp, nvar=V, path=path, M=1.0, L=1.0, kappa=1.5, kappa_c=1.5, dt=0.005, S=54.0,
                   noise=0.001, seed=3)
@@ -107,10 +111,12 @@ def test_box_prec_spectral_matches_oracle(gpu, dims, sp, path):
 
 # (16, 32, 96): z rows of 24 quads, which do not divide a wave, so the update kernel loads the z
 # neighbours across quad boundaries instead of taking them from the neighbouring lane
-@pytest.mark.parametrize("dims,sp", BOXES[:2] + [((16, 32, 96), (1.0, 0.8, 1.2))])
-def test_box_mart_reference_rule_matches_oracle(gpu, dims, sp):
-    say(f"box mart reference rule dims={dims} sp={sp}")
-    V, steps = 3, 40
+# V = 4: R of two full pairs
+@pytest.mark.parametrize("dims,sp,V", [(d, sp, 3) for d, sp in BOXES[:2] + [((16, 32, 96), (1.0, 0.8, 1.2))]]
+                         + [(*BOXES[1], 4)])
+def test_box_mart_reference_rule_matches_oracle(gpu, dims, sp, V):
+    say(f"box mart reference rule dims={dims} sp={sp} V={V}")
+    steps = 40
     C = gsk.cubic_modulus(3.5, 1.5, 1.0)
     eps = gsk.mart_eigenstrains(V, base=0.0, axis=0.01)
     cfg = box_cfg(gsk.PHYS_MART, dims, sp, ref=True, nvar=V, temperature=300.0, noise=1e-4, w_el=1e3, seed=6, dt=0.005)
@@ -141,10 +147,11 @@ def test_box_mart_reference_rule_matches_oracle(gpu, dims, sp):
     assert np.isfinite(fm).all() and fm[3] != 0.0
 
 
-def test_box_prec_reference_rule_matches_oracle(gpu):
+@pytest.mark.parametrize("V", [3, 2])
+def test_box_prec_reference_rule_matches_oracle(gpu, V):
     dims, sp = (32, 64, 16), (1.0, 0.5, 1.0)
-    say(f"box precipitate reference rule dims={dims} sp={sp}")
-    V, steps = 3, 30
+    say(f"box precipitate reference rule dims={dims} sp={sp} V={V}")
+    steps = 30
     C = gsk.cubic_modulus(3.5, 1.5, 1.0)
     eps = gsk.mart_eigenstrains(V, base=0.0, axis=0.02)
     cfg = box_cfg(gsk.PHYS_PREC, dims, sp, ref=True, nvar=V, M=1.0, L=1.0, kappa=1.5, kappa_c=1.5, dt=0.002, seed=8,
