@@ -130,7 +130,14 @@ template <typename T> int elastic_R(gsk_sim* s, const T* eta) {
   PlainKT<T> k = mart_k<T>(s);
   LAUNCH(p_square_pair<T>, grid_1d(N3 / 4), 256, 0, s, eta, S, N3, s->V);
   CHK(c2c(s, S, true, P));
-  LAUNCH(p_contract_pair<T>, grid_1d(N3), 256, 0, s, S, R, (const T*)w.B, N3, k);
+  switch (s->V) {
+#define GSK_BOX_CONTRACT(V) \
+  case V: LAUNCH((p_contract_pair<T, V>), grid_1d(N3), 256, 0, s, S, R, (const T*)w.B, N3, k); break;
+    GSK_BOX_CONTRACT(1) GSK_BOX_CONTRACT(2) GSK_BOX_CONTRACT(3) GSK_BOX_CONTRACT(4)
+    GSK_BOX_CONTRACT(5) GSK_BOX_CONTRACT(6) GSK_BOX_CONTRACT(7) GSK_BOX_CONTRACT(8)
+#undef GSK_BOX_CONTRACT
+    default: return fail(GSK_ERR_ARG, "nvar out of range");
+  }
   return c2c(s, R, false, P);
 }
 

@@ -409,35 +409,40 @@ __global__ void p_square_pair(const T* __restrict__ eta, typename Cx<T>::T2* __r
   }
 }
 
-// R_p = sum_q B_pq S_q / N with S_q unpacked from the transformed pairs; R packed as pairs
-template <typename T>
-__global__ void p_contract_pair(const typename Cx<T>::T2* __restrict__ S, typename Cx<T>::T2* __restrict__ R,
+// R_p = sum_q B_pq S_q / N with S_q unpacked from the transformed pairs; R packed as pairs. V is a
+// template argument so that the V^2 loads of B carry no guards and are all in flight together.
+template <typename T, int V>
+__global__ void __launch_bounds__(256) p_contract_pair(const typename Cx<T>::T2* __restrict__ S, typename Cx<T>::T2* __restrict__ R,
                                 const T* __restrict__ B, long N3, PlainKT<T> k) {
   using T2 = typename Cx<T>::T2;
-  const int V = k.V;
+  constexpr int P = (V + 1) / 2;
   for (long v = blockIdx.x * (long)blockDim.x + threadIdx.x; v < N3; v += (long)gridDim.x * blockDim.x) {
     int x, y, z;
     k.d.coords(v, x, y, z);
     const long nv = neg_voxel(k.d, x, y, z);
-    T2 sq[kPlainMaxV], r[kPlainMaxV];
+    T2 g[P], gm[P], sq[2 * P], r[2 * P];
+    T b[V][V];
 #pragma unroll
-    for (int j = 0; j < kPlainMaxV / 2; ++j)
-      if (2 * j < V) unpack2<T>(S[j * N3 + v], S[j * N3 + nv], sq[2 * j], sq[2 * j + 1]);
+    for (int j = 0; j < P; ++j) g[j] = S[j * N3 + v], gm[j] = S[j * N3 + nv];
 #pragma unroll
-    for (int p = 0; p < kPlainMaxV; ++p) {
+    for (int p = 0; p < V; ++p)
+#pragma unroll
+      for (int q = 0; q < V; ++q) b[p][q] = B[(long)(p * V + q) * N3 + v];
+#pragma unroll
+    for (int j = 0; j < P; ++j) unpack2<T>(g[j], gm[j], sq[2 * j], sq[2 * j + 1]);
+#pragma unroll
+    for (int p = 0; p < V; ++p) {
       T ax = 0, ay = 0;
 #pragma unroll
-      for (int q = 0; q < kPlainMaxV; ++q)
-        if (p < V && q < V) {
-          const T b = B[(long)(p * V + q) * N3 + v];
-          ax += b * sq[q].x;
-          ay += b * sq[q].y;
-        }
+      for (int q = 0; q < V; ++q) {
+        ax += b[p][q] * sq[q].x;
+        ay += b[p][q] * sq[q].y;
+      }
       r[p] = cx<T>(ax * k.inv_n3, ay * k.inv_n3);
     }
+    if constexpr ((V & 1) != 0) r[V] = cx<T>(T(0), T(0));
 #pragma unroll
-    for (int j = 0; j < kPlainMaxV / 2; ++j)
-      if (2 * j < V) R[j * N3 + v] = pack2<T>(r[2 * j], 2 * j + 1 < V ? r[2 * j + 1] : cx<T>(T(0), T(0)));
+    for (int j = 0; j < P; ++j) R[j * N3 + v] = pack2<T>(r[2 * j], r[2 * j + 1]);
   }
 }
 
