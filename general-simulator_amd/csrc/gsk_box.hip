@@ -172,8 +172,7 @@ template <typename T> int step_prec(gsk_sim* s) {
   const PlainPKT<T> k = prec_k<T>(s);
   if (ref) CHK(elastic_R<T>(s, fld + N3));
   // forces and fields share transforms (F_f + i field_f), the new fields share theirs in pairs
-  LAUNCH((p_prec_local<T, true>), grid_1d(N3 / 4), 256, 0, s, fld, Fh, E, N3, k,
-         ref ? (const C2<T>*)w.R : nullptr);
+  LAUNCH(p_prec_local<T>, grid_1d(N3 / 4), 256, 0, s, fld, Fh, N3, k, ref ? (const C2<T>*)w.R : nullptr);
   CHK(c2c(s, Fh, true, s->F));
   LAUNCH(p_prec_update_pair<T>, grid_1d(N3), 256, 0, s, Fh, E, ref ? nullptr : (const T*)w.B, k);
   CHK(c2c(s, E, false, (s->F + 1) / 2));

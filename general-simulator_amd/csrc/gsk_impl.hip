@@ -686,34 +686,44 @@ struct Impl {
     return GSK_OK;
   }
 
+  // Every field the step transforms is real, so two share one C2C transform and the consumer takes
+  // them apart from G(k) and G(-k): squares and R_p in ceil(V/2) pairs, each force with its own
+  // field (F_p + i eta_p), the new fields in pairs for the inverse (the box steps do the same).
   static int step_mart_plain(gsk_sim* s) {
-    const long N3 = (long)nvox(s), VN = s->V * N3;
+    const long N3 = (long)nvox(s);
+    const int P = (s->V + 1) / 2;
     PlainK k = plain_consts(s);
-    LAUNCH(p_square<double>, grid_1d(VN), 256, 0, s, s->eta64, s->S64, VN);
-    CHK(c2c(s, s->S64, true, s->V));
-    LAUNCH(p_contract<double>, grid_1d(N3), 256, 0, s, s->S64, s->R64, s->B64, N3, k);
-    CHK(c2c(s, s->R64, false, s->V));
-    LAUNCH(p_force<double>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->R64, s->F64, s->E64, N3, k);
+    LAUNCH(p_square_pair<double>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->S64, N3, s->V);
+    CHK(c2c(s, s->S64, true, P));
+    switch (s->V) {
+#define GSK_PLAIN_CONTRACT(V) \
+  case V: LAUNCH((p_contract_pair<double, V>), grid_1d(N3), 256, 0, s, s->S64, s->R64, s->B64, N3, k); break;
+      GSK_PLAIN_CONTRACT(1) GSK_PLAIN_CONTRACT(2) GSK_PLAIN_CONTRACT(3) GSK_PLAIN_CONTRACT(4)
+      GSK_PLAIN_CONTRACT(5) GSK_PLAIN_CONTRACT(6) GSK_PLAIN_CONTRACT(7) GSK_PLAIN_CONTRACT(8)
+#undef GSK_PLAIN_CONTRACT
+      default: return fail(GSK_ERR_ARG, "nvar out of range");
+    }
+    CHK(c2c(s, s->R64, false, P));
+    LAUNCH(p_force_pair<double>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->R64, s->F64, N3, k);
     CHK(c2c(s, s->F64, true, s->V));
-    CHK(c2c(s, s->E64, true, s->V));
-    LAUNCH(p_update<double>, grid_1d(VN), 256, 0, s, s->E64, s->F64, k);
-    CHK(c2c(s, s->E64, false, s->V));
-    LAUNCH(p_extract<double>, grid_1d(VN), 256, 0, s, s->E64, s->eta64, VN);
+    LAUNCH(p_update_pair<double>, grid_1d(N3), 256, 0, s, s->F64, s->E64, k);
+    CHK(c2c(s, s->E64, false, P));
+    LAUNCH(p_extract_pair<double>, grid_1d(N3 / 4), 256, 0, s, s->E64, s->eta64, N3, s->V);
     return GSK_OK;
   }
 
   // Precipitate spectral rule on the FP64 plain path (src/dynamics_precipitate.cu:118-275): local
-  // forces, C2C of forces and fields, the semi-implicit CH and AC updates, inverse C2C.
+  // forces, C2C of forces and fields, the semi-implicit CH and AC updates, inverse C2C; forces and
+  // fields share transforms (F_f + i field_f) and the new fields share theirs in pairs.
   static int step_prec_plain(gsk_sim* s) {
     const int F = s->F;
-    const long N3 = (long)nvox(s), FN = F * N3;
+    const long N3 = (long)nvox(s);
     const PlainPK k = plain_prec_consts(s);
-    LAUNCH(p_prec_local<double>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->F64, s->E64, N3, k, nullptr);
+    LAUNCH(p_prec_local<double>, grid_1d(N3 / 4), 256, 0, s, s->eta64, s->F64, N3, k, nullptr);
     CHK(c2c(s, s->F64, true, F));
-    CHK(c2c(s, s->E64, true, F));
-    LAUNCH(p_prec_update<double>, grid_1d(FN), 256, 0, s, s->E64, s->F64, s->B64, k);
-    CHK(c2c(s, s->E64, false, F));
-    LAUNCH(p_extract<double>, grid_1d(FN), 256, 0, s, s->E64, s->eta64, FN);
+    LAUNCH(p_prec_update_pair<double>, grid_1d(N3), 256, 0, s, s->F64, s->E64, s->B64, k);
+    CHK(c2c(s, s->E64, false, (F + 1) / 2));
+    LAUNCH(p_extract_pair<double>, grid_1d(N3 / 4), 256, 0, s, s->E64, s->eta64, N3, F);
     return GSK_OK;
   }
 

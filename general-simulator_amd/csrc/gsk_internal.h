@@ -567,44 +567,36 @@ inline void build_slots(gsk_sim* s) {
   const double hs = (double)hs_count(s) * 8.0;  // one half-spectrum complex field
   const double rv = (double)nvox(s) * 4.0;      // one real fp32 field
   const int F = s->F;
-  if (s->cfg.path == GSK_PATH_PLAIN64 && s->cfg.physics == GSK_PHYS_PREC) {
-    const double cf = (double)nvox(s) * 16.0, r8 = (double)nvox(s) * 8.0;
-    auto add_c2c = [&](const char* a, const char* b, const char* c) {
-      for (int f = 0; f < F; ++f) {
-        s->slots.push_back({a, 2 * cf});
-        s->slots.push_back({b, 2 * cf});
-        s->slots.push_back({c, 2 * cf});
-      }
-    };
-    s->slots.push_back({"p_prec_local", F * (r8 + 2 * cf)});
-    add_c2c("p_z_fwdF", "p_y_fwdF", "p_x_fwdF");
-    add_c2c("p_z_fwdE", "p_y_fwdE", "p_x_fwdE");
-    s->slots.push_back({"p_prec_update", F * 3 * cf + r8});
-    add_c2c("p_z_invE", "p_y_invE", "p_x_invE");
-    s->slots.push_back({"p_extract", F * (cf + r8)});
-    return;
-  }
   if (s->cfg.path == GSK_PATH_PLAIN64) {
+    // one kernel per operation; real fields share C2C transforms (pairs, or a force with its own
+    // field), and the unpacking kernels read each packed spectrum at k and at -k
     const double cf = (double)nvox(s) * 16.0, r8 = (double)nvox(s) * 8.0;
-    const int V = s->V;
-    auto add_c2c = [&](const char* a, const char* b, const char* c) {
-      for (int f = 0; f < V; ++f) {
+    auto add_c2c = [&](int nf, const char* a, const char* b, const char* c) {
+      for (int f = 0; f < nf; ++f) {
         s->slots.push_back({a, 2 * cf});
         s->slots.push_back({b, 2 * cf});
         s->slots.push_back({c, 2 * cf});
       }
     };
-    s->slots.push_back({"p_square", V * (r8 + cf)});
-    add_c2c("p_z_fwdS", "p_y_fwdS", "p_x_fwdS");
-    s->slots.push_back({"p_contract", V * 2 * cf + V * V * r8});
-    add_c2c("p_z_invR", "p_y_invR", "p_x_invR");
-    s->slots.push_back({"p_force", V * (r8 + cf + cf)});
-    s->slots.push_back({"p_load", V * (r8 + cf)});
-    add_c2c("p_z_fwdF", "p_y_fwdF", "p_x_fwdF");
-    add_c2c("p_z_fwdE", "p_y_fwdE", "p_x_fwdE");
-    s->slots.push_back({"p_update", V * 3 * cf});
-    add_c2c("p_z_invE", "p_y_invE", "p_x_invE");
-    s->slots.push_back({"p_extract", V * (cf + r8)});
+    if (s->cfg.physics == GSK_PHYS_PREC) {
+      const int PF = (F + 1) / 2;
+      s->slots.push_back({"p_prec_local", F * (r8 + cf)});
+      add_c2c(F, "p_z_fwdG", "p_y_fwdG", "p_x_fwdG");
+      s->slots.push_back({"p_prec_update_pair", 2 * F * cf + r8 + PF * cf});
+      add_c2c(PF, "p_z_invE", "p_y_invE", "p_x_invE");
+      s->slots.push_back({"p_extract_pair", PF * cf + F * r8});
+      return;
+    }
+    const int V = s->V, P = (V + 1) / 2;
+    s->slots.push_back({"p_square_pair", V * r8 + P * cf});
+    add_c2c(P, "p_z_fwdS", "p_y_fwdS", "p_x_fwdS");
+    s->slots.push_back({"p_contract_pair", 3 * P * cf + V * V * r8});
+    add_c2c(P, "p_z_invR", "p_y_invR", "p_x_invR");
+    s->slots.push_back({"p_force_pair", V * r8 + P * cf + V * cf});
+    add_c2c(V, "p_z_fwdG", "p_y_fwdG", "p_x_fwdG");
+    s->slots.push_back({"p_update_pair", 2 * V * cf + P * cf});
+    add_c2c(P, "p_z_invE", "p_y_invE", "p_x_invE");
+    s->slots.push_back({"p_extract_pair", P * cf + V * r8});
     return;
   }
   if (s->cfg.physics == GSK_PHYS_MART && s->cfg.rule == GSK_RULE_REFERENCE) {

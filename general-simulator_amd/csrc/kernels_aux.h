@@ -173,22 +173,6 @@ __device__ __forceinline__ void ld_quad(const double* p, double (&o)[4]) {
   const double2 a = reinterpret_cast<const double2*>(p)[0], b = reinterpret_cast<const double2*>(p)[1];
   o[0] = a.x, o[1] = a.y, o[2] = b.x, o[3] = b.y;
 }
-// real parts of a quad of complex values
-__device__ __forceinline__ void ld_quad_re(const float2* p, float (&o)[4]) {
-  const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
-  o[0] = a.x, o[1] = a.z, o[2] = b.x, o[3] = b.z;
-}
-__device__ __forceinline__ void ld_quad_re(const double2* p, double (&o)[4]) {
-  for (int i = 0; i < 4; ++i) o[i] = p[i].x;
-}
-// a quad of complex values with zero imaginary parts
-__device__ __forceinline__ void st_quad_re(float2* p, const float (&v)[4]) {
-  reinterpret_cast<float4*>(p)[0] = make_float4(v[0], 0.f, v[1], 0.f);
-  reinterpret_cast<float4*>(p)[1] = make_float4(v[2], 0.f, v[3], 0.f);
-}
-__device__ __forceinline__ void st_quad_re(double2* p, const double (&v)[4]) {
-  for (int i = 0; i < 4; ++i) p[i] = make_double2(v[i], 0.0);
-}
 // a quad of complex values as real and imaginary parts
 __device__ __forceinline__ void ld_quad(const float2* p, float (&re)[4], float (&im)[4]) {
   const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
@@ -321,70 +305,6 @@ __global__ void p_load(const T* __restrict__ eta, typename Cx<T>::T2* __restrict
     E[i] = cx<T>(eta[i], T(0));
 }
 
-// R_p = sum_q B_pq S_q / N (B: [p][q][voxel])
-template <typename T>
-__global__ void p_contract(const typename Cx<T>::T2* __restrict__ S, typename Cx<T>::T2* __restrict__ R,
-                           const T* __restrict__ B, long N3, PlainKT<T> k) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N3; i += (long)gridDim.x * blockDim.x) {
-    for (int p = 0; p < k.V; ++p) {
-      T ax = 0, ay = 0;
-      for (int q = 0; q < k.V; ++q) {
-        const T b = B[(long)(p * k.V + q) * N3 + i];
-        const typename Cx<T>::T2 s = S[q * N3 + i];
-        ax += b * s.x;
-        ay += b * s.y;
-      }
-      R[p * N3 + i] = cx<T>(ax * k.inv_n3, ay * k.inv_n3);
-    }
-  }
-}
-
-// F_p = A eta_p - B eta_p^2 + C eta_p sum_q eta_q^2 + 2 w_el eta_p R_p (+ noise xi_p, Philox field
-// p, keyed by the quad) into Fh, and (E != null) the fields themselves into E, both as complex with
-// zero imaginary parts. One thread per quad of voxels; the variants stay in registers.
-template <typename T>
-__global__ void __launch_bounds__(256) p_force(const T* __restrict__ eta, const typename Cx<T>::T2* __restrict__ R,
-                        typename Cx<T>::T2* __restrict__ Fh, typename Cx<T>::T2* __restrict__ E, long N3,
-                        PlainKT<T> k) {
-  for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
-    const long v0 = g4 * 4;
-    T e[kPlainMaxV][4], s2[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int p = 0; p < kPlainMaxV; ++p)
-      if (p < k.V) {
-        ld_quad(eta + p * N3 + v0, e[p]);
-        for (int i = 0; i < 4; ++i) s2[i] += e[p][i] * e[p][i];
-      }
-#pragma unroll
-    for (int p = 0; p < kPlainMaxV; ++p)
-      if (p < k.V) {
-        T r[4], f[4];
-        ld_quad_re(R + p * N3 + v0, r);
-        for (int i = 0; i < 4; ++i) {
-          const T ev = e[p][i];
-          f[i] = k.A * ev - k.B * ev * ev + k.C * ev * s2[i] + k.two_wel * ev * r[i];
-        }
-        if (k.noise != T(0)) {
-          float xi[4];
-          normal4(k.seed, (uint32_t)g4, (uint32_t)p, k.step, xi);
-          for (int i = 0; i < 4; ++i) f[i] += k.noise * (T)xi[i];
-        }
-        st_quad_re(Fh + p * N3 + v0, f);
-        if (E) st_quad_re(E + p * N3 + v0, e[p]);
-      }
-  }
-}
-
-template <typename T>
-__global__ void p_update(typename Cx<T>::T2* __restrict__ E, const typename Cx<T>::T2* __restrict__ Fh, PlainKT<T> k) {
-  const long N3 = (long)k.d.nx * k.d.ny * k.d.nz;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < (long)k.V * N3; i += (long)gridDim.x * blockDim.x) {
-    const T den = T(1) / (T(1) + k.dtLk * ksq_at<T>(k, i % N3)) * k.inv_n3;
-    const typename Cx<T>::T2 e = E[i], f = Fh[i];
-    E[i] = cx<T>((e.x - k.dtL * f.x) * den, (e.y - k.dtL * f.y) * den);
-  }
-}
-
 template <typename T>
 __global__ void p_extract(const typename Cx<T>::T2* __restrict__ E, T* __restrict__ eta, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) eta[i] = E[i].x;
@@ -446,7 +366,9 @@ __global__ void __launch_bounds__(256) p_contract_pair(const typename Cx<T>::T2*
   }
 }
 
-// G_p = F_p + i eta_p with F_p as p_force (R_p from the packed pairs, noise keyed as p_force)
+// G_p = F_p + i eta_p: F_p = A eta_p - B eta_p^2 + C eta_p sum_q eta_q^2 + 2 w_el eta_p R_p
+// (+ noise xi_p, Philox field p, keyed by the quad), R_p from the packed pairs. One thread per quad
+// of voxels; the variants stay in registers.
 template <typename T>
 __global__ void __launch_bounds__(256) p_force_pair(const T* __restrict__ eta, const typename Cx<T>::T2* __restrict__ R,
                              typename Cx<T>::T2* __restrict__ G, long N3, PlainKT<T> k) {
@@ -540,12 +462,12 @@ using PlainPK = PlainPKT<double>;
 // mu_c = A2/2 sum eta^2 + A1 (c - c1) (:118-127) and F_v (:136-162) + noise a xi_v (Philox field v)
 // of every voxel into Fh; the fields themselves into E (both as complex with zero imaginary part).
 // Rel (reference rule, may be null): the elastic R_v of the eta^2 source as complex spectra whose
-// real parts hold R_v. PK (box grids): Fh_f = F_f + i field_f instead (E unused), and Rel holds R
-// as packed pairs.
-template <typename T, bool PK = false>
+// real parts hold R_v. Each force shares a complex field with its own field (Fh_f = F_f + i
+// field_f, one C2C transform for both), and Rel holds R as packed pairs.
+template <typename T>
 __global__ void __launch_bounds__(256) p_prec_local(const T* __restrict__ fld, typename Cx<T>::T2* __restrict__ Fh,
-                             typename Cx<T>::T2* __restrict__ E, long N3, PlainPKT<T> k,
-                             const typename Cx<T>::T2* __restrict__ Rel) {
+                                                    long N3, PlainPKT<T> k,
+                                                    const typename Cx<T>::T2* __restrict__ Rel) {
   const int V = k.V;
   for (long g4 = blockIdx.x * (long)blockDim.x + threadIdx.x; g4 < N3 / 4; g4 += (long)gridDim.x * blockDim.x) {
     const long v0 = g4 * 4;
@@ -563,24 +485,15 @@ __global__ void __launch_bounds__(256) p_prec_local(const T* __restrict__ fld, t
       normal4(k.seed, (uint32_t)g4, 0x8000u, k.step, xi);
       for (int i = 0; i < 4; ++i) mu[i] += k.noise * (T)xi[i];
     }
-    if (PK) {
-      st_quad(Fh + v0, mu, c0);
-    } else {
-      st_quad_re(Fh + v0, mu);
-      st_quad_re(E + v0, c0);
-    }
+    st_quad(Fh + v0, mu, c0);
     T rp[2][4];
 #pragma unroll
     for (int a = 0; a < kPlainMaxV; ++a)
       if (a < V) {
         T r[4], f[4];
         if (Rel) {
-          if (PK) {
-            if (!(a & 1)) ld_quad(Rel + (a >> 1) * N3 + v0, rp[0], rp[1]);
-            for (int i = 0; i < 4; ++i) r[i] = rp[a & 1][i];
-          } else {
-            ld_quad_re(Rel + a * N3 + v0, r);
-          }
+          if (!(a & 1)) ld_quad(Rel + (a >> 1) * N3 + v0, rp[0], rp[1]);
+          for (int i = 0; i < 4; ++i) r[i] = rp[a & 1][i];
         }
         for (int i = 0; i < 4; ++i) {
           T t2 = 0, t4 = 0;
@@ -603,12 +516,7 @@ __global__ void __launch_bounds__(256) p_prec_local(const T* __restrict__ fld, t
           normal4(k.seed, (uint32_t)g4, (uint32_t)a, k.step, xi);
           for (int i = 0; i < 4; ++i) f[i] += k.noise * (T)xi[i];
         }
-        if (PK) {
-          st_quad(Fh + (1 + a) * N3 + v0, f, e[a]);
-        } else {
-          st_quad_re(Fh + (1 + a) * N3 + v0, f);
-          st_quad_re(E + (1 + a) * N3 + v0, e[a]);
-        }
+        st_quad(Fh + (1 + a) * N3 + v0, f, e[a]);
       }
   }
 }
@@ -617,30 +525,8 @@ __global__ void __launch_bounds__(256) p_prec_local(const T* __restrict__ fld, t
 //   c_hat   <- (c_hat - dt M k^2 (mu_hat + (xi B(k) - S) c_hat)) / (1 + dt M (S k^2 + kappa_c k^4))
 //   eta_hat <- (eta_hat - dt L F_hat) / (1 + dt L kappa_eta k^2)
 // (celastic = 0: no xi B(k) term; the reference rule also has S = 0)
-template <typename T>
-__global__ void p_prec_update(typename Cx<T>::T2* __restrict__ E, const typename Cx<T>::T2* __restrict__ Fh,
-                              const T* __restrict__ Bc, PlainPKT<T> k) {
-  const long N3 = (long)k.d.nx * k.d.ny * k.d.nz;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < (long)(k.V + 1) * N3;
-       i += (long)gridDim.x * blockDim.x) {
-    const long v = i % N3;
-    const T k2 = ksq_at<T>(k, v);
-    const typename Cx<T>::T2 e = E[i], f = Fh[i];
-    if (i < N3) {
-      const T a = (k.celastic ? k.xi * Bc[v] : T(0)) - k.S;
-      const T mx = f.x + a * e.x, my = f.y + a * e.y;
-      const T den = k.inv_n3 / (T(1) + k.dtM * k.S * k2 + k.dtMkc * k2 * k2);
-      E[i] = cx<T>((e.x - k.dtM * k2 * mx) * den, (e.y - k.dtM * k2 * my) * den);
-    } else {
-      const T den = k.inv_n3 / (T(1) + k.dtLke * k2);
-      E[i] = cx<T>((e.x - k.dtL * f.x) * den, (e.y - k.dtL * f.y) * den);
-    }
-  }
-}
-
-// p_prec_update on box grids: c_hat, mu_hat and eta_hat_v, F_hat_v unpacked from G_f = F_f + i
-// field_f (the output of p_prec_local<T, true> after its transform); the new spectra packed as
-// pairs into H
+// c_hat, mu_hat and eta_hat_v, F_hat_v unpacked from G_f = F_f + i field_f (the output of
+// p_prec_local after its transform); the new spectra packed as pairs into H
 template <typename T>
 __global__ void p_prec_update_pair(const typename Cx<T>::T2* __restrict__ G, typename Cx<T>::T2* __restrict__ H,
                                    const T* __restrict__ Bc, PlainPKT<T> k) {

@@ -561,3 +561,27 @@ def test_prec_plain64_matches_oracle(gpu, n, steps):
     assert abs(m1 - m0) / m0 < 1e-13
     Ew = O.prec_energy(cw, ew, Bc, p)
     np.testing.assert_allclose(E, Ew, rtol=1e-10, atol=1e-12 * abs(Ew[3]))
+
+
+@pytest.mark.parametrize("physics,V", [("mart", 3), ("mart", 2), ("prec", 4), ("prec", 3)])
+def test_plain_kernel_times_follow_the_paired_step(gpu, physics, V):
+    """Per-kernel timing on the FP64 plain path names one slot per launch of the paired step: a
+    slot list that missed a launch (or kept a removed one) fails with a slot count mismatch."""
+    say(f"plain64 kernel_times physics={physics} V={V}")
+    n = 32
+    s = mart_sim(n, V, path=gsk.PATH_PLAIN64) if physics == "mart" else prec_sim(n, V, path=gsk.PATH_PLAIN64)
+    with s:
+        names, ms = s.kernel_times(2)
+        nbytes = s.kernel_bytes()
+    P = (V + 1) // 2
+    if physics == "mart":
+        want = (["p_square_pair"] + ["p_z_fwdS", "p_y_fwdS", "p_x_fwdS"] * P + ["p_contract_pair"]
+                + ["p_z_invR", "p_y_invR", "p_x_invR"] * P + ["p_force_pair"] + ["p_z_fwdG", "p_y_fwdG", "p_x_fwdG"] * V
+                + ["p_update_pair"] + ["p_z_invE", "p_y_invE", "p_x_invE"] * P + ["p_extract_pair"])
+    else:
+        F = V + 1
+        want = (["p_prec_local"] + ["p_z_fwdG", "p_y_fwdG", "p_x_fwdG"] * F + ["p_prec_update_pair"]
+                + ["p_z_invE", "p_y_invE", "p_x_invE"] * ((F + 1) // 2) + ["p_extract_pair"])
+    assert names == want
+    assert len(ms) == len(want) and (ms > 0).all()
+    assert len(nbytes) == len(want) and (nbytes > 0).all()
